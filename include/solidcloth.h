@@ -1,0 +1,224 @@
+/* solidcloth.h — C ABI of the MI355X-native SolidCloth substep library
+ * (libsolidcloth.so).
+ *
+ * The exported names, argument order and argument meaning are those of the
+ * reference's ClothLib ABI (Demo/ClothAPI.h:65-146). The reference passes C++
+ * value types across that boundary; here they are layout-identical C structs
+ * (scVector3 = Math::Vector3, Math/Vector3.h:12-16; scVector2 = Math::Vector2;
+ * scAabb3 = Geometry::Aabb3, Geometry/Aabb3.h:10; scQuaternion =
+ * Math::Quaternion {float s; Vector3 v}, Math/Quaternion.h:13-14) and C++
+ * references become pointers, which is the same machine ABI on x86-64.
+ *
+ * Conventions kept from the reference: handles are {void* ptr} structs passed
+ * by value (Demo/ClothAPI.h:11-32); scCreate* allocates and scFree* releases;
+ * scAddCollidable transfers ownership of the collidable to the world; a patch
+ * keeps a reference to its world, which must outlive it; type-mismatched
+ * getters return without writing their outputs (Demo/ClothAPI.cpp:92-124).
+ *
+ * Difference: the reference reports no errors. This library additionally
+ * records the last error (scGetLastError) and refuses calls it cannot honour
+ * (for example a link-based model, which this build does not implement)
+ * instead of computing something else. All device work started by a call has
+ * completed before any getter returns data.
+ */
+#ifndef SOLIDCLOTH_H
+#define SOLIDCLOTH_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCEXP __attribute__((visibility("default")))
+
+typedef struct { float x, y, z; } scVector3;
+typedef struct { float x, y; } scVector2;
+typedef struct { scVector3 min, max; } scAabb3;
+typedef struct { float s; scVector3 v; } scQuaternion;
+
+typedef void* scHandle; /* a collidable (Demo/ClothAPI.h:9) */
+typedef struct { void* ptr; } scClothPatch;
+typedef struct { void* ptr; } scCollisionWorld;
+typedef struct { void* ptr; } scMesh;
+typedef struct { void* ptr; } scModel;
+typedef struct { void* ptr; } scSDF; /* replaces Geometry::SDF* (Demo/ClothAPI.h:80) */
+
+enum scCollFlags { /* Demo/ClothAPI.h:34-41 */
+    scCollisionFlagVertices = 1,
+    scCollisionFlagTriangles = 2,
+    scCollisionFlagSelf = 4,
+    scCollisionFlagWalls = 8,
+    scCollisionFlagEdges = 16,
+};
+
+enum scCollidableType { /* Demo/ClothAPI.h:43-53 */
+    scCollidableTypeInvalid,
+    scCollidableTypeWalls,
+    scCollidableTypeSlab,
+    scCollidableTypeSphere,
+    scCollidableTypeMesh,
+    scCollidableTypeBox,
+    scCollidableTypeCapsule,
+    scCollidableTypeSDF,
+};
+
+/* ---- collision world (Demo/ClothAPI.h:69-74) ---- */
+SCEXP scCollisionWorld scCreateCollisionWorld(void);                    /* :69 */
+SCEXP void scFreeCollisionWorld(scCollisionWorld collWorld);             /* :70 */
+SCEXP void scClearCollidables(scCollisionWorld collWorld);               /* :71 */
+SCEXP int scGetNumCollidables(scCollisionWorld collWorld);               /* :72 */
+SCEXP void scAddCollidable(scCollisionWorld collWorld, scHandle coll);   /* :73 */
+SCEXP scHandle scGetCollidable(scCollisionWorld collWorld, int i);       /* :74 */
+
+/* ---- collidables (Demo/ClothAPI.h:77-81) ---- */
+SCEXP scHandle scCreateWalls(scAabb3 box);                                                     /* :77 */
+SCEXP scHandle scCreateSphere(scVector3 center, float radius);                                 /* :78 */
+SCEXP scHandle scCreateCapsule(scVector3 c, float rad, float halfHeight, scQuaternion quat);    /* :79 */
+SCEXP scHandle scCreateSDF(scSDF s);                                                           /* :80 */
+SCEXP scHandle scCreateMesh(scMesh m);                                                         /* :81 */
+
+/* ---- introspection (Demo/ClothAPI.h:83-87) ---- */
+SCEXP int scGetCollidableType(scHandle collidable);                                            /* :83 */
+SCEXP void* scGetMeshTree(scHandle collMesh);                                                  /* :84 */
+SCEXP void scGetWallsInfo(scHandle walls, scAabb3* box);                                       /* :85 */
+SCEXP void scGetSphereInfo(scHandle sphere, float* radius, scVector3* center);                 /* :86 */
+SCEXP void scGetCapsuleInfo(scHandle capsule, float* radius, float* halfHeight, scVector3* center,
+                            scQuaternion* rot);                                                /* :87 */
+
+/* ---- cloth patch (Demo/ClothAPI.h:91-99) ---- */
+SCEXP scClothPatch scCreateClothPatch(scCollisionWorld collWorld);                             /* :91 */
+SCEXP void scFreeClothPatch(scClothPatch patch);                                               /* :92 */
+SCEXP scModel scGetClothModel(scClothPatch patch);                                             /* :93 */
+SCEXP scMesh scGetClothMesh(scClothPatch patch);                                               /* :94 */
+SCEXP void scInitCloth(scClothPatch patch, int divX, int divY, float inc, scVector3 offset, bool horizontal,
+                       bool attached);                                                         /* :95 */
+SCEXP void scStepCloth(scClothPatch patch, float dt);                                          /* :97 */
+SCEXP void scUpdateClothMesh(scClothPatch patch);                                              /* :98 */
+SCEXP void scComputeStrainMap(scClothPatch patch, scVector3* colors, int count);               /* :99 */
+
+/* ---- cloth model getters (Demo/ClothAPI.h:102-119) ---- */
+SCEXP int scGetNumParticles(scModel model);                                                    /* :102 */
+SCEXP void scGetParticlePos(scModel model, int i, scVector3* pos);                             /* :103 */
+SCEXP float scGetParticleInvMass(scModel model, int i);                                        /* :104 */
+SCEXP int scGetNumTris(scModel model);                                                         /* :106 */
+SCEXP void scGetTriangle(scModel model, int i, int* i1, int* i2, int* i3);                     /* :107 */
+SCEXP void scGetTriangleInfo(scModel model, int i, float* invDet, scVector2* du, scVector2* dv,
+                             scVector2* s);                                                    /* :108 */
+SCEXP int scGetNumQuads(scModel model);                                                        /* :110 */
+SCEXP void scGetQuad(scModel model, int i, int* i1, int* i2, int* i3, int* i4);                /* :111 */
+SCEXP int scGetNumLinks(scModel model);                                                        /* :113 */
+SCEXP void scGetLink(scModel model, int i, int* i1, int* i2);                                  /* :114 */
+SCEXP int scGetNumContacts(scModel model);                                                     /* :116 */
+SCEXP void scGetContact(scModel model, int i, scVector3* point, scVector3* normal);            /* :117 */
+SCEXP int scGetNumTriContacts(scModel model);                                                  /* :118 */
+SCEXP void scGetTriContact(scModel model, int i, scVector3* point, scVector3* normal);         /* :119 */
+
+/* ---- parameters (Demo/ClothAPI.h:121-132) ---- */
+SCEXP int scGetCollisionFlags(scModel model);                                                  /* :121 */
+SCEXP void scSetCollisionFlags(scModel model, int val);                                        /* :122 */
+SCEXP float scGetThickness(scModel model);                                                     /* :123 */
+SCEXP void scSetThickness(scModel model, float val);                                           /* :124 */
+SCEXP float scGetTolerance(scModel model);                                                     /* :125 */
+SCEXP void scSetTolerance(scModel model, float val);                                           /* :126 */
+SCEXP float scGetFriction(scModel model);                                                      /* :127 */
+SCEXP void scSetFriction(scModel model, float val);                                            /* :128 */
+SCEXP int scGetNumIterations(scModel model);                                                   /* :129 */
+SCEXP void scSetNumIterations(scModel model, int val);                                         /* :130 */
+SCEXP bool scGetUseFEM(scModel model);                                                         /* :131 */
+SCEXP void scSetUseFEM(scModel model, bool val);                                               /* :132 */
+
+/* ---- tree and mouse spring (Demo/ClothAPI.h:134-138) ---- */
+SCEXP void* scGetClothTree(scModel model);                                                     /* :134 */
+SCEXP void scAddMouseSpring(scModel model, int i1, int i2, int i3, scVector3 coords, scVector3 p); /* :136 */
+SCEXP void scUpdateMouseSpring(scModel model, scVector3 p);                                    /* :137 */
+SCEXP void scRemoveMouseSpring(scModel model);                                                 /* :138 */
+
+/* ---- mesh buffers (Demo/ClothAPI.h:141-145) ---- */
+SCEXP int scGetMeshVertexCount(scMesh mesh);                                                   /* :141 */
+SCEXP int scGetMeshIndexCount(scMesh mesh);                                                    /* :142 */
+SCEXP scVector3* scGetMeshNormalBuffer(scMesh mesh);                                           /* :143 */
+SCEXP scVector3* scGetMeshVertexBuffer(scMesh mesh);                                           /* :144 */
+SCEXP unsigned int* scGetMeshIndexBuffer(scMesh mesh);                                         /* :145 */
+
+/* =====================================================================
+ * Extensions. The reference sets these through C++ only
+ * (Physics/ClothModel.h:110-146, Physics/ClothPatch.h:50) or has no
+ * counterpart; the benchmark configurations need them through the ABI.
+ * ===================================================================== */
+
+/* ClothModel setters/getters, Physics/ClothModel.h:110-146 */
+SCEXP void scSetUnit(scModel model, float val);               /* SetUnit :133 */
+SCEXP float scGetUnit(scModel model);
+SCEXP void scSetShearStiffness(scModel model, float val);     /* SetShearStiffness :113 (FEM Young modulus) */
+SCEXP void scSetStretchStiffness(scModel model, float val);   /* SetStretchStiffness :111 */
+SCEXP void scSetBendStiffness(scModel model, float val);      /* SetBendStiffness :115 */
+SCEXP void scSetPoisson(scModel model, float val);            /* SetPoisson :129 */
+SCEXP void scSetDihedral(scModel model, bool val);            /* SetDihedral :123 */
+SCEXP void scSetDragCoeff(scModel model, float val);          /* SetDragCoeff :146 */
+SCEXP void scSetNumSteps(scClothPatch patch, int val);        /* ClothPatch::SetNumSteps, ClothPatch.h:50 */
+SCEXP int scGetNumSteps(scClothPatch patch);
+
+/* Iteration control. fixed = true removes the early exit of
+ * Physics/ClothPBD.cpp:356-358 so exactly NumIterations iterations run. */
+SCEXP void scSetFixedIterations(scModel model, bool fixed);
+
+/* Vertical-mode incline of scInitCloth (Physics/ClothPatch.cpp:153,183).
+ * The reference draws it from rand(); this sets it explicitly. Default 0. */
+SCEXP void scSetInitIncline(scClothPatch patch, float incline);
+
+/* Grid SDF with (nx+1)(ny+1)(nz+1) node samples, x fastest (Geometry/SDF.h:
+ * 49-52), over box: Create + CreateFromImplicit's stored samples +
+ * ComputeGradient (Geometry/SDF.cpp:10-56,210-256). */
+SCEXP scSDF scCreateGridSDF(const float* values, int nx, int ny, int nz, scAabb3 box);
+SCEXP void scFreeSDF(scSDF sdf);
+
+/* Indexed triangle mesh from arrays (replaces Geometry::Mesh* in scCreateMesh). */
+SCEXP scMesh scCreateMeshFromArrays(const scVector3* vertices, int numVertices, const unsigned int* indices,
+                                    int numIndices);
+SCEXP void scFreeMesh(scMesh mesh);
+
+/* Bulk state access (host arrays of count elements). */
+SCEXP int scGetParticlePositions(scModel model, scVector3* out, int count);
+SCEXP int scSetParticlePositions(scModel model, const scVector3* in, int count);
+SCEXP int scGetParticleVelocities(scModel model, scVector3* out, int count);
+SCEXP int scSetParticleVelocities(scModel model, const scVector3* in, int count);
+
+/* Number of bend constraints and their vertex quadruples (i1,i2,i3,i4). */
+SCEXP int scGetNumBends(scModel model);
+SCEXP int scGetBends(scModel model, int* out, int count);
+
+/* Gauss-Seidel order the GPU applies: kind 0 = triangles, 1 = bends. Writes
+ * the constraint indices in processing order and returns their count;
+ * colourOffsets (may be NULL) receives numColours+1 offsets. */
+SCEXP int scGetConstraintOrder(scModel model, int kind, int* order, int count);
+SCEXP int scGetColouring(scModel model, int kind, int* colourOffsets, int maxColours);
+
+/* Solver statistics of the last substep (Physics/ClothModel.h:32-40):
+ * out[0] iterations, out[1] bend error, out[2] vertex-contact error. */
+SCEXP void scGetSolverStats(scModel model, float* out);
+
+/* Kernel timing with HIP events on the solver's stream. which: 0 FEM sweep,
+ * 1 bend sweep, 2 contacts, 3 detection, 4 predict+velocity. While enabled,
+ * steps run without graph replay so each launch is bracketed. */
+SCEXP void scEnableKernelTiming(scModel model, bool enable);
+SCEXP int scGetKernelTiming(scModel model, int which, int* launches, double* totalMs);
+
+/* Use a captured hipGraph for the substep (default true). */
+SCEXP void scSetUseGraph(scModel model, bool enable);
+
+/* Last error message of the calling thread, or "" (not in the reference),
+ * and the number of errors recorded on the calling thread so far, so a
+ * caller can tell whether its own call failed. */
+SCEXP const char* scGetLastError(void);
+SCEXP int scGetErrorCount(void);
+
+/* Library identification. */
+SCEXP const char* scGetBuildInfo(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOLIDCLOTH_H */

@@ -1,0 +1,950 @@
+// TEST INFRASTRUCTURE ONLY — the CPU oracle for the GPU cloth substep.
+// Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg;
+// never by the product library.
+//
+// A restatement of SolidCloth's position-based-dynamics substep, templated on
+// the real type (float = parity oracle, double = FP64 rerun). Each function
+// cites the reference lines it follows. Where the reference's result depends
+// on unspecified behaviour the restatement fixes a canonical choice:
+//   * std::sort in Mesh::ConstructEdges (Geometry/Mesh.cpp:110) is unstable,
+//     so which of an interior edge's two triangles becomes t1 depends on the
+//     standard library. The restatement makes the same std::sort call on the
+//     same keys in the same initial order, so with libstdc++ it reproduces the
+//     compiled reference exactly (pinned by tests/golden).
+//   * Contacts are generated in collidable-major, vertex-minor order, i.e. the
+//     reference's order with OMP_NUM_THREADS=1 (ClothCollision.cpp:440-457).
+//   * Triangle::invDet is left uninitialised by the reference when det == 0
+//     (ClothModel.cpp:67-68); here it is 0.
+//   * The Gauss-Seidel loop may be given a constraint permutation (the GPU's
+//     colour order) and a fixed-iteration switch that removes the early exit
+//     (ClothPBD.cpp:356-358), which otherwise fires after one iteration in FEM
+//     mode because the link error stays 0.
+#include "oracle_vec.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace oracle {
+
+enum { CF_VERTICES = 1, CF_TRIANGLES = 2, CF_SELF = 4, CF_WALLS = 8, CF_EDGES = 16 };
+enum { CT_WALLS = 1, CT_SPHERE = 3, CT_SDF = 7 };
+
+struct Params {
+    // defaults: Physics/ClothModel.cpp:10-36
+    float unit = 10.f, thickness = 0.3f, tolerance = 0.2f, shear = 5e8f, bend = 5.f, stretch = 2e10f;
+    float poisson = 0.5f, friction = 0.4f, area = 10.f, drag = 1.f, selfShrink = 0.9f, selfCollThreshold = 0.003f;
+    int iterations = 50, collFlags = 0, bendFreq = 2, numSteps = 1;
+    bool dihedral = true, fem = false, fixedIterations = false;
+};
+
+struct Stats {
+    int iterations = 0;
+    float bendError = 0, contactError = 0;
+};
+
+template <class R>
+struct Tri {
+    uint32_t i1, i2, i3;
+    R du1, du2, dv1, dv2, det, invDet, lu0, lv0, area, lu2, lv2;
+};
+
+struct Bend {
+    uint32_t i1, i2, i3, i4;
+};
+
+struct Edge {
+    uint32_t i1, i2;
+    int t1 = -1, t2 = -1;
+};
+
+template <class R>
+struct Contact {
+    uint32_t idx;
+    int coll;  // index of the collidable that produced it
+    V3<R> point, normal, vel;
+    R lambda = 0, lambdaF = 0, depth = 0;
+};
+
+// Node-sampled grid SDF: Geometry/SDF.h:72-155, Geometry/SDF.cpp:10-40,210-256
+template <class R>
+struct GridSDF {
+    int nc[3];  // cells per side
+    int nv[3];  // values per side
+    V3<R> bmin, bmax, steps;
+    std::vector<R> vals;
+    std::vector<V3<R>> grads;
+
+    int index(int x, int y, int z) const { return x + y * nv[0] + z * nv[0] * nv[1]; }
+    bool oob(int x, int y, int z) const
+    {
+        return x > nc[0] || y > nc[1] || z > nc[2] || x < 0 || y < 0 || z < 0;
+    }
+    R value(int x, int y, int z) const { return oob(x, y, z) ? R(0) : vals[index(x, y, z)]; }
+    V3<R> grad(int x, int y, int z) const { return oob(x, y, z) ? V3<R>{0, 0, 0} : grads[index(x, y, z)]; }
+
+    void create(int nx, int ny, int nz, V3<R> mn, V3<R> mx)
+    {
+        nc[0] = nx; nc[1] = ny; nc[2] = nz;
+        nv[0] = nx + 1; nv[1] = ny + 1; nv[2] = nz + 1;
+        bmin = mn; bmax = mx;
+        vals.assign((size_t)nv[0] * nv[1] * nv[2], R(0));
+        // SDF.cpp:37-39: steps = extent scaled by the reciprocal cell counts
+        steps = bmax - bmin;
+        V3<R> inv{R(1.f) / R(nx), R(1.f) / R(ny), R(1.f) / R(nz)};
+        steps.x *= inv.x; steps.y *= inv.y; steps.z *= inv.z;
+    }
+
+    // SDF.cpp:210-256 forward differences, last layer copied
+    void compute_gradient()
+    {
+        grads.assign(vals.size(), V3<R>{0, 0, 0});
+        auto set = [&](int x, int y, int z, V3<R> g) {
+            if (!oob(x, y, z)) grads[index(x, y, z)] = g;
+        };
+        for (int x = 0; x < nc[0]; x++) {
+            for (int y = 0; y < nc[1]; y++) {
+                for (int z = 0; z < nc[2]; z++) {
+                    R v = value(x, y, z);
+                    R dfdx = (value(x + 1, y, z) - v) / steps.x;
+                    R dfdy = (value(x, y + 1, z) - v) / steps.y;
+                    R dfdz = (value(x, y, z + 1) - v) / steps.z;
+                    set(x, y, z, V3<R>{dfdx, dfdy, dfdz});
+                }
+                int z = nc[2];
+                set(x, y, z, grad(x, y, z - 1));
+            }
+            int y = nc[1];
+            for (int z = 0; z <= nc[2]; z++) set(x, y, z, grad(x, y - 1, z));
+        }
+        int x = nc[0];
+        for (int y = 0; y <= nc[1]; y++)
+            for (int z = 0; z <= nc[2]; z++) set(x, y, z, grad(x - 1, y, z));
+    }
+
+    // SDF.h:105-117. The reference truncates floorf() to int and treats a
+    // negative or too-large cell as outside; NaN and out-of-int-range values
+    // land outside as well (x86 cvttss2si yields INT_MIN for them).
+    bool coords(V3<R> p, int& x, int& y, int& z, R& fx, R& fy, R& fz) const
+    {
+        R rx = (p.x - bmin.x) / steps.x;
+        R ry = (p.y - bmin.y) / steps.y;
+        R rz = (p.z - bmin.z) / steps.z;
+        R flx = std::floor(rx), fly = std::floor(ry), flz = std::floor(rz);
+        if (!(flx >= R(0) && flx <= R(nc[0]) && fly >= R(0) && fly <= R(nc[1]) && flz >= R(0) && flz <= R(nc[2])))
+            return false;
+        x = (int)flx; y = (int)fly; z = (int)flz;
+        fx = std::fabs(rx - R(x));
+        fy = std::fabs(ry - R(y));
+        fz = std::fabs(rz - R(z));
+        return true;
+    }
+
+    // SDF.h:138-155
+    R value_at(V3<R> p) const
+    {
+        int x, y, z;
+        R fx, fy, fz;
+        if (!coords(p, x, y, z, fx, fy, fz)) return R(FLT_MAX);
+        const R one = 1;
+        return (one - fx) * (one - fy) * (one - fz) * value(x, y, z) +
+               (one - fx) * (one - fy) * fz * value(x, y, z + 1) +
+               (one - fx) * fy * (one - fz) * value(x, y + 1, z) +
+               (one - fx) * fy * fz * value(x, y + 1, z + 1) +
+               fx * (one - fy) * (one - fz) * value(x + 1, y, z) +
+               fx * (one - fy) * fz * value(x + 1, y, z + 1) +
+               fx * fy * (one - fz) * value(x + 1, y + 1, z) +
+               fx * fy * fz * value(x + 1, y + 1, z + 1);
+    }
+
+    // SDF.h:119-136
+    V3<R> grad_at(V3<R> p) const
+    {
+        int x, y, z;
+        R fx, fy, fz;
+        if (!coords(p, x, y, z, fx, fy, fz)) return V3<R>{0, 0, 0};
+        const R one = 1;
+        return (one - fx) * (one - fy) * (one - fz) * grad(x, y, z) +
+               (one - fx) * (one - fy) * fz * grad(x, y, z + 1) +
+               (one - fx) * fy * (one - fz) * grad(x, y + 1, z) +
+               (one - fx) * fy * fz * grad(x, y + 1, z + 1) +
+               fx * (one - fy) * (one - fz) * grad(x + 1, y, z) +
+               fx * (one - fy) * fz * grad(x + 1, y, z + 1) +
+               fx * fy * (one - fz) * grad(x + 1, y + 1, z) +
+               fx * fy * fz * grad(x + 1, y + 1, z + 1);
+    }
+
+    // Physics/ClothCollisionSDF.h:130-139 (GridSDF::QueryPoint)
+    R query_point(V3<R> p, V3<R>& cp, V3<R>& n) const
+    {
+        R val = value_at(p);
+        V3<R> g = grad_at(p);
+        normalize(g);
+        n = g;
+        cp = p - val * g;
+        return val;
+    }
+};
+
+template <class R>
+struct Collidable {
+    int type;
+    V3<R> bmin, bmax;  // walls box
+    V3<R> center;      // sphere
+    R radius = 0;
+    int sdf = -1;      // index into sdfs
+};
+
+template <class R>
+class Cloth {
+public:
+    Params prm;
+    Stats stats;
+    std::vector<V3<R>> pos, prev, vel;
+    std::vector<R> invMass;
+    std::vector<R> uvx, uvy;
+    std::vector<Tri<R>> tris;
+    std::vector<Bend> bends;
+    std::vector<Edge> edges;
+    std::vector<uint32_t> meshIndices;
+    std::vector<Contact<R>> contacts;
+    std::vector<Collidable<R>> world;
+    std::vector<GridSDF<R>> sdfs;
+    std::vector<int> triOrder, bendOrder;
+
+    void clear()
+    {
+        pos.clear(); prev.clear(); vel.clear(); invMass.clear(); uvx.clear(); uvy.clear();
+        tris.clear(); bends.clear(); edges.clear(); meshIndices.clear(); contacts.clear();
+        triOrder.clear(); bendOrder.clear();
+    }
+
+    // Physics/ClothModel.cpp:54-92
+    void add_triangle(uint32_t i1, uint32_t i2, uint32_t i3)
+    {
+        Tri<R> t;
+        t.i1 = i1; t.i2 = i2; t.i3 = i3;
+        R d1x = uvx[i2] - uvx[i1], d1y = uvy[i2] - uvy[i1];
+        R d2x = uvx[i3] - uvx[i1], d2y = uvy[i3] - uvy[i1];
+        t.du1 = d1x; t.du2 = d2x; t.dv1 = d1y; t.dv2 = d2y;
+        t.det = t.du1 * t.dv2 - t.du2 * t.dv1;
+        t.invDet = (t.det != R(0)) ? R(1.f) / t.det : R(0);
+        V3<R> dx1 = pos[i2] - pos[i1];
+        V3<R> dx2 = pos[i3] - pos[i1];
+        V3<R> wu = t.invDet * (t.dv2 * dx1 - t.dv1 * dx2);
+        V3<R> wv = t.invDet * (-t.du2 * dx1 + t.du1 * dx2);
+        t.lu0 = length(wu);
+        t.lv0 = length(wv);
+        t.area = R(0.5f) * t.det;
+        t.lu2 = t.lu0 * t.lu0;
+        t.lv2 = t.lv0 * t.lv0;
+        tris.push_back(t);
+        meshIndices.push_back(i1); meshIndices.push_back(i2); meshIndices.push_back(i3);
+    }
+
+    // Geometry/Mesh.cpp:65-147 (unique sorted edges with winged triangles)
+    void construct_edges()
+    {
+        edges.clear();
+        int ne = (int)meshIndices.size();
+        std::vector<Edge> dup(ne);
+        for (int i = 0; i < ne; i++) {
+            uint32_t a = meshIndices[i];
+            uint32_t b = ((i + 1) % 3 == 0) ? meshIndices[i - 2] : meshIndices[i + 1];
+            if (a > b) std::swap(a, b);
+            dup[i].i1 = a; dup[i].i2 = b; dup[i].t1 = i / 3;
+        }
+        std::sort(dup.begin(), dup.end(), [](const Edge& e1, const Edge& e2) {
+            return e1.i1 < e2.i1 || (e1.i1 == e2.i1 && e1.i2 < e2.i2);
+        });
+        if (dup.empty()) return;
+        Edge cur = dup[0];
+        for (size_t i = 1; i < dup.size(); i++) {
+            if (dup[i].i1 == dup[i - 1].i1 && dup[i].i2 == dup[i - 1].i2) {
+                cur.t2 = dup[i].t1;
+                continue;
+            }
+            edges.push_back(cur);
+            cur = dup[i];
+        }
+        edges.push_back(cur);
+    }
+
+    static uint32_t opposite(const Tri<R>& t, uint32_t a, uint32_t b)
+    {
+        // ClothTypes.h:54-61
+        if (t.i1 != a && t.i1 != b) return t.i1;
+        if (t.i2 != a && t.i2 != b) return t.i2;
+        return t.i3;
+    }
+
+    // Physics/ClothModel.cpp:94-126 (dihedral branch; theta0 is unused by the solver)
+    void add_bend(const Edge& e)
+    {
+        if (e.t2 < 0) return;
+        Bend b;
+        b.i1 = e.i1; b.i2 = e.i2;
+        b.i3 = opposite(tris[e.t1], e.i1, e.i2);
+        b.i4 = opposite(tris[e.t2], e.i1, e.i2);
+        bends.push_back(b);
+    }
+
+    // Physics/ClothPatch.cpp:135-246 (grid branch)
+    void init_grid(int divX, int divY, float incF, V3<float> offF, bool horizontal, bool attached, float inclineF)
+    {
+        clear();
+        const int n = divX * divY;
+        pos.assign(n, V3<R>{0, 0, 0}); prev = pos; vel = pos;
+        invMass.assign(n, R(1)); uvx.assign(n, R(0)); uvy.assign(n, R(0));
+        const R inc = incF, incline = inclineF, scale = R(1.f);
+        const V3<R> offset{R(offF.x), R(offF.y), R(offF.z)};
+        R x = R(-0.5f) * R(divX) * inc;
+        const R startY = x;
+        for (int j = 0, base = 0; j < divX; j++, base += divY) {
+            pos[base] = horizontal ? (V3<R>{x, 0, startY} + offset) * scale : (V3<R>{x, startY, 0} + offset) * scale;
+            prev[base] = pos[base];
+            vel[base] = V3<R>{0, 0, 0};
+            invMass[base] = 1;
+            uvx[base] = x; uvy[base] = 0;
+            for (int i = 1; i < divY; i++) {
+                int idx = base + i;
+                if (horizontal)
+                    pos[idx] = (V3<R>{x, 0, startY + inc * R(i)} + offset) * scale;
+                else
+                    pos[idx] = (V3<R>{x, startY + inc * R(i), R(i) * incline} + offset) * scale;
+                prev[idx] = pos[idx];
+                vel[idx] = V3<R>{0, 0, 0};
+                invMass[idx] = 1;
+                uvx[idx] = x; uvy[idx] = inc * R(i);
+                if (idx > divY) {
+                    if ((i & 1) == (j & 1)) {
+                        add_triangle(idx - divY - 1, idx, idx - divY);
+                        add_triangle(idx - divY - 1, idx - 1, idx);
+                    } else {
+                        add_triangle(idx - divY, idx - 1, idx);
+                        add_triangle(idx - 1, idx - divY, idx - divY - 1);
+                    }
+                }
+            }
+            if (attached && (j == 0 || j == divX - 1)) invMass[base + divY - 1] = 0;
+            x += inc;
+        }
+        construct_edges();
+        for (const Edge& e : edges) add_bend(e);
+    }
+
+    // Physics/ClothCollision.cpp:199-220
+    void wall_collisions(const Collidable<R>& w, int ci)
+    {
+        const R r = R(prm.thickness) + R(prm.tolerance);
+        for (size_t i = 0; i < pos.size(); i++) {
+            const V3<R>& p = pos[i];
+            if (p.y - r <= w.bmin.y) add_contact(i, V3<R>{p.x, w.bmin.y, p.z}, V3<R>{0, 1, 0}, ci);
+            if (p.x - r <= w.bmin.x) add_contact(i, V3<R>{w.bmin.x, p.y, p.z}, V3<R>{1, 0, 0}, ci);
+            if (p.x + r >= w.bmax.x) add_contact(i, V3<R>{w.bmax.x, p.y, p.z}, V3<R>{-1, 0, 0}, ci);
+            if (p.z - r <= w.bmin.z) add_contact(i, V3<R>{p.x, p.y, w.bmin.z}, V3<R>{0, 0, 1}, ci);
+            if (p.z + r >= w.bmax.z) add_contact(i, V3<R>{p.x, p.y, w.bmax.z}, V3<R>{0, 0, -1}, ci);
+        }
+    }
+
+    // Physics/ClothModel.inl:33-47
+    void add_contact(size_t idx, V3<R> p, V3<R> n, int ci)
+    {
+        Contact<R> c;
+        c.idx = (uint32_t)idx;
+        c.coll = ci;
+        c.point = p; c.normal = n; c.vel = V3<R>{0, 0, 0};
+        R len = dot(n, pos[idx] - p);
+        c.depth = R(prm.thickness) - len;
+        contacts.push_back(c);
+    }
+
+    // Physics/ClothCollisionSDF.h:14-24 (SphereSDF::QueryPoint)
+    static R sphere_query(const Collidable<R>& s, V3<R> p, V3<R>& cp, V3<R>& n)
+    {
+        V3<R> d = p - s.center;
+        R len = length(d);
+        scale(d, R(1) / len);
+        n = d;
+        cp = s.center + s.radius * d;
+        return len - s.radius;
+    }
+
+    // Physics/ClothCollision.cpp:430-470 (vertex branch of SDFCollisions; the
+    // defaults handleInside = handleCurrent = false)
+    template <class Q>
+    void sdf_vertex_collisions(Q query, int ci)
+    {
+        if (!(prm.collFlags & CF_VERTICES)) return;
+        for (size_t i = 0; i < pos.size(); i++) {
+            V3<R> p, n;
+            R dist = query(prev[i], p, n);
+            R tv = length(prev[i] - pos[i]);
+            R radTol = R(prm.thickness) + smax(R(prm.tolerance), tv * R(1.1f));
+            if (dist < radTol && dist >= R(0)) add_contact(i, p, n, ci);
+        }
+    }
+
+    // Physics/ClothCollision.cpp:17-87 (external collidables; the cloth tree it
+    // also rebuilds is consumed by nothing on this path)
+    void detect_collisions()
+    {
+        contacts.clear();
+        for (size_t ci = 0; ci < world.size(); ci++) {
+            const Collidable<R>& c = world[ci];
+            if (c.type == CT_WALLS && (prm.collFlags & CF_WALLS)) {
+                wall_collisions(c, (int)ci);
+            } else if (c.type == CT_SPHERE) {
+                sdf_vertex_collisions([&](V3<R> p, V3<R>& cp, V3<R>& n) { return sphere_query(c, p, cp, n); }, (int)ci);
+            } else if (c.type == CT_SDF) {
+                const GridSDF<R>& g = sdfs[c.sdf];
+                sdf_vertex_collisions([&](V3<R> p, V3<R>& cp, V3<R>& n) { return g.query_point(p, cp, n); }, (int)ci);
+            }
+        }
+    }
+
+    // Physics/ClothPBD.cpp:52-108
+    void project_triangle(Tri<R>& t, R h2, R epsilonUV)
+    {
+        V3<R>& x1 = pos[t.i1];
+        V3<R>& x2 = pos[t.i2];
+        V3<R>& x3 = pos[t.i3];
+        const R m1 = invMass[t.i1], m2 = invMass[t.i2], m3 = invMass[t.i3];
+        V3<R> dx1 = x2 - x1;
+        V3<R> dx2 = x3 - x1;
+        V3<R> wu = t.invDet * (t.dv2 * dx1 - t.dv1 * dx2);
+        V3<R> wv = t.invDet * (-t.du2 * dx1 + t.du1 * dx2);
+        R err, diag, lambda;
+        V3<R> q1, q2, q3;
+        // warp
+        q2 = (t.invDet * t.dv2) * wu;
+        q3 = (-t.invDet * t.dv1) * wu;
+        q1 = -q2 - q3;
+        diag = dot(m1 * q1, q1) + dot(m2 * q2, q2) + dot(m3 * q3, q3);
+        err = R(0.5f) * (dot(wu, wu) - t.lu2);
+        lambda = -err / diag;
+        x1 += (m1 * lambda) * q1;
+        x2 += (m2 * lambda) * q2;
+        x3 += (m3 * lambda) * q3;
+        // weft
+        q2 = (-t.invDet * t.du2) * wv;
+        q3 = (t.invDet * t.du1) * wv;
+        q1 = -q2 - q3;
+        diag = dot(m1 * q1, q1) + dot(m2 * q2, q2) + dot(m3 * q3, q3);
+        err = R(0.5f) * (dot(wv, wv) - t.lv2);
+        lambda = -err / diag;
+        x1 += (m1 * lambda) * q1;
+        x2 += (m2 * lambda) * q2;
+        x3 += (m3 * lambda) * q3;
+        // shear
+        q2 = t.dv2 * wv - t.du2 * wu;
+        q3 = -t.dv1 * wv + t.du1 * wu;
+        q1 = -q2 - q3;
+        diag = t.invDet * (dot(m1 * q1, q1) + dot(m2 * q2, q2) + dot(m3 * q3, q3));
+        err = dot(wu, wv);
+        R s = R(1.f) / (R(1.f) + t.lu0 * t.lv0 * epsilonUV / (diag * h2 * t.area));
+        lambda = -s * err / diag;
+        x1 += (m1 * lambda) * q1;
+        x2 += (m2 * lambda) * q2;
+        x3 += (m3 * lambda) * q3;
+    }
+
+    void solve_triangles(R h)
+    {
+        const R h2 = h * h;
+        const R epsilonUV = R(2) * (R(1) + R(prm.poisson)) / (smax(R(prm.shear), R(0.1f)) * R(prm.unit) * R(prm.unit));
+        if (triOrder.empty()) {
+            for (auto& t : tris) project_triangle(t, h2, epsilonUV);
+        } else {
+            for (int k : triOrder) project_triangle(tris[k], h2, epsilonUV);
+        }
+    }
+
+    // Physics/ClothPBD.cpp:179-224 with EvaluateAngle 110-139, EvaluateGradient 141-177
+    R project_bend(const Bend& bc, R sStiff, R& error)
+    {
+        const R m1 = invMass[bc.i1], m2 = invMass[bc.i2], m3 = invMass[bc.i3], m4 = invMass[bc.i4];
+        if (m1 + m2 + m3 + m4 == R(0)) return error;
+        const V3<R> x1 = pos[bc.i1], x2 = pos[bc.i2], x3 = pos[bc.i3], x4 = pos[bc.i4];
+        // EvaluateAngle
+        V3<R> x12 = x2 - x1;
+        R l = length(x12);
+        V3<R> x13 = x3 - x1;
+        V3<R> n1 = cross(x12, x13);
+        R l1 = length(n1);
+        V3<R> x14 = x4 - x1;
+        V3<R> n2 = cross(x14, x12);
+        R l2 = length(n2);
+        R cosine, theta;
+        if (l1 == R(0) || l2 == R(0)) {
+            cosine = 1;
+            theta = 0;
+        } else {
+            R il1 = R(1.f) / l1, il2 = R(1.f) / l2;
+            V3<R> n1n = il1 * n1, n2n = il2 * n2;
+            cosine = dot(n1n, n2n);
+            cosine = smin(R(1.f), smax(R(-1.f), cosine));
+            theta = acos_as(cosine);
+        }
+        if (std::fabs(cosine) >= R(1 - 0.001f)) return error;
+        // EvaluateGradient (recomputes the same intermediate quantities)
+        R h1 = l1 / l, h2 = l2 / l;
+        R he = R(0.5f) * (h1 + h2);
+        R il1 = R(1.f) / l1, il2 = R(1.f) / l2;
+        V3<R> n1n = il1 * n1, n2n = il2 * n2;
+        V3<R> n1p = n1n - n2n * cosine;
+        V3<R> n2p = n2n - n1n * cosine;
+        R sine = rsqrt_len(R(1) - cosine * cosine);
+        V3<R> q3 = (il1 / sine / he) * cross(x12, n2p);
+        V3<R> q4 = (-il2 / sine / he) * cross(x12, n1p);
+        V3<R> q2 = (-il1 / sine / he) * cross(x13, n2p) + (il2 / sine / he) * cross(x14, n1p);
+        V3<R> q1 = -q2 - q3 - q4;
+        R diag = m1 * dot(q1, q1) + m2 * dot(q2, q2) + m3 * dot(q3, q3) + m4 * dot(q4, q4);
+        if (diag == R(0)) return error;
+        R e = sStiff * theta;
+        if (std::fabs(e) > error) error = std::fabs(e);
+        R lambda = -sStiff * theta / he / diag;
+        pos[bc.i1] += (lambda * m1) * q1;
+        pos[bc.i2] += (lambda * m2) * q2;
+        pos[bc.i3] += (lambda * m3) * q3;
+        pos[bc.i4] += (lambda * m4) * q4;
+        return error;
+    }
+
+    R solve_bends()
+    {
+        R error = 0;
+        const R s = R(0.0005f) * R(prm.bend);
+        if (bendOrder.empty()) {
+            for (const Bend& b : bends) project_bend(b, s, error);
+        } else {
+            for (int k : bendOrder) project_bend(bends[k], s, error);
+        }
+        return error;
+    }
+
+    // Physics/ClothCollisionHandler.cpp:134-202 (external vertex contacts;
+    // no mesh re-projection for walls and SDFs)
+    R solve_contacts(R h)
+    {
+        const R invH = R(1.f) / h;
+        const R mu = prm.friction;
+        const R thickness = prm.thickness;
+        R error = 0;
+        for (Contact<R>& c : contacts) {
+            V3<R>& p = pos[c.idx];
+            V3<R> delta = p - c.point;
+            R len = dot(delta, c.normal);
+            V3<R> disp = c.normal;
+            if (len > thickness) continue;
+            R depth = thickness - len;
+            if (std::fabs(depth) > error) error = std::fabs(depth);
+            R dLambda = depth;
+            c.lambda += dLambda;
+            scale(disp, dLambda);
+            V3<R> v12 = p - prev[c.idx] - c.vel;
+            R vnrel = dot(c.normal, v12);
+            V3<R> vt = v12 - vnrel * c.normal;
+            R vtrel = length(vt);
+            const R limit = mu * c.lambda;
+            if (vtrel > R(0.001f)) {
+                R dLambdaF = vtrel;
+                R lambdaF0 = c.lambdaF;
+                c.lambdaF = lambdaF0 + dLambdaF;
+                if (c.lambdaF >= limit) c.lambdaF = limit;
+                dLambdaF = c.lambdaF - lambdaF0;
+                scale(vt, R(1.0f) / vtrel);
+                disp += -dLambdaF * vt;
+            }
+            p += disp;
+            vel[c.idx] += invH * disp;
+        }
+        return error;
+    }
+
+    // Physics/ClothPBD.cpp:324-369
+    void solve_gs(R h)
+    {
+        R cb = 0, cv = 0;
+        int k = 0;
+        while (true) {
+            if (prm.fem) solve_triangles(h);
+            if (prm.dihedral && k % prm.bendFreq == 0) cb = solve_bends();
+            cv = solve_contacts(h);
+            // edge, triangle and self contacts are not generated on this path,
+            // so their errors ce = ct = sct = selfEE = 0 in the exit test
+            const R ce = 0, ct = 0, sct = 0, sce = 0, cl = 0;
+            k++;
+            const float eps = 0.01f;
+            if (!prm.fixedIterations && cl < R(0.05f) && cb < R(0.005f) && sce < R(prm.selfCollThreshold) &&
+                sct < R(prm.selfCollThreshold) && cv < R(eps) && ce < R(eps) && ct < R(eps))
+                break;
+            if (k >= prm.iterations) break;
+        }
+        stats.iterations = k;
+        stats.bendError = (float)cb;
+        stats.contactError = (float)cv;
+    }
+
+    // Physics/ClothPBD.cpp:255-322
+    void substep(R h)
+    {
+        const V3<R> gravity{R(0), R(-9.8f), R(0)};
+        const R unit = prm.unit, drag = prm.drag;
+        for (size_t i = 0; i < pos.size(); i++) {
+            prev[i] = pos[i];
+            if (invMass[i] == R(0)) continue;
+            vel[i] += (h * gravity) * unit;
+            scale(vel[i], drag);
+            pos[i] += h * vel[i];
+        }
+        detect_collisions();
+        for (auto& c : contacts) { c.lambda = 0; c.lambdaF = 0; }
+        solve_gs(h);
+        const R invH = R(1.f) / h;
+        for (size_t i = 0; i < pos.size(); i++) {
+            if (invMass[i] == R(0)) continue;
+            vel[i] = invH * (pos[i] - prev[i]);
+        }
+    }
+
+    // Physics/ClothPatch.cpp:323-335. Only the FEM stretch model is restated
+    // (ClothPBD.cpp:341-344); the link model returns an error.
+    int step(float dt)
+    {
+        if (!prm.fem) return -1;
+        const int n = prm.numSteps;
+        const R h = R(dt / n);
+        for (int i = 0; i < n; i++) substep(h);
+        return 0;
+    }
+};
+
+struct Handle {
+    bool real64;
+    std::unique_ptr<Cloth<float>> f;
+    std::unique_ptr<Cloth<double>> d;
+};
+
+template <class F>
+static auto with(void* hp, F&& fn)
+{
+    Handle* h = (Handle*)hp;
+    if (h->real64) return fn(*h->d);
+    return fn(*h->f);
+}
+
+}  // namespace oracle
+
+using namespace oracle;
+
+extern "C" {
+
+void* oc_create(int real64)
+{
+    Handle* h = new Handle;
+    h->real64 = real64 != 0;
+    if (h->real64) h->d.reset(new Cloth<double>);
+    else h->f.reset(new Cloth<float>);
+    return h;
+}
+
+void oc_free(void* h) { delete (Handle*)h; }
+
+int oc_set_param_f(void* h, const char* name, float v)
+{
+    return with(h, [&](auto& c) {
+        std::string n(name);
+        Params& p = c.prm;
+        if (n == "unit") p.unit = v;
+        else if (n == "thickness") p.thickness = v;
+        else if (n == "tolerance") p.tolerance = v;
+        else if (n == "shear") p.shear = v;
+        else if (n == "bend") p.bend = v;
+        else if (n == "stretch") p.stretch = v;
+        else if (n == "poisson") p.poisson = v;
+        else if (n == "friction") p.friction = v;
+        else if (n == "area") p.area = v;
+        else if (n == "drag") p.drag = v;
+        else return -1;
+        return 0;
+    });
+}
+
+int oc_set_param_i(void* h, const char* name, int v)
+{
+    return with(h, [&](auto& c) {
+        std::string n(name);
+        Params& p = c.prm;
+        if (n == "iterations") p.iterations = v;
+        else if (n == "collFlags") p.collFlags = v;
+        else if (n == "bendFreq") p.bendFreq = v;
+        else if (n == "numSteps") p.numSteps = v;
+        else if (n == "dihedral") p.dihedral = v != 0;
+        else if (n == "fem") p.fem = v != 0;
+        else if (n == "fixedIterations") p.fixedIterations = v != 0;
+        else return -1;
+        return 0;
+    });
+}
+
+void oc_init_grid(void* h, int divX, int divY, float inc, float ox, float oy, float oz, int horizontal, int attached,
+                  float incline)
+{
+    with(h, [&](auto& c) { c.init_grid(divX, divY, inc, V3<float>{ox, oy, oz}, horizontal != 0, attached != 0, incline); });
+}
+
+// A cloth from arrays: particles (positions, material coordinates, inverse
+// masses) and triangles, each added with ClothModel::AddTriangle's rest-state
+// precompute (ClothModel.cpp:54-92), then edges and dihedral bends as
+// ClothPatch::Init does (ClothPatch.cpp:232-241).
+void oc_init_arrays(void* h, int nv, const double* pos, const double* uv, const double* invMass, int nt,
+                    const int* tri)
+{
+    with(h, [&](auto& c) {
+        using R = typename std::remove_reference<decltype(c.pos[0].x)>::type;
+        c.clear();
+        c.pos.resize(nv); c.prev.resize(nv); c.vel.assign(nv, V3<R>{0, 0, 0});
+        c.invMass.resize(nv); c.uvx.resize(nv); c.uvy.resize(nv);
+        for (int i = 0; i < nv; i++) {
+            c.pos[i] = V3<R>{R(pos[3 * i]), R(pos[3 * i + 1]), R(pos[3 * i + 2])};
+            c.prev[i] = c.pos[i];
+            c.invMass[i] = R(invMass[i]);
+            c.uvx[i] = R(uv[2 * i]); c.uvy[i] = R(uv[2 * i + 1]);
+        }
+        for (int t = 0; t < nt; t++) c.add_triangle(tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]);
+        c.construct_edges();
+        for (const Edge& e : c.edges) c.add_bend(e);
+    });
+}
+
+int oc_add_walls(void* h, const float* box6)
+{
+    return with(h, [&](auto& c) {
+        using R = typename std::remove_reference<decltype(c.pos[0].x)>::type;
+        Collidable<R> w;
+        w.type = CT_WALLS;
+        w.bmin = V3<R>{R(box6[0]), R(box6[1]), R(box6[2])};
+        w.bmax = V3<R>{R(box6[3]), R(box6[4]), R(box6[5])};
+        c.world.push_back(w);
+        return (int)c.world.size() - 1;
+    });
+}
+
+int oc_add_sphere(void* h, float cx, float cy, float cz, float r)
+{
+    return with(h, [&](auto& c) {
+        using R = typename std::remove_reference<decltype(c.pos[0].x)>::type;
+        Collidable<R> s;
+        s.type = CT_SPHERE;
+        s.center = V3<R>{R(cx), R(cy), R(cz)};
+        s.radius = r;
+        c.world.push_back(s);
+        return (int)c.world.size() - 1;
+    });
+}
+
+// values: (nx+1)(ny+1)(nz+1) node samples, x fastest (Geometry/SDF.h:49-52)
+int oc_add_grid_sdf(void* h, int nx, int ny, int nz, const float* box6, const float* values)
+{
+    return with(h, [&](auto& c) {
+        using R = typename std::remove_reference<decltype(c.pos[0].x)>::type;
+        GridSDF<R> g;
+        g.create(nx, ny, nz, V3<R>{R(box6[0]), R(box6[1]), R(box6[2])}, V3<R>{R(box6[3]), R(box6[4]), R(box6[5])});
+        for (size_t i = 0; i < g.vals.size(); i++) g.vals[i] = R(values[i]);
+        g.compute_gradient();
+        c.sdfs.push_back(g);
+        Collidable<R> s;
+        s.type = CT_SDF;
+        s.sdf = (int)c.sdfs.size() - 1;
+        c.world.push_back(s);
+        return (int)c.world.size() - 1;
+    });
+}
+
+// Query a grid SDF the way GridSDF::QueryPoint does; out = [val, grad(3), normal(3), cp(3)]
+void oc_sdf_query(void* h, int sdf, float px, float py, float pz, double* out)
+{
+    with(h, [&](auto& c) {
+        using R = typename std::remove_reference<decltype(c.pos[0].x)>::type;
+        const GridSDF<R>& g = c.sdfs[sdf];
+        V3<R> p{R(px), R(py), R(pz)};
+        V3<R> gr = g.grad_at(p);
+        V3<R> cp, n;
+        R v = g.query_point(p, cp, n);
+        double o[10] = {(double)v, (double)gr.x, (double)gr.y, (double)gr.z, (double)n.x, (double)n.y, (double)n.z,
+                        (double)cp.x, (double)cp.y, (double)cp.z};
+        memcpy(out, o, sizeof(o));
+    });
+}
+
+void oc_get_sdf_grads(void* h, int sdf, double* out)
+{
+    with(h, [&](auto& c) {
+        const auto& g = c.sdfs[sdf];
+        for (size_t i = 0; i < g.grads.size(); i++) {
+            out[3 * i] = g.grads[i].x; out[3 * i + 1] = g.grads[i].y; out[3 * i + 2] = g.grads[i].z;
+        }
+    });
+}
+
+void oc_set_order(void* h, int kind, const int* order, int n)
+{
+    with(h, [&](auto& c) {
+        std::vector<int>& o = kind == 0 ? c.triOrder : c.bendOrder;
+        o.assign(order, order + n);
+    });
+}
+
+void oc_get_counts(void* h, int* out)
+{
+    with(h, [&](auto& c) {
+        out[0] = (int)c.pos.size();
+        out[1] = (int)c.tris.size();
+        out[2] = (int)c.edges.size();
+        out[3] = (int)c.bends.size();
+        out[4] = (int)c.contacts.size();
+    });
+}
+
+// idx: T*3 ints; rest: T*11 doubles (du1 du2 dv1 dv2 det invDet lu0 lv0 area lu2 lv2)
+void oc_get_triangles(void* h, int* idx, double* rest)
+{
+    with(h, [&](auto& c) {
+        for (size_t k = 0; k < c.tris.size(); k++) {
+            const auto& t = c.tris[k];
+            idx[3 * k] = t.i1; idx[3 * k + 1] = t.i2; idx[3 * k + 2] = t.i3;
+            double r[11] = {(double)t.du1, (double)t.du2, (double)t.dv1, (double)t.dv2, (double)t.det, (double)t.invDet,
+                            (double)t.lu0, (double)t.lv0, (double)t.area, (double)t.lu2, (double)t.lv2};
+            memcpy(rest + 11 * k, r, sizeof(r));
+        }
+    });
+}
+
+void oc_get_bends(void* h, int* out)
+{
+    with(h, [&](auto& c) {
+        for (size_t k = 0; k < c.bends.size(); k++) {
+            out[4 * k] = c.bends[k].i1; out[4 * k + 1] = c.bends[k].i2;
+            out[4 * k + 2] = c.bends[k].i3; out[4 * k + 3] = c.bends[k].i4;
+        }
+    });
+}
+
+void oc_get_edges(void* h, int* out)
+{
+    with(h, [&](auto& c) {
+        for (size_t k = 0; k < c.edges.size(); k++) {
+            out[4 * k] = c.edges[k].i1; out[4 * k + 1] = c.edges[k].i2;
+            out[4 * k + 2] = c.edges[k].t1; out[4 * k + 3] = c.edges[k].t2;
+        }
+    });
+}
+
+// which: 0 pos, 1 prev, 2 vel, 3 invMass (V doubles)
+void oc_get_state(void* h, int which, double* out)
+{
+    with(h, [&](auto& c) {
+        size_t n = c.pos.size();
+        if (which == 3) {
+            for (size_t i = 0; i < n; i++) out[i] = c.invMass[i];
+            return;
+        }
+        const auto& a = which == 0 ? c.pos : which == 1 ? c.prev : c.vel;
+        for (size_t i = 0; i < n; i++) {
+            out[3 * i] = a[i].x; out[3 * i + 1] = a[i].y; out[3 * i + 2] = a[i].z;
+        }
+    });
+}
+
+void oc_set_state(void* h, int which, const double* in)
+{
+    with(h, [&](auto& c) {
+        using R = typename std::remove_reference<decltype(c.pos[0].x)>::type;
+        size_t n = c.pos.size();
+        if (which == 3) {
+            for (size_t i = 0; i < n; i++) c.invMass[i] = R(in[i]);
+            return;
+        }
+        auto& a = which == 0 ? c.pos : which == 1 ? c.prev : c.vel;
+        for (size_t i = 0; i < n; i++) a[i] = V3<R>{R(in[3 * i]), R(in[3 * i + 1]), R(in[3 * i + 2])};
+    });
+}
+
+// contacts: idx (int), coll (int), point(3), normal(3), lambda, lambdaF -> 8 doubles each
+void oc_get_contacts(void* h, int* ids, double* data)
+{
+    with(h, [&](auto& c) {
+        for (size_t k = 0; k < c.contacts.size(); k++) {
+            const auto& ct = c.contacts[k];
+            ids[2 * k] = ct.idx; ids[2 * k + 1] = ct.coll;
+            double r[8] = {(double)ct.point.x, (double)ct.point.y, (double)ct.point.z, (double)ct.normal.x,
+                           (double)ct.normal.y, (double)ct.normal.z, (double)ct.lambda, (double)ct.lambdaF};
+            memcpy(data + 8 * k, r, sizeof(r));
+        }
+    });
+}
+
+void oc_get_stats(void* h, double* out)
+{
+    with(h, [&](auto& c) {
+        out[0] = c.stats.iterations;
+        out[1] = c.stats.bendError;
+        out[2] = c.stats.contactError;
+    });
+}
+
+int oc_step(void* h, float dt)
+{
+    return with(h, [&](auto& c) { return c.step(dt); });
+}
+
+void oc_substep(void* h, float hstep)
+{
+    with(h, [&](auto& c) {
+        using R = typename std::remove_reference<decltype(c.pos[0].x)>::type;
+        c.substep(R(hstep));
+    });
+}
+
+// Pieces of the substep, for tests that pin one stage at a time.
+void oc_detect(void* h) { with(h, [&](auto& c) { c.detect_collisions(); }); }
+void oc_solve_triangles(void* h, float hstep)
+{
+    with(h, [&](auto& c) {
+        using R = typename std::remove_reference<decltype(c.pos[0].x)>::type;
+        c.solve_triangles(R(hstep));
+    });
+}
+double oc_solve_bends(void* h) { return with(h, [&](auto& c) { return (double)c.solve_bends(); }); }
+double oc_solve_contacts(void* h, float hstep)
+{
+    return with(h, [&](auto& c) {
+        using R = typename std::remove_reference<decltype(c.pos[0].x)>::type;
+        return (double)c.solve_contacts(R(hstep));
+    });
+}
+
+float oc_acos(float x) { return acos_as(x); }
+
+// Mesh::ConstructEdges on an arbitrary index list; out = (i1, i2, t1, t2) per
+// unique edge. Returns the number of edges.
+int oc_construct_edges(const unsigned* idx, int ni, int* out, int cap)
+{
+    Cloth<float> c;
+    c.meshIndices.assign(idx, idx + ni);
+    c.construct_edges();
+    int n = (int)c.edges.size();
+    for (int e = 0; e < n && e < cap; e++) {
+        out[4 * e] = c.edges[e].i1; out[4 * e + 1] = c.edges[e].i2;
+        out[4 * e + 2] = c.edges[e].t1; out[4 * e + 3] = c.edges[e].t2;
+    }
+    return n;
+}
+
+}  // extern "C"

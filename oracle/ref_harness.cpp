@@ -1,0 +1,93 @@
+// TEST INFRASTRUCTURE ONLY. A C-callable harness around the reference's own
+// Geometry translation units, compiled from /root/reference by
+// oracle/build_ref.sh into oracle/_ref/libref_geometry.so. It is run only by
+// tests/golden/make_golden.py, in a container that has the reference, to
+// write golden vectors; nothing on a GPU host loads it.
+//
+// Only Geometry/{Mesh,Collision3D}.cpp are compiled: they build with the
+// image's own headers. The Physics translation units do not (they need Win32
+// types from Engine/Engine.h and std::floorf, which libstdc++ 11 lacks), so
+// the solver itself is pinned by the oracle's mathematical tests instead.
+#include "Geometry/Collision3D.h"
+#include "Geometry/Mesh.h"
+
+#include <cstring>
+
+using Math::Vector3;
+
+extern "C" {
+
+// Mesh::ConstructEdges on an indexed triangle list (Geometry/Mesh.cpp:65-147).
+// Returns the number of unique edges; out = (i1, i2, t1, t2) per edge.
+int ref_construct_edges(const float* verts, int nv, const unsigned* idx, int ni, int* out, int cap)
+{
+    Geometry::Mesh m;
+    m.vertices.resize(nv);
+    for (int i = 0; i < nv; i++) m.vertices[i] = Vector3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
+    m.indices.assign(idx, idx + ni);
+    m.ComputeNormals();
+    m.ConstructEdges();
+    int n = (int)m.edges.size();
+    for (int e = 0; e < n && e < cap; e++) {
+        out[4 * e] = (int)m.edges[e].i1;
+        out[4 * e + 1] = (int)m.edges[e].i2;
+        out[4 * e + 2] = m.edges[e].t1;
+        out[4 * e + 3] = m.edges[e].t2;
+    }
+    return n;
+}
+
+// Mesh::ComputeNormals (angle weighted, Geometry/Mesh.cpp:10-63)
+void ref_compute_normals(const float* verts, int nv, const unsigned* idx, int ni, float* out)
+{
+    Geometry::Mesh m;
+    m.vertices.resize(nv);
+    for (int i = 0; i < nv; i++) m.vertices[i] = Vector3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
+    m.indices.assign(idx, idx + ni);
+    m.ComputeNormals();
+    for (int i = 0; i < nv; i++) {
+        out[3 * i] = m.normals[i].x;
+        out[3 * i + 1] = m.normals[i].y;
+        out[3 * i + 2] = m.normals[i].z;
+    }
+}
+
+// Geometry/Collision3D.cpp ClosestPtPointTriangle; out = cp(3), bar(3)
+int ref_closest_pt_point_triangle(const float* p, const float* a, const float* b, const float* c, float* out)
+{
+    Vector3 cp, bar;
+    int r = Geometry::ClosestPtPointTriangle(Vector3(p[0], p[1], p[2]), Vector3(a[0], a[1], a[2]),
+                                             Vector3(b[0], b[1], b[2]), Vector3(c[0], c[1], c[2]), cp, bar);
+    float o[6] = {cp.x, cp.y, cp.z, bar.x, bar.y, bar.z};
+    memcpy(out, o, sizeof(o));
+    return r;
+}
+
+// Geometry/Collision3D.cpp:10-58 ClosestPtSegmSegm; out = s, t, c1(3), c2(3)
+int ref_closest_pt_segm_segm(const float* p1, const float* q1, const float* p2, const float* q2, float* out)
+{
+    float s, t;
+    Vector3 c1, c2;
+    int r = Geometry::ClosestPtSegmSegm(Vector3(p1[0], p1[1], p1[2]), Vector3(q1[0], q1[1], q1[2]),
+                                        Vector3(p2[0], p2[1], p2[2]), Vector3(q2[0], q2[1], q2[2]), s, t, c1, c2);
+    float o[8] = {s, t, c1.x, c1.y, c1.z, c2.x, c2.y, c2.z};
+    memcpy(out, o, sizeof(o));
+    return r;
+}
+
+// Geometry/Collision3D.cpp:298-301
+float ref_signed_volume(const float* a0, const float* a1, const float* a2, const float* a3)
+{
+    return Geometry::SignedVolume(Vector3(a0[0], a0[1], a0[2]), Vector3(a1[0], a1[1], a1[2]),
+                                  Vector3(a2[0], a2[1], a2[2]), Vector3(a3[0], a3[1], a3[2]));
+}
+
+// Geometry/Collision3D.h:42-58
+void ref_barycentric(const float* a, const float* b, const float* c, const float* p, float* out)
+{
+    Vector3 r = Geometry::Barycentric(Vector3(a[0], a[1], a[2]), Vector3(b[0], b[1], b[2]), Vector3(c[0], c[1], c[2]),
+                                      Vector3(p[0], p[1], p[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+}  // extern "C"

@@ -1,0 +1,362 @@
+"""solidcloth_amd — Python mirror of SolidCloth's cloth API over the MI355X
+C-ABI library libsolidcloth.so (include/solidcloth.h).
+
+The classes keep the reference's names and method names (Physics/ClothPatch.h,
+Physics/ClothModel.h, Physics/CollisionWorld.h, Demo/ClothAPI.h) so code and
+tests read like the reference's. Every call goes through the C ABI into the
+HIP implementation; there is no CPU fallback — loading fails loudly when the
+library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsolidcloth.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "solidcloth.h")
+
+CF_VERTICES, CF_TRIANGLES, CF_SELF, CF_WALLS, CF_EDGES = 1, 2, 4, 8, 16
+CT_INVALID, CT_WALLS, CT_SLAB, CT_SPHERE, CT_MESH, CT_BOX, CT_CAPSULE, CT_SDF = range(8)
+
+
+class scVector3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class scVector2(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float)]
+
+
+class scAabb3(C.Structure):
+    _fields_ = [("min", scVector3), ("max", scVector3)]
+
+
+class scQuaternion(C.Structure):
+    _fields_ = [("s", C.c_float), ("v", scVector3)]
+
+
+class _Handle(C.Structure):
+    _fields_ = [("ptr", C.c_void_p)]
+
+
+scClothPatch = scCollisionWorld = scMesh = scModel = scSDF = _Handle
+
+_lib = None
+
+
+class SolidClothError(RuntimeError):
+    pass
+
+
+def v3(p):
+    return scVector3(float(p[0]), float(p[1]), float(p[2]))
+
+
+def _sig(L):
+    H, vp, i, f, b = _Handle, C.c_void_p, C.c_int, C.c_float, C.c_bool
+    P3 = C.POINTER(scVector3)
+    sigs = {
+        "scCreateCollisionWorld": (H, []), "scFreeCollisionWorld": (None, [H]),
+        "scClearCollidables": (None, [H]), "scGetNumCollidables": (i, [H]),
+        "scAddCollidable": (None, [H, vp]), "scGetCollidable": (vp, [H, i]),
+        "scCreateWalls": (vp, [scAabb3]), "scCreateSphere": (vp, [scVector3, f]),
+        "scCreateCapsule": (vp, [scVector3, f, f, scQuaternion]), "scCreateSDF": (vp, [H]),
+        "scCreateMesh": (vp, [H]), "scGetCollidableType": (i, [vp]), "scGetMeshTree": (vp, [vp]),
+        "scGetWallsInfo": (None, [vp, C.POINTER(scAabb3)]),
+        "scGetSphereInfo": (None, [vp, C.POINTER(f), P3]),
+        "scGetCapsuleInfo": (None, [vp, C.POINTER(f), C.POINTER(f), P3, C.POINTER(scQuaternion)]),
+        "scCreateClothPatch": (H, [H]), "scFreeClothPatch": (None, [H]), "scGetClothModel": (H, [H]),
+        "scGetClothMesh": (H, [H]), "scInitCloth": (None, [H, i, i, f, scVector3, b, b]),
+        "scStepCloth": (None, [H, f]), "scUpdateClothMesh": (None, [H]),
+        "scComputeStrainMap": (None, [H, P3, i]),
+        "scGetNumParticles": (i, [H]), "scGetParticlePos": (None, [H, i, P3]),
+        "scGetParticleInvMass": (f, [H, i]), "scGetNumTris": (i, [H]),
+        "scGetTriangle": (None, [H, i, C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
+        "scGetTriangleInfo": (None, [H, i, C.POINTER(f), C.POINTER(scVector2), C.POINTER(scVector2),
+                                     C.POINTER(scVector2)]),
+        "scGetNumQuads": (i, [H]), "scGetQuad": (None, [H, i] + [C.POINTER(i)] * 4),
+        "scGetNumLinks": (i, [H]), "scGetLink": (None, [H, i, C.POINTER(i), C.POINTER(i)]),
+        "scGetNumContacts": (i, [H]), "scGetContact": (None, [H, i, P3, P3]),
+        "scGetNumTriContacts": (i, [H]), "scGetTriContact": (None, [H, i, P3, P3]),
+        "scGetCollisionFlags": (i, [H]), "scSetCollisionFlags": (None, [H, i]),
+        "scGetThickness": (f, [H]), "scSetThickness": (None, [H, f]),
+        "scGetTolerance": (f, [H]), "scSetTolerance": (None, [H, f]),
+        "scGetFriction": (f, [H]), "scSetFriction": (None, [H, f]),
+        "scGetNumIterations": (i, [H]), "scSetNumIterations": (None, [H, i]),
+        "scGetUseFEM": (b, [H]), "scSetUseFEM": (None, [H, b]), "scGetClothTree": (vp, [H]),
+        "scAddMouseSpring": (None, [H, i, i, i, scVector3, scVector3]),
+        "scUpdateMouseSpring": (None, [H, scVector3]), "scRemoveMouseSpring": (None, [H]),
+        "scGetMeshVertexCount": (i, [H]), "scGetMeshIndexCount": (i, [H]),
+        "scGetMeshNormalBuffer": (P3, [H]), "scGetMeshVertexBuffer": (P3, [H]),
+        "scGetMeshIndexBuffer": (C.POINTER(C.c_uint), [H]),
+        "scSetUnit": (None, [H, f]), "scGetUnit": (f, [H]), "scSetShearStiffness": (None, [H, f]),
+        "scSetStretchStiffness": (None, [H, f]), "scSetBendStiffness": (None, [H, f]),
+        "scSetPoisson": (None, [H, f]), "scSetDihedral": (None, [H, b]), "scSetDragCoeff": (None, [H, f]),
+        "scSetNumSteps": (None, [H, i]), "scGetNumSteps": (i, [H]),
+        "scSetFixedIterations": (None, [H, b]), "scSetInitIncline": (None, [H, f]),
+        "scCreateGridSDF": (H, [vp, i, i, i, scAabb3]), "scFreeSDF": (None, [H]),
+        "scCreateMeshFromArrays": (H, [vp, i, vp, i]), "scFreeMesh": (None, [H]),
+        "scGetParticlePositions": (i, [H, vp, i]), "scSetParticlePositions": (i, [H, vp, i]),
+        "scGetParticleVelocities": (i, [H, vp, i]), "scSetParticleVelocities": (i, [H, vp, i]),
+        "scGetNumBends": (i, [H]), "scGetBends": (i, [H, vp, i]),
+        "scGetConstraintOrder": (i, [H, i, vp, i]), "scGetColouring": (i, [H, i, vp, i]),
+        "scGetSolverStats": (None, [H, vp]), "scEnableKernelTiming": (None, [H, b]),
+        "scGetKernelTiming": (i, [H, i, C.POINTER(i), C.POINTER(C.c_double)]),
+        "scSetUseGraph": (None, [H, b]), "scGetLastError": (C.c_char_p, []),
+        "scGetErrorCount": (i, []), "scGetBuildInfo": (C.c_char_p, []),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return sigs
+
+
+def lib():
+    """Load libsolidcloth.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python __graft_entry__.py` "
+                              "or `make -C solidcloth_amd`")
+        L = C.CDLL(LIB_PATH)
+        _sig(L)
+        _lib = L
+    return _lib
+
+
+def exported_names():
+    """Names bound by this module (every function of include/solidcloth.h)."""
+    return sorted(_sig(lib()).keys())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _checked(name, *args):
+    """Call a C-ABI function; raise SolidClothError if it recorded an error."""
+    L = lib()
+    before = L.scGetErrorCount()
+    r = getattr(L, name)(*args)
+    if L.scGetErrorCount() != before:
+        raise SolidClothError(L.scGetLastError().decode())
+    return r
+
+
+# ------------------------------------------------------------------ objects
+
+class SDF:
+    """Node-sampled grid SDF (Geometry/SDF.h); values (nx+1)(ny+1)(nz+1), x fastest."""
+
+    def __init__(self, values, n, bmin, bmax):
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        nx, ny, nz = n
+        if v.size != (nx + 1) * (ny + 1) * (nz + 1):
+            raise ValueError("SDF value count does not match the grid")
+        box = scAabb3(v3(bmin), v3(bmax))
+        self.h = _checked("scCreateGridSDF", _ptr(v), nx, ny, nz, box)
+        self.n, self.bmin, self.bmax = tuple(n), tuple(bmin), tuple(bmax)
+
+
+class CollisionWorld:
+    """Physics/CollisionWorld.h:10-29 over scCreateCollisionWorld."""
+
+    def __init__(self):
+        self.h = _checked("scCreateCollisionWorld")
+        self._keep = []
+
+    def AddCollidable(self, handle):
+        _checked("scAddCollidable", self.h, handle)
+
+    def ClearCollidables(self):
+        _checked("scClearCollidables", self.h)
+
+    def GetNumCollidables(self):
+        return _checked("scGetNumCollidables", self.h)
+
+    def AddWalls(self, bmin, bmax):
+        h = _checked("scCreateWalls", scAabb3(v3(bmin), v3(bmax)))
+        self.AddCollidable(h)
+        return h
+
+    def AddSphere(self, center, radius):
+        h = _checked("scCreateSphere", v3(center), float(radius))
+        self.AddCollidable(h)
+        return h
+
+    def AddSDF(self, sdf):
+        self._keep.append(sdf)
+        h = _checked("scCreateSDF", sdf.h)
+        self.AddCollidable(h)
+        return h
+
+
+class ClothModel:
+    """Physics/ClothModel.h:42-251 — parameters and particle state."""
+
+    def __init__(self, h, owner):
+        self.h = h
+        self._owner = owner  # the patch owns the model; keep it alive
+
+    def _set(self, name, v):
+        _checked(name, self.h, v)
+
+    SetThickness = lambda self, v: self._set("scSetThickness", float(v))
+    SetTolerance = lambda self, v: self._set("scSetTolerance", float(v))
+    SetFriction = lambda self, v: self._set("scSetFriction", float(v))
+    SetNumIterations = lambda self, v: self._set("scSetNumIterations", int(v))
+    SetFEM = lambda self, v: self._set("scSetUseFEM", bool(v))
+    SetCollisionFlags = lambda self, v: self._set("scSetCollisionFlags", int(v))
+    SetUnit = lambda self, v: self._set("scSetUnit", float(v))
+    SetShearStiffness = lambda self, v: self._set("scSetShearStiffness", float(v))
+    SetStretchStiffness = lambda self, v: self._set("scSetStretchStiffness", float(v))
+    SetBendStiffness = lambda self, v: self._set("scSetBendStiffness", float(v))
+    SetPoisson = lambda self, v: self._set("scSetPoisson", float(v))
+    SetDihedral = lambda self, v: self._set("scSetDihedral", bool(v))
+    SetDragCoeff = lambda self, v: self._set("scSetDragCoeff", float(v))
+    SetFixedIterations = lambda self, v: self._set("scSetFixedIterations", bool(v))
+    SetUseGraph = lambda self, v: self._set("scSetUseGraph", bool(v))
+
+    def GetThickness(self):
+        return _checked("scGetThickness", self.h)
+
+    def GetNumIterations(self):
+        return _checked("scGetNumIterations", self.h)
+
+    def GetNumParticles(self):
+        return _checked("scGetNumParticles", self.h)
+
+    def GetNumTris(self):
+        return _checked("scGetNumTris", self.h)
+
+    def GetNumLinks(self):
+        return _checked("scGetNumLinks", self.h)
+
+    def GetNumQuads(self):
+        return _checked("scGetNumQuads", self.h)
+
+    def GetNumBends(self):
+        return _checked("scGetNumBends", self.h)
+
+    def GetParticlePos(self, i):
+        p = scVector3()
+        _checked("scGetParticlePos", self.h, i, C.byref(p))
+        return (p.x, p.y, p.z)
+
+    def GetParticleInvMass(self, i):
+        return _checked("scGetParticleInvMass", self.h, i)
+
+    def positions(self):
+        n = self.GetNumParticles()
+        out = np.zeros((n, 3), dtype=np.float32)
+        _checked("scGetParticlePositions", self.h, _ptr(out), n)
+        return out
+
+    def set_positions(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        _checked("scSetParticlePositions", self.h, _ptr(a), a.shape[0])
+
+    def velocities(self):
+        n = self.GetNumParticles()
+        out = np.zeros((n, 3), dtype=np.float32)
+        _checked("scGetParticleVelocities", self.h, _ptr(out), n)
+        return out
+
+    def set_velocities(self, v):
+        a = np.ascontiguousarray(v, dtype=np.float32)
+        _checked("scSetParticleVelocities", self.h, _ptr(a), a.shape[0])
+
+    def inv_masses(self):
+        return np.array([self.GetParticleInvMass(i) for i in range(self.GetNumParticles())], dtype=np.float32)
+
+    def triangles(self):
+        n = self.GetNumTris()
+        out = np.zeros((n, 3), dtype=np.int32)
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        for k in range(n):
+            _checked("scGetTriangle", self.h, k, C.byref(a), C.byref(b), C.byref(c))
+            out[k] = (a.value, b.value, c.value)
+        return out
+
+    def bends(self):
+        n = self.GetNumBends()
+        out = np.zeros((n, 4), dtype=np.int32)
+        _checked("scGetBends", self.h, _ptr(out), n)
+        return out
+
+    def constraint_order(self, kind):
+        k = 0 if kind in (0, "tri") else 1
+        n = _checked("scGetConstraintOrder", self.h, k, None, 0)
+        out = np.zeros(n, dtype=np.int32)
+        _checked("scGetConstraintOrder", self.h, k, _ptr(out), n)
+        return out
+
+    def colour_offsets(self, kind):
+        k = 0 if kind in (0, "tri") else 1
+        nc = _checked("scGetColouring", self.h, k, None, 0)
+        out = np.zeros(nc + 1, dtype=np.int32)
+        _checked("scGetColouring", self.h, k, _ptr(out), nc)
+        return out
+
+    def contacts(self):
+        n = _checked("scGetNumContacts", self.h)
+        pts = np.zeros((n, 3), dtype=np.float32)
+        nrm = np.zeros((n, 3), dtype=np.float32)
+        p, q = scVector3(), scVector3()
+        for k in range(n):
+            _checked("scGetContact", self.h, k, C.byref(p), C.byref(q))
+            pts[k] = (p.x, p.y, p.z)
+            nrm[k] = (q.x, q.y, q.z)
+        return pts, nrm
+
+    def stats(self):
+        out = np.zeros(3, dtype=np.float32)
+        _checked("scGetSolverStats", self.h, _ptr(out))
+        return dict(iterations=int(out[0]), bendError=float(out[1]), contactError=float(out[2]))
+
+    def enable_kernel_timing(self, on=True):
+        _checked("scEnableKernelTiming", self.h, bool(on))
+
+    def kernel_timing(self, which):
+        n, ms = C.c_int(), C.c_double()
+        _checked("scGetKernelTiming", self.h, int(which), C.byref(n), C.byref(ms))
+        return n.value, ms.value
+
+
+class ClothPatch:
+    """Physics/ClothPatch.h:24-72 over the scClothPatch handle."""
+
+    def __init__(self, world):
+        self.world = world
+        self.h = _checked("scCreateClothPatch", world.h)
+        self._model = ClothModel(_checked("scGetClothModel", self.h), self)
+
+    def __del__(self):
+        try:
+            lib().scFreeClothPatch(self.h)
+        except Exception:
+            pass
+
+    def Init(self, divX, divY, inc, offset=(0, 0, 0), horizontal=True, attached=False):
+        _checked("scInitCloth", self.h, int(divX), int(divY), float(inc), v3(offset), bool(horizontal),
+                 bool(attached))
+
+    def SetInitIncline(self, incline):
+        _checked("scSetInitIncline", self.h, float(incline))
+
+    def Step(self, dt):
+        _checked("scStepCloth", self.h, float(dt))
+
+    def SetNumSteps(self, n):
+        _checked("scSetNumSteps", self.h, int(n))
+
+    def GetNumSteps(self):
+        return _checked("scGetNumSteps", self.h)
+
+    def GetModel(self):
+        return self._model
+
+    def UpdateMesh(self):
+        _checked("scUpdateClothMesh", self.h)

@@ -1,0 +1,653 @@
+// The C ABI (include/solidcloth.h). Each export mirrors one function of the
+// reference's Demo/ClothAPI.cpp (cited per function) over the MI355X-native
+// objects of model.h. Exceptions never cross the boundary: they become the
+// thread's last error and the call returns a neutral value.
+#include <cstring>
+#include <stdexcept>
+
+#include "model.h"
+
+namespace sc {
+
+static thread_local std::string g_lastError;
+static thread_local int g_errorCount = 0;
+
+void set_error(const std::string& msg)
+{
+    g_lastError = msg;
+    g_errorCount++;
+}
+
+template <class F>
+static void guard(F&& f)
+{
+    try {
+        f();
+    } catch (const std::exception& e) {
+        set_error(e.what());
+    } catch (...) {
+        set_error("unknown error");
+    }
+}
+
+template <class R, class F>
+static R guard_r(R fallback, F&& f)
+{
+    try {
+        return f();
+    } catch (const std::exception& e) {
+        set_error(e.what());
+    } catch (...) {
+        set_error("unknown error");
+    }
+    return fallback;
+}
+
+static CollisionWorld* W(scCollisionWorld w)
+{
+    if (!w.ptr) throw std::runtime_error("null collision world");
+    return (CollisionWorld*)w.ptr;
+}
+static ClothPatch* P(scClothPatch p)
+{
+    if (!p.ptr) throw std::runtime_error("null cloth patch");
+    return (ClothPatch*)p.ptr;
+}
+static ClothModel* M(scModel m)
+{
+    if (!m.ptr) throw std::runtime_error("null cloth model");
+    return (ClothModel*)m.ptr;
+}
+static Mesh* ME(scMesh m)
+{
+    if (!m.ptr) throw std::runtime_error("null mesh");
+    return (Mesh*)m.ptr;
+}
+static Collidable* C(scHandle h)
+{
+    if (!h) throw std::runtime_error("null collidable");
+    return (Collidable*)h;
+}
+static void check_index(int i, size_t n)
+{
+    if (i < 0 || (size_t)i >= n) throw std::runtime_error("index out of range");
+}
+
+}  // namespace sc
+
+using namespace sc;
+
+extern "C" {
+
+// ------------------------------------------------- world (ClothAPI.cpp:13-49)
+scCollisionWorld scCreateCollisionWorld(void)
+{
+    scCollisionWorld w;
+    w.ptr = guard_r<void*>(nullptr, [] { return (void*)new CollisionWorld(); });
+    return w;
+}
+
+void scFreeCollisionWorld(scCollisionWorld collWorld) { guard([&] { delete (CollisionWorld*)collWorld.ptr; }); }
+
+void scClearCollidables(scCollisionWorld collWorld) { guard([&] { W(collWorld)->colls.clear(); }); }
+
+int scGetNumCollidables(scCollisionWorld collWorld)
+{
+    return guard_r<int>(0, [&] { return (int)W(collWorld)->colls.size(); });
+}
+
+void scAddCollidable(scCollisionWorld collWorld, scHandle coll)
+{
+    guard([&] { W(collWorld)->colls.push_back(std::shared_ptr<Collidable>(C(coll))); });
+}
+
+scHandle scGetCollidable(scCollisionWorld collWorld, int i)
+{
+    return guard_r<scHandle>(nullptr, [&] {
+        CollisionWorld* w = W(collWorld);
+        check_index(i, w->colls.size());
+        return (scHandle)w->colls[i].get();
+    });
+}
+
+// --------------------------------------------- collidables (ClothAPI.cpp:51-74)
+scHandle scCreateWalls(scAabb3 box)
+{
+    return guard_r<scHandle>(nullptr, [&] {
+        Walls* w = new Walls();
+        w->type = CT_WALLS;
+        w->box = box;
+        return (scHandle)w;
+    });
+}
+
+scHandle scCreateSphere(scVector3 center, float radius)
+{
+    return guard_r<scHandle>(nullptr, [&] {
+        Sphere* s = new Sphere();
+        s->type = CT_SPHERE;
+        s->center = f3{center.x, center.y, center.z};
+        s->radius = radius;
+        return (scHandle)s;
+    });
+}
+
+scHandle scCreateCapsule(scVector3 c, float rad, float halfHeight, scQuaternion quat)
+{
+    return guard_r<scHandle>(nullptr, [&] {
+        Capsule* k = new Capsule();
+        k->type = CT_CAPSULE;
+        k->center = f3{c.x, c.y, c.z};
+        k->r = rad;
+        k->hh = halfHeight;
+        k->rot = quat;
+        return (scHandle)k;
+    });
+}
+
+scHandle scCreateSDF(scSDF s)
+{
+    return guard_r<scHandle>(nullptr, [&] {
+        if (!s.ptr) throw std::runtime_error("null SDF");
+        CollisionSDF* c = new CollisionSDF();
+        c->type = CT_SDF;
+        c->sdf = (SDF*)s.ptr;
+        return (scHandle)c;
+    });
+}
+
+scHandle scCreateMesh(scMesh m)
+{
+    return guard_r<scHandle>(nullptr, [&] {
+        CollisionMesh* c = new CollisionMesh();
+        c->type = CT_MESH;
+        c->mesh = ME(m);
+        return (scHandle)c;
+    });
+}
+
+// ------------------------------------------- introspection (ClothAPI.cpp:76-124)
+int scGetCollidableType(scHandle collidable)
+{
+    return guard_r<int>(CT_INVALID, [&] { return C(collidable)->type; });
+}
+
+void* scGetMeshTree(scHandle collMesh)
+{
+    // The obstacle tree is not built by this library; the reference returns
+    // nullptr for non-mesh collidables and for a mesh not yet stepped.
+    return guard_r<void*>(nullptr, [&] {
+        (void)C(collMesh);
+        return (void*)nullptr;
+    });
+}
+
+void scGetWallsInfo(scHandle walls, scAabb3* box)
+{
+    guard([&] {
+        Collidable* c = C(walls);
+        if (c->type != CT_WALLS || !box) return;
+        *box = ((Walls*)c)->box;
+    });
+}
+
+void scGetSphereInfo(scHandle sphere, float* radius, scVector3* center)
+{
+    guard([&] {
+        Collidable* c = C(sphere);
+        if (c->type != CT_SPHERE) return;
+        Sphere* s = (Sphere*)c;
+        if (center) *center = scVector3{s->center.x, s->center.y, s->center.z};
+        if (radius) *radius = s->radius;
+    });
+}
+
+void scGetCapsuleInfo(scHandle capsule, float* radius, float* halfHeight, scVector3* center, scQuaternion* rot)
+{
+    guard([&] {
+        Collidable* c = C(capsule);
+        if (c->type != CT_CAPSULE) return;
+        Capsule* k = (Capsule*)c;
+        if (radius) *radius = k->r;
+        if (halfHeight) *halfHeight = k->hh;
+        if (center) *center = scVector3{k->center.x, k->center.y, k->center.z};
+        if (rot) *rot = k->rot;
+    });
+}
+
+// ------------------------------------------------ patch (ClothAPI.cpp:126-178)
+scClothPatch scCreateClothPatch(scCollisionWorld collWorld)
+{
+    scClothPatch p;
+    p.ptr = guard_r<void*>(nullptr, [&] { return (void*)new ClothPatch(W(collWorld)); });
+    return p;
+}
+
+void scFreeClothPatch(scClothPatch patch) { guard([&] { delete (ClothPatch*)patch.ptr; }); }
+
+scModel scGetClothModel(scClothPatch patch)
+{
+    scModel m;
+    m.ptr = guard_r<void*>(nullptr, [&] { return (void*)P(patch)->mModel.get(); });
+    return m;
+}
+
+scMesh scGetClothMesh(scClothPatch patch)
+{
+    scMesh m;
+    m.ptr = guard_r<void*>(nullptr, [&] { return (void*)&P(patch)->mMesh; });
+    return m;
+}
+
+void scInitCloth(scClothPatch patch, int divX, int divY, float inc, scVector3 offset, bool horizontal, bool attached)
+{
+    guard([&] {
+        if (divX < 2 || divY < 2) throw std::runtime_error("scInitCloth needs divX, divY >= 2");
+        P(patch)->init_grid(divX, divY, inc, f3{offset.x, offset.y, offset.z}, horizontal, attached);
+    });
+}
+
+void scStepCloth(scClothPatch patch, float dt)
+{
+    guard([&] { P(patch)->step(dt); });
+}
+
+void scUpdateClothMesh(scClothPatch patch)
+{
+    guard([&] { P(patch)->update_mesh(); });
+}
+
+void scComputeStrainMap(scClothPatch patch, scVector3* colors, int count)
+{
+    guard([&] {
+        if (!colors) throw std::runtime_error("null colour buffer");
+        P(patch)->compute_strain_map(colors, count);
+    });
+}
+
+// ---------------------------------------------- model getters (ClothAPI.cpp:180-300)
+int scGetNumParticles(scModel model)
+{
+    return guard_r<int>(0, [&] { return (int)M(model)->pos.size(); });
+}
+
+void scGetParticlePos(scModel model, int i, scVector3* pos)
+{
+    guard([&] {
+        ClothModel* m = M(model);
+        check_index(i, m->pos.size());
+        m->sync_to_host();
+        if (pos) *pos = scVector3{m->pos[i].x, m->pos[i].y, m->pos[i].z};
+    });
+}
+
+float scGetParticleInvMass(scModel model, int i)
+{
+    return guard_r<float>(0.f, [&] {
+        ClothModel* m = M(model);
+        check_index(i, m->invMass.size());
+        return m->invMass[i];
+    });
+}
+
+int scGetNumTris(scModel model)
+{
+    return guard_r<int>(0, [&] { return (int)M(model)->tris.size(); });
+}
+
+void scGetTriangle(scModel model, int i, int* i1, int* i2, int* i3)
+{
+    guard([&] {
+        ClothModel* m = M(model);
+        check_index(i, m->tris.size());
+        const Triangle& t = m->tris[i];
+        if (i1) *i1 = (int)t.i1;
+        if (i2) *i2 = (int)t.i2;
+        if (i3) *i3 = (int)t.i3;
+    });
+}
+
+void scGetTriangleInfo(scModel model, int i, float* invDet, scVector2* du, scVector2* dv, scVector2* s)
+{
+    guard([&] {
+        ClothModel* m = M(model);
+        check_index(i, m->tris.size());
+        const Triangle& t = m->tris[i];
+        if (invDet) *invDet = t.invDet;
+        if (du) *du = scVector2{t.du1, t.du2};
+        if (dv) *dv = scVector2{t.dv1, t.dv2};
+        if (s) *s = scVector2{t.su, t.sv};
+    });
+}
+
+int scGetNumQuads(scModel model)
+{
+    return guard_r<int>(0, [&] { return (int)M(model)->quads.size(); });
+}
+
+void scGetQuad(scModel model, int i, int* i1, int* i2, int* i3, int* i4)
+{
+    guard([&] {
+        ClothModel* m = M(model);
+        check_index(i, m->quads.size());
+        const Quad& q = m->quads[i];
+        if (i1) *i1 = (int)q.i1;
+        if (i2) *i2 = (int)q.i2;
+        if (i3) *i3 = (int)q.i3;
+        if (i4) *i4 = (int)q.i4;
+    });
+}
+
+int scGetNumLinks(scModel model)
+{
+    return guard_r<int>(0, [&] { return (int)M(model)->links.size(); });
+}
+
+void scGetLink(scModel model, int i, int* i1, int* i2)
+{
+    guard([&] {
+        ClothModel* m = M(model);
+        check_index(i, m->links.size());
+        if (i1) *i1 = (int)m->links[i].i1;
+        if (i2) *i2 = (int)m->links[i].i2;
+    });
+}
+
+int scGetNumContacts(scModel model)
+{
+    return guard_r<int>(0, [&] {
+        ClothModel* m = M(model);
+        m->download_contacts();
+        return (int)m->contactsHost.size();
+    });
+}
+
+void scGetContact(scModel model, int i, scVector3* point, scVector3* normal)
+{
+    guard([&] {
+        ClothModel* m = M(model);
+        m->download_contacts();
+        check_index(i, m->contactsHost.size());
+        const ContactOut& c = m->contactsHost[i];
+        if (point) *point = scVector3{c.point.x, c.point.y, c.point.z};
+        if (normal) *normal = scVector3{c.normal.x, c.normal.y, c.normal.z};
+    });
+}
+
+int scGetNumTriContacts(scModel model)
+{
+    // Triangle contacts come from mesh obstacles and SDF triangle queries,
+    // which the GPU substep does not generate.
+    return guard_r<int>(0, [&] {
+        (void)M(model);
+        return 0;
+    });
+}
+
+void scGetTriContact(scModel model, int i, scVector3* point, scVector3* normal)
+{
+    guard([&] {
+        (void)M(model);
+        (void)point;
+        (void)normal;
+        check_index(i, 0);
+    });
+}
+
+// ------------------------------------------------- parameters (ClothAPI.cpp:302-380)
+int scGetCollisionFlags(scModel model) { return guard_r<int>(0, [&] { return M(model)->mCollFlags; }); }
+void scSetCollisionFlags(scModel model, int val) { guard([&] { M(model)->mCollFlags = val; }); }
+float scGetThickness(scModel model) { return guard_r<float>(0.f, [&] { return M(model)->mThickness; }); }
+void scSetThickness(scModel model, float val) { guard([&] { M(model)->mThickness = val; }); }
+float scGetTolerance(scModel model) { return guard_r<float>(0.f, [&] { return M(model)->mTolerance; }); }
+void scSetTolerance(scModel model, float val) { guard([&] { M(model)->mTolerance = val; }); }
+float scGetFriction(scModel model) { return guard_r<float>(0.f, [&] { return M(model)->mFriction; }); }
+void scSetFriction(scModel model, float val) { guard([&] { M(model)->mFriction = val; }); }
+int scGetNumIterations(scModel model) { return guard_r<int>(0, [&] { return M(model)->mNumIterations; }); }
+void scSetNumIterations(scModel model, int val) { guard([&] { M(model)->mNumIterations = val; }); }
+bool scGetUseFEM(scModel model) { return guard_r<bool>(false, [&] { return M(model)->mFEM; }); }
+void scSetUseFEM(scModel model, bool val) { guard([&] { M(model)->mFEM = val; }); }
+
+// ------------------------------------------ tree and mouse spring (ClothAPI.cpp:382-410)
+void* scGetClothTree(scModel model)
+{
+    // No cloth AABB tree exists on this path: the reference rebuilds one per
+    // substep but nothing on the external-contact path consumes it.
+    return guard_r<void*>(nullptr, [&] {
+        (void)M(model);
+        return (void*)nullptr;
+    });
+}
+
+// Physics/ClothModel.inl:86-98
+void scAddMouseSpring(scModel model, int i1, int i2, int i3, scVector3 coords, scVector3 p)
+{
+    guard([&] {
+        ClothModel* m = M(model);
+        m->msActive = true;
+        m->msLen = 0.f;
+        m->msStiffness = 100.f;
+        m->msI[0] = i1; m->msI[1] = i2; m->msI[2] = i3;
+        m->msA[0] = coords.x; m->msA[1] = coords.y; m->msA[2] = coords.z;
+        m->msPoint = f3{p.x, p.y, p.z};
+    });
+}
+
+void scUpdateMouseSpring(scModel model, scVector3 p)
+{
+    guard([&] {
+        ClothModel* m = M(model);
+        if (m->msActive) m->msPoint = f3{p.x, p.y, p.z};
+    });
+}
+
+void scRemoveMouseSpring(scModel model) { guard([&] { M(model)->msActive = false; }); }
+
+// -------------------------------------------------- mesh buffers (ClothAPI.cpp:412-440)
+int scGetMeshVertexCount(scMesh mesh) { return guard_r<int>(0, [&] { return (int)ME(mesh)->vertices.size(); }); }
+int scGetMeshIndexCount(scMesh mesh) { return guard_r<int>(0, [&] { return (int)ME(mesh)->indices.size(); }); }
+scVector3* scGetMeshNormalBuffer(scMesh mesh)
+{
+    return guard_r<scVector3*>(nullptr, [&] { return ME(mesh)->normals.empty() ? nullptr : ME(mesh)->normals.data(); });
+}
+scVector3* scGetMeshVertexBuffer(scMesh mesh)
+{
+    return guard_r<scVector3*>(nullptr,
+                               [&] { return ME(mesh)->vertices.empty() ? nullptr : ME(mesh)->vertices.data(); });
+}
+unsigned int* scGetMeshIndexBuffer(scMesh mesh)
+{
+    return guard_r<unsigned int*>(nullptr,
+                                  [&] { return ME(mesh)->indices.empty() ? nullptr : ME(mesh)->indices.data(); });
+}
+
+// ================================================================ extensions
+void scSetUnit(scModel model, float val) { guard([&] { M(model)->mUnit = val; }); }
+float scGetUnit(scModel model) { return guard_r<float>(0.f, [&] { return M(model)->mUnit; }); }
+void scSetShearStiffness(scModel model, float val) { guard([&] { M(model)->mShearStiff = val; }); }
+void scSetStretchStiffness(scModel model, float val) { guard([&] { M(model)->mStretchStiff = val; }); }
+void scSetBendStiffness(scModel model, float val) { guard([&] { M(model)->mBendStiff = val; }); }
+void scSetPoisson(scModel model, float val) { guard([&] { M(model)->mPoisson = val; }); }
+void scSetDihedral(scModel model, bool val) { guard([&] { M(model)->mDihedral = val; }); }
+void scSetDragCoeff(scModel model, float val) { guard([&] { M(model)->mDragCoeff = val; }); }
+void scSetNumSteps(scClothPatch patch, int val)
+{
+    guard([&] {
+        if (val < 1) throw std::runtime_error("NumSteps must be >= 1");
+        P(patch)->mNumSteps = val;
+    });
+}
+int scGetNumSteps(scClothPatch patch) { return guard_r<int>(0, [&] { return P(patch)->mNumSteps; }); }
+void scSetFixedIterations(scModel model, bool fixed) { guard([&] { M(model)->mFixedIterations = fixed; }); }
+void scSetInitIncline(scClothPatch patch, float incline) { guard([&] { P(patch)->mIncline = incline; }); }
+
+scSDF scCreateGridSDF(const float* values, int nx, int ny, int nz, scAabb3 box)
+{
+    scSDF s;
+    s.ptr = guard_r<void*>(nullptr, [&] {
+        if (!values || nx < 1 || ny < 1 || nz < 1) throw std::runtime_error("bad grid SDF arguments");
+        SDF* g = new SDF();
+        try {
+            g->create(nx, ny, nz, box);
+            memcpy(g->vals.data(), values, g->vals.size() * sizeof(float));
+            g->compute_gradient();
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        return (void*)g;
+    });
+    return s;
+}
+
+void scFreeSDF(scSDF sdf) { guard([&] { delete (SDF*)sdf.ptr; }); }
+
+scMesh scCreateMeshFromArrays(const scVector3* vertices, int numVertices, const unsigned int* indices, int numIndices)
+{
+    scMesh m;
+    m.ptr = guard_r<void*>(nullptr, [&] {
+        if (!vertices || !indices || numVertices < 0 || numIndices < 0 || numIndices % 3)
+            throw std::runtime_error("bad mesh arrays");
+        for (int i = 0; i < numIndices; i++)
+            if ((int)indices[i] >= numVertices) throw std::runtime_error("mesh index out of range");
+        Mesh* me = new Mesh();
+        me->vertices.assign(vertices, vertices + numVertices);
+        me->indices.assign(indices, indices + numIndices);
+        me->compute_normals();
+        me->construct_edges();
+        return (void*)me;
+    });
+    return m;
+}
+
+void scFreeMesh(scMesh mesh) { guard([&] { delete (Mesh*)mesh.ptr; }); }
+
+int scGetParticlePositions(scModel model, scVector3* out, int count)
+{
+    return guard_r<int>(-1, [&] {
+        ClothModel* m = M(model);
+        m->sync_to_host();
+        int n = std::min(count, (int)m->pos.size());
+        for (int i = 0; i < n; i++) out[i] = scVector3{m->pos[i].x, m->pos[i].y, m->pos[i].z};
+        return n;
+    });
+}
+
+int scSetParticlePositions(scModel model, const scVector3* in, int count)
+{
+    return guard_r<int>(-1, [&] {
+        ClothModel* m = M(model);
+        m->sync_to_host();
+        int n = std::min(count, (int)m->pos.size());
+        for (int i = 0; i < n; i++) m->pos[i] = f3{in[i].x, in[i].y, in[i].z};
+        m->mark_host_dirty();
+        return n;
+    });
+}
+
+int scGetParticleVelocities(scModel model, scVector3* out, int count)
+{
+    return guard_r<int>(-1, [&] {
+        ClothModel* m = M(model);
+        m->sync_to_host();
+        int n = std::min(count, (int)m->vel.size());
+        for (int i = 0; i < n; i++) out[i] = scVector3{m->vel[i].x, m->vel[i].y, m->vel[i].z};
+        return n;
+    });
+}
+
+int scSetParticleVelocities(scModel model, const scVector3* in, int count)
+{
+    return guard_r<int>(-1, [&] {
+        ClothModel* m = M(model);
+        m->sync_to_host();
+        int n = std::min(count, (int)m->vel.size());
+        for (int i = 0; i < n; i++) m->vel[i] = f3{in[i].x, in[i].y, in[i].z};
+        m->mark_host_dirty();
+        return n;
+    });
+}
+
+int scGetNumBends(scModel model) { return guard_r<int>(0, [&] { return (int)M(model)->bends.size(); }); }
+
+int scGetBends(scModel model, int* out, int count)
+{
+    return guard_r<int>(-1, [&] {
+        ClothModel* m = M(model);
+        int n = std::min(count, (int)m->bends.size());
+        for (int i = 0; i < n; i++) {
+            out[4 * i] = (int)m->bends[i].i1;
+            out[4 * i + 1] = (int)m->bends[i].i2;
+            out[4 * i + 2] = (int)m->bends[i].i3;
+            out[4 * i + 3] = (int)m->bends[i].i4;
+        }
+        return n;
+    });
+}
+
+int scGetConstraintOrder(scModel model, int kind, int* order, int count)
+{
+    return guard_r<int>(-1, [&] {
+        ClothModel* m = M(model);
+        m->ensure_colouring();
+        const Colouring& c = kind == 0 ? m->triCol : m->bendCol;
+        int n = std::min(count, (int)c.order.size());
+        if (order) memcpy(order, c.order.data(), n * sizeof(int));
+        return (int)c.order.size();
+    });
+}
+
+int scGetColouring(scModel model, int kind, int* colourOffsets, int maxColours)
+{
+    return guard_r<int>(-1, [&] {
+        ClothModel* m = M(model);
+        m->ensure_colouring();
+        const Colouring& c = kind == 0 ? m->triCol : m->bendCol;
+        int nc = (int)c.offsets.size() - 1;
+        if (nc < 0) nc = 0;
+        if (colourOffsets)
+            for (int k = 0; k <= nc && k <= maxColours; k++) colourOffsets[k] = c.offsets[k];
+        return nc;
+    });
+}
+
+void scGetSolverStats(scModel model, float* out)
+{
+    guard([&] {
+        ClothModel* m = M(model);
+        m->fetch_stats();
+        out[0] = (float)m->statIterations;
+        out[1] = m->statBendError;
+        out[2] = m->statContactError;
+    });
+}
+
+void scEnableKernelTiming(scModel model, bool enable)
+{
+    guard([&] {
+        ClothModel* m = M(model);
+        m->gpu.timer.enabled = enable;
+        m->gpu.timer.reset();
+    });
+}
+
+int scGetKernelTiming(scModel model, int which, int* launches, double* totalMs)
+{
+    return guard_r<int>(-1, [&] {
+        ClothModel* m = M(model);
+        if (which < 0 || which > 4) throw std::runtime_error("kernel id out of range");
+        if (launches) *launches = m->gpu.timer.launches[which];
+        if (totalMs) *totalMs = m->gpu.timer.totalMs[which];
+        return 0;
+    });
+}
+
+void scSetUseGraph(scModel model, bool enable) { guard([&] { M(model)->mUseGraph = enable; }); }
+
+const char* scGetLastError(void) { return g_lastError.c_str(); }
+
+int scGetErrorCount(void) { return g_errorCount; }
+
+const char* scGetBuildInfo(void) { return "solidcloth-mi355x gfx950 fp32 coloured-GS"; }
+
+}  // extern "C"

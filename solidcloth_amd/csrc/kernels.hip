@@ -1,0 +1,401 @@
+// gfx950 kernels for one position-based-dynamics cloth substep.
+//
+// Each kernel restates one loop of the reference solver; the per-element
+// arithmetic follows the reference's operation order exactly (no
+// contraction, correctly rounded division and square root), so a kernel that
+// processes a set of vertex-disjoint constraints in parallel produces the
+// same bits as the reference processing them one after another.
+#include "kernels.h"
+
+namespace sc {
+
+static constexpr int kBlock = 256;
+
+__device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
+__device__ __forceinline__ float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+// Wave-wide maximum of non-negative, NaN-free values, then one atomic per wave.
+__device__ __forceinline__ void wave_max_atomic(unsigned* dst, float v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        float w = __shfl_xor(v, o, 64);
+        v = v < w ? w : v;
+    }
+    if ((threadIdx.x & 63) == 0 && v > 0.f) atomicMax(dst, __float_as_uint(v));
+}
+
+// Physics/ClothPBD.cpp:258-266 — symplectic Euler prediction. Thread 0 also
+// resets the solver control block for the substep.
+__global__ void k_predict(float4* __restrict__ pos, float4* __restrict__ prev, float4* __restrict__ vel, int n, float h,
+                          f3 gdv, float drag, SolverCtl* ctl)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        ctl->done = 0;
+        ctl->k = 0;
+        ctl->cb = 0;
+        ctl->cv = 0;
+        ctl->iterations = 0;
+    }
+    if (i >= n) return;
+    float4 P = pos[i];
+    prev[i] = make_float4(P.x, P.y, P.z, 0.f);
+    if (P.w == 0.f) return;
+    float4 V = vel[i];
+    f3 v = add(xyz(V), gdv);
+    v = f3{v.x * drag, v.y * drag, v.z * drag};
+    f3 p = add(xyz(P), mul(h, v));
+    vel[i] = f4(v, 0.f);
+    pos[i] = f4(p, P.w);
+}
+
+// Geometry/SDF.h:105-155 — trilinear value and gradient of a node-sampled
+// grid. Returns false when the cell lies outside (value FLT_MAX, gradient 0).
+__device__ __forceinline__ bool sdf_lookup(const CollDev& c, f3 p, float& val, f3& g)
+{
+    float rx = (p.x - c.ax) / c.bx;
+    float ry = (p.y - c.ay) / c.by;
+    float rz = (p.z - c.az) / c.bz;
+    float flx = floorf(rx), fly = floorf(ry), flz = floorf(rz);
+    if (!(flx >= 0.f && flx <= (float)c.nc0 && fly >= 0.f && fly <= (float)c.nc1 && flz >= 0.f &&
+          flz <= (float)c.nc2))
+        return false;
+    int x = (int)flx, y = (int)fly, z = (int)flz;
+    float fx = fabsf(rx - (float)x), fy = fabsf(ry - (float)y), fz = fabsf(rz - (float)z);
+    float4 nd[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        // term order of the reference: (x,y,z) (x,y,z+1) (x,y+1,z) ... (x+1,y+1,z+1)
+        int dx = (k >> 2) & 1, dy = (k >> 1) & 1, dz = k & 1;
+        int X = x + dx, Y = y + dy, Z = z + dz;
+        bool out = X > c.nc0 || Y > c.nc1 || Z > c.nc2;
+        nd[k] = out ? make_float4(0.f, 0.f, 0.f, 0.f) : c.nodes[X + Y * c.nv0 + Z * c.nv0 * c.nv1];
+    }
+    const float ox = 1.f - fx, oy = 1.f - fy, oz = 1.f - fz;
+    float w[8] = {ox * oy * oz, ox * oy * fz, ox * fy * oz, ox * fy * fz,
+                  fx * oy * oz, fx * oy * fz, fx * fy * oz, fx * fy * fz};
+    float v = w[0] * nd[0].w;
+    f3 gr = mul(w[0], xyz(nd[0]));
+#pragma unroll
+    for (int k = 1; k < 8; k++) {
+        v = v + w[k] * nd[k].w;
+        gr = add(gr, mul(w[k], xyz(nd[k])));
+    }
+    val = v;
+    g = gr;
+    return true;
+}
+
+__device__ __forceinline__ void put_contact(ContactBuf cb, int v, int& cnt, f3 p, f3 n, int ci)
+{
+    cb.P[cnt * cb.nv + v] = f4(p, 0.f);
+    cb.N[cnt * cb.nv + v] = f4(n, 0.f);
+    cb.src[cnt * cb.nv + v] = ci;
+    cnt++;
+}
+
+// Physics/ClothCollision.cpp:50-87, 199-220, 430-470 — external vertex
+// contacts, one thread per vertex, collidables visited in world order.
+__global__ void k_detect(const float4* __restrict__ pos, const float4* __restrict__ prev, int n, CollSet cs,
+                         ContactBuf cb, float thickness, float tolerance, int collFlags)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    f3 p = xyz(pos[v]);
+    f3 pv = xyz(prev[v]);
+    int cnt = 0;
+    const float tv = len(sub(pv, p));
+    const float radTol = thickness + smax(tolerance, tv * 1.1f);
+    for (int ci = 0; ci < cs.n; ci++) {
+        const CollDev& c = cs.c[ci];
+        if (c.type == CT_WALLS) {
+            if (!(collFlags & CF_WALLS)) continue;
+            const float r = thickness + tolerance;
+            if (p.y - r <= c.ay) put_contact(cb, v, cnt, f3{p.x, c.ay, p.z}, f3{0.f, 1.f, 0.f}, ci);
+            if (p.x - r <= c.ax) put_contact(cb, v, cnt, f3{c.ax, p.y, p.z}, f3{1.f, 0.f, 0.f}, ci);
+            if (p.x + r >= c.bx) put_contact(cb, v, cnt, f3{c.bx, p.y, p.z}, f3{-1.f, 0.f, 0.f}, ci);
+            if (p.z - r <= c.az) put_contact(cb, v, cnt, f3{p.x, p.y, c.az}, f3{0.f, 0.f, 1.f}, ci);
+            if (p.z + r >= c.bz) put_contact(cb, v, cnt, f3{p.x, p.y, c.bz}, f3{0.f, 0.f, -1.f}, ci);
+        } else if (c.type == CT_SPHERE) {
+            if (!(collFlags & CF_VERTICES)) continue;
+            // Physics/ClothCollisionSDF.h:14-24
+            f3 ctr{c.ax, c.ay, c.az};
+            f3 d = sub(pv, ctr);
+            float l = len(d);
+            d = mul(1.f / l, d);
+            f3 cp = add(ctr, mul(c.r, d));
+            float dist = l - c.r;
+            if (dist < radTol && dist >= 0.f) put_contact(cb, v, cnt, cp, d, ci);
+        } else if (c.type == CT_SDF) {
+            if (!(collFlags & CF_VERTICES)) continue;
+            // Physics/ClothCollisionSDF.h:130-139
+            float val;
+            f3 g;
+            if (!sdf_lookup(c, pv, val, g)) continue;  // value FLT_MAX: never a contact
+            g = normalized(g);
+            f3 cp = sub(pv, mul(val, g));
+            if (val < radTol && val >= 0.f) put_contact(cb, v, cnt, cp, g, ci);
+        }
+    }
+    cb.count[v] = cnt;
+}
+
+// Physics/ClothPBD.cpp:52-108 — FEM triangle projection (warp, weft, shear),
+// one thread per triangle of one colour.
+__global__ void __launch_bounds__(kBlock) k_fem(float4* __restrict__ pos, TriStreams tr, int start, int count,
+                                                float h2, float epsUV, const SolverCtl* __restrict__ ctl)
+{
+    if (ctl->done) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    t += start;
+    const int4 id = tr.idx[t];
+    const float4 A = tr.a[t];
+    const float4 B = tr.b[t];
+    const float2 C = tr.c[t];
+    const float du1 = A.x, du2 = A.y, dv1 = A.z, dv2 = A.w;
+    const float invDet = B.x, lu2 = B.y, lv2 = B.z, area = B.w;
+    const float lu0 = C.x, lv0 = C.y;
+    float4 P1 = pos[id.x], P2 = pos[id.y], P3 = pos[id.z];
+    const float m1 = P1.w, m2 = P2.w, m3 = P3.w;
+    f3 x1 = xyz(P1), x2 = xyz(P2), x3 = xyz(P3);
+    f3 dx1 = sub(x2, x1);
+    f3 dx2 = sub(x3, x1);
+    f3 wu = mul(invDet, sub(mul(dv2, dx1), mul(dv1, dx2)));
+    f3 wv = mul(invDet, add(mul(-du2, dx1), mul(du1, dx2)));
+    float err, diag, lambda;
+    f3 q1, q2, q3;
+    // warp
+    q2 = mul(invDet * dv2, wu);
+    q3 = mul(-invDet * dv1, wu);
+    q1 = sub(neg(q2), q3);
+    diag = dot(mul(m1, q1), q1) + dot(mul(m2, q2), q2) + dot(mul(m3, q3), q3);
+    err = 0.5f * (dot(wu, wu) - lu2);
+    lambda = -err / diag;
+    x1 = add(x1, mul(m1 * lambda, q1));
+    x2 = add(x2, mul(m2 * lambda, q2));
+    x3 = add(x3, mul(m3 * lambda, q3));
+    // weft
+    q2 = mul(-invDet * du2, wv);
+    q3 = mul(invDet * du1, wv);
+    q1 = sub(neg(q2), q3);
+    diag = dot(mul(m1, q1), q1) + dot(mul(m2, q2), q2) + dot(mul(m3, q3), q3);
+    err = 0.5f * (dot(wv, wv) - lv2);
+    lambda = -err / diag;
+    x1 = add(x1, mul(m1 * lambda, q1));
+    x2 = add(x2, mul(m2 * lambda, q2));
+    x3 = add(x3, mul(m3 * lambda, q3));
+    // shear
+    q2 = sub(mul(dv2, wv), mul(du2, wu));
+    q3 = add(mul(-dv1, wv), mul(du1, wu));
+    q1 = sub(neg(q2), q3);
+    diag = invDet * (dot(mul(m1, q1), q1) + dot(mul(m2, q2), q2) + dot(mul(m3, q3), q3));
+    err = dot(wu, wv);
+    float s = 1.f / (1.f + lu0 * lv0 * epsUV / (diag * h2 * area));
+    lambda = -s * err / diag;
+    x1 = add(x1, mul(m1 * lambda, q1));
+    x2 = add(x2, mul(m2 * lambda, q2));
+    x3 = add(x3, mul(m3 * lambda, q3));
+    pos[id.x] = f4(x1, m1);
+    pos[id.y] = f4(x2, m2);
+    pos[id.z] = f4(x3, m3);
+}
+
+// Physics/ClothPBD.cpp:179-224 with EvaluateAngle (110-139) and
+// EvaluateGradient (141-177) — dihedral bending, one thread per bend.
+__global__ void __launch_bounds__(kBlock) k_bend(float4* __restrict__ pos, const int4* __restrict__ bidx, int start,
+                                                 int count, float sStiff, SolverCtl* ctl)
+{
+    if (ctl->done) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    float errOut = 0.f;
+    if (t < count) {
+        const int4 b = bidx[start + t];
+        float4 P1 = pos[b.x], P2 = pos[b.y], P3 = pos[b.z], P4 = pos[b.w];
+        const float m1 = P1.w, m2 = P2.w, m3 = P3.w, m4 = P4.w;
+        const f3 x1 = xyz(P1), x2 = xyz(P2), x3 = xyz(P3), x4 = xyz(P4);
+        bool active = !(m1 + m2 + m3 + m4 == 0.f);
+        f3 x12 = sub(x2, x1);
+        float l = len(x12);
+        f3 x13 = sub(x3, x1);
+        f3 n1 = cross(x12, x13);
+        float l1 = len(n1);
+        f3 x14 = sub(x4, x1);
+        f3 n2 = cross(x14, x12);
+        float l2 = len(n2);
+        float cosine = 1.f, theta = 0.f;
+        if (!(l1 == 0.f || l2 == 0.f)) {
+            float il1 = 1.f / l1, il2 = 1.f / l2;
+            f3 n1n = mul(il1, n1), n2n = mul(il2, n2);
+            cosine = dot(n1n, n2n);
+            cosine = smin(1.f, smax(-1.f, cosine));
+            theta = acos_as(cosine);
+        }
+        if (fabsf(cosine) >= 1 - 0.001f) active = false;
+        if (active) {
+            float h1 = l1 / l, h2 = l2 / l;
+            float he = 0.5f * (h1 + h2);
+            float il1 = 1.f / l1, il2 = 1.f / l2;
+            f3 n1n = mul(il1, n1), n2n = mul(il2, n2);
+            f3 n1p = sub(n1n, mul(cosine, n2n));
+            f3 n2p = sub(n2n, mul(cosine, n1n));
+            float sine = sqrtf(1 - cosine * cosine);
+            f3 q3 = mul(il1 / sine / he, cross(x12, n2p));
+            f3 q4 = mul(-il2 / sine / he, cross(x12, n1p));
+            f3 q2 = add(mul(-il1 / sine / he, cross(x13, n2p)), mul(il2 / sine / he, cross(x14, n1p)));
+            f3 q1 = sub(sub(neg(q2), q3), q4);
+            float diag = m1 * dot(q1, q1) + m2 * dot(q2, q2) + m3 * dot(q3, q3) + m4 * dot(q4, q4);
+            if (!(diag == 0.f)) {
+                float e = fabsf(sStiff * theta);
+                errOut = e > 0.f ? e : 0.f;
+                float lambda = -sStiff * theta / he / diag;
+                pos[b.x] = f4(add(x1, mul(lambda * m1, q1)), m1);
+                pos[b.y] = f4(add(x2, mul(lambda * m2, q2)), m2);
+                pos[b.z] = f4(add(x3, mul(lambda * m3, q3)), m3);
+                pos[b.w] = f4(add(x4, mul(lambda * m4, q4)), m4);
+            }
+        }
+    }
+    wave_max_atomic(&ctl->cb, errOut);
+}
+
+// Physics/ClothCollisionHandler.cpp:134-202 — vertex contacts with friction.
+// One thread per vertex applies that vertex's contacts in list order;
+// contacts of different vertices commute, so this equals the reference's
+// sequential pass over the contact list.
+__global__ void __launch_bounds__(kBlock) k_contacts(float4* __restrict__ pos, const float4* __restrict__ prev,
+                                                     float4* __restrict__ vel, ContactBuf cb, float thickness,
+                                                     float mu, float invH, SolverCtl* ctl)
+{
+    if (ctl->done) return;
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    float err = 0.f;
+    int n = v < cb.nv ? cb.count[v] : 0;
+    if (n > 0) {
+        float4 P = pos[v];
+        f3 p = xyz(P);
+        const f3 pp = xyz(prev[v]);
+        f3 ve = xyz(vel[v]);
+        for (int s = 0; s < n; s++) {
+            float4 CP = cb.P[s * cb.nv + v];
+            float4 CN = cb.N[s * cb.nv + v];
+            f3 cpt = xyz(CP), nn = xyz(CN);
+            float lam = CP.w, lamF = CN.w;
+            f3 delta = sub(p, cpt);
+            float l = dot(delta, nn);
+            if (l > thickness) continue;
+            float depth = thickness - l;
+            if (fabsf(depth) > err) err = fabsf(depth);
+            lam += depth;
+            f3 disp = mul(depth, nn);
+            f3 v12 = sub(p, pp);
+            float vnrel = dot(nn, v12);
+            f3 vt = sub(v12, mul(vnrel, nn));
+            float vtrel = len(vt);
+            const float limit = mu * lam;
+            if (vtrel > 0.001f) {
+                float dLF = vtrel;
+                float lf0 = lamF;
+                lamF = lf0 + dLF;
+                if (lamF >= limit) lamF = limit;
+                dLF = lamF - lf0;
+                vt = mul(1.0f / vtrel, vt);
+                disp = add(disp, mul(-dLF, vt));
+            }
+            p = add(p, disp);
+            ve = add(ve, mul(invH, disp));
+            cb.P[s * cb.nv + v] = make_float4(CP.x, CP.y, CP.z, lam);
+            cb.N[s * cb.nv + v] = make_float4(CN.x, CN.y, CN.z, lamF);
+        }
+        pos[v] = f4(p, P.w);
+        vel[v] = f4(ve, 0.f);
+    }
+    wave_max_atomic(&ctl->cv, err);
+}
+
+// Physics/ClothPBD.cpp:354-361 — iteration counter and the early-exit test.
+// Edge, triangle and self contacts are not generated on this path and the
+// FEM model reports no link error, so those terms of the test are 0.
+__global__ void k_iter_end(SolverCtl* ctl, int maxIter, int fixedIter, int bendFreq, int dihedral, float selfThr)
+{
+    if (ctl->done) return;
+    int k = ctl->k + 1;
+    ctl->k = k;
+    float cb = __uint_as_float(ctl->cb), cv = __uint_as_float(ctl->cv);
+    const float cl = 0.f, ce = 0.f, ct = 0.f, sct = 0.f, sce = 0.f, eps = 0.01f;
+    bool exitNow = !fixedIter && cl < 0.05f && cb < 0.005f && sce < selfThr && sct < selfThr && cv < eps &&
+                   ce < eps && ct < eps;
+    if (exitNow || k >= maxIter) {
+        ctl->done = 1;
+        ctl->iterations = k;
+        ctl->bendError = cb;
+        ctl->contactError = cv;
+        return;
+    }
+    ctl->cv = 0;
+    if (dihedral && k % bendFreq == 0) ctl->cb = 0;
+}
+
+// Physics/ClothPBD.cpp:314-321
+__global__ void k_velocity(const float4* __restrict__ pos, const float4* __restrict__ prev, float4* __restrict__ vel,
+                           int n, float invH)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 P = pos[i];
+    if (P.w == 0.f) return;
+    vel[i] = f4(mul(invH, sub(xyz(P), xyz(prev[i]))), 0.f);
+}
+
+static inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
+
+void launch_predict(hipStream_t s, float4* pos, float4* prev, float4* vel, int n, float h, f3 gdv, float drag,
+                    SolverCtl* ctl)
+{
+    int nb = blocks(n > 0 ? n : 1);
+    hipLaunchKernelGGL(k_predict, dim3(nb), dim3(kBlock), 0, s, pos, prev, vel, n, h, gdv, drag, ctl);
+}
+
+void launch_detect(hipStream_t s, const float4* pos, const float4* prev, int n, const CollSet& cs, ContactBuf cb,
+                   float thickness, float tolerance, int collFlags)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_detect, dim3(blocks(n)), dim3(kBlock), 0, s, pos, prev, n, cs, cb, thickness, tolerance,
+                       collFlags);
+}
+
+void launch_fem(hipStream_t s, float4* pos, TriStreams tr, int start, int count, float h2, float epsUV,
+                const SolverCtl* ctl)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_fem, dim3(blocks(count)), dim3(kBlock), 0, s, pos, tr, start, count, h2, epsUV, ctl);
+}
+
+void launch_bend(hipStream_t s, float4* pos, const int4* bidx, int start, int count, float stiff, SolverCtl* ctl)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_bend, dim3(blocks(count)), dim3(kBlock), 0, s, pos, bidx, start, count, stiff, ctl);
+}
+
+void launch_contacts(hipStream_t s, float4* pos, const float4* prev, float4* vel, ContactBuf cb, float thickness,
+                     float mu, float invH, SolverCtl* ctl)
+{
+    if (cb.nv <= 0) return;
+    hipLaunchKernelGGL(k_contacts, dim3(blocks(cb.nv)), dim3(kBlock), 0, s, pos, prev, vel, cb, thickness, mu, invH,
+                       ctl);
+}
+
+void launch_iter_end(hipStream_t s, SolverCtl* ctl, int maxIter, int fixedIter, int bendFreq, int dihedral,
+                     float selfThreshold)
+{
+    hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(1), 0, s, ctl, maxIter, fixedIter, bendFreq, dihedral,
+                       selfThreshold);
+}
+
+void launch_velocity(hipStream_t s, float4* pos, const float4* prev, float4* vel, int n, float invH)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_velocity, dim3(blocks(n)), dim3(kBlock), 0, s, pos, prev, vel, n, invH);
+}
+
+}  // namespace sc

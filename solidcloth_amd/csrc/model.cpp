@@ -1,0 +1,758 @@
+// Host implementation: cloth construction, constraint colouring, device
+// buffers, substep recording and hipGraph replay.
+#include "model.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+namespace sc {
+
+#define HIP_OK(expr)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            throw std::runtime_error(std::string(#expr) + " failed: " + hipGetErrorString(e_));      \
+    } while (0)
+
+template <class T>
+static void dev_alloc(T*& p, size_t n)
+{
+    if (p) HIP_OK(hipFree(p));
+    p = nullptr;
+    if (n) HIP_OK(hipMalloc((void**)&p, n * sizeof(T)));
+}
+
+template <class T>
+static void dev_free(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// ---------------------------------------------------------------- SDF ----
+
+SDF::~SDF() { dev_free(d_nodes); }
+
+// Geometry/SDF.cpp:10-40
+void SDF::create(int nx, int ny, int nz, const scAabb3& b)
+{
+    nc[0] = nx; nc[1] = ny; nc[2] = nz;
+    nv[0] = nx + 1; nv[1] = ny + 1; nv[2] = nz + 1;
+    box = b;
+    vals.assign((size_t)nv[0] * nv[1] * nv[2], 0.f);
+    f3 ext = sub(f3{b.max.x, b.max.y, b.max.z}, f3{b.min.x, b.min.y, b.min.z});
+    f3 inv{1.f / (float)nx, 1.f / (float)ny, 1.f / (float)nz};
+    steps = f3{ext.x * inv.x, ext.y * inv.y, ext.z * inv.z};
+}
+
+// Geometry/SDF.cpp:210-256: forward differences, the last layer of each axis
+// copies its neighbour.
+void SDF::compute_gradient()
+{
+    grads.assign(vals.size(), f3{0.f, 0.f, 0.f});
+    auto oob = [&](int x, int y, int z) { return x > nc[0] || y > nc[1] || z > nc[2] || x < 0 || y < 0 || z < 0; };
+    auto idx = [&](int x, int y, int z) { return (size_t)x + (size_t)y * nv[0] + (size_t)z * nv[0] * nv[1]; };
+    auto val = [&](int x, int y, int z) { return oob(x, y, z) ? 0.f : vals[idx(x, y, z)]; };
+    auto grd = [&](int x, int y, int z) { return oob(x, y, z) ? f3{0.f, 0.f, 0.f} : grads[idx(x, y, z)]; };
+    auto set = [&](int x, int y, int z, f3 g) {
+        if (!oob(x, y, z)) grads[idx(x, y, z)] = g;
+    };
+    for (int x = 0; x < nc[0]; x++) {
+        for (int y = 0; y < nc[1]; y++) {
+            for (int z = 0; z < nc[2]; z++) {
+                float v = val(x, y, z);
+                float dfdx = (val(x + 1, y, z) - v) / steps.x;
+                float dfdy = (val(x, y + 1, z) - v) / steps.y;
+                float dfdz = (val(x, y, z + 1) - v) / steps.z;
+                set(x, y, z, f3{dfdx, dfdy, dfdz});
+            }
+            set(x, y, nc[2], grd(x, y, nc[2] - 1));
+        }
+        for (int z = 0; z <= nc[2]; z++) set(x, nc[1], z, grd(x, nc[1] - 1, z));
+    }
+    for (int y = 0; y <= nc[1]; y++)
+        for (int z = 0; z <= nc[2]; z++) set(nc[0], y, z, grd(nc[0] - 1, y, z));
+}
+
+void SDF::upload()
+{
+    std::vector<float4> nodes(vals.size());
+    for (size_t i = 0; i < vals.size(); i++) nodes[i] = make_float4(grads[i].x, grads[i].y, grads[i].z, vals[i]);
+    dev_alloc(d_nodes, nodes.size());
+    HIP_OK(hipMemcpy(d_nodes, nodes.data(), nodes.size() * sizeof(float4), hipMemcpyHostToDevice));
+}
+
+// --------------------------------------------------------------- Mesh ----
+
+void Mesh::clear()
+{
+    vertices.clear(); indices.clear(); normals.clear(); uvs.clear(); edges.clear();
+}
+
+// Geometry/Mesh.cpp:65-147. The duplicate list is sorted with std::sort by
+// (i1, i2) from index order, exactly as the reference does, so which of an
+// interior edge's triangles becomes t1 is the same as in the reference
+// compiled with the same standard library.
+void Mesh::construct_edges()
+{
+    edges.clear();
+    const int ne = (int)indices.size();
+    if (ne == 0) return;
+    std::vector<MeshEdge> dup(ne);
+    for (int i = 0; i < ne; i++) {
+        uint32_t a = indices[i];
+        uint32_t b = ((i + 1) % 3 == 0) ? indices[i - 2] : indices[i + 1];
+        if (a > b) std::swap(a, b);
+        dup[i].i1 = a; dup[i].i2 = b; dup[i].t1 = i / 3; dup[i].t2 = -1;
+    }
+    std::sort(dup.begin(), dup.end(), [](const MeshEdge& e1, const MeshEdge& e2) {
+        return e1.i1 < e2.i1 || (e1.i1 == e2.i1 && e1.i2 < e2.i2);
+    });
+    MeshEdge cur = dup[0];
+    for (size_t i = 1; i < dup.size(); i++) {
+        if (dup[i].i1 == dup[i - 1].i1 && dup[i].i2 == dup[i - 1].i2) {
+            cur.t2 = dup[i].t1;
+            continue;
+        }
+        edges.push_back(cur);
+        cur = dup[i];
+    }
+    edges.push_back(cur);
+}
+
+// Geometry/Mesh.cpp:10-63 (angle-weighted vertex normals), for rendering.
+void Mesh::compute_normals()
+{
+    normals.assign(vertices.size(), scVector3{0.f, 0.f, 0.f});
+    std::vector<f3> acc(vertices.size(), f3{0.f, 0.f, 0.f});
+    auto V = [&](unsigned i) { return f3{vertices[i].x, vertices[i].y, vertices[i].z}; };
+    for (size_t i = 0; i + 2 < indices.size(); i += 3) {
+        unsigned i1 = indices[i], i2 = indices[i + 1], i3 = indices[i + 2];
+        f3 v1 = V(i1), v2 = V(i2), v3 = V(i3);
+        f3 nt = normalized(cross(sub(v2, v1), sub(v3, v1)));
+        f3 e1 = normalized(sub(v2, v1)), e2 = normalized(sub(v3, v2)), e3 = normalized(sub(v1, v3));
+        float c1 = -dot(e1, e3), c2 = -dot(e1, e2), c3 = -dot(e2, e3);
+        float t1 = std::fabs(c1) > 1 ? std::fabs(std::atan2(len(cross(e1, neg(e3))), c1)) : std::acos(c1);
+        float t2 = std::fabs(c2) > 1 ? std::fabs(std::atan2(len(cross(e2, neg(e1))), c2)) : std::acos(c2);
+        float t3 = std::fabs(c3) > 1 ? std::fabs(std::atan2(len(cross(e3, neg(e1))), c3)) : std::acos(c3);
+        acc[i1] = add(acc[i1], mul(t1, nt));
+        acc[i2] = add(acc[i2], mul(t2, nt));
+        acc[i3] = add(acc[i3], mul(t3, nt));
+    }
+    for (size_t i = 0; i < vertices.size(); i++) {
+        if (len(acc[i]) == 0.f) continue;
+        f3 n = normalized(acc[i]);
+        normals[i] = scVector3{n.x, n.y, n.z};
+    }
+}
+
+// ----------------------------------------------------------- colouring ----
+
+// Greedy first-fit colouring in constraint index order: each constraint takes
+// the smallest colour not yet used by any constraint sharing one of its
+// vertices. Constraints of one colour are vertex-disjoint, so processing them
+// in parallel equals processing them one after another; the Gauss-Seidel
+// order is colour-major with ascending index inside a colour.
+Colouring colour_greedy(int numConstraints, int arity, const uint32_t* verts, int strideWords, int numVertices)
+{
+    const int kWords = 4;  // up to 256 colours
+    std::vector<uint64_t> used((size_t)numVertices * kWords, 0);
+    std::vector<int> col(numConstraints);
+    int numColours = 0;
+    for (int c = 0; c < numConstraints; c++) {
+        uint64_t forb[kWords] = {0, 0, 0, 0};
+        const uint32_t* v = verts + (size_t)c * strideWords;
+        for (int a = 0; a < arity; a++)
+            for (int w = 0; w < kWords; w++) forb[w] |= used[(size_t)v[a] * kWords + w];
+        int colour = -1;
+        for (int w = 0; w < kWords && colour < 0; w++)
+            if (~forb[w]) colour = w * 64 + __builtin_ctzll(~forb[w]);
+        if (colour < 0) throw std::runtime_error("constraint colouring needs more than 256 colours");
+        col[c] = colour;
+        for (int a = 0; a < arity; a++) used[(size_t)v[a] * kWords + colour / 64] |= 1ull << (colour % 64);
+        numColours = std::max(numColours, colour + 1);
+    }
+    Colouring out;
+    out.offsets.assign(numColours + 1, 0);
+    for (int c = 0; c < numConstraints; c++) out.offsets[col[c] + 1]++;
+    for (int k = 0; k < numColours; k++) out.offsets[k + 1] += out.offsets[k];
+    out.order.resize(numConstraints);
+    std::vector<int> fill(out.offsets.begin(), out.offsets.end() - 1);
+    for (int c = 0; c < numConstraints; c++) out.order[fill[col[c]]++] = c;
+    return out;
+}
+
+// ------------------------------------------------------------- timing ----
+
+hipEvent_t KernelTimer::get()
+{
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIP_OK(hipEventCreate(&e));
+    return e;
+}
+
+void KernelTimer::harvest()
+{
+    for (auto& p : pending) {
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+        launches[p.first]++;
+        totalMs[p.first] += ms;
+        pool.push_back(p.second.first);
+        pool.push_back(p.second.second);
+    }
+    pending.clear();
+}
+
+void KernelTimer::reset()
+{
+    for (int i = 0; i < 5; i++) { launches[i] = 0; totalMs[i] = 0; }
+}
+
+KernelTimer::~KernelTimer()
+{
+    for (auto e : pool) (void)hipEventDestroy(e);
+    for (auto& p : pending) { (void)hipEventDestroy(p.second.first); (void)hipEventDestroy(p.second.second); }
+}
+
+// ------------------------------------------------------------- solver ----
+
+GpuSolver::~GpuSolver() { release(); }
+
+void GpuSolver::release()
+{
+    if (graphExec) (void)hipGraphExecDestroy(graphExec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graphExec = nullptr; graph = nullptr; graphKey.clear();
+    dev_free(d_pos); dev_free(d_prev); dev_free(d_vel);
+    dev_free(d_tidx); dev_free(d_tA); dev_free(d_tB); dev_free(d_tC); dev_free(d_bidx);
+    dev_free(d_cCount); dev_free(d_cP); dev_free(d_cN); dev_free(d_cSrc); dev_free(d_ctl);
+    slotCap = 0;
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+}
+
+// -------------------------------------------------------------- model ----
+
+void ClothModel::clear()
+{
+    pos.clear(); prev.clear(); vel.clear(); invMass.clear(); uv.clear();
+    tris.clear(); quads.clear(); links.clear(); edges.clear(); bends.clear();
+    contactsHost.clear(); contactsValid = false;
+    mark_topology_dirty();
+}
+
+// Physics/ClothModel.cpp:54-92 (rest state from material coordinates)
+void ClothModel::add_triangle(uint32_t i1, uint32_t i2, uint32_t i3)
+{
+    Triangle t{};
+    t.i1 = i1; t.i2 = i2; t.i3 = i3;
+    float d1x = uv[i2].x - uv[i1].x, d1y = uv[i2].y - uv[i1].y;
+    float d2x = uv[i3].x - uv[i1].x, d2y = uv[i3].y - uv[i1].y;
+    t.du1 = d1x; t.du2 = d2x; t.dv1 = d1y; t.dv2 = d2y;
+    t.det = t.du1 * t.dv2 - t.du2 * t.dv1;
+    t.invDet = (t.det != 0.f) ? 1.f / t.det : 0.f;
+    f3 dx1 = sub(pos[i2], pos[i1]);
+    f3 dx2 = sub(pos[i3], pos[i1]);
+    f3 wu = mul(t.invDet, sub(mul(t.dv2, dx1), mul(t.dv1, dx2)));
+    f3 wv = mul(t.invDet, add(mul(-t.du2, dx1), mul(t.du1, dx2)));
+    t.lu0 = len(wu);
+    t.lv0 = len(wv);
+    t.su = t.lu0 != 0.f ? 1.f / t.lu0 : 0.f;
+    t.sv = t.lv0 != 0.f ? 1.f / t.lv0 : 0.f;
+    t.dotuv = dot(wu, wv);
+    t.area = 0.5f * t.det;
+    t.lu2 = t.lu0 * t.lu0;
+    t.lv2 = t.lv0 * t.lv0;
+    tris.push_back(t);
+}
+
+// Physics/ClothModel.inl:12-31
+void ClothModel::add_link(uint32_t i1, uint32_t i2, float stiffness)
+{
+    if (invMass[i1] == 0.f && invMass[i2] == 0.f) return;
+    if (stiffness == 0.f) return;
+    links.push_back(Link{i1, i2, len(sub(pos[i1], pos[i2])), stiffness});
+}
+
+// Physics/ClothModel.cpp:94-126 (dihedral branch)
+void ClothModel::add_bend(const MeshEdge& e)
+{
+    if (e.t2 < 0) return;
+    auto opp = [](const Triangle& t, uint32_t a, uint32_t b) {
+        if (t.i1 != a && t.i1 != b) return t.i1;
+        if (t.i2 != a && t.i2 != b) return t.i2;
+        return t.i3;
+    };
+    bends.push_back(Bend{e.i1, e.i2, opp(tris[e.t1], e.i1, e.i2), opp(tris[e.t2], e.i1, e.i2)});
+}
+
+void ClothModel::sync_to_host()
+{
+    if (!mDeviceNewer) return;
+    const int n = (int)pos.size();
+    std::vector<float4> b(n);
+    HIP_OK(hipMemcpy(b.data(), gpu.d_pos, n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) { pos[i] = f3{b[i].x, b[i].y, b[i].z}; invMass[i] = b[i].w; }
+    HIP_OK(hipMemcpy(b.data(), gpu.d_prev, n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) prev[i] = f3{b[i].x, b[i].y, b[i].z};
+    HIP_OK(hipMemcpy(b.data(), gpu.d_vel, n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) vel[i] = f3{b[i].x, b[i].y, b[i].z};
+    mDeviceNewer = false;
+}
+
+void ClothModel::build_device_topology()
+{
+    GpuSolver& g = gpu;
+    if (!g.stream) HIP_OK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    if (g.graphExec) { (void)hipGraphExecDestroy(g.graphExec); g.graphExec = nullptr; }
+    if (g.graph) { (void)hipGraphDestroy(g.graph); g.graph = nullptr; }
+    g.graphKey.clear();
+    g.nv = (int)pos.size();
+    g.nt = (int)tris.size();
+    g.nb = (int)bends.size();
+    dev_alloc(g.d_pos, g.nv);
+    dev_alloc(g.d_prev, g.nv);
+    dev_alloc(g.d_vel, g.nv);
+    if (!g.d_ctl) dev_alloc(g.d_ctl, 1);
+    HIP_OK(hipMemset(g.d_ctl, 0, sizeof(SolverCtl)));
+
+    ensure_colouring();
+    // triangles in colour order
+    std::vector<int4> ti(g.nt);
+    std::vector<float4> ta(g.nt), tb(g.nt);
+    std::vector<float2> tc(g.nt);
+    for (int k = 0; k < g.nt; k++) {
+        const Triangle& t = tris[triCol.order[k]];
+        ti[k] = make_int4((int)t.i1, (int)t.i2, (int)t.i3, triCol.order[k]);
+        ta[k] = make_float4(t.du1, t.du2, t.dv1, t.dv2);
+        tb[k] = make_float4(t.invDet, t.lu2, t.lv2, t.area);
+        tc[k] = make_float2(t.lu0, t.lv0);
+    }
+    dev_alloc(g.d_tidx, g.nt); dev_alloc(g.d_tA, g.nt); dev_alloc(g.d_tB, g.nt); dev_alloc(g.d_tC, g.nt);
+    if (g.nt) {
+        HIP_OK(hipMemcpy(g.d_tidx, ti.data(), g.nt * sizeof(int4), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(g.d_tA, ta.data(), g.nt * sizeof(float4), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(g.d_tB, tb.data(), g.nt * sizeof(float4), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(g.d_tC, tc.data(), g.nt * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    // bends in colour order
+    std::vector<int4> bi(g.nb);
+    for (int k = 0; k < g.nb; k++) {
+        const Bend& b = bends[bendCol.order[k]];
+        bi[k] = make_int4((int)b.i1, (int)b.i2, (int)b.i3, (int)b.i4);
+    }
+    dev_alloc(g.d_bidx, g.nb);
+    if (g.nb) HIP_OK(hipMemcpy(g.d_bidx, bi.data(), g.nb * sizeof(int4), hipMemcpyHostToDevice));
+    dev_alloc(g.d_cCount, g.nv);
+    if (g.nv) HIP_OK(hipMemset(g.d_cCount, 0, g.nv * sizeof(int)));
+    g.slotCap = 0;
+    dev_free(g.d_cP);
+    dev_free(g.d_cN);
+    dev_free(g.d_cSrc);
+    mTopoDirty = false;
+}
+
+void ClothModel::ensure_colouring()
+{
+    if (!mColourDirty) return;
+    const int nv = (int)pos.size(), nt = (int)tris.size(), nb = (int)bends.size();
+    std::vector<uint32_t> tv(3 * (size_t)nt);
+    for (int t = 0; t < nt; t++) { tv[3 * t] = tris[t].i1; tv[3 * t + 1] = tris[t].i2; tv[3 * t + 2] = tris[t].i3; }
+    triCol = colour_greedy(nt, 3, tv.data(), 3, nv);
+    std::vector<uint32_t> bv(4 * (size_t)nb);
+    for (int b = 0; b < nb; b++) {
+        bv[4 * b] = bends[b].i1; bv[4 * b + 1] = bends[b].i2; bv[4 * b + 2] = bends[b].i3; bv[4 * b + 3] = bends[b].i4;
+    }
+    bendCol = colour_greedy(nb, 4, bv.data(), 4, nv);
+    mColourDirty = false;
+}
+
+void ClothModel::sync_to_device()
+{
+    if (mTopoDirty) build_device_topology();
+    if (!mHostDirty) return;
+    const int n = (int)pos.size();
+    std::vector<float4> b(n);
+    for (int i = 0; i < n; i++) b[i] = make_float4(pos[i].x, pos[i].y, pos[i].z, invMass[i]);
+    if (n) HIP_OK(hipMemcpy(gpu.d_pos, b.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    for (int i = 0; i < n; i++) b[i] = make_float4(prev[i].x, prev[i].y, prev[i].z, 0.f);
+    if (n) HIP_OK(hipMemcpy(gpu.d_prev, b.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    for (int i = 0; i < n; i++) b[i] = make_float4(vel[i].x, vel[i].y, vel[i].z, 0.f);
+    if (n) HIP_OK(hipMemcpy(gpu.d_vel, b.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    mHostDirty = false;
+}
+
+void ClothModel::ensure_contact_capacity(int slots)
+{
+    GpuSolver& g = gpu;
+    g.slots = slots;
+    if (slots <= g.slotCap) return;
+    dev_alloc(g.d_cP, (size_t)slots * g.nv);
+    dev_alloc(g.d_cN, (size_t)slots * g.nv);
+    dev_alloc(g.d_cSrc, (size_t)slots * g.nv);
+    g.slotCap = slots;
+    if (g.graphExec) { (void)hipGraphExecDestroy(g.graphExec); g.graphExec = nullptr; }
+    if (g.graph) { (void)hipGraphDestroy(g.graph); g.graph = nullptr; }
+    g.graphKey.clear();
+}
+
+// Device descriptors of the world's collidables, in world order.
+CollSet ClothModel::make_collset(int& slots) const
+{
+    CollSet cs;
+    memset(&cs, 0, sizeof(cs));
+    slots = 0;
+    const CollisionWorld* w = mOwner->mWorld;
+    for (const auto& sp : w->colls) {
+        const Collidable* c = sp.get();
+        CollDev d;
+        memset(&d, 0, sizeof(d));
+        d.type = c->type;
+        if (c->type == CT_WALLS) {
+            const Walls* wl = (const Walls*)c;
+            d.ax = wl->box.min.x; d.ay = wl->box.min.y; d.az = wl->box.min.z;
+            d.bx = wl->box.max.x; d.by = wl->box.max.y; d.bz = wl->box.max.z;
+            if (mCollFlags & CF_WALLS) slots += 5;
+        } else if (c->type == CT_SPHERE) {
+            const Sphere* s = (const Sphere*)c;
+            d.ax = s->center.x; d.ay = s->center.y; d.az = s->center.z; d.r = s->radius;
+            if (mCollFlags & CF_VERTICES) slots += 1;
+        } else if (c->type == CT_SDF) {
+            const SDF* s = ((const CollisionSDF*)c)->sdf;
+            d.ax = s->box.min.x; d.ay = s->box.min.y; d.az = s->box.min.z;
+            d.bx = s->steps.x; d.by = s->steps.y; d.bz = s->steps.z;
+            d.nodes = s->d_nodes;
+            d.nc0 = s->nc[0]; d.nc1 = s->nc[1]; d.nc2 = s->nc[2];
+            d.nv0 = s->nv[0]; d.nv1 = s->nv[1];
+            if (mCollFlags & CF_VERTICES) slots += 1;
+        } else {
+            throw std::runtime_error("collidable type " + std::to_string(c->type) +
+                                     " is not supported by the GPU substep");
+        }
+        if (cs.n >= MAX_COLLIDABLES) throw std::runtime_error("too many collidables (max 8)");
+        cs.c[cs.n++] = d;
+    }
+    if (mCollFlags & (CF_EDGES | CF_TRIANGLES | CF_SELF))
+        throw std::runtime_error("edge, triangle and self collision flags are not supported by the GPU substep");
+    cs.maxSlots = slots;
+    return cs;
+}
+
+std::string ClothModel::graph_key(float h, const CollSet& cs) const
+{
+    std::string k;
+    auto put = [&](const void* p, size_t n) { k.append((const char*)p, n); };
+    put(&h, 4); put(&mUnit, 4); put(&mThickness, 4); put(&mTolerance, 4); put(&mShearStiff, 4);
+    put(&mBendStiff, 4); put(&mPoisson, 4); put(&mFriction, 4); put(&mDragCoeff, 4);
+    put(&mNumIterations, 4); put(&mCollFlags, 4); put(&mBendFreq, 4);
+    put(&mDihedral, 1); put(&mFEM, 1); put(&mFixedIterations, 1);
+    put(&cs, sizeof(cs));
+    put(&gpu.d_cP, sizeof(void*));
+    return k;
+}
+
+static inline void timed_begin(KernelTimer& t, bool timed, int which, hipStream_t s, hipEvent_t& a)
+{
+    if (!timed) return;
+    a = t.get();
+    HIP_OK(hipEventRecord(a, s));
+    (void)which;
+}
+
+static inline void timed_end(KernelTimer& t, bool timed, int which, hipStream_t s, hipEvent_t a)
+{
+    if (!timed) return;
+    hipEvent_t b = t.get();
+    HIP_OK(hipEventRecord(b, s));
+    t.pending.push_back({which, {a, b}});
+}
+
+// One substep (Physics/ClothPBD.cpp:255-322) as a fixed kernel sequence. In
+// early-exit mode every iteration is recorded and the kernels of iterations
+// after the exit return at once (SolverCtl::done).
+void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
+{
+    GpuSolver& g = gpu;
+    hipStream_t s = g.stream;
+    hipEvent_t ev = nullptr;
+    // h * gravity * unit, evaluated in the reference's order (ClothPBD.cpp:263)
+    const f3 gravity{0.f, -9.8f, 0.f};
+    const f3 hg = mul(h, gravity);
+    const f3 gdv = f3{mUnit * hg.x, mUnit * hg.y, mUnit * hg.z};
+    timed_begin(g.timer, timed, 4, s, ev);
+    launch_predict(s, g.d_pos, g.d_prev, g.d_vel, g.nv, h, gdv, mDragCoeff, g.d_ctl);
+    timed_end(g.timer, timed, 4, s, ev);
+
+    ContactBuf cb{g.d_cCount, g.d_cP, g.d_cN, g.d_cSrc, g.nv, g.slots};
+    const bool anyContacts = cs.maxSlots > 0;
+    if (anyContacts) {
+        timed_begin(g.timer, timed, 3, s, ev);
+        launch_detect(s, g.d_pos, g.d_prev, g.nv, cs, cb, mThickness, mTolerance, mCollFlags);
+        timed_end(g.timer, timed, 3, s, ev);
+    }
+
+    const float h2 = h * h;
+    const float epsUV = 2 * (1 + mPoisson) / (smax(mShearStiff, 0.1f) * mUnit * mUnit);
+    const float sStiff = 0.0005f * mBendStiff;
+    const float invH = 1.f / h;
+    TriStreams tr{g.d_tidx, g.d_tA, g.d_tB, g.d_tC};
+    for (int k = 0; k < mNumIterations; k++) {
+        if (mFEM) {
+            for (size_t c = 0; c + 1 < triCol.offsets.size(); c++) {
+                int st = triCol.offsets[c], cnt = triCol.offsets[c + 1] - st;
+                timed_begin(g.timer, timed, 0, s, ev);
+                launch_fem(s, g.d_pos, tr, st, cnt, h2, epsUV, g.d_ctl);
+                timed_end(g.timer, timed, 0, s, ev);
+            }
+        }
+        if (mDihedral && k % mBendFreq == 0) {
+            for (size_t c = 0; c + 1 < bendCol.offsets.size(); c++) {
+                int st = bendCol.offsets[c], cnt = bendCol.offsets[c + 1] - st;
+                timed_begin(g.timer, timed, 1, s, ev);
+                launch_bend(s, g.d_pos, g.d_bidx, st, cnt, sStiff, g.d_ctl);
+                timed_end(g.timer, timed, 1, s, ev);
+            }
+        }
+        if (anyContacts) {
+            timed_begin(g.timer, timed, 2, s, ev);
+            launch_contacts(s, g.d_pos, g.d_prev, g.d_vel, cb, mThickness, mFriction, invH, g.d_ctl);
+            timed_end(g.timer, timed, 2, s, ev);
+        }
+        if (!mFixedIterations)
+            launch_iter_end(s, g.d_ctl, mNumIterations, 0, mBendFreq, mDihedral ? 1 : 0, mSelfCollThreshold);
+    }
+    timed_begin(g.timer, timed, 4, s, ev);
+    launch_velocity(s, g.d_pos, g.d_prev, g.d_vel, g.nv, invH);
+    timed_end(g.timer, timed, 4, s, ev);
+}
+
+void ClothModel::step_substeps(float h, int n)
+{
+    if (!mFEM) throw std::runtime_error("the link (non-FEM) stretch model is not implemented; call scSetUseFEM(true)");
+    if (mNumIterations < 1) throw std::runtime_error("NumIterations must be >= 1");
+    if (mBendFreq < 1) throw std::runtime_error("bend frequency must be >= 1");
+    if (msActive) throw std::runtime_error("the mouse spring is not implemented on the GPU substep");
+    sync_to_device();
+    if (pos.empty()) return;
+    for (const auto& c : mOwner->mWorld->colls)
+        if (c->type == CT_SDF && !((CollisionSDF*)c.get())->sdf->d_nodes) ((CollisionSDF*)c.get())->sdf->upload();
+    int slots = 0;
+    CollSet cs = make_collset(slots);
+    ensure_contact_capacity(std::max(slots, 1));
+    GpuSolver& g = gpu;
+    const bool timed = g.timer.enabled;
+    if (mUseGraph && !timed) {
+        std::string key = graph_key(h, cs);
+        if (!g.graphExec || key != g.graphKey) {
+            if (g.graphExec) { (void)hipGraphExecDestroy(g.graphExec); g.graphExec = nullptr; }
+            if (g.graph) { (void)hipGraphDestroy(g.graph); g.graph = nullptr; }
+            HIP_OK(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+            record_substep(h, cs, false);
+            HIP_OK(hipStreamEndCapture(g.stream, &g.graph));
+            HIP_OK(hipGraphInstantiate(&g.graphExec, g.graph, nullptr, nullptr, 0));
+            g.graphKey = key;
+        }
+        for (int i = 0; i < n; i++) HIP_OK(hipGraphLaunch(g.graphExec, g.stream));
+    } else {
+        for (int i = 0; i < n; i++) record_substep(h, cs, timed);
+    }
+    HIP_OK(hipStreamSynchronize(g.stream));
+    HIP_OK(hipGetLastError());
+    if (timed) g.timer.harvest();
+    mDeviceNewer = true;
+    mStatsPending = true;
+    contactsValid = false;
+}
+
+void ClothModel::fetch_stats()
+{
+    if (!mStatsPending || !gpu.d_ctl) return;
+    SolverCtl c;
+    HIP_OK(hipMemcpy(&c, gpu.d_ctl, sizeof(c), hipMemcpyDeviceToHost));
+    if (mFixedIterations) {
+        statIterations = mNumIterations;
+        statBendError = __builtin_bit_cast(float, c.cb);
+        statContactError = __builtin_bit_cast(float, c.cv);
+    } else {
+        statIterations = c.iterations;
+        statBendError = c.bendError;
+        statContactError = c.contactError;
+    }
+    mStatsPending = false;
+}
+
+// Contacts of the last substep in the reference's list order: collidable
+// order, then vertex, then the wall planes' order (ClothCollision.cpp:199-220).
+void ClothModel::download_contacts()
+{
+    if (contactsValid) return;
+    contactsHost.clear();
+    GpuSolver& g = gpu;
+    if (!g.d_cP || g.nv == 0 || g.slots == 0 || mTopoDirty) { contactsValid = true; return; }
+    std::vector<int> cnt(g.nv), src((size_t)g.slots * g.nv);
+    std::vector<float4> P((size_t)g.slots * g.nv), N((size_t)g.slots * g.nv);
+    HIP_OK(hipMemcpy(cnt.data(), g.d_cCount, g.nv * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(P.data(), g.d_cP, P.size() * sizeof(float4), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(N.data(), g.d_cN, N.size() * sizeof(float4), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(src.data(), g.d_cSrc, src.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int v = 0; v < g.nv; v++) {
+        for (int s = 0; s < cnt[v] && s < g.slots; s++) {
+            const size_t k = (size_t)s * g.nv + v;
+            ContactOut c;
+            c.vertex = v;
+            c.slot = s;
+            c.coll = src[k];
+            c.point = f3{P[k].x, P[k].y, P[k].z};
+            c.normal = f3{N[k].x, N[k].y, N[k].z};
+            contactsHost.push_back(c);
+        }
+    }
+    std::stable_sort(contactsHost.begin(), contactsHost.end(), [](const ContactOut& a, const ContactOut& b) {
+        if (a.coll != b.coll) return a.coll < b.coll;
+        if (a.vertex != b.vertex) return a.vertex < b.vertex;
+        return a.slot < b.slot;
+    });
+    contactsValid = true;
+}
+
+// -------------------------------------------------------------- patch ----
+
+// Physics/ClothPatch.cpp:135-246 (grid branch)
+void ClothPatch::init_grid(int divX, int divY, float inc, f3 offset, bool horizontal, bool attached)
+{
+    ClothModel& m = *mModel;
+    mMesh.clear();
+    m.clear();
+    const int n = divX * divY;
+    m.pos.assign(n, f3{0.f, 0.f, 0.f});
+    m.prev = m.pos;
+    m.vel = m.pos;
+    m.invMass.assign(n, 1.f);
+    m.uv.assign(n, scVector2{0.f, 0.f});
+    const float scale = 1.f;
+    float x = -0.5f * divX * inc;
+    const float startY = x;
+    const float incUVX = 1.f / divX;
+    const float incUVY = 1.f / divY;
+    const float stretch = m.mStretchStiff, shear = m.mShearStiff, bend = m.mBendStiff;
+    const float incline = mIncline;
+    mMesh.vertices.resize(n);
+    mMesh.uvs.resize(n);
+    for (int j = 0, base = 0; j < divX; j++, base += divY) {
+        f3 p0 = horizontal ? f3{x, 0.f, startY} : f3{x, startY, 0.f};
+        m.pos[base] = mul(scale, add(p0, offset));
+        m.prev[base] = m.pos[base];
+        m.vel[base] = f3{0.f, 0.f, 0.f};
+        m.invMass[base] = 1.f;
+        m.uv[base] = scVector2{x, 0.f};
+        mMesh.vertices[base] = scVector3{m.pos[base].x, m.pos[base].y, m.pos[base].z};
+        mMesh.uvs[base] = scVector2{j * incUVX, 0.f};
+        if (base >= divY) {
+            m.add_link(base - divY, base, stretch);
+            m.add_link(base - divY + 1, base, shear);
+            if (base >= 2 * divY && !m.mDihedral) m.add_link(base - 2 * divY, base, bend);
+        }
+        for (int i = 1; i < divY; i++) {
+            const int idx = base + i;
+            f3 pi = horizontal ? f3{x, 0.f, startY + inc * i} : f3{x, startY + inc * i, i * incline};
+            m.pos[idx] = mul(scale, add(pi, offset));
+            m.prev[idx] = m.pos[idx];
+            m.vel[idx] = f3{0.f, 0.f, 0.f};
+            m.invMass[idx] = 1.f;
+            m.uv[idx] = scVector2{x, inc * i};
+            mMesh.vertices[idx] = scVector3{m.pos[idx].x, m.pos[idx].y, m.pos[idx].z};
+            mMesh.uvs[idx] = scVector2{j * incUVX, i * incUVY};
+            m.add_link(idx - 1, idx, stretch);
+            if (idx > base + 1 && !m.mDihedral) m.add_link(idx - 2, idx, bend);
+            if (idx >= divY) {
+                m.add_link(idx - divY, idx, stretch);
+                if (i < divY - 1) m.add_link(idx - divY + 1, idx, shear);
+                if (j > 1 && !m.mDihedral) m.add_link(idx - 2 * divY, idx, bend);
+            }
+            if (idx > divY) {
+                m.add_link(idx - divY - 1, idx, shear);
+                m.quads.push_back(Quad{(uint32_t)(idx - divY - 1), (uint32_t)(idx - divY), (uint32_t)idx,
+                                       (uint32_t)(idx - 1)});
+                uint32_t t[6];
+                if ((i & 1) == (j & 1)) {
+                    t[0] = idx - divY - 1; t[1] = idx; t[2] = idx - divY;
+                    t[3] = idx - divY - 1; t[4] = idx - 1; t[5] = idx;
+                } else {
+                    t[0] = idx - divY; t[1] = idx - 1; t[2] = idx;
+                    t[3] = idx - 1; t[4] = idx - divY; t[5] = idx - divY - 1;
+                }
+                m.add_triangle(t[0], t[1], t[2]);
+                m.add_triangle(t[3], t[4], t[5]);
+                for (int q = 0; q < 6; q++) mMesh.indices.push_back(t[q]);
+            }
+        }
+        if (attached && (j == 0 || j == divX - 1)) m.invMass[base + divY - 1] = 0.f;
+        x += inc;
+    }
+    mMesh.compute_normals();
+    mMesh.construct_edges();
+    for (const MeshEdge& e : mMesh.edges) {
+        m.edges.push_back(e);
+        m.add_bend(e);
+    }
+    m.mark_topology_dirty();
+}
+
+// Physics/ClothPatch.cpp:323-335
+void ClothPatch::step(float dt)
+{
+    const int n = mNumSteps;
+    const float h = dt / n;
+    mModel->step_substeps(h, n);
+}
+
+// Physics/ClothPatch.cpp:266-273 with ClothModel::UpdateMesh (ClothModel.inl:123-130)
+void ClothPatch::update_mesh()
+{
+    ClothModel& m = *mModel;
+    m.sync_to_host();
+    for (size_t i = 0; i < mMesh.vertices.size() && i < m.pos.size(); i++) {
+        f3 p = add(m.pos[i], mPosition);
+        mMesh.vertices[i] = scVector3{p.x, p.y, p.z};
+    }
+    mMesh.compute_normals();
+}
+
+// Physics/ClothPatch.cpp:337-381
+void ClothPatch::compute_strain_map(scVector3* colors, int count)
+{
+    ClothModel& m = *mModel;
+    m.sync_to_host();
+    const int n = std::min(count, (int)m.pos.size());
+    for (int i = 0; i < n; i++) colors[i] = scVector3{0.f, 0.f, 0.f};
+    const float PI = 3.1415926535897932384626433832795f;
+    for (const Triangle& tri : m.tris) {
+        f3 dx1 = sub(m.pos[tri.i2], m.pos[tri.i1]);
+        f3 dx2 = sub(m.pos[tri.i3], m.pos[tri.i1]);
+        f3 wu = mul(tri.invDet, sub(mul(tri.dv2, dx1), mul(tri.dv1, dx2)));
+        f3 wv = mul(tri.invDet, add(mul(-tri.du2, dx1), mul(tri.du1, dx2)));
+        float euu = 0.5f * (dot(wu, wu) - 1), evv = 0.5f * (dot(wv, wv) - 1), euv = dot(wu, wv);
+        float s = sqrtf(euu * euu + evv * evv + euv * euv) * 10.f;
+        f3 e1 = normalized(dx1), e2 = normalized(sub(m.pos[tri.i3], m.pos[tri.i2])), e3 = normalized(neg(dx2));
+        float t1 = std::acos(-dot(e1, e3)), t2 = std::acos(-dot(e1, e2)), t3 = std::acos(-dot(e2, e3));
+        const uint32_t id[3] = {tri.i1, tri.i2, tri.i3};
+        const float tt[3] = {t1, t2, t3};
+        for (int k = 0; k < 3; k++) {
+            if ((int)id[k] >= n) continue;
+            float w = tt[k] / PI;
+            colors[id[k]].x += w * s;
+            colors[id[k]].z += w * s;
+        }
+    }
+}
+
+}  // namespace sc

@@ -1,0 +1,120 @@
+"""Writes the golden vectors under tests/golden/ from the reference's own
+Geometry code (Geometry/Mesh.cpp, Geometry/Collision3D.cpp), compiled from
+/root/reference by oracle/build_ref.sh into oracle/_ref/. Run in a container
+that has the reference:  python tests/golden/make_golden.py
+
+The fixtures are data only (inputs and the reference's outputs).
+"""
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+
+def ref_lib():
+    subprocess.check_call(["sh", os.path.join(ROOT, "oracle", "build_ref.sh")])
+    path = os.path.join(ROOT, "oracle", "_ref", "libref_geometry.so")
+    if not os.path.exists(path):
+        raise SystemExit("the reference is not available here; fixtures cannot be regenerated")
+    return C.CDLL(path)
+
+
+def p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def grid_mesh(n, inc):
+    """Vertices and triangle list of ClothPatch::Init's horizontal grid."""
+    import oracle
+
+    c = oracle.Cloth().init_grid(n, n, inc, (0, 0, 0), True, False)
+    return c.get("pos").astype(np.float32), c.triangles()[0].astype(np.uint32)
+
+
+def irregular_mesh(seed):
+    """A perturbed, randomly re-ordered triangulated sphere-like mesh with
+    consistently oriented faces (many interior edges, irregular valence)."""
+    rng = np.random.default_rng(seed)
+    # icosahedron subdivided once
+    t = (1 + 5 ** 0.5) / 2
+    V = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    F = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+         (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    V = [np.array(v, float) for v in V]
+    mid = {}
+
+    def m(a, b):
+        k = (min(a, b), max(a, b))
+        if k not in mid:
+            V.append((V[a] + V[b]) / 2)
+            mid[k] = len(V) - 1
+        return mid[k]
+
+    F2 = []
+    for a, b, c in F:
+        ab, bc, ca = m(a, b), m(b, c), m(c, a)
+        F2 += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+    verts = np.array(V) + rng.normal(scale=0.05, size=(len(V), 3))
+    F2 = np.array(F2)[rng.permutation(len(F2))]
+    # random rotation of each triangle's corner order keeps the orientation
+    rot = rng.integers(0, 3, size=len(F2))
+    F2 = np.array([np.roll(f, r) for f, r in zip(F2, rot)])
+    return verts.astype(np.float32), F2.astype(np.uint32)
+
+
+def main():
+    L = ref_lib()
+    out = {}
+    meshes = {"grid32": grid_mesh(32, np.float32(1 / 31)), "grid16": grid_mesh(16, np.float32(0.1)),
+              "ico1": irregular_mesh(7)}
+    for name, (verts, tris) in meshes.items():
+        idx = tris.ravel().astype(np.uint32)
+        cap = idx.size
+        e = np.zeros((cap, 4), dtype=np.int32)
+        n = L.ref_construct_edges(p(verts), len(verts), p(idx), idx.size, p(e), cap)
+        nrm = np.zeros((len(verts), 3), dtype=np.float32)
+        L.ref_compute_normals(p(verts), len(verts), p(idx), idx.size, p(nrm))
+        out[f"{name}_vertices"] = verts
+        out[f"{name}_indices"] = tris
+        out[f"{name}_edges"] = e[:n]
+        out[f"{name}_normals"] = nrm
+    np.savez_compressed(os.path.join(HERE, "ref_meshes.npz"), **out)
+
+    # closest-point primitives (Geometry/Collision3D.cpp) on random and
+    # degenerate inputs
+    rng = np.random.default_rng(11)
+    K = 400
+    P = rng.normal(size=(K, 4, 3)).astype(np.float32)
+    P[:20, 3] = P[:20, 0]                       # point on a vertex
+    P[20:40, 3] = 0.5 * (P[20:40, 0] + P[20:40, 1])  # point on an edge
+    P[40:50, 2] = P[40:50, 1]                   # degenerate triangle / segment
+    cpt = np.zeros((K, 6), dtype=np.float32)
+    reg_t = np.zeros(K, dtype=np.int32)
+    seg = np.zeros((K, 8), dtype=np.float32)
+    reg_s = np.zeros(K, dtype=np.int32)
+    vol = np.zeros(K, dtype=np.float32)
+    L.ref_signed_volume.restype = C.c_float
+    for k in range(K):
+        a, b, c, q = (np.ascontiguousarray(P[k, i]) for i in range(4))
+        o = np.zeros(6, dtype=np.float32)
+        reg_t[k] = L.ref_closest_pt_point_triangle(p(q), p(a), p(b), p(c), p(o))
+        cpt[k] = o
+        o2 = np.zeros(8, dtype=np.float32)
+        reg_s[k] = L.ref_closest_pt_segm_segm(p(a), p(b), p(c), p(q), p(o2))
+        seg[k] = o2
+        vol[k] = L.ref_signed_volume(p(a), p(b), p(c), p(q))
+    np.savez_compressed(os.path.join(HERE, "ref_collision3d.npz"), points=P, tri_out=cpt, tri_region=reg_t,
+                        segm_out=seg, segm_region=reg_s, signed_volume=vol)
+    print("wrote", sorted(out.keys())[:3], "... and ref_collision3d.npz")
+
+
+if __name__ == "__main__":
+    main()

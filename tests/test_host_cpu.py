@@ -1,0 +1,169 @@
+"""Host-side checks of libsolidcloth.so that need no GPU: the C ABI exports
+every function include/solidcloth.h declares, cloth construction matches the
+oracle bit for bit, the Gauss-Seidel colouring is a valid vertex-disjoint
+partition, and misuse is reported instead of crashing. No compute call is made."""
+import ctypes
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from solidcloth_amd import scenes as S
+
+
+def header_functions(sc):
+    text = open(sc.HEADER_PATH).read()
+    return sorted(set(re.findall(r"SCEXP\s+[\w\s\*]+?\b(sc\w+)\s*\(", text)))
+
+
+def test_library_exports_every_header_function(sc):
+    declared = header_functions(sc)
+    assert len(declared) >= 59 + 20
+    out = subprocess.check_output(["nm", "-D", "--defined-only", sc.LIB_PATH]).decode()
+    exported = set(re.findall(r"\bT (sc\w+)$", out, re.M))
+    missing = [f for f in declared if f not in exported]
+    assert not missing, missing
+    L = sc.lib()
+    for f in declared:
+        assert getattr(L, f) is not None
+    # the Python mirror binds exactly the declared functions
+    assert sc.exported_names() == declared
+
+
+def test_reference_abi_names_present(sc):
+    """The 59 exports of the reference's ClothLib (Demo/ClothAPI.h:65-146)."""
+    ref = """scCreateCollisionWorld scFreeCollisionWorld scClearCollidables scGetNumCollidables scAddCollidable
+    scGetCollidable scCreateWalls scCreateSphere scCreateCapsule scCreateSDF scCreateMesh scGetCollidableType
+    scGetMeshTree scGetWallsInfo scGetSphereInfo scGetCapsuleInfo scCreateClothPatch scFreeClothPatch
+    scGetClothModel scGetClothMesh scInitCloth scStepCloth scUpdateClothMesh scComputeStrainMap
+    scGetNumParticles scGetParticlePos scGetParticleInvMass scGetNumTris scGetTriangle scGetTriangleInfo
+    scGetNumQuads scGetQuad scGetNumLinks scGetLink scGetNumContacts scGetContact scGetNumTriContacts
+    scGetTriContact scGetCollisionFlags scSetCollisionFlags scGetThickness scSetThickness scGetTolerance
+    scSetTolerance scGetFriction scSetFriction scGetNumIterations scSetNumIterations scGetUseFEM scSetUseFEM
+    scGetClothTree scAddMouseSpring scUpdateMouseSpring scRemoveMouseSpring scGetMeshVertexCount
+    scGetMeshIndexCount scGetMeshNormalBuffer scGetMeshVertexBuffer scGetMeshIndexBuffer""".split()
+    assert len(ref) == 59
+    assert set(ref) <= set(header_functions(sc))
+
+
+@pytest.mark.parametrize("scene", [S.DRAPE32, S.DRAPE256.with_(n=40)], ids=lambda s: s.name)
+def test_construction_matches_oracle(sc, oracle_mod, scene):
+    world, patch = S.build_patch(scene, sc)
+    m = patch.GetModel()
+    o = oracle_mod.from_scene(scene)
+    assert np.array_equal(m.positions().astype(np.float64), o.get("pos"))
+    assert np.array_equal(m.inv_masses().astype(np.float64), o.get("invMass"))
+    assert np.array_equal(m.triangles(), o.triangles()[0])
+    assert np.array_equal(m.bends(), o.bends())
+    # rest state (ClothModel::AddTriangle) bit for bit: invDet, du, dv
+    _, rest = o.triangles()
+    L = sc.lib()
+    inv = ctypes.c_float()
+    du, dv, s = sc.scVector2(), sc.scVector2(), sc.scVector2()
+    for k in range(0, len(rest), max(1, len(rest) // 97)):
+        L.scGetTriangleInfo(m.h, k, ctypes.byref(inv), ctypes.byref(du), ctypes.byref(dv), ctypes.byref(s))
+        assert inv.value == np.float32(rest[k, 5])
+        assert (du.x, du.y, dv.x, dv.y) == tuple(np.float32(rest[k, [0, 1, 2, 3]]))
+
+
+def test_reference_counts(sc):
+    """Counts of the reference's grid construction at 32x32 and 256x256."""
+    for scene, V, T, B, links in [(S.DRAPE32, 1024, 1922, 2821, 3906), (S.DRAPE256, 65536, 130050, 194565, None)]:
+        world, patch = S.build_patch(scene, sc)
+        m = patch.GetModel()
+        assert (m.GetNumParticles(), m.GetNumTris(), m.GetNumBends()) == (V, T, B)
+        if links:
+            assert m.GetNumLinks() == links
+        assert m.GetNumQuads() == (scene.n - 1) ** 2
+
+
+def validate_colouring(order, offsets, cons, nv):
+    """Every constraint exactly once; constraints of one colour vertex-disjoint;
+    ascending index inside a colour."""
+    assert sorted(order.tolist()) == list(range(len(cons)))
+    for c in range(len(offsets) - 1):
+        ids = order[offsets[c]:offsets[c + 1]]
+        assert np.all(np.diff(ids) > 0)
+        verts = cons[ids].ravel()
+        assert len(np.unique(verts)) == len(verts), f"colour {c} shares a vertex"
+
+
+@pytest.mark.parametrize("n", [32, 97, 256])
+def test_colouring_is_valid_partition(sc, n):
+    scene = S.DRAPE32.with_(n=n)
+    world, patch = S.build_patch(scene, sc)
+    m = patch.GetModel()
+    tris = m.triangles()
+    bends = m.bends()
+    to, toff = m.constraint_order("tri"), m.colour_offsets("tri")
+    bo, boff = m.constraint_order("bend"), m.colour_offsets("bend")
+    validate_colouring(to, toff, tris, m.GetNumParticles())
+    validate_colouring(bo, boff, bends, m.GetNumParticles())
+    # a grid vertex of valence 8 makes 8 triangle colours the minimum; the
+    # greedy colouring reaches it. Bends need at least 16.
+    assert len(toff) - 1 == 8
+    assert 16 <= len(boff) - 1 <= 20
+
+
+def test_colouring_deterministic(sc):
+    a = S.build_patch(S.DRAPE32, sc)[1].GetModel()
+    b = S.build_patch(S.DRAPE32, sc)[1].GetModel()
+    assert np.array_equal(a.constraint_order("bend"), b.constraint_order("bend"))
+
+
+def test_errors_are_reported_not_fatal(sc):
+    world = sc.CollisionWorld()
+    patch = sc.ClothPatch(world)
+    m = patch.GetModel()
+    with pytest.raises(sc.SolidClothError):
+        m.GetParticlePos(0)  # no cloth yet: index out of range
+    with pytest.raises(sc.SolidClothError):
+        patch.Init(1, 5, 0.1)
+    with pytest.raises(sc.SolidClothError):
+        patch.SetNumSteps(0)
+    with pytest.raises(sc.SolidClothError):
+        sc.SDF(np.zeros(4, np.float32), (0, 1, 1), (0, 0, 0), (1, 1, 1))  # zero cells
+    L = sc.lib()
+    n0 = L.scGetErrorCount()
+    assert L.scGetNumParticles(sc.scModel()) == 0  # null handle
+    assert L.scGetErrorCount() == n0 + 1
+    assert b"null" in L.scGetLastError()
+
+
+def test_collidable_introspection(sc):
+    L = sc.lib()
+    w = sc.CollisionWorld()
+    hw = w.AddWalls((-1, -2, -3), (4, 5, 6))
+    hs = w.AddSphere((0.5, 0.25, 0.0), 0.75)
+    assert w.GetNumCollidables() == 2
+    assert L.scGetCollidableType(hw) == sc.CT_WALLS and L.scGetCollidableType(hs) == sc.CT_SPHERE
+    box = sc.scAabb3()
+    L.scGetWallsInfo(hw, ctypes.byref(box))
+    assert (box.min.x, box.min.y, box.min.z, box.max.x, box.max.y, box.max.z) == (-1, -2, -3, 4, 5, 6)
+    r, c = ctypes.c_float(), sc.scVector3()
+    L.scGetSphereInfo(hs, ctypes.byref(r), ctypes.byref(c))
+    assert (r.value, c.x, c.y, c.z) == (0.75, 0.5, 0.25, 0.0)
+    # type-mismatched getter leaves its outputs untouched (Demo/ClothAPI.cpp:92-124)
+    r2 = ctypes.c_float(-1)
+    L.scGetSphereInfo(hw, ctypes.byref(r2), ctypes.byref(c))
+    assert r2.value == -1
+    assert L.scGetCollidable(w.h, 1) == hs
+
+
+def test_mesh_buffers_and_strain_map(sc):
+    world, patch = S.build_patch(S.DRAPE32, sc)
+    L = sc.lib()
+    mesh = L.scGetClothMesh(patch.h)
+    assert L.scGetMeshVertexCount(mesh) == 1024
+    assert L.scGetMeshIndexCount(mesh) == 3 * 1922
+    vb = L.scGetMeshVertexBuffer(mesh)
+    v = np.ctypeslib.as_array(ctypes.cast(vb, ctypes.POINTER(ctypes.c_float)), shape=(1024 * 3,))
+    assert np.array_equal(v.reshape(-1, 3), patch.GetModel().positions())
+    nb = L.scGetMeshNormalBuffer(mesh)
+    n = np.ctypeslib.as_array(ctypes.cast(nb, ctypes.POINTER(ctypes.c_float)), shape=(1024 * 3,)).reshape(-1, 3)
+    np.testing.assert_allclose(np.linalg.norm(n, axis=1), 1, atol=1e-6)
+    # rest state: zero strain everywhere
+    colors = (sc.scVector3 * 1024)()
+    L.scComputeStrainMap(patch.h, colors, 1024)
+    assert max(abs(c.x) for c in colors) < 1e-4

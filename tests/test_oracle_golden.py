@@ -1,0 +1,50 @@
+"""The oracle against golden vectors written by the reference's own Geometry
+code (tests/golden/make_golden.py compiles Geometry/Mesh.cpp and
+Geometry/Collision3D.cpp from the reference sources)."""
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MESHES = np.load(os.path.join(HERE, "golden", "ref_meshes.npz"))
+
+
+@pytest.mark.parametrize("name", ["grid32", "grid16", "ico1"])
+def test_construct_edges_matches_reference(oracle_mod, name):
+    """Mesh::ConstructEdges (Geometry/Mesh.cpp:65-147): unique sorted edges and
+    their winged triangles t1/t2, including which triangle std::sort leaves
+    first among duplicates."""
+    got = oracle_mod.construct_edges(MESHES[f"{name}_indices"])
+    exp = MESHES[f"{name}_edges"]
+    assert got.shape == exp.shape
+    assert np.array_equal(got, exp)
+
+
+def test_grid_topology_matches_reference_mesh(oracle_mod):
+    """ClothPatch::Init's grid (Physics/ClothPatch.cpp:135-246) at 32x32 yields
+    the reference's triangle list and the edges the reference builds from it."""
+    c = oracle_mod.Cloth().init_grid(32, 32, np.float32(1 / 31), (0, 0, 0), True, False)
+    tris, _ = c.triangles()
+    assert np.array_equal(tris.astype(np.uint32), MESHES["grid32_indices"])
+    assert np.array_equal(c.edges(), MESHES["grid32_edges"])
+    pos = c.get("pos").astype(np.float32)
+    assert np.array_equal(pos, MESHES["grid32_vertices"])
+
+
+def test_bends_from_reference_edges(oracle_mod):
+    """Each interior edge (t2 >= 0) of the reference's edge list gives one bend
+    (i1, i2, opposite vertex of t1, opposite vertex of t2),
+    Physics/ClothModel.cpp:94-126."""
+    c = oracle_mod.Cloth().init_grid(32, 32, np.float32(1 / 31), (0, 0, 0), True, False)
+    tris = MESHES["grid32_indices"]
+    edges = MESHES["grid32_edges"]
+    exp = []
+    for i1, i2, t1, t2 in edges:
+        if t2 < 0:
+            continue
+        o1 = [v for v in tris[t1] if v not in (i1, i2)][0]
+        o2 = [v for v in tris[t2] if v not in (i1, i2)][0]
+        exp.append((i1, i2, o1, o2))
+    assert np.array_equal(c.bends(), np.array(exp, dtype=np.int32))
+    assert len(exp) == 2821

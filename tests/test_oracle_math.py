@@ -1,0 +1,393 @@
+"""Mathematical pins of the oracle where no reference vector exists.
+
+Each test derives its expectation independently of the oracle's formulas
+(finite differences of the constraint functions, closed-form free fall,
+exactness of trilinear interpolation on linear fields, the geometry of a
+plane contact), so a dropped term, a wrong sign, a wrong index or a swapped
+operand in the restated solver fails one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+
+def one_triangle(oracle_mod, young=1e12, poisson=0.3, unit=1.0):
+    c = oracle_mod.Cloth(real64=False)
+    c.set(unit=unit, shear=young, poisson=poisson, fem=1)
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 0, 1]], float)
+    uv = np.array([[0, 0], [1, 0], [0, 1]], float)
+    c.init_arrays(pos, uv, np.ones(3), [[0, 1, 2]])
+    return c
+
+
+def deformation_gradient(x, uv):
+    """F (3x2) with [x2-x1, x3-x1] = F [uv2-uv1, uv3-uv1], by a linear solve."""
+    dX = np.stack([x[1] - x[0], x[2] - x[0]], axis=1)
+    dU = np.stack([uv[1] - uv[0], uv[2] - uv[0]], axis=1)
+    return dX @ np.linalg.inv(dU)
+
+
+def fd_grad(f, x, h=1e-6):
+    g = np.zeros_like(x)
+    for i in range(x.shape[0]):
+        for k in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[i, k] += h
+            xm[i, k] -= h
+            g[i, k] = (f(xp) - f(xm)) / (2 * h)
+    return g
+
+
+UV = np.array([[0, 0], [1, 0], [0, 1]], float)
+
+
+def c_warp(x):
+    F = deformation_gradient(x, UV)
+    return 0.5 * (F[:, 0] @ F[:, 0] - 1.0)
+
+
+def c_weft(x):
+    F = deformation_gradient(x, UV)
+    return 0.5 * (F[:, 1] @ F[:, 1] - 1.0)
+
+
+def c_shear(x):
+    F = deformation_gradient(x, UV)
+    return F[:, 0] @ F[:, 1]
+
+
+def newton_step(C, x):
+    g = fd_grad(C, x)
+    return -C(x) * g / np.sum(g * g)
+
+
+@pytest.mark.parametrize("eps", [1e-3, -2e-3, 5e-2])
+def test_fem_warp_is_a_linearised_constraint_projection(oracle_mod, eps):
+    """SolveTriangles' warp step (ClothPBD.cpp:73-82) moves the vertices by
+    -C grad C / |grad C|^2 for C = (|F e_u|^2 - |F0 e_u|^2) / 2."""
+    c = one_triangle(oracle_mod)
+    x = np.array([[0, 0, 0], [1 + eps, 0, 0], [0, 0, 1]], float)
+    c.put("pos", x)
+    c.solve_triangles(1 / 240)
+    dx = c.get("pos") - x
+    np.testing.assert_allclose(dx, newton_step(c_warp, x), atol=2e-6 * max(1, abs(eps) * 10))
+    assert abs(c_warp(c.get("pos"))) < abs(c_warp(x)) * max(0.05, 4 * abs(eps))
+
+
+@pytest.mark.parametrize("eps", [1e-3, -2e-3])
+def test_fem_weft_is_a_linearised_constraint_projection(oracle_mod, eps):
+    c = one_triangle(oracle_mod)
+    x = np.array([[0, 0, 0], [1, 0, 0], [0, 0, 1 + eps]], float)
+    c.put("pos", x)
+    c.solve_triangles(1 / 240)
+    dx = c.get("pos") - x
+    np.testing.assert_allclose(dx, newton_step(c_weft, x), atol=2e-6)
+
+
+def test_fem_shear_hard_limit(oracle_mod):
+    """With a huge shear modulus the compliant shear step (ClothPBD.cpp:95-106)
+    becomes the hard projection of C = F e_u . F e_v; the weft step sees the
+    second-order stretch of the sheared edge. Both use the gradients of the
+    initial configuration."""
+    g = 1e-3
+    c = one_triangle(oracle_mod, young=1e15)
+    x = np.array([[0, 0, 0], [1, 0, 0], [g, 0, 1]], float)
+    c.put("pos", x)
+    c.solve_triangles(1 / 240)
+    dx = c.get("pos") - x
+    exp = newton_step(c_weft, x) + newton_step(c_shear, x)
+    np.testing.assert_allclose(dx, exp, atol=2e-6)
+
+
+@pytest.mark.parametrize("young,poisson,unit,h", [(1e4, 0.3, 1.0, 1 / 240), (3e3, 0.1, 1.0, 1 / 120),
+                                                 (1e4, 0.45, 2.0, 1 / 240)])
+def test_fem_shear_compliance(oracle_mod, young, poisson, unit, h):
+    """The compliant shear step scales the hard step by s = 1/(1 + a) with
+    a = lu0 lv0 * 2(1+nu)/(E unit^2) / (diag h^2 area) (ClothPBD.cpp:56,100-103),
+    where diag = invDet * sum m|q|^2 = sum m|grad C|^2 / invDet. For the unit
+    right triangle (lu0 = lv0 = invDet = 1, area 1/2) a = 4(1+nu)/(E unit^2 h^2 W),
+    W = sum |grad C|^2 from finite differences."""
+    g = 1e-3
+    c = one_triangle(oracle_mod, young=young, poisson=poisson, unit=unit)
+    x = np.array([[0, 0, 0], [1, 0, 0], [g, 0, 1]], float)
+    c.put("pos", x)
+    c.solve_triangles(h)
+    dx = c.get("pos") - x - newton_step(c_weft, x)
+    W = np.sum(fd_grad(c_shear, x) ** 2)
+    a = 4 * (1 + poisson) / (young * unit * unit * h * h * W)
+    exp = newton_step(c_shear, x) / (1 + a)
+    np.testing.assert_allclose(dx, exp, rtol=2e-3, atol=2e-7)
+
+
+def test_fem_rest_state_is_fixed_point(oracle_mod):
+    c = oracle_mod.Cloth().set(unit=1, shear=1e4, poisson=0.3, fem=1)
+    c.init_grid(12, 12, np.float32(0.1), (0.3, 0.2, -0.1), True, False)
+    x0 = c.get("pos")
+    c.solve_triangles(1 / 240)
+    assert np.abs(c.get("pos") - x0).max() < 1e-6
+
+
+def test_fem_rigid_motion_invariance(oracle_mod):
+    """A rotated and translated rest configuration is (up to rounding) also at rest."""
+    c = oracle_mod.Cloth().set(unit=1, shear=1e4, poisson=0.3, fem=1)
+    c.init_grid(10, 10, np.float32(0.1), (0, 0, 0), True, False)
+    x = c.get("pos")
+    th = 0.7
+    R = np.array([[math.cos(th), -math.sin(th), 0], [math.sin(th), math.cos(th), 0], [0, 0, 1]])
+    y = x @ R.T + np.array([0.5, -1.0, 2.0])
+    c.put("pos", y)
+    c.solve_triangles(1 / 240)
+    assert np.abs(c.get("pos") - y).max() < 5e-6
+
+
+def theta_ref(x):
+    """Dihedral angle as EvaluateAngle defines it (ClothPBD.cpp:110-139):
+    acos of the dot of the unit normals n1 ~ x12 x x13, n2 ~ x14 x x12."""
+    x12, x13, x14 = x[1] - x[0], x[2] - x[0], x[3] - x[0]
+    n1 = np.cross(x12, x13)
+    n2 = np.cross(x14, x12)
+    n1 /= np.linalg.norm(n1)
+    n2 /= np.linalg.norm(n2)
+    return math.acos(np.clip(n1 @ n2, -1, 1))
+
+
+def two_triangles(oracle_mod, fold, bend=5.0):
+    c = oracle_mod.Cloth().set(unit=1, bend=bend, fem=1)
+    # edge (0,1) along x, wings in z; vertex 3 folded up by angle `fold`
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0.4, 0, 1], [0.6, 0, -1]], float)
+    uv = pos[:, [0, 2]]
+    c.init_arrays(pos, uv, np.ones(4), [[0, 1, 2], [1, 0, 3]])
+    b = c.bends()
+    assert len(b) == 1
+    x = pos.copy()
+    x[3] = [0.6, math.sin(fold), -math.cos(fold)]
+    c.put("pos", x)
+    return c, b[0], x
+
+
+@pytest.mark.parametrize("fold", [0.3, 0.9, -0.6, 2.0])
+def test_bend_step_is_gradient_projection(oracle_mod, fold):
+    """SolveBends (ClothPBD.cpp:179-224) moves vertex i by
+    -s theta grad_i theta / sum_j |grad_j theta|^2 with s = 0.0005 * stiffness,
+    grad theta from finite differences of the EvaluateAngle definition."""
+    c, b, x = two_triangles(oracle_mod, fold)
+    xb = x[list(b)]
+    theta = theta_ref(xb)
+    g = fd_grad(theta_ref, xb)
+    s = 0.0005 * 5.0
+    exp = -s * theta * g / np.sum(g * g)
+    c.solve_bends()
+    dx = (c.get("pos") - x)[list(b)]
+    np.testing.assert_allclose(dx, exp, rtol=1e-3, atol=1e-7)
+
+
+def test_bend_skips_flat_and_pinned(oracle_mod):
+    c, b, x = two_triangles(oracle_mod, 0.0)  # flat: |cos| >= 0.999 is skipped
+    x32 = x.astype(np.float32).astype(np.float64)
+    c.solve_bends()
+    assert np.array_equal(c.get("pos"), x32)
+    c, b, x = two_triangles(oracle_mod, 0.5)
+    c.put("invMass", np.zeros(4))
+    x32 = x.astype(np.float32).astype(np.float64)
+    c.solve_bends()
+    assert np.array_equal(c.get("pos"), x32)
+
+
+def test_free_fall_closed_form(oracle_mod):
+    """Prediction and velocity update (ClothPBD.cpp:258-266, 314-321) with no
+    constraint active integrate x_k = x_0 + h^2 g unit k(k+1)/2 (symplectic
+    Euler), g = (0, -9.8, 0); pinned vertices stay put."""
+    unit = 2.0
+    c = oracle_mod.Cloth(real64=True).set(unit=unit, fem=0, dihedral=0)
+    c.init_grid(4, 4, 0.1, (0, 1, 0), True, True)
+    x0 = c.get("pos")
+    m = c.get("invMass")
+    h = 2.0 ** -8  # exact in FP32, the type of the step argument
+    g = float(np.float32(-9.8))
+    K = 50
+    for _ in range(K):
+        c.substep(h)
+    exp = x0.copy()
+    exp[m != 0, 1] += g * unit * h * h * K * (K + 1) / 2
+    np.testing.assert_allclose(c.get("pos"), exp, rtol=0, atol=1e-12)
+    v = c.get("vel")
+    np.testing.assert_allclose(v[m != 0, 1], g * unit * h * K, rtol=1e-9)
+
+
+def test_drag_scales_velocity(oracle_mod):
+    c = oracle_mod.Cloth(real64=True).set(unit=1, fem=0, dihedral=0, drag=0.5)
+    c.init_grid(3, 3, 0.1, (0, 0, 0), True, False)
+    h = 2.0 ** -7
+    c.substep(h)
+    np.testing.assert_allclose(c.get("vel")[:, 1], 0.5 * (float(np.float32(-9.8)) * h), rtol=1e-12)
+
+
+def sdf_cloth(oracle_mod, values, n, bmin, bmax):
+    c = oracle_mod.Cloth(real64=False)
+    idx = c.add_grid_sdf(values, (n, n, n), bmin, bmax)
+    return c, 0
+
+
+def node_positions(n, bmin, bmax):
+    bmin = np.asarray(bmin, np.float32)
+    bmax = np.asarray(bmax, np.float32)
+    step = (bmax - bmin) * (np.float32(1) / np.float32(n))
+    i = np.arange(n + 1, dtype=np.float32)
+    Z, Y, X = np.meshgrid(i, i, i, indexing="ij")
+    return X * step[0] + bmin[0], Y * step[1] + bmin[1], Z * step[2] + bmin[2]
+
+
+def test_sdf_trilinear_reproduces_linear_fields(oracle_mod):
+    """GetValue/GetGrad (Geometry/SDF.h:105-155) interpolate trilinearly, so a
+    linear field is reproduced exactly up to rounding, and its forward-
+    difference gradient (SDF.cpp:210-256) is the constant coefficient vector."""
+    n, bmin, bmax = 8, (-1, -0.5, 0), (1, 1.5, 2)
+    X, Y, Z = node_positions(n, bmin, bmax)
+    a = np.array([0.3, -0.7, 0.2])
+    vals = (a[0] * X + a[1] * Y + a[2] * Z + 0.1).astype(np.float32).ravel()
+    c, s = sdf_cloth(oracle_mod, vals, n, bmin, bmax)
+    rng = np.random.default_rng(1)
+    for p in rng.uniform([-0.99, -0.49, 0.01], [0.99, 1.49, 1.99], size=(50, 3)):
+        out = c.sdf_query(s, p)
+        assert abs(out[0] - (a @ p + 0.1)) < 5e-6
+        np.testing.assert_allclose(out[1:4], a, atol=5e-5)
+        np.testing.assert_allclose(out[4:7], a / np.linalg.norm(a), atol=5e-5)
+        # closest point = p - value * unit gradient
+        np.testing.assert_allclose(out[7:10], p - out[0] * out[4:7], atol=1e-6)
+
+
+def test_sdf_outside_box(oracle_mod):
+    n = 4
+    vals = np.ones((n + 1) ** 3, np.float32)
+    c, s = sdf_cloth(oracle_mod, vals, n, (0, 0, 0), (1, 1, 1))
+    out = c.sdf_query(s, (-0.1, 0.5, 0.5))
+    assert out[0] == np.float32(np.finfo(np.float32).max)
+    assert np.all(out[1:4] == 0)
+    # the far face belongs to the grid: the cell index may equal the cell
+    # count, and the corners beyond it read as 0 (SDF.h:72-77, 144)
+    out = c.sdf_query(s, (1.0, 0.5, 0.5))
+    assert out[0] == pytest.approx(1.0, abs=1e-6)
+    out = c.sdf_query(s, (1.1, 0.5, 0.5))
+    assert out[0] == pytest.approx(0.6, abs=1e-6)
+    out = c.sdf_query(s, (1.26, 0.5, 0.5))
+    assert out[0] == np.float32(np.finfo(np.float32).max)
+
+
+def test_sampled_sphere_sdf_accuracy(oracle_mod):
+    """A sampled sphere distance interpolates to within the O(step^2 / r)
+    curvature error of the analytic distance."""
+    from solidcloth_amd import scenes as S
+
+    scene = S.DRAPE256
+    vals = S.sphere_sdf_values(scene)
+    bmin, bmax = S.sdf_box(scene)
+    c, s = sdf_cloth(oracle_mod, vals, scene.sdf_cells, bmin, bmax)
+    step = (bmax[0] - bmin[0]) / scene.sdf_cells
+    rng = np.random.default_rng(3)
+    for _ in range(100):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        r = rng.uniform(0.31, 0.39)
+        p = d * r
+        out = c.sdf_query(s, p)
+        assert abs(out[0] - (r - 0.3)) < step * step / 0.3
+        # normal points away from the centre
+        assert out[4:7] @ d > 0.99
+
+
+def test_ground_contact_projects_to_thickness(oracle_mod):
+    """A vertex below the ground plane is pushed along the normal to the
+    thickness offset (ClothCollisionHandler.cpp:162-173); with no tangential
+    motion friction does nothing."""
+    c = oracle_mod.Cloth(real64=True).set(unit=1, fem=1, thickness=0.01, tolerance=0.0, friction=0.3,
+                                          collFlags=8)
+    c.add_walls((-10, -1, -10), (10, 10, 10))
+    c.init_grid(2, 2, 0.1, (0, 0, 0), True, False)
+    x = c.get("pos")
+    x[:, 1] = -1.02
+    c.put("pos", x)
+    c.put("prev", x)
+    c.detect()
+    ids, data = c.contacts()
+    assert len(ids) == 4 and np.all(data[:, 4] == 1)
+    c.solve_contacts(0.01)
+    np.testing.assert_allclose(c.get("pos")[:, 1], -1 + 0.01, atol=1e-12)
+    np.testing.assert_allclose(c.get("pos")[:, [0, 2]], x[:, [0, 2]], atol=0)
+
+
+def test_friction_is_bounded_by_mu_lambda(oracle_mod):
+    """Tangential correction per contact is min(|v_t|, mu * lambda) along -v_t
+    (ClothCollisionHandler.cpp:175-194)."""
+    mu = 0.3
+    c = oracle_mod.Cloth(real64=True).set(unit=1, fem=1, thickness=0.0, tolerance=0.0, friction=mu, collFlags=8)
+    c.add_walls((-10, 0, -10), (10, 10, 10))
+    c.init_grid(2, 2, 0.1, (0, 0, 0), True, False)
+    prev = c.get("pos")
+    x = prev.copy()
+    x[:, 1] = -0.01          # penetration depth 0.01 -> lambda 0.01
+    x[:, 0] += 0.02          # sliding 0.02 > mu * lambda
+    c.put("prev", prev)
+    c.put("pos", x)
+    c.detect()
+    c.solve_contacts(0.01)
+    got = c.get("pos")
+    np.testing.assert_allclose(got[:, 1], 0.0, atol=1e-12)
+    np.testing.assert_allclose(got[:, 0] - x[:, 0], -mu * 0.01, rtol=1e-6)
+    # small slip below the cone is removed entirely
+    x2 = prev.copy()
+    x2[:, 1] = -0.01
+    x2[:, 0] += 0.002
+    c.put("pos", x2)
+    c.detect()
+    c.solve_contacts(0.01)
+    np.testing.assert_allclose(c.get("pos")[:, 0], prev[:, 0], atol=1e-12)
+
+
+def test_wall_contacts_follow_planes(oracle_mod):
+    """WallCollisions (ClothCollision.cpp:199-220): contact when the vertex is
+    within thickness + tolerance of a plane, in the order ground, -x, +x, -z, +z."""
+    c = oracle_mod.Cloth(real64=True).set(unit=1, fem=1, thickness=0.1, tolerance=0.05, collFlags=8)
+    c.add_walls((-1, 0, -1), (1, 5, 1))
+    c.init_grid(2, 2, 1.0, (0, 0, 0), True, False)
+    x = np.array([[-0.9, 0.1, -0.9], [0.0, 0.16, 0.0], [0.86, 2.0, 0.0], [0.0, 0.149, 0.9]])
+    c.put("pos", x)
+    c.detect()
+    ids, data = c.contacts()
+    normals = [tuple(d[3:6]) for d in data]
+    assert list(ids[:, 0]) == [0, 0, 0, 2, 3, 3]
+    assert normals == [(0, 1, 0), (1, 0, 0), (0, 0, 1), (-1, 0, 0), (0, 1, 0), (0, 0, -1)]
+
+
+def test_acos_accuracy_against_libm(oracle_mod):
+    """The project-written arccosine (Abramowitz & Stegun 4.4.46) stays within
+    5e-7 (about 2 ulp of pi) of libm's double-precision acos over [-1, 1],
+    including the ends."""
+    xs = np.concatenate([np.linspace(-1, 1, 20001), [-1.0, -0.999, 0.999, 1.0, 0.0, -0.0, 1e-8]])
+    err = max(abs(oracle_mod.acos(np.float32(x)) - math.acos(float(np.float32(x)))) for x in xs)
+    assert err < 5e-7
+    assert oracle_mod.acos(1.0) == 0.0
+    assert abs(oracle_mod.acos(-1.0) - math.pi) < 3e-7
+
+
+def test_grid_init_closed_form(oracle_mod):
+    """ClothPatch::Init (horizontal): vertex (j, i) at (x_j, 0, x_0 + inc i) +
+    offset with x_j accumulated as x_0 + inc + ... (FP32), x_0 = -n inc / 2;
+    attached pins (j = 0 and n-1, i = n-1)."""
+    n, inc, off = 6, np.float32(0.2), np.float32([0.5, 1.0, -0.25])
+    c = oracle_mod.Cloth().init_grid(n, n, inc, tuple(off), True, True)
+    x = np.float32(-0.5) * np.float32(n) * inc
+    x0 = x
+    exp = np.zeros((n * n, 3), np.float32)
+    for j in range(n):
+        for i in range(n):
+            z = x0 if i == 0 else np.float32(x0 + inc * np.float32(i))
+            exp[j * n + i] = np.float32([x, 0, z]) + off
+        x = np.float32(x + inc)
+    assert np.array_equal(c.get("pos").astype(np.float32), exp)
+    m = c.get("invMass")
+    assert set(np.nonzero(m == 0)[0]) == {n - 1, n * n - 1}
+    tris, rest = c.triangles()
+    assert len(tris) == 2 * (n - 1) ** 2
+    # rest areas are half the cell (uv = material coordinates)
+    np.testing.assert_allclose(np.abs(rest[:, 8]), 0.5 * float(inc) ** 2, rtol=1e-5)
