@@ -10,12 +10,15 @@ A step is one frame (scStepCloth: 4 substeps). W warm-up frames, then K
 timed frames; the device is idle on both sides of the timed region (every
 scStepCloth ends with a stream synchronisation). Prints one JSON line.
 
-roofline: the dominant kernel's average duration is measured with HIP events
-on the solver's stream over further frames of the same run (the library
-brackets every launch of that kernel), and achieved bandwidth is its
-algorithmic bytes per launch (136 B per FEM triangle, 128 B per bend) over
-that duration. cpu_baseline: the CPU oracle (a single-threaded restatement of
-the reference solver) on a bounded sample of the same workload.
+roofline: the timed frames (warm-up included) are replayed from a fresh scene
+— the trajectory is bit-identical — with every kernel launch bracketed by HIP
+events on the solver's stream; the dominant kernel's average duration comes
+from those events, and achieved bandwidth is its algorithmic bytes per launch
+(136 B per FEM triangle, 128 B per bend) over that duration. A rocprofv3
+kernel trace of `bench.py --no-graph --no-cpu --copy-probe-mb 0` with the same
+--steps/--warmup sees exactly these frames twice, so its per-kernel averages
+are directly comparable. cpu_baseline: the CPU oracle (a single-threaded
+restatement of the reference solver) on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -33,29 +36,39 @@ BYTES_PER_BEND = 128
 KERNEL_NAMES = {0: "k_fem", 1: "k_bend", 2: "k_contacts", 3: "k_detect", 4: "k_predict+k_velocity"}
 
 
-def run_gpu(scene, steps, warmup, roof_frames, use_graph=True):
+def run_gpu(scene, steps, warmup, use_graph=True, sweep=1):
+    """The timed run: `warmup` frames, then `steps` frames under the clock."""
     import solidcloth_amd as sc
     from solidcloth_amd import scenes as S
 
     world, patch = S.build_patch(scene, sc)
     m = patch.GetModel()
     m.SetUseGraph(use_graph)
+    m.SetSweep(sweep)
     for _ in range(warmup):
         patch.Step(scene.dt)
     t0 = time.perf_counter()
     for _ in range(steps):
         patch.Step(scene.dt)
     t1 = time.perf_counter()
-    elapsed = t1 - t0
-    stats = m.stats()
+    return t1 - t0, m.stats(), (world, patch, m)
 
-    # per-kernel durations with HIP events over further frames of the run
+
+def run_events(scene, frames, sweep=1):
+    """The same frames again from a fresh scene (the trajectory is bit-identical),
+    every kernel launch bracketed by HIP events on the solver's stream."""
+    import solidcloth_amd as sc
+    from solidcloth_amd import scenes as S
+
+    world, patch = S.build_patch(scene, sc)
+    m = patch.GetModel()
+    m.SetSweep(sweep)
     m.enable_kernel_timing(True)
-    for _ in range(roof_frames):
+    for _ in range(frames):
         patch.Step(scene.dt)
     timing = {k: m.kernel_timing(k) for k in KERNEL_NAMES}
     m.enable_kernel_timing(False)
-    return elapsed, stats, timing, m
+    return timing
 
 
 def cpu_baseline(scene, budget_s):
@@ -79,11 +92,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--roofline-frames", type=int, default=5)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels directly instead of replaying the captured substep graph")
+    ap.add_argument("--sweep", type=int, default=1, choices=[0, 1],
+                    help="0: one launch per colour; 1: slab phases staged in LDS")
+    ap.add_argument("--copy-probe-mb", type=int, default=1024,
+                    help="size of the streaming-copy bandwidth probe (0 disables)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only "
@@ -93,8 +109,10 @@ def main():
     from solidcloth_amd import scenes as S
 
     scene = S.DRAPE256
-    elapsed, stats, timing, m = run_gpu(scene, args.steps, args.warmup, args.roofline_frames,
-                                        use_graph=not args.no_graph)
+    elapsed, stats, keep = run_gpu(scene, args.steps, args.warmup, use_graph=not args.no_graph, sweep=args.sweep)
+    m = keep[2]
+    event_frames = args.warmup + args.steps
+    timing = run_events(scene, event_frames, sweep=args.sweep)
     substeps = args.steps * scene.substeps
     value = substeps / elapsed
     T = scene.num_triangles
@@ -107,7 +125,7 @@ def main():
     dom = max(timing, key=lambda k: timing[k][1])
     launches, total_ms = timing[dom]
     avg_us = 1000.0 * total_ms / max(1, launches)
-    frames_t = args.roofline_frames
+    frames_t = event_frames
     if dom == 0:
         units = T * its * scene.substeps * frames_t
         bytes_per_launch = BYTES_PER_TRI * units / launches
@@ -139,6 +157,9 @@ def main():
             "iterations": its,
             "substeps_per_frame": scene.substeps,
             "graph_replay": not args.no_graph,
+            "sweep": ["colour launches", "slab phases (LDS)"][args.sweep],
+            "launch_phases_per_pass": {"fem": len(m.colour_offsets("tri")) - 1,
+                                       "bend": len(m.colour_offsets("bend")) - 1},
         },
         "constraint_projections_per_s": round(proj_per_substep * value, 1),
         "solver_iterations_last_substep": stats["iterations"],
@@ -156,6 +177,14 @@ def main():
             "algorithmic_bytes_per_launch": round(bytes_per_launch) if bytes_per_launch else None,
         },
     }
+    if args.copy_probe_mb > 0:
+        import solidcloth_amd as sc
+
+        nbytes = args.copy_probe_mb << 20
+        ms = sc.lib().scBandwidthProbe(nbytes, 20)
+        if ms > 0:
+            out["copy_probe"] = {"bytes_copied": nbytes, "avg_ms": round(ms, 4),
+                                 "GB/s (read+write)": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
     if not args.no_cpu:
         cpu_val, frames = cpu_baseline(scene, args.cpu_budget)
         out["cpu_baseline"] = {

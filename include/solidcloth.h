@@ -190,10 +190,19 @@ SCEXP int scGetNumBends(scModel model);
 SCEXP int scGetBends(scModel model, int* out, int count);
 
 /* Gauss-Seidel order the GPU applies: kind 0 = triangles, 1 = bends. Writes
- * the constraint indices in processing order and returns their count;
- * colourOffsets (may be NULL) receives numColours+1 offsets. */
+ * the constraint indices in processing order and returns their count.
+ * scGetColouring returns the number of launch phases and writes phase start
+ * offsets into that order (numPhases + 1 values, colourOffsets may be NULL).
+ * scGetSchedule writes, per position of the order, the phase, the slab (-1
+ * when the sweep launches one colour per phase) and the colour inside the slab. */
 SCEXP int scGetConstraintOrder(scModel model, int kind, int* order, int count);
 SCEXP int scGetColouring(scModel model, int kind, int* colourOffsets, int maxColours);
+SCEXP int scGetSchedule(scModel model, int kind, int* phase, int* slab, int* colour, int count);
+
+/* Sweep schedule: mode 0 = one launch per colour (positions in HBM),
+ * mode 1 = slab phases with the slab's vertices staged in LDS (default).
+ * slabSize 0 lets the library choose. Changes the Gauss-Seidel order. */
+SCEXP void scSetSweep(scModel model, int mode, int slabSize);
 
 /* Solver statistics of the last substep (Physics/ClothModel.h:32-40):
  * out[0] iterations, out[1] bend error, out[2] vertex-contact error. */
@@ -207,6 +216,12 @@ SCEXP int scGetKernelTiming(scModel model, int which, int* launches, double* tot
 
 /* Use a captured hipGraph for the substep (default true). */
 SCEXP void scSetUseGraph(scModel model, bool enable);
+
+/* Device-to-device copy of `bytes` bytes (rounded down to 16) with a plain
+ * streaming kernel, repeated `reps` times; returns the average kernel time in
+ * milliseconds (HIP events), or a negative value on error. Used to measure
+ * attainable HBM bandwidth and to calibrate traffic counters. */
+SCEXP double scBandwidthProbe(long long bytes, int reps);
 
 /* Last error message of the calling thread, or "" (not in the reference),
  * and the number of errors recorded on the calling thread so far, so a

@@ -101,9 +101,11 @@ def _sig(L):
         "scGetParticleVelocities": (i, [H, vp, i]), "scSetParticleVelocities": (i, [H, vp, i]),
         "scGetNumBends": (i, [H]), "scGetBends": (i, [H, vp, i]),
         "scGetConstraintOrder": (i, [H, i, vp, i]), "scGetColouring": (i, [H, i, vp, i]),
+        "scGetSchedule": (i, [H, i, vp, vp, vp, i]), "scSetSweep": (None, [H, i, i]),
         "scGetSolverStats": (None, [H, vp]), "scEnableKernelTiming": (None, [H, b]),
         "scGetKernelTiming": (i, [H, i, C.POINTER(i), C.POINTER(C.c_double)]),
-        "scSetUseGraph": (None, [H, b]), "scGetLastError": (C.c_char_p, []),
+        "scSetUseGraph": (None, [H, b]), "scBandwidthProbe": (C.c_double, [C.c_longlong, i]),
+        "scGetLastError": (C.c_char_p, []),
         "scGetErrorCount": (i, []), "scGetBuildInfo": (C.c_char_p, []),
     }
     for name, (res, args) in sigs.items():
@@ -294,11 +296,24 @@ class ClothModel:
         return out
 
     def colour_offsets(self, kind):
+        """Start offsets of the launch phases in constraint_order(kind)."""
         k = 0 if kind in (0, "tri") else 1
         nc = _checked("scGetColouring", self.h, k, None, 0)
         out = np.zeros(nc + 1, dtype=np.int32)
         _checked("scGetColouring", self.h, k, _ptr(out), nc)
         return out
+
+    def schedule(self, kind):
+        """Per position of constraint_order(kind): (phase, slab, colour)."""
+        k = 0 if kind in (0, "tri") else 1
+        n = _checked("scGetSchedule", self.h, k, None, None, None, 0)
+        ph, sl, co = (np.zeros(n, dtype=np.int32) for _ in range(3))
+        _checked("scGetSchedule", self.h, k, _ptr(ph), _ptr(sl), _ptr(co), n)
+        return ph, sl, co
+
+    def SetSweep(self, mode, slab_size=0):
+        """mode 0: one launch per colour; mode 1: slab phases staged in LDS."""
+        _checked("scSetSweep", self.h, int(mode), int(slab_size))
 
     def contacts(self):
         n = _checked("scGetNumContacts", self.h)

@@ -590,7 +590,7 @@ int scGetConstraintOrder(scModel model, int kind, int* order, int count)
     return guard_r<int>(-1, [&] {
         ClothModel* m = M(model);
         m->ensure_colouring();
-        const Colouring& c = kind == 0 ? m->triCol : m->bendCol;
+        const Schedule& c = kind == 0 ? m->triCol : m->bendCol;
         int n = std::min(count, (int)c.order.size());
         if (order) memcpy(order, c.order.data(), n * sizeof(int));
         return (int)c.order.size();
@@ -602,14 +602,30 @@ int scGetColouring(scModel model, int kind, int* colourOffsets, int maxColours)
     return guard_r<int>(-1, [&] {
         ClothModel* m = M(model);
         m->ensure_colouring();
-        const Colouring& c = kind == 0 ? m->triCol : m->bendCol;
-        int nc = (int)c.offsets.size() - 1;
+        const Schedule& c = kind == 0 ? m->triCol : m->bendCol;
+        int nc = c.numPhases();
         if (nc < 0) nc = 0;
         if (colourOffsets)
-            for (int k = 0; k <= nc && k <= maxColours; k++) colourOffsets[k] = c.offsets[k];
+            for (int k = 0; k <= nc && k <= maxColours; k++) colourOffsets[k] = c.phaseOffsets[k];
         return nc;
     });
 }
+
+int scGetSchedule(scModel model, int kind, int* phase, int* slab, int* colour, int count)
+{
+    return guard_r<int>(-1, [&] {
+        ClothModel* m = M(model);
+        m->ensure_colouring();
+        const Schedule& c = kind == 0 ? m->triCol : m->bendCol;
+        int n = std::min(count, (int)c.order.size());
+        if (phase) memcpy(phase, c.phaseOf.data(), n * sizeof(int));
+        if (slab) memcpy(slab, c.slabOf.data(), n * sizeof(int));
+        if (colour) memcpy(colour, c.colourOf.data(), n * sizeof(int));
+        return (int)c.order.size();
+    });
+}
+
+void scSetSweep(scModel model, int mode, int slabSize) { guard([&] { M(model)->set_sweep(mode, slabSize); }); }
 
 void scGetSolverStats(scModel model, float* out)
 {
@@ -643,6 +659,50 @@ int scGetKernelTiming(scModel model, int which, int* launches, double* totalMs)
 }
 
 void scSetUseGraph(scModel model, bool enable) { guard([&] { M(model)->mUseGraph = enable; }); }
+
+double scBandwidthProbe(long long bytes, int reps)
+{
+    return guard_r<double>(-1.0, [&] {
+        if (bytes < 16 || reps < 1) throw std::runtime_error("bad probe size");
+        const long n = (long)(bytes / 16);
+        float4 *a = nullptr, *b = nullptr;
+        hipStream_t s = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        double ms = -1.0;
+        auto cleanup = [&] {
+            if (a) (void)hipFree(a);
+            if (b) (void)hipFree(b);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (s) (void)hipStreamDestroy(s);
+        };
+        try {
+            auto ok = [](hipError_t e) {
+                if (e != hipSuccess) throw std::runtime_error(hipGetErrorString(e));
+            };
+            ok(hipMalloc((void**)&a, n * sizeof(float4)));
+            ok(hipMalloc((void**)&b, n * sizeof(float4)));
+            ok(hipMemset(a, 0, n * sizeof(float4)));
+            ok(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            ok(hipEventCreate(&e0));
+            ok(hipEventCreate(&e1));
+            launch_copy(s, a, b, n);  // warm-up
+            ok(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; r++) launch_copy(s, a, b, n);
+            ok(hipEventRecord(e1, s));
+            ok(hipEventSynchronize(e1));
+            ok(hipGetLastError());
+            float t = 0.f;
+            ok(hipEventElapsedTime(&t, e0, e1));
+            ms = t / reps;
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+        return ms;
+    });
+}
 
 const char* scGetLastError(void) { return g_lastError.c_str(); }
 

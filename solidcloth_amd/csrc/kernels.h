@@ -72,6 +72,29 @@ struct TriStreams {
     const float2* c;
 };
 
+// Slab schedule on the device (slabs numbered phase-major): per slab its
+// first vertex in `verts`, vertex count, first colour in `colOff` and colour
+// count; colour k of a slab spans constraint positions [colOff[k], colOff[k+1]).
+// Constraint vertices are given as slab-local indices (ushort4).
+struct SlabSet {
+    const int4* info;
+    const int* colOff;
+    const int* verts;
+    int base;  // first slab of the launched phase
+};
+
+enum { kSlabBlock = 256, kMaxSlabColours = 63 };
+
+// Vertex contacts applied by the FEM slab kernel while it stages vertices
+// (the previous iteration's contacts, by each vertex's owner slab).
+struct SlabContacts {
+    ContactBuf cb;
+    const float4* prev;
+    float4* vel;
+    float thickness, mu, invH;
+    int enabled;
+};
+
 // Launchers (kernels.hip). All are asynchronous on the given stream.
 void launch_predict(hipStream_t s, float4* pos, float4* prev, float4* vel, int n, float h, f3 gdv, float drag,
                     SolverCtl* ctl);
@@ -80,10 +103,20 @@ void launch_detect(hipStream_t s, const float4* pos, const float4* prev, int n, 
 void launch_fem(hipStream_t s, float4* pos, TriStreams tr, int start, int count, float h2, float epsUV,
                 const SolverCtl* ctl);
 void launch_bend(hipStream_t s, float4* pos, const int4* bidx, int start, int count, float stiff, SolverCtl* ctl);
+void launch_fem_slab(hipStream_t s, float4* pos, TriStreams tr, const ushort4* lidx, SlabSet ss, int numSlabs,
+                     int maxVerts, float h2, float epsUV, const SolverCtl* ctl, const SlabContacts& kc);
+void launch_predict_detect(hipStream_t s, float4* pos, float4* prev, float4* vel, int n, float h, f3 gdv, float drag,
+                           SolverCtl* ctl, const CollSet& cs, ContactBuf cb, float thickness, float tolerance,
+                           int collFlags);
+void launch_contacts_velocity(hipStream_t s, float4* pos, const float4* prev, float4* vel, ContactBuf cb,
+                              float thickness, float mu, float invH);
+void launch_bend_slab(hipStream_t s, float4* pos, const ushort4* lidx, SlabSet ss, int numSlabs, int maxVerts,
+                      float stiff, SolverCtl* ctl);
 void launch_contacts(hipStream_t s, float4* pos, const float4* prev, float4* vel, ContactBuf cb, float thickness,
                      float mu, float invH, SolverCtl* ctl);
 void launch_iter_end(hipStream_t s, SolverCtl* ctl, int maxIter, int fixedIter, int bendFreq, int dihedral,
                      float selfThreshold);
 void launch_velocity(hipStream_t s, float4* pos, const float4* prev, float4* vel, int n, float invH);
+void launch_copy(hipStream_t s, const float4* src, float4* dst, long n);
 
 }  // namespace sc

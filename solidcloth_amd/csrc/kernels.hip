@@ -95,14 +95,11 @@ __device__ __forceinline__ void put_contact(ContactBuf cb, int v, int& cnt, f3 p
 }
 
 // Physics/ClothCollision.cpp:50-87, 199-220, 430-470 — external vertex
-// contacts, one thread per vertex, collidables visited in world order.
-__global__ void k_detect(const float4* __restrict__ pos, const float4* __restrict__ prev, int n, CollSet cs,
-                         ContactBuf cb, float thickness, float tolerance, int collFlags)
+// contacts of vertex v (predicted position p, previous position pv),
+// collidables visited in world order.
+__device__ __forceinline__ void detect_vertex(int v, const f3 p, const f3 pv, const CollSet& cs, ContactBuf cb,
+                                              float thickness, float tolerance, int collFlags)
 {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    f3 p = xyz(pos[v]);
-    f3 pv = xyz(prev[v]);
     int cnt = 0;
     const float tv = len(sub(pv, p));
     const float radTol = thickness + smax(tolerance, tv * 1.1f);
@@ -140,23 +137,52 @@ __global__ void k_detect(const float4* __restrict__ pos, const float4* __restric
     cb.count[v] = cnt;
 }
 
-// Physics/ClothPBD.cpp:52-108 — FEM triangle projection (warp, weft, shear),
-// one thread per triangle of one colour.
-__global__ void __launch_bounds__(kBlock) k_fem(float4* __restrict__ pos, TriStreams tr, int start, int count,
-                                                float h2, float epsUV, const SolverCtl* __restrict__ ctl)
+__global__ void k_detect(const float4* __restrict__ pos, const float4* __restrict__ prev, int n, CollSet cs,
+                         ContactBuf cb, float thickness, float tolerance, int collFlags)
 {
-    if (ctl->done) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    t += start;
-    const int4 id = tr.idx[t];
-    const float4 A = tr.a[t];
-    const float4 B = tr.b[t];
-    const float2 C = tr.c[t];
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    detect_vertex(v, xyz(pos[v]), xyz(prev[v]), cs, cb, thickness, tolerance, collFlags);
+}
+
+// Prediction (ClothPBD.cpp:258-266) and detection of the same vertex in one
+// pass; detection of a vertex reads only that vertex's positions, so this
+// equals the reference's two loops.
+__global__ void k_predict_detect(float4* __restrict__ pos, float4* __restrict__ prev, float4* __restrict__ vel, int n,
+                                 float h, f3 gdv, float drag, SolverCtl* ctl, CollSet cs, ContactBuf cb,
+                                 float thickness, float tolerance, int collFlags)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        ctl->done = 0;
+        ctl->k = 0;
+        ctl->cb = 0;
+        ctl->cv = 0;
+        ctl->iterations = 0;
+    }
+    if (i >= n) return;
+    float4 P = pos[i];
+    const f3 pv = xyz(P);
+    prev[i] = make_float4(P.x, P.y, P.z, 0.f);
+    f3 p = pv;
+    if (P.w != 0.f) {
+        f3 v = add(xyz(vel[i]), gdv);
+        v = f3{v.x * drag, v.y * drag, v.z * drag};
+        p = add(p, mul(h, v));
+        vel[i] = f4(v, 0.f);
+        pos[i] = f4(p, P.w);
+    }
+    detect_vertex(i, p, pv, cs, cb, thickness, tolerance, collFlags);
+}
+
+// Physics/ClothPBD.cpp:52-108 — one FEM triangle projection (warp, weft,
+// shear). P1..P3 hold position and inverse mass; only positions change.
+__device__ __forceinline__ void fem_project(float4& P1, float4& P2, float4& P3, const float4 A, const float4 B,
+                                            const float2 C, float h2, float epsUV)
+{
     const float du1 = A.x, du2 = A.y, dv1 = A.z, dv2 = A.w;
     const float invDet = B.x, lu2 = B.y, lv2 = B.z, area = B.w;
     const float lu0 = C.x, lv0 = C.y;
-    float4 P1 = pos[id.x], P2 = pos[id.y], P3 = pos[id.z];
     const float m1 = P1.w, m2 = P2.w, m3 = P3.w;
     f3 x1 = xyz(P1), x2 = xyz(P2), x3 = xyz(P3);
     f3 dx1 = sub(x2, x1);
@@ -196,13 +222,80 @@ __global__ void __launch_bounds__(kBlock) k_fem(float4* __restrict__ pos, TriStr
     x1 = add(x1, mul(m1 * lambda, q1));
     x2 = add(x2, mul(m2 * lambda, q2));
     x3 = add(x3, mul(m3 * lambda, q3));
-    pos[id.x] = f4(x1, m1);
-    pos[id.y] = f4(x2, m2);
-    pos[id.z] = f4(x3, m3);
+    P1 = f4(x1, m1);
+    P2 = f4(x2, m2);
+    P3 = f4(x3, m3);
 }
 
 // Physics/ClothPBD.cpp:179-224 with EvaluateAngle (110-139) and
-// EvaluateGradient (141-177) — dihedral bending, one thread per bend.
+// EvaluateGradient (141-177) — one dihedral bend projection. Returns the
+// bend's error |s theta| when it was applied, else 0 (skipped bends and NaN
+// errors do not count, as in the reference's `>` comparison).
+__device__ __forceinline__ float bend_project(float4& P1, float4& P2, float4& P3, float4& P4, float sStiff)
+{
+    const float m1 = P1.w, m2 = P2.w, m3 = P3.w, m4 = P4.w;
+    if (m1 + m2 + m3 + m4 == 0.f) return 0.f;
+    const f3 x1 = xyz(P1), x2 = xyz(P2), x3 = xyz(P3), x4 = xyz(P4);
+    f3 x12 = sub(x2, x1);
+    float l = len(x12);
+    f3 x13 = sub(x3, x1);
+    f3 n1 = cross(x12, x13);
+    float l1 = len(n1);
+    f3 x14 = sub(x4, x1);
+    f3 n2 = cross(x14, x12);
+    float l2 = len(n2);
+    float cosine = 1.f, theta = 0.f;
+    if (!(l1 == 0.f || l2 == 0.f)) {
+        float il1 = 1.f / l1, il2 = 1.f / l2;
+        f3 n1n = mul(il1, n1), n2n = mul(il2, n2);
+        cosine = dot(n1n, n2n);
+        cosine = smin(1.f, smax(-1.f, cosine));
+        theta = acos_as(cosine);
+    }
+    if (fabsf(cosine) >= 1 - 0.001f) return 0.f;
+    float h1 = l1 / l, h2 = l2 / l;
+    float he = 0.5f * (h1 + h2);
+    float il1 = 1.f / l1, il2 = 1.f / l2;
+    f3 n1n = mul(il1, n1), n2n = mul(il2, n2);
+    f3 n1p = sub(n1n, mul(cosine, n2n));
+    f3 n2p = sub(n2n, mul(cosine, n1n));
+    float sine = sqrtf(1 - cosine * cosine);
+    // The reference's four factors (+-il1 / sine / he, +-il2 / sine / he) from
+    // two division chains: rounding to nearest is symmetric in sign, so
+    // (-a) / b is exactly -(a / b).
+    const float f1 = il1 / sine / he, f2 = il2 / sine / he;
+    f3 q3 = mul(f1, cross(x12, n2p));
+    f3 q4 = mul(-f2, cross(x12, n1p));
+    f3 q2 = add(mul(-f1, cross(x13, n2p)), mul(f2, cross(x14, n1p)));
+    f3 q1 = sub(sub(neg(q2), q3), q4);
+    float diag = m1 * dot(q1, q1) + m2 * dot(q2, q2) + m3 * dot(q3, q3) + m4 * dot(q4, q4);
+    if (diag == 0.f) return 0.f;
+    float e = fabsf(sStiff * theta);
+    float lambda = -(sStiff * theta / he / diag);
+    P1 = f4(add(x1, mul(lambda * m1, q1)), m1);
+    P2 = f4(add(x2, mul(lambda * m2, q2)), m2);
+    P3 = f4(add(x3, mul(lambda * m3, q3)), m3);
+    P4 = f4(add(x4, mul(lambda * m4, q4)), m4);
+    return e > 0.f ? e : 0.f;
+}
+
+// Colour schedule: one thread per triangle of one colour, positions in HBM.
+__global__ void __launch_bounds__(kBlock) k_fem(float4* __restrict__ pos, TriStreams tr, int start, int count,
+                                                float h2, float epsUV, const SolverCtl* __restrict__ ctl)
+{
+    if (ctl->done) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    t += start;
+    const int4 id = tr.idx[t];
+    float4 P1 = pos[id.x], P2 = pos[id.y], P3 = pos[id.z];
+    fem_project(P1, P2, P3, tr.a[t], tr.b[t], tr.c[t], h2, epsUV);
+    pos[id.x] = P1;
+    pos[id.y] = P2;
+    pos[id.z] = P3;
+}
+
+// Colour schedule: one thread per bend of one colour.
 __global__ void __launch_bounds__(kBlock) k_bend(float4* __restrict__ pos, const int4* __restrict__ bidx, int start,
                                                  int count, float sStiff, SolverCtl* ctl)
 {
@@ -212,51 +305,191 @@ __global__ void __launch_bounds__(kBlock) k_bend(float4* __restrict__ pos, const
     if (t < count) {
         const int4 b = bidx[start + t];
         float4 P1 = pos[b.x], P2 = pos[b.y], P3 = pos[b.z], P4 = pos[b.w];
-        const float m1 = P1.w, m2 = P2.w, m3 = P3.w, m4 = P4.w;
-        const f3 x1 = xyz(P1), x2 = xyz(P2), x3 = xyz(P3), x4 = xyz(P4);
-        bool active = !(m1 + m2 + m3 + m4 == 0.f);
-        f3 x12 = sub(x2, x1);
-        float l = len(x12);
-        f3 x13 = sub(x3, x1);
-        f3 n1 = cross(x12, x13);
-        float l1 = len(n1);
-        f3 x14 = sub(x4, x1);
-        f3 n2 = cross(x14, x12);
-        float l2 = len(n2);
-        float cosine = 1.f, theta = 0.f;
-        if (!(l1 == 0.f || l2 == 0.f)) {
-            float il1 = 1.f / l1, il2 = 1.f / l2;
-            f3 n1n = mul(il1, n1), n2n = mul(il2, n2);
-            cosine = dot(n1n, n2n);
-            cosine = smin(1.f, smax(-1.f, cosine));
-            theta = acos_as(cosine);
-        }
-        if (fabsf(cosine) >= 1 - 0.001f) active = false;
-        if (active) {
-            float h1 = l1 / l, h2 = l2 / l;
-            float he = 0.5f * (h1 + h2);
-            float il1 = 1.f / l1, il2 = 1.f / l2;
-            f3 n1n = mul(il1, n1), n2n = mul(il2, n2);
-            f3 n1p = sub(n1n, mul(cosine, n2n));
-            f3 n2p = sub(n2n, mul(cosine, n1n));
-            float sine = sqrtf(1 - cosine * cosine);
-            f3 q3 = mul(il1 / sine / he, cross(x12, n2p));
-            f3 q4 = mul(-il2 / sine / he, cross(x12, n1p));
-            f3 q2 = add(mul(-il1 / sine / he, cross(x13, n2p)), mul(il2 / sine / he, cross(x14, n1p)));
-            f3 q1 = sub(sub(neg(q2), q3), q4);
-            float diag = m1 * dot(q1, q1) + m2 * dot(q2, q2) + m3 * dot(q3, q3) + m4 * dot(q4, q4);
-            if (!(diag == 0.f)) {
-                float e = fabsf(sStiff * theta);
-                errOut = e > 0.f ? e : 0.f;
-                float lambda = -sStiff * theta / he / diag;
-                pos[b.x] = f4(add(x1, mul(lambda * m1, q1)), m1);
-                pos[b.y] = f4(add(x2, mul(lambda * m2, q2)), m2);
-                pos[b.z] = f4(add(x3, mul(lambda * m3, q3)), m3);
-                pos[b.w] = f4(add(x4, mul(lambda * m4, q4)), m4);
-            }
-        }
+        errOut = bend_project(P1, P2, P3, P4, sStiff);
+        pos[b.x] = P1;
+        pos[b.y] = P2;
+        pos[b.z] = P3;
+        pos[b.w] = P4;
     }
     wave_max_atomic(&ctl->cb, errOut);
+}
+
+// The contacts of vertex v, in slot order, applied to its position p and
+// velocity ve (ClothCollisionHandler.cpp:141-199). Returns the largest depth.
+__device__ __forceinline__ float vertex_contacts(int v, int n, f3& p, const f3 pp, f3& ve, ContactBuf cb,
+                                                 float thickness, float mu, float invH)
+{
+    float err = 0.f;
+    for (int s = 0; s < n; s++) {
+        float4 CP = cb.P[s * cb.nv + v];
+        float4 CN = cb.N[s * cb.nv + v];
+        f3 cpt = xyz(CP), nn = xyz(CN);
+        float lam = CP.w, lamF = CN.w;
+        f3 delta = sub(p, cpt);
+        float l = dot(delta, nn);
+        if (l > thickness) continue;
+        float depth = thickness - l;
+        if (fabsf(depth) > err) err = fabsf(depth);
+        lam += depth;
+        f3 disp = mul(depth, nn);
+        f3 v12 = sub(p, pp);
+        float vnrel = dot(nn, v12);
+        f3 vt = sub(v12, mul(vnrel, nn));
+        float vtrel = len(vt);
+        const float limit = mu * lam;
+        if (vtrel > 0.001f) {
+            float dLF = vtrel;
+            float lf0 = lamF;
+            lamF = lf0 + dLF;
+            if (lamF >= limit) lamF = limit;
+            dLF = lamF - lf0;
+            vt = mul(1.0f / vtrel, vt);
+            disp = add(disp, mul(-dLF, vt));
+        }
+        p = add(p, disp);
+        ve = add(ve, mul(invH, disp));
+        cb.P[s * cb.nv + v] = make_float4(CP.x, CP.y, CP.z, lam);
+        cb.N[s * cb.nv + v] = make_float4(CN.x, CN.y, CN.z, lamF);
+    }
+    return err;
+}
+
+// Slab schedule: one workgroup per slab of the phase. The slab's vertices
+// are staged in LDS, its local colours are swept with a barrier between
+// colours, and the vertices are written back. Slabs of one phase touch
+// disjoint vertices, so workgroups never race. The constraint data of colour
+// c+1 is loaded into registers while colour c is projected, so the sweep
+// waits on global memory once per launch instead of once per colour.
+// Slab vertex ids carry an owner flag in bit 31: the vertex's first slab in
+// phase order. With `kc.enabled`, the owner applies the vertex's contacts of
+// the previous iteration while staging it — after every constraint of that
+// iteration (earlier launches) and before any constraint of this one, which
+// is where the reference's sequential loop puts them.
+__device__ __forceinline__ int slab_begin(const SlabSet& ss, int4& info, int* scol, float4* sv, const float4* pos,
+                                          const SlabContacts& kc)
+{
+    info = ss.info[ss.base + blockIdx.x];
+    const int* vg = ss.verts + info.x;
+    for (int k = threadIdx.x; k <= info.w; k += blockDim.x) scol[k] = ss.colOff[info.z + k];
+    for (int v = threadIdx.x; v < info.y; v += blockDim.x) {
+        const int raw = vg[v];
+        const int g = raw & 0x7fffffff;
+        float4 P = pos[g];
+        if (kc.enabled && raw < 0) {
+            const int n = kc.cb.count[g];
+            if (n > 0) {
+                f3 p = xyz(P), ve = xyz(kc.vel[g]);
+                vertex_contacts(g, n, p, xyz(kc.prev[g]), ve, kc.cb, kc.thickness, kc.mu, kc.invH);
+                P = f4(p, P.w);
+                kc.vel[g] = f4(ve, 0.f);
+            }
+        }
+        sv[v] = P;
+    }
+    __syncthreads();
+    return info.w;
+}
+
+__device__ __forceinline__ void slab_end(const SlabSet& ss, const int4& info, const float4* sv, float4* pos)
+{
+    __syncthreads();
+    const int* vg = ss.verts + info.x;
+    for (int v = threadIdx.x; v < info.y; v += blockDim.x) pos[vg[v] & 0x7fffffff] = sv[v];
+}
+
+__global__ void __launch_bounds__(kSlabBlock) k_fem_slab(float4* __restrict__ pos, TriStreams tr,
+                                                         const ushort4* __restrict__ lidx, SlabSet ss, float h2,
+                                                         float epsUV, const SolverCtl* __restrict__ ctl,
+                                                         SlabContacts kc)
+{
+    if (ctl->done) return;
+    extern __shared__ float4 sv[];
+    __shared__ int scol[kMaxSlabColours + 1];
+    int4 info;
+    const int nc = slab_begin(ss, info, scol, sv, pos, kc);
+    int t = scol[0] + threadIdx.x;
+    bool have = t < scol[1];
+    ushort4 id = make_ushort4(0, 0, 0, 0);
+    float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = A;
+    float2 C = make_float2(0.f, 0.f);
+    if (have) { id = lidx[t]; A = tr.a[t]; B = tr.b[t]; C = tr.c[t]; }
+    for (int c = 0; c < nc; c++) {
+        const int e = scol[c + 1];
+        const int tn = e + threadIdx.x;
+        const bool haveN = c + 1 < nc && tn < scol[c + 2];
+        ushort4 idN = id;
+        float4 AN = A, BN = B;
+        float2 CN = C;
+        if (haveN) { idN = lidx[tn]; AN = tr.a[tn]; BN = tr.b[tn]; CN = tr.c[tn]; }
+        if (have) {
+            float4 P1 = sv[id.x], P2 = sv[id.y], P3 = sv[id.z];
+            fem_project(P1, P2, P3, A, B, C, h2, epsUV);
+            sv[id.x] = P1;
+            sv[id.y] = P2;
+            sv[id.z] = P3;
+        }
+        // a colour wider than the workgroup: remaining constraints, in order
+        for (int t2 = t + blockDim.x; t2 < e; t2 += blockDim.x) {
+            const ushort4 i2 = lidx[t2];
+            float4 P1 = sv[i2.x], P2 = sv[i2.y], P3 = sv[i2.z];
+            fem_project(P1, P2, P3, tr.a[t2], tr.b[t2], tr.c[t2], h2, epsUV);
+            sv[i2.x] = P1;
+            sv[i2.y] = P2;
+            sv[i2.z] = P3;
+        }
+        __syncthreads();
+        t = tn;
+        have = haveN;
+        id = idN; A = AN; B = BN; C = CN;
+    }
+    slab_end(ss, info, sv, pos);
+}
+
+__global__ void __launch_bounds__(kSlabBlock) k_bend_slab(float4* __restrict__ pos, const ushort4* __restrict__ lidx,
+                                                          SlabSet ss, float sStiff, SolverCtl* ctl)
+{
+    if (ctl->done) return;
+    extern __shared__ float4 sv[];
+    __shared__ int scol[kMaxSlabColours + 1];
+    int4 info;
+    SlabContacts none;
+    none.enabled = 0;
+    const int nc = slab_begin(ss, info, scol, sv, pos, none);
+    float err = 0.f;
+    int t = scol[0] + threadIdx.x;
+    bool have = t < scol[1];
+    ushort4 id = have ? lidx[t] : make_ushort4(0, 0, 0, 0);
+    for (int c = 0; c < nc; c++) {
+        const int e = scol[c + 1];
+        const int tn = e + threadIdx.x;
+        const bool haveN = c + 1 < nc && tn < scol[c + 2];
+        const ushort4 idN = haveN ? lidx[tn] : id;
+        if (have) {
+            float4 P1 = sv[id.x], P2 = sv[id.y], P3 = sv[id.z], P4 = sv[id.w];
+            float e1 = bend_project(P1, P2, P3, P4, sStiff);
+            err = err < e1 ? e1 : err;
+            sv[id.x] = P1;
+            sv[id.y] = P2;
+            sv[id.z] = P3;
+            sv[id.w] = P4;
+        }
+        for (int t2 = t + blockDim.x; t2 < e; t2 += blockDim.x) {
+            const ushort4 i2 = lidx[t2];
+            float4 P1 = sv[i2.x], P2 = sv[i2.y], P3 = sv[i2.z], P4 = sv[i2.w];
+            float e1 = bend_project(P1, P2, P3, P4, sStiff);
+            err = err < e1 ? e1 : err;
+            sv[i2.x] = P1;
+            sv[i2.y] = P2;
+            sv[i2.z] = P3;
+            sv[i2.w] = P4;
+        }
+        __syncthreads();
+        t = tn;
+        have = haveN;
+        id = idN;
+    }
+    slab_end(ss, info, sv, pos);
+    wave_max_atomic(&ctl->cb, err);
 }
 
 // Physics/ClothCollisionHandler.cpp:134-202 — vertex contacts with friction.
@@ -274,43 +507,33 @@ __global__ void __launch_bounds__(kBlock) k_contacts(float4* __restrict__ pos, c
     if (n > 0) {
         float4 P = pos[v];
         f3 p = xyz(P);
-        const f3 pp = xyz(prev[v]);
         f3 ve = xyz(vel[v]);
-        for (int s = 0; s < n; s++) {
-            float4 CP = cb.P[s * cb.nv + v];
-            float4 CN = cb.N[s * cb.nv + v];
-            f3 cpt = xyz(CP), nn = xyz(CN);
-            float lam = CP.w, lamF = CN.w;
-            f3 delta = sub(p, cpt);
-            float l = dot(delta, nn);
-            if (l > thickness) continue;
-            float depth = thickness - l;
-            if (fabsf(depth) > err) err = fabsf(depth);
-            lam += depth;
-            f3 disp = mul(depth, nn);
-            f3 v12 = sub(p, pp);
-            float vnrel = dot(nn, v12);
-            f3 vt = sub(v12, mul(vnrel, nn));
-            float vtrel = len(vt);
-            const float limit = mu * lam;
-            if (vtrel > 0.001f) {
-                float dLF = vtrel;
-                float lf0 = lamF;
-                lamF = lf0 + dLF;
-                if (lamF >= limit) lamF = limit;
-                dLF = lamF - lf0;
-                vt = mul(1.0f / vtrel, vt);
-                disp = add(disp, mul(-dLF, vt));
-            }
-            p = add(p, disp);
-            ve = add(ve, mul(invH, disp));
-            cb.P[s * cb.nv + v] = make_float4(CP.x, CP.y, CP.z, lam);
-            cb.N[s * cb.nv + v] = make_float4(CN.x, CN.y, CN.z, lamF);
-        }
+        err = vertex_contacts(v, n, p, xyz(prev[v]), ve, cb, thickness, mu, invH);
         pos[v] = f4(p, P.w);
         vel[v] = f4(ve, 0.f);
     }
     wave_max_atomic(&ctl->cv, err);
+}
+
+// The last iteration's vertex contacts followed by the velocity update
+// (ClothPBD.cpp:348, 314-321), one thread per vertex.
+__global__ void __launch_bounds__(kBlock) k_contacts_velocity(float4* __restrict__ pos, const float4* __restrict__ prev,
+                                                              float4* __restrict__ vel, ContactBuf cb, float thickness,
+                                                              float mu, float invH)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= cb.nv) return;
+    float4 P = pos[v];
+    const f3 pp = xyz(prev[v]);
+    f3 p = xyz(P);
+    f3 ve = xyz(vel[v]);
+    const int n = cb.count[v];
+    if (n > 0) {
+        vertex_contacts(v, n, p, pp, ve, cb, thickness, mu, invH);
+        pos[v] = f4(p, P.w);
+    }
+    if (P.w != 0.f) ve = mul(invH, sub(p, pp));
+    if (n > 0 || P.w != 0.f) vel[v] = f4(ve, 0.f);
 }
 
 // Physics/ClothPBD.cpp:354-361 — iteration counter and the early-exit test.
@@ -347,6 +570,23 @@ __global__ void k_velocity(const float4* __restrict__ pos, const float4* __restr
     vel[i] = f4(mul(invH, sub(xyz(P), xyz(prev[i]))), 0.f);
 }
 
+// Plain streaming copy (16-byte loads and stores, grid-stride), used to
+// measure the attainable HBM bandwidth and to calibrate the FETCH_SIZE and
+// WRITE_SIZE counters against a kernel of known traffic.
+__global__ void __launch_bounds__(kBlock) k_copy(const float4* __restrict__ src, float4* __restrict__ dst, long n)
+{
+    long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+void launch_copy(hipStream_t s, const float4* src, float4* dst, long n)
+{
+    if (n <= 0) return;
+    long nb = (n + kBlock - 1) / kBlock;
+    if (nb > 256 * 64) nb = 256 * 64;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)nb), dim3(kBlock), 0, s, src, dst, n);
+}
+
 static inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
 
 void launch_predict(hipStream_t s, float4* pos, float4* prev, float4* vel, int n, float h, f3 gdv, float drag,
@@ -369,6 +609,39 @@ void launch_fem(hipStream_t s, float4* pos, TriStreams tr, int start, int count,
 {
     if (count <= 0) return;
     hipLaunchKernelGGL(k_fem, dim3(blocks(count)), dim3(kBlock), 0, s, pos, tr, start, count, h2, epsUV, ctl);
+}
+
+void launch_fem_slab(hipStream_t s, float4* pos, TriStreams tr, const ushort4* lidx, SlabSet ss, int numSlabs,
+                     int maxVerts, float h2, float epsUV, const SolverCtl* ctl, const SlabContacts& kc)
+{
+    if (numSlabs <= 0) return;
+    hipLaunchKernelGGL(k_fem_slab, dim3(numSlabs), dim3(kSlabBlock), maxVerts * sizeof(float4), s, pos, tr, lidx, ss,
+                       h2, epsUV, ctl, kc);
+}
+
+void launch_predict_detect(hipStream_t s, float4* pos, float4* prev, float4* vel, int n, float h, f3 gdv, float drag,
+                           SolverCtl* ctl, const CollSet& cs, ContactBuf cb, float thickness, float tolerance,
+                           int collFlags)
+{
+    int nb = blocks(n > 0 ? n : 1);
+    hipLaunchKernelGGL(k_predict_detect, dim3(nb), dim3(kBlock), 0, s, pos, prev, vel, n, h, gdv, drag, ctl, cs, cb,
+                       thickness, tolerance, collFlags);
+}
+
+void launch_contacts_velocity(hipStream_t s, float4* pos, const float4* prev, float4* vel, ContactBuf cb,
+                              float thickness, float mu, float invH)
+{
+    if (cb.nv <= 0) return;
+    hipLaunchKernelGGL(k_contacts_velocity, dim3(blocks(cb.nv)), dim3(kBlock), 0, s, pos, prev, vel, cb, thickness,
+                       mu, invH);
+}
+
+void launch_bend_slab(hipStream_t s, float4* pos, const ushort4* lidx, SlabSet ss, int numSlabs, int maxVerts,
+                      float stiff, SolverCtl* ctl)
+{
+    if (numSlabs <= 0) return;
+    hipLaunchKernelGGL(k_bend_slab, dim3(numSlabs), dim3(kSlabBlock), maxVerts * sizeof(float4), s, pos, lidx, ss,
+                       stiff, ctl);
 }
 
 void launch_bend(hipStream_t s, float4* pos, const int4* bidx, int start, int count, float stiff, SolverCtl* ctl)

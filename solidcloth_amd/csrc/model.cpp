@@ -149,42 +149,6 @@ void Mesh::compute_normals()
     }
 }
 
-// ----------------------------------------------------------- colouring ----
-
-// Greedy first-fit colouring in constraint index order: each constraint takes
-// the smallest colour not yet used by any constraint sharing one of its
-// vertices. Constraints of one colour are vertex-disjoint, so processing them
-// in parallel equals processing them one after another; the Gauss-Seidel
-// order is colour-major with ascending index inside a colour.
-Colouring colour_greedy(int numConstraints, int arity, const uint32_t* verts, int strideWords, int numVertices)
-{
-    const int kWords = 4;  // up to 256 colours
-    std::vector<uint64_t> used((size_t)numVertices * kWords, 0);
-    std::vector<int> col(numConstraints);
-    int numColours = 0;
-    for (int c = 0; c < numConstraints; c++) {
-        uint64_t forb[kWords] = {0, 0, 0, 0};
-        const uint32_t* v = verts + (size_t)c * strideWords;
-        for (int a = 0; a < arity; a++)
-            for (int w = 0; w < kWords; w++) forb[w] |= used[(size_t)v[a] * kWords + w];
-        int colour = -1;
-        for (int w = 0; w < kWords && colour < 0; w++)
-            if (~forb[w]) colour = w * 64 + __builtin_ctzll(~forb[w]);
-        if (colour < 0) throw std::runtime_error("constraint colouring needs more than 256 colours");
-        col[c] = colour;
-        for (int a = 0; a < arity; a++) used[(size_t)v[a] * kWords + colour / 64] |= 1ull << (colour % 64);
-        numColours = std::max(numColours, colour + 1);
-    }
-    Colouring out;
-    out.offsets.assign(numColours + 1, 0);
-    for (int c = 0; c < numConstraints; c++) out.offsets[col[c] + 1]++;
-    for (int k = 0; k < numColours; k++) out.offsets[k + 1] += out.offsets[k];
-    out.order.resize(numConstraints);
-    std::vector<int> fill(out.offsets.begin(), out.offsets.end() - 1);
-    for (int c = 0; c < numConstraints; c++) out.order[fill[col[c]]++] = c;
-    return out;
-}
-
 // ------------------------------------------------------------- timing ----
 
 hipEvent_t KernelTimer::get()
@@ -235,6 +199,9 @@ void GpuSolver::release()
     dev_free(d_pos); dev_free(d_prev); dev_free(d_vel);
     dev_free(d_tidx); dev_free(d_tA); dev_free(d_tB); dev_free(d_tC); dev_free(d_bidx);
     dev_free(d_cCount); dev_free(d_cP); dev_free(d_cN); dev_free(d_cSrc); dev_free(d_ctl);
+    for (DeviceSlabs* d : {&triSlabs, &bendSlabs}) {
+        dev_free(d->lidx); dev_free(d->info); dev_free(d->colOff); dev_free(d->verts);
+    }
     slotCap = 0;
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
@@ -309,6 +276,49 @@ void ClothModel::sync_to_host()
     mDeviceNewer = false;
 }
 
+// Slab structure of a schedule: local vertex indices per constraint position,
+// per-slab (first vertex, vertex count, first colour, colour count), colour
+// offsets and the slabs' global vertex lists.
+// With `owners`, bit 31 of a slab vertex id marks the vertex's first slab in
+// phase order; returns the number of vertices below `numVertices` that no
+// slab contains.
+static int upload_slabs(const Schedule& s, DeviceSlabs& d, bool owners, int numVertices)
+{
+    dev_free(d.lidx); dev_free(d.info); dev_free(d.colOff); dev_free(d.verts);
+    if (!s.slabbed || s.order.empty()) return numVertices;
+    const int n = (int)s.order.size();
+    std::vector<ushort4> li(n);
+    for (int k = 0; k < n; k++) {
+        const uint16_t* l = &s.localIdx[(size_t)k * s.arity];
+        li[k] = make_ushort4(l[0], l[1], l[2], s.arity > 3 ? l[3] : 0);
+    }
+    const int ns = (int)s.slabVertBegin.size() - 1;
+    std::vector<int4> info(ns);
+    for (int i = 0; i < ns; i++)
+        info[i] = make_int4(s.slabVertBegin[i], s.slabVertBegin[i + 1] - s.slabVertBegin[i], s.slabColBegin[i],
+                            s.slabColBegin[i + 1] - s.slabColBegin[i]);
+    std::vector<int> verts(s.slabVerts.begin(), s.slabVerts.end());
+    std::vector<char> seen(numVertices, 0);
+    int orphans = numVertices;
+    for (size_t i = 0; i < verts.size(); i++) {
+        const int v = verts[i];
+        if (v >= 0 && v < numVertices && !seen[v]) {
+            seen[v] = 1;
+            orphans--;
+            if (owners) verts[i] = (int)((unsigned)v | 0x80000000u);
+        }
+    }
+    dev_alloc(d.lidx, n);
+    dev_alloc(d.info, ns);
+    dev_alloc(d.colOff, s.colOffsets.size());
+    dev_alloc(d.verts, verts.size());
+    HIP_OK(hipMemcpy(d.lidx, li.data(), n * sizeof(ushort4), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(d.info, info.data(), ns * sizeof(int4), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(d.colOff, s.colOffsets.data(), s.colOffsets.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(d.verts, verts.data(), verts.size() * sizeof(int), hipMemcpyHostToDevice));
+    return orphans;
+}
+
 void ClothModel::build_device_topology()
 {
     GpuSolver& g = gpu;
@@ -352,6 +362,8 @@ void ClothModel::build_device_topology()
     }
     dev_alloc(g.d_bidx, g.nb);
     if (g.nb) HIP_OK(hipMemcpy(g.d_bidx, bi.data(), g.nb * sizeof(int4), hipMemcpyHostToDevice));
+    g.triOrphans = upload_slabs(triCol, g.triSlabs, true, g.nv);
+    upload_slabs(bendCol, g.bendSlabs, false, g.nv);
     dev_alloc(g.d_cCount, g.nv);
     if (g.nv) HIP_OK(hipMemset(g.d_cCount, 0, g.nv * sizeof(int)));
     g.slotCap = 0;
@@ -367,13 +379,36 @@ void ClothModel::ensure_colouring()
     const int nv = (int)pos.size(), nt = (int)tris.size(), nb = (int)bends.size();
     std::vector<uint32_t> tv(3 * (size_t)nt);
     for (int t = 0; t < nt; t++) { tv[3 * t] = tris[t].i1; tv[3 * t + 1] = tris[t].i2; tv[3 * t + 2] = tris[t].i3; }
-    triCol = colour_greedy(nt, 3, tv.data(), 3, nv);
     std::vector<uint32_t> bv(4 * (size_t)nb);
     for (int b = 0; b < nb; b++) {
         bv[4 * b] = bends[b].i1; bv[4 * b + 1] = bends[b].i2; bv[4 * b + 2] = bends[b].i3; bv[4 * b + 3] = bends[b].i4;
     }
-    bendCol = colour_greedy(nb, 4, bv.data(), 4, nv);
+    // LDS budget of a slab workgroup: 4000 float4 vertices (64,000 bytes)
+    const int maxVerts = 4000;
+    if (mSweepMode == SWEEP_COLOURS) {
+        triCol = schedule_colours(nt, 3, tv.data(), nv);
+        bendCol = schedule_colours(nb, 4, bv.data(), nv);
+    } else if (mSlabSize > 0) {
+        triCol = schedule_slabs(nt, 3, tv.data(), nv, pos.data(), mSlabSize, maxVerts);
+        bendCol = schedule_slabs(nb, 4, bv.data(), nv, pos.data(), mSlabSize, maxVerts);
+    } else {
+        triCol = schedule_slabs_auto(nt, 3, tv.data(), nv, pos.data(), maxVerts);
+        bendCol = schedule_slabs_auto(nb, 4, bv.data(), nv, pos.data(), maxVerts);
+    }
     mColourDirty = false;
+}
+
+void ClothModel::set_sweep(int mode, int slabSize)
+{
+    if (mode != SWEEP_COLOURS && mode != SWEEP_SLABS) throw std::runtime_error("unknown sweep mode");
+    if (slabSize < 0 || slabSize > 65536) throw std::runtime_error("slab size out of range");
+    if (mode == mSweepMode && slabSize == mSlabSize) return;
+    sync_to_host();
+    mSweepMode = mode;
+    mSlabSize = slabSize;
+    mColourDirty = true;
+    mTopoDirty = true;
+    mHostDirty = true;
 }
 
 void ClothModel::sync_to_device()
@@ -454,7 +489,7 @@ std::string ClothModel::graph_key(float h, const CollSet& cs) const
     put(&h, 4); put(&mUnit, 4); put(&mThickness, 4); put(&mTolerance, 4); put(&mShearStiff, 4);
     put(&mBendStiff, 4); put(&mPoisson, 4); put(&mFriction, 4); put(&mDragCoeff, 4);
     put(&mNumIterations, 4); put(&mCollFlags, 4); put(&mBendFreq, 4);
-    put(&mDihedral, 1); put(&mFEM, 1); put(&mFixedIterations, 1);
+    put(&mDihedral, 1); put(&mFEM, 1); put(&mFixedIterations, 1); put(&mSweepMode, 4); put(&mSlabSize, 4);
     put(&cs, sizeof(cs));
     put(&gpu.d_cP, sizeof(void*));
     return k;
@@ -488,41 +523,64 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
     const f3 gravity{0.f, -9.8f, 0.f};
     const f3 hg = mul(h, gravity);
     const f3 gdv = f3{mUnit * hg.x, mUnit * hg.y, mUnit * hg.z};
-    timed_begin(g.timer, timed, 4, s, ev);
-    launch_predict(s, g.d_pos, g.d_prev, g.d_vel, g.nv, h, gdv, mDragCoeff, g.d_ctl);
-    timed_end(g.timer, timed, 4, s, ev);
-
     ContactBuf cb{g.d_cCount, g.d_cP, g.d_cN, g.d_cSrc, g.nv, g.slots};
     const bool anyContacts = cs.maxSlots > 0;
+    const float invH = 1.f / h;
+    // With a fixed iteration count and every vertex in some triangle slab, the
+    // vertex contacts of iteration k are applied by the FEM slab kernels of
+    // iteration k+1 (by each vertex's owner slab, before its constraints) and
+    // the last iteration's by the velocity kernel: one launch per iteration
+    // fewer, same sequential order. The early-exit test needs the contact
+    // error at the end of each iteration, so that mode keeps separate launches.
+    const bool fuse = anyContacts && mFEM && mFixedIterations && triCol.slabbed && triCol.numPhases() > 0 &&
+                      g.triOrphans == 0;
     if (anyContacts) {
         timed_begin(g.timer, timed, 3, s, ev);
-        launch_detect(s, g.d_pos, g.d_prev, g.nv, cs, cb, mThickness, mTolerance, mCollFlags);
+        launch_predict_detect(s, g.d_pos, g.d_prev, g.d_vel, g.nv, h, gdv, mDragCoeff, g.d_ctl, cs, cb, mThickness,
+                              mTolerance, mCollFlags);
         timed_end(g.timer, timed, 3, s, ev);
+    } else {
+        timed_begin(g.timer, timed, 4, s, ev);
+        launch_predict(s, g.d_pos, g.d_prev, g.d_vel, g.nv, h, gdv, mDragCoeff, g.d_ctl);
+        timed_end(g.timer, timed, 4, s, ev);
     }
 
     const float h2 = h * h;
     const float epsUV = 2 * (1 + mPoisson) / (smax(mShearStiff, 0.1f) * mUnit * mUnit);
     const float sStiff = 0.0005f * mBendStiff;
-    const float invH = 1.f / h;
     TriStreams tr{g.d_tidx, g.d_tA, g.d_tB, g.d_tC};
     for (int k = 0; k < mNumIterations; k++) {
         if (mFEM) {
-            for (size_t c = 0; c + 1 < triCol.offsets.size(); c++) {
-                int st = triCol.offsets[c], cnt = triCol.offsets[c + 1] - st;
+            for (int p = 0; p < triCol.numPhases(); p++) {
                 timed_begin(g.timer, timed, 0, s, ev);
-                launch_fem(s, g.d_pos, tr, st, cnt, h2, epsUV, g.d_ctl);
+                if (triCol.slabbed) {
+                    const int b = triCol.phaseSlabBegin[p], ns = triCol.phaseSlabBegin[p + 1] - b;
+                    SlabSet ss{g.triSlabs.info, g.triSlabs.colOff, g.triSlabs.verts, b};
+                    SlabContacts kc{cb, g.d_prev, g.d_vel, mThickness, mFriction, invH, (fuse && k > 0) ? 1 : 0};
+                    launch_fem_slab(s, g.d_pos, tr, g.triSlabs.lidx, ss, ns, triCol.maxSlabVerts, h2, epsUV, g.d_ctl,
+                                    kc);
+                } else {
+                    int st = triCol.phaseOffsets[p], cnt = triCol.phaseOffsets[p + 1] - st;
+                    launch_fem(s, g.d_pos, tr, st, cnt, h2, epsUV, g.d_ctl);
+                }
                 timed_end(g.timer, timed, 0, s, ev);
             }
         }
         if (mDihedral && k % mBendFreq == 0) {
-            for (size_t c = 0; c + 1 < bendCol.offsets.size(); c++) {
-                int st = bendCol.offsets[c], cnt = bendCol.offsets[c + 1] - st;
+            for (int p = 0; p < bendCol.numPhases(); p++) {
                 timed_begin(g.timer, timed, 1, s, ev);
-                launch_bend(s, g.d_pos, g.d_bidx, st, cnt, sStiff, g.d_ctl);
+                if (bendCol.slabbed) {
+                    const int b = bendCol.phaseSlabBegin[p], ns = bendCol.phaseSlabBegin[p + 1] - b;
+                    SlabSet ss{g.bendSlabs.info, g.bendSlabs.colOff, g.bendSlabs.verts, b};
+                    launch_bend_slab(s, g.d_pos, g.bendSlabs.lidx, ss, ns, bendCol.maxSlabVerts, sStiff, g.d_ctl);
+                } else {
+                    int st = bendCol.phaseOffsets[p], cnt = bendCol.phaseOffsets[p + 1] - st;
+                    launch_bend(s, g.d_pos, g.d_bidx, st, cnt, sStiff, g.d_ctl);
+                }
                 timed_end(g.timer, timed, 1, s, ev);
             }
         }
-        if (anyContacts) {
+        if (anyContacts && !fuse) {
             timed_begin(g.timer, timed, 2, s, ev);
             launch_contacts(s, g.d_pos, g.d_prev, g.d_vel, cb, mThickness, mFriction, invH, g.d_ctl);
             timed_end(g.timer, timed, 2, s, ev);
@@ -531,7 +589,10 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
             launch_iter_end(s, g.d_ctl, mNumIterations, 0, mBendFreq, mDihedral ? 1 : 0, mSelfCollThreshold);
     }
     timed_begin(g.timer, timed, 4, s, ev);
-    launch_velocity(s, g.d_pos, g.d_prev, g.d_vel, g.nv, invH);
+    if (fuse)
+        launch_contacts_velocity(s, g.d_pos, g.d_prev, g.d_vel, cb, mThickness, mFriction, invH);
+    else
+        launch_velocity(s, g.d_pos, g.d_prev, g.d_vel, g.nv, invH);
     timed_end(g.timer, timed, 4, s, ev);
 }
 

@@ -12,6 +12,7 @@
 
 #include "../../include/solidcloth.h"
 #include "kernels.h"
+#include "schedule.h"
 
 namespace sc {
 
@@ -103,13 +104,16 @@ struct ContactOut {
 
 class ClothPatch;
 
-// Colour classes of a constraint set (greedy first-fit in index order).
-struct Colouring {
-    std::vector<int> order;    // constraint indices, colour-major, ascending within a colour
-    std::vector<int> offsets;  // numColours + 1
-};
+// Sweep schedules: 0 = one launch per colour, 1 = slab phases with LDS staging
+enum { SWEEP_COLOURS = 0, SWEEP_SLABS = 1 };
 
-Colouring colour_greedy(int numConstraints, int arity, const uint32_t* verts, int strideWords, int numVertices);
+// Device copy of one constraint schedule's slab structure.
+struct DeviceSlabs {
+    ushort4* lidx = nullptr;
+    int4* info = nullptr;
+    int* colOff = nullptr;
+    int* verts = nullptr;
+};
 
 struct KernelTimer {
     bool enabled = false;
@@ -137,6 +141,8 @@ public:
     int* d_cCount = nullptr;
     float4 *d_cP = nullptr, *d_cN = nullptr;
     int* d_cSrc = nullptr;
+    DeviceSlabs triSlabs, bendSlabs;
+    int triOrphans = 0;  // vertices in no triangle slab
     int slots = 0, slotCap = 0;
     SolverCtl* d_ctl = nullptr;
     hipGraph_t graph = nullptr;
@@ -194,8 +200,11 @@ public:
     void download_contacts();
     void fetch_stats();
     void ensure_colouring();
+    void set_sweep(int mode, int slabSize);
 
-    Colouring triCol, bendCol;
+    Schedule triCol, bendCol;
+    int mSweepMode = SWEEP_SLABS;
+    int mSlabSize = 0;  // 0: chosen per constraint type by schedule_slabs_auto
 
     GpuSolver gpu;
     ClothPatch* mOwner;

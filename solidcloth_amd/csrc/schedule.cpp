@@ -1,0 +1,240 @@
+// Constraint schedules (see schedule.h).
+#include "schedule.h"
+
+#include <algorithm>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+
+namespace sc {
+
+namespace {
+
+// Greedy first-fit colouring of `ids` (in the given order); colours limited
+// to 64 * kWords. `used` is indexed by vertex and must be zero on entry for
+// every vertex the constraints touch; it is cleared again on exit.
+struct Greedy {
+    static const int kWords = 4;
+    std::vector<uint64_t> used;
+    explicit Greedy(int nv) : used((size_t)nv * kWords, 0) {}
+    int colour(const std::vector<int>& ids, int arity, const uint32_t* verts, std::vector<int>& col)
+    {
+        int nc = 0;
+        for (size_t k = 0; k < ids.size(); k++) {
+            const uint32_t* v = verts + (size_t)ids[k] * arity;
+            uint64_t forb[kWords] = {0, 0, 0, 0};
+            for (int a = 0; a < arity; a++)
+                for (int w = 0; w < kWords; w++) forb[w] |= used[(size_t)v[a] * kWords + w];
+            int c = -1;
+            for (int w = 0; w < kWords && c < 0; w++)
+                if (~forb[w]) c = w * 64 + __builtin_ctzll(~forb[w]);
+            if (c < 0) throw std::runtime_error("constraint colouring needs more than 256 colours");
+            col[k] = c;
+            for (int a = 0; a < arity; a++) used[(size_t)v[a] * kWords + c / 64] |= 1ull << (c % 64);
+            nc = std::max(nc, c + 1);
+        }
+        for (size_t k = 0; k < ids.size(); k++) {
+            const uint32_t* v = verts + (size_t)ids[k] * arity;
+            for (int a = 0; a < arity; a++)
+                for (int w = 0; w < kWords; w++) used[(size_t)v[a] * kWords + w] = 0;
+        }
+        return nc;
+    }
+};
+
+}  // namespace
+
+Schedule schedule_colours(int n, int arity, const uint32_t* verts, int nv)
+{
+    Schedule s;
+    s.arity = arity;
+    s.slabbed = false;
+    Greedy g(nv);
+    std::vector<int> ids(n), col(n);
+    std::iota(ids.begin(), ids.end(), 0);
+    int nc = n ? g.colour(ids, arity, verts, col) : 0;
+    std::vector<int> cnt(nc + 1, 0);
+    for (int c : col) cnt[c + 1]++;
+    for (int k = 0; k < nc; k++) cnt[k + 1] += cnt[k];
+    s.phaseOffsets = cnt;
+    if (s.phaseOffsets.empty()) s.phaseOffsets.push_back(0);
+    s.order.resize(n);
+    s.phaseOf.resize(n);
+    s.slabOf.assign(n, -1);
+    s.colourOf.resize(n);
+    std::vector<int> fill(cnt.begin(), cnt.end() - (nc ? 1 : 0));
+    for (int k = 0; k < n; k++) {
+        int p = fill[col[k]]++;
+        s.order[p] = k;
+        s.phaseOf[p] = col[k];
+        s.colourOf[p] = 0;
+    }
+    return s;
+}
+
+Schedule schedule_slabs(int n, int arity, const uint32_t* verts, int nv, const f3* rest, int slabSize, int maxVerts)
+{
+    Schedule s;
+    s.arity = arity;
+    s.slabbed = true;
+    s.slabSize = slabSize;
+    if (n == 0) {
+        s.phaseOffsets = {0};
+        s.phaseSlabBegin = {0};
+        s.slabVertBegin = {0};
+        s.slabColBegin = {0};
+        s.colOffsets = {0};
+        return s;
+    }
+    // longest axis of the rest shape
+    f3 lo = rest[0], hi = rest[0];
+    for (int i = 1; i < nv; i++) {
+        lo = f3{std::min(lo.x, rest[i].x), std::min(lo.y, rest[i].y), std::min(lo.z, rest[i].z)};
+        hi = f3{std::max(hi.x, rest[i].x), std::max(hi.y, rest[i].y), std::max(hi.z, rest[i].z)};
+    }
+    const float ext[3] = {hi.x - lo.x, hi.y - lo.y, hi.z - lo.z};
+    const int axis = (ext[1] > ext[0] && ext[1] >= ext[2]) ? 1 : (ext[2] > ext[0] && ext[2] > ext[1]) ? 2 : 0;
+    auto coord = [&](uint32_t v) { return axis == 0 ? rest[v].x : axis == 1 ? rest[v].y : rest[v].z; };
+    // sort by centroid along the axis (sum of coordinates; ties by index)
+    std::vector<float> key(n);
+    for (int c = 0; c < n; c++) {
+        float k = 0.f;
+        for (int a = 0; a < arity; a++) k += coord(verts[(size_t)c * arity + a]);
+        key[c] = k;
+    }
+    std::vector<int> byKey(n);
+    std::iota(byKey.begin(), byKey.end(), 0);
+    std::stable_sort(byKey.begin(), byKey.end(), [&](int a, int b) { return key[a] < key[b]; });
+    // cut into slabs
+    const int numSlabs = (n + slabSize - 1) / slabSize;
+    std::vector<std::vector<int>> slabs(numSlabs);
+    for (int k = 0; k < n; k++) slabs[k / slabSize].push_back(byKey[k]);
+    for (auto& sl : slabs) std::sort(sl.begin(), sl.end());
+    // slab conflict graph through shared vertices, greedy-coloured in slab order
+    std::vector<std::vector<int>> vslabs(nv);
+    for (int sI = 0; sI < numSlabs; sI++)
+        for (int c : slabs[sI])
+            for (int a = 0; a < arity; a++) {
+                auto& l = vslabs[verts[(size_t)c * arity + a]];
+                if (l.empty() || l.back() != sI) l.push_back(sI);
+            }
+    std::vector<std::vector<int>> nbr(numSlabs);
+    for (auto& l : vslabs)
+        for (size_t i = 0; i < l.size(); i++)
+            for (size_t j = 0; j < l.size(); j++)
+                if (i != j) nbr[l[i]].push_back(l[j]);
+    std::vector<int> phase(numSlabs, -1);
+    int numPhases = 0;
+    for (int sI = 0; sI < numSlabs; sI++) {
+        std::vector<char> taken(nbr[sI].size() + 1, 0);
+        for (int o : nbr[sI])
+            if (phase[o] >= 0 && phase[o] < (int)taken.size()) taken[phase[o]] = 1;
+        int p = 0;
+        while (taken[p]) p++;
+        phase[sI] = p;
+        numPhases = std::max(numPhases, p + 1);
+    }
+    // renumber slabs phase-major (stable in slab order)
+    std::vector<int> slabOrder(numSlabs);
+    std::iota(slabOrder.begin(), slabOrder.end(), 0);
+    std::stable_sort(slabOrder.begin(), slabOrder.end(), [&](int a, int b) { return phase[a] < phase[b]; });
+    s.phaseSlabBegin.assign(numPhases + 1, 0);
+    for (int sI = 0; sI < numSlabs; sI++) s.phaseSlabBegin[phase[sI] + 1]++;
+    for (int p = 0; p < numPhases; p++) s.phaseSlabBegin[p + 1] += s.phaseSlabBegin[p];
+
+    Greedy g(nv);
+    std::vector<int> localOf(nv, -1);
+    s.order.reserve(n);
+    s.localIdx.reserve((size_t)n * arity);
+    s.slabVertBegin.push_back(0);
+    s.slabColBegin.push_back(0);
+    s.phaseOffsets.assign(1, 0);
+    int curPhase = 0;
+    for (int r = 0; r < numSlabs; r++) {
+        const int sI = slabOrder[r];
+        while (phase[sI] > curPhase) {
+            s.phaseOffsets.push_back((int)s.order.size());
+            curPhase++;
+        }
+        const std::vector<int>& ids = slabs[sI];
+        // vertices of the slab, ascending
+        std::vector<uint32_t> vs;
+        for (int c : ids)
+            for (int a = 0; a < arity; a++) vs.push_back(verts[(size_t)c * arity + a]);
+        std::sort(vs.begin(), vs.end());
+        vs.erase(std::unique(vs.begin(), vs.end()), vs.end());
+        if ((int)vs.size() > maxVerts)
+            throw std::runtime_error("slab has " + std::to_string(vs.size()) + " vertices, more than " +
+                                     std::to_string(maxVerts));
+        for (size_t i = 0; i < vs.size(); i++) localOf[vs[i]] = (int)i;
+        s.maxSlabVerts = std::max(s.maxSlabVerts, (int)vs.size());
+        // local colours
+        std::vector<int> col(ids.size());
+        int nc = g.colour(ids, arity, verts, col);
+        if (nc > 63) throw std::runtime_error("a slab needs more than 63 colours");
+        s.maxSlabColours = std::max(s.maxSlabColours, nc);
+        std::vector<int> cnt(nc + 1, 0);
+        for (int c : col) cnt[c + 1]++;
+        for (int k = 0; k < nc; k++) cnt[k + 1] += cnt[k];
+        const int base = (int)s.order.size();
+        std::vector<int> pos(ids.size());
+        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+        for (size_t k = 0; k < ids.size(); k++) pos[fill[col[k]]++] = (int)k;
+        for (int k = 0; k < nc; k++) s.colOffsets.push_back(base + cnt[k]);
+        for (size_t q = 0; q < ids.size(); q++) {
+            const int k = pos[q];
+            const int c = ids[k];
+            s.order.push_back(c);
+            s.phaseOf.push_back(phase[sI]);
+            s.slabOf.push_back(r);
+            s.colourOf.push_back(col[k]);
+            for (int a = 0; a < arity; a++) s.localIdx.push_back((uint16_t)localOf[verts[(size_t)c * arity + a]]);
+        }
+        for (uint32_t v : vs) { s.slabVerts.push_back(v); localOf[v] = -1; }
+        s.slabVertBegin.push_back((int)s.slabVerts.size());
+        s.slabColBegin.push_back((int)s.colOffsets.size());
+    }
+    s.colOffsets.push_back((int)s.order.size());
+    s.phaseOffsets.push_back((int)s.order.size());
+    return s;
+}
+
+Schedule schedule_slabs_auto(int n, int arity, const uint32_t* verts, int nv, const f3* rest, int maxVerts)
+{
+    // largest first: at equal cost fewer, fuller workgroups gather and
+    // scatter fewer shared vertices
+    const int sizes[] = {8192, 4096, 2048, 1024, 512, 256};
+    Schedule best;
+    long bestCost = -1;
+    int bestPhases = 1 << 30;
+    for (int sz : sizes) {
+        Schedule s;
+        try {
+            s = schedule_slabs(n, arity, verts, nv, rest, sz, maxVerts);
+        } catch (const std::runtime_error&) {
+            continue;
+        }
+        // critical path of one pass: per phase, the slowest slab's number of
+        // barrier-separated rounds of 256 constraints
+        long cost = 0;
+        for (int p = 0; p < s.numPhases(); p++) {
+            long worst = 0;
+            for (int sl = s.phaseSlabBegin[p]; sl < s.phaseSlabBegin[p + 1]; sl++) {
+                long rounds = 0;
+                for (int k = s.slabColBegin[sl]; k < s.slabColBegin[sl + 1]; k++)
+                    rounds += (s.colOffsets[k + 1] - s.colOffsets[k] + 255) / 256;
+                worst = std::max(worst, rounds);
+            }
+            cost += worst + 8;  // a phase launch costs about as much as a few rounds
+        }
+        if (bestCost < 0 || s.numPhases() < bestPhases || (s.numPhases() == bestPhases && cost < bestCost)) {
+            best = std::move(s);
+            bestCost = cost;
+            bestPhases = best.numPhases();
+        }
+    }
+    if (bestCost < 0) throw std::runtime_error("no slab size fits the LDS budget");
+    return best;
+}
+
+}  // namespace sc

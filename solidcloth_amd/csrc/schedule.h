@@ -1,0 +1,60 @@
+// Gauss-Seidel schedules for a set of constraints (FEM triangles or bends).
+//
+// Two schedules are built, both deterministic and both equal in effect to
+// processing the constraints one after another in the exported order:
+//
+//  * colour schedule: greedy first-fit colouring in index order; one launch
+//    per colour, one constraint per thread, positions in HBM.
+//
+//  * slab schedule: constraints sorted along the longest axis of the rest
+//    shape and cut into slabs of about `slabSize` constraints; slabs sharing a
+//    vertex are coloured apart (a chain of slabs needs two phases); inside a
+//    slab the constraints are greedily coloured again. One launch per phase,
+//    one workgroup per slab: the workgroup stages the slab's vertices in LDS,
+//    sweeps its local colours with workgroup barriers and writes the vertices
+//    back. Slabs of one phase touch disjoint vertices.
+//
+// The exported order is phase-major, then slab, then local colour, then
+// ascending constraint index.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "scmath.h"
+
+namespace sc {
+
+struct Schedule {
+    int arity = 0;
+    bool slabbed = false;
+    std::vector<int> order;        // constraint indices in Gauss-Seidel order
+    std::vector<int> phaseOffsets;  // positions in `order` where each phase starts (+ end)
+    // per order position: phase, slab (global slab id, -1 in colour mode), local colour
+    std::vector<int> phaseOf, slabOf, colourOf;
+    // slab mode
+    std::vector<int> phaseSlabBegin;   // slabs of phase p: [phaseSlabBegin[p], phaseSlabBegin[p+1])
+    std::vector<int> slabVertBegin;    // into slabVerts (+ end)
+    std::vector<int> slabColBegin;     // into colOffsets (+ end): colours of slab s
+    std::vector<int> colOffsets;       // positions in `order`; colour k of slab s spans [colOffsets[i], colOffsets[i+1])
+    std::vector<uint32_t> slabVerts;   // global vertex ids per slab, ascending
+    std::vector<uint16_t> localIdx;    // per order position, `arity` local vertex indices
+    int maxSlabVerts = 0;
+    int maxSlabColours = 0;
+    int slabSize = 0;
+    int numPhases() const { return (int)phaseOffsets.size() - 1; }
+};
+
+Schedule schedule_colours(int numConstraints, int arity, const uint32_t* verts, int numVertices);
+
+// Returns a slab schedule with at most `maxVerts` vertices per slab; throws if
+// the constraints cannot be split that way.
+Schedule schedule_slabs(int numConstraints, int arity, const uint32_t* verts, int numVertices, const f3* rest,
+                        int slabSize, int maxVerts);
+
+// Largest slab size from {4096, 2048, 1024, 512, 256} whose schedule needs the
+// fewest phases (two for a strip-like cloth) and fits maxVerts.
+Schedule schedule_slabs_auto(int numConstraints, int arity, const uint32_t* verts, int numVertices,
+                             const f3* rest, int maxVerts);
+
+}  // namespace sc

@@ -1,11 +1,13 @@
 """GPU parity: the HIP substep through the C ABI against the CPU oracle run
-with its constraints permuted into the GPU's colour order.
+with its constraints permuted into the GPU's Gauss-Seidel order.
 
 Bar: per-vertex positions equal to the oracle's. The kernels and the oracle
-evaluate the same FP32 operation sequence without contraction, and the
-constraints of one colour are vertex-disjoint, so equality is expected to be
-bit-exact; the stated tolerance (1e-5 relative to the cloth extent) is the
-acceptance bar.
+evaluate the same FP32 operation sequence without contraction, and every
+group of constraints that runs concurrently touches disjoint vertices, so
+equality is expected to be bit-exact; the stated tolerance (1e-5 relative to
+the cloth extent) is the acceptance bar and bit-exactness is asserted on top.
+Both sweep schedules are covered: one launch per colour (mode 0) and slab
+phases staged in LDS (mode 1, the default).
 """
 import numpy as np
 import pytest
@@ -14,11 +16,13 @@ from solidcloth_amd import scenes as S
 
 pytestmark = pytest.mark.gpu
 REL_TOL = 1e-5
+MODES = [0, 1]
 
 
-def _pair(sc, oracle_mod, scene):
+def _pair(sc, oracle_mod, scene, mode=1):
     world, patch = S.build_patch(scene, sc)
     m = patch.GetModel()
+    m.SetSweep(mode)
     o = oracle_mod.from_scene(scene, tri_order=m.constraint_order("tri"), bend_order=m.constraint_order("bend"))
     return world, patch, m, o
 
@@ -31,24 +35,24 @@ def _compare(gpu, ora, what):
     return err
 
 
-def test_drape32_100_substeps_bit_exact(sc, oracle_mod):
+@pytest.mark.parametrize("mode", MODES)
+def test_drape32_100_substeps_bit_exact(sc, oracle_mod, mode):
     scene = S.DRAPE32
-    world, patch, m, o = _pair(sc, oracle_mod, scene)
-    frames = 25  # 25 frames x 4 substeps = 100 substeps
-    for f in range(frames):
+    world, patch, m, o = _pair(sc, oracle_mod, scene, mode)
+    for f in range(25):  # 25 frames x 4 substeps = 100 substeps
         patch.Step(scene.dt)
         o.step(scene.dt)
     gp = m.positions()
     op = o.get("pos")
     err = _compare(gp, op, "drape32 positions after 100 substeps")
     assert err == 0.0, f"expected bit-exact parity, max abs diff {err:.3e}"
-    gv = m.velocities()
-    assert np.array_equal(gv.astype(np.float64), o.get("vel"))
+    assert np.array_equal(m.velocities().astype(np.float64), o.get("vel"))
 
 
-def test_drape32_reference_early_exit(sc, oracle_mod):
+@pytest.mark.parametrize("mode", MODES)
+def test_drape32_reference_early_exit(sc, oracle_mod, mode):
     scene = S.DRAPE32.with_(fixed_iterations=False)
-    world, patch, m, o = _pair(sc, oracle_mod, scene)
+    world, patch, m, o = _pair(sc, oracle_mod, scene, mode)
     for f in range(10):
         patch.Step(scene.dt)
         o.step(scene.dt)
@@ -56,11 +60,12 @@ def test_drape32_reference_early_exit(sc, oracle_mod):
     assert np.array_equal(m.positions().astype(np.float64), o.get("pos"))
 
 
-def test_drape_with_sdf_and_ground_reduced(sc, oracle_mod):
+@pytest.mark.parametrize("mode", MODES)
+def test_drape_with_sdf_and_ground_reduced(sc, oracle_mod, mode):
     # the drape256 workload on a 48x48 grid, long enough to land on the
     # sphere and the ground
     scene = S.DRAPE256.with_(n=48)
-    world, patch, m, o = _pair(sc, oracle_mod, scene)
+    world, patch, m, o = _pair(sc, oracle_mod, scene, mode)
     for f in range(40):
         patch.Step(scene.dt)
         o.step(scene.dt)
@@ -75,15 +80,45 @@ def test_drape_with_sdf_and_ground_reduced(sc, oracle_mod):
     assert g == r
 
 
-def test_drape256_full_size_parity(sc, oracle_mod):
+@pytest.mark.parametrize("mode", MODES)
+def test_drape_with_contacts_reference_early_exit(sc, oracle_mod, mode):
+    """Early-exit mode keeps one contact launch per iteration (the exit test
+    reads that iteration's contact error); the iteration counts and positions
+    must follow the oracle while the cloth lies on the sphere and ground."""
+    scene = S.DRAPE256.with_(n=40, fixed_iterations=False, iterations=12)
+    world, patch, m, o = _pair(sc, oracle_mod, scene, mode)
+    for f in range(45):
+        patch.Step(scene.dt)
+        o.step(scene.dt)
+        assert m.stats()["iterations"] == o.stats()["iterations"], f
+    assert np.array_equal(m.positions().astype(np.float64), o.get("pos"))
+    assert len(o.contacts()[0]) > 0
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_drape256_full_size_parity(sc, oracle_mod, mode):
     scene = S.DRAPE256
-    world, patch, m, o = _pair(sc, oracle_mod, scene)
+    world, patch, m, o = _pair(sc, oracle_mod, scene, mode)
     for f in range(3):
         patch.Step(scene.dt)
         o.step(scene.dt)
     gp, op = m.positions(), o.get("pos")
     _compare(gp, op, "drape256 after 12 substeps")
     assert np.array_equal(gp.astype(np.float64), op)
+
+
+def test_drape256_slab_schedule_long_run(sc, oracle_mod):
+    """25 frames (100 substeps) of the full-size workload: the cloth reaches
+    the sphere, so contacts and friction are active in the compared run."""
+    scene = S.DRAPE256
+    world, patch, m, o = _pair(sc, oracle_mod, scene, 1)
+    for f in range(25):
+        patch.Step(scene.dt)
+        o.step(scene.dt)
+    gp, op = m.positions(), o.get("pos")
+    _compare(gp, op, "drape256 after 100 substeps")
+    assert np.array_equal(gp.astype(np.float64), op)
+    assert len(o.contacts()[0]) > 0
 
 
 def test_graph_and_direct_launch_identical(sc):
@@ -96,4 +131,15 @@ def test_graph_and_direct_launch_identical(sc):
         for f in range(20):
             patch.Step(scene.dt)
         out.append(m.positions())
+    assert np.array_equal(out[0], out[1])
+
+
+def test_run_twice_deterministic(sc):
+    scene = S.DRAPE256.with_(n=64)
+    out = []
+    for _ in range(2):
+        world, patch = S.build_patch(scene, sc)
+        for f in range(10):
+            patch.Step(scene.dt)
+        out.append(patch.GetModel().positions())
     assert np.array_equal(out[0], out[1])

@@ -78,32 +78,50 @@ def test_reference_counts(sc):
         assert m.GetNumQuads() == (scene.n - 1) ** 2
 
 
-def validate_colouring(order, offsets, cons, nv):
-    """Every constraint exactly once; constraints of one colour vertex-disjoint;
-    ascending index inside a colour."""
+def validate_schedule(order, offsets, sched, cons):
+    """The race-freedom conditions of a parallel Gauss-Seidel schedule:
+    every constraint exactly once; the order is sorted by (phase, slab,
+    colour, index); slabs that run in the same phase touch disjoint vertex
+    sets; constraints that run concurrently inside a slab (one colour) touch
+    disjoint vertices. In colour mode (slab -1) a phase is one colour."""
+    phase, slab, colour = sched
     assert sorted(order.tolist()) == list(range(len(cons)))
-    for c in range(len(offsets) - 1):
-        ids = order[offsets[c]:offsets[c + 1]]
-        assert np.all(np.diff(ids) > 0)
-        verts = cons[ids].ravel()
-        assert len(np.unique(verts)) == len(verts), f"colour {c} shares a vertex"
+    keys = list(zip(phase.tolist(), slab.tolist(), colour.tolist(), order.tolist()))
+    assert keys == sorted(keys)
+    assert np.array_equal(np.searchsorted(phase, np.arange(len(offsets) - 1)), offsets[:-1])
+    for p in range(len(offsets) - 1):
+        sel = phase == p
+        slabs = np.unique(slab[sel])
+        owner = {}
+        for s in slabs:
+            in_slab = sel & (slab == s)
+            for v in np.unique(cons[order[in_slab]]):
+                assert owner.setdefault(int(v), int(s)) == s, f"phase {p}: slabs {owner[int(v)]} and {s} share {v}"
+            for c in np.unique(colour[in_slab]):
+                verts = cons[order[in_slab & (colour == c)]].ravel()
+                assert len(np.unique(verts)) == len(verts), f"slab {s} colour {c} shares a vertex"
 
 
 @pytest.mark.parametrize("n", [32, 97, 256])
-def test_colouring_is_valid_partition(sc, n):
+@pytest.mark.parametrize("mode", [0, 1])
+def test_schedule_is_race_free(sc, n, mode):
     scene = S.DRAPE32.with_(n=n)
     world, patch = S.build_patch(scene, sc)
     m = patch.GetModel()
-    tris = m.triangles()
-    bends = m.bends()
-    to, toff = m.constraint_order("tri"), m.colour_offsets("tri")
-    bo, boff = m.constraint_order("bend"), m.colour_offsets("bend")
-    validate_colouring(to, toff, tris, m.GetNumParticles())
-    validate_colouring(bo, boff, bends, m.GetNumParticles())
-    # a grid vertex of valence 8 makes 8 triangle colours the minimum; the
-    # greedy colouring reaches it. Bends need at least 16.
-    assert len(toff) - 1 == 8
-    assert 16 <= len(boff) - 1 <= 20
+    m.SetSweep(mode)
+    for kind, cons in (("tri", m.triangles()), ("bend", m.bends())):
+        validate_schedule(m.constraint_order(kind), m.colour_offsets(kind), m.schedule(kind), cons)
+    nt = len(m.colour_offsets("tri")) - 1
+    nb = len(m.colour_offsets("bend")) - 1
+    if mode == 0:
+        # a grid vertex of valence 8 makes 8 triangle colours the minimum; the
+        # greedy colouring reaches it. Bends need at least 16.
+        assert nt == 8 and 16 <= nb <= 20
+    else:
+        # a strip of slabs along the cloth needs two phases (one if it fits one slab)
+        assert nt <= 2 and nb <= 2
+        ph, sl, co = m.schedule("bend")
+        assert co.max() + 1 <= 20
 
 
 def test_colouring_deterministic(sc):
