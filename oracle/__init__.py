@@ -65,6 +65,21 @@ def lib():
         L.oc_construct_edges.argtypes = [vp, _C.c_int, vp, _C.c_int]
         L.oc_construct_edges.restype = _C.c_int
         L.oc_init_arrays.argtypes = [vp, _C.c_int, vp, vp, vp, _C.c_int, vp]
+        L.oc_add_mesh.argtypes = [vp, _C.c_int, vp, _C.c_int, vp]
+        L.oc_add_mesh.restype = _C.c_int
+        L.oc_set_contact_order.argtypes = [vp, _C.c_int, vp, _C.c_int]
+        L.oc_get_edge_contacts.argtypes = [vp, vp, vp, _C.c_int]
+        L.oc_get_edge_contacts.restype = _C.c_int
+        L.oc_get_tri_contacts.argtypes = [vp, vp, vp, _C.c_int]
+        L.oc_get_tri_contacts.restype = _C.c_int
+        L.oc_get_contact_features.argtypes = [vp, vp]
+        L.oc_closest_pt_point_triangle.argtypes = [vp, vp, vp, vp, vp]
+        L.oc_closest_pt_point_triangle.restype = _C.c_int
+        L.oc_closest_pt_segm_segm.argtypes = [vp, vp, vp, vp, vp]
+        L.oc_closest_pt_segm_segm.restype = _C.c_int
+        L.oc_signed_volume.argtypes = [vp, vp, vp, vp]
+        L.oc_signed_volume.restype = _C.c_float
+        L.oc_mesh_normals.argtypes = [_C.c_int, vp, _C.c_int, vp, vp]
         _lib = L
     return _lib
 
@@ -124,6 +139,37 @@ class Cloth:
         t = _np.ascontiguousarray(tris, dtype=_np.int32)
         self.L.oc_init_arrays(self.h, len(p), _ptr(p), _ptr(u), _ptr(m), len(t), _ptr(t))
         return self
+
+    def add_mesh(self, vertices, indices):
+        v = _np.ascontiguousarray(vertices, dtype=_np.float32)
+        i = _np.ascontiguousarray(indices, dtype=_np.uint32).ravel()
+        return self.L.oc_add_mesh(self.h, len(v), _ptr(v), i.size, _ptr(i))
+
+    def set_contact_order(self, kind, order):
+        """kind 'tri' / 'edge': processing order of triangle / edge contacts
+        given as cloth triangle / edge indices."""
+        o = _np.ascontiguousarray(order, dtype=_np.int32)
+        self.L.oc_set_contact_order(self.h, 2 if kind == "tri" else 3, _ptr(o), len(o))
+
+    def edge_contacts(self):
+        n = self.L.oc_get_edge_contacts(self.h, None, None, 0)
+        ids = _np.zeros((n, 4), dtype=_np.int32)
+        data = _np.zeros((n, 10))
+        self.L.oc_get_edge_contacts(self.h, _ptr(ids), _ptr(data), n)
+        return ids, data
+
+    def tri_contacts(self):
+        n = self.L.oc_get_tri_contacts(self.h, None, None, 0)
+        ids = _np.zeros((n, 2), dtype=_np.int32)
+        data = _np.zeros((n, 11))
+        self.L.oc_get_tri_contacts(self.h, _ptr(ids), _ptr(data), n)
+        return ids, data
+
+    def contact_features(self):
+        n = self.counts()["C"]
+        out = _np.zeros((n, 2), dtype=_np.int32)
+        self.L.oc_get_contact_features(self.h, _ptr(out))
+        return out
 
     def add_walls(self, bmin, bmax):
         box = _np.array(list(bmin) + list(bmax), dtype=_np.float32)
@@ -218,6 +264,39 @@ class Cloth:
 
 def acos(x):
     return lib().oc_acos(float(x))
+
+
+def _f3(a):
+    return _np.ascontiguousarray(a, dtype=_np.float32)
+
+
+def closest_pt_point_triangle(p, a, b, c):
+    """(region, closest point, barycentric) in FP32."""
+    out = _np.zeros(6, dtype=_np.float32)
+    args = [_f3(x) for x in (p, a, b, c)]
+    r = lib().oc_closest_pt_point_triangle(*[_ptr(x) for x in args], _ptr(out))
+    return r, out[:3], out[3:]
+
+
+def closest_pt_segm_segm(p1, q1, p2, q2):
+    """(region, s, t, c1, c2) in FP32."""
+    out = _np.zeros(8, dtype=_np.float32)
+    args = [_f3(x) for x in (p1, q1, p2, q2)]
+    r = lib().oc_closest_pt_segm_segm(*[_ptr(x) for x in args], _ptr(out))
+    return r, out[0], out[1], out[2:5], out[5:8]
+
+
+def signed_volume(a0, a1, a2, a3):
+    args = [_f3(x) for x in (a0, a1, a2, a3)]
+    return lib().oc_signed_volume(*[_ptr(x) for x in args])
+
+
+def mesh_normals(vertices, indices):
+    v = _f3(vertices)
+    i = _np.ascontiguousarray(indices, dtype=_np.uint32).ravel()
+    out = _np.zeros((len(v), 3), dtype=_np.float32)
+    lib().oc_mesh_normals(len(v), _ptr(v), i.size, _ptr(i), _ptr(out))
+    return out
 
 
 def from_scene(scene, real64=False, tri_order=None, bend_order=None):

@@ -19,6 +19,7 @@
 //     colour order) and a fixed-iteration switch that removes the early exit
 //     (ClothPBD.cpp:356-358), which otherwise fires after one iteration in FEM
 //     mode because the link error stays 0.
+#include "oracle_geom.h"
 #include "oracle_vec.h"
 
 #include <algorithm>
@@ -33,7 +34,7 @@
 namespace oracle {
 
 enum { CF_VERTICES = 1, CF_TRIANGLES = 2, CF_SELF = 4, CF_WALLS = 8, CF_EDGES = 16 };
-enum { CT_WALLS = 1, CT_SPHERE = 3, CT_SDF = 7 };
+enum { CT_WALLS = 1, CT_SPHERE = 3, CT_MESH = 4, CT_SDF = 7 };
 
 struct Params {
     // defaults: Physics/ClothModel.cpp:10-36
@@ -69,6 +70,26 @@ struct Contact {
     int coll;  // index of the collidable that produced it
     V3<R> point, normal, vel;
     R lambda = 0, lambdaF = 0, depth = 0;
+    int feature = -1;  // obstacle triangle (mesh contacts)
+    int mesh = -1;     // obstacle mesh index, -1 for walls and SDFs
+};
+
+// Physics/ClothTypes.h:110-120: an obstacle edge against a cloth edge, an
+// obstacle vertex against a cloth triangle
+template <class R>
+struct EdgeContact {
+    uint32_t i1, i2;
+    int clothEdge, feature, mesh;
+    V3<R> point, normal;
+    R w1, w2, lambda = 0, lambdaF = 0;
+};
+
+template <class R>
+struct TriContact {
+    uint32_t i1, i2, i3;
+    int clothTri, feature, mesh;
+    V3<R> point, normal;
+    R w1, w2, w3, lambda = 0, lambdaF = 0;
 };
 
 // Node-sampled grid SDF: Geometry/SDF.h:72-155, Geometry/SDF.cpp:10-40,210-256
@@ -198,6 +219,7 @@ struct Collidable {
     V3<R> center;      // sphere
     R radius = 0;
     int sdf = -1;      // index into sdfs
+    int mesh = -1;     // index into meshes
 };
 
 template <class R>
@@ -215,7 +237,13 @@ public:
     std::vector<Contact<R>> contacts;
     std::vector<Collidable<R>> world;
     std::vector<GridSDF<R>> sdfs;
+    std::vector<ObstacleMesh> meshes;
+    std::vector<EdgeContact<R>> edgeContacts;
+    std::vector<TriContact<R>> triContacts;
     std::vector<int> triOrder, bendOrder;
+    // processing order of edge and triangle contacts by cloth edge / triangle
+    // (empty: ascending index, the reference's list order)
+    std::vector<int> edgeContactRank, triContactRank;
 
     void clear()
     {
@@ -395,6 +423,8 @@ public:
     void detect_collisions()
     {
         contacts.clear();
+        edgeContacts.clear();
+        triContacts.clear();
         for (size_t ci = 0; ci < world.size(); ci++) {
             const Collidable<R>& c = world[ci];
             if (c.type == CT_WALLS && (prm.collFlags & CF_WALLS)) {
@@ -404,8 +434,251 @@ public:
             } else if (c.type == CT_SDF) {
                 const GridSDF<R>& g = sdfs[c.sdf];
                 sdf_vertex_collisions([&](V3<R> p, V3<R>& cp, V3<R>& n) { return g.query_point(p, cp, n); }, (int)ci);
+            } else if (c.type == CT_MESH) {
+                mesh_collisions(meshes[c.mesh], (int)ci, c.mesh);
             }
         }
+        auto byRank = [](const std::vector<int>& rank) {
+            return [&rank](int a, int b) { return rank[a] < rank[b]; };
+        };
+        if (!edgeContactRank.empty())
+            std::stable_sort(edgeContacts.begin(), edgeContacts.end(),
+                             [&](const EdgeContact<R>& a, const EdgeContact<R>& b) {
+                                 return byRank(edgeContactRank)(a.clothEdge, b.clothEdge);
+                             });
+        if (!triContactRank.empty())
+            std::stable_sort(triContacts.begin(), triContacts.end(), [&](const TriContact<R>& a, const TriContact<R>& b) {
+                return byRank(triContactRank)(a.clothTri, b.clothTri);
+            });
+    }
+
+    static V3<R> vR(const V3<float>& a) { return V3<R>{R(a.x), R(a.y), R(a.z)}; }
+
+    // Physics/ClothCollision.cpp:89-197 with MeshClosestPoints::ClosestPoints
+    // (Geometry/ClosestPointsBetweenMeshes.cpp:595-695): nearest obstacle
+    // triangle per cloth vertex, nearest obstacle vertex per cloth triangle,
+    // nearest obstacle edge per cloth edge, all from the cloth's previous
+    // positions. The reference finds the nearest feature by a branch-and-bound
+    // tree walk that keeps the first strictly nearer candidate in its walk
+    // order; here the search is exhaustive and ties at equal FP32 distance go
+    // to the lowest feature index. Distance 0 is skipped as in the reference.
+    void mesh_collisions(const ObstacleMesh& M, int ci, int mi)
+    {
+        const size_t n = pos.size();
+        std::vector<R> tvs(n);
+        R maxTv = 0;
+        for (size_t i = 0; i < n; i++) {
+            tvs[i] = length(pos[i] - prev[i]);
+            if (tvs[i] > maxTv) maxTv = tvs[i];
+        }
+        const R radTol = R(prm.thickness) + smax(R(prm.tolerance), maxTv * R(1.5f));
+        const size_t nt = M.idx.size() / 3;
+        if (prm.collFlags & CF_VERTICES) {
+            for (size_t i = 0; i < n; i++) {
+                const V3<R> q = prev[i];
+                R best = R(FLT_MAX);
+                int tri = -1;
+                V3<R> bcp{0, 0, 0}, bn{0, 0, 0};
+                for (size_t t = 0; t < nt; t++) {
+                    V3<R> pt, bar;
+                    closest_pt_point_triangle(q, vR(M.v[M.idx[3 * t]]), vR(M.v[M.idx[3 * t + 1]]),
+                                              vR(M.v[M.idx[3 * t + 2]]), pt, bar);
+                    V3<R> delta = q - pt;
+                    R d = length(delta);
+                    if (d == R(0)) continue;
+                    if (d < best) {
+                        best = d;
+                        tri = (int)t;
+                        bcp = pt;
+                        bn = (R(1.f) / d) * delta;
+                    }
+                }
+                if (tri >= 0 && best < radTol && best >= R(0)) {
+                    add_contact(i, bcp, bn, ci);
+                    contacts.back().feature = tri;
+                    contacts.back().mesh = mi;
+                }
+            }
+        }
+        if (prm.collFlags & CF_TRIANGLES) {
+            for (size_t j = 0; j < tris.size(); j++) {
+                const uint32_t i1 = tris[j].i1, i2 = tris[j].i2, i3 = tris[j].i3;
+                const V3<R> a = prev[i1], b = prev[i2], c = prev[i3];
+                R best = R(FLT_MAX), dist = R(-FLT_MAX);
+                int vtx = -1;
+                V3<R> bpt{0, 0, 0}, bn{0, 0, 0}, bbar{0, 0, 0};
+                for (size_t v = 0; v < M.v.size(); v++) {
+                    V3<R> cp, bar;
+                    const V3<R> mv = vR(M.v[v]);
+                    closest_pt_point_triangle(mv, a, b, c, cp, bar);
+                    V3<R> delta = cp - mv;
+                    R d = length(delta);
+                    if (d == R(0)) continue;
+                    if (d < best) {
+                        best = d;
+                        dist = d;
+                        vtx = (int)v;
+                        bpt = mv;
+                        bbar = bar;
+                        bn = (R(1.f) / d) * delta;
+                    }
+                }
+                const R tv = smax(tvs[i1], smax(tvs[i2], tvs[i3]));
+                const R rt = R(prm.thickness) + smax(R(prm.tolerance), tv * R(1.2f));
+                if (dist < rt && dist >= R(0)) {
+                    TriContact<R> tc;
+                    tc.i1 = i1; tc.i2 = i2; tc.i3 = i3;
+                    tc.clothTri = (int)j; tc.feature = vtx; tc.mesh = mi;
+                    tc.point = bpt; tc.normal = bn;
+                    tc.w1 = bbar.x; tc.w2 = bbar.y; tc.w3 = bbar.z;
+                    triContacts.push_back(tc);
+                }
+            }
+        }
+        if (prm.collFlags & CF_EDGES) {
+            for (size_t e1 = 0; e1 < edges.size(); e1++) {
+                const uint32_t i1 = edges[e1].i1, i2 = edges[e1].i2;
+                const V3<R> p = prev[i1], q = prev[i2];
+                R best = R(FLT_MAX), dist = R(-FLT_MAX);
+                int edge = -1;
+                V3<R> bcp{0, 0, 0}, bn{0, 0, 0};
+                R bws = 0;
+                for (size_t e = 0; e < M.edges.size(); e++) {
+                    R ps, pm;
+                    V3<R> cs, cm;
+                    closest_pt_segm_segm(p, q, vR(M.v[M.edges[e].i1]), vR(M.v[M.edges[e].i2]), ps, pm, cs, cm);
+                    V3<R> delta = cm - cs;
+                    R d = length(delta);
+                    if (d == R(0)) continue;
+                    if (d < best) {
+                        best = d;
+                        dist = d;
+                        edge = (int)e;
+                        bcp = cm;
+                        bws = ps;
+                        // ComputeEdgeNormal (ClosestPointOnMesh.cpp:483-497)
+                        V3<float> nf = M.en[e];
+                        if (pm == R(0)) nf = M.vn[M.edges[e].i1];
+                        if (pm == R(1)) nf = M.vn[M.edges[e].i2];
+                        bn = vR(nf);
+                    }
+                }
+                if (dist < radTol && dist >= R(0)) {
+                    EdgeContact<R> ec;
+                    ec.i1 = i1; ec.i2 = i2;
+                    ec.clothEdge = (int)e1; ec.feature = edge; ec.mesh = mi;
+                    ec.point = bcp; ec.normal = bn;
+                    ec.w1 = R(1.0f) - bws; ec.w2 = bws;
+                    edgeContacts.push_back(ec);
+                }
+            }
+        }
+    }
+
+    // Physics/ClothCollisionHandler.cpp:289-362
+    R solve_edge_contacts()
+    {
+        const R mu = prm.friction, len0 = prm.thickness;
+        R error = 0;
+        for (EdgeContact<R>& c : edgeContacts) {
+            V3<R>& x1 = pos[c.i1];
+            V3<R>& x2 = pos[c.i2];
+            const R m1 = invMass[c.i1], m2 = invMass[c.i2];
+            V3<R> p = c.w1 * x1 + c.w2 * x2;
+            if (c.mesh >= 0) {
+                const ObstacleMesh& M = meshes[c.mesh];
+                R s1, t;
+                closest_pt_segm_segm(vR(M.v[M.edges[c.feature].i1]), vR(M.v[M.edges[c.feature].i2]), x1, x2, s1, t,
+                                     c.point, p);
+                c.w1 = R(1.0f) - t;
+                c.w2 = t;
+            }
+            R len = dot(c.normal, p - c.point);
+            if (len > len0) continue;
+            R s = R(1.f) / (c.w1 * c.w1 / m1 + c.w2 * c.w2 / m2);
+            R depth = len0 - len;
+            if (depth > error) error = depth;
+            R dLambda = s * depth;
+            c.lambda += dLambda;
+            V3<R> disp = dLambda * c.normal;
+            if (mu != R(0)) {
+                V3<R> p0 = c.w1 * prev[c.i1] + c.w2 * prev[c.i2];
+                V3<R> v12 = p - p0;
+                R vnrel = dot(c.normal, v12);
+                V3<R> vt = v12 - vnrel * c.normal;
+                R vtrel = length(vt);
+                const R limit = mu * c.lambda;
+                if (vtrel > R(0.001f)) {
+                    R dLF = vtrel;
+                    R l0 = c.lambdaF;
+                    c.lambdaF = l0 + dLF;
+                    if (c.lambdaF >= limit) c.lambdaF = limit;
+                    dLambda = c.lambdaF - l0;
+                    scale(vt, R(1.f) / vtrel);
+                    V3<R> pf = (-s * dLambda) * vt;
+                    x1 += pf * (c.w1 * m1);
+                    x2 += pf * (c.w2 * m2);
+                }
+            }
+            x1 += disp * (c.w1 * m1);
+            x2 += disp * (c.w2 * m2);
+        }
+        return error;
+    }
+
+    // Physics/ClothCollisionHandler.cpp:204-287
+    R solve_tri_contacts()
+    {
+        const R mu = prm.friction, len0 = prm.thickness;
+        R error = 0;
+        for (TriContact<R>& c : triContacts) {
+            V3<R>& x1 = pos[c.i1];
+            V3<R>& x2 = pos[c.i2];
+            V3<R>& x3 = pos[c.i3];
+            const R m1 = invMass[c.i1], m2 = invMass[c.i2], m3 = invMass[c.i3];
+            V3<R> p, coords;
+            closest_pt_point_triangle(c.point, x1, x2, x3, p, coords);
+            c.w1 = coords.x; c.w2 = coords.y; c.w3 = coords.z;
+            V3<R> n = -cross(x2 - x1, x3 - x1);
+            normalize(n);
+            c.normal = n;
+            if (c.mesh >= 0 && c.feature >= 0) {
+                const V3<R> mn = vR(meshes[c.mesh].vn[c.feature]);
+                if (dot(n, mn) < R(0)) c.normal = -n;
+            }
+            R len = dot(c.normal, p - c.point);
+            if (len > len0) continue;
+            R s = R(1.f) / (c.w1 * c.w1 / m1 + c.w2 * c.w2 / m2 + c.w3 * c.w3 / m3);
+            R depth = len0 - len;
+            if (depth > error) error = depth;
+            R dLambda = s * depth;
+            c.lambda += dLambda;
+            V3<R> disp = dLambda * c.normal;
+            if (mu != R(0)) {
+                V3<R> p0 = c.w1 * prev[c.i1] + c.w2 * prev[c.i2] + c.w3 * prev[c.i3];
+                V3<R> v12 = p - p0;
+                R vnrel = dot(c.normal, v12);
+                V3<R> vt = v12 - vnrel * c.normal;
+                R vtrel = length(vt);
+                const R limit = mu * c.lambda;
+                if (vtrel > R(0.001f)) {
+                    R dLF = vtrel;
+                    R l0 = c.lambdaF;
+                    c.lambdaF = l0 + dLF;
+                    if (c.lambdaF >= limit) c.lambdaF = limit;
+                    dLambda = c.lambdaF - l0;
+                    scale(vt, R(1.f) / vtrel);
+                    V3<R> pf = (-s * dLambda) * vt;
+                    x1 += pf * (c.w1 * m1);
+                    x2 += pf * (c.w2 * m2);
+                    x3 += pf * (c.w3 * m3);
+                }
+            }
+            x1 += disp * (c.w1 * m1);
+            x2 += disp * (c.w2 * m2);
+            x3 += disp * (c.w3 * m3);
+        }
+        return error;
     }
 
     // Physics/ClothPBD.cpp:52-108
@@ -538,6 +811,19 @@ public:
         R error = 0;
         for (Contact<R>& c : contacts) {
             V3<R>& p = pos[c.idx];
+            if (c.mesh >= 0) {
+                // UPDATE_CLOSEST_POINT (ClothCollisionHandler.cpp:145-160):
+                // re-project onto the same obstacle triangle; interpolated
+                // vertex normal (ComputeTriangleNormal, ClosestPointOnMesh.cpp:37-62)
+                const ObstacleMesh& M = meshes[c.mesh];
+                const uint32_t i0 = M.idx[3 * c.feature], i1 = M.idx[3 * c.feature + 1], i2 = M.idx[3 * c.feature + 2];
+                V3<R> cp, co;
+                closest_pt_point_triangle(p, vR(M.v[i0]), vR(M.v[i1]), vR(M.v[i2]), cp, co);
+                c.point = cp;
+                V3<R> nn = co.x * vR(M.vn[i0]) + co.y * vR(M.vn[i1]) + co.z * vR(M.vn[i2]);
+                normalize(nn);
+                c.normal = nn;
+            }
             V3<R> delta = p - c.point;
             R len = dot(delta, c.normal);
             V3<R> disp = c.normal;
@@ -576,9 +862,11 @@ public:
             if (prm.fem) solve_triangles(h);
             if (prm.dihedral && k % prm.bendFreq == 0) cb = solve_bends();
             cv = solve_contacts(h);
-            // edge, triangle and self contacts are not generated on this path,
-            // so their errors ce = ct = sct = selfEE = 0 in the exit test
-            const R ce = 0, ct = 0, sct = 0, sce = 0, cl = 0;
+            const R ce = solve_edge_contacts();
+            const R ct = solve_tri_contacts();
+            // self contacts are not generated on this path and the FEM model
+            // reports no link error, so sct = selfEE = cl = 0 in the exit test
+            const R sct = 0, sce = 0, cl = 0;
             k++;
             const float eps = 0.01f;
             if (!prm.fixedIterations && cl < R(0.05f) && cb < R(0.005f) && sce < R(prm.selfCollThreshold) &&
@@ -931,6 +1219,120 @@ double oc_solve_contacts(void* h, float hstep)
 }
 
 float oc_acos(float x) { return acos_as(x); }
+
+// A static triangle-mesh obstacle (CollisionMesh, Physics/Common.h:113-138)
+int oc_add_mesh(void* h, int nv, const float* verts, int ni, const unsigned* idx)
+{
+    return with(h, [&](auto& c) {
+        using R = typename std::remove_reference<decltype(c.pos[0].x)>::type;
+        ObstacleMesh M;
+        M.v.resize(nv);
+        for (int i = 0; i < nv; i++) M.v[i] = V3<float>{verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]};
+        M.idx.assign(idx, idx + ni);
+        M.build();
+        c.meshes.push_back(M);
+        Collidable<R> col;
+        col.type = CT_MESH;
+        col.mesh = (int)c.meshes.size() - 1;
+        c.world.push_back(col);
+        return (int)c.world.size() - 1;
+    });
+}
+
+// Processing order of triangle (kind 2) or edge (kind 3) contacts, given as
+// the cloth triangle / edge indices in order.
+void oc_set_contact_order(void* h, int kind, const int* order, int n)
+{
+    with(h, [&](auto& c) {
+        std::vector<int>& rank = kind == 2 ? c.triContactRank : c.edgeContactRank;
+        rank.assign(n, 0);
+        for (int k = 0; k < n; k++) rank[order[k]] = k;
+    });
+}
+
+// edge contacts: ids (i1, i2, clothEdge, feature), data (point 3, normal 3, w1, w2, lambda, lambdaF)
+int oc_get_edge_contacts(void* h, int* ids, double* data, int cap)
+{
+    return with(h, [&](auto& c) {
+        int n = (int)c.edgeContacts.size();
+        for (int k = 0; k < n && k < cap; k++) {
+            const auto& e = c.edgeContacts[k];
+            ids[4 * k] = e.i1; ids[4 * k + 1] = e.i2; ids[4 * k + 2] = e.clothEdge; ids[4 * k + 3] = e.feature;
+            double r[10] = {(double)e.point.x, (double)e.point.y, (double)e.point.z, (double)e.normal.x,
+                            (double)e.normal.y, (double)e.normal.z, (double)e.w1, (double)e.w2, (double)e.lambda,
+                            (double)e.lambdaF};
+            memcpy(data + 10 * k, r, sizeof(r));
+        }
+        return n;
+    });
+}
+
+// triangle contacts: ids (clothTri, feature), data (point 3, normal 3, w1, w2, w3, lambda, lambdaF)
+int oc_get_tri_contacts(void* h, int* ids, double* data, int cap)
+{
+    return with(h, [&](auto& c) {
+        int n = (int)c.triContacts.size();
+        for (int k = 0; k < n && k < cap; k++) {
+            const auto& t = c.triContacts[k];
+            ids[2 * k] = t.clothTri; ids[2 * k + 1] = t.feature;
+            double r[11] = {(double)t.point.x, (double)t.point.y, (double)t.point.z, (double)t.normal.x,
+                            (double)t.normal.y, (double)t.normal.z, (double)t.w1, (double)t.w2, (double)t.w3,
+                            (double)t.lambda, (double)t.lambdaF};
+            memcpy(data + 11 * k, r, sizeof(r));
+        }
+        return n;
+    });
+}
+
+// contact features of the vertex contacts: (mesh, feature) per contact
+void oc_get_contact_features(void* h, int* out)
+{
+    with(h, [&](auto& c) {
+        for (size_t k = 0; k < c.contacts.size(); k++) {
+            out[2 * k] = c.contacts[k].mesh;
+            out[2 * k + 1] = c.contacts[k].feature;
+        }
+    });
+}
+
+// Geometry primitives in FP32 for the golden-vector tests
+int oc_closest_pt_point_triangle(const float* p, const float* a, const float* b, const float* c, float* out)
+{
+    V3<float> cp, bar;
+    int r = closest_pt_point_triangle(V3<float>{p[0], p[1], p[2]}, V3<float>{a[0], a[1], a[2]},
+                                      V3<float>{b[0], b[1], b[2]}, V3<float>{c[0], c[1], c[2]}, cp, bar);
+    float o[6] = {cp.x, cp.y, cp.z, bar.x, bar.y, bar.z};
+    memcpy(out, o, sizeof(o));
+    return r;
+}
+
+int oc_closest_pt_segm_segm(const float* p1, const float* q1, const float* p2, const float* q2, float* out)
+{
+    float s, t;
+    V3<float> c1, c2;
+    int r = closest_pt_segm_segm(V3<float>{p1[0], p1[1], p1[2]}, V3<float>{q1[0], q1[1], q1[2]},
+                                 V3<float>{p2[0], p2[1], p2[2]}, V3<float>{q2[0], q2[1], q2[2]}, s, t, c1, c2);
+    float o[8] = {s, t, c1.x, c1.y, c1.z, c2.x, c2.y, c2.z};
+    memcpy(out, o, sizeof(o));
+    return r;
+}
+
+float oc_signed_volume(const float* a0, const float* a1, const float* a2, const float* a3)
+{
+    return signed_volume(V3<float>{a0[0], a0[1], a0[2]}, V3<float>{a1[0], a1[1], a1[2]}, V3<float>{a2[0], a2[1], a2[2]},
+                         V3<float>{a3[0], a3[1], a3[2]});
+}
+
+// ObstacleMesh::build normals (vertex normals, nv*3 floats)
+void oc_mesh_normals(int nv, const float* verts, int ni, const unsigned* idx, float* out)
+{
+    ObstacleMesh M;
+    M.v.resize(nv);
+    for (int i = 0; i < nv; i++) M.v[i] = V3<float>{verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]};
+    M.idx.assign(idx, idx + ni);
+    M.build();
+    for (int i = 0; i < nv; i++) { out[3 * i] = M.vn[i].x; out[3 * i + 1] = M.vn[i].y; out[3 * i + 2] = M.vn[i].z; }
+}
 
 // Mesh::ConstructEdges on an arbitrary index list; out = (i1, i2, t1, t2) per
 // unique edge. Returns the number of edges.

@@ -1,0 +1,200 @@
+// TEST INFRASTRUCTURE ONLY (see oracle_vec.h).
+//
+// Closest-point primitives and the static obstacle mesh, restated from the
+// reference's geometry code with its operation order:
+//   Geometry/Collision3D.cpp:62-141  ClosestPtPointTriangle (Ericson 5.1.5)
+//   Geometry/Collision3D.cpp:10-58   ClosestPtSegmSegm (Ericson 5.1.9)
+//   Geometry/Collision3D.cpp:298-301 SignedVolume
+//   Geometry/Mesh.cpp:10-63          angle-weighted vertex normals
+//   Geometry/Mesh.cpp:65-147         unique edges with winged triangles
+// These are pinned bit for bit against the reference's compiled Geometry code
+// (tests/golden/ref_collision3d.npz, ref_meshes.npz).
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "oracle_vec.h"
+
+namespace oracle {
+
+enum { TR_VERTEX_A, TR_VERTEX_B, TR_VERTEX_C, TR_EDGE_AB, TR_EDGE_AC, TR_EDGE_BC, TR_FACE_INTERIOR };
+enum { ER_EDGE_INTERIOR = 0, ER_VERTEX_P1 = 1, ER_VERTEX_Q1 = 2, ER_VERTEX_P2 = 4, ER_VERTEX_Q2 = 8 };
+
+template <class R>
+int closest_pt_point_triangle(V3<R> p, V3<R> a, V3<R> b, V3<R> c, V3<R>& cp, V3<R>& bar)
+{
+    bar = V3<R>{0, 0, 0};
+    V3<R> ab = b - a, ac = c - a, ap = p - a;
+    R d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= R(0) && d2 <= R(0)) {
+        bar.x = 1;
+        cp = a;
+        return TR_VERTEX_A;
+    }
+    V3<R> bp = p - b;
+    R d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= R(0) && d4 <= d3) {
+        bar.y = 1;
+        cp = b;
+        return TR_VERTEX_B;
+    }
+    R vc = d1 * d4 - d3 * d2;
+    if (vc <= R(0) && d1 >= R(0) && d3 <= R(0)) {
+        bar.y = d1 / (d1 - d3);
+        bar.x = R(1) - bar.y;
+        cp = a + bar.y * ab;
+        return TR_EDGE_AB;
+    }
+    V3<R> cq = p - c;
+    R d5 = dot(ab, cq), d6 = dot(ac, cq);
+    if (d6 >= R(0) && d5 <= d6) {
+        bar.z = 1;
+        cp = c;
+        return TR_VERTEX_C;
+    }
+    R vb = d5 * d2 - d1 * d6;
+    if (vb <= R(0) && d2 >= R(0) && d6 <= R(0)) {
+        bar.z = d2 / (d2 - d6);
+        bar.x = R(1) - bar.z;
+        cp = a + bar.z * ac;
+        return TR_EDGE_AC;
+    }
+    R va = d3 * d6 - d5 * d4;
+    R d43 = d4 - d3, d56 = d5 - d6;
+    if (va <= R(0) && d43 >= R(0) && d56 >= R(0)) {
+        bar.z = d43 / (d43 + d56);
+        bar.y = R(1) - bar.z;
+        cp = b + bar.z * (c - b);
+        return TR_EDGE_BC;
+    }
+    R denom = R(1.f) / (va + vb + vc);
+    bar.y = vb * denom;
+    bar.z = vc * denom;
+    bar.x = R(1.f) - bar.y - bar.z;
+    cp = a + ab * bar.y + ac * bar.z;
+    return TR_FACE_INTERIOR;
+}
+
+// Math::clamp = std::max(lo, std::min(hi, v)) (Math/Utils.h:24-28)
+template <class R> inline R clampf(R v, R lo, R hi) { return smax(lo, smin(hi, v)); }
+
+template <class R>
+int closest_pt_segm_segm(V3<R> p1, V3<R> q1, V3<R> p2, V3<R> q2, R& s, R& t, V3<R>& c1, V3<R>& c2)
+{
+    V3<R> d1 = q1 - p1, d2 = q2 - p2, r = p1 - p2;
+    R a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+    R c = dot(d1, r), b = dot(d1, d2);
+    R denom = a * e - b * b;
+    if (denom != R(0)) s = clampf((b * f - c * e) / denom, R(0), R(1));
+    else s = 0;
+    int region = ER_EDGE_INTERIOR;
+    t = (b * s + f) / e;
+    if (t < R(0)) {
+        region |= ER_VERTEX_P2;
+        t = 0;
+        s = clampf(-c / a, R(0), R(1));
+    } else if (t > R(1)) {
+        region |= ER_VERTEX_Q2;
+        t = 1;
+        s = clampf((b - c) / a, R(0), R(1));
+    }
+    if (s == R(0)) region |= ER_VERTEX_P1;
+    if (s == R(1)) region |= ER_VERTEX_Q1;
+    c1 = p1 + s * d1;
+    c2 = p2 + t * d2;
+    return region;
+}
+
+template <class R>
+inline R signed_volume(V3<R> a0, V3<R> a1, V3<R> a2, V3<R> a3)
+{
+    // triple(u, v, w) = w . (u x v), Math/Vector3.h:289-292
+    V3<R> u = a1 - a0, v = a2 - a0, w = a3 - a0;
+    return dot(w, cross(u, v));
+}
+
+struct MeshEdge {
+    uint32_t i1, i2;
+    int t1 = -1, t2 = -1;
+};
+
+// Geometry/Mesh.cpp:65-147 on an index list (std::sort as the reference)
+inline std::vector<MeshEdge> mesh_edges(const std::vector<uint32_t>& idx)
+{
+    std::vector<MeshEdge> edges;
+    const int ne = (int)idx.size();
+    if (ne == 0) return edges;
+    std::vector<MeshEdge> dup(ne);
+    for (int i = 0; i < ne; i++) {
+        uint32_t a = idx[i];
+        uint32_t b = ((i + 1) % 3 == 0) ? idx[i - 2] : idx[i + 1];
+        if (a > b) std::swap(a, b);
+        dup[i].i1 = a; dup[i].i2 = b; dup[i].t1 = i / 3;
+    }
+    std::sort(dup.begin(), dup.end(), [](const MeshEdge& e1, const MeshEdge& e2) {
+        return e1.i1 < e2.i1 || (e1.i1 == e2.i1 && e1.i2 < e2.i2);
+    });
+    MeshEdge cur = dup[0];
+    for (size_t i = 1; i < dup.size(); i++) {
+        if (dup[i].i1 == dup[i - 1].i1 && dup[i].i2 == dup[i - 1].i2) {
+            cur.t2 = dup[i].t1;
+            continue;
+        }
+        edges.push_back(cur);
+        cur = dup[i];
+    }
+    edges.push_back(cur);
+    return edges;
+}
+
+// A static triangle mesh obstacle in float (the reference's Mesh stores
+// Vector3 = FP32): vertices, indices, angle-weighted vertex normals, face
+// normals, unique edges and their unweighted-average normals.
+struct ObstacleMesh {
+    std::vector<V3<float>> v;
+    std::vector<uint32_t> idx;
+    std::vector<V3<float>> vn, tn, en;
+    std::vector<MeshEdge> edges;
+
+    void build()
+    {
+        const size_t nt = idx.size() / 3;
+        vn.assign(v.size(), V3<float>{0, 0, 0});
+        tn.assign(nt, V3<float>{0, 0, 0});
+        for (size_t t = 0; t < nt; t++) {
+            const uint32_t i1 = idx[3 * t], i2 = idx[3 * t + 1], i3 = idx[3 * t + 2];
+            const V3<float> v1 = v[i1], v2 = v[i2], v3 = v[i3];
+            V3<float> n = cross(v2 - v1, v3 - v1);
+            normalize(n);
+            tn[t] = n;
+            V3<float> e1 = v2 - v1, e2 = v3 - v2, e3 = v1 - v3;
+            normalize(e1);
+            normalize(e2);
+            normalize(e3);
+            float c1 = -dot(e1, e3), c2 = -dot(e1, e2), c3 = -dot(e2, e3);
+            float s1 = length(cross(e1, -e3)), s2 = length(cross(e2, -e1)), s3 = length(cross(e3, -e1));
+            float t1 = std::fabs(c1) > 1 ? std::fabs((float)std::atan2(s1, c1)) : std::acos(c1);
+            float t2 = std::fabs(c2) > 1 ? std::fabs((float)std::atan2(s2, c2)) : std::acos(c2);
+            float t3 = std::fabs(c3) > 1 ? std::fabs((float)std::atan2(s3, c3)) : std::acos(c3);
+            vn[i1] += t1 * n;
+            vn[i2] += t2 * n;
+            vn[i3] += t3 * n;
+        }
+        for (auto& n : vn)
+            if (length(n) != 0.f) normalize(n);
+        edges = mesh_edges(idx);
+        en.resize(edges.size());
+        for (size_t e = 0; e < edges.size(); e++) {
+            V3<float> n = tn[edges[e].t1];
+            if (edges[e].t2 >= 0) {
+                n = tn[edges[e].t1] + tn[edges[e].t2];
+                normalize(n);
+            }
+            en[e] = n;
+        }
+    }
+};
+
+}  // namespace oracle

@@ -32,6 +32,48 @@ def test_grid_topology_matches_reference_mesh(oracle_mod):
     assert np.array_equal(pos, MESHES["grid32_vertices"])
 
 
+COLL = np.load(os.path.join(HERE, "golden", "ref_collision3d.npz"))
+
+
+def test_closest_pt_point_triangle_matches_reference(oracle_mod):
+    """ClosestPtPointTriangle (Geometry/Collision3D.cpp:62-141): region, point
+    and barycentric coordinates bit for bit, including vertex, edge and
+    degenerate cases."""
+    P = COLL["points"]
+    for k in range(len(P)):
+        a, b, c, q = P[k]
+        r, cp, bar = oracle_mod.closest_pt_point_triangle(q, a, b, c)
+        assert r == COLL["tri_region"][k], k
+        assert np.array_equal(np.r_[cp, bar], COLL["tri_out"][k]), k
+
+
+def test_closest_pt_segm_segm_matches_reference(oracle_mod):
+    """ClosestPtSegmSegm (Geometry/Collision3D.cpp:10-58), bit for bit."""
+    P = COLL["points"]
+    for k in range(len(P)):
+        a, b, c, q = P[k]
+        r, s, t, c1, c2 = oracle_mod.closest_pt_segm_segm(a, b, c, q)
+        assert r == COLL["segm_region"][k], k
+        got = np.r_[s, t, c1, c2].astype(np.float32)
+        exp = COLL["segm_out"][k]
+        assert np.array_equal(got, exp) or (np.isnan(got).sum() == np.isnan(exp).sum()
+                                             and np.array_equal(got[~np.isnan(got)], exp[~np.isnan(exp)])), k
+
+
+def test_signed_volume_matches_reference(oracle_mod):
+    P = COLL["points"]
+    for k in range(len(P)):
+        assert oracle_mod.signed_volume(*P[k]) == COLL["signed_volume"][k], k
+
+
+@pytest.mark.parametrize("name", ["grid16", "ico1"])
+def test_mesh_normals_match_reference(oracle_mod, name):
+    """Angle-weighted vertex normals (Geometry/Mesh.cpp:10-63) bit for bit
+    with the same libm acosf."""
+    got = oracle_mod.mesh_normals(MESHES[f"{name}_vertices"], MESHES[f"{name}_indices"])
+    assert np.array_equal(got, MESHES[f"{name}_normals"])
+
+
 def test_bends_from_reference_edges(oracle_mod):
     """Each interior edge (t2 >= 0) of the reference's edge list gives one bend
     (i1, i2, opposite vertex of t1, opposite vertex of t2),
