@@ -189,8 +189,10 @@ SCEXP int scSetParticleVelocities(scModel model, const scVector3* in, int count)
 SCEXP int scGetNumBends(scModel model);
 SCEXP int scGetBends(scModel model, int* out, int count);
 
-/* Gauss-Seidel order the GPU applies: kind 0 = triangles, 1 = bends. Writes
- * the constraint indices in processing order and returns their count.
+/* Gauss-Seidel order the GPU applies: kind 0 = triangles, 1 = bends,
+ * 2 = obstacle-edge contacts (given as cloth edge indices), 3 = obstacle-
+ * vertex contacts (given as cloth triangle indices). Writes the constraint
+ * indices in processing order and returns their count.
  * scGetColouring returns the number of launch phases and writes phase start
  * offsets into that order (numPhases + 1 values, colourOffsets may be NULL).
  * scGetSchedule writes, per position of the order, the phase, the slab (-1
@@ -203,6 +205,19 @@ SCEXP int scGetSchedule(scModel model, int kind, int* phase, int* slab, int* col
  * mode 1 = slab phases with the slab's vertices staged in LDS (default).
  * slabSize 0 lets the library choose. Changes the Gauss-Seidel order. */
 SCEXP void scSetSweep(scModel model, int mode, int slabSize);
+
+/* Contacts of the last substep against mesh obstacles, in the reference's
+ * list order (collidable, then cloth vertex / edge / triangle).
+ * scGetContactFeatures: the obstacle triangle of each vertex contact of
+ * scGetContact (-1 for walls, spheres and SDFs).
+ * scGetEdgeContactData (Physics/ClothTypes.h:110-120 EdgeContact): ids =
+ * (cloth edge, obstacle edge, collidable), data = (point[3], normal[3], w1,
+ * w2, lambda, lambdaF) per contact. scGetTriContactData (TriContact): ids =
+ * (cloth triangle, obstacle vertex, collidable), data = (point[3], normal[3],
+ * w1, w2, w3, lambda, lambdaF). All return the contact count. */
+SCEXP int scGetContactFeatures(scModel model, int* out, int count);
+SCEXP int scGetEdgeContactData(scModel model, int* ids, float* data, int cap);
+SCEXP int scGetTriContactData(scModel model, int* ids, float* data, int cap);
 
 /* Solver statistics of the last substep (Physics/ClothModel.h:32-40):
  * out[0] iterations, out[1] bend error, out[2] vertex-contact error. */

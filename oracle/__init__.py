@@ -60,6 +60,10 @@ def lib():
         L.oc_solve_bends.restype = _C.c_double
         L.oc_solve_contacts.argtypes = [vp, _C.c_float]
         L.oc_solve_contacts.restype = _C.c_double
+        L.oc_solve_edge_contacts.argtypes = [vp]
+        L.oc_solve_edge_contacts.restype = _C.c_double
+        L.oc_solve_tri_contacts.argtypes = [vp]
+        L.oc_solve_tri_contacts.restype = _C.c_double
         L.oc_acos.argtypes = [_C.c_float]
         L.oc_acos.restype = _C.c_float
         L.oc_construct_edges.argtypes = [vp, _C.c_int, vp, _C.c_int]
@@ -149,7 +153,7 @@ class Cloth:
         """kind 'tri' / 'edge': processing order of triangle / edge contacts
         given as cloth triangle / edge indices."""
         o = _np.ascontiguousarray(order, dtype=_np.int32)
-        self.L.oc_set_contact_order(self.h, 2 if kind == "tri" else 3, _ptr(o), len(o))
+        self.L.oc_set_contact_order(self.h, 3 if kind == "tri" else 2, _ptr(o), len(o))
 
     def edge_contacts(self):
         n = self.L.oc_get_edge_contacts(self.h, None, None, 0)
@@ -261,6 +265,12 @@ class Cloth:
     def solve_contacts(self, h):
         return self.L.oc_solve_contacts(self.h, h)
 
+    def solve_edge_contacts(self):
+        return self.L.oc_solve_edge_contacts(self.h)
+
+    def solve_tri_contacts(self):
+        return self.L.oc_solve_tri_contacts(self.h)
+
 
 def acos(x):
     return lib().oc_acos(float(x))
@@ -313,9 +323,14 @@ def from_scene(scene, real64=False, tri_order=None, bend_order=None):
     if scene.sdf_cells:
         bmin, bmax = S.sdf_box(scene)
         c.add_grid_sdf(S.sphere_sdf_values(scene), (scene.sdf_cells,) * 3, bmin, bmax)
+    if scene.mesh_subdiv >= 0:
+        v, t = S.icosphere(scene.mesh_subdiv, scene.mesh_radius, scene.mesh_center)
+        c.add_mesh(v, t)
     if scene.walls_min is not None:
         c.add_walls(scene.walls_min, scene.walls_max)
     c.init_grid(scene.n, scene.n, float(scene.inc), scene.offset, scene.horizontal, scene.attached)
+    if scene.perturb_amplitude != 0.0:
+        c.put("pos", S.perturbed_positions(scene, c.get("pos")))
     if tri_order is not None:
         c.set_order("tri", tri_order)
     if bend_order is not None:

@@ -1218,6 +1218,9 @@ double oc_solve_contacts(void* h, float hstep)
     });
 }
 
+double oc_solve_edge_contacts(void* h) { return with(h, [&](auto& c) { return (double)c.solve_edge_contacts(); }); }
+double oc_solve_tri_contacts(void* h) { return with(h, [&](auto& c) { return (double)c.solve_tri_contacts(); }); }
+
 float oc_acos(float x) { return acos_as(x); }
 
 // A static triangle-mesh obstacle (CollisionMesh, Physics/Common.h:113-138)
@@ -1239,12 +1242,12 @@ int oc_add_mesh(void* h, int nv, const float* verts, int ni, const unsigned* idx
     });
 }
 
-// Processing order of triangle (kind 2) or edge (kind 3) contacts, given as
+// Processing order of edge (kind 2) or triangle (kind 3) contacts, given as
 // the cloth triangle / edge indices in order.
 void oc_set_contact_order(void* h, int kind, const int* order, int n)
 {
     with(h, [&](auto& c) {
-        std::vector<int>& rank = kind == 2 ? c.triContactRank : c.edgeContactRank;
+        std::vector<int>& rank = kind == 3 ? c.triContactRank : c.edgeContactRank;
         rank.assign(n, 0);
         for (int k = 0; k < n; k++) rank[order[k]] = k;
     });

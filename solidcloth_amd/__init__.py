@@ -105,6 +105,8 @@ def _sig(L):
         "scGetSolverStats": (None, [H, vp]), "scEnableKernelTiming": (None, [H, b]),
         "scGetKernelTiming": (i, [H, i, C.POINTER(i), C.POINTER(C.c_double)]),
         "scSetUseGraph": (None, [H, b]), "scBandwidthProbe": (C.c_double, [C.c_longlong, i]),
+        "scGetContactFeatures": (i, [H, vp, i]), "scGetEdgeContactData": (i, [H, vp, vp, i]),
+        "scGetTriContactData": (i, [H, vp, vp, i]),
         "scGetLastError": (C.c_char_p, []),
         "scGetErrorCount": (i, []), "scGetBuildInfo": (C.c_char_p, []),
     }
@@ -162,6 +164,24 @@ class SDF:
         self.n, self.bmin, self.bmax = tuple(n), tuple(bmin), tuple(bmax)
 
 
+class Mesh:
+    """Indexed triangle mesh (Geometry/Mesh.h) built from vertex and index
+    arrays; normals and unique edges are computed on creation."""
+
+    def __init__(self, vertices, indices):
+        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        i = np.ascontiguousarray(indices, dtype=np.uint32).ravel()
+        self.h = _checked("scCreateMeshFromArrays", _ptr(v), len(v), _ptr(i), i.size)
+        self.vertices, self.indices = v, i
+
+    def __del__(self):
+        try:
+            if self.h.ptr:
+                lib().scFreeMesh(self.h)
+        except Exception:
+            pass
+
+
 class CollisionWorld:
     """Physics/CollisionWorld.h:10-29 over scCreateCollisionWorld."""
 
@@ -191,6 +211,13 @@ class CollisionWorld:
     def AddSDF(self, sdf):
         self._keep.append(sdf)
         h = _checked("scCreateSDF", sdf.h)
+        self.AddCollidable(h)
+        return h
+
+    def AddMesh(self, mesh):
+        """A static triangle-mesh obstacle (CollisionMesh, Physics/Common.h:113-138)."""
+        self._keep.append(mesh)
+        h = _checked("scCreateMesh", mesh.h)
         self.AddCollidable(h)
         return h
 
@@ -288,8 +315,11 @@ class ClothModel:
         _checked("scGetBends", self.h, _ptr(out), n)
         return out
 
+    _KINDS = {0: 0, "tri": 0, 1: 1, "bend": 1, 2: 2, "edge_contact": 2, 3: 3, "tri_contact": 3}
+
     def constraint_order(self, kind):
-        k = 0 if kind in (0, "tri") else 1
+        """kind 'tri' / 'bend' / 'edge_contact' / 'tri_contact' (or 0-3)."""
+        k = self._KINDS[kind]
         n = _checked("scGetConstraintOrder", self.h, k, None, 0)
         out = np.zeros(n, dtype=np.int32)
         _checked("scGetConstraintOrder", self.h, k, _ptr(out), n)
@@ -297,7 +327,7 @@ class ClothModel:
 
     def colour_offsets(self, kind):
         """Start offsets of the launch phases in constraint_order(kind)."""
-        k = 0 if kind in (0, "tri") else 1
+        k = self._KINDS[kind]
         nc = _checked("scGetColouring", self.h, k, None, 0)
         out = np.zeros(nc + 1, dtype=np.int32)
         _checked("scGetColouring", self.h, k, _ptr(out), nc)
@@ -305,7 +335,7 @@ class ClothModel:
 
     def schedule(self, kind):
         """Per position of constraint_order(kind): (phase, slab, colour)."""
-        k = 0 if kind in (0, "tri") else 1
+        k = self._KINDS[kind]
         n = _checked("scGetSchedule", self.h, k, None, None, None, 0)
         ph, sl, co = (np.zeros(n, dtype=np.int32) for _ in range(3))
         _checked("scGetSchedule", self.h, k, _ptr(ph), _ptr(sl), _ptr(co), n)
@@ -325,6 +355,30 @@ class ClothModel:
             pts[k] = (p.x, p.y, p.z)
             nrm[k] = (q.x, q.y, q.z)
         return pts, nrm
+
+    def contact_features(self):
+        """Obstacle triangle of each vertex contact (-1 if not a mesh contact)."""
+        n = _checked("scGetContactFeatures", self.h, None, 0)
+        out = np.zeros(n, dtype=np.int32)
+        _checked("scGetContactFeatures", self.h, _ptr(out), n)
+        return out
+
+    def _feature_contacts(self, name, width):
+        n = _checked(name, self.h, None, None, 0)
+        ids = np.zeros((n, 3), dtype=np.int32)
+        data = np.zeros((n, width), dtype=np.float32)
+        _checked(name, self.h, _ptr(ids), _ptr(data), n)
+        return ids, data
+
+    def edge_contacts(self):
+        """ids (cloth edge, obstacle edge, collidable), data (point, normal, w1,
+        w2, lambda, lambdaF)."""
+        return self._feature_contacts("scGetEdgeContactData", 10)
+
+    def tri_contacts(self):
+        """ids (cloth triangle, obstacle vertex, collidable), data (point,
+        normal, w1, w2, w3, lambda, lambdaF)."""
+        return self._feature_contacts("scGetTriContactData", 11)
 
     def stats(self):
         out = np.zeros(3, dtype=np.float32)

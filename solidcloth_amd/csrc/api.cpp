@@ -376,21 +376,22 @@ void scGetContact(scModel model, int i, scVector3* point, scVector3* normal)
 
 int scGetNumTriContacts(scModel model)
 {
-    // Triangle contacts come from mesh obstacles and SDF triangle queries,
-    // which the GPU substep does not generate.
     return guard_r<int>(0, [&] {
-        (void)M(model);
-        return 0;
+        ClothModel* m = M(model);
+        m->download_contacts();
+        return (int)m->triContactsHost.size();
     });
 }
 
 void scGetTriContact(scModel model, int i, scVector3* point, scVector3* normal)
 {
     guard([&] {
-        (void)M(model);
-        (void)point;
-        (void)normal;
-        check_index(i, 0);
+        ClothModel* m = M(model);
+        m->download_contacts();
+        check_index(i, m->triContactsHost.size());
+        const FeatureContactOut& c = m->triContactsHost[i];
+        if (point) *point = scVector3{c.point.x, c.point.y, c.point.z};
+        if (normal) *normal = scVector3{c.normal.x, c.normal.y, c.normal.z};
     });
 }
 
@@ -515,6 +516,7 @@ scMesh scCreateMeshFromArrays(const scVector3* vertices, int numVertices, const 
         me->indices.assign(indices, indices + numIndices);
         me->compute_normals();
         me->construct_edges();
+        me->compute_edge_normals();
         return (void*)me;
     });
     return m;
@@ -585,12 +587,25 @@ int scGetBends(scModel model, int* out, int count)
     });
 }
 
+// kind 0: FEM triangles, 1: bends, 2: obstacle-edge contacts (by cloth
+// edge), 3: obstacle-vertex contacts (by cloth triangle)
+static const Schedule& schedule_of(ClothModel* m, int kind)
+{
+    switch (kind) {
+    case 0: return m->triCol;
+    case 1: return m->bendCol;
+    case 2: return m->edgeContactCol;
+    case 3: return m->triContactCol;
+    default: throw std::runtime_error("unknown constraint kind " + std::to_string(kind));
+    }
+}
+
 int scGetConstraintOrder(scModel model, int kind, int* order, int count)
 {
     return guard_r<int>(-1, [&] {
         ClothModel* m = M(model);
         m->ensure_colouring();
-        const Schedule& c = kind == 0 ? m->triCol : m->bendCol;
+        const Schedule& c = schedule_of(m, kind);
         int n = std::min(count, (int)c.order.size());
         if (order) memcpy(order, c.order.data(), n * sizeof(int));
         return (int)c.order.size();
@@ -602,7 +617,7 @@ int scGetColouring(scModel model, int kind, int* colourOffsets, int maxColours)
     return guard_r<int>(-1, [&] {
         ClothModel* m = M(model);
         m->ensure_colouring();
-        const Schedule& c = kind == 0 ? m->triCol : m->bendCol;
+        const Schedule& c = schedule_of(m, kind);
         int nc = c.numPhases();
         if (nc < 0) nc = 0;
         if (colourOffsets)
@@ -616,12 +631,64 @@ int scGetSchedule(scModel model, int kind, int* phase, int* slab, int* colour, i
     return guard_r<int>(-1, [&] {
         ClothModel* m = M(model);
         m->ensure_colouring();
-        const Schedule& c = kind == 0 ? m->triCol : m->bendCol;
+        const Schedule& c = schedule_of(m, kind);
         int n = std::min(count, (int)c.order.size());
         if (phase) memcpy(phase, c.phaseOf.data(), n * sizeof(int));
         if (slab) memcpy(slab, c.slabOf.data(), n * sizeof(int));
         if (colour) memcpy(colour, c.colourOf.data(), n * sizeof(int));
         return (int)c.order.size();
+    });
+}
+
+int scGetContactFeatures(scModel model, int* out, int count)
+{
+    return guard_r<int>(-1, [&] {
+        ClothModel* m = M(model);
+        m->download_contacts();
+        const int n = std::min(count, (int)m->contactsHost.size());
+        for (int i = 0; i < n && out; i++) out[i] = m->contactsHost[i].feature;
+        return (int)m->contactsHost.size();
+    });
+}
+
+static int feature_contacts(const std::vector<FeatureContactOut>& v, int nw, int* ids, float* data, int cap)
+{
+    const int n = std::min(cap, (int)v.size());
+    const int stride = 6 + nw + 2;
+    for (int i = 0; i < n; i++) {
+        const FeatureContactOut& c = v[i];
+        if (ids) {
+            ids[3 * i] = c.id;
+            ids[3 * i + 1] = c.feature;
+            ids[3 * i + 2] = c.coll;
+        }
+        if (data) {
+            float* d = data + (size_t)stride * i;
+            d[0] = c.point.x; d[1] = c.point.y; d[2] = c.point.z;
+            d[3] = c.normal.x; d[4] = c.normal.y; d[5] = c.normal.z;
+            for (int k = 0; k < nw; k++) d[6 + k] = c.w[k];
+            d[6 + nw] = c.lambda;
+            d[7 + nw] = c.lambdaF;
+        }
+    }
+    return (int)v.size();
+}
+
+int scGetEdgeContactData(scModel model, int* ids, float* data, int cap)
+{
+    return guard_r<int>(-1, [&] {
+        ClothModel* m = M(model);
+        m->download_contacts();
+        return feature_contacts(m->edgeContactsHost, 2, ids, data, cap);
+    });
+}
+
+int scGetTriContactData(scModel model, int* ids, float* data, int cap)
+{
+    return guard_r<int>(-1, [&] {
+        ClothModel* m = M(model);
+        m->download_contacts();
+        return feature_contacts(m->triContactsHost, 3, ids, data, cap);
     });
 }
 

@@ -17,6 +17,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "lbvh.h"
 #include "scmath.h"
 
 namespace sc {
@@ -34,10 +35,28 @@ struct SolverCtl {
     int iterations;
     float bendError;
     float contactError;
-    int pad;
+    unsigned ce;     // edge-contact error of the current iteration
+    unsigned ct;     // triangle-contact error of the current iteration
+    unsigned maxTv;  // largest |pos - prev| after prediction (mesh contacts)
+    int pad[2];
 };
 
 enum { MAX_COLLIDABLES = 8 };
+
+// A static triangle-mesh obstacle on the device: vertices, triangles (i1, i2,
+// i3, 0), angle-weighted vertex normals, edges (i1, i2) with their
+// pseudo-normals, and one LBVH each over triangles, vertices and edges.
+// `scale` is the largest absolute vertex coordinate (pruning slack).
+struct MeshGeo {
+    const float4* v;
+    const int4* tri;
+    const float4* vn;
+    const int2* edge;
+    const float4* en;
+    BvhView triBvh, vtxBvh, edgeBvh;
+    float scale;
+    int nv, nt, ne;
+};
 
 struct CollDev {
     int type;
@@ -46,6 +65,7 @@ struct CollDev {
     float bx, by, bz;  // walls: box max; sdf: cell steps
     float r;           // sphere radius
     const float4* nodes;
+    const MeshGeo* mesh;  // mesh obstacle (device copy)
     int nc0, nc1, nc2;
     int nv0, nv1;
 };
@@ -60,9 +80,31 @@ struct ContactBuf {
     int* count;
     float4* P;
     float4* N;
-    int* src;  // producing collidable's world index, per slot
+    int* src;   // producing collidable's world index, per slot
+    int* feat;  // obstacle triangle of a mesh contact, -1 otherwise, per slot
+    const MeshGeo* geo;  // per collidable index; null when the world has no mesh
     int nv;
     int slots;
+};
+
+// Obstacle-edge contacts per cloth edge and obstacle-vertex contacts per cloth
+// triangle (Physics/ClothTypes.h:110-120), slot-major like ContactBuf. Cloth
+// edges (triangles) carrying a contact are listed per colour group: group g
+// holds up to groupBase[g+1] - groupBase[g] entries, groupCount[g] used; the
+// groups are processed in order, their members in parallel.
+struct FeatureContactBuf {
+    int* count;        // per cloth edge / triangle
+    float4* P;         // point, lambda
+    float4* N;         // normal, lambdaF
+    float4* W;         // w1, w2 (, w3)
+    int2* F;           // obstacle feature, collidable
+    const int* group;  // colour group of each cloth edge / triangle
+    const int* groupBase;
+    int* groupCount;
+    int* list;
+    int n;             // cloth edges / triangles
+    int slots;
+    int groups;
 };
 
 struct TriStreams {
@@ -99,7 +141,7 @@ struct SlabContacts {
 void launch_predict(hipStream_t s, float4* pos, float4* prev, float4* vel, int n, float h, f3 gdv, float drag,
                     SolverCtl* ctl);
 void launch_detect(hipStream_t s, const float4* pos, const float4* prev, int n, const CollSet& cs, ContactBuf cb,
-                   float thickness, float tolerance, int collFlags);
+                   float thickness, float tolerance, int collFlags, const SolverCtl* ctl);
 void launch_fem(hipStream_t s, float4* pos, TriStreams tr, int start, int count, float h2, float epsUV,
                 const SolverCtl* ctl);
 void launch_bend(hipStream_t s, float4* pos, const int4* bidx, int start, int count, float stiff, SolverCtl* ctl);
@@ -109,14 +151,27 @@ void launch_predict_detect(hipStream_t s, float4* pos, float4* prev, float4* vel
                            SolverCtl* ctl, const CollSet& cs, ContactBuf cb, float thickness, float tolerance,
                            int collFlags);
 void launch_contacts_velocity(hipStream_t s, float4* pos, const float4* prev, float4* vel, ContactBuf cb,
-                              float thickness, float mu, float invH);
+                              float thickness, float mu, float invH, SolverCtl* ctl);
 void launch_bend_slab(hipStream_t s, float4* pos, const ushort4* lidx, SlabSet ss, int numSlabs, int maxVerts,
                       float stiff, SolverCtl* ctl);
 void launch_contacts(hipStream_t s, float4* pos, const float4* prev, float4* vel, ContactBuf cb, float thickness,
                      float mu, float invH, SolverCtl* ctl);
 void launch_iter_end(hipStream_t s, SolverCtl* ctl, int maxIter, int fixedIter, int bendFreq, int dihedral,
                      float selfThreshold);
-void launch_velocity(hipStream_t s, float4* pos, const float4* prev, float4* vel, int n, float invH);
+void launch_velocity(hipStream_t s, float4* pos, const float4* prev, float4* vel, int n, float invH, SolverCtl* ctl);
 void launch_copy(hipStream_t s, const float4* src, float4* dst, long n);
+
+// Mesh obstacles (mesh.hip). Prediction that also records max |pos - prev|
+// and clears the feature-contact groups; detection of obstacle-edge and
+// obstacle-vertex contacts; their Gauss-Seidel passes.
+void launch_predict_mesh(hipStream_t s, float4* pos, float4* prev, float4* vel, int n, float h, f3 gdv, float drag,
+                         SolverCtl* ctl, FeatureContactBuf ec, FeatureContactBuf tc);
+void launch_detect_features(hipStream_t s, const float4* pos, const float4* prev, const int2* cEdges,
+                            const int4* cTris, const CollSet& cs, FeatureContactBuf ec, FeatureContactBuf tc,
+                            float thickness, float tolerance, int collFlags, const SolverCtl* ctl);
+void launch_edge_contacts(hipStream_t s, float4* pos, const float4* prev, const int2* cEdges, FeatureContactBuf ec,
+                          const MeshGeo* geo, float thickness, float mu, SolverCtl* ctl);
+void launch_tri_contacts(hipStream_t s, float4* pos, const float4* prev, const int4* cTris, FeatureContactBuf tc,
+                         const MeshGeo* geo, float thickness, float mu, SolverCtl* ctl);
 
 }  // namespace sc

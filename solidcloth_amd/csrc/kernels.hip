@@ -7,22 +7,12 @@
 // same bits as the reference processing them one after another.
 #include "kernels.h"
 
+#include "bvh_query.h"
+#include "devutil.h"
+
 namespace sc {
 
 static constexpr int kBlock = 256;
-
-__device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
-__device__ __forceinline__ float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
-
-// Wave-wide maximum of non-negative, NaN-free values, then one atomic per wave.
-__device__ __forceinline__ void wave_max_atomic(unsigned* dst, float v)
-{
-    for (int o = 32; o > 0; o >>= 1) {
-        float w = __shfl_xor(v, o, 64);
-        v = v < w ? w : v;
-    }
-    if ((threadIdx.x & 63) == 0 && v > 0.f) atomicMax(dst, __float_as_uint(v));
-}
 
 // Physics/ClothPBD.cpp:258-266 — symplectic Euler prediction. Thread 0 also
 // resets the solver control block for the substep.
@@ -35,6 +25,8 @@ __global__ void k_predict(float4* __restrict__ pos, float4* __restrict__ prev, f
         ctl->k = 0;
         ctl->cb = 0;
         ctl->cv = 0;
+        ctl->ce = 0;
+        ctl->ct = 0;
         ctl->iterations = 0;
     }
     if (i >= n) return;
@@ -86,19 +78,23 @@ __device__ __forceinline__ bool sdf_lookup(const CollDev& c, f3 p, float& val, f
     return true;
 }
 
-__device__ __forceinline__ void put_contact(ContactBuf cb, int v, int& cnt, f3 p, f3 n, int ci)
+__device__ __forceinline__ void put_contact(ContactBuf cb, int v, int& cnt, f3 p, f3 n, int ci, int feat = -1)
 {
     cb.P[cnt * cb.nv + v] = f4(p, 0.f);
     cb.N[cnt * cb.nv + v] = f4(n, 0.f);
     cb.src[cnt * cb.nv + v] = ci;
+    cb.feat[cnt * cb.nv + v] = feat;
     cnt++;
 }
 
-// Physics/ClothCollision.cpp:50-87, 199-220, 430-470 — external vertex
+// Physics/ClothCollision.cpp:50-135, 199-220, 430-470 — external vertex
 // contacts of vertex v (predicted position p, previous position pv),
-// collidables visited in world order.
+// collidables visited in world order. `meshRad` is the mesh query radius
+// thickness + max(tolerance, 1.5 max_i |pos_i - prev_i|) (ClothCollision.cpp:95-104).
+// Instantiated without the mesh branch for worlds that have no mesh.
+template <bool kMesh>
 __device__ __forceinline__ void detect_vertex(int v, const f3 p, const f3 pv, const CollSet& cs, ContactBuf cb,
-                                              float thickness, float tolerance, int collFlags)
+                                              float thickness, float tolerance, int collFlags, float meshRad)
 {
     int cnt = 0;
     const float tv = len(sub(pv, p));
@@ -132,17 +128,23 @@ __device__ __forceinline__ void detect_vertex(int v, const f3 p, const f3 pv, co
             g = normalized(g);
             f3 cp = sub(pv, mul(val, g));
             if (val < radTol && val >= 0.f) put_contact(cb, v, cnt, cp, g, ci);
+        } else if (kMesh && c.type == CT_MESH) {
+            if (!(collFlags & CF_VERTICES)) continue;
+            NearestResult r;
+            if (nearest_triangle(*c.mesh, pv, meshRad, r)) put_contact(cb, v, cnt, r.point, r.normal, ci, r.feature);
         }
     }
     cb.count[v] = cnt;
 }
 
 __global__ void k_detect(const float4* __restrict__ pos, const float4* __restrict__ prev, int n, CollSet cs,
-                         ContactBuf cb, float thickness, float tolerance, int collFlags)
+                         ContactBuf cb, float thickness, float tolerance, int collFlags,
+                         const SolverCtl* __restrict__ ctl)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    detect_vertex(v, xyz(pos[v]), xyz(prev[v]), cs, cb, thickness, tolerance, collFlags);
+    const float meshRad = thickness + smax(tolerance, __uint_as_float(ctl->maxTv) * 1.5f);
+    detect_vertex<true>(v, xyz(pos[v]), xyz(prev[v]), cs, cb, thickness, tolerance, collFlags, meshRad);
 }
 
 // Prediction (ClothPBD.cpp:258-266) and detection of the same vertex in one
@@ -158,6 +160,8 @@ __global__ void k_predict_detect(float4* __restrict__ pos, float4* __restrict__ 
         ctl->k = 0;
         ctl->cb = 0;
         ctl->cv = 0;
+        ctl->ce = 0;
+        ctl->ct = 0;
         ctl->iterations = 0;
     }
     if (i >= n) return;
@@ -172,7 +176,7 @@ __global__ void k_predict_detect(float4* __restrict__ pos, float4* __restrict__ 
         vel[i] = f4(v, 0.f);
         pos[i] = f4(p, P.w);
     }
-    detect_vertex(i, p, pv, cs, cb, thickness, tolerance, collFlags);
+    detect_vertex<false>(i, p, pv, cs, cb, thickness, tolerance, collFlags, 0.f);
 }
 
 // Physics/ClothPBD.cpp:52-108 — one FEM triangle projection (warp, weft,
@@ -323,11 +327,33 @@ __device__ __forceinline__ float vertex_contacts(int v, int n, f3& p, const f3 p
     for (int s = 0; s < n; s++) {
         float4 CP = cb.P[s * cb.nv + v];
         float4 CN = cb.N[s * cb.nv + v];
+        if (cb.geo) {
+            const int f = cb.feat[s * cb.nv + v];
+            if (f >= 0) {
+                // ClothCollisionHandler.cpp:145-160: re-project onto the same
+                // obstacle triangle, normal interpolated from its vertex
+                // normals (ComputeTriangleNormal, ClosestPointOnMesh.cpp:37-62)
+                const MeshGeo& g = cb.geo[cb.src[s * cb.nv + v]];
+                const int4 ix = g.tri[f];
+                f3 cp, co;
+                closest_pt_point_triangle(p, xyz(g.v[ix.x]), xyz(g.v[ix.y]), xyz(g.v[ix.z]), cp, co);
+                f3 nn = add(add(mul(co.x, xyz(g.vn[ix.x])), mul(co.y, xyz(g.vn[ix.y]))), mul(co.z, xyz(g.vn[ix.z])));
+                nn = normalized(nn);
+                CP = f4(cp, CP.w);
+                CN = f4(nn, CN.w);
+            }
+        }
         f3 cpt = xyz(CP), nn = xyz(CN);
         float lam = CP.w, lamF = CN.w;
         f3 delta = sub(p, cpt);
         float l = dot(delta, nn);
-        if (l > thickness) continue;
+        if (l > thickness) {
+            if (cb.geo) {
+                cb.P[s * cb.nv + v] = CP;
+                cb.N[s * cb.nv + v] = CN;
+            }
+            continue;
+        }
         float depth = thickness - l;
         if (fabsf(depth) > err) err = fabsf(depth);
         lam += depth;
@@ -519,9 +545,10 @@ __global__ void __launch_bounds__(kBlock) k_contacts(float4* __restrict__ pos, c
 // (ClothPBD.cpp:348, 314-321), one thread per vertex.
 __global__ void __launch_bounds__(kBlock) k_contacts_velocity(float4* __restrict__ pos, const float4* __restrict__ prev,
                                                               float4* __restrict__ vel, ContactBuf cb, float thickness,
-                                                              float mu, float invH)
+                                                              float mu, float invH, SolverCtl* ctl)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v == 0) ctl->maxTv = 0;  // accumulated afresh by the next prediction
     if (v >= cb.nv) return;
     float4 P = pos[v];
     const f3 pp = xyz(prev[v]);
@@ -537,15 +564,16 @@ __global__ void __launch_bounds__(kBlock) k_contacts_velocity(float4* __restrict
 }
 
 // Physics/ClothPBD.cpp:354-361 — iteration counter and the early-exit test.
-// Edge, triangle and self contacts are not generated on this path and the
-// FEM model reports no link error, so those terms of the test are 0.
+// Self contacts are not generated on this path and the FEM model reports no
+// link error, so those terms of the test are 0.
 __global__ void k_iter_end(SolverCtl* ctl, int maxIter, int fixedIter, int bendFreq, int dihedral, float selfThr)
 {
     if (ctl->done) return;
     int k = ctl->k + 1;
     ctl->k = k;
     float cb = __uint_as_float(ctl->cb), cv = __uint_as_float(ctl->cv);
-    const float cl = 0.f, ce = 0.f, ct = 0.f, sct = 0.f, sce = 0.f, eps = 0.01f;
+    const float ce = __uint_as_float(ctl->ce), ct = __uint_as_float(ctl->ct);
+    const float cl = 0.f, sct = 0.f, sce = 0.f, eps = 0.01f;
     bool exitNow = !fixedIter && cl < 0.05f && cb < 0.005f && sce < selfThr && sct < selfThr && cv < eps &&
                    ce < eps && ct < eps;
     if (exitNow || k >= maxIter) {
@@ -556,14 +584,17 @@ __global__ void k_iter_end(SolverCtl* ctl, int maxIter, int fixedIter, int bendF
         return;
     }
     ctl->cv = 0;
+    ctl->ce = 0;
+    ctl->ct = 0;
     if (dihedral && k % bendFreq == 0) ctl->cb = 0;
 }
 
 // Physics/ClothPBD.cpp:314-321
 __global__ void k_velocity(const float4* __restrict__ pos, const float4* __restrict__ prev, float4* __restrict__ vel,
-                           int n, float invH)
+                           int n, float invH, SolverCtl* ctl)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) ctl->maxTv = 0;  // accumulated afresh by the next prediction
     if (i >= n) return;
     float4 P = pos[i];
     if (P.w == 0.f) return;
@@ -597,11 +628,11 @@ void launch_predict(hipStream_t s, float4* pos, float4* prev, float4* vel, int n
 }
 
 void launch_detect(hipStream_t s, const float4* pos, const float4* prev, int n, const CollSet& cs, ContactBuf cb,
-                   float thickness, float tolerance, int collFlags)
+                   float thickness, float tolerance, int collFlags, const SolverCtl* ctl)
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_detect, dim3(blocks(n)), dim3(kBlock), 0, s, pos, prev, n, cs, cb, thickness, tolerance,
-                       collFlags);
+                       collFlags, ctl);
 }
 
 void launch_fem(hipStream_t s, float4* pos, TriStreams tr, int start, int count, float h2, float epsUV,
@@ -629,11 +660,10 @@ void launch_predict_detect(hipStream_t s, float4* pos, float4* prev, float4* vel
 }
 
 void launch_contacts_velocity(hipStream_t s, float4* pos, const float4* prev, float4* vel, ContactBuf cb,
-                              float thickness, float mu, float invH)
+                              float thickness, float mu, float invH, SolverCtl* ctl)
 {
-    if (cb.nv <= 0) return;
-    hipLaunchKernelGGL(k_contacts_velocity, dim3(blocks(cb.nv)), dim3(kBlock), 0, s, pos, prev, vel, cb, thickness,
-                       mu, invH);
+    hipLaunchKernelGGL(k_contacts_velocity, dim3(blocks(cb.nv > 0 ? cb.nv : 1)), dim3(kBlock), 0, s, pos, prev, vel,
+                       cb, thickness, mu, invH, ctl);
 }
 
 void launch_bend_slab(hipStream_t s, float4* pos, const ushort4* lidx, SlabSet ss, int numSlabs, int maxVerts,
@@ -665,10 +695,9 @@ void launch_iter_end(hipStream_t s, SolverCtl* ctl, int maxIter, int fixedIter, 
                        selfThreshold);
 }
 
-void launch_velocity(hipStream_t s, float4* pos, const float4* prev, float4* vel, int n, float invH)
+void launch_velocity(hipStream_t s, float4* pos, const float4* prev, float4* vel, int n, float invH, SolverCtl* ctl)
 {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_velocity, dim3(blocks(n)), dim3(kBlock), 0, s, pos, prev, vel, n, invH);
+    hipLaunchKernelGGL(k_velocity, dim3(blocks(n > 0 ? n : 1)), dim3(kBlock), 0, s, pos, prev, vel, n, invH, ctl);
 }
 
 }  // namespace sc

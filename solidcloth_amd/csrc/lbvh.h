@@ -1,0 +1,49 @@
+// Linear BVH over axis-aligned boxes, built on the GPU:
+//   1. scene bounds of the box centroids (block reduction + atomics),
+//   2. 30-bit Morton codes of the centroids,
+//   3. stable LSD radix sort of (code, primitive id), four 8-bit passes; the
+//      scatter ranks equal digits inside a wavefront with ballots and across
+//      wavefronts with LDS prefix sums, so the order is deterministic,
+//   4. Karras (HPG 2012) hierarchy: internal node i covers a key range found by
+//      binary search on common-prefix lengths (ties broken by index),
+//   5. bottom-up refit: the second child to arrive at a node (atomic counter,
+//      acquire/release at agent scope) writes its box and continues upward.
+// Internal node k stores (min, left child) and (max, right child); a child
+// c >= 0 is internal node c, c < 0 is leaf ~c, i.e. sorted position ~c.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace sc {
+
+struct BvhView {
+    const float4* lo;     // internal nodes: xyz = box min, w = left child (int bits)
+    const float4* hi;     // internal nodes: xyz = box max, w = right child
+    const float4* leafLo; // leaf boxes in sorted order
+    const float4* leafHi;
+    const int* leafPrim;  // sorted position -> primitive id
+    int n;                // number of primitives (leaves)
+};
+
+class Lbvh {
+public:
+    ~Lbvh();
+    // Builds over n boxes (device arrays, xyz used). Asynchronous on `s`;
+    // allocates on the first build and when n grows.
+    void build(hipStream_t s, const float4* boxLo, const float4* boxHi, int n);
+    BvhView view() const;
+    int size() const { return n_; }
+
+private:
+    void reserve(int n);
+    int n_ = 0, cap_ = 0, tiles_ = 0;
+    float4 *lo_ = nullptr, *hi_ = nullptr, *leafLo_ = nullptr, *leafHi_ = nullptr;
+    int* leafPrim_ = nullptr;
+    unsigned *keys_ = nullptr, *keys2_ = nullptr;
+    int *vals_ = nullptr, *vals2_ = nullptr;
+    unsigned* hist_ = nullptr;
+    int *parentNode_ = nullptr, *parentLeaf_ = nullptr, *flags_ = nullptr;
+    unsigned* bounds_ = nullptr;  // 6 orderable-encoded floats
+};
+
+}  // namespace sc

@@ -1,0 +1,312 @@
+// gfx950 kernels for triangle-mesh obstacles: prediction with the travel
+// reduction the mesh query radius needs, detection of obstacle-edge and
+// obstacle-vertex contacts, and their Gauss-Seidel passes. Arithmetic follows
+// the reference's operation order (see kernels.hip).
+#include "bvh_query.h"
+#include "devutil.h"
+#include "kernels.h"
+
+namespace sc {
+
+static constexpr int kBlock = 256;
+static constexpr int kSolveBlock = 1024;
+
+// Physics/ClothPBD.cpp:258-266 plus max_i |pos_i - prev_i|
+// (ClothCollision.cpp:95-104). Thread 0 resets the control block, the first
+// threads clear the contact groups of the substep.
+__global__ void __launch_bounds__(kBlock) k_predict_mesh(float4* __restrict__ pos, float4* __restrict__ prev,
+                                                         float4* __restrict__ vel, int n, float h, f3 gdv, float drag,
+                                                         SolverCtl* ctl, FeatureContactBuf ec, FeatureContactBuf tc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        ctl->done = 0;
+        ctl->k = 0;
+        ctl->cb = 0;
+        ctl->cv = 0;
+        ctl->ce = 0;
+        ctl->ct = 0;
+        ctl->iterations = 0;
+    }
+    if (i < ec.groups) ec.groupCount[i] = 0;
+    if (i < tc.groups) tc.groupCount[i] = 0;
+    float tv = 0.f;
+    if (i < n) {
+        const float4 P = pos[i];
+        prev[i] = make_float4(P.x, P.y, P.z, 0.f);
+        if (P.w != 0.f) {
+            f3 v = add(xyz(vel[i]), gdv);
+            v = f3{v.x * drag, v.y * drag, v.z * drag};
+            const f3 p = add(xyz(P), mul(h, v));
+            vel[i] = f4(v, 0.f);
+            pos[i] = f4(p, P.w);
+            tv = len(sub(p, xyz(P)));
+            tv = tv > 0.f ? tv : 0.f;  // a NaN never raises the reference's maximum
+        }
+    }
+    wave_max_atomic(&ctl->maxTv, tv);
+}
+
+__device__ __forceinline__ void group_append(const FeatureContactBuf& b, int id)
+{
+    const int g = b.group[id];
+    const int k = atomicAdd(&b.groupCount[g], 1);
+    b.list[b.groupBase[g] + k] = id;
+}
+
+// Physics/ClothCollision.cpp:137-197: per cloth edge the nearest obstacle
+// edge, per cloth triangle the nearest obstacle vertex, from the previous
+// positions, for every mesh obstacle in world order. Blocks [0, edgeBlocks)
+// take edges, the rest triangles.
+__global__ void __launch_bounds__(kBlock) k_detect_features(const float4* __restrict__ pos,
+                                                            const float4* __restrict__ prev,
+                                                            const int2* __restrict__ cEdges,
+                                                            const int4* __restrict__ cTris, CollSet cs,
+                                                            FeatureContactBuf ec, FeatureContactBuf tc,
+                                                            int edgeBlocks, float thickness, float tolerance,
+                                                            const SolverCtl* __restrict__ ctl)
+{
+    if ((int)blockIdx.x < edgeBlocks) {
+        const int e = blockIdx.x * blockDim.x + threadIdx.x;
+        if (e >= ec.n) return;
+        const int2 ev = cEdges[e];
+        const f3 p = xyz(prev[ev.x]), q = xyz(prev[ev.y]);
+        const float radius = thickness + smax(tolerance, __uint_as_float(ctl->maxTv) * 1.5f);
+        int cnt = 0;
+        for (int ci = 0; ci < cs.n; ci++) {
+            if (cs.c[ci].type != CT_MESH) continue;
+            NearestResult r;
+            if (!nearest_edge(*cs.c[ci].mesh, p, q, radius, r)) continue;
+            const int k = cnt * ec.n + e;
+            ec.P[k] = f4(r.point, 0.f);
+            ec.N[k] = f4(r.normal, 0.f);
+            ec.W[k] = make_float4(1.0f - r.param, r.param, 0.f, 0.f);
+            ec.F[k] = make_int2(r.feature, ci);
+            cnt++;
+        }
+        ec.count[e] = cnt;
+        if (cnt) group_append(ec, e);
+    } else {
+        const int t = (blockIdx.x - edgeBlocks) * blockDim.x + threadIdx.x;
+        if (t >= tc.n) return;
+        const int4 ix = cTris[t];
+        const float4 A = prev[ix.x], B = prev[ix.y], C = prev[ix.z];
+        const float tv1 = len(sub(xyz(pos[ix.x]), xyz(A)));
+        const float tv2 = len(sub(xyz(pos[ix.y]), xyz(B)));
+        const float tv3 = len(sub(xyz(pos[ix.z]), xyz(C)));
+        const float tv = smax(tv1, smax(tv2, tv3));
+        const float radius = thickness + smax(tolerance, tv * 1.2f);
+        int cnt = 0;
+        for (int ci = 0; ci < cs.n; ci++) {
+            if (cs.c[ci].type != CT_MESH) continue;
+            NearestResult r;
+            if (!nearest_vertex(*cs.c[ci].mesh, xyz(A), xyz(B), xyz(C), radius, r)) continue;
+            const int k = cnt * tc.n + t;
+            tc.P[k] = f4(r.point, 0.f);
+            tc.N[k] = f4(r.normal, 0.f);
+            tc.W[k] = make_float4(r.bary.x, r.bary.y, r.bary.z, 0.f);
+            tc.F[k] = make_int2(r.feature, ci);
+            cnt++;
+        }
+        tc.count[t] = cnt;
+        if (cnt) group_append(tc, t);
+    }
+}
+
+// Friction and position update shared by edge and triangle contacts
+// (ClothCollisionHandler.cpp:236-286, 318-361). Returns the displacement
+// factors; the caller applies them per vertex in the reference's order.
+struct FeatureStep {
+    f3 pf;      // friction displacement (zero when not applied)
+    bool fric;  // friction applied
+    f3 disp;
+};
+
+__device__ __forceinline__ FeatureStep feature_step(f3 normal, float s, float depth, f3 p, f3 p0, float mu,
+                                                    float& lambda, float& lambdaF)
+{
+    FeatureStep st;
+    float dLambda = s * depth;
+    lambda += dLambda;
+    st.disp = mul(dLambda, normal);
+    st.fric = false;
+    st.pf = f3{0.f, 0.f, 0.f};
+    if (mu != 0.f) {
+        const f3 v12 = sub(p, p0);
+        const float vnrel = dot(normal, v12);
+        f3 vt = sub(v12, mul(vnrel, normal));
+        const float vtrel = len(vt);
+        const float limit = mu * lambda;
+        if (vtrel > 0.001f) {
+            const float dLF = vtrel;
+            const float l0 = lambdaF;
+            lambdaF = l0 + dLF;
+            if (lambdaF >= limit) lambdaF = limit;
+            dLambda = lambdaF - l0;
+            vt = mul(1.f / vtrel, vt);
+            st.pf = mul(-s * dLambda, vt);
+            st.fric = true;
+        }
+    }
+    return st;
+}
+
+// Physics/ClothCollisionHandler.cpp:289-362 — obstacle-edge contacts. One
+// workgroup walks the colour groups in order; the cloth edges of a group share
+// no vertex, so their contacts are solved in parallel.
+__global__ void __launch_bounds__(kSolveBlock) k_edge_contacts(float4* __restrict__ pos,
+                                                               const float4* __restrict__ prev,
+                                                               const int2* __restrict__ cEdges, FeatureContactBuf ec,
+                                                               const MeshGeo* __restrict__ geo, float len0, float mu,
+                                                               SolverCtl* ctl)
+{
+    if (ctl->done) return;
+    float err = 0.f;
+    for (int g = 0; g < ec.groups; g++) {
+        const int cnt = ec.groupCount[g], base = ec.groupBase[g];
+        for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+            const int e = ec.list[base + k];
+            const int2 ev = cEdges[e];
+            const float4 X1 = pos[ev.x], X2 = pos[ev.y];
+            f3 x1 = xyz(X1), x2 = xyz(X2);
+            const float m1 = X1.w, m2 = X2.w;
+            const int ns = ec.count[e];
+            for (int sl = 0; sl < ns; sl++) {
+                const int q = sl * ec.n + e;
+                const float4 CP = ec.P[q], CN = ec.N[q];
+                const int2 F = ec.F[q];
+                const MeshGeo& M = geo[F.y];
+                const int2 me = M.edge[F.x];
+                float s1, t;
+                f3 point, p;
+                closest_pt_segm_segm(xyz(M.v[me.x]), xyz(M.v[me.y]), x1, x2, s1, t, point, p);
+                const float w1 = 1.0f - t, w2 = t;
+                const f3 normal = xyz(CN);
+                float lambda = CP.w, lambdaF = CN.w;
+                const float l = dot(normal, sub(p, point));
+                if (!(l > len0)) {
+                    const float s = 1.f / (w1 * w1 / m1 + w2 * w2 / m2);
+                    const float depth = len0 - l;
+                    if (depth > err) err = depth;
+                    const f3 p0 = add(mul(w1, xyz(prev[ev.x])), mul(w2, xyz(prev[ev.y])));
+                    const FeatureStep st = feature_step(normal, s, depth, p, p0, mu, lambda, lambdaF);
+                    if (st.fric) {
+                        x1 = add(x1, mul(w1 * m1, st.pf));
+                        x2 = add(x2, mul(w2 * m2, st.pf));
+                    }
+                    x1 = add(x1, mul(w1 * m1, st.disp));
+                    x2 = add(x2, mul(w2 * m2, st.disp));
+                }
+                ec.P[q] = f4(point, lambda);
+                ec.N[q] = make_float4(CN.x, CN.y, CN.z, lambdaF);
+                ec.W[q] = make_float4(w1, w2, 0.f, 0.f);
+            }
+            pos[ev.x] = f4(x1, m1);
+            pos[ev.y] = f4(x2, m2);
+        }
+        __syncthreads();
+    }
+    wave_max_atomic(&ctl->ce, err);
+}
+
+// Physics/ClothCollisionHandler.cpp:204-287 — obstacle-vertex contacts on
+// cloth triangles, by colour group as above.
+__global__ void __launch_bounds__(kSolveBlock) k_tri_contacts(float4* __restrict__ pos,
+                                                              const float4* __restrict__ prev,
+                                                              const int4* __restrict__ cTris, FeatureContactBuf tc,
+                                                              const MeshGeo* __restrict__ geo, float len0, float mu,
+                                                              SolverCtl* ctl)
+{
+    if (ctl->done) return;
+    float err = 0.f;
+    for (int g = 0; g < tc.groups; g++) {
+        const int cnt = tc.groupCount[g], base = tc.groupBase[g];
+        for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+            const int t = tc.list[base + k];
+            const int4 ix = cTris[t];
+            const float4 X1 = pos[ix.x], X2 = pos[ix.y], X3 = pos[ix.z];
+            f3 x1 = xyz(X1), x2 = xyz(X2), x3 = xyz(X3);
+            const float m1 = X1.w, m2 = X2.w, m3 = X3.w;
+            const int ns = tc.count[t];
+            for (int sl = 0; sl < ns; sl++) {
+                const int q = sl * tc.n + t;
+                const float4 CP = tc.P[q], CN = tc.N[q];
+                const int2 F = tc.F[q];
+                const f3 point = xyz(CP);
+                f3 p, co;
+                closest_pt_point_triangle(point, x1, x2, x3, p, co);
+                const float w1 = co.x, w2 = co.y, w3 = co.z;
+                f3 n = normalized(neg(cross(sub(x2, x1), sub(x3, x1))));
+                f3 normal = n;
+                if (F.x >= 0) {
+                    const f3 mn = xyz(geo[F.y].vn[F.x]);
+                    if (dot(n, mn) < 0.f) normal = neg(n);
+                }
+                float lambda = CP.w, lambdaF = CN.w;
+                const float l = dot(normal, sub(p, point));
+                if (!(l > len0)) {
+                    const float s = 1.f / (w1 * w1 / m1 + w2 * w2 / m2 + w3 * w3 / m3);
+                    const float depth = len0 - l;
+                    if (depth > err) err = depth;
+                    const f3 p0 =
+                        add(add(mul(w1, xyz(prev[ix.x])), mul(w2, xyz(prev[ix.y]))), mul(w3, xyz(prev[ix.z])));
+                    const FeatureStep st = feature_step(normal, s, depth, p, p0, mu, lambda, lambdaF);
+                    if (st.fric) {
+                        x1 = add(x1, mul(w1 * m1, st.pf));
+                        x2 = add(x2, mul(w2 * m2, st.pf));
+                        x3 = add(x3, mul(w3 * m3, st.pf));
+                    }
+                    x1 = add(x1, mul(w1 * m1, st.disp));
+                    x2 = add(x2, mul(w2 * m2, st.disp));
+                    x3 = add(x3, mul(w3 * m3, st.disp));
+                }
+                tc.P[q] = make_float4(CP.x, CP.y, CP.z, lambda);
+                tc.N[q] = f4(normal, lambdaF);
+                tc.W[q] = make_float4(w1, w2, w3, 0.f);
+            }
+            pos[ix.x] = f4(x1, m1);
+            pos[ix.y] = f4(x2, m2);
+            pos[ix.z] = f4(x3, m3);
+        }
+        __syncthreads();
+    }
+    wave_max_atomic(&ctl->ct, err);
+}
+
+static inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
+
+void launch_predict_mesh(hipStream_t s, float4* pos, float4* prev, float4* vel, int n, float h, f3 gdv, float drag,
+                         SolverCtl* ctl, FeatureContactBuf ec, FeatureContactBuf tc)
+{
+    int m = n > ec.groups ? n : ec.groups;
+    m = m > tc.groups ? m : tc.groups;
+    hipLaunchKernelGGL(k_predict_mesh, dim3(blocks(m > 0 ? m : 1)), dim3(kBlock), 0, s, pos, prev, vel, n, h, gdv,
+                       drag, ctl, ec, tc);
+}
+
+void launch_detect_features(hipStream_t s, const float4* pos, const float4* prev, const int2* cEdges,
+                            const int4* cTris, const CollSet& cs, FeatureContactBuf ec, FeatureContactBuf tc,
+                            float thickness, float tolerance, int collFlags, const SolverCtl* ctl)
+{
+    const int eb = (collFlags & CF_EDGES) && ec.slots > 0 ? blocks(ec.n) : 0;
+    const int tb = (collFlags & CF_TRIANGLES) && tc.slots > 0 ? blocks(tc.n) : 0;
+    if (eb + tb == 0) return;
+    hipLaunchKernelGGL(k_detect_features, dim3(eb + tb), dim3(kBlock), 0, s, pos, prev, cEdges, cTris, cs, ec, tc,
+                       eb, thickness, tolerance, ctl);
+}
+
+void launch_edge_contacts(hipStream_t s, float4* pos, const float4* prev, const int2* cEdges, FeatureContactBuf ec,
+                          const MeshGeo* geo, float thickness, float mu, SolverCtl* ctl)
+{
+    hipLaunchKernelGGL(k_edge_contacts, dim3(1), dim3(kSolveBlock), 0, s, pos, prev, cEdges, ec, geo, thickness, mu,
+                       ctl);
+}
+
+void launch_tri_contacts(hipStream_t s, float4* pos, const float4* prev, const int4* cTris, FeatureContactBuf tc,
+                         const MeshGeo* geo, float thickness, float mu, SolverCtl* ctl)
+{
+    hipLaunchKernelGGL(k_tri_contacts, dim3(1), dim3(kSolveBlock), 0, s, pos, prev, cTris, tc, geo, thickness, mu,
+                       ctl);
+}
+
+}  // namespace sc

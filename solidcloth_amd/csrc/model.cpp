@@ -8,29 +8,9 @@
 #include <cstring>
 #include <stdexcept>
 
+#include "hiputil.h"
+
 namespace sc {
-
-#define HIP_OK(expr)                                                                                  \
-    do {                                                                                              \
-        hipError_t e_ = (expr);                                                                       \
-        if (e_ != hipSuccess)                                                                         \
-            throw std::runtime_error(std::string(#expr) + " failed: " + hipGetErrorString(e_));      \
-    } while (0)
-
-template <class T>
-static void dev_alloc(T*& p, size_t n)
-{
-    if (p) HIP_OK(hipFree(p));
-    p = nullptr;
-    if (n) HIP_OK(hipMalloc((void**)&p, n * sizeof(T)));
-}
-
-template <class T>
-static void dev_free(T*& p)
-{
-    if (p) (void)hipFree(p);
-    p = nullptr;
-}
 
 // ---------------------------------------------------------------- SDF ----
 
@@ -90,6 +70,8 @@ void SDF::upload()
 void Mesh::clear()
 {
     vertices.clear(); indices.clear(); normals.clear(); uvs.clear(); edges.clear();
+    triNormals.clear(); edgeNormals.clear();
+    dev.reset();
 }
 
 // Geometry/Mesh.cpp:65-147. The duplicate list is sorted with std::sort by
@@ -127,12 +109,14 @@ void Mesh::construct_edges()
 void Mesh::compute_normals()
 {
     normals.assign(vertices.size(), scVector3{0.f, 0.f, 0.f});
+    triNormals.assign(indices.size() / 3, scVector3{0.f, 0.f, 0.f});
     std::vector<f3> acc(vertices.size(), f3{0.f, 0.f, 0.f});
     auto V = [&](unsigned i) { return f3{vertices[i].x, vertices[i].y, vertices[i].z}; };
     for (size_t i = 0; i + 2 < indices.size(); i += 3) {
         unsigned i1 = indices[i], i2 = indices[i + 1], i3 = indices[i + 2];
         f3 v1 = V(i1), v2 = V(i2), v3 = V(i3);
         f3 nt = normalized(cross(sub(v2, v1), sub(v3, v1)));
+        triNormals[i / 3] = scVector3{nt.x, nt.y, nt.z};
         f3 e1 = normalized(sub(v2, v1)), e2 = normalized(sub(v3, v2)), e3 = normalized(sub(v1, v3));
         float c1 = -dot(e1, e3), c2 = -dot(e1, e2), c3 = -dot(e2, e3);
         float t1 = std::fabs(c1) > 1 ? std::fabs(std::atan2(len(cross(e1, neg(e3))), c1)) : std::acos(c1);
@@ -147,6 +131,80 @@ void Mesh::compute_normals()
         f3 n = normalized(acc[i]);
         normals[i] = scVector3{n.x, n.y, n.z};
     }
+}
+
+// Geometry/Mesh.cpp:105, 124-126: a border edge takes its triangle's normal,
+// an interior edge the normalised sum of both (the last duplicate's
+// triangle for a non-manifold edge, as construct_edges records it).
+void Mesh::compute_edge_normals()
+{
+    edgeNormals.assign(edges.size(), scVector3{0.f, 0.f, 0.f});
+    if (triNormals.size() != indices.size() / 3) compute_normals();
+    auto T = [&](int t) { return f3{triNormals[t].x, triNormals[t].y, triNormals[t].z}; };
+    for (size_t e = 0; e < edges.size(); e++) {
+        f3 n = T(edges[e].t1);
+        if (edges[e].t2 >= 0) n = normalized(add(T(edges[e].t1), T(edges[e].t2)));
+        edgeNormals[e] = scVector3{n.x, n.y, n.z};
+    }
+}
+
+MeshDevice::~MeshDevice()
+{
+    dev_free(v); dev_free(vn); dev_free(en); dev_free(tri); dev_free(edge);
+    dev_free(triLo); dev_free(triHi); dev_free(vtxLo); dev_free(edgeLo); dev_free(edgeHi);
+}
+
+const MeshGeo& Mesh::device_geo(hipStream_t s)
+{
+    if (dev) return dev->geo;
+    if (edgeNormals.size() != edges.size()) compute_edge_normals();
+    std::unique_ptr<MeshDevice> d(new MeshDevice());
+    const int nv = (int)vertices.size(), nt = (int)indices.size() / 3, ne = (int)edges.size();
+    auto P = [&](unsigned i) { return make_float4(vertices[i].x, vertices[i].y, vertices[i].z, 0.f); };
+    std::vector<float4> hv(nv), hvn(nv), hen(ne), tlo(nt), thi(nt), elo(ne), ehi(ne);
+    std::vector<int4> ht(nt);
+    std::vector<int2> he(ne);
+    float scale = 0.f;
+    for (int i = 0; i < nv; i++) {
+        hv[i] = P(i);
+        hvn[i] = make_float4(normals[i].x, normals[i].y, normals[i].z, 0.f);
+        scale = std::max(scale, std::max(std::fabs(vertices[i].x), std::max(std::fabs(vertices[i].y),
+                                                                            std::fabs(vertices[i].z))));
+    }
+    for (int t = 0; t < nt; t++) {
+        const unsigned a = indices[3 * t], b = indices[3 * t + 1], c = indices[3 * t + 2];
+        ht[t] = make_int4((int)a, (int)b, (int)c, 0);
+        const float4 A = P(a), B = P(b), C = P(c);
+        tlo[t] = make_float4(std::min(std::min(A.x, B.x), C.x), std::min(std::min(A.y, B.y), C.y),
+                             std::min(std::min(A.z, B.z), C.z), 0.f);
+        thi[t] = make_float4(std::max(std::max(A.x, B.x), C.x), std::max(std::max(A.y, B.y), C.y),
+                             std::max(std::max(A.z, B.z), C.z), 0.f);
+    }
+    for (int e = 0; e < ne; e++) {
+        he[e] = make_int2((int)edges[e].i1, (int)edges[e].i2);
+        hen[e] = make_float4(edgeNormals[e].x, edgeNormals[e].y, edgeNormals[e].z, 0.f);
+        const float4 A = P(edges[e].i1), B = P(edges[e].i2);
+        elo[e] = make_float4(std::min(A.x, B.x), std::min(A.y, B.y), std::min(A.z, B.z), 0.f);
+        ehi[e] = make_float4(std::max(A.x, B.x), std::max(A.y, B.y), std::max(A.z, B.z), 0.f);
+    }
+    auto up = [&](auto*& dst, const auto& src) {
+        dev_alloc(dst, src.size());
+        if (!src.empty())
+            HIP_OK(hipMemcpy(dst, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice));
+    };
+    up(d->v, hv); up(d->vn, hvn); up(d->en, hen); up(d->tri, ht); up(d->edge, he);
+    up(d->triLo, tlo); up(d->triHi, thi); up(d->vtxLo, hv); up(d->edgeLo, elo); up(d->edgeHi, ehi);
+    d->triB.build(s, d->triLo, d->triHi, nt);
+    d->vtxB.build(s, d->vtxLo, d->vtxLo, nv);
+    d->edgeB.build(s, d->edgeLo, d->edgeHi, ne);
+    HIP_OK(hipStreamSynchronize(s));
+    MeshGeo& g = d->geo;
+    g.v = d->v; g.tri = d->tri; g.vn = d->vn; g.edge = d->edge; g.en = d->en;
+    g.triBvh = d->triB.view(); g.vtxBvh = d->vtxB.view(); g.edgeBvh = d->edgeB.view();
+    g.scale = scale;
+    g.nv = nv; g.nt = nt; g.ne = ne;
+    dev = std::move(d);
+    return dev->geo;
 }
 
 // ------------------------------------------------------------- timing ----
@@ -198,7 +256,14 @@ void GpuSolver::release()
     graphExec = nullptr; graph = nullptr; graphKey.clear();
     dev_free(d_pos); dev_free(d_prev); dev_free(d_vel);
     dev_free(d_tidx); dev_free(d_tA); dev_free(d_tB); dev_free(d_tC); dev_free(d_bidx);
-    dev_free(d_cCount); dev_free(d_cP); dev_free(d_cN); dev_free(d_cSrc); dev_free(d_ctl);
+    dev_free(d_cCount); dev_free(d_cP); dev_free(d_cN); dev_free(d_cSrc); dev_free(d_cFeat); dev_free(d_ctl);
+    dev_free(d_geo); dev_free(d_cEdges); dev_free(d_cTris);
+    geoHost.clear();
+    for (Features* f : {&ec, &tc}) {
+        dev_free(f->count); dev_free(f->P); dev_free(f->N); dev_free(f->W); dev_free(f->F);
+        dev_free(f->group); dev_free(f->groupBase); dev_free(f->groupCount); dev_free(f->list);
+        f->slotCap = 0;
+    }
     for (DeviceSlabs* d : {&triSlabs, &bendSlabs}) {
         dev_free(d->lidx); dev_free(d->info); dev_free(d->colOff); dev_free(d->verts);
     }
@@ -370,6 +435,38 @@ void ClothModel::build_device_topology()
     dev_free(g.d_cP);
     dev_free(g.d_cN);
     dev_free(g.d_cSrc);
+    dev_free(g.d_cFeat);
+    // cloth edges and triangles in index order, for obstacle-feature contacts
+    g.ne = (int)edges.size();
+    std::vector<int2> he(g.ne);
+    for (int e = 0; e < g.ne; e++) he[e] = make_int2((int)edges[e].i1, (int)edges[e].i2);
+    std::vector<int4> ht(g.nt);
+    for (int t = 0; t < g.nt; t++) ht[t] = make_int4((int)tris[t].i1, (int)tris[t].i2, (int)tris[t].i3, 0);
+    dev_alloc(g.d_cEdges, g.ne);
+    dev_alloc(g.d_cTris, g.nt);
+    if (g.ne) HIP_OK(hipMemcpy(g.d_cEdges, he.data(), g.ne * sizeof(int2), hipMemcpyHostToDevice));
+    if (g.nt) HIP_OK(hipMemcpy(g.d_cTris, ht.data(), g.nt * sizeof(int4), hipMemcpyHostToDevice));
+    auto groups = [&](GpuSolver::Features& f, const Schedule& c, int n) {
+        f.n = n;
+        f.groups = c.numPhases() > 0 ? c.numPhases() : 0;
+        std::vector<int> grp(n);
+        for (int p = 0; p < f.groups; p++)
+            for (int k = c.phaseOffsets[p]; k < c.phaseOffsets[p + 1]; k++) grp[c.order[k]] = p;
+        dev_alloc(f.group, n);
+        dev_alloc(f.groupBase, f.groups + 1);
+        dev_alloc(f.groupCount, std::max(f.groups, 1));
+        dev_alloc(f.list, n);
+        dev_alloc(f.count, n);
+        if (n) HIP_OK(hipMemcpy(f.group, grp.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        if (n) HIP_OK(hipMemset(f.count, 0, n * sizeof(int)));
+        HIP_OK(hipMemset(f.groupCount, 0, std::max(f.groups, 1) * sizeof(int)));
+        HIP_OK(hipMemcpy(f.groupBase, c.phaseOffsets.data(), (f.groups + 1) * sizeof(int), hipMemcpyHostToDevice));
+        f.slotCap = 0;
+        f.slots = 0;
+        dev_free(f.P); dev_free(f.N); dev_free(f.W); dev_free(f.F);
+    };
+    groups(g.ec, edgeContactCol, g.ne);
+    groups(g.tc, triContactCol, g.nt);
     mTopoDirty = false;
 }
 
@@ -383,6 +480,11 @@ void ClothModel::ensure_colouring()
     for (int b = 0; b < nb; b++) {
         bv[4 * b] = bends[b].i1; bv[4 * b + 1] = bends[b].i2; bv[4 * b + 2] = bends[b].i3; bv[4 * b + 3] = bends[b].i4;
     }
+    const int ne = (int)edges.size();
+    std::vector<uint32_t> ev(2 * (size_t)ne);
+    for (int e = 0; e < ne; e++) { ev[2 * e] = edges[e].i1; ev[2 * e + 1] = edges[e].i2; }
+    edgeContactCol = schedule_colours(ne, 2, ev.data(), nv);
+    triContactCol = schedule_colours(nt, 3, tv.data(), nv);
     // LDS budget of a slab workgroup: 4000 float4 vertices (64,000 bytes)
     const int maxVerts = 4000;
     if (mSweepMode == SWEEP_COLOURS) {
@@ -434,19 +536,53 @@ void ClothModel::ensure_contact_capacity(int slots)
     dev_alloc(g.d_cP, (size_t)slots * g.nv);
     dev_alloc(g.d_cN, (size_t)slots * g.nv);
     dev_alloc(g.d_cSrc, (size_t)slots * g.nv);
+    dev_alloc(g.d_cFeat, (size_t)slots * g.nv);
     g.slotCap = slots;
     if (g.graphExec) { (void)hipGraphExecDestroy(g.graphExec); g.graphExec = nullptr; }
     if (g.graph) { (void)hipGraphDestroy(g.graph); g.graph = nullptr; }
     g.graphKey.clear();
 }
 
-// Device descriptors of the world's collidables, in world order.
-CollSet ClothModel::make_collset(int& slots) const
+void ClothModel::ensure_feature_capacity(int edgeSlots, int triSlots)
+{
+    GpuSolver& g = gpu;
+    bool grown = false;
+    for (auto fs : {std::make_pair(&g.ec, edgeSlots), std::make_pair(&g.tc, triSlots)}) {
+        GpuSolver::Features& f = *fs.first;
+        f.slots = fs.second;
+        if (f.slots <= f.slotCap) continue;
+        const size_t m = (size_t)f.slots * f.n;
+        dev_alloc(f.P, m); dev_alloc(f.N, m); dev_alloc(f.W, m); dev_alloc(f.F, m);
+        f.slotCap = f.slots;
+        grown = true;
+    }
+    if (grown) {
+        if (g.graphExec) { (void)hipGraphExecDestroy(g.graphExec); g.graphExec = nullptr; }
+        if (g.graph) { (void)hipGraphDestroy(g.graph); g.graph = nullptr; }
+        g.graphKey.clear();
+    }
+}
+
+FeatureContactBuf ClothModel::feature_buf(GpuSolver::Features& f) const
+{
+    return FeatureContactBuf{f.count, f.P, f.N, f.W, f.F, f.group, f.groupBase, f.groupCount, f.list,
+                             f.n, f.slots, f.groups};
+}
+
+// Device descriptors of the world's collidables, in world order. Mesh
+// obstacles are uploaded (with their LBVHs) on first use and their
+// descriptors written to the device table, indexed by world position.
+CollSet ClothModel::make_collset(int& slots, int& meshes)
 {
     CollSet cs;
     memset(&cs, 0, sizeof(cs));
     slots = 0;
+    meshes = 0;
+    GpuSolver& gs = gpu;
+    std::vector<MeshGeo> geo(MAX_COLLIDABLES);
+    memset(geo.data(), 0, geo.size() * sizeof(MeshGeo));
     const CollisionWorld* w = mOwner->mWorld;
+    if (w->colls.size() > MAX_COLLIDABLES) throw std::runtime_error("too many collidables (max 8)");
     for (const auto& sp : w->colls) {
         const Collidable* c = sp.get();
         CollDev d;
@@ -469,15 +605,30 @@ CollSet ClothModel::make_collset(int& slots) const
             d.nc0 = s->nc[0]; d.nc1 = s->nc[1]; d.nc2 = s->nc[2];
             d.nv0 = s->nv[0]; d.nv1 = s->nv[1];
             if (mCollFlags & CF_VERTICES) slots += 1;
+        } else if (c->type == CT_MESH) {
+            Mesh* me = ((const CollisionMesh*)c)->mesh;
+            if (!me) throw std::runtime_error("mesh collidable without a mesh");
+            if (!gs.stream) HIP_OK(hipStreamCreateWithFlags(&gs.stream, hipStreamNonBlocking));
+            geo[cs.n] = me->device_geo(gs.stream);
+            if (!gs.d_geo) dev_alloc(gs.d_geo, MAX_COLLIDABLES);
+            d.mesh = gs.d_geo + cs.n;
+            if (mCollFlags & CF_VERTICES) slots += 1;
+            meshes++;
         } else {
             throw std::runtime_error("collidable type " + std::to_string(c->type) +
                                      " is not supported by the GPU substep");
         }
-        if (cs.n >= MAX_COLLIDABLES) throw std::runtime_error("too many collidables (max 8)");
         cs.c[cs.n++] = d;
     }
-    if (mCollFlags & (CF_EDGES | CF_TRIANGLES | CF_SELF))
-        throw std::runtime_error("edge, triangle and self collision flags are not supported by the GPU substep");
+    if (mCollFlags & CF_SELF) throw std::runtime_error("self collision is not supported by the GPU substep");
+    if (meshes) {
+        const bool same = gs.geoHost.size() == geo.size() &&
+                          memcmp(gs.geoHost.data(), geo.data(), geo.size() * sizeof(MeshGeo)) == 0;
+        if (!same) {
+            HIP_OK(hipMemcpy(gs.d_geo, geo.data(), geo.size() * sizeof(MeshGeo), hipMemcpyHostToDevice));
+            gs.geoHost = geo;
+        }
+    }
     cs.maxSlots = slots;
     return cs;
 }
@@ -492,6 +643,9 @@ std::string ClothModel::graph_key(float h, const CollSet& cs) const
     put(&mDihedral, 1); put(&mFEM, 1); put(&mFixedIterations, 1); put(&mSweepMode, 4); put(&mSlabSize, 4);
     put(&cs, sizeof(cs));
     put(&gpu.d_cP, sizeof(void*));
+    put(&gpu.d_cFeat, sizeof(void*));
+    put(&gpu.ec.P, sizeof(void*)); put(&gpu.ec.slots, 4);
+    put(&gpu.tc.P, sizeof(void*)); put(&gpu.tc.slots, 4);
     return k;
 }
 
@@ -523,8 +677,13 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
     const f3 gravity{0.f, -9.8f, 0.f};
     const f3 hg = mul(h, gravity);
     const f3 gdv = f3{mUnit * hg.x, mUnit * hg.y, mUnit * hg.z};
-    ContactBuf cb{g.d_cCount, g.d_cP, g.d_cN, g.d_cSrc, g.nv, g.slots};
+    bool anyMesh = false;
+    for (int i = 0; i < cs.n; i++) anyMesh = anyMesh || cs.c[i].type == CT_MESH;
+    ContactBuf cb{g.d_cCount, g.d_cP, g.d_cN, g.d_cSrc, g.d_cFeat, anyMesh ? g.d_geo : nullptr, g.nv, g.slots};
     const bool anyContacts = cs.maxSlots > 0;
+    const bool edgeContacts = anyMesh && (mCollFlags & CF_EDGES) && g.ec.slots > 0 && g.ne > 0;
+    const bool triContacts = anyMesh && (mCollFlags & CF_TRIANGLES) && g.tc.slots > 0 && g.nt > 0;
+    const FeatureContactBuf ecb = feature_buf(g.ec), tcb = feature_buf(g.tc);
     const float invH = 1.f / h;
     // With a fixed iteration count and every vertex in some triangle slab, the
     // vertex contacts of iteration k are applied by the FEM slab kernels of
@@ -532,9 +691,24 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
     // the last iteration's by the velocity kernel: one launch per iteration
     // fewer, same sequential order. The early-exit test needs the contact
     // error at the end of each iteration, so that mode keeps separate launches.
+    // Edge and triangle contacts follow the vertex contacts inside an
+    // iteration, so they also keep the launches separate.
     const bool fuse = anyContacts && mFEM && mFixedIterations && triCol.slabbed && triCol.numPhases() > 0 &&
-                      g.triOrphans == 0;
-    if (anyContacts) {
+                      g.triOrphans == 0 && !edgeContacts && !triContacts;
+    if (anyMesh) {
+        // the mesh query radius needs max |pos - prev| over all vertices
+        // before any detection, so prediction and detection are separate
+        timed_begin(g.timer, timed, 4, s, ev);
+        launch_predict_mesh(s, g.d_pos, g.d_prev, g.d_vel, g.nv, h, gdv, mDragCoeff, g.d_ctl, ecb, tcb);
+        timed_end(g.timer, timed, 4, s, ev);
+        timed_begin(g.timer, timed, 3, s, ev);
+        if (anyContacts)
+            launch_detect(s, g.d_pos, g.d_prev, g.nv, cs, cb, mThickness, mTolerance, mCollFlags, g.d_ctl);
+        if (edgeContacts || triContacts)
+            launch_detect_features(s, g.d_pos, g.d_prev, g.d_cEdges, g.d_cTris, cs, ecb, tcb, mThickness,
+                                   mTolerance, mCollFlags, g.d_ctl);
+        timed_end(g.timer, timed, 3, s, ev);
+    } else if (anyContacts) {
         timed_begin(g.timer, timed, 3, s, ev);
         launch_predict_detect(s, g.d_pos, g.d_prev, g.d_vel, g.nv, h, gdv, mDragCoeff, g.d_ctl, cs, cb, mThickness,
                               mTolerance, mCollFlags);
@@ -585,14 +759,24 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
             launch_contacts(s, g.d_pos, g.d_prev, g.d_vel, cb, mThickness, mFriction, invH, g.d_ctl);
             timed_end(g.timer, timed, 2, s, ev);
         }
+        if (edgeContacts) {
+            timed_begin(g.timer, timed, 2, s, ev);
+            launch_edge_contacts(s, g.d_pos, g.d_prev, g.d_cEdges, ecb, g.d_geo, mThickness, mFriction, g.d_ctl);
+            timed_end(g.timer, timed, 2, s, ev);
+        }
+        if (triContacts) {
+            timed_begin(g.timer, timed, 2, s, ev);
+            launch_tri_contacts(s, g.d_pos, g.d_prev, g.d_cTris, tcb, g.d_geo, mThickness, mFriction, g.d_ctl);
+            timed_end(g.timer, timed, 2, s, ev);
+        }
         if (!mFixedIterations)
             launch_iter_end(s, g.d_ctl, mNumIterations, 0, mBendFreq, mDihedral ? 1 : 0, mSelfCollThreshold);
     }
     timed_begin(g.timer, timed, 4, s, ev);
     if (fuse)
-        launch_contacts_velocity(s, g.d_pos, g.d_prev, g.d_vel, cb, mThickness, mFriction, invH);
+        launch_contacts_velocity(s, g.d_pos, g.d_prev, g.d_vel, cb, mThickness, mFriction, invH, g.d_ctl);
     else
-        launch_velocity(s, g.d_pos, g.d_prev, g.d_vel, g.nv, invH);
+        launch_velocity(s, g.d_pos, g.d_prev, g.d_vel, g.nv, invH, g.d_ctl);
     timed_end(g.timer, timed, 4, s, ev);
 }
 
@@ -606,9 +790,11 @@ void ClothModel::step_substeps(float h, int n)
     if (pos.empty()) return;
     for (const auto& c : mOwner->mWorld->colls)
         if (c->type == CT_SDF && !((CollisionSDF*)c.get())->sdf->d_nodes) ((CollisionSDF*)c.get())->sdf->upload();
-    int slots = 0;
-    CollSet cs = make_collset(slots);
+    int slots = 0, meshes = 0;
+    CollSet cs = make_collset(slots, meshes);
     ensure_contact_capacity(std::max(slots, 1));
+    gpu.activeSlots = slots;
+    ensure_feature_capacity((mCollFlags & CF_EDGES) ? meshes : 0, (mCollFlags & CF_TRIANGLES) ? meshes : 0);
     GpuSolver& g = gpu;
     const bool timed = g.timer.enabled;
     if (mUseGraph && !timed) {
@@ -657,31 +843,72 @@ void ClothModel::download_contacts()
 {
     if (contactsValid) return;
     contactsHost.clear();
+    edgeContactsHost.clear();
+    triContactsHost.clear();
     GpuSolver& g = gpu;
-    if (!g.d_cP || g.nv == 0 || g.slots == 0 || mTopoDirty) { contactsValid = true; return; }
-    std::vector<int> cnt(g.nv), src((size_t)g.slots * g.nv);
-    std::vector<float4> P((size_t)g.slots * g.nv), N((size_t)g.slots * g.nv);
-    HIP_OK(hipMemcpy(cnt.data(), g.d_cCount, g.nv * sizeof(int), hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(P.data(), g.d_cP, P.size() * sizeof(float4), hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(N.data(), g.d_cN, N.size() * sizeof(float4), hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(src.data(), g.d_cSrc, src.size() * sizeof(int), hipMemcpyDeviceToHost));
-    for (int v = 0; v < g.nv; v++) {
-        for (int s = 0; s < cnt[v] && s < g.slots; s++) {
-            const size_t k = (size_t)s * g.nv + v;
-            ContactOut c;
-            c.vertex = v;
-            c.slot = s;
-            c.coll = src[k];
-            c.point = f3{P[k].x, P[k].y, P[k].z};
-            c.normal = f3{N[k].x, N[k].y, N[k].z};
-            contactsHost.push_back(c);
+    if (mTopoDirty) { contactsValid = true; return; }
+    if (g.d_cP && g.nv > 0 && g.slots > 0 && g.activeSlots > 0) {
+        std::vector<int> cnt(g.nv), src((size_t)g.slots * g.nv), feat((size_t)g.slots * g.nv);
+        std::vector<float4> P((size_t)g.slots * g.nv), N((size_t)g.slots * g.nv);
+        HIP_OK(hipMemcpy(cnt.data(), g.d_cCount, g.nv * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(P.data(), g.d_cP, P.size() * sizeof(float4), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(N.data(), g.d_cN, N.size() * sizeof(float4), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(src.data(), g.d_cSrc, src.size() * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(feat.data(), g.d_cFeat, feat.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (int v = 0; v < g.nv; v++) {
+            for (int s = 0; s < cnt[v] && s < g.slots; s++) {
+                const size_t k = (size_t)s * g.nv + v;
+                ContactOut c;
+                c.vertex = v;
+                c.slot = s;
+                c.coll = src[k];
+                c.feature = feat[k];
+                c.point = f3{P[k].x, P[k].y, P[k].z};
+                c.normal = f3{N[k].x, N[k].y, N[k].z};
+                contactsHost.push_back(c);
+            }
         }
+        std::stable_sort(contactsHost.begin(), contactsHost.end(), [](const ContactOut& a, const ContactOut& b) {
+            if (a.coll != b.coll) return a.coll < b.coll;
+            if (a.vertex != b.vertex) return a.vertex < b.vertex;
+            return a.slot < b.slot;
+        });
     }
-    std::stable_sort(contactsHost.begin(), contactsHost.end(), [](const ContactOut& a, const ContactOut& b) {
-        if (a.coll != b.coll) return a.coll < b.coll;
-        if (a.vertex != b.vertex) return a.vertex < b.vertex;
-        return a.slot < b.slot;
-    });
+    // obstacle-edge and obstacle-vertex contacts in the reference's list
+    // order: collidable, then cloth edge / triangle
+    auto features = [&](GpuSolver::Features& f, std::vector<FeatureContactOut>& out) {
+        if (!f.P || f.n == 0 || f.slots == 0) return;
+        const size_t m = (size_t)f.slots * f.n;
+        std::vector<int> cnt(f.n);
+        std::vector<float4> P(m), N(m), W(m);
+        std::vector<int2> F(m);
+        HIP_OK(hipMemcpy(cnt.data(), f.count, f.n * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(P.data(), f.P, m * sizeof(float4), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(N.data(), f.N, m * sizeof(float4), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(W.data(), f.W, m * sizeof(float4), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(F.data(), f.F, m * sizeof(int2), hipMemcpyDeviceToHost));
+        for (int i = 0; i < f.n; i++) {
+            for (int s = 0; s < cnt[i] && s < f.slots; s++) {
+                const size_t k = (size_t)s * f.n + i;
+                FeatureContactOut c;
+                c.id = i;
+                c.feature = F[k].x;
+                c.coll = F[k].y;
+                c.point = f3{P[k].x, P[k].y, P[k].z};
+                c.normal = f3{N[k].x, N[k].y, N[k].z};
+                c.w[0] = W[k].x; c.w[1] = W[k].y; c.w[2] = W[k].z;
+                c.lambda = P[k].w;
+                c.lambdaF = N[k].w;
+                out.push_back(c);
+            }
+        }
+        std::stable_sort(out.begin(), out.end(), [](const FeatureContactOut& a, const FeatureContactOut& b) {
+            if (a.coll != b.coll) return a.coll < b.coll;
+            return a.id < b.id;
+        });
+    };
+    if (mCollFlags & CF_EDGES) features(g.ec, edgeContactsHost);
+    if (mCollFlags & CF_TRIANGLES) features(g.tc, triContactsHost);
     contactsValid = true;
 }
 

@@ -12,6 +12,7 @@
 
 #include "../../include/solidcloth.h"
 #include "kernels.h"
+#include "lbvh.h"
 #include "schedule.h"
 
 namespace sc {
@@ -59,16 +60,33 @@ struct MeshEdge {
     int t1 = -1, t2 = -1;
 };
 
+// Device copy of a static obstacle mesh with its three LBVHs.
+struct MeshDevice {
+    float4 *v = nullptr, *vn = nullptr, *en = nullptr;
+    int4* tri = nullptr;
+    int2* edge = nullptr;
+    float4 *triLo = nullptr, *triHi = nullptr, *vtxLo = nullptr, *edgeLo = nullptr, *edgeHi = nullptr;
+    Lbvh triB, vtxB, edgeB;
+    MeshGeo geo{};
+    ~MeshDevice();
+};
+
 // Indexed triangle mesh (Geometry/Mesh.h:12-264)
 struct Mesh {
     std::vector<scVector3> vertices;
     std::vector<unsigned> indices;
     std::vector<scVector3> normals;
+    std::vector<scVector3> triNormals;   // unit face normals (Mesh.cpp:24-27)
+    std::vector<scVector3> edgeNormals;  // edge pseudo-normals (Mesh.cpp:105, 125-126)
     std::vector<scVector2> uvs;
     std::vector<MeshEdge> edges;
+    std::unique_ptr<MeshDevice> dev;
     void clear();
     void construct_edges();
     void compute_normals();
+    void compute_edge_normals();
+    // Uploads the mesh and builds its LBVHs on first use (synchronous).
+    const MeshGeo& device_geo(hipStream_t s);
 };
 
 struct CollisionMesh : Collidable {
@@ -98,8 +116,17 @@ struct Quad {
 };
 
 struct ContactOut {
-    int coll, vertex, slot;
+    int coll, vertex, slot, feature;
     f3 point, normal;
+};
+
+// An obstacle-edge contact of a cloth edge (ids = edge, feature, collidable)
+// or an obstacle-vertex contact of a cloth triangle; w = (w1, w2[, w3]).
+struct FeatureContactOut {
+    int id, feature, coll;
+    f3 point, normal;
+    float w[3];
+    float lambda, lambdaF;
 };
 
 class ClothPatch;
@@ -141,9 +168,25 @@ public:
     int* d_cCount = nullptr;
     float4 *d_cP = nullptr, *d_cN = nullptr;
     int* d_cSrc = nullptr;
+    int* d_cFeat = nullptr;
+    MeshGeo* d_geo = nullptr;  // MAX_COLLIDABLES entries, by world index
+    std::vector<MeshGeo> geoHost;
+    // cloth edges (i1, i2) and triangles (i1, i2, i3, 0) in index order
+    int2* d_cEdges = nullptr;
+    int4* d_cTris = nullptr;
+    int ne = 0;
+    // obstacle-edge contacts per cloth edge, obstacle-vertex contacts per cloth triangle
+    struct Features {
+        int* count = nullptr;
+        float4 *P = nullptr, *N = nullptr, *W = nullptr;
+        int2* F = nullptr;
+        int *group = nullptr, *groupBase = nullptr, *groupCount = nullptr, *list = nullptr;
+        int n = 0, groups = 0, slots = 0, slotCap = 0;
+    } ec, tc;
     DeviceSlabs triSlabs, bendSlabs;
     int triOrphans = 0;  // vertices in no triangle slab
     int slots = 0, slotCap = 0;
+    int activeSlots = 0;  // vertex-contact slots the world can fill (0: no vertex contacts)
     SolverCtl* d_ctl = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -183,6 +226,7 @@ public:
     float statBendError = 0.f, statContactError = 0.f;
 
     std::vector<ContactOut> contactsHost;
+    std::vector<FeatureContactOut> edgeContactsHost, triContactsHost;
     bool contactsValid = false;
 
     void clear();
@@ -203,6 +247,9 @@ public:
     void set_sweep(int mode, int slabSize);
 
     Schedule triCol, bendCol;
+    // colour groups for obstacle-edge (cloth edges) and obstacle-vertex
+    // (cloth triangles) contacts: greedy colourings in index order
+    Schedule edgeContactCol, triContactCol;
     int mSweepMode = SWEEP_SLABS;
     int mSlabSize = 0;  // 0: chosen per constraint type by schedule_slabs_auto
 
@@ -213,7 +260,9 @@ private:
     bool mHostDirty = true, mTopoDirty = true, mDeviceNewer = false, mStatsPending = false, mColourDirty = true;
     void build_device_topology();
     void ensure_contact_capacity(int slots);
-    CollSet make_collset(int& slots) const;
+    void ensure_feature_capacity(int edgeSlots, int triSlots);
+    CollSet make_collset(int& slots, int& meshes);
+    FeatureContactBuf feature_buf(GpuSolver::Features& f) const;
     void record_substep(float h, const CollSet& cs, bool timed);
     std::string graph_key(float h, const CollSet& cs) const;
 };

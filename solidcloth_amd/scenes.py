@@ -12,6 +12,14 @@ grid (Physics/ClothPatch.cpp:181), which lies in the y = offset.y plane.
             radius 0.3 given as a 64^3-cell grid SDF (analytic distance
             sampled in FP32) plus a ground plane; 20 iterations, 4 substeps,
             friction 0.3, thickness 2 mm, 200 frames.
+  mesh512   512x512 grid, side 2 m, height perturbed by three seeded
+            sinusoids of amplitude 0.05 m (|k| <= 8 pi), dropped onto an
+            icosphere triangle mesh (subdivision 6: 40,962 vertices, 81,920
+            triangles) of radius 0.4 plus a ground plane; vertex, edge and
+            triangle contacts against the mesh, thickness 2 mm, 10 iterations,
+            4 substeps, 300 frames.
+  meshsmall 24x24 grid over an icosphere of subdivision 2, the mesh
+            contact path at a size the exhaustive oracle runs in seconds.
 """
 from dataclasses import dataclass, field, replace
 
@@ -49,6 +57,13 @@ class Scene:
     # walls box (None = no walls)
     walls_min: tuple = None
     walls_max: tuple = None
+    # icosphere triangle-mesh obstacle (subdivision < 0 = none)
+    mesh_subdiv: int = -1
+    mesh_center: tuple = (0.0, 0.0, 0.0)
+    mesh_radius: float = 0.0
+    # height perturbation: three sinusoids of this amplitude (0 = flat)
+    perturb_amplitude: float = 0.0
+    perturb_seed: int = 0
     extra: dict = field(default_factory=dict)
 
     @property
@@ -75,6 +90,74 @@ DRAPE256 = Scene(
     sphere_center=(0.0, 0.0, 0.0), sphere_radius=0.3, sdf_cells=64, sdf_half=0.4,
     walls_min=(-10.0, -0.3, -10.0), walls_max=(10.0, 10.0, 10.0),
 )
+
+
+MESH512 = Scene(
+    name="mesh512", n=512, side=2.0, offset=(0.0, 0.5, 0.0), iterations=10, frames=300,
+    flags=CF_VERTICES | CF_EDGES | CF_TRIANGLES | CF_WALLS, thickness=0.002, tolerance=0.005, friction=0.3,
+    walls_min=(-10.0, -0.45, -10.0), walls_max=(10.0, 10.0, 10.0),
+    mesh_subdiv=6, mesh_radius=0.4, perturb_amplitude=0.05, perturb_seed=3,
+)
+
+MESHSMALL = Scene(
+    name="meshsmall", n=24, side=1.0, offset=(0.0, 0.3, 0.0), iterations=10, frames=30,
+    flags=CF_VERTICES | CF_EDGES | CF_TRIANGLES | CF_WALLS, thickness=0.01, tolerance=0.01, friction=0.3,
+    walls_min=(-10.0, -0.3, -10.0), walls_max=(10.0, 10.0, 10.0),
+    mesh_subdiv=2, mesh_radius=0.25, perturb_amplitude=0.01, perturb_seed=1,
+)
+
+
+def icosphere(subdiv, radius=1.0, center=(0.0, 0.0, 0.0)):
+    """Icosahedron subdivided `subdiv` times (every triangle into four, new
+    vertices projected onto the sphere). Computed in float64 and rounded to
+    FP32 once, so the arrays are identical on every host. Returns vertices
+    (V, 3) float32 and triangle indices (T, 3) uint32, counter-clockwise seen
+    from outside: V = 10 * 4^subdiv + 2, T = 20 * 4^subdiv."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.asarray(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+             (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(a, b):
+            key = (a, b) if a < b else (b, a)
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    v = np.asarray(verts) * radius + np.asarray(center, dtype=np.float64)
+    return v.astype(np.float32), np.asarray(faces, dtype=np.uint32)
+
+
+def perturbed_positions(scene, pos):
+    """Grid positions with the scene's height perturbation added to y:
+    sum over three seeded sinusoids A sin(kx x + kz z + phi), |k| <= 8 pi,
+    evaluated in float64 from the FP32 grid and rounded to FP32."""
+    pos = np.asarray(pos, dtype=np.float32).copy()
+    if scene.perturb_amplitude == 0.0:
+        return pos
+    rng = np.random.default_rng(scene.perturb_seed)
+    ang = rng.uniform(0.0, 2.0 * np.pi, 3)
+    mag = rng.uniform(0.0, 8.0 * np.pi, 3)
+    phi = rng.uniform(0.0, 2.0 * np.pi, 3)
+    x = pos[:, 0].astype(np.float64)
+    z = pos[:, 2].astype(np.float64)
+    h = np.zeros_like(x)
+    for j in range(3):
+        h += scene.perturb_amplitude * np.sin(mag[j] * np.cos(ang[j]) * x + mag[j] * np.sin(ang[j]) * z + phi[j])
+    pos[:, 1] = (pos[:, 1].astype(np.float64) + h).astype(np.float32)
+    return pos
 
 
 def sdf_box(scene):
@@ -116,6 +199,9 @@ def build_patch(scene, sc=None):
         bmin, bmax = sdf_box(scene)
         sdf = sc.SDF(sphere_sdf_values(scene), (scene.sdf_cells,) * 3, bmin, bmax)
         world.AddSDF(sdf)
+    if scene.mesh_subdiv >= 0:
+        v, t = icosphere(scene.mesh_subdiv, scene.mesh_radius, scene.mesh_center)
+        world.AddMesh(sc.Mesh(v, t))
     if scene.walls_min is not None:
         world.AddWalls(scene.walls_min, scene.walls_max)
     patch = sc.ClothPatch(world)
@@ -133,4 +219,6 @@ def build_patch(scene, sc=None):
     m.SetFriction(scene.friction)
     patch.SetNumSteps(scene.substeps)
     patch.Init(scene.n, scene.n, scene.inc, scene.offset, scene.horizontal, scene.attached)
+    if scene.perturb_amplitude != 0.0:
+        m.set_positions(perturbed_positions(scene, m.positions()))
     return world, patch

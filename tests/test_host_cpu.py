@@ -185,3 +185,39 @@ def test_mesh_buffers_and_strain_map(sc):
     colors = (sc.scVector3 * 1024)()
     L.scComputeStrainMap(patch.h, colors, 1024)
     assert max(abs(c.x) for c in colors) < 1e-4
+
+
+@pytest.mark.parametrize("subdiv", [1, 3])
+def test_obstacle_mesh_normals_and_edges_match_oracle(sc, oracle_mod, subdiv):
+    """An obstacle mesh built through scCreateMeshFromArrays has the oracle's
+    (reference-pinned) angle-weighted vertex normals, bit for bit."""
+    v, t = S.icosphere(subdiv, 0.4, (0.1, -0.2, 0.05))
+    mesh = sc.Mesh(v, t)
+    L = sc.lib()
+    nv = L.scGetMeshVertexCount(mesh.h)
+    assert nv == len(v) and L.scGetMeshIndexCount(mesh.h) == t.size
+    nb = L.scGetMeshNormalBuffer(mesh.h)
+    n = np.ctypeslib.as_array(ctypes.cast(nb, ctypes.POINTER(ctypes.c_float)), shape=(nv * 3,)).reshape(-1, 3)
+    assert np.array_equal(n, oracle_mod.mesh_normals(v, t))
+    assert len(oracle_mod.construct_edges(t)) == 30 * 4 ** subdiv
+
+
+@pytest.mark.parametrize("n", [16, 57])
+def test_feature_contact_groups_are_vertex_disjoint(sc, n):
+    """Obstacle-edge contacts are grouped by a colouring of the cloth edges,
+    obstacle-vertex contacts by one of the cloth triangles; members of a group
+    share no vertex, so they are solved in parallel in the order exported."""
+    world, patch = S.build_patch(S.DRAPE32.with_(n=n), sc)
+    m = patch.GetModel()
+    edges = np.array(sorted({tuple(sorted((int(a), int(b)))) for tri in m.triangles()
+                             for a, b in ((tri[0], tri[1]), (tri[1], tri[2]), (tri[2], tri[0]))}))
+    for kind, cons in (("edge_contact", edges), ("tri_contact", m.triangles())):
+        order = m.constraint_order(kind)
+        off = m.colour_offsets(kind)
+        assert sorted(order) == list(range(len(cons)))
+        assert off[0] == 0 and off[-1] == len(cons)
+        for g in range(len(off) - 1):
+            verts = cons[order[off[g]:off[g + 1]]].ravel()
+            assert len(np.unique(verts)) == len(verts)
+    assert len(m.colour_offsets("tri_contact")) - 1 == 8
+    assert len(m.colour_offsets("edge_contact")) - 1 <= 11
