@@ -34,6 +34,15 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_TRI = 136
 BYTES_PER_BEND = 128
 KERNEL_NAMES = {0: "k_fem", 1: "k_bend", 2: "k_contacts", 3: "k_detect", 4: "k_predict+k_velocity"}
+WORKLOADS = {
+    "drape256": "drape256: 256x256 cloth (65,536 verts, 130,050 FEM triangles, 194,565 bends) dropped onto a "
+                "sphere (64^3 grid SDF) + ground; 20 coloured-GS iterations, 4 substeps/frame, friction 0.3, "
+                "thickness 2 mm",
+    "mesh512": "mesh512: 512x512 cloth (262,144 verts, 522,242 FEM triangles), height perturbed by three "
+               "sinusoids (0.05 m), dropped onto an icosphere mesh (subdivision 6: 40,962 verts, 81,920 "
+               "triangles, r 0.4) + ground; vertex, edge and triangle mesh contacts; 10 iterations, "
+               "4 substeps/frame, friction 0.3, thickness 2 mm",
+}
 
 
 def run_gpu(scene, steps, warmup, use_graph=True, sweep=1):
@@ -100,6 +109,9 @@ def main():
                     help="0: one launch per colour; 1: slab phases staged in LDS")
     ap.add_argument("--copy-probe-mb", type=int, default=1024,
                     help="size of the streaming-copy bandwidth probe (0 disables)")
+    ap.add_argument("--workload", default="drape256", choices=["drape256", "mesh512"],
+                    help="drape256: the metric's workload; mesh512: the triangle-mesh obstacle path "
+                         "(reported beside it, not the metric)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only "
@@ -108,7 +120,7 @@ def main():
 
     from solidcloth_amd import scenes as S
 
-    scene = S.DRAPE256
+    scene = S.DRAPE256 if args.workload == "drape256" else S.MESH512
     elapsed, stats, keep = run_gpu(scene, args.steps, args.warmup, use_graph=not args.no_graph, sweep=args.sweep)
     m = keep[2]
     event_frames = args.warmup + args.steps
@@ -150,9 +162,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "drape256: 256x256 cloth (65,536 verts, 130,050 FEM triangles, 194,565 bends) dropped "
-                        "onto a sphere (64^3 grid SDF) + ground; 20 coloured-GS iterations, 4 substeps/frame, "
-                        "friction 0.3, thickness 2 mm",
+            "workload": WORKLOADS[args.workload],
             "step": "one frame = 4 substeps",
             "iterations": its,
             "substeps_per_frame": scene.substeps,
@@ -185,7 +195,10 @@ def main():
         if ms > 0:
             out["copy_probe"] = {"bytes_copied": nbytes, "avg_ms": round(ms, 4),
                                  "GB/s (read+write)": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
-    if not args.no_cpu:
+    if args.workload != "drape256":
+        out["metric"] = "PBD substeps/sec, " + args.workload
+        out["cpu_baseline"] = None  # the exhaustive oracle needs minutes per substep at this size
+    elif not args.no_cpu:
         cpu_val, frames = cpu_baseline(scene, args.cpu_budget)
         out["cpu_baseline"] = {
             "value": round(cpu_val, 3),

@@ -247,8 +247,10 @@ class Cloth:
 
     def step(self, dt):
         rc = self.L.oc_step(self.h, dt)
-        if rc != 0:
+        if rc == -1:
             raise RuntimeError("oracle step failed (only the FEM stretch model is restated)")
+        if rc != 0:
+            raise RuntimeError("oracle step refused: grid-SDF edge contacts need vertex contacts")
 
     def substep(self, h):
         self.L.oc_substep(self.h, h)
@@ -320,7 +322,9 @@ def from_scene(scene, real64=False, tri_order=None, bend_order=None):
           iterations=scene.iterations, fixedIterations=int(scene.fixed_iterations), collFlags=scene.flags,
           thickness=scene.thickness, tolerance=scene.tolerance, friction=scene.friction,
           numSteps=scene.substeps)
-    if scene.sdf_cells:
+    if scene.analytic_sphere:
+        c.add_sphere(scene.sphere_center, scene.sphere_radius)
+    elif scene.sdf_cells:
         bmin, bmax = S.sdf_box(scene)
         c.add_grid_sdf(S.sphere_sdf_values(scene), (scene.sdf_cells,) * 3, bmin, bmax)
     if scene.mesh_subdiv >= 0:

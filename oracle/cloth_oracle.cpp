@@ -210,6 +210,39 @@ struct GridSDF {
         cp = p - val * g;
         return val;
     }
+
+    // Physics/ClothCollisionSDF.h:148-205 (GridSDF::QueryTriangle): Frank-Wolfe
+    // iterations from 0.33 (a + b + c) with step 2 / (i + 2), stopped when the
+    // value changes by less than 1e-4 relative, or when the point leaves the grid.
+    R query_triangle(V3<R> a, V3<R> b, V3<R> c, V3<R>& closest, V3<R>& n, V3<R>& bar) const
+    {
+        V3<R> cp = R(0.33f) * ((a + b) + c);
+        R distOld = 0;
+        for (int i = 1; i < 100; i++) {
+            const R dist = value_at(cp);
+            if (dist == R(FLT_MAX)) return R(FLT_MAX);
+            const V3<R> g = grad_at(cp);
+            const R dotA = dot(a, g), dotB = dot(b, g), dotC = dot(c, g);
+            R d = dotA;
+            V3<R> sv = a;
+            if (dotB < d) { d = dotB; sv = b; }
+            if (dotC < d) { d = dotC; sv = c; }
+            if (distOld > R(0)) {
+                const R rel = std::fabs(distOld - dist) / distOld;
+                if (rel < R(0.0001f)) break;
+            }
+            distOld = dist;
+            const R alpha = R(2.f) / (R(float(i)) + R(2.f));
+            cp = cp + alpha * (sv - cp);
+        }
+        const R val = value_at(cp);
+        V3<R> g = grad_at(cp);
+        normalize(g);
+        n = g;
+        closest = cp - val * g;
+        bar = barycentric(a, b, c, cp);
+        return val;
+    }
 };
 
 template <class R>
@@ -418,6 +451,87 @@ public:
         }
     }
 
+    // Physics/ClothCollisionSDF.h:26-41 (SphereSDF::QuerySegment)
+    static R sphere_query_segment(const Collidable<R>& s, V3<R> p, V3<R> q, V3<R>& cp, V3<R>& n, R& w1, R& w2)
+    {
+        R t;
+        V3<R> d = closest_pt_segm(s.center, p, q, t) - s.center;
+        R len = length(d);
+        scale(d, R(1) / len);
+        n = d;
+        cp = s.center + s.radius * d;
+        w1 = R(1.0f) - t;
+        w2 = t;
+        return len - s.radius;
+    }
+
+    // Physics/ClothCollisionSDF.h:43-55 (SphereSDF::QueryTriangle)
+    static R sphere_query_triangle(const Collidable<R>& s, V3<R> a, V3<R> b, V3<R> c, V3<R>& cp, V3<R>& n,
+                                   V3<R>& bar)
+    {
+        V3<R> pt;
+        closest_pt_point_triangle(s.center, a, b, c, pt, bar);
+        V3<R> d = pt - s.center;
+        R len = length(d);
+        scale(d, R(1) / len);
+        n = d;
+        cp = s.center + s.radius * d;
+        return len - s.radius;
+    }
+
+    // Physics/ClothCollision.cpp:472-579: the edge and triangle branches of
+    // SDFCollisions with its defaults (handleInside = handleCurrent = false,
+    // handleBorder = false, widened when vertex / edge contacts are off).
+    // Queries use the previous positions; the radius uses 1.1 x the largest
+    // travel of the primitive's vertices. `seg` returns false for a query
+    // that never yields a contact.
+    template <class Seg, class Tri>
+    void sdf_feature_collisions(Seg seg, Tri tri, int ci)
+    {
+        bool handleBorder = (prm.collFlags & CF_VERTICES) == 0;
+        auto tv = [&](uint32_t i) { return length(prev[i] - pos[i]); };
+        if (prm.collFlags & CF_EDGES) {
+            const R eps = R(0.02f);
+            for (size_t e = 0; e < edges.size(); e++) {
+                const uint32_t i1 = edges[e].i1, i2 = edges[e].i2;
+                V3<R> cp, n;
+                R w1 = 0, w2 = 0;
+                R dist;
+                if (!seg(prev[i1], prev[i2], cp, n, w1, w2, dist)) continue;
+                const R t = smax(tv(i1), tv(i2));
+                const R radTol = R(prm.thickness) + smax(R(prm.tolerance), t * R(1.1f));
+                if (dist < radTol && dist >= R(0) &&
+                    ((w1 > eps && w1 < R(1.f) - eps && w2 > eps && w2 < R(1.f) - eps) || handleBorder)) {
+                    EdgeContact<R> ec;
+                    ec.i1 = i1; ec.i2 = i2;
+                    ec.clothEdge = (int)e; ec.feature = -1; ec.mesh = -1;
+                    ec.point = cp; ec.normal = n;
+                    ec.w1 = w1; ec.w2 = w2;
+                    edgeContacts.push_back(ec);
+                }
+            }
+        }
+        handleBorder = handleBorder || (prm.collFlags & CF_EDGES) == 0;
+        if (prm.collFlags & CF_TRIANGLES) {
+            const R eps = R(0.001f);
+            for (size_t j = 0; j < tris.size(); j++) {
+                const uint32_t i1 = tris[j].i1, i2 = tris[j].i2, i3 = tris[j].i3;
+                V3<R> cp, n, bar;
+                const R dist = tri(prev[i1], prev[i2], prev[i3], cp, n, bar);
+                const R t = smax(tv(i1), smax(tv(i2), tv(i3)));
+                const R radTol = R(prm.thickness) + smax(R(prm.tolerance), t * R(1.1f));
+                if (dist < radTol && dist >= R(0) && ((bar.x > eps && bar.y > eps && bar.z > eps) || handleBorder)) {
+                    TriContact<R> tc;
+                    tc.i1 = i1; tc.i2 = i2; tc.i3 = i3;
+                    tc.clothTri = (int)j; tc.feature = -1; tc.mesh = -1;
+                    tc.point = cp; tc.normal = n;
+                    tc.w1 = bar.x; tc.w2 = bar.y; tc.w3 = bar.z;
+                    triContacts.push_back(tc);
+                }
+            }
+        }
+    }
+
     // Physics/ClothCollision.cpp:17-87 (external collidables; the cloth tree it
     // also rebuilds is consumed by nothing on this path)
     void detect_collisions()
@@ -431,9 +545,27 @@ public:
                 wall_collisions(c, (int)ci);
             } else if (c.type == CT_SPHERE) {
                 sdf_vertex_collisions([&](V3<R> p, V3<R>& cp, V3<R>& n) { return sphere_query(c, p, cp, n); }, (int)ci);
+                sdf_feature_collisions(
+                    [&](V3<R> p, V3<R> q, V3<R>& cp, V3<R>& n, R& w1, R& w2, R& d) {
+                        d = sphere_query_segment(c, p, q, cp, n, w1, w2);
+                        return true;
+                    },
+                    [&](V3<R> a, V3<R> b, V3<R> cc, V3<R>& cp, V3<R>& n, V3<R>& bar) {
+                        return sphere_query_triangle(c, a, b, cc, cp, n, bar);
+                    },
+                    (int)ci);
             } else if (c.type == CT_SDF) {
                 const GridSDF<R>& g = sdfs[c.sdf];
                 sdf_vertex_collisions([&](V3<R> p, V3<R>& cp, V3<R>& n) { return g.query_point(p, cp, n); }, (int)ci);
+                // GridSDF::QuerySegment is a stub returning 0 with coordinates
+                // (0, 0): with vertex contacts on, the border test rejects it
+                // (ClothCollision.cpp:474,497-498); without, the step refuses.
+                sdf_feature_collisions(
+                    [&](V3<R>, V3<R>, V3<R>&, V3<R>&, R&, R&, R&) { return false; },
+                    [&](V3<R> a, V3<R> b, V3<R> cc, V3<R>& cp, V3<R>& n, V3<R>& bar) {
+                        return g.query_triangle(a, b, cc, cp, n, bar);
+                    },
+                    (int)ci);
             } else if (c.type == CT_MESH) {
                 mesh_collisions(meshes[c.mesh], (int)ci, c.mesh);
             }
@@ -906,6 +1038,10 @@ public:
     int step(float dt)
     {
         if (!prm.fem) return -1;
+        // grid-SDF edge queries without vertex contacts yield undefined
+        // contacts in the reference (a stub query accepted at the border)
+        for (const auto& c : world)
+            if (c.type == CT_SDF && (prm.collFlags & CF_EDGES) && !(prm.collFlags & CF_VERTICES)) return -2;
         const int n = prm.numSteps;
         const R h = R(dt / n);
         for (int i = 0; i < n; i++) substep(h);

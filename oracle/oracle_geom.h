@@ -80,6 +80,38 @@ int closest_pt_point_triangle(V3<R> p, V3<R> a, V3<R> b, V3<R> c, V3<R>& cp, V3<
 // Math::clamp = std::max(lo, std::min(hi, v)) (Math/Utils.h:24-28)
 template <class R> inline R clampf(R v, R lo, R hi) { return smax(lo, smin(hi, v)); }
 
+// Geometry/Collision3D.h:125-141: closest point to p on segment [a, b]
+template <class R>
+V3<R> closest_pt_segm(V3<R> p, V3<R> a, V3<R> b, R& t)
+{
+    const V3<R> u = b - a, v = p - a;
+    t = dot(u, v) / dot(u, u);
+    if (t <= R(0)) {
+        t = R(0);
+        return a;
+    }
+    if (t >= R(1)) {
+        t = R(1);
+        return b;
+    }
+    return a + t * u;
+}
+
+// Geometry/Collision3D.h:42-58: barycentric coordinates of p in (a, b, c)
+template <class R>
+V3<R> barycentric(V3<R> a, V3<R> b, V3<R> c, V3<R> p)
+{
+    const V3<R> v0 = b - a, v1 = c - a, v2 = p - a;
+    const R d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1);
+    const R d20 = dot(v2, v0), d21 = dot(v2, v1);
+    const R denom = d00 * d11 - d01 * d01;
+    V3<R> r;
+    r.y = (d11 * d20 - d01 * d21) / denom;
+    r.z = (d00 * d21 - d01 * d20) / denom;
+    r.x = R(1.f) - r.y - r.z;
+    return r;
+}
+
 template <class R>
 int closest_pt_segm_segm(V3<R> p1, V3<R> q1, V3<R> p2, V3<R> q2, R& s, R& t, V3<R>& c1, V3<R>& c2)
 {

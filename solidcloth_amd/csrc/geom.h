@@ -91,6 +91,36 @@ SC_HD int closest_pt_segm_segm(f3 p1, f3 q1, f3 p2, f3 q2, float& s, float& t, f
     return region;
 }
 
+// Collision3D.h:125-141: closest point to p on segment [a, b], parameter t
+SC_HD f3 closest_pt_segm(f3 p, f3 a, f3 b, float& t)
+{
+    const f3 u = sub(b, a), v = sub(p, a);
+    t = dot(u, v) / dot(u, u);
+    if (t <= 0.f) {
+        t = 0.f;
+        return a;
+    }
+    if (t >= 1.f) {
+        t = 1.f;
+        return b;
+    }
+    return add(a, mul(t, u));
+}
+
+// Collision3D.h:42-58: barycentric coordinates of p in (a, b, c)
+SC_HD f3 barycentric(f3 a, f3 b, f3 c, f3 p)
+{
+    const f3 v0 = sub(b, a), v1 = sub(c, a), v2 = sub(p, a);
+    const float d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1);
+    const float d20 = dot(v2, v0), d21 = dot(v2, v1);
+    const float denom = d00 * d11 - d01 * d01;
+    f3 r;
+    r.y = (d11 * d20 - d01 * d21) / denom;
+    r.z = (d00 * d21 - d01 * d20) / denom;
+    r.x = 1.f - r.y - r.z;
+    return r;
+}
+
 // triple(a1 - a0, a2 - a0, a3 - a0) = (a3 - a0) . ((a1 - a0) x (a2 - a0))
 SC_HD float signed_volume(f3 a0, f3 a1, f3 a2, f3 a3)
 {

@@ -8,6 +8,7 @@
 #include "kernels.h"
 
 #include "bvh_query.h"
+#include "collide_dev.h"
 #include "devutil.h"
 
 namespace sc {
@@ -39,43 +40,6 @@ __global__ void k_predict(float4* __restrict__ pos, float4* __restrict__ prev, f
     f3 p = add(xyz(P), mul(h, v));
     vel[i] = f4(v, 0.f);
     pos[i] = f4(p, P.w);
-}
-
-// Geometry/SDF.h:105-155 — trilinear value and gradient of a node-sampled
-// grid. Returns false when the cell lies outside (value FLT_MAX, gradient 0).
-__device__ __forceinline__ bool sdf_lookup(const CollDev& c, f3 p, float& val, f3& g)
-{
-    float rx = (p.x - c.ax) / c.bx;
-    float ry = (p.y - c.ay) / c.by;
-    float rz = (p.z - c.az) / c.bz;
-    float flx = floorf(rx), fly = floorf(ry), flz = floorf(rz);
-    if (!(flx >= 0.f && flx <= (float)c.nc0 && fly >= 0.f && fly <= (float)c.nc1 && flz >= 0.f &&
-          flz <= (float)c.nc2))
-        return false;
-    int x = (int)flx, y = (int)fly, z = (int)flz;
-    float fx = fabsf(rx - (float)x), fy = fabsf(ry - (float)y), fz = fabsf(rz - (float)z);
-    float4 nd[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        // term order of the reference: (x,y,z) (x,y,z+1) (x,y+1,z) ... (x+1,y+1,z+1)
-        int dx = (k >> 2) & 1, dy = (k >> 1) & 1, dz = k & 1;
-        int X = x + dx, Y = y + dy, Z = z + dz;
-        bool out = X > c.nc0 || Y > c.nc1 || Z > c.nc2;
-        nd[k] = out ? make_float4(0.f, 0.f, 0.f, 0.f) : c.nodes[X + Y * c.nv0 + Z * c.nv0 * c.nv1];
-    }
-    const float ox = 1.f - fx, oy = 1.f - fy, oz = 1.f - fz;
-    float w[8] = {ox * oy * oz, ox * oy * fz, ox * fy * oz, ox * fy * fz,
-                  fx * oy * oz, fx * oy * fz, fx * fy * oz, fx * fy * fz};
-    float v = w[0] * nd[0].w;
-    f3 gr = mul(w[0], xyz(nd[0]));
-#pragma unroll
-    for (int k = 1; k < 8; k++) {
-        v = v + w[k] * nd[k].w;
-        gr = add(gr, mul(w[k], xyz(nd[k])));
-    }
-    val = v;
-    g = gr;
-    return true;
 }
 
 __device__ __forceinline__ void put_contact(ContactBuf cb, int v, int& cnt, f3 p, f3 n, int ci, int feat = -1)

@@ -3,6 +3,7 @@
 // obstacle-vertex contacts, and their Gauss-Seidel passes. Arithmetic follows
 // the reference's operation order (see kernels.hip).
 #include "bvh_query.h"
+#include "collide_dev.h"
 #include "devutil.h"
 #include "kernels.h"
 
@@ -54,35 +55,65 @@ __device__ __forceinline__ void group_append(const FeatureContactBuf& b, int id)
     b.list[b.groupBase[g] + k] = id;
 }
 
-// Physics/ClothCollision.cpp:137-197: per cloth edge the nearest obstacle
-// edge, per cloth triangle the nearest obstacle vertex, from the previous
-// positions, for every mesh obstacle in world order. Blocks [0, edgeBlocks)
-// take edges, the rest triangles.
+__device__ __forceinline__ void put_feature(const FeatureContactBuf& b, int id, int& cnt, f3 point, f3 normal,
+                                            float w1, float w2, float w3, int feature, int ci)
+{
+    const int k = cnt * b.n + id;
+    b.P[k] = f4(point, 0.f);
+    b.N[k] = f4(normal, 0.f);
+    b.W[k] = make_float4(w1, w2, w3, 0.f);
+    b.F[k] = make_int2(feature, ci);
+    cnt++;
+}
+
+// Edge and triangle contacts of the cloth against every collidable, in world
+// order, from the previous positions (Physics/ClothCollision.cpp:50-87):
+//  * mesh obstacles (ClothCollision.cpp:137-197): per cloth edge the nearest
+//    obstacle edge within thickness + max(tolerance, 1.5 max travel), per
+//    cloth triangle the nearest obstacle vertex within thickness +
+//    max(tolerance, 1.2 x its vertices' travel);
+//  * spheres and grid SDFs (ClothCollision.cpp:472-579): the collidable's
+//    segment / triangle query within thickness + max(tolerance, 1.1 x
+//    travel), border coordinates rejected unless vertex (for edges) or edge
+//    (for triangles) contacts are off. The grid SDF's segment query is a
+//    stub that the border test always rejects (the host requires vertex
+//    contacts with it).
+// Blocks [0, edgeBlocks) take edges, the rest triangles.
 __global__ void __launch_bounds__(kBlock) k_detect_features(const float4* __restrict__ pos,
                                                             const float4* __restrict__ prev,
                                                             const int2* __restrict__ cEdges,
                                                             const int4* __restrict__ cTris, CollSet cs,
                                                             FeatureContactBuf ec, FeatureContactBuf tc,
                                                             int edgeBlocks, float thickness, float tolerance,
-                                                            const SolverCtl* __restrict__ ctl)
+                                                            int collFlags, const SolverCtl* __restrict__ ctl)
 {
+    const bool borderE = (collFlags & CF_VERTICES) == 0;
+    const bool borderT = borderE || (collFlags & CF_EDGES) == 0;
     if ((int)blockIdx.x < edgeBlocks) {
         const int e = blockIdx.x * blockDim.x + threadIdx.x;
         if (e >= ec.n) return;
         const int2 ev = cEdges[e];
-        const f3 p = xyz(prev[ev.x]), q = xyz(prev[ev.y]);
-        const float radius = thickness + smax(tolerance, __uint_as_float(ctl->maxTv) * 1.5f);
+        const float4 P1 = prev[ev.x], P2 = prev[ev.y];
+        const f3 p = xyz(P1), q = xyz(P2);
+        const float meshRad = thickness + smax(tolerance, __uint_as_float(ctl->maxTv) * 1.5f);
+        const float tv = smax(len(sub(p, xyz(pos[ev.x]))), len(sub(q, xyz(pos[ev.y]))));
+        const float sdfRad = thickness + smax(tolerance, tv * 1.1f);
+        const float eps = 0.02f;
         int cnt = 0;
         for (int ci = 0; ci < cs.n; ci++) {
-            if (cs.c[ci].type != CT_MESH) continue;
-            NearestResult r;
-            if (!nearest_edge(*cs.c[ci].mesh, p, q, radius, r)) continue;
-            const int k = cnt * ec.n + e;
-            ec.P[k] = f4(r.point, 0.f);
-            ec.N[k] = f4(r.normal, 0.f);
-            ec.W[k] = make_float4(1.0f - r.param, r.param, 0.f, 0.f);
-            ec.F[k] = make_int2(r.feature, ci);
-            cnt++;
+            const CollDev& c = cs.c[ci];
+            if (c.type == CT_MESH) {
+                NearestResult r;
+                if (nearest_edge(*c.mesh, p, q, meshRad, r))
+                    put_feature(ec, e, cnt, r.point, r.normal, 1.0f - r.param, r.param, 0.f, r.feature, ci);
+            } else if (c.type == CT_SPHERE) {
+                f3 cp, n;
+                float w1, w2;
+                const float dist = sphere_segment(c, p, q, cp, n, w1, w2);
+                if (dist < sdfRad && dist >= 0.f &&
+                    ((w1 > eps && w1 < 1.f - eps && w2 > eps && w2 < 1.f - eps) || borderE))
+                    put_feature(ec, e, cnt, cp, n, w1, w2, 0.f, -1, ci);
+            }
         }
         ec.count[e] = cnt;
         if (cnt) group_append(ec, e);
@@ -90,23 +121,28 @@ __global__ void __launch_bounds__(kBlock) k_detect_features(const float4* __rest
         const int t = (blockIdx.x - edgeBlocks) * blockDim.x + threadIdx.x;
         if (t >= tc.n) return;
         const int4 ix = cTris[t];
-        const float4 A = prev[ix.x], B = prev[ix.y], C = prev[ix.z];
-        const float tv1 = len(sub(xyz(pos[ix.x]), xyz(A)));
-        const float tv2 = len(sub(xyz(pos[ix.y]), xyz(B)));
-        const float tv3 = len(sub(xyz(pos[ix.z]), xyz(C)));
+        const f3 a = xyz(prev[ix.x]), b = xyz(prev[ix.y]), cc = xyz(prev[ix.z]);
+        const float tv1 = len(sub(xyz(pos[ix.x]), a));
+        const float tv2 = len(sub(xyz(pos[ix.y]), b));
+        const float tv3 = len(sub(xyz(pos[ix.z]), cc));
         const float tv = smax(tv1, smax(tv2, tv3));
-        const float radius = thickness + smax(tolerance, tv * 1.2f);
+        const float meshRad = thickness + smax(tolerance, tv * 1.2f);
+        const float sdfRad = thickness + smax(tolerance, tv * 1.1f);
+        const float eps = 0.001f;
         int cnt = 0;
         for (int ci = 0; ci < cs.n; ci++) {
-            if (cs.c[ci].type != CT_MESH) continue;
-            NearestResult r;
-            if (!nearest_vertex(*cs.c[ci].mesh, xyz(A), xyz(B), xyz(C), radius, r)) continue;
-            const int k = cnt * tc.n + t;
-            tc.P[k] = f4(r.point, 0.f);
-            tc.N[k] = f4(r.normal, 0.f);
-            tc.W[k] = make_float4(r.bary.x, r.bary.y, r.bary.z, 0.f);
-            tc.F[k] = make_int2(r.feature, ci);
-            cnt++;
+            const CollDev& c = cs.c[ci];
+            if (c.type == CT_MESH) {
+                NearestResult r;
+                if (nearest_vertex(*c.mesh, a, b, cc, meshRad, r))
+                    put_feature(tc, t, cnt, r.point, r.normal, r.bary.x, r.bary.y, r.bary.z, r.feature, ci);
+            } else if (c.type == CT_SPHERE || c.type == CT_SDF) {
+                f3 cp, n, bar;
+                const float dist = c.type == CT_SPHERE ? sphere_triangle(c, a, b, cc, cp, n, bar)
+                                                       : sdf_triangle(c, a, b, cc, cp, n, bar);
+                if (dist < sdfRad && dist >= 0.f && ((bar.x > eps && bar.y > eps && bar.z > eps) || borderT))
+                    put_feature(tc, t, cnt, cp, n, bar.x, bar.y, bar.z, -1, ci);
+            }
         }
         tc.count[t] = cnt;
         if (cnt) group_append(tc, t);
@@ -175,12 +211,22 @@ __global__ void __launch_bounds__(kSolveBlock) k_edge_contacts(float4* __restric
                 const int q = sl * ec.n + e;
                 const float4 CP = ec.P[q], CN = ec.N[q];
                 const int2 F = ec.F[q];
-                const MeshGeo& M = geo[F.y];
-                const int2 me = M.edge[F.x];
-                float s1, t;
-                f3 point, p;
-                closest_pt_segm_segm(xyz(M.v[me.x]), xyz(M.v[me.y]), x1, x2, s1, t, point, p);
-                const float w1 = 1.0f - t, w2 = t;
+                f3 point = xyz(CP), p;
+                float w1, w2;
+                if (F.x >= 0) {
+                    // mesh contact: closest points on the same obstacle edge again
+                    const MeshGeo& M = geo[F.y];
+                    const int2 me = M.edge[F.x];
+                    float s1, t;
+                    closest_pt_segm_segm(xyz(M.v[me.x]), xyz(M.v[me.y]), x1, x2, s1, t, point, p);
+                    w1 = 1.0f - t;
+                    w2 = t;
+                } else {
+                    const float4 W = ec.W[q];
+                    w1 = W.x;
+                    w2 = W.y;
+                    p = add(mul(w1, x1), mul(w2, x2));
+                }
                 const f3 normal = xyz(CN);
                 float lambda = CP.w, lambdaF = CN.w;
                 const float l = dot(normal, sub(p, point));
@@ -292,7 +338,7 @@ void launch_detect_features(hipStream_t s, const float4* pos, const float4* prev
     const int tb = (collFlags & CF_TRIANGLES) && tc.slots > 0 ? blocks(tc.n) : 0;
     if (eb + tb == 0) return;
     hipLaunchKernelGGL(k_detect_features, dim3(eb + tb), dim3(kBlock), 0, s, pos, prev, cEdges, cTris, cs, ec, tc,
-                       eb, thickness, tolerance, ctl);
+                       eb, thickness, tolerance, collFlags, ctl);
 }
 
 void launch_edge_contacts(hipStream_t s, float4* pos, const float4* prev, const int2* cEdges, FeatureContactBuf ec,

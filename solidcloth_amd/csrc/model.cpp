@@ -572,12 +572,15 @@ FeatureContactBuf ClothModel::feature_buf(GpuSolver::Features& f) const
 // Device descriptors of the world's collidables, in world order. Mesh
 // obstacles are uploaded (with their LBVHs) on first use and their
 // descriptors written to the device table, indexed by world position.
-CollSet ClothModel::make_collset(int& slots, int& meshes)
+CollSet ClothModel::make_collset(int& slots, int& edgeSlots, int& triSlots)
 {
     CollSet cs;
     memset(&cs, 0, sizeof(cs));
     slots = 0;
-    meshes = 0;
+    edgeSlots = 0;
+    triSlots = 0;
+    int meshes = 0;
+    const bool fe = (mCollFlags & CF_EDGES) != 0, ft = (mCollFlags & CF_TRIANGLES) != 0;
     GpuSolver& gs = gpu;
     std::vector<MeshGeo> geo(MAX_COLLIDABLES);
     memset(geo.data(), 0, geo.size() * sizeof(MeshGeo));
@@ -597,6 +600,8 @@ CollSet ClothModel::make_collset(int& slots, int& meshes)
             const Sphere* s = (const Sphere*)c;
             d.ax = s->center.x; d.ay = s->center.y; d.az = s->center.z; d.r = s->radius;
             if (mCollFlags & CF_VERTICES) slots += 1;
+            edgeSlots += fe;
+            triSlots += ft;
         } else if (c->type == CT_SDF) {
             const SDF* s = ((const CollisionSDF*)c)->sdf;
             d.ax = s->box.min.x; d.ay = s->box.min.y; d.az = s->box.min.z;
@@ -605,6 +610,13 @@ CollSet ClothModel::make_collset(int& slots, int& meshes)
             d.nc0 = s->nc[0]; d.nc1 = s->nc[1]; d.nc2 = s->nc[2];
             d.nv0 = s->nv[0]; d.nv1 = s->nv[1];
             if (mCollFlags & CF_VERTICES) slots += 1;
+            // GridSDF::QuerySegment is a stub (ClothCollisionSDF.h:141-146):
+            // with vertex contacts on the border test rejects every edge, and
+            // without them the reference produces undefined contacts
+            if (fe && !(mCollFlags & CF_VERTICES))
+                throw std::runtime_error("edge contacts against a grid SDF need CF_VERTICES "
+                                         "(the reference's grid-SDF segment query is a stub)");
+            triSlots += ft;
         } else if (c->type == CT_MESH) {
             Mesh* me = ((const CollisionMesh*)c)->mesh;
             if (!me) throw std::runtime_error("mesh collidable without a mesh");
@@ -613,6 +625,8 @@ CollSet ClothModel::make_collset(int& slots, int& meshes)
             if (!gs.d_geo) dev_alloc(gs.d_geo, MAX_COLLIDABLES);
             d.mesh = gs.d_geo + cs.n;
             if (mCollFlags & CF_VERTICES) slots += 1;
+            edgeSlots += fe;
+            triSlots += ft;
             meshes++;
         } else {
             throw std::runtime_error("collidable type " + std::to_string(c->type) +
@@ -681,8 +695,8 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
     for (int i = 0; i < cs.n; i++) anyMesh = anyMesh || cs.c[i].type == CT_MESH;
     ContactBuf cb{g.d_cCount, g.d_cP, g.d_cN, g.d_cSrc, g.d_cFeat, anyMesh ? g.d_geo : nullptr, g.nv, g.slots};
     const bool anyContacts = cs.maxSlots > 0;
-    const bool edgeContacts = anyMesh && (mCollFlags & CF_EDGES) && g.ec.slots > 0 && g.ne > 0;
-    const bool triContacts = anyMesh && (mCollFlags & CF_TRIANGLES) && g.tc.slots > 0 && g.nt > 0;
+    const bool edgeContacts = (mCollFlags & CF_EDGES) && g.ec.slots > 0 && g.ne > 0;
+    const bool triContacts = (mCollFlags & CF_TRIANGLES) && g.tc.slots > 0 && g.nt > 0;
     const FeatureContactBuf ecb = feature_buf(g.ec), tcb = feature_buf(g.tc);
     const float invH = 1.f / h;
     // With a fixed iteration count and every vertex in some triangle slab, the
@@ -695,9 +709,10 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
     // iteration, so they also keep the launches separate.
     const bool fuse = anyContacts && mFEM && mFixedIterations && triCol.slabbed && triCol.numPhases() > 0 &&
                       g.triOrphans == 0 && !edgeContacts && !triContacts;
-    if (anyMesh) {
+    if (anyMesh || edgeContacts || triContacts) {
         // the mesh query radius needs max |pos - prev| over all vertices
-        // before any detection, so prediction and detection are separate
+        // before any detection, and the contact groups are cleared there, so
+        // prediction and detection are separate launches
         timed_begin(g.timer, timed, 4, s, ev);
         launch_predict_mesh(s, g.d_pos, g.d_prev, g.d_vel, g.nv, h, gdv, mDragCoeff, g.d_ctl, ecb, tcb);
         timed_end(g.timer, timed, 4, s, ev);
@@ -786,15 +801,26 @@ void ClothModel::step_substeps(float h, int n)
     if (mNumIterations < 1) throw std::runtime_error("NumIterations must be >= 1");
     if (mBendFreq < 1) throw std::runtime_error("bend frequency must be >= 1");
     if (msActive) throw std::runtime_error("the mouse spring is not implemented on the GPU substep");
+    if (mCollFlags & CF_SELF) throw std::runtime_error("self collision is not supported by the GPU substep");
+    const CollisionWorld* w = mOwner->mWorld;
+    if (w->colls.size() > MAX_COLLIDABLES) throw std::runtime_error("too many collidables (max 8)");
+    for (const auto& c : w->colls) {
+        if (c->type != CT_WALLS && c->type != CT_SPHERE && c->type != CT_SDF && c->type != CT_MESH)
+            throw std::runtime_error("collidable type " + std::to_string(c->type) +
+                                     " is not supported by the GPU substep");
+        if (c->type == CT_SDF && (mCollFlags & CF_EDGES) && !(mCollFlags & CF_VERTICES))
+            throw std::runtime_error("edge contacts against a grid SDF need CF_VERTICES "
+                                     "(the reference's grid-SDF segment query is a stub)");
+    }
     sync_to_device();
     if (pos.empty()) return;
     for (const auto& c : mOwner->mWorld->colls)
         if (c->type == CT_SDF && !((CollisionSDF*)c.get())->sdf->d_nodes) ((CollisionSDF*)c.get())->sdf->upload();
-    int slots = 0, meshes = 0;
-    CollSet cs = make_collset(slots, meshes);
+    int slots = 0, edgeSlots = 0, triSlots = 0;
+    CollSet cs = make_collset(slots, edgeSlots, triSlots);
     ensure_contact_capacity(std::max(slots, 1));
     gpu.activeSlots = slots;
-    ensure_feature_capacity((mCollFlags & CF_EDGES) ? meshes : 0, (mCollFlags & CF_TRIANGLES) ? meshes : 0);
+    ensure_feature_capacity(edgeSlots, triSlots);
     GpuSolver& g = gpu;
     const bool timed = g.timer.enabled;
     if (mUseGraph && !timed) {

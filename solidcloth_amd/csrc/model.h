@@ -261,7 +261,7 @@ private:
     void build_device_topology();
     void ensure_contact_capacity(int slots);
     void ensure_feature_capacity(int edgeSlots, int triSlots);
-    CollSet make_collset(int& slots, int& meshes);
+    CollSet make_collset(int& slots, int& edgeSlots, int& triSlots);
     FeatureContactBuf feature_buf(GpuSolver::Features& f) const;
     void record_substep(float h, const CollSet& cs, bool timed);
     std::string graph_key(float h, const CollSet& cs) const;

@@ -54,6 +54,8 @@ class Scene:
     sphere_radius: float = 0.0
     sdf_cells: int = 0
     sdf_half: float = 0.0
+    # the sphere as an analytic collidable (scCreateSphere) instead of an SDF
+    analytic_sphere: bool = False
     # walls box (None = no walls)
     walls_min: tuple = None
     walls_max: tuple = None
@@ -195,7 +197,9 @@ def build_patch(scene, sc=None):
     if sc is None:
         import solidcloth_amd as sc
     world = sc.CollisionWorld()
-    if scene.sdf_cells:
+    if scene.analytic_sphere:
+        world.AddSphere(scene.sphere_center, scene.sphere_radius)
+    elif scene.sdf_cells:
         bmin, bmax = sdf_box(scene)
         sdf = sc.SDF(sphere_sdf_values(scene), (scene.sdf_cells,) * 3, bmin, bmax)
         world.AddSDF(sdf)

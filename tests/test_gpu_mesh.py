@@ -14,51 +14,11 @@ import pytest
 
 from solidcloth_amd import scenes as S
 
+from parity_util import pair as _pair
+from parity_util import same_contacts as _same_contacts
+from parity_util import same_positions as _same_positions
+
 pytestmark = pytest.mark.gpu
-REL_TOL = 1e-5
-
-
-def _pair(sc, oracle_mod, scene, mode=1):
-    world, patch = S.build_patch(scene, sc)
-    m = patch.GetModel()
-    m.SetSweep(mode)
-    o = oracle_mod.from_scene(scene, tri_order=m.constraint_order("tri"), bend_order=m.constraint_order("bend"))
-    o.set_contact_order("edge", m.constraint_order("edge_contact"))
-    o.set_contact_order("tri", m.constraint_order("tri_contact"))
-    return world, patch, m, o
-
-
-def _same_positions(m, o, what):
-    gp, op = m.positions(), o.get("pos")
-    assert np.isfinite(gp).all(), what
-    ext = max(1e-6, float(np.abs(op).max()))
-    err = float(np.abs(gp.astype(np.float64) - op).max())
-    assert err <= REL_TOL * ext, f"{what}: max abs diff {err:.3e}"
-    assert err == 0.0, f"{what}: expected bit-exact, max abs diff {err:.3e}"
-
-
-def _key_rows(ids, data):
-    return sorted((tuple(int(x) for x in i), tuple(np.float32(d).tolist())) for i, d in zip(ids, data))
-
-
-def _same_contacts(m, o):
-    # vertex contacts: (point, normal) and obstacle feature, as sets
-    pts, nrm = m.contacts()
-    feats = m.contact_features()
-    ids, data = o.contacts()
-    ofeat = o.contact_features()[:, 1]
-    g = sorted(tuple(np.r_[p, n].astype(np.float32).tolist()) + (int(f),) for p, n, f in zip(pts, nrm, feats))
-    r = sorted(tuple(d[:6].astype(np.float32).tolist()) + (int(f),) for d, f in zip(data, ofeat))
-    assert g == r
-    # edge contacts: (cloth edge, obstacle edge) and point, normal, w1, w2, lambda, lambdaF
-    gi, gd = m.edge_contacts()
-    oi, od = o.edge_contacts()
-    assert _key_rows(gi[:, :2], gd) == _key_rows(oi[:, 2:4], od)
-    # triangle contacts: (cloth triangle, obstacle vertex) and point, normal, w1..w3, lambda, lambdaF
-    gi, gd = m.tri_contacts()
-    oi, od = o.tri_contacts()
-    assert _key_rows(gi[:, :2], gd) == _key_rows(oi, od)
-    return len(pts), len(gi), len(m.edge_contacts()[0])
 
 
 @pytest.mark.parametrize("mode", [0, 1])

@@ -221,3 +221,13 @@ def test_feature_contact_groups_are_vertex_disjoint(sc, n):
             assert len(np.unique(verts)) == len(verts)
     assert len(m.colour_offsets("tri_contact")) - 1 == 8
     assert len(m.colour_offsets("edge_contact")) - 1 <= 11
+
+
+def test_grid_sdf_edges_without_vertex_contacts_refused(sc):
+    """The reference's grid-SDF segment query is a stub: without vertex
+    contacts it yields undefined edge contacts, so the step refuses before
+    touching the device."""
+    scene = S.DRAPE256.with_(n=16, flags=S.CF_EDGES | S.CF_WALLS)
+    world, patch = S.build_patch(scene, sc)
+    with pytest.raises(sc.SolidClothError, match="grid SDF"):
+        patch.Step(scene.dt)
