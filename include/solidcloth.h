@@ -216,6 +216,13 @@ SCEXP void scSetSweep(scModel model, int mode, int slabSize);
  * (cloth triangle, obstacle vertex, collidable), data = (point[3], normal[3],
  * w1, w2, w3, lambda, lambdaF). All return the contact count. */
 SCEXP int scGetContactFeatures(scModel model, int* out, int count);
+
+/* Diagnostic: builds the device LBVH over n random boxes (seeded), refits it
+ * `refits` times to new random boxes and checks every result on the host.
+ * Returns the number of violations (0 = correct); msOut[0] = build time,
+ * msOut[1] = average refit time (ms, device events). No counterpart in the
+ * reference (its tree is a host AabbTree). */
+SCEXP int scLbvhSelfCheck(int n, unsigned int seed, int refits, double* msOut);
 SCEXP int scGetEdgeContactData(scModel model, int* ids, float* data, int cap);
 SCEXP int scGetTriContactData(scModel model, int* ids, float* data, int cap);
 

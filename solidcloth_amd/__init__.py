@@ -107,6 +107,7 @@ def _sig(L):
         "scSetUseGraph": (None, [H, b]), "scBandwidthProbe": (C.c_double, [C.c_longlong, i]),
         "scGetContactFeatures": (i, [H, vp, i]), "scGetEdgeContactData": (i, [H, vp, vp, i]),
         "scGetTriContactData": (i, [H, vp, vp, i]),
+        "scLbvhSelfCheck": (i, [i, C.c_uint, i, vp]),
         "scGetLastError": (C.c_char_p, []),
         "scGetErrorCount": (i, []), "scGetBuildInfo": (C.c_char_p, []),
     }

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <stdexcept>
 
+#include "lbvh.h"
 #include "model.h"
 
 namespace sc {
@@ -768,6 +769,82 @@ double scBandwidthProbe(long long bytes, int reps)
         }
         cleanup();
         return ms;
+    });
+}
+
+int scLbvhSelfCheck(int n, unsigned seed, int refits, double* msOut)
+{
+    return guard_r<int>(-1, [&] {
+        if (n < 1 || refits < 0) throw std::runtime_error("bad LBVH check size");
+        // random boxes from a 32-bit LCG: clustered centres, varied extents
+        uint32_t st = seed * 2654435761u + 1u;
+        auto rnd = [&] {
+            st = st * 1664525u + 1013904223u;
+            return (float)(st >> 8) * (1.0f / 16777216.0f);
+        };
+        std::vector<float4> lo(n), hi(n);
+        auto fill = [&](float jitter) {
+            for (int i = 0; i < n; i++) {
+                const float cx = rnd() * rnd(), cy = rnd(), cz = (i % 7) * 0.1f + jitter * rnd();
+                const float e = 0.001f + 0.01f * rnd();
+                lo[i] = make_float4(cx - e, cy - e, cz - e, 0.f);
+                hi[i] = make_float4(cx + e, cy + e, cz + e, 0.f);
+            }
+        };
+        float4 *dlo = nullptr, *dhi = nullptr;
+        hipStream_t s = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        int bad = 0;
+        auto cleanup = [&] {
+            if (dlo) (void)hipFree(dlo);
+            if (dhi) (void)hipFree(dhi);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (s) (void)hipStreamDestroy(s);
+        };
+        try {
+            auto ok = [](hipError_t e) {
+                if (e != hipSuccess) throw std::runtime_error(hipGetErrorString(e));
+            };
+            ok(hipMalloc((void**)&dlo, n * sizeof(float4)));
+            ok(hipMalloc((void**)&dhi, n * sizeof(float4)));
+            ok(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            ok(hipEventCreate(&e0));
+            ok(hipEventCreate(&e1));
+            Lbvh bvh;
+            fill(0.f);
+            ok(hipMemcpy(dlo, lo.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+            ok(hipMemcpy(dhi, hi.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+            bvh.build(s, dlo, dhi, n);  // allocates; the timed build below reuses the buffers
+            ok(hipEventRecord(e0, s));
+            bvh.build(s, dlo, dhi, n);
+            ok(hipEventRecord(e1, s));
+            ok(hipEventSynchronize(e1));
+            float t = 0.f;
+            ok(hipEventElapsedTime(&t, e0, e1));
+            if (msOut) msOut[0] = t;
+            bad += bvh.verify(s, dlo, dhi);
+            double refitMs = 0.0;
+            for (int r = 0; r < refits; r++) {
+                fill(0.05f);
+                ok(hipMemcpy(dlo, lo.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+                ok(hipMemcpy(dhi, hi.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+                ok(hipEventRecord(e0, s));
+                bvh.refit(s, dlo, dhi);
+                ok(hipEventRecord(e1, s));
+                ok(hipEventSynchronize(e1));
+                ok(hipGetLastError());
+                ok(hipEventElapsedTime(&t, e0, e1));
+                refitMs += t;
+                bad += bvh.verify(s, dlo, dhi);
+            }
+            if (msOut) msOut[1] = refits ? refitMs / refits : 0.0;
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+        return bad;
     });
 }
 

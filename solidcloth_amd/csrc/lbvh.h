@@ -6,8 +6,10 @@
 //      wavefronts with LDS prefix sums, so the order is deterministic,
 //   4. Karras (HPG 2012) hierarchy: internal node i covers a key range found by
 //      binary search on common-prefix lengths (ties broken by index),
-//   5. bottom-up refit: the second child to arrive at a node (atomic counter,
-//      acquire/release at agent scope) writes its box and continues upward.
+//   5. bottom-up refit: the second child to arrive at a node (atomic counter)
+//      writes its box and continues upward; nodes inside one workgroup's
+//      chunk of leaves synchronise at workgroup scope, the few whose leaf
+//      range crosses chunks through device-scope atomic min/max words.
 // Internal node k stores (min, left child) and (max, right child); a child
 // c >= 0 is internal node c, c < 0 is leaf ~c, i.e. sorted position ~c.
 #pragma once
@@ -31,8 +33,14 @@ public:
     // Builds over n boxes (device arrays, xyz used). Asynchronous on `s`;
     // allocates on the first build and when n grows.
     void build(hipStream_t s, const float4* boxLo, const float4* boxHi, int n);
+    // New boxes for the same primitives, same hierarchy (asynchronous).
+    void refit(hipStream_t s, const float4* boxLo, const float4* boxHi);
     BvhView view() const;
     int size() const { return n_; }
+    // Downloads the hierarchy and checks it on the host: every leaf appears
+    // once, parent links agree with child links, node boxes equal the union
+    // of their children. Returns the number of violations (synchronous).
+    int verify(hipStream_t s, const float4* boxLo, const float4* boxHi) const;
 
 private:
     void reserve(int n);
@@ -44,6 +52,8 @@ private:
     unsigned* hist_ = nullptr;
     int *parentNode_ = nullptr, *parentLeaf_ = nullptr, *flags_ = nullptr;
     unsigned* bounds_ = nullptr;  // 6 orderable-encoded floats
+    int2* range_ = nullptr;       // sorted-leaf range of each internal node
+    unsigned* enc_ = nullptr;     // per internal node: order-encoded min (3) / max (3) words
 };
 
 }  // namespace sc

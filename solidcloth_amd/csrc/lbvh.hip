@@ -1,5 +1,8 @@
 // GPU LBVH builder: Morton codes, deterministic stable radix sort, Karras
 // hierarchy and bottom-up refit.
+#include <algorithm>
+#include <vector>
+
 #include "hiputil.h"
 #include "lbvh.h"
 
@@ -11,6 +14,8 @@ constexpr int kBlock = 256;
 constexpr int kItems = 8;                  // keys per thread per sort tile
 constexpr int kTile = kBlock * kItems;     // keys per sort tile
 constexpr int kScanBlock = 1024;
+constexpr int kRefitBlock = 1024;          // leaves per refit chunk
+constexpr float FLT_MAX_F = 3.402823466e+38f;
 
 // float <-> unsigned with the same order, for atomicMin/atomicMax on floats
 __device__ __forceinline__ unsigned ord_enc(float f)
@@ -36,17 +41,21 @@ __global__ void k_bounds_init(unsigned* b)
     else if (t < 6) b[t] = 0u;
 }
 
-// Bounds of the box centroids: wave reduction, one atomic per wave and axis.
+// Bounds of the box centroids: grid-stride loop, wave and workgroup
+// reductions, one atomic per workgroup and axis.
 __global__ void __launch_bounds__(kBlock) k_bounds(const float4* __restrict__ lo, const float4* __restrict__ hi, int n,
                                                    unsigned* b)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ unsigned red[kBlock / 64][6];
     unsigned mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
-    if (i < n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float3 c = centroid(lo, hi, i);
-        mn[0] = mx[0] = ord_enc(c.x);
-        mn[1] = mx[1] = ord_enc(c.y);
-        mn[2] = mx[2] = ord_enc(c.z);
+        const unsigned e[3] = {ord_enc(c.x), ord_enc(c.y), ord_enc(c.z)};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            mn[a] = min(mn[a], e[a]);
+            mx[a] = max(mx[a], e[a]);
+        }
     }
     for (int o = 32; o > 0; o >>= 1) {
 #pragma unroll
@@ -55,12 +64,21 @@ __global__ void __launch_bounds__(kBlock) k_bounds(const float4* __restrict__ lo
             mx[a] = max(mx[a], (unsigned)__shfl_xor((int)mx[a], o, 64));
         }
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            atomicMin(&b[a], mn[a]);
-            atomicMax(&b[3 + a], mx[a]);
+            red[w][a] = mn[a];
+            red[w][3 + a] = mx[a];
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        unsigned v = red[0][a];
+        for (int k = 1; k < kBlock / 64; k++) v = a < 3 ? min(v, red[k][a]) : max(v, red[k][a]);
+        if (a < 3) atomicMin(&b[a], v);
+        else atomicMax(&b[a], v);
     }
 }
 
@@ -200,7 +218,8 @@ __device__ __forceinline__ int delta(const unsigned* keys, int n, int i, int j)
 __device__ __forceinline__ int enc_child(int c, bool leaf) { return leaf ? ~c : c; }
 
 __global__ void __launch_bounds__(kBlock) k_karras(const unsigned* __restrict__ keys, int n, float4* lo, float4* hi,
-                                                   int* parentNode, int* parentLeaf, int* flags)
+                                                   int* parentNode, int* parentLeaf, int* flags, int2* range,
+                                                   unsigned* enc)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
@@ -227,6 +246,10 @@ __global__ void __launch_bounds__(kBlock) k_karras(const unsigned* __restrict__ 
     if (rightLeaf) parentLeaf[gamma + 1] = i;
     else parentNode[gamma + 1] = i;
     flags[i] = 0;
+    range[i] = make_int2(min(i, j), max(i, j));
+    unsigned* e = enc + 6 * (size_t)i;
+    e[0] = e[1] = e[2] = 0xffffffffu;
+    e[3] = e[4] = e[5] = 0u;
     if (i == 0) parentNode[0] = -1;
 }
 
@@ -242,36 +265,97 @@ __device__ __forceinline__ void child_box(int c, const float4* lo, const float4*
     }
 }
 
-// Bottom-up boxes: each leaf thread walks towards the root; at every node the
-// first arriving child stops and the second, which then sees both children's
-// boxes, writes the node's box and goes on.
-__global__ void __launch_bounds__(kBlock) k_refit(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
-                                                  const int* __restrict__ sorted, int n, float4* lo, float4* hi,
-                                                  float4* leafLo, float4* leafHi, int* leafPrim,
-                                                  const int* __restrict__ parentNode,
-                                                  const int* __restrict__ parentLeaf, int* flags)
+__device__ __forceinline__ void node_union(int p, float4* lo, float4* hi, const float4* leafLo,
+                                           const float4* leafHi)
+{
+    const float4 L = lo[p], H = hi[p];
+    float4 a0, b0, a1, b1;
+    child_box(__float_as_int(L.w), lo, hi, leafLo, leafHi, a0, b0);
+    child_box(__float_as_int(H.w), lo, hi, leafLo, leafHi, a1, b1);
+    lo[p] = make_float4(fminf(a0.x, a1.x), fminf(a0.y, a1.y), fminf(a0.z, a1.z), L.w);
+    hi[p] = make_float4(fmaxf(b0.x, b1.x), fmaxf(b0.y, b1.y), fmaxf(b0.z, b1.z), H.w);
+}
+
+// Bottom-up boxes. Each leaf thread walks towards the root; at every node the
+// first arriving child stops and the second, which then holds both children's
+// boxes, completes the node and goes on (arrival parity: two arrivals per node
+// and pass, so the counters need no reset between passes).
+// A workgroup owns kRefitBlock consecutive sorted leaves. A node whose leaf
+// range lies inside that chunk is reached only by the workgroup's own threads:
+// the box goes through global memory under workgroup-scope ordering, which
+// costs nothing. A node whose range crosses chunks is completed through
+// device-scope atomics only: each arrival merges its box into the node's
+// order-encoded min/max words, waits until the merges are performed, then
+// counts itself; the second to count reads the union back with a neutral
+// merge. No cache write-back or invalidation is needed.
+__device__ __forceinline__ void merge_box(unsigned* enc, float4& a, float4& b)
+{
+    const unsigned m0 = __hip_atomic_fetch_min(&enc[0], ord_enc(a.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned m1 = __hip_atomic_fetch_min(&enc[1], ord_enc(a.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned m2 = __hip_atomic_fetch_min(&enc[2], ord_enc(a.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned x0 = __hip_atomic_fetch_max(&enc[3], ord_enc(b.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned x1 = __hip_atomic_fetch_max(&enc[4], ord_enc(b.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned x2 = __hip_atomic_fetch_max(&enc[5], ord_enc(b.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    a.x = fminf(a.x, ord_dec(m0));
+    a.y = fminf(a.y, ord_dec(m1));
+    a.z = fminf(a.z, ord_dec(m2));
+    b.x = fmaxf(b.x, ord_dec(x0));
+    b.y = fmaxf(b.y, ord_dec(x1));
+    b.z = fmaxf(b.z, ord_dec(x2));
+}
+
+__global__ void __launch_bounds__(kRefitBlock) k_refit(const float4* __restrict__ boxLo,
+                                                       const float4* __restrict__ boxHi,
+                                                       const int* __restrict__ sorted, int n, float4* lo, float4* hi,
+                                                       float4* leafLo, float4* leafHi, int* leafPrim,
+                                                       const int* __restrict__ parentNode,
+                                                       const int* __restrict__ parentLeaf, int* flags,
+                                                       const int2* __restrict__ range, unsigned* enc)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int prim = sorted[k];
-    const float4 a = boxLo[prim], b = boxHi[prim];
+    float4 a = boxLo[prim], b = boxHi[prim];
     leafLo[k] = make_float4(a.x, a.y, a.z, __int_as_float(prim));
     leafHi[k] = make_float4(b.x, b.y, b.z, 0.f);
     leafPrim[k] = prim;
     if (n == 1) return;
+    const int c0 = blockIdx.x * blockDim.x, c1 = c0 + blockDim.x;
     int p = parentLeaf[k];
     while (p >= 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const int2 r = range[p];
+        if (r.x < c0 || r.y >= c1) break;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        const int old = __hip_atomic_fetch_add(&flags[p], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if ((old & 1) == 0) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        node_union(p, lo, hi, leafLo, leafHi);
+        a = lo[p];
+        b = hi[p];
+        p = parentNode[p];
+    }
+    // (a, b) is the box of the child through which this thread reaches p
+    while (p >= 0) {
+        unsigned* e = enc + 6 * (size_t)p;
+        // Both arrivals merge first and count afterwards; the merges are
+        // waited for before counting, so the second to count sees, in the
+        // node's words, both boxes: it re-reads them with a neutral merge.
+        merge_box(e, a, b);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         const int old = __hip_atomic_fetch_add(&flags[p], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0) return;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if ((old & 1) == 0) return;
+        float4 ua = make_float4(FLT_MAX_F, FLT_MAX_F, FLT_MAX_F, 0.f);
+        float4 ub = make_float4(-FLT_MAX_F, -FLT_MAX_F, -FLT_MAX_F, 0.f);
+        merge_box(e, ua, ub);  // reads the union of both children
+        a = ua;
+        b = ub;
         const float4 L = lo[p], H = hi[p];
-        const int cl = __float_as_int(L.w), cr = __float_as_int(H.w);
-        float4 a0, b0, a1, b1;
-        child_box(cl, lo, hi, leafLo, leafHi, a0, b0);
-        child_box(cr, lo, hi, leafLo, leafHi, a1, b1);
-        lo[p] = make_float4(fminf(a0.x, a1.x), fminf(a0.y, a1.y), fminf(a0.z, a1.z), L.w);
-        hi[p] = make_float4(fmaxf(b0.x, b1.x), fmaxf(b0.y, b1.y), fmaxf(b0.z, b1.z), H.w);
+        lo[p] = make_float4(a.x, a.y, a.z, L.w);
+        hi[p] = make_float4(b.x, b.y, b.z, H.w);
+        // words back to the empty box for the next pass (no other arrival
+        // touches this node again in this pass)
+        e[0] = e[1] = e[2] = 0xffffffffu;
+        e[3] = e[4] = e[5] = 0u;
         p = parentNode[p];
     }
 }
@@ -284,7 +368,7 @@ Lbvh::~Lbvh()
 {
     dev_free(lo_); dev_free(hi_); dev_free(leafLo_); dev_free(leafHi_); dev_free(leafPrim_);
     dev_free(keys_); dev_free(keys2_); dev_free(vals_); dev_free(vals2_); dev_free(hist_);
-    dev_free(parentNode_); dev_free(parentLeaf_); dev_free(flags_); dev_free(bounds_);
+    dev_free(parentNode_); dev_free(parentLeaf_); dev_free(flags_); dev_free(bounds_); dev_free(range_); dev_free(enc_);
 }
 
 void Lbvh::reserve(int n)
@@ -294,7 +378,8 @@ void Lbvh::reserve(int n)
     dev_alloc(lo_, n); dev_alloc(hi_, n); dev_alloc(leafLo_, n); dev_alloc(leafHi_, n); dev_alloc(leafPrim_, n);
     dev_alloc(keys_, n); dev_alloc(keys2_, n); dev_alloc(vals_, n); dev_alloc(vals2_, n);
     dev_alloc(hist_, (size_t)256 * tiles);
-    dev_alloc(parentNode_, n); dev_alloc(parentLeaf_, n); dev_alloc(flags_, n);
+    dev_alloc(parentNode_, n); dev_alloc(parentLeaf_, n); dev_alloc(flags_, n); dev_alloc(range_, n);
+    dev_alloc(enc_, (size_t)6 * n);
     if (!bounds_) dev_alloc(bounds_, 6);
     cap_ = n;
 }
@@ -308,7 +393,7 @@ void Lbvh::build(hipStream_t s, const float4* boxLo, const float4* boxHi, int n)
     tiles_ = nblocks(n, kTile);
     const int nb = nblocks(n, kBlock);
     hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, bounds_);
-    hipLaunchKernelGGL(k_bounds, dim3(nb), dim3(kBlock), 0, s, boxLo, boxHi, n, bounds_);
+    hipLaunchKernelGGL(k_bounds, dim3(nb < 256 ? nb : 256), dim3(kBlock), 0, s, boxLo, boxHi, n, bounds_);
     hipLaunchKernelGGL(k_morton, dim3(nb), dim3(kBlock), 0, s, boxLo, boxHi, n, bounds_, keys_, vals_);
     unsigned *ki = keys_, *ko = keys2_;
     int *vi = vals_, *vo = vals2_;
@@ -322,10 +407,68 @@ void Lbvh::build(hipStream_t s, const float4* boxLo, const float4* boxHi, int n)
     // four passes: the sorted keys are back in keys_/vals_
     if (n > 1)
         hipLaunchKernelGGL(k_karras, dim3(nblocks(n - 1, kBlock)), dim3(kBlock), 0, s, ki, n, lo_, hi_, parentNode_,
-                           parentLeaf_, flags_);
-    hipLaunchKernelGGL(k_refit, dim3(nb), dim3(kBlock), 0, s, boxLo, boxHi, vi, n, lo_, hi_, leafLo_, leafHi_,
-                       leafPrim_, parentNode_, parentLeaf_, flags_);
+                           parentLeaf_, flags_, range_, enc_);
+    hipLaunchKernelGGL(k_refit, dim3(nblocks(n, kRefitBlock)), dim3(kRefitBlock), 0, s, boxLo, boxHi, vi, n, lo_, hi_,
+                       leafLo_, leafHi_, leafPrim_, parentNode_, parentLeaf_, flags_, range_, enc_);
     HIP_OK(hipGetLastError());
+}
+
+void Lbvh::refit(hipStream_t s, const float4* boxLo, const float4* boxHi)
+{
+    if (n_ == 0) return;
+    // the sorted primitive order of the last build lives in vals_
+    hipLaunchKernelGGL(k_refit, dim3(nblocks(n_, kRefitBlock)), dim3(kRefitBlock), 0, s, boxLo, boxHi, vals_, n_,
+                       lo_, hi_, leafLo_, leafHi_, leafPrim_, parentNode_, parentLeaf_, flags_, range_, enc_);
+}
+
+int Lbvh::verify(hipStream_t s, const float4* boxLo, const float4* boxHi) const
+{
+    HIP_OK(hipStreamSynchronize(s));
+    const int n = n_;
+    if (n == 0) return 0;
+    std::vector<float4> lo(n > 1 ? n - 1 : 0), hi(lo.size()), llo(n), lhi(n), blo(n), bhi(n);
+    std::vector<int> prim(n), pn(n), pl(n);
+    auto get = [](auto& dst, const void* src) {
+        if (!dst.empty()) HIP_OK(hipMemcpy(dst.data(), src, dst.size() * sizeof(dst[0]), hipMemcpyDeviceToHost));
+    };
+    get(lo, lo_); get(hi, hi_); get(llo, leafLo_); get(lhi, leafHi_); get(prim, leafPrim_);
+    get(blo, boxLo); get(bhi, boxHi);
+    if (n > 1) { get(pn, parentNode_); get(pl, parentLeaf_); }
+    int bad = 0;
+    std::vector<int> seen(n, 0);
+    for (int k = 0; k < n; k++) {
+        const int p = prim[k];
+        if (p < 0 || p >= n || seen[p]++) { bad++; continue; }
+        const float4 a = blo[p], b = bhi[p];
+        if (llo[k].x != a.x || llo[k].y != a.y || llo[k].z != a.z || lhi[k].x != b.x || lhi[k].y != b.y ||
+            lhi[k].z != b.z)
+            bad++;
+    }
+    if (n == 1) return bad;
+    std::vector<int> visits(n, 0);
+    for (int i = 0; i < n - 1; i++) {
+        float4 cl[2], ch[2];
+        for (int c = 0; c < 2; c++) {
+            const int e = __builtin_bit_cast(int, c == 0 ? lo[i].w : hi[i].w);
+            if (e < 0) {
+                const int leaf = ~e;
+                if (leaf >= n || pl[leaf] != i) { bad++; continue; }
+                visits[leaf]++;
+                cl[c] = llo[leaf]; ch[c] = lhi[leaf];
+            } else {
+                if (e >= n - 1 || pn[e] != i) { bad++; continue; }
+                cl[c] = lo[e]; ch[c] = hi[e];
+            }
+        }
+        if (bad) continue;
+        if (lo[i].x != std::min(cl[0].x, cl[1].x) || lo[i].y != std::min(cl[0].y, cl[1].y) ||
+            lo[i].z != std::min(cl[0].z, cl[1].z) || hi[i].x != std::max(ch[0].x, ch[1].x) ||
+            hi[i].y != std::max(ch[0].y, ch[1].y) || hi[i].z != std::max(ch[0].z, ch[1].z))
+            bad++;
+    }
+    for (int k = 0; k < n; k++) bad += visits[k] != 1;
+    if (pn[0] != -1) bad++;
+    return bad;
 }
 
 BvhView Lbvh::view() const { return BvhView{lo_, hi_, leafLo_, leafHi_, leafPrim_, n_}; }
