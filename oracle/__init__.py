@@ -60,6 +60,13 @@ def lib():
         L.oc_solve_bends.restype = _C.c_double
         L.oc_solve_contacts.argtypes = [vp, _C.c_float]
         L.oc_solve_contacts.restype = _C.c_double
+        L.oc_substep_detect.argtypes = [vp, _C.c_float]
+        L.oc_substep_solve.argtypes = [vp, _C.c_float]
+        L.oc_set_self_order.argtypes = [vp, _C.c_int, vp, _C.c_int]
+        L.oc_solve_self.argtypes = [vp, _C.c_int, _C.c_float]
+        L.oc_solve_self.restype = _C.c_double
+        L.oc_get_self_contacts.argtypes = [vp, _C.c_int, vp, vp, vp, _C.c_int]
+        L.oc_get_self_contacts.restype = _C.c_int
         L.oc_solve_edge_contacts.argtypes = [vp]
         L.oc_solve_edge_contacts.restype = _C.c_double
         L.oc_solve_tri_contacts.argtypes = [vp]
@@ -105,7 +112,7 @@ class Cloth:
     """One cloth in the oracle (float or double arithmetic)."""
 
     FLOAT_PARAMS = ("unit", "thickness", "tolerance", "shear", "bend", "stretch", "poisson", "friction",
-                    "area", "drag")
+                    "area", "drag", "selfShrink", "selfThreshold", "ccdVolume", "ccdBisect", "ccdVtDistance")
     INT_PARAMS = ("iterations", "collFlags", "bendFreq", "numSteps", "dihedral", "fem", "fixedIterations")
 
     def __init__(self, real64=False):
@@ -237,9 +244,37 @@ class Cloth:
         return ids, data
 
     def stats(self):
-        s = _np.zeros(3)
+        s = _np.zeros(6)
         self.L.oc_get_stats(self.h, _ptr(s))
-        return dict(iterations=int(s[0]), bendError=s[1], contactError=s[2])
+        return dict(iterations=int(s[0]), bendError=s[1], contactError=s[2], selfVTError=s[3],
+                    selfEEError=s[4], selfEECount=int(s[5]))
+
+    def substep_detect(self, h):
+        self.L.oc_substep_detect(self.h, h)
+
+    def substep_solve(self, h):
+        self.L.oc_substep_solve(self.h, h)
+
+    def solve_self(self, kind, h):
+        """One pass over the 'vt' or 'ee' self contacts; returns the error."""
+        return self.L.oc_solve_self(self.h, 0 if kind == "vt" else 1, h)
+
+    def set_self_order(self, kind, keys):
+        """kind 'vt' / 'ee': processing order of the next substep's self
+        contacts as their keys."""
+        k = _np.ascontiguousarray(keys, dtype=_np.uint64)
+        self.L.oc_set_self_order(self.h, 0 if kind == "vt" else 1, _ptr(k), len(k))
+
+    def self_contacts(self, kind):
+        """kind 'vt' / 'ee' -> keys (n,), ids (n, 4), data (n, 8): w1 w2 w3,
+        normal, side, lambda."""
+        kk = 0 if kind == "vt" else 1
+        n = self.L.oc_get_self_contacts(self.h, kk, None, None, None, 0)
+        keys = _np.zeros(n, dtype=_np.uint64)
+        ids = _np.zeros((n, 4), dtype=_np.int32)
+        data = _np.zeros((n, 8))
+        self.L.oc_get_self_contacts(self.h, kk, _ptr(keys), _ptr(ids), _ptr(data), n)
+        return keys, ids, data
 
     def set_order(self, kind, order):
         o = _np.ascontiguousarray(order, dtype=_np.int32)

@@ -29,6 +29,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 namespace oracle {
@@ -40,13 +41,18 @@ struct Params {
     // defaults: Physics/ClothModel.cpp:10-36
     float unit = 10.f, thickness = 0.3f, tolerance = 0.2f, shear = 5e8f, bend = 5.f, stretch = 2e10f;
     float poisson = 0.5f, friction = 0.4f, area = 10.f, drag = 1.f, selfShrink = 0.9f, selfCollThreshold = 0.003f;
+    // continuous self-collision thresholds, the reference's absolute constants
+    // (SelfCollisionDetector.cpp:235, 296, 313-317, 446): coplanarity volume,
+    // bisection stop, vertex-triangle distance
+    float ccdVolume = 0.001f, ccdBisect = 0.01f, ccdVtDistance = 0.1f;
     int iterations = 50, collFlags = 0, bendFreq = 2, numSteps = 1;
     bool dihedral = true, fem = false, fixedIterations = false;
 };
 
 struct Stats {
     int iterations = 0;
-    float bendError = 0, contactError = 0;
+    float bendError = 0, contactError = 0, selfVTError = 0, selfEEError = 0;
+    int selfEECount = 0;
 };
 
 template <class R>
@@ -90,6 +96,18 @@ struct TriContact {
     int clothTri, feature, mesh;
     V3<R> point, normal;
     R w1, w2, w3, lambda = 0, lambdaF = 0;
+};
+
+// Physics/ClothTypes.h:122-129: a vertex-triangle (i4 against i1 i2 i3) or
+// edge-edge (i1 i2 against i3 i4) self contact. `key` is (triangle << 32 |
+// vertex) or (edge1 << 32 | edge2), the canonical list order.
+template <class R>
+struct SelfContact {
+    uint32_t i1, i2, i3, i4;
+    R w1 = 0, w2 = 0, w3 = 0;
+    V3<R> normal{0, 0, 0};
+    R side = R(FLT_MAX), lambda = 0, error = 0;
+    uint64_t key = 0;
 };
 
 // Node-sampled grid SDF: Geometry/SDF.h:72-155, Geometry/SDF.cpp:10-40,210-256
@@ -277,6 +295,10 @@ public:
     // processing order of edge and triangle contacts by cloth edge / triangle
     // (empty: ascending index, the reference's list order)
     std::vector<int> edgeContactRank, triContactRank;
+    std::vector<SelfContact<R>> selfTris, selfEdges;
+    // processing order of the next substep's self contacts, as keys (empty:
+    // the canonical key order); consumed by substep_solve
+    std::vector<uint64_t> selfTriOrder, selfEdgeOrder;
 
     void clear()
     {
@@ -570,6 +592,7 @@ public:
                 mesh_collisions(meshes[c.mesh], (int)ci, c.mesh);
             }
         }
+        if (prm.collFlags & CF_SELF) self_collisions();
         auto byRank = [](const std::vector<int>& rank) {
             return [&rank](int a, int b) { return rank[a] < rank[b]; };
         };
@@ -985,20 +1008,343 @@ public:
         return error;
     }
 
+    // ------------------------------------------------------ self collision
+    // Swept boxes of ClothModel::ComputeTree (ClothCollision.cpp:240-315):
+    // prev and pos of the primitive's vertices, extruded by
+    // thickness + 0.5 tolerance; closed-interval overlap (Aabb3.h:72-87).
+    struct Box {
+        V3<float> lo, hi;
+    };
+    Box swept_box(std::initializer_list<uint32_t> ids, float tol) const
+    {
+        Box b{V3<float>{1e7f, 1e7f, 1e7f}, V3<float>{-1e7f, -1e7f, -1e7f}};
+        auto add = [&](V3<R> p) {
+            const V3<float> q{float(p.x), float(p.y), float(p.z)};
+            b.lo = V3<float>{std::min(b.lo.x, q.x), std::min(b.lo.y, q.y), std::min(b.lo.z, q.z)};
+            b.hi = V3<float>{std::max(b.hi.x, q.x), std::max(b.hi.y, q.y), std::max(b.hi.z, q.z)};
+        };
+        for (uint32_t i : ids) add(prev[i]);
+        for (uint32_t i : ids) add(pos[i]);
+        b.lo = V3<float>{b.lo.x - tol, b.lo.y - tol, b.lo.z - tol};
+        b.hi = V3<float>{b.hi.x + tol, b.hi.y + tol, b.hi.z + tol};
+        return b;
+    }
+    static bool overlap(const Box& a, const Box& b)
+    {
+        return !(a.hi.x < b.lo.x || a.hi.y < b.lo.y || a.hi.z < b.lo.z || b.hi.x < a.lo.x || b.hi.y < a.lo.y ||
+                 b.hi.z < a.lo.z);
+    }
+
+    // SelfCollisionDetector.cpp:263-276, in FP64
+    static double coplanarity(double t, const V3<double> x[4], const V3<double> v[4])
+    {
+        const V3<double> z1 = x[0] + t * v[0], z2 = x[1] + t * v[1], z3 = x[2] + t * v[2], z4 = x[3] + t * v[3];
+        return dot(cross(z2 - z1, z3 - z1), z4 - z1);
+    }
+    static double triple(V3<double> a, V3<double> b, V3<double> c) { return dot(c, cross(a, b)); }
+
+    // SelfCollisionDetector.cpp:414-488: bisection for the sign change of the
+    // coplanarity volume on [ta, tb]
+    bool check_interval(double ta, double tb, const V3<double> x[4], const V3<double> v[4], double& tx) const
+    {
+        const double eps = double(prm.ccdBisect);
+        double tx0 = ta, tx1 = tb;
+        double fx0 = coplanarity(tx0, x, v), fx1 = coplanarity(tx1, x, v);
+        if (fx0 * fx1 > 0) return false;
+        tx = tx0;
+        for (int l = 0; l < 40; l++) {
+            const double tmid = 0.5 * (tx0 + tx1);
+            const double fmid = coplanarity(tmid, x, v);
+            if (std::fabs(fmid) < eps) {
+                tx = tmid;
+                break;
+            }
+            if (fx0 * fmid < 0) {
+                tx1 = tmid;
+                fx1 = fmid;
+                tx = tx0;
+            } else if (fmid * fx1 < 0) {
+                tx0 = tmid;
+                fx0 = fmid;
+                tx = tmid;
+            } else {
+                return false;
+            }
+        }
+        return true;
+    }
+
+    // SelfCollisionDetector.cpp:278-412 (SolveCubic, CheckDegenerateCase,
+    // FindInflexionPoints), evaluated in FP64: the time in [0, 1] at which the
+    // four moving points become coplanar.
+    bool solve_cubic(const V3<R> xf[4], const V3<R> vf[4], double& toi) const
+    {
+        V3<double> x[4], v[4];
+        for (int k = 0; k < 4; k++) {
+            x[k] = V3<double>{double(xf[k].x), double(xf[k].y), double(xf[k].z)};
+            v[k] = V3<double>{double(vf[k].x), double(vf[k].y), double(vf[k].z)};
+        }
+        const double volEps = double(prm.ccdVolume);
+        const double f0 = coplanarity(0.0, x, v);
+        if (std::fabs(f0) < volEps) return false;
+        const V3<double> v10 = v[0] - v[3], v20 = v[1] - v[3], v30 = v[2] - v[3];
+        const V3<double> x10 = x[0] - x[3], x20 = x[1] - x[3], x30 = x[2] - x[3];
+        const double a = triple(v10, v20, v30);
+        const double b = triple(x10, v20, v30) + triple(v10, x20, v30) + triple(v10, v20, x30);
+        const double c = triple(x10, x20, v30) + triple(x10, v20, x30) + triple(v10, x20, x30);
+        if (std::fabs(a) < volEps) {
+            if (std::fabs(b) < volEps) return false;
+            const double d = triple(x10, x20, x30);
+            const double discr = c * c - 4 * b * d;
+            if (discr < 0) return false;
+            if (discr == 0) {
+                const double t = -0.5 * c / b;
+                if (t < 0 || t > 1) return false;
+                toi = t;
+                return true;
+            }
+            const double sq = std::sqrt(discr);
+            const double t1 = 0.5 * (-c - sq) / b, t2 = 0.5 * (-c + sq) / b;
+            const double t = std::min(t1, t2);
+            if (t < 0 || t > 1) return false;
+            toi = t;
+            return true;
+        }
+        double t1 = 0, t2 = 0;
+        int np;
+        const double discr = b * b - 3 * a * c;
+        if (discr < 0) {
+            np = 0;
+        } else if (discr == 0) {
+            t1 = t2 = -b / (3 * a);
+            np = 1;
+        } else {
+            const double sq = std::sqrt(discr);
+            t1 = (-b - sq) / (3 * a);
+            t2 = (-b + sq) / (3 * a);
+            if (t2 < t1) std::swap(t1, t2);
+            np = 2;
+        }
+        auto ci = [&](double ta, double tb) { return check_interval(ta, tb, x, v, toi); };
+        if (np == 0) return ci(0, 1);
+        if (np == 1) {
+            if (t1 < 0 || t1 > 1) return ci(0, 1);
+            return ci(0, t1) || ci(t1, 1);
+        }
+        if (t1 > 1 || t2 < 0 || (t1 < 0 && t2 > 1)) return ci(0, 1);
+        if (t1 < 0 && t2 < 1) return ci(0, t2) || ci(t2, 1);
+        if (t1 < 1 && t2 > 1) return ci(0, t1) || ci(t1, 1);
+        if (t1 > 0 && t2 < 1) return ci(0, t1) || ci(t1, t2) || ci(t2, 1);
+        return false;
+    }
+
+    // SelfCollisionDetector.cpp:137-261: vertex i4 against triangle (i1 i2 i3)
+    bool vertex_triangle(uint32_t i1, uint32_t i2, uint32_t i3, uint32_t i4, SelfContact<R>& out) const
+    {
+        const V3<R> x[4] = {prev[i1], prev[i2], prev[i3], prev[i4]};
+        const V3<R> v[4] = {pos[i1] - prev[i1], pos[i2] - prev[i2], pos[i3] - prev[i3], pos[i4] - prev[i4]};
+        double toid;
+        if (!solve_cubic(x, v, toid)) return false;
+        const R toi = R(float(toid));
+        const V3<R> z1 = x[0] + toi * v[0], z2 = x[1] + toi * v[1], z3 = x[2] + toi * v[2], z4 = x[3] + toi * v[3];
+        V3<R> p, coords;
+        closest_pt_point_triangle(z4, z1, z2, z3, p, coords);
+        const V3<R> delta = z4 - p;
+        const R eps = R(prm.ccdVtDistance);
+        if (!(dot(delta, delta) < eps * eps)) return false;
+        V3<R> triNormal = cross(x[1] - x[0], x[2] - x[0]);
+        normalize(triNormal);
+        const R side = dot(triNormal, x[3] - x[0]);
+        V3<R> n = cross(z2 - z1, z3 - z1);
+        normalize(n);
+        if (side < R(0)) n = -n;
+        out.i1 = i1; out.i2 = i2; out.i3 = i3; out.i4 = i4;
+        out.w1 = coords.x; out.w2 = coords.y; out.w3 = coords.z;
+        out.normal = n;
+        out.side = signed_volume(x[3], x[0], x[1], x[2]);
+        return true;
+    }
+
+    // SelfCollisionDetector.cpp:491-606: edge (i1 i2) against edge (i3 i4)
+    bool edge_edge(uint32_t i1, uint32_t i2, uint32_t i3, uint32_t i4, SelfContact<R>& out) const
+    {
+        const V3<R> x[4] = {prev[i1], prev[i2], prev[i3], prev[i4]};
+        const V3<R> v[4] = {pos[i1] - prev[i1], pos[i2] - prev[i2], pos[i3] - prev[i3], pos[i4] - prev[i4]};
+        double toid;
+        if (!solve_cubic(x, v, toid)) return false;
+        const R toi = R(float(toid));
+        const V3<R> z1 = x[0] + toi * v[0], z2 = x[1] + toi * v[1], z3 = x[2] + toi * v[2], z4 = x[3] + toi * v[3];
+        R s, t;
+        V3<R> c1, c2;
+        closest_pt_segm_segm(z1, z2, z3, z4, s, t, c1, c2);
+        const V3<R> d = c1 - c2;
+        const R eps = R(0.5f) * R(prm.thickness);
+        if (!(dot(d, d) <= eps * eps)) return false;
+        out.i1 = i1; out.i2 = i2; out.i3 = i3; out.i4 = i4;
+        out.w1 = s; out.w2 = t; out.w3 = 0;
+        R s0, t0;
+        closest_pt_segm_segm(x[0], x[1], x[2], x[3], s0, t0, c1, c2);
+        V3<R> n = c2 - c1;
+        normalize(n);
+        out.normal = n;
+        out.side = signed_volume(x[0], x[1], x[2], x[3]);
+        return true;
+    }
+
+    // SelfCollisionDetector.cpp:60-135 over the exact candidate set: every
+    // (triangle, vertex) pair, vertex not of the triangle and not pinned, and
+    // every (edge1 < edge2) pair without a shared vertex, whose swept boxes
+    // overlap. The reference's tree duplicates straddling primitives into
+    // several leaves and so may repeat a pair; each pair is tested once here.
+    void self_collisions()
+    {
+        selfTris.clear();
+        selfEdges.clear();
+        const float tol = prm.thickness + prm.tolerance * 0.5f;
+        const size_t nt = tris.size(), nv = pos.size(), ne = edges.size();
+        std::vector<Box> tb(nt), vb(nv), eb(ne);
+        for (size_t t = 0; t < nt; t++) tb[t] = swept_box({tris[t].i1, tris[t].i2, tris[t].i3}, tol);
+        for (size_t i = 0; i < nv; i++) vb[i] = swept_box({(uint32_t)i}, tol);
+        for (size_t e = 0; e < ne; e++) eb[e] = swept_box({edges[e].i1, edges[e].i2}, tol);
+        for (size_t t = 0; t < nt; t++) {
+            const uint32_t j1 = tris[t].i1, j2 = tris[t].i2, j3 = tris[t].i3;
+            for (size_t i = 0; i < nv; i++) {
+                if (i == j1 || i == j2 || i == j3 || invMass[i] == R(0)) continue;
+                if (!overlap(tb[t], vb[i])) continue;
+                SelfContact<R> c;
+                if (vertex_triangle(j1, j2, j3, (uint32_t)i, c)) {
+                    c.key = (uint64_t(t) << 32) | uint64_t(i);
+                    selfTris.push_back(c);
+                }
+            }
+        }
+        for (size_t e1 = 0; e1 + 1 < ne; e1++) {
+            for (size_t e2 = e1 + 1; e2 < ne; e2++) {
+                const uint32_t i1 = edges[e1].i1, i2 = edges[e1].i2, i3 = edges[e2].i1, i4 = edges[e2].i2;
+                if (i1 == i3 || i2 == i4 || i1 == i4 || i2 == i3) continue;
+                if (!overlap(eb[e1], eb[e2])) continue;
+                SelfContact<R> c;
+                if (edge_edge(i1, i2, i3, i4, c)) {
+                    c.key = (uint64_t(e1) << 32) | uint64_t(e2);
+                    selfEdges.push_back(c);
+                }
+            }
+        }
+    }
+
+    // processing order of the self contacts, given as their keys
+    void order_self(std::vector<SelfContact<R>>& list, const std::vector<uint64_t>& keys)
+    {
+        std::unordered_map<uint64_t, size_t> rank;
+        for (size_t k = 0; k < keys.size(); k++) rank[keys[k]] = k;
+        std::stable_sort(list.begin(), list.end(), [&](const SelfContact<R>& a, const SelfContact<R>& b) {
+            auto ra = rank.find(a.key), rb = rank.find(b.key);
+            const size_t xa = ra == rank.end() ? SIZE_MAX : ra->second;
+            const size_t xb = rb == rank.end() ? SIZE_MAX : rb->second;
+            return xa < xb;
+        });
+    }
+
+    // ClothCollisionHandler.cpp:364-436
+    R solve_self_tris(R h)
+    {
+        const R shrink = R(prm.selfShrink), soften = R(.9f);
+        R error = 0;
+        for (SelfContact<R>& c : selfTris) {
+            R len0 = R(prm.thickness) * shrink;
+            V3<R>& x1 = pos[c.i1];
+            V3<R>& x2 = pos[c.i2];
+            V3<R>& x3 = pos[c.i3];
+            V3<R>& x4 = pos[c.i4];
+            const R m1 = invMass[c.i1], m2 = invMass[c.i2], m3 = invMass[c.i3], m4 = invMass[c.i4];
+            V3<R> p, coords;
+            closest_pt_point_triangle(x4, x1, x2, x3, p, coords);
+            c.w1 = coords.x; c.w2 = coords.y; c.w3 = coords.z;
+            const V3<R> delta = x4 - p;
+            V3<R> n = cross(x2 - x1, x3 - x1);
+            normalize(n);
+            const R side = signed_volume(x4, x1, x2, x3);
+            if (c.side * side <= R(0)) len0 = R(prm.thickness) * R(0.1f);
+            if (dot(n, c.normal) < R(0)) n = -n;
+            const R len = dot(n, delta);
+            if (len > len0) continue;
+            const R im = c.w1 * c.w1 * m1 + c.w2 * c.w2 * m2 + c.w3 * c.w3 * m3 + m4;
+            const R s = R(1.f) / im;
+            const R err = len - len0;
+            c.error = err;
+            const R absErr = smax(R(0.f), -err);
+            if (absErr > error) error = absErr;
+            const R dLambda = soften * s * err;
+            const V3<R> disp = dLambda * n;
+            c.lambda += -dLambda / h;
+            x1 += disp * (c.w1 * m1);
+            x2 += disp * (c.w2 * m2);
+            x3 += disp * (c.w3 * m3);
+            x4 -= disp * m4;
+        }
+        return error;
+    }
+
+    // ClothCollisionHandler.cpp:439-541
+    R solve_self_edges(R h, int& count)
+    {
+        const R shrink = R(prm.selfShrink), soften = R(0.9f);
+        R error = 0;
+        count = 0;
+        for (SelfContact<R>& c : selfEdges) {
+            R thick = R(prm.thickness) * shrink;
+            V3<R>& x1 = pos[c.i1];
+            V3<R>& x2 = pos[c.i2];
+            V3<R>& x3 = pos[c.i3];
+            V3<R>& x4 = pos[c.i4];
+            const R m1 = invMass[c.i1], m2 = invMass[c.i2], m3 = invMass[c.i3], m4 = invMass[c.i4];
+            const R side = signed_volume(x1, x2, x3, x4);
+            V3<R> p, q;
+            R t1, t2;
+            closest_pt_segm_segm(x1, x2, x3, x4, t1, t2, p, q);
+            c.w1 = R(1) - t1;
+            c.w2 = R(1) - t2;
+            const R omw1 = R(1.f) - c.w1, omw2 = R(1.f) - c.w2;
+            const V3<R> delta = q - p;
+            V3<R> n = delta;
+            normalize(n);
+            if (dot(n, c.normal) < R(0.0f)) n = -n;
+            if (side * c.side < R(0)) thick = R(prm.thickness) * R(0.1f);
+            const R len = dot(n, delta);
+            if (len > thick) continue;
+            const R im = c.w1 * c.w1 * m1 + omw1 * omw1 * m2 + c.w2 * c.w2 * m3 + omw2 * omw2 * m4;
+            const R s = R(1.f) / im;
+            const R err = len - thick;
+            c.error = err;
+            const R absErr = smax(R(0.f), -err);
+            if (absErr > error) error = absErr;
+            const R dLambda = -soften * s * err;
+            c.lambda += dLambda / h;
+            const V3<R> disp = dLambda * n;
+            x1 -= disp * (omw1 * m1);
+            x2 -= disp * (c.w1 * m2);
+            x3 += disp * (omw2 * m3);
+            x4 += disp * (c.w2 * m4);
+            count++;
+        }
+        return error;
+    }
+
     // Physics/ClothPBD.cpp:324-369
     void solve_gs(R h)
     {
-        R cb = 0, cv = 0;
-        int k = 0;
+        R cb = 0, cv = 0, sct = 0, sce = 0;
+        int k = 0, sceCount = 0;
         while (true) {
             if (prm.fem) solve_triangles(h);
             if (prm.dihedral && k % prm.bendFreq == 0) cb = solve_bends();
             cv = solve_contacts(h);
             const R ce = solve_edge_contacts();
             const R ct = solve_tri_contacts();
-            // self contacts are not generated on this path and the FEM model
-            // reports no link error, so sct = selfEE = cl = 0 in the exit test
-            const R sct = 0, sce = 0, cl = 0;
+            sct = solve_self_tris(h);
+            sce = solve_self_edges(h, sceCount);
+            // the FEM model reports no link error, so cl = 0 in the exit test
+            const R cl = 0;
             k++;
             const float eps = 0.01f;
             if (!prm.fixedIterations && cl < R(0.05f) && cb < R(0.005f) && sce < R(prm.selfCollThreshold) &&
@@ -1009,10 +1355,20 @@ public:
         stats.iterations = k;
         stats.bendError = (float)cb;
         stats.contactError = (float)cv;
+        stats.selfVTError = (float)sct;
+        stats.selfEEError = (float)sce;
+        stats.selfEECount = sceCount;
     }
 
     // Physics/ClothPBD.cpp:255-322
     void substep(R h)
+    {
+        substep_detect(h);
+        substep_solve(h);
+    }
+
+    // prediction, collision detection, constraint reset
+    void substep_detect(R h)
     {
         const V3<R> gravity{R(0), R(-9.8f), R(0)};
         const R unit = prm.unit, drag = prm.drag;
@@ -1025,6 +1381,15 @@ public:
         }
         detect_collisions();
         for (auto& c : contacts) { c.lambda = 0; c.lambdaF = 0; }
+    }
+
+    // Gauss-Seidel iterations and velocity update
+    void substep_solve(R h)
+    {
+        if (!selfTriOrder.empty()) order_self(selfTris, selfTriOrder);
+        if (!selfEdgeOrder.empty()) order_self(selfEdges, selfEdgeOrder);
+        selfTriOrder.clear();
+        selfEdgeOrder.clear();
         solve_gs(h);
         const R invH = R(1.f) / h;
         for (size_t i = 0; i < pos.size(); i++) {
@@ -1095,6 +1460,11 @@ int oc_set_param_f(void* h, const char* name, float v)
         else if (n == "friction") p.friction = v;
         else if (n == "area") p.area = v;
         else if (n == "drag") p.drag = v;
+        else if (n == "selfShrink") p.selfShrink = v;
+        else if (n == "selfThreshold") p.selfCollThreshold = v;
+        else if (n == "ccdVolume") p.ccdVolume = v;
+        else if (n == "ccdBisect") p.ccdBisect = v;
+        else if (n == "ccdVtDistance") p.ccdVtDistance = v;
         else return -1;
         return 0;
     });
@@ -1320,6 +1690,9 @@ void oc_get_stats(void* h, double* out)
         out[0] = c.stats.iterations;
         out[1] = c.stats.bendError;
         out[2] = c.stats.contactError;
+        out[3] = c.stats.selfVTError;
+        out[4] = c.stats.selfEEError;
+        out[5] = c.stats.selfEECount;
     });
 }
 
@@ -1333,6 +1706,58 @@ void oc_substep(void* h, float hstep)
     with(h, [&](auto& c) {
         using R = typename std::remove_reference<decltype(c.pos[0].x)>::type;
         c.substep(R(hstep));
+    });
+}
+
+// The two halves of a substep, so that a test can hand the self contacts'
+// processing order in between.
+void oc_substep_detect(void* h, float hstep)
+{
+    with(h, [&](auto& c) {
+        using R = typename std::remove_reference<decltype(c.pos[0].x)>::type;
+        c.substep_detect(R(hstep));
+    });
+}
+void oc_substep_solve(void* h, float hstep)
+{
+    with(h, [&](auto& c) {
+        using R = typename std::remove_reference<decltype(c.pos[0].x)>::type;
+        c.substep_solve(R(hstep));
+    });
+}
+
+// one pass over the self contacts (kind 0: vertex-triangle, 1: edge-edge)
+double oc_solve_self(void* h, int kind, float hstep)
+{
+    return with(h, [&](auto& c) {
+        using R = typename std::remove_reference<decltype(c.pos[0].x)>::type;
+        int count = 0;
+        return kind == 0 ? (double)c.solve_self_tris(R(hstep)) : (double)c.solve_self_edges(R(hstep), count);
+    });
+}
+
+// kind 0: vertex-triangle, 1: edge-edge self contacts; processing order as keys
+void oc_set_self_order(void* h, int kind, const uint64_t* keys, int n)
+{
+    with(h, [&](auto& c) { (kind == 0 ? c.selfTriOrder : c.selfEdgeOrder).assign(keys, keys + n); });
+}
+
+// self contacts in list order: keys, ids (i1 i2 i3 i4), data (w1 w2 w3,
+// normal 3, side, lambda)
+int oc_get_self_contacts(void* h, int kind, uint64_t* keys, int* ids, double* data, int cap)
+{
+    return with(h, [&](auto& c) {
+        const auto& list = kind == 0 ? c.selfTris : c.selfEdges;
+        const int n = (int)list.size();
+        for (int k = 0; k < n && k < cap; k++) {
+            const auto& x = list[k];
+            keys[k] = x.key;
+            ids[4 * k] = x.i1; ids[4 * k + 1] = x.i2; ids[4 * k + 2] = x.i3; ids[4 * k + 3] = x.i4;
+            const double r[8] = {(double)x.w1, (double)x.w2, (double)x.w3, (double)x.normal.x,
+                                 (double)x.normal.y, (double)x.normal.z, (double)x.side, (double)x.lambda};
+            memcpy(data + 8 * k, r, sizeof(r));
+        }
+        return n;
     });
 }
 

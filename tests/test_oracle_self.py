@@ -1,0 +1,140 @@
+"""The oracle's continuous self-collision (SelfCollisionDetector.cpp restated
+with an FP64 cubic) and self-contact projection (ClothCollisionHandler.cpp:
+364-541), pinned by closed forms on constructed crossings.
+
+Scene: triangle A = (0, 1, 2) in the plane y = 0 whose corners slide in the
+plane (so the coplanarity volume is a true cubic), and vertex 3 (of a far
+triangle B = (3, 4, 5)) moving straight down through A's interior. The
+coplanarity time is when vertex 3 reaches y = 0: t* = h / u.
+"""
+import numpy as np
+import pytest
+
+CF_SELF = 4
+
+A = np.array([[0.0, 0.0, 0.0], [0.4, 0.0, 0.0], [0.0, 0.0, 0.4]])
+VA = np.array([[0.01, 0.0, 0.0], [0.0, 0.0, 0.01], [-0.005, 0.0, 0.003]])
+FAR = np.array([[3.0, 0.3, 3.0], [3.2, 0.3, 3.0]])
+
+
+def scene(oracle_mod, h=0.005, u=0.01, x4=(0.1, 0.1), params=None):
+    prev = np.vstack([A, [[x4[0], h, x4[1]]], FAR])
+    pos = prev.copy()
+    pos[:3] += VA
+    pos[3, 1] -= u
+    c = oracle_mod.Cloth()
+    p = dict(thickness=0.01, tolerance=0.01, collFlags=CF_SELF, friction=0.0, fem=1,
+             ccdVolume=1e-12, ccdBisect=1e-14, ccdVtDistance=0.001)
+    p.update(params or {})
+    c.set(**p)
+    c.init_arrays(prev, prev[:, [0, 2]], np.ones(6), np.array([[0, 1, 2], [3, 4, 5]], dtype=np.int32))
+    c.put("prev", prev)
+    c.put("pos", pos)
+    return c, prev, pos
+
+
+def test_vertex_triangle_ccd_closed_form(oracle_mod):
+    h, u = 0.005, 0.01
+    c, prev, pos = scene(oracle_mod, h, u)
+    c.detect()
+    keys, ids, data = c.self_contacts("vt")
+    hits = [k for k in range(len(keys)) if tuple(ids[k]) == (0, 1, 2, 3)]
+    assert len(hits) == 1
+    k = hits[0]
+    assert keys[k] == (0 << 32) | 3
+    # at t* = 0.5 the triangle has slid by t* VA; the vertex projects to (0.1, 0.1)
+    t = h / u
+    z = A + t * VA
+    q = np.array([0.1, 0.0, 0.1])
+    # barycentric coordinates of q in the slid triangle (2D, x-z)
+    m = np.array([[z[1, 0] - z[0, 0], z[2, 0] - z[0, 0]], [z[1, 2] - z[0, 2], z[2, 2] - z[0, 2]]])
+    w23 = np.linalg.solve(m, q[[0, 2]] - z[0, [0, 2]])
+    np.testing.assert_allclose(data[k, :3], [1 - w23.sum(), w23[0], w23[1]], atol=1e-5)
+    # normal: the slid triangle's normal turned towards where the vertex came from (+y)
+    np.testing.assert_allclose(data[k, 3:6], [0, 1, 0], atol=1e-6)
+    assert data[k, 6] == oracle_mod.signed_volume(prev[3], prev[0], prev[1], prev[2])
+
+
+def test_vertex_starting_coplanar_is_ignored(oracle_mod):
+    """|f(0)| below the volume threshold: "already coplanar" (SolveCubic)."""
+    c, _, _ = scene(oracle_mod, h=0.0)
+    c.detect()
+    assert len(c.self_contacts("vt")[0]) == 0
+
+
+def test_vertex_passing_beside_the_triangle_is_ignored(oracle_mod):
+    """The vertex crosses the plane outside the triangle: coplanar at t*, but
+    farther than the distance threshold from the triangle."""
+    c, _, _ = scene(oracle_mod, x4=(0.5, 0.5))
+    c.detect()
+    assert len(c.self_contacts("vt")[0]) == 0
+
+
+def test_pinned_vertex_is_not_a_candidate(oracle_mod):
+    c, prev, pos = scene(oracle_mod)
+    m = np.ones(6)
+    m[3] = 0.0
+    c.put("invMass", m)
+    c.detect()
+    assert len(c.self_contacts("vt")[0]) == 0
+
+
+def test_self_triangle_projection_closes_90_percent(oracle_mod):
+    """One pass of SolveSelfTrisPosition (unit masses): the distance of the
+    vertex from the triangle along the contact normal moves 90 % of the way
+    to the target, which is 0.1 x thickness once the vertex has crossed."""
+    c, prev, pos = scene(oracle_mod)
+    c.detect()
+    assert len(c.self_contacts("vt")[0]) == 1
+    p0 = c.get("pos")
+    d0 = p0[3, 1]  # the slid triangle stays in y = 0; the contact normal is +y
+    assert d0 < 0
+    err = c.solve_self("vt", 1.0 / 240)
+    len0 = 0.1 * 0.01
+    assert err == pytest.approx(len0 - d0, abs=1e-7)
+    p1 = c.get("pos")
+    _, _, data = c.self_contacts("vt")
+    w = data[0, :3]
+    plane_pt = w[0] * p1[0] + w[1] * p1[1] + w[2] * p1[2]
+    assert p1[3, 1] - plane_pt[1] == pytest.approx(len0 + 0.1 * (d0 - len0), abs=1e-7)
+    # the triangle moved only along the normal
+    np.testing.assert_allclose(p1[:3, [0, 2]], p0[:3, [0, 2]], atol=1e-7)
+
+
+def test_edge_edge_ccd_and_projection(oracle_mod):
+    """Two edges rotating about their midpoints, which approach along the
+    common normal: edge (0, 1) at height h falls through edge (2, 3) at
+    y = 0. The crossing is found at the midpoints (weights 1/2), the contact
+    normal points from the first edge to the second at t = 0 (down), and one
+    pass (unit masses) closes 90 % of the gap to 0.1 x thickness."""
+    h, u, a, b = 0.004, 0.01, 0.003, 0.002
+    prev = np.array([[-0.25, h, 0.05], [0.25, h, 0.05], [0.0, 0.0, -0.2], [0.0, 0.0, 0.3],
+                     [5.0, 1.0, 5.0], [5.2, 1.0, 5.0]])
+    vel = np.array([[0, -u, a], [0, -u, -a], [b, 0, 0], [-b, 0, 0], [0, 0, 0], [0, 0, 0]])
+    pos = prev + vel
+    c = oracle_mod.Cloth()
+    c.set(thickness=0.01, tolerance=0.01, collFlags=CF_SELF, friction=0.0, fem=1,
+          ccdVolume=1e-14, ccdBisect=1e-16, ccdVtDistance=0.001)
+    tris = np.array([[0, 1, 4], [2, 3, 5]], dtype=np.int32)
+    c.init_arrays(prev, prev[:, [0, 2]], np.ones(6), tris)
+    c.put("prev", prev)
+    c.put("pos", pos)
+    c.detect()
+    keys, ids, data = c.self_contacts("ee")
+    sel = [k for k in range(len(keys)) if tuple(ids[k]) == (0, 1, 2, 3)]
+    assert len(sel) == 1
+    k = sel[0]
+    np.testing.assert_allclose(data[k, :2], [0.5, 0.5], atol=1e-6)
+    np.testing.assert_allclose(data[k, 3:6], [0, -1, 0], atol=1e-6)
+    assert data[k, 6] == oracle_mod.signed_volume(prev[0], prev[1], prev[2], prev[3])
+    p0 = c.get("pos")
+    gap0 = 0.0 - p0[0, 1]  # edge 2's midpoint minus edge 1's, along +y
+    assert gap0 > 0
+    c.solve_self("ee", 1.0 / 240)
+    p1 = c.get("pos")
+    m1 = 0.5 * (p1[0] + p1[1])
+    m2 = 0.5 * (p1[2] + p1[3])
+    # along the (flipped) normal -y: distance = m1.y - m2.y, target 0.1 thickness
+    d0 = -gap0
+    thick = 0.1 * 0.01
+    assert m1[1] - m2[1] == pytest.approx(thick + 0.1 * (d0 - thick), abs=1e-7)
