@@ -226,8 +226,30 @@ SCEXP int scLbvhSelfCheck(int n, unsigned int seed, int refits, double* msOut);
 SCEXP int scGetEdgeContactData(scModel model, int* ids, float* data, int cap);
 SCEXP int scGetTriContactData(scModel model, int* ids, float* data, int cap);
 
+/* Continuous self collision (CF_SELF): the detector of
+ * Physics/SelfCollisionDetector.cpp (vertex-triangle and edge-edge CCD over
+ * swept boxes; the coplanarity cubic in FP64) and the projections of
+ * ClothCollisionHandler.cpp:364-541.
+ * scGetSelfContacts: kind 0 = vertex-triangle (ids i1 i2 i3 = triangle,
+ * i4 = vertex), 1 = edge-edge (i1 i2 against i3 i4), in the order they are
+ * solved (colour group by colour group, then key); keys = (triangle << 32 |
+ * vertex) or (edge1 << 32 | edge2); data = (w1, w2, w3, normal[3], side,
+ * lambda) per contact. Returns the count.
+ * scSetSelfParams: ClothModel's self shrink and exit threshold
+ * (Physics/ClothModel.cpp:10-36 defaults 0.9, 0.003).
+ * scSetSelfCCD: the detector's absolute thresholds (coplanarity volume 0.001,
+ * bisection 0.01, vertex-triangle distance 0.1 in the reference) for scenes
+ * at other length scales.
+ * scSetSelfCapacity: contacts per kind per substep (0 = automatic); a step
+ * that exceeds it reports an error. */
+SCEXP int scGetSelfContacts(scModel model, int kind, unsigned long long* keys, int* ids, float* data, int cap);
+SCEXP void scSetSelfParams(scModel model, float shrink, float threshold);
+SCEXP void scSetSelfCCD(scModel model, float volume, float bisect, float vtDistance);
+SCEXP void scSetSelfCapacity(scModel model, int capacity);
+
 /* Solver statistics of the last substep (Physics/ClothModel.h:32-40):
- * out[0] iterations, out[1] bend error, out[2] vertex-contact error. */
+ * out[0] iterations, out[1] bend error, out[2] vertex-contact error,
+ * out[3] self vertex-triangle error, out[4] self edge-edge error. */
 SCEXP void scGetSolverStats(scModel model, float* out);
 
 /* Kernel timing with HIP events on the solver's stream. which: 0 FEM sweep,

@@ -370,6 +370,10 @@ def from_scene(scene, real64=False, tri_order=None, bend_order=None):
     c.init_grid(scene.n, scene.n, float(scene.inc), scene.offset, scene.horizontal, scene.attached)
     if scene.perturb_amplitude != 0.0:
         c.put("pos", S.perturbed_positions(scene, c.get("pos")))
+    if scene.fold_gap != 0.0:
+        c.put("pos", S.folded_positions(scene, c.get("pos")))
+    if scene.ccd is not None:
+        c.set(ccdVolume=scene.ccd[0], ccdBisect=scene.ccd[1], ccdVtDistance=scene.ccd[2])
     if tri_order is not None:
         c.set_order("tri", tri_order)
     if bend_order is not None:

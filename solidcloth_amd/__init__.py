@@ -108,6 +108,8 @@ def _sig(L):
         "scGetContactFeatures": (i, [H, vp, i]), "scGetEdgeContactData": (i, [H, vp, vp, i]),
         "scGetTriContactData": (i, [H, vp, vp, i]),
         "scLbvhSelfCheck": (i, [i, C.c_uint, i, vp]),
+        "scGetSelfContacts": (i, [H, i, vp, vp, vp, i]), "scSetSelfParams": (None, [H, f, f]),
+        "scSetSelfCCD": (None, [H, f, f, f]), "scSetSelfCapacity": (None, [H, i]),
         "scGetLastError": (C.c_char_p, []),
         "scGetErrorCount": (i, []), "scGetBuildInfo": (C.c_char_p, []),
     }
@@ -382,9 +384,30 @@ class ClothModel:
         return self._feature_contacts("scGetTriContactData", 11)
 
     def stats(self):
-        out = np.zeros(3, dtype=np.float32)
+        out = np.zeros(5, dtype=np.float32)
         _checked("scGetSolverStats", self.h, _ptr(out))
-        return dict(iterations=int(out[0]), bendError=float(out[1]), contactError=float(out[2]))
+        return dict(iterations=int(out[0]), bendError=float(out[1]), contactError=float(out[2]),
+                    selfVTError=float(out[3]), selfEEError=float(out[4]))
+
+    def self_contacts(self, kind):
+        """kind 'vt' / 'ee' -> keys (n,), ids (n, 4), data (n, 8): w1 w2 w3,
+        normal, side, lambda; in the order they are solved."""
+        k = 0 if kind == "vt" else 1
+        n = _checked("scGetSelfContacts", self.h, k, None, None, None, 0)
+        keys = np.zeros(n, dtype=np.uint64)
+        ids = np.zeros((n, 4), dtype=np.int32)
+        data = np.zeros((n, 8), dtype=np.float32)
+        _checked("scGetSelfContacts", self.h, k, _ptr(keys), _ptr(ids), _ptr(data), n)
+        return keys, ids, data
+
+    def SetSelfParams(self, shrink, threshold):
+        _checked("scSetSelfParams", self.h, float(shrink), float(threshold))
+
+    def SetSelfCCD(self, volume, bisect, vt_distance):
+        _checked("scSetSelfCCD", self.h, float(volume), float(bisect), float(vt_distance))
+
+    def SetSelfCapacity(self, capacity):
+        _checked("scSetSelfCapacity", self.h, int(capacity))
 
     def enable_kernel_timing(self, on=True):
         _checked("scEnableKernelTiming", self.h, bool(on))

@@ -675,6 +675,58 @@ static int feature_contacts(const std::vector<FeatureContactOut>& v, int nw, int
     return (int)v.size();
 }
 
+int scGetSelfContacts(scModel model, int kind, unsigned long long* keys, int* ids, float* data, int cap)
+{
+    return guard_r<int>(-1, [&] {
+        if (kind != 0 && kind != 1) throw std::runtime_error("self contact kind must be 0 or 1");
+        ClothModel* m = M(model);
+        m->download_contacts();
+        const auto& v = m->selfContactsHost[kind];
+        const int n = std::min(cap, (int)v.size());
+        for (int i = 0; i < n; i++) {
+            if (keys) keys[i] = v[i].key;
+            if (ids)
+                for (int k = 0; k < 4; k++) ids[4 * i + k] = v[i].ids[k];
+            if (data) {
+                float* d = data + 8 * (size_t)i;
+                d[0] = v[i].w[0]; d[1] = v[i].w[1]; d[2] = v[i].w[2];
+                d[3] = v[i].normal[0]; d[4] = v[i].normal[1]; d[5] = v[i].normal[2];
+                d[6] = v[i].side; d[7] = v[i].lambda;
+            }
+        }
+        return (int)v.size();
+    });
+}
+
+void scSetSelfParams(scModel model, float shrink, float threshold)
+{
+    guard([&] {
+        ClothModel* m = M(model);
+        m->mSelfShrink = shrink;
+        m->mSelfCollThreshold = threshold;
+    });
+}
+
+void scSetSelfCCD(scModel model, float volume, float bisect, float vtDistance)
+{
+    guard([&] {
+        if (!(volume >= 0.f) || !(bisect >= 0.f) || !(vtDistance >= 0.f))
+            throw std::runtime_error("self-collision thresholds must be non-negative");
+        ClothModel* m = M(model);
+        m->mCcdVolume = volume;
+        m->mCcdBisect = bisect;
+        m->mCcdVtDistance = vtDistance;
+    });
+}
+
+void scSetSelfCapacity(scModel model, int capacity)
+{
+    guard([&] {
+        if (capacity < 0) throw std::runtime_error("negative self-contact capacity");
+        M(model)->mSelfCapacity = capacity;
+    });
+}
+
 int scGetEdgeContactData(scModel model, int* ids, float* data, int cap)
 {
     return guard_r<int>(-1, [&] {
@@ -703,6 +755,8 @@ void scGetSolverStats(scModel model, float* out)
         out[0] = (float)m->statIterations;
         out[1] = m->statBendError;
         out[2] = m->statContactError;
+        out[3] = m->statSelfVTError;
+        out[4] = m->statSelfEEError;
     });
 }
 

@@ -38,7 +38,12 @@ struct SolverCtl {
     unsigned ce;     // edge-contact error of the current iteration
     unsigned ct;     // triangle-contact error of the current iteration
     unsigned maxTv;  // largest |pos - prev| after prediction (mesh contacts)
-    int pad[2];
+    unsigned sct;    // self vertex-triangle error of the current iteration
+    unsigned sce;    // self edge-edge error of the current iteration
+    int selfCount[2];    // self contacts found this substep (vertex-triangle, edge-edge)
+    int selfColours[2];  // colour groups of each kind
+    int selfOverflow;    // contacts beyond capacity, or colouring rounds exhausted
+    int pad;
 };
 
 enum { MAX_COLLIDABLES = 8 };
@@ -125,7 +130,7 @@ struct SlabSet {
     int base;  // first slab of the launched phase
 };
 
-enum { kSlabBlock = 256, kMaxSlabColours = 63 };
+enum { kSlabBlock = 256, kMaxSlabColours = 63, kMaxSelfColours = 512 };
 
 // Vertex contacts applied by the FEM slab kernel while it stages vertices
 // (the previous iteration's contacts, by each vertex's owner slab).
@@ -135,6 +140,31 @@ struct SlabContacts {
     float4* vel;
     float thickness, mu, invH;
     int enabled;
+};
+
+// Self contacts of one kind (vertex i4 against triangle i1 i2 i3, or edge i1 i2
+// against edge i3 i4), in detection order; `order` lists them colour group by
+// colour group (group g spans order[groupOff[g] .. groupOff[g+1])), members of
+// a group sharing no vertex. key = (triangle << 32 | vertex) or
+// (edge1 << 32 | edge2).
+struct SelfBuf {
+    int4* ids;
+    float4* W;  // w1, w2, w3, side
+    float4* N;  // normal, lambda
+    unsigned long long* key;
+    int* colour;
+    int* order;
+    int* groupOff;
+    int cap;
+    int maxColours;
+};
+
+struct SelfParams {
+    float tol;       // swept-box extrusion: thickness + 0.5 tolerance
+    float thickness;
+    float shrink;
+    double volume, bisect;  // CCD thresholds (FP64 cubic)
+    float vtDistance;
 };
 
 // Launchers (kernels.hip). All are asynchronous on the given stream.
@@ -173,5 +203,17 @@ void launch_edge_contacts(hipStream_t s, float4* pos, const float4* prev, const 
                           const MeshGeo* geo, float thickness, float mu, SolverCtl* ctl);
 void launch_tri_contacts(hipStream_t s, float4* pos, const float4* prev, const int4* cTris, FeatureContactBuf tc,
                          const MeshGeo* geo, float thickness, float mu, SolverCtl* ctl);
+
+// Self collision (self.hip): swept boxes of triangles and edges, continuous
+// vertex-triangle and edge-edge detection over the LBVHs, colouring of the
+// contacts, and their Gauss-Seidel passes.
+void launch_self_boxes(hipStream_t s, const float4* pos, const float4* prev, const int4* cTris, int nt,
+                       const int2* cEdges, int ne, float tol, float4* triLo, float4* triHi, float4* edgeLo,
+                       float4* edgeHi);
+void launch_self_detect(hipStream_t s, const float4* pos, const float4* prev, const int4* cTris, const int2* cEdges,
+                        int nv, int ne, BvhView triBvh, BvhView edgeBvh, SelfParams sp, SelfBuf vt, SelfBuf ee,
+                        SolverCtl* ctl);
+void launch_self_colour(hipStream_t s, SelfBuf b, int kind, unsigned long long* vmax, SolverCtl* ctl);
+void launch_self_solve(hipStream_t s, float4* pos, SelfBuf b, int kind, SelfParams sp, float h, SolverCtl* ctl);
 
 }  // namespace sc

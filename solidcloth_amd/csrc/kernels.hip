@@ -28,6 +28,10 @@ __global__ void k_predict(float4* __restrict__ pos, float4* __restrict__ prev, f
         ctl->cv = 0;
         ctl->ce = 0;
         ctl->ct = 0;
+        ctl->sct = 0;
+        ctl->sce = 0;
+        ctl->selfCount[0] = 0;
+        ctl->selfCount[1] = 0;
         ctl->iterations = 0;
     }
     if (i >= n) return;
@@ -126,6 +130,10 @@ __global__ void k_predict_detect(float4* __restrict__ pos, float4* __restrict__ 
         ctl->cv = 0;
         ctl->ce = 0;
         ctl->ct = 0;
+        ctl->sct = 0;
+        ctl->sce = 0;
+        ctl->selfCount[0] = 0;
+        ctl->selfCount[1] = 0;
         ctl->iterations = 0;
     }
     if (i >= n) return;
@@ -528,8 +536,7 @@ __global__ void __launch_bounds__(kBlock) k_contacts_velocity(float4* __restrict
 }
 
 // Physics/ClothPBD.cpp:354-361 — iteration counter and the early-exit test.
-// Self contacts are not generated on this path and the FEM model reports no
-// link error, so those terms of the test are 0.
+// The FEM model reports no link error, so that term of the test is 0.
 __global__ void k_iter_end(SolverCtl* ctl, int maxIter, int fixedIter, int bendFreq, int dihedral, float selfThr)
 {
     if (ctl->done) return;
@@ -537,7 +544,8 @@ __global__ void k_iter_end(SolverCtl* ctl, int maxIter, int fixedIter, int bendF
     ctl->k = k;
     float cb = __uint_as_float(ctl->cb), cv = __uint_as_float(ctl->cv);
     const float ce = __uint_as_float(ctl->ce), ct = __uint_as_float(ctl->ct);
-    const float cl = 0.f, sct = 0.f, sce = 0.f, eps = 0.01f;
+    const float sct = __uint_as_float(ctl->sct), sce = __uint_as_float(ctl->sce);
+    const float cl = 0.f, eps = 0.01f;
     bool exitNow = !fixedIter && cl < 0.05f && cb < 0.005f && sce < selfThr && sct < selfThr && cv < eps &&
                    ce < eps && ct < eps;
     if (exitNow || k >= maxIter) {
@@ -550,6 +558,8 @@ __global__ void k_iter_end(SolverCtl* ctl, int maxIter, int fixedIter, int bendF
     ctl->cv = 0;
     ctl->ce = 0;
     ctl->ct = 0;
+    ctl->sct = 0;
+    ctl->sce = 0;
     if (dihedral && k % bendFreq == 0) ctl->cb = 0;
 }
 

@@ -27,6 +27,10 @@ __global__ void __launch_bounds__(kBlock) k_predict_mesh(float4* __restrict__ po
         ctl->cv = 0;
         ctl->ce = 0;
         ctl->ct = 0;
+        ctl->sct = 0;
+        ctl->sce = 0;
+        ctl->selfCount[0] = 0;
+        ctl->selfCount[1] = 0;
         ctl->iterations = 0;
     }
     if (i < ec.groups) ec.groupCount[i] = 0;

@@ -264,6 +264,14 @@ void GpuSolver::release()
         dev_free(f->group); dev_free(f->groupBase); dev_free(f->groupCount); dev_free(f->list);
         f->slotCap = 0;
     }
+    dev_free(self.triLo); dev_free(self.triHi); dev_free(self.edgeLo); dev_free(self.edgeHi); dev_free(self.vmax);
+    for (SelfBuf& b : self.buf) {
+        dev_free(b.ids); dev_free(b.W); dev_free(b.N); dev_free(b.key); dev_free(b.colour); dev_free(b.order);
+        dev_free(b.groupOff);
+        b.cap = 0;
+    }
+    self.cap = 0;
+    self.built = false;
     for (DeviceSlabs* d : {&triSlabs, &bendSlabs}) {
         dev_free(d->lidx); dev_free(d->info); dev_free(d->colOff); dev_free(d->verts);
     }
@@ -467,6 +475,7 @@ void ClothModel::build_device_topology()
     };
     groups(g.ec, edgeContactCol, g.ne);
     groups(g.tc, triContactCol, g.nt);
+    g.self.built = false;
     mTopoDirty = false;
 }
 
@@ -563,6 +572,50 @@ void ClothModel::ensure_feature_capacity(int edgeSlots, int triSlots)
     }
 }
 
+SelfParams ClothModel::self_params() const
+{
+    SelfParams sp;
+    sp.tol = mThickness + mTolerance * 0.5f;
+    sp.thickness = mThickness;
+    sp.shrink = mSelfShrink;
+    sp.volume = (double)mCcdVolume;
+    sp.bisect = (double)mCcdBisect;
+    sp.vtDistance = mCcdVtDistance;
+    return sp;
+}
+
+// Buffers of the self-collision path and a fresh LBVH pair over the current
+// swept boxes (called outside graph capture, once per step).
+void ClothModel::ensure_self()
+{
+    GpuSolver& g = gpu;
+    GpuSolver::Self& S = g.self;
+    const int cap = mSelfCapacity > 0 ? mSelfCapacity : std::max(4096, 2 * (g.nv + g.ne));
+    if (cap != S.cap) {
+        for (SelfBuf& b : S.buf) {
+            dev_alloc(b.ids, cap); dev_alloc(b.W, cap); dev_alloc(b.N, cap); dev_alloc(b.key, cap);
+            dev_alloc(b.colour, cap); dev_alloc(b.order, cap); dev_alloc(b.groupOff, kMaxSelfColours + 1);
+            b.cap = cap;
+            b.maxColours = kMaxSelfColours;
+        }
+        S.cap = cap;
+        if (g.graphExec) { (void)hipGraphExecDestroy(g.graphExec); g.graphExec = nullptr; }
+        if (g.graph) { (void)hipGraphDestroy(g.graph); g.graph = nullptr; }
+        g.graphKey.clear();
+    }
+    if (!S.triLo) {
+        dev_alloc(S.triLo, std::max(g.nt, 1)); dev_alloc(S.triHi, std::max(g.nt, 1));
+        dev_alloc(S.edgeLo, std::max(g.ne, 1)); dev_alloc(S.edgeHi, std::max(g.ne, 1));
+        dev_alloc(S.vmax, std::max(g.nv, 1));
+    }
+    // hierarchy from the current configuration; substeps refit it
+    launch_self_boxes(g.stream, g.d_pos, g.d_prev, g.d_cTris, g.nt, g.d_cEdges, g.ne, mThickness + mTolerance * 0.5f,
+                      S.triLo, S.triHi, S.edgeLo, S.edgeHi);
+    S.triB.build(g.stream, S.triLo, S.triHi, g.nt);
+    S.edgeB.build(g.stream, S.edgeLo, S.edgeHi, g.ne);
+    S.built = true;
+}
+
 FeatureContactBuf ClothModel::feature_buf(GpuSolver::Features& f) const
 {
     return FeatureContactBuf{f.count, f.P, f.N, f.W, f.F, f.group, f.groupBase, f.groupCount, f.list,
@@ -634,7 +687,6 @@ CollSet ClothModel::make_collset(int& slots, int& edgeSlots, int& triSlots)
         }
         cs.c[cs.n++] = d;
     }
-    if (mCollFlags & CF_SELF) throw std::runtime_error("self collision is not supported by the GPU substep");
     if (meshes) {
         const bool same = gs.geoHost.size() == geo.size() &&
                           memcmp(gs.geoHost.data(), geo.data(), geo.size() * sizeof(MeshGeo)) == 0;
@@ -660,6 +712,11 @@ std::string ClothModel::graph_key(float h, const CollSet& cs) const
     put(&gpu.d_cFeat, sizeof(void*));
     put(&gpu.ec.P, sizeof(void*)); put(&gpu.ec.slots, 4);
     put(&gpu.tc.P, sizeof(void*)); put(&gpu.tc.slots, 4);
+    put(&mSelfShrink, 4); put(&mSelfCollThreshold, 4); put(&mCcdVolume, 4); put(&mCcdBisect, 4);
+    put(&mCcdVtDistance, 4); put(&gpu.self.cap, 4);
+    const BvhView tv = gpu.self.triB.view(), ev = gpu.self.edgeB.view();
+    put(&tv, sizeof(tv)); put(&ev, sizeof(ev));
+    put(&gpu.self.buf[0].ids, sizeof(void*)); put(&gpu.self.buf[1].ids, sizeof(void*));
     return k;
 }
 
@@ -697,6 +754,8 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
     const bool anyContacts = cs.maxSlots > 0;
     const bool edgeContacts = (mCollFlags & CF_EDGES) && g.ec.slots > 0 && g.ne > 0;
     const bool triContacts = (mCollFlags & CF_TRIANGLES) && g.tc.slots > 0 && g.nt > 0;
+    const bool selfColl = (mCollFlags & CF_SELF) && g.self.built;
+    const SelfParams sp = self_params();
     const FeatureContactBuf ecb = feature_buf(g.ec), tcb = feature_buf(g.tc);
     const float invH = 1.f / h;
     // With a fixed iteration count and every vertex in some triangle slab, the
@@ -708,8 +767,8 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
     // Edge and triangle contacts follow the vertex contacts inside an
     // iteration, so they also keep the launches separate.
     const bool fuse = anyContacts && mFEM && mFixedIterations && triCol.slabbed && triCol.numPhases() > 0 &&
-                      g.triOrphans == 0 && !edgeContacts && !triContacts;
-    if (anyMesh || edgeContacts || triContacts) {
+                      g.triOrphans == 0 && !edgeContacts && !triContacts && !selfColl;
+    if (anyMesh || edgeContacts || triContacts || selfColl) {
         // the mesh query radius needs max |pos - prev| over all vertices
         // before any detection, and the contact groups are cleared there, so
         // prediction and detection are separate launches
@@ -722,6 +781,17 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
         if (edgeContacts || triContacts)
             launch_detect_features(s, g.d_pos, g.d_prev, g.d_cEdges, g.d_cTris, cs, ecb, tcb, mThickness,
                                    mTolerance, mCollFlags, g.d_ctl);
+        if (selfColl) {
+            GpuSolver::Self& S = g.self;
+            launch_self_boxes(s, g.d_pos, g.d_prev, g.d_cTris, g.nt, g.d_cEdges, g.ne, sp.tol, S.triLo, S.triHi,
+                              S.edgeLo, S.edgeHi);
+            S.triB.refit(s, S.triLo, S.triHi);
+            S.edgeB.refit(s, S.edgeLo, S.edgeHi);
+            launch_self_detect(s, g.d_pos, g.d_prev, g.d_cTris, g.d_cEdges, g.nv, g.ne, S.triB.view(),
+                               S.edgeB.view(), sp, S.buf[0], S.buf[1], g.d_ctl);
+            launch_self_colour(s, S.buf[0], 0, S.vmax, g.d_ctl);
+            launch_self_colour(s, S.buf[1], 1, S.vmax, g.d_ctl);
+        }
         timed_end(g.timer, timed, 3, s, ev);
     } else if (anyContacts) {
         timed_begin(g.timer, timed, 3, s, ev);
@@ -784,6 +854,12 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
             launch_tri_contacts(s, g.d_pos, g.d_prev, g.d_cTris, tcb, g.d_geo, mThickness, mFriction, g.d_ctl);
             timed_end(g.timer, timed, 2, s, ev);
         }
+        if (selfColl) {
+            timed_begin(g.timer, timed, 2, s, ev);
+            launch_self_solve(s, g.d_pos, g.self.buf[0], 0, sp, h, g.d_ctl);
+            launch_self_solve(s, g.d_pos, g.self.buf[1], 1, sp, h, g.d_ctl);
+            timed_end(g.timer, timed, 2, s, ev);
+        }
         if (!mFixedIterations)
             launch_iter_end(s, g.d_ctl, mNumIterations, 0, mBendFreq, mDihedral ? 1 : 0, mSelfCollThreshold);
     }
@@ -801,7 +877,6 @@ void ClothModel::step_substeps(float h, int n)
     if (mNumIterations < 1) throw std::runtime_error("NumIterations must be >= 1");
     if (mBendFreq < 1) throw std::runtime_error("bend frequency must be >= 1");
     if (msActive) throw std::runtime_error("the mouse spring is not implemented on the GPU substep");
-    if (mCollFlags & CF_SELF) throw std::runtime_error("self collision is not supported by the GPU substep");
     const CollisionWorld* w = mOwner->mWorld;
     if (w->colls.size() > MAX_COLLIDABLES) throw std::runtime_error("too many collidables (max 8)");
     for (const auto& c : w->colls) {
@@ -821,6 +896,11 @@ void ClothModel::step_substeps(float h, int n)
     ensure_contact_capacity(std::max(slots, 1));
     gpu.activeSlots = slots;
     ensure_feature_capacity(edgeSlots, triSlots);
+    if (mCollFlags & CF_SELF) {
+        ensure_self();
+        static const int zero = 0;
+        HIP_OK(hipMemcpyAsync(&gpu.d_ctl->selfOverflow, &zero, sizeof(int), hipMemcpyHostToDevice, gpu.stream));
+    }
     GpuSolver& g = gpu;
     const bool timed = g.timer.enabled;
     if (mUseGraph && !timed) {
@@ -840,6 +920,15 @@ void ClothModel::step_substeps(float h, int n)
     }
     HIP_OK(hipStreamSynchronize(g.stream));
     HIP_OK(hipGetLastError());
+    if (mCollFlags & CF_SELF) {
+        int overflow = 0;
+        HIP_OK(hipMemcpy(&overflow, &g.d_ctl->selfOverflow, sizeof(int), hipMemcpyDeviceToHost));
+        if (overflow)
+            throw std::runtime_error("self-collision contacts exceeded their capacity (" + std::to_string(g.self.cap) +
+                                     ") or " + std::to_string(kMaxSelfColours) +
+                                     " colours; the step's self contacts are incomplete "
+                                     "(raise it with scSetSelfCapacity)");
+    }
     if (timed) g.timer.harvest();
     mDeviceNewer = true;
     mStatsPending = true;
@@ -860,6 +949,8 @@ void ClothModel::fetch_stats()
         statBendError = c.bendError;
         statContactError = c.contactError;
     }
+    statSelfVTError = __builtin_bit_cast(float, c.sct);
+    statSelfEEError = __builtin_bit_cast(float, c.sce);
     mStatsPending = false;
 }
 
@@ -935,6 +1026,43 @@ void ClothModel::download_contacts()
     };
     if (mCollFlags & CF_EDGES) features(g.ec, edgeContactsHost);
     if (mCollFlags & CF_TRIANGLES) features(g.tc, triContactsHost);
+    selfContactsHost[0].clear();
+    selfContactsHost[1].clear();
+    if ((mCollFlags & CF_SELF) && g.self.cap > 0 && g.d_ctl) {
+        SolverCtl c;
+        HIP_OK(hipMemcpy(&c, g.d_ctl, sizeof(c), hipMemcpyDeviceToHost));
+        for (int kind = 0; kind < 2; kind++) {
+            const SelfBuf& b = g.self.buf[kind];
+            const int n = std::min(c.selfCount[kind], b.cap);
+            const int nc = c.selfColours[kind];
+            if (n <= 0 || nc <= 0) continue;
+            std::vector<int4> ids(n);
+            std::vector<float4> W(n), N(n);
+            std::vector<unsigned long long> key(n);
+            std::vector<int> order(n), off(nc + 1);
+            HIP_OK(hipMemcpy(ids.data(), b.ids, n * sizeof(int4), hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(W.data(), b.W, n * sizeof(float4), hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(N.data(), b.N, n * sizeof(float4), hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(key.data(), b.key, n * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(off.data(), b.groupOff, (nc + 1) * sizeof(int), hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(order.data(), b.order, off[nc] * sizeof(int), hipMemcpyDeviceToHost));
+            // colour groups in order; inside a group by key (members commute)
+            for (int grp = 0; grp < nc; grp++) {
+                std::vector<int> m(order.begin() + off[grp], order.begin() + off[grp + 1]);
+                std::sort(m.begin(), m.end(), [&](int a, int bb) { return key[a] < key[bb]; });
+                for (int i : m) {
+                    SelfContactOut o;
+                    o.key = key[i];
+                    o.ids[0] = ids[i].x; o.ids[1] = ids[i].y; o.ids[2] = ids[i].z; o.ids[3] = ids[i].w;
+                    o.w[0] = W[i].x; o.w[1] = W[i].y; o.w[2] = W[i].z;
+                    o.side = W[i].w;
+                    o.normal[0] = N[i].x; o.normal[1] = N[i].y; o.normal[2] = N[i].z;
+                    o.lambda = N[i].w;
+                    selfContactsHost[kind].push_back(o);
+                }
+            }
+        }
+    }
     contactsValid = true;
 }
 

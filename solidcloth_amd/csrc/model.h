@@ -183,6 +183,16 @@ public:
         int *group = nullptr, *groupBase = nullptr, *groupCount = nullptr, *list = nullptr;
         int n = 0, groups = 0, slots = 0, slotCap = 0;
     } ec, tc;
+    // continuous self collision: swept boxes, LBVHs over triangles and edges,
+    // contact lists of both kinds, per-vertex colouring bids
+    struct Self {
+        float4 *triLo = nullptr, *triHi = nullptr, *edgeLo = nullptr, *edgeHi = nullptr;
+        Lbvh triB, edgeB;
+        SelfBuf buf[2]{};
+        unsigned long long* vmax = nullptr;
+        int cap = 0;
+        bool built = false;
+    } self;
     DeviceSlabs triSlabs, bendSlabs;
     int triOrphans = 0;  // vertices in no triangle slab
     int slots = 0, slotCap = 0;
@@ -202,6 +212,10 @@ public:
     float mUnit = 10.f, mThickness = 0.3f, mTolerance = 0.2f, mShearStiff = 5e8f, mBendStiff = 5.f;
     float mStretchStiff = 2e10f, mPoisson = 0.5f, mFriction = 0.4f, mArea = 10.f, mDragCoeff = 1.f;
     float mSelfShrink = 0.9f, mSelfCollThreshold = 0.003f;
+    // continuous self-collision thresholds (SelfCollisionDetector.cpp:235,
+    // 296, 313-317, 446 constants by default); capacity 0 = automatic
+    float mCcdVolume = 0.001f, mCcdBisect = 0.01f, mCcdVtDistance = 0.1f;
+    int mSelfCapacity = 0;
     int mNumIterations = 50, mCollFlags = 0, mBendFreq = 2;
     bool mDihedral = true, mFEM = false, mFixedIterations = false, mUseGraph = true;
 
@@ -223,10 +237,16 @@ public:
     float msLen = 0.f, msStiffness = 100.f;
 
     int statIterations = 0;
-    float statBendError = 0.f, statContactError = 0.f;
+    float statBendError = 0.f, statContactError = 0.f, statSelfVTError = 0.f, statSelfEEError = 0.f;
 
     std::vector<ContactOut> contactsHost;
     std::vector<FeatureContactOut> edgeContactsHost, triContactsHost;
+    struct SelfContactOut {
+        unsigned long long key;
+        int ids[4];
+        float w[3], normal[3], side, lambda;
+    };
+    std::vector<SelfContactOut> selfContactsHost[2];  // processing order
     bool contactsValid = false;
 
     void clear();
@@ -261,6 +281,8 @@ private:
     void build_device_topology();
     void ensure_contact_capacity(int slots);
     void ensure_feature_capacity(int edgeSlots, int triSlots);
+    void ensure_self();
+    SelfParams self_params() const;
     CollSet make_collset(int& slots, int& edgeSlots, int& triSlots);
     FeatureContactBuf feature_buf(GpuSolver::Features& f) const;
     void record_substep(float h, const CollSet& cs, bool timed);

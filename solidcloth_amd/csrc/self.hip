@@ -1,0 +1,564 @@
+// gfx950 kernels for continuous self collision of the cloth:
+//   swept boxes -> LBVH refit -> per-vertex / per-edge overlap walks with the
+//   vertex-triangle / edge-edge CCD test (Physics/SelfCollisionDetector.cpp,
+//   coplanarity cubic in FP64) -> deterministic colouring of the contacts ->
+//   coloured Gauss-Seidel passes (Physics/ClothCollisionHandler.cpp:364-541).
+// FP32 arithmetic follows the reference's operation order; the cubic follows
+// it in FP64, as the CPU oracle does.
+#include "devutil.h"
+#include "geom.h"
+#include "kernels.h"
+
+namespace sc {
+
+static constexpr int kBlock = 256;
+static constexpr int kSoloBlock = 1024;
+
+// --------------------------------------------------------------- FP64 ----
+
+struct d3 {
+    double x, y, z;
+};
+__device__ __forceinline__ d3 dd(f3 a) { return d3{(double)a.x, (double)a.y, (double)a.z}; }
+__device__ __forceinline__ d3 dadd(d3 a, d3 b) { return d3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ d3 dsub(d3 a, d3 b) { return d3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ d3 dmul(double s, d3 a) { return d3{s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ d3 dcross(d3 a, d3 b)
+{
+    return d3{a.y * b.z - b.y * a.z, -a.x * b.z + a.z * b.x, a.x * b.y - b.x * a.y};
+}
+// Math/Vector3.h:289-292: triple(a, b, c) = c . (a x b)
+__device__ __forceinline__ double dtriple(d3 a, d3 b, d3 c) { return ddot(c, dcross(a, b)); }
+
+// SelfCollisionDetector.cpp:263-276: volume spanned by the four moving points at t
+__device__ __forceinline__ double coplanarity(double t, const d3* x, const d3* v)
+{
+    const d3 z1 = dadd(x[0], dmul(t, v[0])), z2 = dadd(x[1], dmul(t, v[1]));
+    const d3 z3 = dadd(x[2], dmul(t, v[2])), z4 = dadd(x[3], dmul(t, v[3]));
+    return ddot(dcross(dsub(z2, z1), dsub(z3, z1)), dsub(z4, z1));
+}
+
+// SelfCollisionDetector.cpp:414-488
+__device__ bool check_interval(double ta, double tb, const d3* x, const d3* v, double eps, double& tx)
+{
+    double tx0 = ta, tx1 = tb;
+    double fx0 = coplanarity(tx0, x, v), fx1 = coplanarity(tx1, x, v);
+    if (fx0 * fx1 > 0) return false;
+    tx = tx0;
+    for (int l = 0; l < 40; l++) {
+        const double tmid = 0.5 * (tx0 + tx1);
+        const double fmid = coplanarity(tmid, x, v);
+        if (fabs(fmid) < eps) {
+            tx = tmid;
+            break;
+        }
+        if (fx0 * fmid < 0) {
+            tx1 = tmid;
+            fx1 = fmid;
+            tx = tx0;
+        } else if (fmid * fx1 < 0) {
+            tx0 = tmid;
+            fx0 = fmid;
+            tx = tmid;
+        } else {
+            return false;
+        }
+    }
+    return true;
+}
+
+// SelfCollisionDetector.cpp:278-412: first time in [0, 1] at which the four
+// points become coplanar (cubic in t, split at its critical points).
+__device__ bool solve_cubic(const f3* xf, const f3* vf, const SelfParams& sp, double& toi)
+{
+    d3 x[4], v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        x[k] = dd(xf[k]);
+        v[k] = dd(vf[k]);
+    }
+    const double volEps = sp.volume;
+    const double f0 = coplanarity(0.0, x, v);
+    if (fabs(f0) < volEps) return false;
+    const d3 v10 = dsub(v[0], v[3]), v20 = dsub(v[1], v[3]), v30 = dsub(v[2], v[3]);
+    const d3 x10 = dsub(x[0], x[3]), x20 = dsub(x[1], x[3]), x30 = dsub(x[2], x[3]);
+    const double a = dtriple(v10, v20, v30);
+    const double b = dtriple(x10, v20, v30) + dtriple(v10, x20, v30) + dtriple(v10, v20, x30);
+    const double c = dtriple(x10, x20, v30) + dtriple(x10, v20, x30) + dtriple(v10, x20, x30);
+    if (fabs(a) < volEps) {
+        if (fabs(b) < volEps) return false;
+        const double d = dtriple(x10, x20, x30);
+        const double discr = c * c - 4 * b * d;
+        if (discr < 0) return false;
+        if (discr == 0) {
+            const double t = -0.5 * c / b;
+            if (t < 0 || t > 1) return false;
+            toi = t;
+            return true;
+        }
+        const double sq = sqrt(discr);
+        const double t1 = 0.5 * (-c - sq) / b, t2 = 0.5 * (-c + sq) / b;
+        const double t = t2 < t1 ? t2 : t1;
+        if (t < 0 || t > 1) return false;
+        toi = t;
+        return true;
+    }
+    double t1 = 0, t2 = 0;
+    int np;
+    const double discr = b * b - 3 * a * c;
+    if (discr < 0) {
+        np = 0;
+    } else if (discr == 0) {
+        t1 = t2 = -b / (3 * a);
+        np = 1;
+    } else {
+        const double sq = sqrt(discr);
+        t1 = (-b - sq) / (3 * a);
+        t2 = (-b + sq) / (3 * a);
+        if (t2 < t1) {
+            const double tmp = t1;
+            t1 = t2;
+            t2 = tmp;
+        }
+        np = 2;
+    }
+    const double eps = sp.bisect;
+    if (np == 0) return check_interval(0, 1, x, v, eps, toi);
+    if (np == 1) {
+        if (t1 < 0 || t1 > 1) return check_interval(0, 1, x, v, eps, toi);
+        return check_interval(0, t1, x, v, eps, toi) || check_interval(t1, 1, x, v, eps, toi);
+    }
+    if (t1 > 1 || t2 < 0 || (t1 < 0 && t2 > 1)) return check_interval(0, 1, x, v, eps, toi);
+    if (t1 < 0 && t2 < 1) return check_interval(0, t2, x, v, eps, toi) || check_interval(t2, 1, x, v, eps, toi);
+    if (t1 < 1 && t2 > 1) return check_interval(0, t1, x, v, eps, toi) || check_interval(t1, 1, x, v, eps, toi);
+    if (t1 > 0 && t2 < 1)
+        return check_interval(0, t1, x, v, eps, toi) || check_interval(t1, t2, x, v, eps, toi) ||
+               check_interval(t2, 1, x, v, eps, toi);
+    return false;
+}
+
+// ------------------------------------------------------------ detection ----
+
+__device__ __forceinline__ bool overlap(f3 alo, f3 ahi, float4 blo, float4 bhi)
+{
+    return !(ahi.x < blo.x || ahi.y < blo.y || ahi.z < blo.z || bhi.x < alo.x || bhi.y < alo.y || bhi.z < alo.z);
+}
+
+// Visits every leaf whose box overlaps [lo, hi] (closed intervals, as
+// Geometry/Aabb3.h:72-87). Karras trees have depth < 64.
+template <class Visit>
+__device__ __forceinline__ void bvh_overlap(const BvhView& bv, f3 lo, f3 hi, Visit visit)
+{
+    if (bv.n <= 0) return;
+    if (bv.n == 1) {
+        if (overlap(lo, hi, bv.leafLo[0], bv.leafHi[0])) visit(0);
+        return;
+    }
+    if (!overlap(lo, hi, bv.lo[0], bv.hi[0])) return;
+    int stack[64];
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        const float4 L = bv.lo[node], H = bv.hi[node];
+        const int c[2] = {__float_as_int(L.w), __float_as_int(H.w)};
+        int next = -1;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            if (c[k] < 0) {
+                const int leaf = ~c[k];
+                if (overlap(lo, hi, bv.leafLo[leaf], bv.leafHi[leaf])) visit(leaf);
+            } else if (overlap(lo, hi, bv.lo[c[k]], bv.hi[c[k]])) {
+                if (next < 0) next = c[k];
+                else if (sp < 64) stack[sp++] = c[k];
+            }
+        }
+        if (next < 0) {
+            if (sp == 0) return;
+            next = stack[--sp];
+        }
+        node = next;
+    }
+}
+
+// ClothCollision.cpp:256-315: prev and pos of the primitive's vertices,
+// extruded by tol (Aabb3::Extrude).
+__device__ __forceinline__ void swept(f3 p, f3& lo, f3& hi)
+{
+    lo = f3{fminf(lo.x, p.x), fminf(lo.y, p.y), fminf(lo.z, p.z)};
+    hi = f3{fmaxf(hi.x, p.x), fmaxf(hi.y, p.y), fmaxf(hi.z, p.z)};
+}
+
+__global__ void __launch_bounds__(kBlock) k_self_boxes(const float4* __restrict__ pos, const float4* __restrict__ prev,
+                                                       const int4* __restrict__ cTris, int nt,
+                                                       const int2* __restrict__ cEdges, int ne, float tol,
+                                                       float4* triLo, float4* triHi, float4* edgeLo, float4* edgeHi)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    f3 lo{1e7f, 1e7f, 1e7f}, hi{-1e7f, -1e7f, -1e7f};
+    if (i < nt) {
+        const int4 t = cTris[i];
+        swept(xyz(prev[t.x]), lo, hi);
+        swept(xyz(prev[t.y]), lo, hi);
+        swept(xyz(prev[t.z]), lo, hi);
+        swept(xyz(pos[t.x]), lo, hi);
+        swept(xyz(pos[t.y]), lo, hi);
+        swept(xyz(pos[t.z]), lo, hi);
+        triLo[i] = make_float4(lo.x - tol, lo.y - tol, lo.z - tol, 0.f);
+        triHi[i] = make_float4(hi.x + tol, hi.y + tol, hi.z + tol, 0.f);
+    } else if (i < nt + ne) {
+        const int2 e = cEdges[i - nt];
+        swept(xyz(prev[e.x]), lo, hi);
+        swept(xyz(prev[e.y]), lo, hi);
+        swept(xyz(pos[e.x]), lo, hi);
+        swept(xyz(pos[e.y]), lo, hi);
+        edgeLo[i - nt] = make_float4(lo.x - tol, lo.y - tol, lo.z - tol, 0.f);
+        edgeHi[i - nt] = make_float4(hi.x + tol, hi.y + tol, hi.z + tol, 0.f);
+    }
+}
+
+__device__ __forceinline__ void self_append(SelfBuf b, int kind, SolverCtl* ctl, int4 ids, float4 W, float4 N,
+                                            unsigned long long key)
+{
+    const int k = atomicAdd(&ctl->selfCount[kind], 1);
+    if (k >= b.cap) {
+        atomicExch(&ctl->selfOverflow, 1);
+        return;
+    }
+    b.ids[k] = ids;
+    b.W[k] = W;
+    b.N[k] = N;
+    b.key[k] = key;
+}
+
+// SelfCollisionDetector.cpp:214-261 (VertexTriangleCCD + VertexTriangleTest)
+__device__ bool vertex_triangle(const float4* pos, const float4* prev, int i1, int i2, int i3, int i4,
+                                const SelfParams& sp, float4& W, float4& N)
+{
+    const f3 x[4] = {xyz(prev[i1]), xyz(prev[i2]), xyz(prev[i3]), xyz(prev[i4])};
+    const f3 v[4] = {sub(xyz(pos[i1]), x[0]), sub(xyz(pos[i2]), x[1]), sub(xyz(pos[i3]), x[2]),
+                     sub(xyz(pos[i4]), x[3])};
+    double toid;
+    if (!solve_cubic(x, v, sp, toid)) return false;
+    const float toi = (float)toid;
+    const f3 z1 = add(x[0], mul(toi, v[0])), z2 = add(x[1], mul(toi, v[1]));
+    const f3 z3 = add(x[2], mul(toi, v[2])), z4 = add(x[3], mul(toi, v[3]));
+    f3 p, co;
+    closest_pt_point_triangle(z4, z1, z2, z3, p, co);
+    const f3 delta = sub(z4, p);
+    const float eps = sp.vtDistance;
+    if (!(dot(delta, delta) < eps * eps)) return false;
+    const f3 triNormal = normalized(cross(sub(x[1], x[0]), sub(x[2], x[0])));
+    const float side = dot(triNormal, sub(x[3], x[0]));
+    f3 n = normalized(cross(sub(z2, z1), sub(z3, z1)));
+    if (side < 0.f) n = neg(n);
+    W = make_float4(co.x, co.y, co.z, signed_volume(x[3], x[0], x[1], x[2]));
+    N = f4(n, 0.f);
+    return true;
+}
+
+// SelfCollisionDetector.cpp:491-606 (EdgeEdgeTest + EdgeEdgeCCD)
+__device__ bool edge_edge(const float4* pos, const float4* prev, int i1, int i2, int i3, int i4,
+                          const SelfParams& sp, float4& W, float4& N)
+{
+    const f3 x[4] = {xyz(prev[i1]), xyz(prev[i2]), xyz(prev[i3]), xyz(prev[i4])};
+    const f3 v[4] = {sub(xyz(pos[i1]), x[0]), sub(xyz(pos[i2]), x[1]), sub(xyz(pos[i3]), x[2]),
+                     sub(xyz(pos[i4]), x[3])};
+    double toid;
+    if (!solve_cubic(x, v, sp, toid)) return false;
+    const float toi = (float)toid;
+    const f3 z1 = add(x[0], mul(toi, v[0])), z2 = add(x[1], mul(toi, v[1]));
+    const f3 z3 = add(x[2], mul(toi, v[2])), z4 = add(x[3], mul(toi, v[3]));
+    float s, t;
+    f3 c1, c2;
+    closest_pt_segm_segm(z1, z2, z3, z4, s, t, c1, c2);
+    const f3 d = sub(c1, c2);
+    const float eps = 0.5f * sp.thickness;
+    if (!(dot(d, d) <= eps * eps)) return false;
+    float s0, t0;
+    closest_pt_segm_segm(x[0], x[1], x[2], x[3], s0, t0, c1, c2);
+    W = make_float4(s, t, 0.f, signed_volume(x[0], x[1], x[2], x[3]));
+    N = f4(normalized(sub(c2, c1)), 0.f);
+    return true;
+}
+
+// SelfCollisionDetector.cpp:60-135, 137-190, 23-58 over the exact candidate
+// set: blocks [0, vertexBlocks) take vertices (against the triangle tree),
+// the rest edges (against the edge tree, partner index greater).
+__global__ void __launch_bounds__(kBlock) k_self_detect(const float4* __restrict__ pos,
+                                                        const float4* __restrict__ prev,
+                                                        const int4* __restrict__ cTris,
+                                                        const int2* __restrict__ cEdges, int nv, int ne,
+                                                        int vertexBlocks, BvhView triBvh, BvhView edgeBvh,
+                                                        SelfParams sp, SelfBuf vt, SelfBuf ee, SolverCtl* ctl)
+{
+    const float tol = sp.tol;
+    if ((int)blockIdx.x < vertexBlocks) {
+        const int v = blockIdx.x * blockDim.x + threadIdx.x;
+        if (v >= nv) return;
+        const float4 P = pos[v];
+        if (P.w == 0.f) return;
+        f3 lo{1e7f, 1e7f, 1e7f}, hi{-1e7f, -1e7f, -1e7f};
+        swept(xyz(prev[v]), lo, hi);
+        swept(xyz(P), lo, hi);
+        lo = f3{lo.x - tol, lo.y - tol, lo.z - tol};
+        hi = f3{hi.x + tol, hi.y + tol, hi.z + tol};
+        bvh_overlap(triBvh, lo, hi, [&](int leaf) {
+            const int t = __float_as_int(triBvh.leafLo[leaf].w);
+            const int4 tv = cTris[t];
+            if (v == tv.x || v == tv.y || v == tv.z) return;
+            float4 W, N;
+            if (vertex_triangle(pos, prev, tv.x, tv.y, tv.z, v, sp, W, N))
+                self_append(vt, 0, ctl, make_int4(tv.x, tv.y, tv.z, v), W, N,
+                            ((unsigned long long)t << 32) | (unsigned)v);
+        });
+    } else {
+        const int e1 = (blockIdx.x - vertexBlocks) * blockDim.x + threadIdx.x;
+        if (e1 >= ne) return;
+        const int2 ev = cEdges[e1];
+        f3 lo{1e7f, 1e7f, 1e7f}, hi{-1e7f, -1e7f, -1e7f};
+        swept(xyz(prev[ev.x]), lo, hi);
+        swept(xyz(prev[ev.y]), lo, hi);
+        swept(xyz(pos[ev.x]), lo, hi);
+        swept(xyz(pos[ev.y]), lo, hi);
+        lo = f3{lo.x - tol, lo.y - tol, lo.z - tol};
+        hi = f3{hi.x + tol, hi.y + tol, hi.z + tol};
+        bvh_overlap(edgeBvh, lo, hi, [&](int leaf) {
+            const int e2 = __float_as_int(edgeBvh.leafLo[leaf].w);
+            if (e2 <= e1) return;
+            const int2 fv = cEdges[e2];
+            if (ev.x == fv.x || ev.y == fv.y || ev.x == fv.y || ev.y == fv.x) return;
+            float4 W, N;
+            if (edge_edge(pos, prev, ev.x, ev.y, fv.x, fv.y, sp, W, N))
+                self_append(ee, 1, ctl, make_int4(ev.x, ev.y, fv.x, fv.y), W, N,
+                            ((unsigned long long)e1 << 32) | (unsigned)e2);
+        });
+    }
+}
+
+// ------------------------------------------------------------ colouring ----
+
+// Jones-Plassmann colouring of one kind's contacts, by one workgroup. In
+// round r every uncoloured contact bids its key at its four vertices; a
+// contact that holds the highest bid at all four takes colour r. Winners of a
+// round share no vertex, and the highest uncoloured key always wins, so every
+// round makes progress; the result depends only on the set of contacts. The
+// contacts are then listed colour by colour.
+__global__ void __launch_bounds__(kSoloBlock) k_self_colour(SelfBuf b, int kind, unsigned long long* vmax,
+                                                            SolverCtl* ctl)
+{
+    __shared__ int remaining;
+    __shared__ int cnt[kMaxSelfColours + 1];
+    const int n = min(ctl->selfCount[kind], b.cap);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    auto vid = [&](int c, int k) {
+        const int4 q = b.ids[c];
+        return k == 0 ? q.x : k == 1 ? q.y : k == 2 ? q.z : q.w;
+    };
+    for (int c = tid; c < n; c += nt) {
+        b.colour[c] = -1;
+        for (int k = 0; k < 4; k++)
+            __hip_atomic_store(&vmax[vid(c, k)], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    int r = 0;
+    while (n > 0) {
+        if (tid == 0) remaining = 0;
+        for (int c = tid; c < n; c += nt) {
+            if (b.colour[c] >= 0) continue;
+            const unsigned long long bid = b.key[c] + 1ull;
+            for (int k = 0; k < 4; k++) atomicMax(&vmax[vid(c, k)], bid);
+        }
+        __syncthreads();
+        for (int c = tid; c < n; c += nt) {
+            if (b.colour[c] >= 0) continue;
+            const unsigned long long bid = b.key[c] + 1ull;
+            bool win = true;
+            for (int k = 0; k < 4; k++)
+                win = win && __hip_atomic_load(&vmax[vid(c, k)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == bid;
+            if (win) b.colour[c] = r;
+        }
+        __syncthreads();
+        for (int c = tid; c < n; c += nt) {
+            const int col = b.colour[c];
+            if (col >= 0 && col != r) continue;
+            for (int k = 0; k < 4; k++)
+                __hip_atomic_store(&vmax[vid(c, k)], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (col < 0) atomicAdd(&remaining, 1);
+        }
+        __syncthreads();
+        const int left = remaining;
+        __syncthreads();
+        if (left == 0) break;
+        if (++r >= b.maxColours) {
+            if (tid == 0) ctl->selfOverflow = 1;
+            break;
+        }
+    }
+    const int nc = n > 0 ? min(r + 1, b.maxColours) : 0;
+    for (int g = tid; g <= nc; g += nt) cnt[g] = 0;
+    __syncthreads();
+    for (int c = tid; c < n; c += nt) {
+        const int col = b.colour[c];
+        if (col >= 0 && col < nc) atomicAdd(&cnt[col], 1);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int g = 0; g < nc; g++) {
+            const int k = cnt[g];
+            cnt[g] = run;
+            b.groupOff[g] = run;
+            run += k;
+        }
+        b.groupOff[nc] = run;
+        ctl->selfColours[kind] = nc;
+    }
+    __syncthreads();
+    for (int c = tid; c < n; c += nt) {
+        const int col = b.colour[c];
+        if (col >= 0 && col < nc) b.order[atomicAdd(&cnt[col], 1)] = c;
+    }
+}
+
+// --------------------------------------------------------- projections ----
+
+// ClothCollisionHandler.cpp:364-436 (SolveSelfTrisPosition)
+__device__ float self_vt(float4* pos, SelfBuf b, int c, const SelfParams& sp, float h)
+{
+    const int4 id = b.ids[c];
+    const float4 P1 = pos[id.x], P2 = pos[id.y], P3 = pos[id.z], P4 = pos[id.w];
+    f3 x1 = xyz(P1), x2 = xyz(P2), x3 = xyz(P3), x4 = xyz(P4);
+    const float m1 = P1.w, m2 = P2.w, m3 = P3.w, m4 = P4.w;
+    const float4 W = b.W[c], N = b.N[c];
+    const f3 cn = xyz(N);
+    float len0 = sp.thickness * sp.shrink;
+    f3 p, co;
+    closest_pt_point_triangle(x4, x1, x2, x3, p, co);
+    const float w1 = co.x, w2 = co.y, w3 = co.z;
+    const f3 delta = sub(x4, p);
+    f3 n = normalized(cross(sub(x2, x1), sub(x3, x1)));
+    const float side = signed_volume(x4, x1, x2, x3);
+    if (W.w * side <= 0.f) len0 = sp.thickness * 0.1f;
+    if (dot(n, cn) < 0.f) n = neg(n);
+    const float len = dot(n, delta);
+    float err = 0.f;
+    float lambda = N.w;
+    if (!(len > len0)) {
+        const float im = w1 * w1 * m1 + w2 * w2 * m2 + w3 * w3 * m3 + m4;
+        const float s = 1.f / im;
+        const float e = len - len0;
+        err = smax(0.f, -e);
+        const float dLambda = 0.9f * s * e;
+        const f3 disp = mul(dLambda, n);
+        lambda += -dLambda / h;
+        x1 = add(x1, mul(w1 * m1, disp));
+        x2 = add(x2, mul(w2 * m2, disp));
+        x3 = add(x3, mul(w3 * m3, disp));
+        x4 = sub(x4, mul(m4, disp));
+        pos[id.x] = f4(x1, m1);
+        pos[id.y] = f4(x2, m2);
+        pos[id.z] = f4(x3, m3);
+        pos[id.w] = f4(x4, m4);
+    }
+    b.W[c] = make_float4(w1, w2, w3, W.w);
+    b.N[c] = make_float4(N.x, N.y, N.z, lambda);
+    return err;
+}
+
+// ClothCollisionHandler.cpp:439-541 (SolveSelfEdgesPosition)
+__device__ float self_ee(float4* pos, SelfBuf b, int c, const SelfParams& sp, float h)
+{
+    const int4 id = b.ids[c];
+    const float4 P1 = pos[id.x], P2 = pos[id.y], P3 = pos[id.z], P4 = pos[id.w];
+    f3 x1 = xyz(P1), x2 = xyz(P2), x3 = xyz(P3), x4 = xyz(P4);
+    const float m1 = P1.w, m2 = P2.w, m3 = P3.w, m4 = P4.w;
+    const float4 W = b.W[c], N = b.N[c];
+    float thick = sp.thickness * sp.shrink;
+    const float side = signed_volume(x1, x2, x3, x4);
+    f3 p, q;
+    float t1, t2;
+    closest_pt_segm_segm(x1, x2, x3, x4, t1, t2, p, q);
+    const float w1 = 1.f - t1, w2 = 1.f - t2;
+    const float omw1 = 1.f - w1, omw2 = 1.f - w2;
+    const f3 delta = sub(q, p);
+    f3 n = normalized(delta);
+    if (dot(n, xyz(N)) < 0.0f) n = neg(n);
+    if (side * W.w < 0.f) thick = sp.thickness * 0.1f;
+    const float len = dot(n, delta);
+    float err = 0.f;
+    float lambda = N.w;
+    if (!(len > thick)) {
+        const float im = w1 * w1 * m1 + omw1 * omw1 * m2 + w2 * w2 * m3 + omw2 * omw2 * m4;
+        const float s = 1.f / im;
+        const float e = len - thick;
+        err = smax(0.f, -e);
+        const float dLambda = -0.9f * s * e;
+        lambda += dLambda / h;
+        const f3 disp = mul(dLambda, n);
+        x1 = sub(x1, mul(omw1 * m1, disp));
+        x2 = sub(x2, mul(w1 * m2, disp));
+        x3 = add(x3, mul(omw2 * m3, disp));
+        x4 = add(x4, mul(w2 * m4, disp));
+        pos[id.x] = f4(x1, m1);
+        pos[id.y] = f4(x2, m2);
+        pos[id.z] = f4(x3, m3);
+        pos[id.w] = f4(x4, m4);
+    }
+    b.W[c] = make_float4(w1, w2, 0.f, W.w);
+    b.N[c] = make_float4(N.x, N.y, N.z, lambda);
+    return err;
+}
+
+// One workgroup walks the colour groups in order; a group's contacts share
+// no vertex and are projected in parallel.
+__global__ void __launch_bounds__(kSoloBlock) k_self_solve(float4* pos, SelfBuf b, int kind, SelfParams sp, float h,
+                                                           SolverCtl* ctl)
+{
+    if (ctl->done) return;
+    const int nc = ctl->selfColours[kind];
+    float err = 0.f;
+    for (int g = 0; g < nc; g++) {
+        const int e = b.groupOff[g + 1];
+        for (int k = b.groupOff[g] + threadIdx.x; k < e; k += blockDim.x) {
+            const int c = b.order[k];
+            const float x = kind == 0 ? self_vt(pos, b, c, sp, h) : self_ee(pos, b, c, sp, h);
+            err = err < x ? x : err;
+        }
+        __syncthreads();
+    }
+    wave_max_atomic(kind == 0 ? &ctl->sct : &ctl->sce, err);
+}
+
+static inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
+
+void launch_self_boxes(hipStream_t s, const float4* pos, const float4* prev, const int4* cTris, int nt,
+                       const int2* cEdges, int ne, float tol, float4* triLo, float4* triHi, float4* edgeLo,
+                       float4* edgeHi)
+{
+    if (nt + ne <= 0) return;
+    hipLaunchKernelGGL(k_self_boxes, dim3(blocks(nt + ne)), dim3(kBlock), 0, s, pos, prev, cTris, nt, cEdges, ne, tol,
+                       triLo, triHi, edgeLo, edgeHi);
+}
+
+void launch_self_detect(hipStream_t s, const float4* pos, const float4* prev, const int4* cTris, const int2* cEdges,
+                        int nv, int ne, BvhView triBvh, BvhView edgeBvh, SelfParams sp, SelfBuf vt, SelfBuf ee,
+                        SolverCtl* ctl)
+{
+    const int vb = blocks(nv), eb = blocks(ne);
+    if (vb + eb == 0) return;
+    hipLaunchKernelGGL(k_self_detect, dim3(vb + eb), dim3(kBlock), 0, s, pos, prev, cTris, cEdges, nv, ne, vb, triBvh,
+                       edgeBvh, sp, vt, ee, ctl);
+}
+
+void launch_self_colour(hipStream_t s, SelfBuf b, int kind, unsigned long long* vmax, SolverCtl* ctl)
+{
+    hipLaunchKernelGGL(k_self_colour, dim3(1), dim3(kSoloBlock), 0, s, b, kind, vmax, ctl);
+}
+
+void launch_self_solve(hipStream_t s, float4* pos, SelfBuf b, int kind, SelfParams sp, float h, SolverCtl* ctl)
+{
+    hipLaunchKernelGGL(k_self_solve, dim3(1), dim3(kSoloBlock), 0, s, pos, b, kind, sp, h, ctl);
+}
+
+}  // namespace sc

@@ -66,6 +66,11 @@ class Scene:
     # height perturbation: three sinusoids of this amplitude (0 = flat)
     perturb_amplitude: float = 0.0
     perturb_seed: int = 0
+    # fold: the half x > 0 laid over the half x < 0 at this height gap (0 = none)
+    fold_gap: float = 0.0
+    # continuous self-collision thresholds (volume, bisection, vertex-triangle
+    # distance); None = the reference's constants
+    ccd: tuple = None
     extra: dict = field(default_factory=dict)
 
     @property
@@ -162,6 +167,46 @@ def perturbed_positions(scene, pos):
     return pos
 
 
+def ccd_for_spacing(d):
+    """The reference's absolute self-collision thresholds (volume 0.001,
+    bisection 0.01, distance 0.1) were set for its demo cloth, whose grid
+    spacing is 80/30 units (Demo/ClothDemo.cpp:141); scaled here to grid
+    spacing d (volumes by the cube of the ratio)."""
+    r = d / (80.0 / 30.0)
+    return (0.001 * r ** 3, 0.01 * r ** 3, 0.1 * r)
+
+
+FOLD24 = Scene(
+    name="fold24", n=24, side=0.5, offset=(0.0, 0.0, 0.0), iterations=10, frames=12,
+    flags=CF_VERTICES | CF_WALLS | CF_SELF, thickness=0.004, tolerance=0.004, friction=0.3,
+    walls_min=(-10.0, -0.004, -10.0), walls_max=(10.0, 10.0, 10.0),
+    fold_gap=0.02, ccd=ccd_for_spacing(0.5 / 23),
+)
+
+
+def folded_positions(scene, pos):
+    """Grid positions with the half x > 0 folded over the half x < 0: it
+    wraps around a half cylinder of diameter `fold_gap` along the line x = 0
+    (arc length preserved) and continues flat at height `fold_gap`."""
+    pos = np.asarray(pos, dtype=np.float32).copy()
+    if scene.fold_gap == 0.0:
+        return pos
+    r = scene.fold_gap / 2.0
+    s = pos[:, 0].astype(np.float64)
+    y = pos[:, 1].astype(np.float64)
+    right = s > 0
+    on_arc = right & (s < np.pi * r)
+    beyond = right & ~on_arc
+    x = s.copy()
+    x[on_arc] = r * np.sin(s[on_arc] / r)
+    y[on_arc] += r * (1.0 - np.cos(s[on_arc] / r))
+    x[beyond] = -(s[beyond] - np.pi * r)
+    y[beyond] += 2.0 * r
+    pos[:, 0] = x.astype(np.float32)
+    pos[:, 1] = y.astype(np.float32)
+    return pos
+
+
 def sdf_box(scene):
     c = np.asarray(scene.sphere_center, dtype=np.float32)
     h = np.float32(scene.sdf_half)
@@ -225,4 +270,8 @@ def build_patch(scene, sc=None):
     patch.Init(scene.n, scene.n, scene.inc, scene.offset, scene.horizontal, scene.attached)
     if scene.perturb_amplitude != 0.0:
         m.set_positions(perturbed_positions(scene, m.positions()))
+    if scene.fold_gap != 0.0:
+        m.set_positions(folded_positions(scene, m.positions()))
+    if scene.ccd is not None:
+        m.SetSelfCCD(*scene.ccd)
     return world, patch

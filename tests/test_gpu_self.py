@@ -1,0 +1,63 @@
+"""GPU parity of continuous self collision (LBVH refit + overlap walks, FP64
+CCD cubic, Jones-Plassmann colouring of the contacts, coloured projections)
+against the CPU oracle's exhaustive candidate enumeration, substep by
+substep: the GPU solves its contacts colour group by colour group and hands
+that order to the oracle, which must then produce the same contacts (keys,
+ids, weights, normals, sides, lambdas) and the same positions bit for bit."""
+import numpy as np
+import pytest
+
+from solidcloth_amd import scenes as S
+
+from parity_util import pair, same_positions
+
+pytestmark = pytest.mark.gpu
+
+
+def _rows(keys, ids, data):
+    return sorted((int(k), tuple(int(x) for x in i), tuple(np.float32(d).tolist())) for k, i, d in zip(keys, ids, data))
+
+
+def _run(sc, oracle_mod, scene, mode, substeps):
+    world, patch, m, o = pair(sc, oracle_mod, scene, mode)
+    patch.SetNumSteps(1)
+    h = float(np.float32(np.float32(scene.dt) / np.float32(scene.substeps)))
+    seen = [0, 0]
+    for k in range(substeps):
+        patch.Step(h)
+        o.substep_detect(h)
+        for kind, j in (("vt", 0), ("ee", 1)):
+            o.set_self_order(kind, m.self_contacts(kind)[0])
+        o.substep_solve(h)
+        for kind, j in (("vt", 0), ("ee", 1)):
+            gk, gi, gd = m.self_contacts(kind)
+            ok, oi, od = o.self_contacts(kind)
+            assert _rows(gk, gi, gd) == _rows(ok, oi, od), f"{kind} contacts differ at substep {k}"
+            # the GPU's list is its processing order, the oracle's too
+            assert np.array_equal(gk, ok), f"{kind} order at substep {k}"
+            seen[j] += len(gk)
+        same_positions(m, o, f"substep {k}")
+    return seen, m, o
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fold_self_collision_bit_exact(sc, oracle_mod, mode):
+    print(f"fold24 mode={mode}", flush=True)
+    seen, m, o = _run(sc, oracle_mod, S.FOLD24, mode, 48)
+    assert seen[0] > 0 and seen[1] > 0, seen
+
+
+def test_fold_self_collision_reference_early_exit(sc, oracle_mod):
+    """The exit test includes the self contact errors against the threshold."""
+    scene = S.FOLD24.with_(fixed_iterations=False, iterations=15)
+    world, patch, m, o = pair(sc, oracle_mod, scene)
+    patch.SetNumSteps(1)
+    h = float(np.float32(np.float32(scene.dt) / np.float32(scene.substeps)))
+    for k in range(48):
+        patch.Step(h)
+        o.substep_detect(h)
+        o.set_self_order("vt", m.self_contacts("vt")[0])
+        o.set_self_order("ee", m.self_contacts("ee")[0])
+        o.substep_solve(h)
+        assert m.stats()["iterations"] == o.stats()["iterations"], k
+    same_positions(m, o, "fold early exit")

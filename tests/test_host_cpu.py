@@ -231,3 +231,16 @@ def test_grid_sdf_edges_without_vertex_contacts_refused(sc):
     world, patch = S.build_patch(scene, sc)
     with pytest.raises(sc.SolidClothError, match="grid SDF"):
         patch.Step(scene.dt)
+
+
+def test_self_collision_parameters_validated(sc):
+    world, patch = S.build_patch(S.FOLD24, sc)
+    m = patch.GetModel()
+    m.SetSelfParams(0.9, 0.003)
+    m.SetSelfCapacity(100)
+    with pytest.raises(sc.SolidClothError):
+        m.SetSelfCCD(-1.0, 0.01, 0.1)
+    with pytest.raises(sc.SolidClothError):
+        m.SetSelfCapacity(-5)
+    keys, ids, data = m.self_contacts("vt")  # before any step: none
+    assert len(keys) == 0
