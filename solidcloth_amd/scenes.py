@@ -16,8 +16,11 @@ grid (Physics/ClothPatch.cpp:181), which lies in the y = offset.y plane.
             sinusoids of amplitude 0.05 m (|k| <= 8 pi), dropped onto an
             icosphere triangle mesh (subdivision 6: 40,962 vertices, 81,920
             triangles) of radius 0.4 plus a ground plane; vertex, edge and
-            triangle contacts against the mesh, thickness 2 mm, 10 iterations,
-            4 substeps, 300 frames.
+            triangle contacts against the mesh, continuous self collision
+            (thresholds scaled to the grid spacing), thickness 2 mm, 10
+            iterations, 4 substeps, 300 frames.
+  fold24    24x24 grid folded over itself around a half cylinder, falling on
+            the ground: the self-collision path at oracle size.
   meshsmall 24x24 grid over an icosphere of subdivision 2, the mesh
             contact path at a size the exhaustive oracle runs in seconds.
 """
@@ -99,11 +102,21 @@ DRAPE256 = Scene(
 )
 
 
+def ccd_for_spacing(d):
+    """The reference's absolute self-collision thresholds (volume 0.001,
+    bisection 0.01, distance 0.1) were set for its demo cloth, whose grid
+    spacing is 80/30 units (Demo/ClothDemo.cpp:141); scaled here to grid
+    spacing d (volumes by the cube of the ratio)."""
+    r = d / (80.0 / 30.0)
+    return (0.001 * r ** 3, 0.01 * r ** 3, 0.1 * r)
+
+
 MESH512 = Scene(
     name="mesh512", n=512, side=2.0, offset=(0.0, 0.5, 0.0), iterations=10, frames=300,
-    flags=CF_VERTICES | CF_EDGES | CF_TRIANGLES | CF_WALLS, thickness=0.002, tolerance=0.005, friction=0.3,
-    walls_min=(-10.0, -0.45, -10.0), walls_max=(10.0, 10.0, 10.0),
+    flags=CF_VERTICES | CF_EDGES | CF_TRIANGLES | CF_WALLS | CF_SELF, thickness=0.002, tolerance=0.005,
+    friction=0.3, walls_min=(-10.0, -0.45, -10.0), walls_max=(10.0, 10.0, 10.0),
     mesh_subdiv=6, mesh_radius=0.4, perturb_amplitude=0.05, perturb_seed=3,
+    ccd=ccd_for_spacing(2.0 / 511),
 )
 
 MESHSMALL = Scene(
@@ -165,15 +178,6 @@ def perturbed_positions(scene, pos):
         h += scene.perturb_amplitude * np.sin(mag[j] * np.cos(ang[j]) * x + mag[j] * np.sin(ang[j]) * z + phi[j])
     pos[:, 1] = (pos[:, 1].astype(np.float64) + h).astype(np.float32)
     return pos
-
-
-def ccd_for_spacing(d):
-    """The reference's absolute self-collision thresholds (volume 0.001,
-    bisection 0.01, distance 0.1) were set for its demo cloth, whose grid
-    spacing is 80/30 units (Demo/ClothDemo.cpp:141); scaled here to grid
-    spacing d (volumes by the cube of the ratio)."""
-    r = d / (80.0 / 30.0)
-    return (0.001 * r ** 3, 0.01 * r ** 3, 0.1 * r)
 
 
 FOLD24 = Scene(

@@ -61,3 +61,16 @@ def test_fold_self_collision_reference_early_exit(sc, oracle_mod):
         o.substep_solve(h)
         assert m.stats()["iterations"] == o.stats()["iterations"], k
     same_positions(m, o, "fold early exit")
+
+
+def test_mesh_obstacle_with_self_collision_mid_size(sc, oracle_mod):
+    """The mesh512 workload's path at 40x40: perturbed cloth on an icosphere
+    mesh (subdivision 2) with vertex, edge, triangle and self contacts; self
+    contacts begin after about 80 substeps, once the cloth folds on the
+    ground."""
+    scene = S.MESH512.with_(n=40, side=1.0, mesh_subdiv=2, mesh_radius=0.25, offset=(0.0, 0.27, 0.0),
+                            walls_min=(-10.0, -0.3, -10.0), thickness=0.004, tolerance=0.005, frames=24,
+                            perturb_amplitude=0.02, ccd=S.ccd_for_spacing(1.0 / 39))
+    print("mesh+self 40", flush=True)
+    seen, m, o = _run(sc, oracle_mod, scene, 1, scene.frames * scene.substeps)
+    assert sum(seen) > 0, seen
