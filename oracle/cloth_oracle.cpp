@@ -1259,9 +1259,13 @@ public:
             const R m1 = invMass[c.i1], m2 = invMass[c.i2], m3 = invMass[c.i3], m4 = invMass[c.i4];
             V3<R> p, coords;
             closest_pt_point_triangle(x4, x1, x2, x3, p, coords);
+            // a collapsed triangle gives no closest point (the reference's
+            // coordinates become NaN and spread to the whole cloth): skipped
+            if (!std::isfinite(double(coords.x + coords.y + coords.z))) continue;
             c.w1 = coords.x; c.w2 = coords.y; c.w3 = coords.z;
             const V3<R> delta = x4 - p;
             V3<R> n = cross(x2 - x1, x3 - x1);
+            if (length(n) == R(0)) continue;
             normalize(n);
             const R side = signed_volume(x4, x1, x2, x3);
             if (c.side * side <= R(0)) len0 = R(prm.thickness) * R(0.1f);
@@ -1307,7 +1311,12 @@ public:
             const R omw1 = R(1.f) - c.w1, omw2 = R(1.f) - c.w2;
             const V3<R> delta = q - p;
             V3<R> n = delta;
-            normalize(n);
+            // edges that touch exactly have no direction between them (the
+            // reference normalises the zero vector into NaN, which spreads to
+            // the whole cloth): use the contact's normal instead
+            if (length(delta) == R(0)) n = c.normal;
+            else normalize(n);
+            if (!std::isfinite(double(n.x + n.y + n.z))) continue;
             if (dot(n, c.normal) < R(0.0f)) n = -n;
             if (side * c.side < R(0)) thick = R(prm.thickness) * R(0.1f);
             const R len = dot(n, delta);

@@ -130,7 +130,7 @@ struct SlabSet {
     int base;  // first slab of the launched phase
 };
 
-enum { kSlabBlock = 256, kMaxSlabColours = 63, kMaxSelfColours = 512 };
+enum { kSlabBlock = 256, kMaxSlabColours = 63, kMaxSelfColours = 4096 };
 
 // Vertex contacts applied by the FEM slab kernel while it stages vertices
 // (the previous iteration's contacts, by each vertex's owner slab).
@@ -154,6 +154,7 @@ struct SelfBuf {
     unsigned long long* key;
     int* colour;
     int* order;
+    int* active;  // scratch list for the colouring rounds
     int* groupOff;
     int cap;
     int maxColours;

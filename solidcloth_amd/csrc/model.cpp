@@ -267,7 +267,7 @@ void GpuSolver::release()
     dev_free(self.triLo); dev_free(self.triHi); dev_free(self.edgeLo); dev_free(self.edgeHi); dev_free(self.vmax);
     for (SelfBuf& b : self.buf) {
         dev_free(b.ids); dev_free(b.W); dev_free(b.N); dev_free(b.key); dev_free(b.colour); dev_free(b.order);
-        dev_free(b.groupOff);
+        dev_free(b.active); dev_free(b.groupOff);
         b.cap = 0;
     }
     self.cap = 0;
@@ -594,7 +594,8 @@ void ClothModel::ensure_self()
     if (cap != S.cap) {
         for (SelfBuf& b : S.buf) {
             dev_alloc(b.ids, cap); dev_alloc(b.W, cap); dev_alloc(b.N, cap); dev_alloc(b.key, cap);
-            dev_alloc(b.colour, cap); dev_alloc(b.order, cap); dev_alloc(b.groupOff, kMaxSelfColours + 1);
+            dev_alloc(b.colour, cap); dev_alloc(b.order, cap); dev_alloc(b.active, cap);
+            dev_alloc(b.groupOff, kMaxSelfColours + 1);
             b.cap = cap;
             b.maxColours = kMaxSelfColours;
         }
@@ -921,13 +922,15 @@ void ClothModel::step_substeps(float h, int n)
     HIP_OK(hipStreamSynchronize(g.stream));
     HIP_OK(hipGetLastError());
     if (mCollFlags & CF_SELF) {
-        int overflow = 0;
-        HIP_OK(hipMemcpy(&overflow, &g.d_ctl->selfOverflow, sizeof(int), hipMemcpyDeviceToHost));
-        if (overflow)
-            throw std::runtime_error("self-collision contacts exceeded their capacity (" + std::to_string(g.self.cap) +
-                                     ") or " + std::to_string(kMaxSelfColours) +
-                                     " colours; the step's self contacts are incomplete "
-                                     "(raise it with scSetSelfCapacity)");
+        SolverCtl c;
+        HIP_OK(hipMemcpy(&c, g.d_ctl, sizeof(c), hipMemcpyDeviceToHost));
+        if (c.selfOverflow)
+            throw std::runtime_error(
+                "self collision overflowed in this step: last substep found " + std::to_string(c.selfCount[0]) +
+                " vertex-triangle and " + std::to_string(c.selfCount[1]) + " edge-edge contacts (capacity " +
+                std::to_string(g.self.cap) + ", raise with scSetSelfCapacity) in " +
+                std::to_string(c.selfColours[0]) + " / " + std::to_string(c.selfColours[1]) + " colour groups (at most " +
+                std::to_string(kMaxSelfColours) + "); the step's self contacts are incomplete");
     }
     if (timed) g.timer.harvest();
     mDeviceNewer = true;

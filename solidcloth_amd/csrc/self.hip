@@ -338,58 +338,84 @@ __global__ void __launch_bounds__(kBlock) k_self_detect(const float4* __restrict
 
 // ------------------------------------------------------------ colouring ----
 
+// Bid of a contact: a bijective 64-bit mix (splitmix64's finaliser) of its
+// key, so that bids are unique and spatially neighbouring contacts do not
+// form long chains of decreasing bids (plain keys would make the colouring
+// take as many rounds as such a chain is long). No valid key maps to 0.
+__device__ __forceinline__ unsigned long long bid_of(unsigned long long key)
+{
+    unsigned long long z = key ^ 0xffffffffffffffffull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
 // Jones-Plassmann colouring of one kind's contacts, by one workgroup. In
-// round r every uncoloured contact bids its key at its four vertices; a
-// contact that holds the highest bid at all four takes colour r. Winners of a
-// round share no vertex, and the highest uncoloured key always wins, so every
-// round makes progress; the result depends only on the set of contacts. The
-// contacts are then listed colour by colour.
+// round r every uncoloured contact bids at its four vertices; a contact that
+// holds the highest bid at all four takes colour r. Winners of a round share
+// no vertex, and the highest uncoloured bid always wins, so every round makes
+// progress; the result depends only on the set of contacts. The contacts are
+// then listed colour by colour.
 __global__ void __launch_bounds__(kSoloBlock) k_self_colour(SelfBuf b, int kind, unsigned long long* vmax,
                                                             SolverCtl* ctl)
 {
-    __shared__ int remaining;
+    __shared__ int nextCount;
     __shared__ int cnt[kMaxSelfColours + 1];
     const int n = min(ctl->selfCount[kind], b.cap);
     const int tid = threadIdx.x, nt = blockDim.x;
-    auto vid = [&](int c, int k) {
+    auto vreset = [&](int c) {
         const int4 q = b.ids[c];
-        return k == 0 ? q.x : k == 1 ? q.y : k == 2 ? q.z : q.w;
+        __hip_atomic_store(&vmax[q.x], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&vmax[q.y], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&vmax[q.z], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&vmax[q.w], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     };
+    // the uncoloured contacts, compacted after every round (two lists)
+    int* cur = b.order;
+    int* nxt = b.active;
     for (int c = tid; c < n; c += nt) {
         b.colour[c] = -1;
-        for (int k = 0; k < 4; k++)
-            __hip_atomic_store(&vmax[vid(c, k)], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        cur[c] = c;
+        vreset(c);
     }
     __syncthreads();
-    int r = 0;
-    while (n > 0) {
-        if (tid == 0) remaining = 0;
-        for (int c = tid; c < n; c += nt) {
-            if (b.colour[c] >= 0) continue;
-            const unsigned long long bid = b.key[c] + 1ull;
-            for (int k = 0; k < 4; k++) atomicMax(&vmax[vid(c, k)], bid);
+    int m = n, r = 0;
+    while (m > 0) {
+        if (tid == 0) nextCount = 0;
+        for (int i = tid; i < m; i += nt) {
+            const int c = cur[i];
+            const int4 q = b.ids[c];
+            const unsigned long long bid = bid_of(b.key[c]);
+            atomicMax(&vmax[q.x], bid);
+            atomicMax(&vmax[q.y], bid);
+            atomicMax(&vmax[q.z], bid);
+            atomicMax(&vmax[q.w], bid);
         }
         __syncthreads();
-        for (int c = tid; c < n; c += nt) {
-            if (b.colour[c] >= 0) continue;
-            const unsigned long long bid = b.key[c] + 1ull;
-            bool win = true;
-            for (int k = 0; k < 4; k++)
-                win = win && __hip_atomic_load(&vmax[vid(c, k)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == bid;
+        for (int i = tid; i < m; i += nt) {
+            const int c = cur[i];
+            const int4 q = b.ids[c];
+            const unsigned long long bid = bid_of(b.key[c]);
+            const bool win =
+                __hip_atomic_load(&vmax[q.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == bid &&
+                __hip_atomic_load(&vmax[q.y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == bid &&
+                __hip_atomic_load(&vmax[q.z], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == bid &&
+                __hip_atomic_load(&vmax[q.w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == bid;
             if (win) b.colour[c] = r;
         }
         __syncthreads();
-        for (int c = tid; c < n; c += nt) {
-            const int col = b.colour[c];
-            if (col >= 0 && col != r) continue;
-            for (int k = 0; k < 4; k++)
-                __hip_atomic_store(&vmax[vid(c, k)], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (col < 0) atomicAdd(&remaining, 1);
+        for (int i = tid; i < m; i += nt) {
+            const int c = cur[i];
+            vreset(c);
+            if (b.colour[c] < 0) nxt[atomicAdd(&nextCount, 1)] = c;
         }
         __syncthreads();
-        const int left = remaining;
+        m = nextCount;
+        int* t = cur;
+        cur = nxt;
+        nxt = t;
         __syncthreads();
-        if (left == 0) break;
+        if (m == 0) break;
         if (++r >= b.maxColours) {
             if (tid == 0) ctl->selfOverflow = 1;
             break;
@@ -435,9 +461,14 @@ __device__ float self_vt(float4* pos, SelfBuf b, int c, const SelfParams& sp, fl
     float len0 = sp.thickness * sp.shrink;
     f3 p, co;
     closest_pt_point_triangle(x4, x1, x2, x3, p, co);
+    // a collapsed triangle gives no closest point (the reference's
+    // coordinates become NaN and spread to the whole cloth): skipped
+    if (!isfinite(co.x + co.y + co.z)) return 0.f;
     const float w1 = co.x, w2 = co.y, w3 = co.z;
     const f3 delta = sub(x4, p);
-    f3 n = normalized(cross(sub(x2, x1), sub(x3, x1)));
+    const f3 nc = cross(sub(x2, x1), sub(x3, x1));
+    if (len(nc) == 0.f) return 0.f;
+    f3 n = normalized(nc);
     const float side = signed_volume(x4, x1, x2, x3);
     if (W.w * side <= 0.f) len0 = sp.thickness * 0.1f;
     if (dot(n, cn) < 0.f) n = neg(n);
@@ -482,7 +513,11 @@ __device__ float self_ee(float4* pos, SelfBuf b, int c, const SelfParams& sp, fl
     const float w1 = 1.f - t1, w2 = 1.f - t2;
     const float omw1 = 1.f - w1, omw2 = 1.f - w2;
     const f3 delta = sub(q, p);
-    f3 n = normalized(delta);
+    // edges that touch exactly have no direction between them (the reference
+    // normalises the zero vector into NaN, which spreads to the whole cloth):
+    // the contact's normal is used instead
+    f3 n = len(delta) == 0.f ? xyz(N) : normalized(delta);
+    if (!isfinite(n.x + n.y + n.z)) return 0.f;
     if (dot(n, xyz(N)) < 0.0f) n = neg(n);
     if (side * W.w < 0.f) thick = sp.thickness * 0.1f;
     const float len = dot(n, delta);

@@ -138,3 +138,37 @@ def test_edge_edge_ccd_and_projection(oracle_mod):
     d0 = -gap0
     thick = 0.1 * 0.01
     assert m1[1] - m2[1] == pytest.approx(thick + 0.1 * (d0 - thick), abs=1e-7)
+
+
+def test_touching_edges_use_the_contact_normal(oracle_mod):
+    """Edges that touch exactly have no direction between them; the
+    reference normalises the zero vector into NaN, which then spreads over the
+    whole cloth. Here the contact's normal is used: the midpoints separate to
+    0.9 x the target distance along it. (Coordinates chosen so that the two
+    closest points coincide exactly in FP32.)"""
+    h, u, a, b = 0.004, 0.01, 0.003, 0.002
+    prev = np.array([[-0.25, h, 0.0], [0.25, h, 0.0], [0.0, 0.0, -0.25], [0.0, 0.0, 0.25],
+                     [5.0, 1.0, 5.0], [5.2, 1.0, 5.0]])
+    vel = np.array([[0, -u, a], [0, -u, -a], [b, 0, 0], [-b, 0, 0], [0, 0, 0], [0, 0, 0]])
+    c = oracle_mod.Cloth()
+    c.set(thickness=0.01, tolerance=0.01, collFlags=CF_SELF, friction=0.0, fem=1,
+          ccdVolume=1e-14, ccdBisect=1e-16, ccdVtDistance=0.001)
+    c.init_arrays(prev, prev[:, [0, 2]], np.ones(6), np.array([[0, 1, 4], [2, 3, 5]], dtype=np.int32))
+    c.put("prev", prev)
+    c.put("pos", prev + vel)
+    c.detect()
+    keys, ids, data = c.self_contacts("ee")
+    assert len(keys) == 1
+    normal = data[0, 3:6]
+    touching = prev.copy()
+    touching[0:2, 1] = 0.0  # edge (0, 1) now crosses edge (2, 3) at the origin
+    r, s_, t_, c1, c2 = oracle_mod.closest_pt_segm_segm(*touching[:4])
+    assert np.array_equal(c1, c2)
+    c.put("pos", touching)
+    c.solve_self("ee", 1.0 / 240)
+    p1 = c.get("pos")
+    assert np.isfinite(p1).all()
+    m1, m2 = 0.5 * (p1[0] + p1[1]), 0.5 * (p1[2] + p1[3])
+    # separated along the contact normal (from edge 1 to edge 2 at t = 0)
+    sep = np.dot(m2 - m1, normal)
+    assert sep == pytest.approx(0.9 * 0.9 * 0.01, abs=2e-7)
