@@ -42,7 +42,115 @@ WORKLOADS = {
                "sinusoids (0.05 m), dropped onto an icosphere mesh (subdivision 6: 40,962 verts, 81,920 "
                "triangles, r 0.4) + ground; vertex, edge and triangle mesh contacts, continuous self "
                "collision (LBVH + FP64-cubic CCD); 10 iterations, 4 substeps/frame, friction 0.3, thickness 2 mm",
+    "batch512": "batch512: 512 independent cloths (grid side U{16..128}, Young logU[1e3,1e5], Poisson "
+                "U[0.1,0.45], bending logU[1e-3,1e-1], 1-4 pinned boundary vertices), each over its own sphere "
+                "(r U[0.2,0.5], 32^3 grid SDF); one launch per substep for all cloths (one workgroup per cloth), "
+                "graph-replayed, launch autotuned; 10 iterations, 4 substeps/frame, seed 4",
 }
+BYTES_PER_CONTACT = 92
+BYTES_PER_VERTEX_PV = 64
+BYTES_PER_SDF_DETECT = 204
+
+
+def run_batch(args):
+    """batch512: every step is one frame of all 512 cloths (4 batch launches)."""
+    import numpy as np
+
+    import solidcloth_amd as sc
+    from solidcloth_amd import scenes as S
+
+    scenes = S.batch_scenes(512)
+    t0 = time.perf_counter()
+    built = [S.build_patch(s, sc) for s in scenes]
+    batch = sc.ClothBatch([p for _, p in built])
+    print(f"batch512: built {len(built)} patches in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    dt = scenes[0].dt
+    warmup = max(args.warmup, 7)  # the autotuner times its 6 candidates on the first steps
+    for _ in range(warmup):
+        batch.Step(dt)
+    dev_ms, launches = 0.0, 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.Step(dt)
+        ms, n = batch.last_step_time()
+        dev_ms += ms
+        launches += n
+    elapsed = time.perf_counter() - t0
+    sub = scenes[0].substeps
+    value = args.steps * sub / elapsed
+    V, T, B = S.batch_counts(scenes)
+    its = scenes[0].iterations
+    passes = (its + 1) // 2
+    contacts = sum(p.GetModel().GetNumContacts() for _, p in built)  # last substep's
+    nonfinite = sum(int(not np.isfinite(p.GetModel().positions()).all()) for _, p in built)
+    proj = its * T + passes * B + its * contacts
+    bytes_launch = (BYTES_PER_VERTEX_PV + BYTES_PER_SDF_DETECT) * V + 136 * its * T + 128 * passes * B \
+        + BYTES_PER_CONTACT * its * contacts
+    avg_us = 1000.0 * dev_ms / max(1, launches)
+    achieved = bytes_launch / (avg_us * 1e-6) / 1e9
+    bs, stage, tuned = batch.GetLaunch()
+    out = {
+        "metric": "PBD substeps/sec, batch512 (every substep advances all 512 cloths)",
+        "value": round(value, 2),
+        "unit": "substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": warmup,
+        "ms_per_step": round(1000.0 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": WORKLOADS["batch512"], "step": "one frame = 4 substeps of every cloth",
+                   "cloths": len(scenes), "vertices": V, "triangles": T, "bends": B,
+                   "launch": {"block_size": bs, "lds_staging": bool(stage), "autotuned": tuned},
+                   "tuning_ms_per_step": [[b, s_, round(ms, 3)] for b, s_, ms in batch.tuning()]},
+        "cloth_substeps_per_s": round(value * len(scenes), 1),
+        "constraint_projections_per_s": round(proj * value, 1),
+        "vertex_contacts_last_substep": contacts,
+        "cloths_with_nonfinite_positions": nonfinite,
+        "roofline": {
+            "kernel": "k_batch_substep",
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "avg_launch_us": round(avg_us, 3),
+            "algorithmic_bytes_per_launch": int(bytes_launch),
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_batch(scenes, args.cpu_budget)
+    print(json.dumps(out))
+
+
+def cpu_baseline_batch(scenes, budget_s):
+    """The oracle steps the cloths one after another (one frame each) until the
+    budget is spent; the batch rate is extrapolated by constraint count."""
+    import oracle
+
+    its = scenes[0].iterations
+    passes = (its + 1) // 2
+
+    def work(s):
+        return its * 2 * (s.n - 1) ** 2 + passes * (3 * (s.n - 1) ** 2 - 2 * (s.n - 1))
+
+    done, k, el = 0, 0, 0.0
+    while k < len(scenes) and el < budget_s:
+        o = oracle.from_scene(scenes[k])  # set-up is not timed
+        t1 = time.perf_counter()
+        o.step(scenes[k].dt)
+        el += time.perf_counter() - t1
+        done += work(scenes[k])
+        k += 1
+    total = sum(work(s) for s in scenes)
+    per_substep_s = el / max(1, done) * total / scenes[0].substeps
+    return {"value": round(1.0 / per_substep_s, 4), "unit": "substeps/s", "cores": 1, "kind": "port",
+            "sample": f"{k} of {len(scenes)} cloths, one frame each, single-threaded oracle; the whole batch's "
+                      "rate extrapolated by constraint-projection count"}
 
 
 def run_gpu(scene, steps, warmup, use_graph=True, sweep=1):
@@ -109,7 +217,7 @@ def main():
                     help="0: one launch per colour; 1: slab phases staged in LDS")
     ap.add_argument("--copy-probe-mb", type=int, default=1024,
                     help="size of the streaming-copy bandwidth probe (0 disables)")
-    ap.add_argument("--workload", default="drape256", choices=["drape256", "mesh512"],
+    ap.add_argument("--workload", default="drape256", choices=["drape256", "mesh512", "batch512"],
                     help="drape256: the metric's workload; mesh512: the triangle-mesh obstacle path "
                          "(reported beside it, not the metric)")
     args = ap.parse_args()
@@ -120,6 +228,9 @@ def main():
 
     from solidcloth_amd import scenes as S
 
+    if args.workload == "batch512":
+        run_batch(args)
+        return
     scene = S.DRAPE256 if args.workload == "drape256" else S.MESH512
     elapsed, stats, keep = run_gpu(scene, args.steps, args.warmup, use_graph=not args.no_graph, sweep=args.sweep)
     m = keep[2]

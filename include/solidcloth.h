@@ -44,6 +44,7 @@ typedef struct { void* ptr; } scCollisionWorld;
 typedef struct { void* ptr; } scMesh;
 typedef struct { void* ptr; } scModel;
 typedef struct { void* ptr; } scSDF; /* replaces Geometry::SDF* (Demo/ClothAPI.h:80) */
+typedef struct { void* ptr; } scClothBatch; /* many patches stepped together (no counterpart) */
 
 enum scCollFlags { /* Demo/ClothAPI.h:34-41 */
     scCollisionFlagVertices = 1,
@@ -266,6 +267,36 @@ SCEXP void scSetUseGraph(scModel model, bool enable);
  * milliseconds (HIP events), or a negative value on error. Used to measure
  * attainable HBM bandwidth and to calibrate traffic counters. */
 SCEXP double scBandwidthProbe(long long bytes, int reps);
+
+/* Inverse masses of all particles (0 = pinned), in the order of
+ * scGetParticleInvMass (Physics/ClothModel.h:102 GetParticle(i).invMass). */
+SCEXP int scSetParticleInvMasses(scModel model, const float* in, int count);
+
+/* Batched stepping (no counterpart in the reference, whose demo steps one
+ * patch per scStepCloth, Demo/ClothAPI.cpp:139-142): patches of any size,
+ * each with its own collision world, stepped by one launch per substep in
+ * which one workgroup runs one cloth's whole substep. The result of each
+ * patch equals scStepCloth with the colour sweep (scSetSweep mode 0), which
+ * scStepClothBatch selects for its patches. Patches support vertex and wall
+ * contacts against walls, spheres and grid SDFs; all need the same NumSteps.
+ * Free the batch before its patches.
+ * scBatchSetLaunch: block size 256, 512 or 1024 (0 = autotune) and position
+ * staging in LDS (0 off, 1 on, -1 = autotune). Autotuning times each
+ * candidate on one real step (every candidate computes identical results).
+ * scBatchGetLaunch: the chosen values and whether tuning is complete.
+ * scBatchGetTuning: the configurations timed so far (block size, staging)
+ * and their step times in ms; returns their count.
+ * scBatchGetStepTime: device time in ms of the last step's substep launches
+ * (HIP events on the batch's stream) and the number of launches. */
+SCEXP scClothBatch scCreateClothBatch(void);
+SCEXP void scFreeClothBatch(scClothBatch batch);
+SCEXP void scBatchAddPatch(scClothBatch batch, scClothPatch patch);
+SCEXP int scBatchGetNumPatches(scClothBatch batch);
+SCEXP void scStepClothBatch(scClothBatch batch, float dt);
+SCEXP void scBatchSetLaunch(scClothBatch batch, int blockSize, int stage);
+SCEXP void scBatchGetLaunch(scClothBatch batch, int* blockSize, int* stage, int* tuned);
+SCEXP int scBatchGetTuning(scClothBatch batch, int* configs, double* ms, int cap);
+SCEXP double scBatchGetStepTime(scClothBatch batch, int* launches);
 
 /* Last error message of the calling thread, or "" (not in the reference),
  * and the number of errors recorded on the calling thread so far, so a

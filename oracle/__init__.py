@@ -374,6 +374,10 @@ def from_scene(scene, real64=False, tri_order=None, bend_order=None):
         c.put("pos", S.folded_positions(scene, c.get("pos")))
     if scene.ccd is not None:
         c.set(ccdVolume=scene.ccd[0], ccdBisect=scene.ccd[1], ccdVtDistance=scene.ccd[2])
+    if scene.pins is not None:
+        im = _np.ones(scene.num_vertices)
+        im[list(scene.pins)] = 0.0
+        c.put("invMass", im)
     if tri_order is not None:
         c.set_order("tri", tri_order)
     if bend_order is not None:

@@ -110,6 +110,12 @@ def _sig(L):
         "scLbvhSelfCheck": (i, [i, C.c_uint, i, vp]),
         "scGetSelfContacts": (i, [H, i, vp, vp, vp, i]), "scSetSelfParams": (None, [H, f, f]),
         "scSetSelfCCD": (None, [H, f, f, f]), "scSetSelfCapacity": (None, [H, i]),
+        "scSetParticleInvMasses": (i, [H, vp, i]),
+        "scCreateClothBatch": (H, []), "scFreeClothBatch": (None, [H]), "scBatchAddPatch": (None, [H, H]),
+        "scBatchGetNumPatches": (i, [H]), "scStepClothBatch": (None, [H, f]),
+        "scBatchSetLaunch": (None, [H, i, i]),
+        "scBatchGetLaunch": (None, [H, C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
+        "scBatchGetTuning": (i, [H, vp, vp, i]), "scBatchGetStepTime": (C.c_double, [H, C.POINTER(i)]),
         "scGetLastError": (C.c_char_p, []),
         "scGetErrorCount": (i, []), "scGetBuildInfo": (C.c_char_p, []),
     }
@@ -303,6 +309,10 @@ class ClothModel:
     def inv_masses(self):
         return np.array([self.GetParticleInvMass(i) for i in range(self.GetNumParticles())], dtype=np.float32)
 
+    def set_inv_masses(self, m):
+        a = np.ascontiguousarray(m, dtype=np.float32)
+        _checked("scSetParticleInvMasses", self.h, _ptr(a), a.shape[0])
+
     def triangles(self):
         n = self.GetNumTris()
         out = np.zeros((n, 3), dtype=np.int32)
@@ -347,6 +357,9 @@ class ClothModel:
     def SetSweep(self, mode, slab_size=0):
         """mode 0: one launch per colour; mode 1: slab phases staged in LDS."""
         _checked("scSetSweep", self.h, int(mode), int(slab_size))
+
+    def GetNumContacts(self):
+        return _checked("scGetNumContacts", self.h)
 
     def contacts(self):
         n = _checked("scGetNumContacts", self.h)
@@ -453,3 +466,52 @@ class ClothPatch:
 
     def UpdateMesh(self):
         _checked("scUpdateClothMesh", self.h)
+
+
+class ClothBatch:
+    """Many patches stepped together, one launch per substep (scClothBatch).
+
+    Keeps references to its patches so they outlive the batch."""
+
+    def __init__(self, patches=()):
+        self.h = _checked("scCreateClothBatch")
+        self.patches = []
+        for p in patches:
+            self.Add(p)
+
+    def __del__(self):
+        try:
+            lib().scFreeClothBatch(self.h)
+        except Exception:
+            pass
+
+    def Add(self, patch):
+        _checked("scBatchAddPatch", self.h, patch.h)
+        self.patches.append(patch)
+
+    def GetNumPatches(self):
+        return _checked("scBatchGetNumPatches", self.h)
+
+    def Step(self, dt):
+        _checked("scStepClothBatch", self.h, float(dt))
+
+    def SetLaunch(self, block_size=0, stage=-1):
+        _checked("scBatchSetLaunch", self.h, int(block_size), int(stage))
+
+    def GetLaunch(self):
+        b, s, t = C.c_int(), C.c_int(), C.c_int()
+        _checked("scBatchGetLaunch", self.h, C.byref(b), C.byref(s), C.byref(t))
+        return b.value, s.value, bool(t.value)
+
+    def last_step_time(self):
+        """(device ms of the last step's substep launches, launch count)."""
+        n = C.c_int()
+        ms = _checked("scBatchGetStepTime", self.h, C.byref(n))
+        return ms, n.value
+
+    def tuning(self):
+        n = _checked("scBatchGetTuning", self.h, None, None, 0)
+        cfg = np.zeros((n, 2), dtype=np.int32)
+        ms = np.zeros(n, dtype=np.float64)
+        _checked("scBatchGetTuning", self.h, _ptr(cfg), _ptr(ms), n)
+        return [(int(c[0]), int(c[1]), float(t)) for c, t in zip(cfg, ms)]

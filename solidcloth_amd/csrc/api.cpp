@@ -548,6 +548,90 @@ int scSetParticlePositions(scModel model, const scVector3* in, int count)
     });
 }
 
+int scSetParticleInvMasses(scModel model, const float* in, int count)
+{
+    return guard_r<int>(-1, [&] {
+        ClothModel* m = M(model);
+        m->sync_to_host();
+        int n = std::min(count, (int)m->invMass.size());
+        for (int i = 0; i < n; i++) {
+            if (!(in[i] >= 0.f)) throw std::runtime_error("inverse masses must be non-negative");
+            m->invMass[i] = in[i];
+        }
+        m->mark_host_dirty();
+        return n;
+    });
+}
+
+static ClothBatch* B(scClothBatch b)
+{
+    if (!b.ptr) throw std::runtime_error("null cloth batch");
+    return (ClothBatch*)b.ptr;
+}
+
+scClothBatch scCreateClothBatch(void)
+{
+    scClothBatch b{nullptr};
+    guard([&] { b.ptr = new ClothBatch(); });
+    return b;
+}
+
+void scFreeClothBatch(scClothBatch batch) { guard([&] { delete (ClothBatch*)batch.ptr; }); }
+
+void scBatchAddPatch(scClothBatch batch, scClothPatch patch) { guard([&] { B(batch)->add(P(patch)); }); }
+
+int scBatchGetNumPatches(scClothBatch batch)
+{
+    return guard_r<int>(0, [&] { return (int)B(batch)->patches.size(); });
+}
+
+void scStepClothBatch(scClothBatch batch, float dt) { guard([&] { B(batch)->step(dt); }); }
+
+void scBatchSetLaunch(scClothBatch batch, int blockSize, int stage)
+{
+    guard([&] {
+        if (blockSize != 0 && blockSize != 256 && blockSize != 512 && blockSize != 1024)
+            throw std::runtime_error("block size must be 0, 256, 512 or 1024");
+        if (stage < -1 || stage > 1) throw std::runtime_error("stage must be -1, 0 or 1");
+        B(batch)->set_launch(blockSize, stage);
+    });
+}
+
+void scBatchGetLaunch(scClothBatch batch, int* blockSize, int* stage, int* tuned)
+{
+    guard([&] {
+        ClothBatch* b = B(batch);
+        if (blockSize) *blockSize = b->blockSize;
+        if (stage) *stage = b->stage;
+        if (tuned) *tuned = b->tuned ? 1 : 0;
+    });
+}
+
+int scBatchGetTuning(scClothBatch batch, int* configs, double* ms, int cap)
+{
+    return guard_r<int>(-1, [&] {
+        ClothBatch* b = B(batch);
+        const int n = (int)b->trialMs.size();
+        for (int i = 0; i < n && i < cap; i++) {
+            if (configs) {
+                configs[2 * i] = b->trialConfigs[i].first;
+                configs[2 * i + 1] = b->trialConfigs[i].second;
+            }
+            if (ms) ms[i] = b->trialMs[i];
+        }
+        return n;
+    });
+}
+
+double scBatchGetStepTime(scClothBatch batch, int* launches)
+{
+    return guard_r<double>(-1.0, [&] {
+        ClothBatch* b = B(batch);
+        if (launches) *launches = b->lastSubsteps;
+        return b->lastStepMs;
+    });
+}
+
 int scGetParticleVelocities(scModel model, scVector3* out, int count)
 {
     return guard_r<int>(-1, [&] {

@@ -168,6 +168,25 @@ struct SelfParams {
     float vtDistance;
 };
 
+// One cloth of a batch: its own device buffers (the patch's), its greedy
+// colour schedule and its substep parameters. The batch kernel runs a whole
+// substep of one cloth in one workgroup.
+struct ClothDesc {
+    float4 *pos, *prev, *vel;
+    TriStreams tr;      // triangles in colour order (global vertex ids)
+    const int* triCol;  // colour c spans [triCol[c], triCol[c+1])
+    const int4* bidx;   // bends in colour order
+    const int* bendCol;
+    ContactBuf cb;
+    const CollSet* cs;  // the cloth's collidables (device copy)
+    SolverCtl* ctl;
+    int nv, nt, nb, triColours, bendColours;
+    float h, h2, epsUV, sStiff, drag, thickness, tolerance, mu, invH, selfThreshold;
+    f3 gdv;
+    int iterations, fixedIterations, bendFreq, dihedral, collFlags, anyContacts;
+    int pad[2];
+};
+
 // Launchers (kernels.hip). All are asynchronous on the given stream.
 void launch_predict(hipStream_t s, float4* pos, float4* prev, float4* vel, int n, float h, f3 gdv, float drag,
                     SolverCtl* ctl);
@@ -191,6 +210,13 @@ void launch_iter_end(hipStream_t s, SolverCtl* ctl, int maxIter, int fixedIter, 
                      float selfThreshold);
 void launch_velocity(hipStream_t s, float4* pos, const float4* prev, float4* vel, int n, float invH, SolverCtl* ctl);
 void launch_copy(hipStream_t s, const float4* src, float4* dst, long n);
+
+// Batch: one workgroup per cloth (cloths in launch order), one launch per
+// substep. blockSize 256, 512 or 1024; cloths with at most stageVerts
+// vertices keep their positions in LDS during the iterations.
+void launch_batch_substep(hipStream_t s, const ClothDesc* descs, int numCloths, int blockSize, int stageVerts);
+// Allows the batch kernel `bytes` of dynamic LDS (called before capture).
+void batch_allow_lds(int bytes);
 
 // Mesh obstacles (mesh.hip). Prediction that also records max |pos - prev|
 // and clears the feature-contact groups; detection of obstacle-edge and

@@ -10,6 +10,7 @@
 #include "bvh_query.h"
 #include "collide_dev.h"
 #include "devutil.h"
+#include "hiputil.h"
 
 namespace sc {
 
@@ -590,6 +591,160 @@ void launch_copy(hipStream_t s, const float4* src, float4* dst, long n)
     long nb = (n + kBlock - 1) / kBlock;
     if (nb > 256 * 64) nb = 256 * 64;
     hipLaunchKernelGGL(k_copy, dim3((unsigned)nb), dim3(kBlock), 0, s, src, dst, n);
+}
+
+// ----------------------------------------------------------------- batch ----
+
+// Workgroup-wide maximum of non-negative floats (every thread gets it).
+template <int BS>
+__device__ __forceinline__ float block_max(float v, float* red)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const float w = __shfl_xor(v, o, 64);
+        v = v < w ? w : v;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float m = red[0];
+    for (int k = 1; k < BS / 64; k++) m = m < red[k] ? red[k] : m;
+    return m;
+}
+
+// The Gauss-Seidel iterations of one cloth on positions P (LDS or HBM):
+// the same projections, in the same colour order, as the single-cloth
+// colour-schedule kernels (k_fem, k_bend, k_contacts, k_iter_end).
+template <int BS, class Pos>
+__device__ __forceinline__ void batch_iterations(const ClothDesc& d, Pos P, float* red, int& itersOut, float& cbOut,
+                                                 float& cvOut)
+{
+    float cb = 0.f, cv = 0.f;
+    int k = 0;
+    while (true) {
+        for (int c = 0; c < d.triColours; c++) {
+            const int e = d.triCol[c + 1];
+            for (int t = d.triCol[c] + threadIdx.x; t < e; t += BS) {
+                const int4 id = d.tr.idx[t];
+                float4 P1 = P[id.x], P2 = P[id.y], P3 = P[id.z];
+                fem_project(P1, P2, P3, d.tr.a[t], d.tr.b[t], d.tr.c[t], d.h2, d.epsUV);
+                P[id.x] = P1;
+                P[id.y] = P2;
+                P[id.z] = P3;
+            }
+            __syncthreads();
+        }
+        if (d.dihedral && k % d.bendFreq == 0) {
+            float err = 0.f;
+            for (int c = 0; c < d.bendColours; c++) {
+                const int e = d.bendCol[c + 1];
+                for (int t = d.bendCol[c] + threadIdx.x; t < e; t += BS) {
+                    const int4 b = d.bidx[t];
+                    float4 P1 = P[b.x], P2 = P[b.y], P3 = P[b.z], P4 = P[b.w];
+                    const float e1 = bend_project(P1, P2, P3, P4, d.sStiff);
+                    err = err < e1 ? e1 : err;
+                    P[b.x] = P1;
+                    P[b.y] = P2;
+                    P[b.z] = P3;
+                    P[b.w] = P4;
+                }
+                __syncthreads();
+            }
+            cb = block_max<BS>(err, red);
+        }
+        if (d.anyContacts) {
+            float err = 0.f;
+            for (int v = threadIdx.x; v < d.nv; v += BS) {
+                const int n = d.cb.count[v];
+                if (n == 0) continue;
+                float4 Q = P[v];
+                f3 p = xyz(Q), ve = xyz(d.vel[v]);
+                const float e1 = vertex_contacts(v, n, p, xyz(d.prev[v]), ve, d.cb, d.thickness, d.mu, d.invH);
+                err = err < e1 ? e1 : err;
+                P[v] = f4(p, Q.w);
+                d.vel[v] = f4(ve, 0.f);
+            }
+            cv = block_max<BS>(err, red);
+        } else {
+            cv = 0.f;
+        }
+        k++;
+        // ClothPBD.cpp:354-361; link, edge, triangle and self terms are 0 here
+        const bool exitNow = !d.fixedIterations && cb < 0.005f && cv < 0.01f && 0.f < d.selfThreshold;
+        if (exitNow || k >= d.iterations) break;
+    }
+    itersOut = k;
+    cbOut = cb;
+    cvOut = cv;
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) k_batch_substep(const ClothDesc* __restrict__ descs, int stageVerts)
+{
+    extern __shared__ float4 sv[];
+    __shared__ float red[BS / 64];
+    const ClothDesc d = descs[blockIdx.x];
+    // prediction and detection (ClothPBD.cpp:258-266, ClothCollision.cpp)
+    for (int i = threadIdx.x; i < d.nv; i += BS) {
+        const float4 P = d.pos[i];
+        const f3 pv = xyz(P);
+        d.prev[i] = make_float4(P.x, P.y, P.z, 0.f);
+        f3 p = pv;
+        if (P.w != 0.f) {
+            f3 v = add(xyz(d.vel[i]), d.gdv);
+            v = f3{v.x * d.drag, v.y * d.drag, v.z * d.drag};
+            p = add(p, mul(d.h, v));
+            d.vel[i] = f4(v, 0.f);
+            d.pos[i] = f4(p, P.w);
+        }
+        if (d.anyContacts) detect_vertex<false>(i, p, pv, *d.cs, d.cb, d.thickness, d.tolerance, d.collFlags, 0.f);
+    }
+    __syncthreads();
+    int iters;
+    float cb, cv;
+    const bool stage = d.nv <= stageVerts;
+    if (stage) {
+        for (int i = threadIdx.x; i < d.nv; i += BS) sv[i] = d.pos[i];
+        __syncthreads();
+        batch_iterations<BS>(d, sv, red, iters, cb, cv);
+        for (int i = threadIdx.x; i < d.nv; i += BS) d.pos[i] = sv[i];
+    } else {
+        batch_iterations<BS>(d, d.pos, red, iters, cb, cv);
+    }
+    __syncthreads();
+    // ClothPBD.cpp:314-321
+    for (int i = threadIdx.x; i < d.nv; i += BS) {
+        const float4 P = d.pos[i];
+        if (P.w == 0.f) continue;
+        d.vel[i] = f4(mul(d.invH, sub(xyz(P), xyz(d.prev[i]))), 0.f);
+    }
+    if (threadIdx.x == 0) {
+        d.ctl->done = 1;
+        d.ctl->k = iters;
+        d.ctl->iterations = iters;
+        d.ctl->cb = __float_as_uint(cb);
+        d.ctl->cv = __float_as_uint(cv);
+        d.ctl->bendError = cb;
+        d.ctl->contactError = cv;
+    }
+}
+
+void batch_allow_lds(int bytes)
+{
+    HIP_OK(hipFuncSetAttribute((const void*)k_batch_substep<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    HIP_OK(hipFuncSetAttribute((const void*)k_batch_substep<512>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    HIP_OK(hipFuncSetAttribute((const void*)k_batch_substep<256>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+}
+
+void launch_batch_substep(hipStream_t s, const ClothDesc* descs, int numCloths, int blockSize, int stageVerts)
+{
+    if (numCloths <= 0) return;
+    const size_t lds = (size_t)(stageVerts > 0 ? stageVerts : 0) * sizeof(float4);
+    if (blockSize == 1024)
+        hipLaunchKernelGGL(k_batch_substep<1024>, dim3(numCloths), dim3(1024), lds, s, descs, stageVerts);
+    else if (blockSize == 512)
+        hipLaunchKernelGGL(k_batch_substep<512>, dim3(numCloths), dim3(512), lds, s, descs, stageVerts);
+    else
+        hipLaunchKernelGGL(k_batch_substep<256>, dim3(numCloths), dim3(256), lds, s, descs, stageVerts);
 }
 
 static inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }

@@ -256,6 +256,7 @@ void GpuSolver::release()
     graphExec = nullptr; graph = nullptr; graphKey.clear();
     dev_free(d_pos); dev_free(d_prev); dev_free(d_vel);
     dev_free(d_tidx); dev_free(d_tA); dev_free(d_tB); dev_free(d_tC); dev_free(d_bidx);
+    dev_free(d_triColOff); dev_free(d_bendColOff);
     dev_free(d_cCount); dev_free(d_cP); dev_free(d_cN); dev_free(d_cSrc); dev_free(d_cFeat); dev_free(d_ctl);
     dev_free(d_geo); dev_free(d_cEdges); dev_free(d_cTris);
     geoHost.clear();
@@ -435,6 +436,12 @@ void ClothModel::build_device_topology()
     }
     dev_alloc(g.d_bidx, g.nb);
     if (g.nb) HIP_OK(hipMemcpy(g.d_bidx, bi.data(), g.nb * sizeof(int4), hipMemcpyHostToDevice));
+    dev_alloc(g.d_triColOff, triCol.phaseOffsets.size());
+    HIP_OK(hipMemcpy(g.d_triColOff, triCol.phaseOffsets.data(), triCol.phaseOffsets.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+    dev_alloc(g.d_bendColOff, bendCol.phaseOffsets.size());
+    HIP_OK(hipMemcpy(g.d_bendColOff, bendCol.phaseOffsets.data(), bendCol.phaseOffsets.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
     g.triOrphans = upload_slabs(triCol, g.triSlabs, true, g.nv);
     upload_slabs(bendCol, g.bendSlabs, false, g.nv);
     dev_alloc(g.d_cCount, g.nv);
@@ -1198,6 +1205,235 @@ void ClothPatch::compute_strain_map(scVector3* colors, int count)
             colors[id[k]].z += w * s;
         }
     }
+}
+
+}  // namespace sc
+
+namespace sc {
+
+// ---------------------------------------------------------------- batch ----
+
+void ClothModel::validate_batch() const
+{
+    if (pos.empty()) throw std::runtime_error("a batched patch has no cloth (scInitCloth first)");
+    if (!mFEM) throw std::runtime_error("the link (non-FEM) stretch model is not implemented; call scSetUseFEM(true)");
+    if (mNumIterations < 1) throw std::runtime_error("NumIterations must be >= 1");
+    if (mBendFreq < 1) throw std::runtime_error("bend frequency must be >= 1");
+    if (msActive) throw std::runtime_error("the mouse spring is not implemented on the GPU substep");
+    if (mCollFlags & (CF_EDGES | CF_TRIANGLES | CF_SELF))
+        throw std::runtime_error("batched cloths support vertex and wall contacts only");
+    for (const auto& c : mOwner->mWorld->colls)
+        if (c->type != CT_WALLS && c->type != CT_SPHERE && c->type != CT_SDF)
+            throw std::runtime_error("batched cloths support walls, spheres and grid SDFs only");
+    if (mOwner->mWorld->colls.size() > MAX_COLLIDABLES) throw std::runtime_error("too many collidables (max 8)");
+}
+
+CollSet ClothModel::prepare_batch(int& slots)
+{
+    if (mSweepMode != SWEEP_COLOURS) set_sweep(SWEEP_COLOURS, 0);
+    sync_to_device();
+    for (const auto& c : mOwner->mWorld->colls)
+        if (c->type == CT_SDF && !((CollisionSDF*)c.get())->sdf->d_nodes) ((CollisionSDF*)c.get())->sdf->upload();
+    int edgeSlots = 0, triSlots = 0;
+    CollSet cs = make_collset(slots, edgeSlots, triSlots);
+    ensure_contact_capacity(std::max(slots, 1));
+    gpu.activeSlots = slots;
+    return cs;
+}
+
+ClothDesc ClothModel::batch_desc(float h, const CollSet* d_cs, bool anyContacts) const
+{
+    const GpuSolver& g = gpu;
+    ClothDesc d;
+    memset(&d, 0, sizeof(d));
+    d.pos = g.d_pos; d.prev = g.d_prev; d.vel = g.d_vel;
+    d.tr = TriStreams{g.d_tidx, g.d_tA, g.d_tB, g.d_tC};
+    d.triCol = g.d_triColOff;
+    d.bidx = g.d_bidx;
+    d.bendCol = g.d_bendColOff;
+    d.cb = ContactBuf{g.d_cCount, g.d_cP, g.d_cN, g.d_cSrc, g.d_cFeat, nullptr, g.nv, g.slots};
+    d.cs = d_cs;
+    d.ctl = g.d_ctl;
+    d.nv = g.nv; d.nt = g.nt; d.nb = g.nb;
+    d.triColours = triCol.numPhases();
+    d.bendColours = bendCol.numPhases();
+    // the same parameter expressions as record_substep
+    const f3 gravity{0.f, -9.8f, 0.f};
+    const f3 hg = mul(h, gravity);
+    d.gdv = f3{mUnit * hg.x, mUnit * hg.y, mUnit * hg.z};
+    d.h = h;
+    d.h2 = h * h;
+    d.epsUV = 2 * (1 + mPoisson) / (smax(mShearStiff, 0.1f) * mUnit * mUnit);
+    d.sStiff = 0.0005f * mBendStiff;
+    d.drag = mDragCoeff;
+    d.thickness = mThickness;
+    d.tolerance = mTolerance;
+    d.mu = mFriction;
+    d.invH = 1.f / h;
+    d.selfThreshold = mSelfCollThreshold;
+    d.iterations = mNumIterations;
+    d.fixedIterations = mFixedIterations ? 1 : 0;
+    d.bendFreq = mBendFreq;
+    d.dihedral = mDihedral ? 1 : 0;
+    d.collFlags = mCollFlags;
+    d.anyContacts = anyContacts ? 1 : 0;
+    return d;
+}
+
+ClothBatch::~ClothBatch()
+{
+    release_graph();
+    dev_free(d_descs);
+    dev_free(d_cs);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void ClothBatch::release_graph()
+{
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    graphKey.clear();
+}
+
+void ClothBatch::add(ClothPatch* p)
+{
+    for (ClothPatch* q : patches)
+        if (q == p) throw std::runtime_error("patch already in the batch");
+    patches.push_back(p);
+    release_graph();
+}
+
+void ClothBatch::set_launch(int bs, int st)
+{
+    blockSize = bs;
+    stage = st;
+    tuned = bs != 0 && st >= 0;
+    trials.clear();
+    trialConfigs.clear();
+    trialMs.clear();
+}
+
+// One step of every cloth: `NumSteps` substeps, each one launch of the
+// batch kernel. With blockSize 0 the first steps time the candidate
+// configurations (every configuration computes identical results: each
+// colour's constraints touch disjoint vertices) and keep the fastest.
+void ClothBatch::step(float dt)
+{
+    if (patches.empty()) return;
+    const int n = patches[0]->mNumSteps;
+    for (ClothPatch* p : patches)
+        if (p->mNumSteps != n) throw std::runtime_error("all patches of a batch need the same number of substeps");
+    if (n < 1) throw std::runtime_error("NumSteps must be >= 1");
+    for (ClothPatch* p : patches) p->mModel->validate_batch();
+    const float h = dt / n;
+    if (!stream) HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    if (stageLimit == 0) {
+        int dev = 0, bytes = 0;
+        HIP_OK(hipGetDevice(&dev));
+        HIP_OK(hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        stageLimit = std::max(1, (bytes - 1024) / (int)sizeof(float4));
+        batch_allow_lds(stageLimit * (int)sizeof(float4));
+    }
+    const int m = (int)patches.size();
+    // largest cloths first: workgroups start in launch order
+    std::vector<int> order(m);
+    for (int i = 0; i < m; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const ClothModel& A = *patches[a]->mModel;
+        const ClothModel& B = *patches[b]->mModel;
+        return (long)A.pos.size() * A.mNumIterations > (long)B.pos.size() * B.mNumIterations;
+    });
+    std::vector<CollSet> cs(m);
+    std::vector<int> any(m);
+    for (int k = 0; k < m; k++) {
+        int slots = 0;
+        cs[k] = patches[order[k]]->mModel->prepare_batch(slots);
+        any[k] = slots > 0;
+    }
+    if (m > cap) {
+        dev_alloc(d_descs, m);
+        dev_alloc(d_cs, m);
+        cap = m;
+        release_graph();
+    }
+    // the preparation's copies run on the null stream and the models' streams
+    HIP_OK(hipDeviceSynchronize());
+    std::vector<ClothDesc> descs(m);
+    for (int k = 0; k < m; k++) descs[k] = patches[order[k]]->mModel->batch_desc(h, d_cs + k, any[k] != 0);
+    if (hostCs.size() != cs.size() || memcmp(hostCs.data(), cs.data(), m * sizeof(CollSet)) != 0) {
+        HIP_OK(hipMemcpyAsync(d_cs, cs.data(), m * sizeof(CollSet), hipMemcpyHostToDevice, stream));
+        hostCs = cs;
+    }
+    if (hostDescs.size() != descs.size() || memcmp(hostDescs.data(), descs.data(), m * sizeof(ClothDesc)) != 0) {
+        HIP_OK(hipMemcpyAsync(d_descs, descs.data(), m * sizeof(ClothDesc), hipMemcpyHostToDevice, stream));
+        hostDescs = descs;
+        release_graph();
+    }
+    HIP_OK(hipStreamSynchronize(stream));
+
+    int bs = blockSize, st = stage;
+    bool timing = false;
+    if (!tuned && (blockSize == 0 || stage < 0)) {
+        if (trials.empty() && trialMs.empty()) {
+            for (int b : {256, 512, 1024})
+                for (int g : {0, 1})
+                    if ((blockSize == 0 || b == blockSize) && (stage < 0 || g == stage)) trials.push_back({b, g});
+        }
+        const size_t t = trialMs.size();
+        bs = trials[t].first;
+        st = trials[t].second;
+        timing = true;
+    } else if (tuned) {
+        bs = blockSize;
+        st = stage;
+    }
+    // LDS sized for the largest cloth that fits (every workgroup allocates it)
+    int stageVerts = 0;
+    if (st)
+        for (ClothPatch* p : patches) {
+            const int nv = (int)p->mModel->pos.size();
+            if (nv <= stageLimit) stageVerts = std::max(stageVerts, nv);
+        }
+    std::string key = std::to_string(bs) + "/" + std::to_string(stageVerts) + "/" + std::to_string(n) + "/" +
+                      std::to_string(m) + "/" + std::to_string((uintptr_t)d_descs);
+    if (!exec || key != graphKey) {
+        release_graph();
+        HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < n; i++) launch_batch_substep(stream, d_descs, m, bs, stageVerts);
+        HIP_OK(hipStreamEndCapture(stream, &graph));
+        HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        graphKey = key;
+    }
+    if (!ev0) {
+        HIP_OK(hipEventCreate(&ev0));
+        HIP_OK(hipEventCreate(&ev1));
+    }
+    HIP_OK(hipEventRecord(ev0, stream));
+    HIP_OK(hipGraphLaunch(exec, stream));
+    HIP_OK(hipEventRecord(ev1, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+    HIP_OK(hipGetLastError());
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));
+    lastStepMs = ms;
+    lastSubsteps = n;
+    if (timing) {
+        trialConfigs.push_back({bs, st});
+        trialMs.push_back(ms);
+        if (trialMs.size() == trials.size()) {
+            size_t best = 0;
+            for (size_t t = 1; t < trialMs.size(); t++)
+                if (trialMs[t] < trialMs[best]) best = t;
+            blockSize = trials[best].first;
+            stage = trials[best].second;
+            tuned = true;
+        }
+    }
+    for (ClothPatch* p : patches) p->mModel->after_device_step();
 }
 
 }  // namespace sc

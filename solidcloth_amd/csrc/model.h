@@ -165,6 +165,7 @@ public:
     float4 *d_tA = nullptr, *d_tB = nullptr;
     float2* d_tC = nullptr;
     int4* d_bidx = nullptr;
+    int *d_triColOff = nullptr, *d_bendColOff = nullptr;  // schedule phase offsets (colours in colour mode)
     int* d_cCount = nullptr;
     float4 *d_cP = nullptr, *d_cN = nullptr;
     int* d_cSrc = nullptr;
@@ -283,6 +284,14 @@ private:
     void ensure_feature_capacity(int edgeSlots, int triSlots);
     void ensure_self();
     SelfParams self_params() const;
+
+public:
+    // batch support: validate and upload everything a substep needs, then
+    // describe the substep for the batch kernel (colour schedule required)
+    void validate_batch() const;
+    CollSet prepare_batch(int& slots);
+    ClothDesc batch_desc(float h, const CollSet* d_cs, bool anyContacts) const;
+    void after_device_step() { mDeviceNewer = true; mStatsPending = true; contactsValid = false; }
     CollSet make_collset(int& slots, int& edgeSlots, int& triSlots);
     FeatureContactBuf feature_buf(GpuSolver::Features& f) const;
     void record_substep(float h, const CollSet& cs, bool timed);
@@ -303,6 +312,41 @@ public:
     int mNumSteps = 1;
     float mIncline = 0.f;
     f3 mPosition{0.f, 0.f, 0.f};
+};
+
+// Many independent cloths stepped together: one launch per substep, one
+// workgroup per cloth (largest first), each patch's own device buffers.
+class ClothBatch {
+public:
+    ~ClothBatch();
+    std::vector<ClothPatch*> patches;
+    int blockSize = 0;  // 0: autotune over the first steps
+    int stage = -1;     // -1: autotune; 0: positions in HBM; 1: in LDS when they fit
+    bool tuned = false;
+    double lastStepMs = 0.0;  // device time of the last step's substep launches
+    int lastSubsteps = 0;
+    void add(ClothPatch* p);
+    void step(float dt);
+    void set_launch(int bs, int st);
+
+private:
+    hipStream_t stream = nullptr;
+    ClothDesc* d_descs = nullptr;
+    CollSet* d_cs = nullptr;
+    int cap = 0;
+    std::vector<ClothDesc> hostDescs;
+    std::vector<CollSet> hostCs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string graphKey;
+    int stageLimit = 0;
+    std::vector<std::pair<int, int>> trials;  // (block size, stage) candidates
+public:
+    std::vector<std::pair<int, int>> trialConfigs;  // timed so far, with trialMs
+    std::vector<double> trialMs;
+private:
+    void release_graph();
 };
 
 void set_error(const std::string& msg);

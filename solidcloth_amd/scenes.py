@@ -23,6 +23,12 @@ grid (Physics/ClothPatch.cpp:181), which lies in the y = offset.y plane.
             the ground: the self-collision path at oracle size.
   meshsmall 24x24 grid over an icosphere of subdivision 2, the mesh
             contact path at a size the exhaustive oracle runs in seconds.
+  batch512  512 independent cloths (batch_scenes): grid side uniform
+            integer in [16, 128], Young log-uniform [1e3, 1e5], Poisson
+            uniform [0.1, 0.45], bending log-uniform [1e-3, 1e-1], 1-4
+            random pinned boundary vertices, each over its own sphere of
+            radius uniform [0.2, 0.5] given as a 32^3-cell grid SDF;
+            side 1 m at 0.6 m, 10 iterations, 4 substeps, 100 frames.
 """
 from dataclasses import dataclass, field, replace
 
@@ -74,6 +80,8 @@ class Scene:
     # continuous self-collision thresholds (volume, bisection, vertex-triangle
     # distance); None = the reference's constants
     ccd: tuple = None
+    # vertices pinned after Init (inverse mass 0); None = Init's own pins
+    pins: tuple = None
     extra: dict = field(default_factory=dict)
 
     @property
@@ -125,6 +133,43 @@ MESHSMALL = Scene(
     walls_min=(-10.0, -0.3, -10.0), walls_max=(10.0, 10.0, 10.0),
     mesh_subdiv=2, mesh_radius=0.25, perturb_amplitude=0.01, perturb_seed=1,
 )
+
+
+def batch_scenes(count=512, seed=4, n_range=(16, 128), frames=100):
+    """The batched workload's cloths, drawn from one seeded generator."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        young = float(10.0 ** rng.uniform(3.0, 5.0))
+        poisson = float(rng.uniform(0.1, 0.45))
+        bend = float(10.0 ** rng.uniform(-3.0, -1.0))
+        radius = float(rng.uniform(0.2, 0.5))
+        border = [j * n + i for j in range(n) for i in range(n) if i in (0, n - 1) or j in (0, n - 1)]
+        want = int(rng.integers(1, 5))
+        chosen = []
+        # no two pins in one triangle: the reference's FEM projection divides
+        # by zero when a triangle has a single free vertex in the right place
+        # (ClothPBD.cpp:52-108), so pins keep a grid distance of at least 2
+        while len(chosen) < want:
+            v = int(border[int(rng.integers(len(border)))])
+            if all(max(abs(v // n - u // n), abs(v % n - u % n)) >= 2 for u in chosen):
+                chosen.append(v)
+        pins = tuple(sorted(chosen))
+        out.append(Scene(
+            name=f"batch{k}", n=n, side=1.0, offset=(0.0, 0.6, 0.0), iterations=10, frames=frames,
+            young=young, poisson=poisson, bend=bend, flags=CF_VERTICES, thickness=0.002, tolerance=0.005,
+            friction=0.3, sphere_center=(0.0, 0.0, 0.0), sphere_radius=radius, sdf_cells=32,
+            sdf_half=radius + 0.1, pins=pins))
+    return out
+
+
+def batch_counts(scenes):
+    """Total vertices, triangles and bends of a list of square-grid scenes."""
+    v = sum(s.n * s.n for s in scenes)
+    t = sum(2 * (s.n - 1) ** 2 for s in scenes)
+    b = sum(3 * (s.n - 1) ** 2 - 2 * (s.n - 1) for s in scenes)
+    return v, t, b
 
 
 def icosphere(subdiv, radius=1.0, center=(0.0, 0.0, 0.0)):
@@ -278,4 +323,8 @@ def build_patch(scene, sc=None):
         m.set_positions(folded_positions(scene, m.positions()))
     if scene.ccd is not None:
         m.SetSelfCCD(*scene.ccd)
+    if scene.pins is not None:
+        im = np.ones(scene.num_vertices, dtype=np.float32)
+        im[list(scene.pins)] = 0.0
+        m.set_inv_masses(im)
     return world, patch

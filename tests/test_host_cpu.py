@@ -244,3 +244,57 @@ def test_self_collision_parameters_validated(sc):
         m.SetSelfCapacity(-5)
     keys, ids, data = m.self_contacts("vt")  # before any step: none
     assert len(keys) == 0
+
+
+def test_batch_scenes_follow_the_workload_description():
+    scenes = S.batch_scenes(512)
+    assert len(scenes) == 512
+    ns = np.array([s.n for s in scenes])
+    assert ns.min() >= 16 and ns.max() <= 128
+    young = np.log10([s.young for s in scenes])
+    assert young.min() >= 3 and young.max() <= 5
+    assert all(0.1 <= s.poisson <= 0.45 for s in scenes)
+    assert all(1e-3 <= s.bend <= 1e-1 for s in scenes)
+    assert all(0.2 <= s.sphere_radius <= 0.5 and s.sdf_cells == 32 for s in scenes)
+    for s in scenes:
+        assert 1 <= len(s.pins) <= 4 and len(set(s.pins)) == len(s.pins)
+        for v in s.pins:
+            j, i = divmod(v, s.n)
+            assert i in (0, s.n - 1) or j in (0, s.n - 1)
+            for u in s.pins:
+                if u != v:
+                    assert max(abs(j - u // s.n), abs(i - u % s.n)) >= 2
+    # seeded: the same list every time
+    assert S.batch_scenes(8) == S.batch_scenes(8)
+
+
+def test_batch_pins_and_validation(sc):
+    """Pins become zero inverse masses; a batch refuses, before touching the
+    device, patches it cannot step (self collision, mismatched substeps)."""
+    scenes = S.batch_scenes(2, n_range=(16, 20))
+    built = [S.build_patch(s, sc) for s in scenes]
+    m = built[0][1].GetModel()
+    im = m.inv_masses()
+    assert set(np.flatnonzero(im == 0).tolist()) == set(scenes[0].pins)
+    with pytest.raises(sc.SolidClothError):
+        m.set_inv_masses(-np.ones(m.GetNumParticles()))
+    b = sc.ClothBatch([p for _, p in built])
+    assert b.GetNumPatches() == 2
+    with pytest.raises(sc.SolidClothError):
+        b.Add(built[0][1])  # already in the batch
+    with pytest.raises(sc.SolidClothError):
+        b.SetLaunch(300, 0)
+    b.SetLaunch(512, 1)
+    assert b.GetLaunch() == (512, 1, True)
+    built[1][1].SetNumSteps(3)
+    with pytest.raises(sc.SolidClothError, match="same number of substeps"):
+        b.Step(1.0 / 60)
+    built[1][1].SetNumSteps(4)
+    world, patch = S.build_patch(S.FOLD24, sc)
+    b.Add(patch)
+    with pytest.raises(sc.SolidClothError, match="vertex and wall contacts"):
+        b.Step(1.0 / 60)
+    empty = sc.ClothPatch(sc.CollisionWorld())
+    b2 = sc.ClothBatch([empty])
+    with pytest.raises(sc.SolidClothError, match="no cloth"):
+        b2.Step(1.0 / 60)
