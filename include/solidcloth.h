@@ -45,6 +45,7 @@ typedef struct { void* ptr; } scMesh;
 typedef struct { void* ptr; } scModel;
 typedef struct { void* ptr; } scSDF; /* replaces Geometry::SDF* (Demo/ClothAPI.h:80) */
 typedef struct { void* ptr; } scClothBatch; /* many patches stepped together (no counterpart) */
+typedef struct { void* ptr; } scWarmStart;  /* learned warm-start network (no counterpart) */
 
 enum scCollFlags { /* Demo/ClothAPI.h:34-41 */
     scCollisionFlagVertices = 1,
@@ -297,6 +298,68 @@ SCEXP void scBatchSetLaunch(scClothBatch batch, int blockSize, int stage);
 SCEXP void scBatchGetLaunch(scClothBatch batch, int* blockSize, int* stage, int* tuned);
 SCEXP int scBatchGetTuning(scClothBatch batch, int* configs, double* ms, int cap);
 SCEXP double scBatchGetStepTime(scClothBatch batch, int* launches);
+
+/* Learned warm start (no counterpart in the reference). A per-vertex MLP
+ * (12 one-ring features -> 64 -> 64 -> 3, GELU, BF16 MFMA with FP32
+ * accumulation, FP32 Adam masters) predicts the correction the Gauss-Seidel
+ * sweeps of a substep will apply (ClothPBD.cpp:324-369); the prediction is
+ * added to the inertial positions before the sweeps.
+ * scCreateWarmStart: seeded Glorot-uniform weights, zero biases.
+ * scWarmStartGetParams / SetParams: the 5187 FP32 parameters (W1 [12][64],
+ * b1, W2 [64][64], b2, W3 [64][3], b3); setting them restarts Adam.
+ * Returns the parameter count. scWarmStartSetLearningRate: Adam's step size
+ * (default 1e-3). scWarmStartGetSteps: Adam steps taken.
+ * Host-array entry points (features n x 12, targets n x 3, weights n or
+ * NULL for all ones): scWarmStartForward writes y (n x 3);
+ * scWarmStartGradient writes the gradient of the weighted mean squared error
+ * sum_v w_v |y_v - t_v|^2 / sum_v w_v and returns that loss;
+ * scWarmStartTrain takes `steps` Adam steps on the data and returns the
+ * loss before the last one.
+ * scSetWarmStart: mode 0 off, 1 train (the plain solve runs; after every
+ * substep the network takes one Adam step towards the correction the sweeps
+ * applied), 2 solve (prediction, then `iterations` sweeps), 3 track (the
+ * plain solve; only the corrections are recorded, so that a later switch to
+ * mode 2 starts with current features). The network must
+ * outlive the attachment; models sharing a network step one at a time.
+ * In mode 2 each scStepCloth is judged by the plain solver: the step's
+ * residual (scGetResidual) is compared with the threshold
+ * (scSetWarmStartFallback; < 0, the default, means the last plain step's
+ * residual, and the first step runs plain to set it); above it, the step is
+ * rerun plain from its starting state.
+ * scGetResidual: RMS constraint residual of the current positions, FP64:
+ * sqrt((sum_tri warp^2 + weft^2 + sum_bend (s theta)^2) / (2T + B)) with
+ * warp, weft as in ClothPBD.cpp:78,89 and s theta the bend error of
+ * ClothPBD.cpp:179-224. The compliant shear constraint (ClothPBD.cpp:96-101),
+ * whose error after a substep is set by its compliance, is left out;
+ * scGetResidualTerms writes the RMS of warp/weft, of shear and of the bend
+ * errors separately.
+ * scGetWarmStartStats: out[0] last step's residual, [1] threshold in use,
+ * [2] fallbacks, [3] calibrations, [4] judged steps, [5] 1 if the last step
+ * ran plain.
+ * scGetWarmStartData: which 0 features (n x 12) of the last substep, 1 its
+ * inertial positions (n x 3), 2 the last correction (n x 3), 3 training
+ * targets and weights (n x 4), 4 (ell, free vertices); returns n (2 for 4).
+ * scSaveState / scRestoreState: one device-side snapshot of positions,
+ * velocities and the warm start's last correction. */
+SCEXP scWarmStart scCreateWarmStart(unsigned int seed);
+SCEXP void scFreeWarmStart(scWarmStart ws);
+SCEXP int scWarmStartGetParams(scWarmStart ws, float* out, int count);
+SCEXP int scWarmStartSetParams(scWarmStart ws, const float* in, int count);
+SCEXP void scWarmStartSetLearningRate(scWarmStart ws, float lr);
+SCEXP int scWarmStartGetSteps(scWarmStart ws);
+SCEXP int scWarmStartForward(scWarmStart ws, const float* features, int n, float* out);
+SCEXP float scWarmStartGradient(scWarmStart ws, const float* features, const float* targets, const float* weights,
+                                int n, float* grad);
+SCEXP float scWarmStartTrain(scWarmStart ws, const float* features, const float* targets, const float* weights,
+                             int n, int steps);
+SCEXP void scSetWarmStart(scModel model, scWarmStart ws, int mode, int iterations);
+SCEXP void scSetWarmStartFallback(scModel model, float threshold);
+SCEXP double scGetResidual(scModel model);
+SCEXP void scGetResidualTerms(scModel model, double* out);
+SCEXP void scGetWarmStartStats(scModel model, double* out);
+SCEXP int scGetWarmStartData(scModel model, int which, float* out, int count);
+SCEXP void scSaveState(scModel model);
+SCEXP void scRestoreState(scModel model);
 
 /* Last error message of the calling thread, or "" (not in the reference),
  * and the number of errors recorded on the calling thread so far, so a

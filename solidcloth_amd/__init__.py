@@ -111,6 +111,15 @@ def _sig(L):
         "scGetSelfContacts": (i, [H, i, vp, vp, vp, i]), "scSetSelfParams": (None, [H, f, f]),
         "scSetSelfCCD": (None, [H, f, f, f]), "scSetSelfCapacity": (None, [H, i]),
         "scSetParticleInvMasses": (i, [H, vp, i]),
+        "scCreateWarmStart": (H, [C.c_uint]), "scFreeWarmStart": (None, [H]),
+        "scWarmStartGetParams": (i, [H, vp, i]), "scWarmStartSetParams": (i, [H, vp, i]),
+        "scWarmStartSetLearningRate": (None, [H, f]), "scWarmStartGetSteps": (i, [H]),
+        "scWarmStartForward": (i, [H, vp, i, vp]), "scWarmStartGradient": (f, [H, vp, vp, vp, i, vp]),
+        "scWarmStartTrain": (f, [H, vp, vp, vp, i, i]), "scSetWarmStart": (None, [H, H, i, i]),
+        "scSetWarmStartFallback": (None, [H, f]), "scGetResidual": (C.c_double, [H]),
+        "scGetResidualTerms": (None, [H, vp]),
+        "scGetWarmStartStats": (None, [H, vp]), "scGetWarmStartData": (i, [H, i, vp, i]),
+        "scSaveState": (None, [H]), "scRestoreState": (None, [H]),
         "scCreateClothBatch": (H, []), "scFreeClothBatch": (None, [H]), "scBatchAddPatch": (None, [H, H]),
         "scBatchGetNumPatches": (i, [H]), "scStepClothBatch": (None, [H, f]),
         "scBatchSetLaunch": (None, [H, i, i]),
@@ -308,6 +317,51 @@ class ClothModel:
 
     def inv_masses(self):
         return np.array([self.GetParticleInvMass(i) for i in range(self.GetNumParticles())], dtype=np.float32)
+
+    # learned warm start
+    WARM_MODES = {"off": 0, "train": 1, "solve": 2, "track": 3}
+
+    def SetWarmStart(self, ws, mode="solve", iterations=4):
+        m = self.WARM_MODES[mode] if isinstance(mode, str) else int(mode)
+        self._warm = ws  # keep the network alive while attached
+        _checked("scSetWarmStart", self.h, ws.h if ws is not None else _Handle(None), m, int(iterations))
+
+    def SetWarmStartFallback(self, threshold=-1.0):
+        _checked("scSetWarmStartFallback", self.h, float(threshold))
+
+    def residual(self):
+        return _checked("scGetResidual", self.h)
+
+    def residual_terms(self):
+        """RMS of (warp/weft, shear, bend error)."""
+        out = np.zeros(3)
+        _checked("scGetResidualTerms", self.h, _ptr(out))
+        return out
+
+    def warm_stats(self):
+        out = np.zeros(6)
+        _checked("scGetWarmStartStats", self.h, _ptr(out))
+        return dict(residual=out[0], threshold=out[1], fallbacks=int(out[2]), calibrations=int(out[3]),
+                    steps=int(out[4]), last_plain=bool(out[5]))
+
+    def warm_data(self, which):
+        """'features' (n,12), 'xhat' (n,3), 'corr' (n,3), 'targets' (n,4), 'scale' (ell, free count)."""
+        k = {"features": 0, "xhat": 1, "corr": 2, "targets": 3, "scale": 4}[which]
+        if k == 4:
+            out = np.zeros(2, dtype=np.float32)
+            _checked("scGetWarmStartData", self.h, 4, _ptr(out), 2)
+            return float(out[0]), int(out[1])
+        n = self.GetNumParticles()
+        w = {0: 12, 1: 3, 2: 3, 3: 4}[k]
+        out = np.zeros((n, w), dtype=np.float32)
+        _checked("scGetWarmStartData", self.h, k, _ptr(out), n)
+        return out
+
+    def save_state(self):
+        _checked("scSaveState", self.h)
+
+    def restore_state(self):
+        _checked("scRestoreState", self.h)
 
     def set_inv_masses(self, m):
         a = np.ascontiguousarray(m, dtype=np.float32)
@@ -515,3 +569,55 @@ class ClothBatch:
         ms = np.zeros(n, dtype=np.float64)
         _checked("scBatchGetTuning", self.h, _ptr(cfg), _ptr(ms), n)
         return [(int(c[0]), int(c[1]), float(t)) for c, t in zip(cfg, ms)]
+
+
+class WarmStart:
+    """The learned warm-start network (scWarmStart): 12-64-64-3 MLP."""
+
+    NUM_PARAMS = 5187
+
+    def __init__(self, seed=0):
+        self.h = _checked("scCreateWarmStart", int(seed))
+
+    def __del__(self):
+        try:
+            lib().scFreeWarmStart(self.h)
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(self.NUM_PARAMS, dtype=np.float32)
+        _checked("scWarmStartGetParams", self.h, _ptr(out), out.size)
+        return out
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32).ravel()
+        _checked("scWarmStartSetParams", self.h, _ptr(a), a.size)
+
+    def SetLearningRate(self, lr):
+        _checked("scWarmStartSetLearningRate", self.h, float(lr))
+
+    def steps(self):
+        return _checked("scWarmStartGetSteps", self.h)
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.zeros((X.shape[0], 3), dtype=np.float32)
+        _checked("scWarmStartForward", self.h, _ptr(X), X.shape[0], _ptr(out))
+        return out
+
+    def _tw(self, X, T, w):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        w = None if w is None else np.ascontiguousarray(w, dtype=np.float32)
+        return X, T, w, (None if w is None else _ptr(w))
+
+    def gradient(self, X, T, w=None):
+        X, T, w, wp = self._tw(X, T, w)
+        g = np.zeros(self.NUM_PARAMS, dtype=np.float32)
+        loss = _checked("scWarmStartGradient", self.h, _ptr(X), _ptr(T), wp, X.shape[0], _ptr(g))
+        return loss, g
+
+    def train(self, X, T, w=None, steps=1):
+        X, T, w, wp = self._tw(X, T, w)
+        return _checked("scWarmStartTrain", self.h, _ptr(X), _ptr(T), wp, X.shape[0], int(steps))

@@ -2,6 +2,8 @@
 // reference's Demo/ClothAPI.cpp (cited per function) over the MI355X-native
 // objects of model.h. Exceptions never cross the boundary: they become the
 // thread's last error and the call returns a neutral value.
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <stdexcept>
 
@@ -622,6 +624,118 @@ int scBatchGetTuning(scClothBatch batch, int* configs, double* ms, int cap)
         return n;
     });
 }
+
+static WarmStart* WS(scWarmStart w)
+{
+    if (!w.ptr) throw std::runtime_error("null warm-start network");
+    return (WarmStart*)w.ptr;
+}
+
+scWarmStart scCreateWarmStart(unsigned int seed)
+{
+    scWarmStart w{nullptr};
+    guard([&] { w.ptr = new WarmStart(seed); });
+    return w;
+}
+
+void scFreeWarmStart(scWarmStart ws) { guard([&] { delete (WarmStart*)ws.ptr; }); }
+
+int scWarmStartGetParams(scWarmStart ws, float* out, int count)
+{
+    return guard_r<int>(-1, [&] {
+        const std::vector<float> p = WS(ws)->params();
+        if (out) memcpy(out, p.data(), (size_t)std::min(count, (int)p.size()) * sizeof(float));
+        return (int)p.size();
+    });
+}
+
+int scWarmStartSetParams(scWarmStart ws, const float* in, int count)
+{
+    return guard_r<int>(-1, [&] {
+        if (count != kMlpParams || !in) throw std::runtime_error("the warm-start network has 5187 parameters");
+        WS(ws)->set_params(std::vector<float>(in, in + count));
+        return count;
+    });
+}
+
+void scWarmStartSetLearningRate(scWarmStart ws, float lr)
+{
+    guard([&] {
+        if (!(lr > 0.f) || !std::isfinite(lr)) throw std::runtime_error("learning rate must be positive");
+        WS(ws)->lr = lr;
+    });
+}
+
+int scWarmStartGetSteps(scWarmStart ws) { return guard_r<int>(-1, [&] { return WS(ws)->steps(); }); }
+
+int scWarmStartForward(scWarmStart ws, const float* features, int n, float* out)
+{
+    return guard_r<int>(-1, [&] {
+        if (n <= 0 || !features || !out) throw std::runtime_error("no samples");
+        WS(ws)->forward_host(features, n, out);
+        return n;
+    });
+}
+
+float scWarmStartGradient(scWarmStart ws, const float* features, const float* targets, const float* weights, int n,
+                          float* grad)
+{
+    return guard_r<float>(-1.f, [&] {
+        if (n <= 0 || !features || !targets) throw std::runtime_error("no samples");
+        return WS(ws)->gradient_host(features, targets, weights, n, grad);
+    });
+}
+
+float scWarmStartTrain(scWarmStart ws, const float* features, const float* targets, const float* weights, int n,
+                       int steps)
+{
+    return guard_r<float>(-1.f, [&] {
+        if (n <= 0 || !features || !targets) throw std::runtime_error("no samples");
+        if (steps < 1) throw std::runtime_error("steps must be >= 1");
+        return WS(ws)->train_host(features, targets, weights, n, steps);
+    });
+}
+
+void scSetWarmStart(scModel model, scWarmStart ws, int mode, int iterations)
+{
+    guard([&] { M(model)->set_warm((WarmStart*)ws.ptr, mode, iterations); });
+}
+
+void scSetWarmStartFallback(scModel model, float threshold)
+{
+    guard([&] {
+        if (std::isnan(threshold)) throw std::runtime_error("threshold must be a number");
+        ClothModel* m = M(model);
+        m->mWarmThreshold = threshold;
+        m->mWarmTau = -1.0;
+    });
+}
+
+double scGetResidual(scModel model) { return guard_r<double>(-1.0, [&] { return M(model)->residual(); }); }
+
+void scGetResidualTerms(scModel model, double* out) { guard([&] { M(model)->residual_terms(out); }); }
+
+void scGetWarmStartStats(scModel model, double* out)
+{
+    guard([&] {
+        ClothModel* m = M(model);
+        out[0] = m->mWarmResidual;
+        out[1] = m->mWarmThreshold >= 0.f ? (double)m->mWarmThreshold : m->mWarmTau;
+        out[2] = m->mWarmFallbacks;
+        out[3] = m->mWarmCalibrations;
+        out[4] = m->mWarmSteps;
+        out[5] = m->mWarmLastPlain ? 1.0 : 0.0;
+    });
+}
+
+int scGetWarmStartData(scModel model, int which, float* out, int count)
+{
+    return guard_r<int>(-1, [&] { return M(model)->warm_data(which, out, count); });
+}
+
+void scSaveState(scModel model) { guard([&] { M(model)->save_state(); }); }
+
+void scRestoreState(scModel model) { guard([&] { M(model)->restore_state(); }); }
 
 double scBatchGetStepTime(scClothBatch batch, int* launches)
 {

@@ -249,11 +249,27 @@ KernelTimer::~KernelTimer()
 
 GpuSolver::~GpuSolver() { release(); }
 
-void GpuSolver::release()
+void GpuSolver::release_graphs()
 {
     if (graphExec) (void)hipGraphExecDestroy(graphExec);
     if (graph) (void)hipGraphDestroy(graph);
-    graphExec = nullptr; graph = nullptr; graphKey.clear();
+    if (graphExec2) (void)hipGraphExecDestroy(graphExec2);
+    if (graph2) (void)hipGraphDestroy(graph2);
+    graphExec = graphExec2 = nullptr;
+    graph = graph2 = nullptr;
+    graphKey.clear();
+    graphKey2.clear();
+}
+
+void GpuSolver::release()
+{
+    release_graphs();
+    dev_free(warm.xhat); dev_free(warm.corr); dev_free(warm.T); dev_free(warm.triStrain); dev_free(warm.X);
+    dev_free(warm.adjOff); dev_free(warm.adj); dev_free(warm.vtOff); dev_free(warm.vt);
+    dev_free(warm.snapPos); dev_free(warm.snapPrev); dev_free(warm.snapVel); dev_free(warm.snapCorr);
+    dev_free(warm.resPartial); dev_free(warm.resOut);
+    warm.nv = -1;
+    warm.built = warm.saved = false;
     dev_free(d_pos); dev_free(d_prev); dev_free(d_vel);
     dev_free(d_tidx); dev_free(d_tA); dev_free(d_tB); dev_free(d_tC); dev_free(d_bidx);
     dev_free(d_triColOff); dev_free(d_bendColOff);
@@ -397,9 +413,8 @@ void ClothModel::build_device_topology()
 {
     GpuSolver& g = gpu;
     if (!g.stream) HIP_OK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
-    if (g.graphExec) { (void)hipGraphExecDestroy(g.graphExec); g.graphExec = nullptr; }
-    if (g.graph) { (void)hipGraphDestroy(g.graph); g.graph = nullptr; }
-    g.graphKey.clear();
+    g.release_graphs();
+    g.warm.built = false;
     g.nv = (int)pos.size();
     g.nt = (int)tris.size();
     g.nb = (int)bends.size();
@@ -554,9 +569,7 @@ void ClothModel::ensure_contact_capacity(int slots)
     dev_alloc(g.d_cSrc, (size_t)slots * g.nv);
     dev_alloc(g.d_cFeat, (size_t)slots * g.nv);
     g.slotCap = slots;
-    if (g.graphExec) { (void)hipGraphExecDestroy(g.graphExec); g.graphExec = nullptr; }
-    if (g.graph) { (void)hipGraphDestroy(g.graph); g.graph = nullptr; }
-    g.graphKey.clear();
+    g.release_graphs();
 }
 
 void ClothModel::ensure_feature_capacity(int edgeSlots, int triSlots)
@@ -573,9 +586,7 @@ void ClothModel::ensure_feature_capacity(int edgeSlots, int triSlots)
         grown = true;
     }
     if (grown) {
-        if (g.graphExec) { (void)hipGraphExecDestroy(g.graphExec); g.graphExec = nullptr; }
-        if (g.graph) { (void)hipGraphDestroy(g.graph); g.graph = nullptr; }
-        g.graphKey.clear();
+        g.release_graphs();
     }
 }
 
@@ -607,9 +618,7 @@ void ClothModel::ensure_self()
             b.maxColours = kMaxSelfColours;
         }
         S.cap = cap;
-        if (g.graphExec) { (void)hipGraphExecDestroy(g.graphExec); g.graphExec = nullptr; }
-        if (g.graph) { (void)hipGraphDestroy(g.graph); g.graph = nullptr; }
-        g.graphKey.clear();
+        g.release_graphs();
     }
     if (!S.triLo) {
         dev_alloc(S.triLo, std::max(g.nt, 1)); dev_alloc(S.triHi, std::max(g.nt, 1));
@@ -725,6 +734,11 @@ std::string ClothModel::graph_key(float h, const CollSet& cs) const
     const BvhView tv = gpu.self.triB.view(), ev = gpu.self.edgeB.view();
     put(&tv, sizeof(tv)); put(&ev, sizeof(ev));
     put(&gpu.self.buf[0].ids, sizeof(void*)); put(&gpu.self.buf[1].ids, sizeof(void*));
+    const int wmode = mWarmPlainNow ? 0 : mWarmMode;
+    const float lr = mWarm ? mWarm->lr : 0.f;
+    put(&mWarm, sizeof(void*)); put(&wmode, 4); put(&mWarmIters, 4); put(&lr, 4);
+    put(&gpu.warm.corr, sizeof(void*)); put(&gpu.warm.X, sizeof(void*)); put(&gpu.warm.triStrain, sizeof(void*));
+    put(&gpu.warm.adj, sizeof(void*)); put(&gpu.warm.vt, sizeof(void*)); put(&gpu.warm.nFree, 4);
     return k;
 }
 
@@ -816,7 +830,18 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
     const float epsUV = 2 * (1 + mPoisson) / (smax(mShearStiff, 0.1f) * mUnit * mUnit);
     const float sStiff = 0.0005f * mBendStiff;
     TriStreams tr{g.d_tidx, g.d_tA, g.d_tB, g.d_tC};
-    for (int k = 0; k < mNumIterations; k++) {
+    // learned warm start: features at the inertial positions, then (solve
+    // mode) the predicted correction added before the sweeps
+    const bool track = mWarm && mWarmMode != 0 && g.warm.built;
+    const int wmode = mWarmPlainNow ? 0 : mWarmMode;
+    const WarmBufs wb = warm_bufs();
+    if (track) {
+        launch_ws_tri_strain(s, g.d_pos, tr, g.nt, g.warm.triStrain);
+        launch_ws_features(s, g.d_pos, g.d_prev, wb, g.nv, 1.f / g.warm.ell);
+        if (wmode == 2) launch_mlp_apply(s, mWarm->weights(), g.warm.X, g.nv, g.d_pos, g.warm.ell / kWsScaleOut);
+    }
+    const int iterations = wmode == 2 ? mWarmIters : mNumIterations;
+    for (int k = 0; k < iterations; k++) {
         if (mFEM) {
             for (int p = 0; p < triCol.numPhases(); p++) {
                 timed_begin(g.timer, timed, 0, s, ev);
@@ -869,7 +894,7 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
             timed_end(g.timer, timed, 2, s, ev);
         }
         if (!mFixedIterations)
-            launch_iter_end(s, g.d_ctl, mNumIterations, 0, mBendFreq, mDihedral ? 1 : 0, mSelfCollThreshold);
+            launch_iter_end(s, g.d_ctl, iterations, 0, mBendFreq, mDihedral ? 1 : 0, mSelfCollThreshold);
     }
     timed_begin(g.timer, timed, 4, s, ev);
     if (fuse)
@@ -877,6 +902,48 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
     else
         launch_velocity(s, g.d_pos, g.d_prev, g.d_vel, g.nv, invH, g.d_ctl);
     timed_end(g.timer, timed, 4, s, ev);
+    if (track) {
+        // the correction of this substep; in training mode, a step of Adam on it
+        launch_ws_after(s, g.d_pos, wb, g.nv, wmode == 1, kWsScaleOut / g.warm.ell);
+        if (wmode == 1 && g.warm.nFree > 0)
+            mWarm->enqueue_train(s, g.warm.X, g.warm.T, g.nv, 1.f / (float)g.warm.nFree);
+    }
+}
+
+// n substeps through one of two cached graphs (keyed by everything a
+// recorded substep depends on), or by direct launches when timing.
+void ClothModel::run_substeps(float h, int n, const CollSet& cs, bool timed)
+{
+    GpuSolver& g = gpu;
+    if (mUseGraph && !timed) {
+        const std::string key = graph_key(h, cs);
+        hipGraphExec_t exec = nullptr;
+        if (g.graphExec && key == g.graphKey) {
+            exec = g.graphExec;
+        } else if (g.graphExec2 && key == g.graphKey2) {
+            exec = g.graphExec2;
+        } else {
+            // the newer graph moves to the second slot; the new one takes the first
+            if (g.graphExec2) (void)hipGraphExecDestroy(g.graphExec2);
+            if (g.graph2) (void)hipGraphDestroy(g.graph2);
+            g.graphExec2 = g.graphExec;
+            g.graph2 = g.graph;
+            g.graphKey2 = g.graphKey;
+            g.graphExec = nullptr;
+            g.graph = nullptr;
+            HIP_OK(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+            record_substep(h, cs, false);
+            HIP_OK(hipStreamEndCapture(g.stream, &g.graph));
+            HIP_OK(hipGraphInstantiate(&g.graphExec, g.graph, nullptr, nullptr, 0));
+            g.graphKey = key;
+            exec = g.graphExec;
+        }
+        for (int i = 0; i < n; i++) HIP_OK(hipGraphLaunch(exec, g.stream));
+    } else {
+        for (int i = 0; i < n; i++) record_substep(h, cs, timed);
+    }
+    HIP_OK(hipStreamSynchronize(g.stream));
+    HIP_OK(hipGetLastError());
 }
 
 void ClothModel::step_substeps(float h, int n)
@@ -911,23 +978,32 @@ void ClothModel::step_substeps(float h, int n)
     }
     GpuSolver& g = gpu;
     const bool timed = g.timer.enabled;
-    if (mUseGraph && !timed) {
-        std::string key = graph_key(h, cs);
-        if (!g.graphExec || key != g.graphKey) {
-            if (g.graphExec) { (void)hipGraphExecDestroy(g.graphExec); g.graphExec = nullptr; }
-            if (g.graph) { (void)hipGraphDestroy(g.graph); g.graph = nullptr; }
-            HIP_OK(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
-            record_substep(h, cs, false);
-            HIP_OK(hipStreamEndCapture(g.stream, &g.graph));
-            HIP_OK(hipGraphInstantiate(&g.graphExec, g.graph, nullptr, nullptr, 0));
-            g.graphKey = key;
+    if (mWarm && mWarmMode != 0) ensure_warm();
+    // the plain solver judges a warm-started step: a step whose residual
+    // exceeds the last plain step's (or the set threshold) is rerun plain
+    // from the saved state, and the first step calibrates
+    const bool judge = mWarm && mWarmMode == 2;
+    if (judge) save_state();
+    mWarmPlainNow = false;
+    run_substeps(h, n, cs, timed);
+    mWarmLastPlain = !judge;
+    if (judge) {
+        const double r = residual();
+        mWarmSteps++;
+        mWarmResidual = r;
+        const double tau = mWarmThreshold >= 0.f ? (double)mWarmThreshold : mWarmTau;
+        if (tau < 0.0 || !(r <= tau)) {
+            if (tau < 0.0) mWarmCalibrations++; else mWarmFallbacks++;
+            restore_state();
+            mWarmPlainNow = true;
+            run_substeps(h, n, cs, timed);
+            mWarmPlainNow = false;
+            mWarmLastPlain = true;
+            const double rp = residual();
+            if (mWarmThreshold < 0.f) mWarmTau = rp;
+            mWarmResidual = rp;
         }
-        for (int i = 0; i < n; i++) HIP_OK(hipGraphLaunch(g.graphExec, g.stream));
-    } else {
-        for (int i = 0; i < n; i++) record_substep(h, cs, timed);
     }
-    HIP_OK(hipStreamSynchronize(g.stream));
-    HIP_OK(hipGetLastError());
     if (mCollFlags & CF_SELF) {
         SolverCtl c;
         HIP_OK(hipMemcpy(&c, g.d_ctl, sizeof(c), hipMemcpyDeviceToHost));
@@ -951,7 +1027,7 @@ void ClothModel::fetch_stats()
     SolverCtl c;
     HIP_OK(hipMemcpy(&c, gpu.d_ctl, sizeof(c), hipMemcpyDeviceToHost));
     if (mFixedIterations) {
-        statIterations = mNumIterations;
+        statIterations = (mWarm && mWarmMode == 2 && !mWarmLastPlain) ? mWarmIters : mNumIterations;
         statBendError = __builtin_bit_cast(float, c.cb);
         statContactError = __builtin_bit_cast(float, c.cv);
     } else {

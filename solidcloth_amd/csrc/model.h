@@ -14,6 +14,7 @@
 #include "kernels.h"
 #include "lbvh.h"
 #include "schedule.h"
+#include "warmstart.h"
 
 namespace sc {
 
@@ -131,6 +132,40 @@ struct FeatureContactOut {
 
 class ClothPatch;
 
+// The learned warm-start network (12-64-64-3 MLP) and its optimiser state on
+// the device. Shared by the models it is attached to, which must step one
+// after another (the training kernels use one set of gradient buffers).
+class WarmStart {
+public:
+    explicit WarmStart(unsigned seed);
+    ~WarmStart();
+    static std::vector<float> initial_params(unsigned seed);
+    std::vector<float> params() const;
+    void set_params(const std::vector<float>& p);
+    int steps() const;
+    float last_loss() const;  // loss sum of the last gradient (times the weight normaliser)
+    MlpWeights weights() const { return MlpWeights{d_master, d_packed}; }
+    void enqueue_gradient(hipStream_t s, const uint16_t* X, const float4* T, int n, float invCount);
+    void enqueue_train(hipStream_t s, const uint16_t* X, const float4* T, int n, float invCount);
+    void forward_host(const float* X, int n, float* Y);
+    float gradient_host(const float* X, const float* T, const float* w, int n, float* grad);
+    float train_host(const float* X, const float* T, const float* w, int n, int count);
+    float lr = 1e-3f;
+
+private:
+    hipStream_t stream = nullptr;
+    float *d_master = nullptr, *d_m = nullptr, *d_v = nullptr, *d_grad = nullptr, *d_partial = nullptr;
+    uint16_t* d_packed = nullptr;
+    int* d_step = nullptr;
+    // staging for the host-array entry points
+    uint16_t* d_X = nullptr;
+    float* d_Xf = nullptr;
+    float4* d_T = nullptr;
+    float* d_Y = nullptr;
+    int cap = 0;
+    void stage(const float* X, const float* T, const float* w, int n, float& invCount);
+};
+
 // Sweep schedules: 0 = one launch per colour, 1 = slab phases with LDS staging
 enum { SWEEP_COLOURS = 0, SWEEP_SLABS = 1 };
 
@@ -199,9 +234,26 @@ public:
     int slots = 0, slotCap = 0;
     int activeSlots = 0;  // vertex-contact slots the world can fill (0: no vertex contacts)
     SolverCtl* d_ctl = nullptr;
+    // warm start: inertial positions, corrections, features, targets,
+    // triangle strains, one-ring and incident-triangle lists, snapshots
+    struct Warm {
+        float4 *xhat = nullptr, *corr = nullptr, *T = nullptr, *triStrain = nullptr;
+        uint16_t* X = nullptr;
+        int *adjOff = nullptr, *adj = nullptr, *vtOff = nullptr, *vt = nullptr;
+        float4 *snapPos = nullptr, *snapPrev = nullptr, *snapVel = nullptr, *snapCorr = nullptr;
+        double *resPartial = nullptr, *resOut = nullptr;
+        int resBlocks = 0, nv = -1, nFree = 0;
+        float ell = 1.f;
+        bool built = false, saved = false;
+    } warm;
+    // two cached substep graphs (the warm-started and the plain substep)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
     std::string graphKey;
+    hipGraph_t graph2 = nullptr;
+    hipGraphExec_t graphExec2 = nullptr;
+    std::string graphKey2;
+    void release_graphs();
     KernelTimer timer;
 };
 
@@ -292,6 +344,25 @@ public:
     CollSet prepare_batch(int& slots);
     ClothDesc batch_desc(float h, const CollSet* d_cs, bool anyContacts) const;
     void after_device_step() { mDeviceNewer = true; mStatsPending = true; contactsValid = false; }
+
+    // learned warm start: mode 0 off, 1 train on the plain solve, 2 warm-started
+    // solve, 3 plain solve that only tracks the corrections (features stay current)
+    WarmStart* mWarm = nullptr;
+    int mWarmMode = 0, mWarmIters = 4;
+    float mWarmThreshold = -1.f;  // fallback residual; < 0: the last plain step's residual
+    double mWarmTau = -1.0, mWarmResidual = -1.0;
+    int mWarmFallbacks = 0, mWarmCalibrations = 0, mWarmSteps = 0;
+    bool mWarmResetCorr = true, mWarmPlainNow = false, mWarmLastPlain = false;
+    void set_warm(WarmStart* ws, int mode, int iterations);
+    void ensure_warm();
+    WarmBufs warm_bufs() const;
+    double residual();
+    void residual_terms(double out[3]);  // RMS of warp/weft, of shear, of bend errors
+    double residualSum = 0.0;
+    void save_state();
+    void restore_state();
+    int warm_data(int which, float* out, int count);
+    void run_substeps(float h, int n, const CollSet& cs, bool timed);
     CollSet make_collset(int& slots, int& edgeSlots, int& triSlots);
     FeatureContactBuf feature_buf(GpuSolver::Features& f) const;
     void record_substep(float h, const CollSet& cs, bool timed);
