@@ -188,6 +188,110 @@ def run_events(scene, frames, sweep=1):
     return timing
 
 
+LEARNED_KS = (1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20)
+
+
+def learned_vs_plain(scene, start_frames=40, window=10, judged_frames=20):
+    """Wall time to the plain 20-iteration residual, learned warm start vs
+    plain, on frames `start_frames`.. of the scene (trained parameters from
+    solidcloth_amd/data/warmstart_mlp.npy; the scene is not among the
+    training scenes).
+
+    Per frame, from one saved state: the plain frame (the scene's iteration
+    count) gives the target residual R and its device time; the warm-started
+    frame is run with k = 1, 2, ... iterations until its residual is at most
+    R; its device time (prediction included) is the learned time. A frame no
+    k reaches costs the learned attempt at k = 20 plus the plain frame (the
+    fallback). Then the same frames are stepped in the production mode
+    (4 warm-started iterations, judged by the plain solver, automatic
+    threshold) and timed against plain frames from the same start."""
+    import numpy as np
+
+    import solidcloth_amd as sc
+    from solidcloth_amd import scenes as S
+    from solidcloth_amd import warmstart_train as WT
+
+    ws = WT.load()
+    if ws is None:
+        return {"status": "no trained parameters: run python -m solidcloth_amd.warmstart_train"}
+    world, patch = S.build_patch(scene, sc)
+    m = patch.GetModel()
+    m.SetWarmStart(ws, "track")
+    for _ in range(start_frames):
+        patch.Step(scene.dt)
+    its = scene.iterations
+
+    def frame(setup):
+        m.restore_state()
+        setup()
+        patch.Step(scene.dt)
+        return m.last_step_ms(), m.residual()
+
+    t_plain, t_learn, ks, r_plain, r_warm4, r_plain4, reached = [], [], [], [], [], [], 0
+    replay_ok = True
+    for f in range(window):
+        m.save_state()
+        tp, R = frame(lambda: (m.SetWarmStart(ws, "track"), m.SetNumIterations(its)))
+        P20 = m.positions()
+        _, r4 = frame(lambda: (m.SetWarmStart(ws, "track"), m.SetNumIterations(4)))
+        m.SetNumIterations(its)
+        _, rw4 = frame(lambda: (m.SetWarmStart(ws, "solve", 4), m.SetWarmStartFallback(1e9)))
+        r_warm4.append(rw4)
+        found = None
+        for k in LEARNED_KS:
+            tk, rk = frame(lambda: (m.SetWarmStart(ws, "solve", k), m.SetWarmStartFallback(1e9)))
+            if rk <= R:
+                found = (k, tk)
+                break
+        if found is None:
+            tl = tk + tp
+        else:
+            tl = found[1]
+            reached += 1
+            ks.append(found[0])
+        t_plain.append(tp)
+        t_learn.append(tl)
+        r_plain.append(R)
+        r_plain4.append(r4)
+        # continue along the plain trajectory
+        m.restore_state()
+        m.SetWarmStart(ws, "track")
+        patch.Step(scene.dt)
+        replay_ok = replay_ok and bool(np.array_equal(m.positions(), P20))
+    # production mode against plain frames over the same span
+    m.save_state()
+    m.SetWarmStart(ws, "track")
+    plain_ms = 0.0
+    for _ in range(judged_frames):
+        patch.Step(scene.dt)
+        plain_ms += m.last_step_ms()
+    m.restore_state()
+    m.SetWarmStart(ws, "solve", 4)
+    m.SetWarmStartFallback(-1.0)  # automatic: the last plain frame's residual; the first frame calibrates
+    judged_ms = 0.0
+    for _ in range(judged_frames):
+        patch.Step(scene.dt)
+        judged_ms += m.last_step_ms()
+    st = m.warm_stats()
+    return {
+        "frames": f"{start_frames}..{start_frames + window - 1}",
+        "residual": "RMS of warp and weft errors and bend errors, FP64 (shear reported apart)",
+        "plain_iterations": its,
+        "plain_residual_mean": float(np.mean(r_plain)),
+        "plain4_residual_mean": float(np.mean(r_plain4)),
+        "warm4_residual_mean": float(np.mean(r_warm4)) if r_warm4 else None,
+        "frames_reaching_plain_residual": reached,
+        "learned_iterations_mean": float(np.mean(ks)) if ks else None,
+        "plain_ms_per_frame": float(np.mean(t_plain)),
+        "learned_ms_per_frame": float(np.mean(t_learn)),
+        "learned_over_plain_time": float(np.sum(t_learn) / np.sum(t_plain)),
+        "plain_replay_identical": replay_ok,
+        "judged_mode": {"frames": judged_frames, "iterations": 4, "plain_ms": round(plain_ms, 3),
+                        "learned_ms": round(judged_ms, 3), "ratio": round(judged_ms / plain_ms, 4),
+                        "fallbacks": st["fallbacks"], "calibrations": st["calibrations"]},
+    }
+
+
 def cpu_baseline(scene, budget_s):
     import oracle
 
@@ -217,6 +321,7 @@ def main():
                     help="0: one launch per colour; 1: slab phases staged in LDS")
     ap.add_argument("--copy-probe-mb", type=int, default=1024,
                     help="size of the streaming-copy bandwidth probe (0 disables)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
     ap.add_argument("--workload", default="drape256", choices=["drape256", "mesh512", "batch512"],
                     help="drape256: the metric's workload; mesh512: the triangle-mesh obstacle path "
                          "(reported beside it, not the metric)")
@@ -306,6 +411,8 @@ def main():
         if ms > 0:
             out["copy_probe"] = {"bytes_copied": nbytes, "avg_ms": round(ms, 4),
                                  "GB/s (read+write)": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
+    if args.workload == "drape256" and not args.no_learned:
+        out["learned_vs_plain"] = learned_vs_plain(scene)
     if args.workload != "drape256":
         out["metric"] = "PBD substeps/sec, " + args.workload
         out["cpu_baseline"] = None  # the exhaustive oracle needs minutes per substep at this size

@@ -358,6 +358,10 @@ SCEXP double scGetResidual(scModel model);
 SCEXP void scGetResidualTerms(scModel model, double* out);
 SCEXP void scGetWarmStartStats(scModel model, double* out);
 SCEXP int scGetWarmStartData(scModel model, int which, float* out, int count);
+/* Device time (ms, HIP events) of the last scStepCloth's substep graph
+ * launches, a fallback rerun included; 0 when the step launched directly
+ * (kernel timing on or graphs off). */
+SCEXP double scGetLastStepTime(scModel model);
 SCEXP void scSaveState(scModel model);
 SCEXP void scRestoreState(scModel model);
 

@@ -119,7 +119,7 @@ def _sig(L):
         "scSetWarmStartFallback": (None, [H, f]), "scGetResidual": (C.c_double, [H]),
         "scGetResidualTerms": (None, [H, vp]),
         "scGetWarmStartStats": (None, [H, vp]), "scGetWarmStartData": (i, [H, i, vp, i]),
-        "scSaveState": (None, [H]), "scRestoreState": (None, [H]),
+        "scSaveState": (None, [H]), "scRestoreState": (None, [H]), "scGetLastStepTime": (C.c_double, [H]),
         "scCreateClothBatch": (H, []), "scFreeClothBatch": (None, [H]), "scBatchAddPatch": (None, [H, H]),
         "scBatchGetNumPatches": (i, [H]), "scStepClothBatch": (None, [H, f]),
         "scBatchSetLaunch": (None, [H, i, i]),
@@ -356,6 +356,9 @@ class ClothModel:
         out = np.zeros((n, w), dtype=np.float32)
         _checked("scGetWarmStartData", self.h, k, _ptr(out), n)
         return out
+
+    def last_step_ms(self):
+        return _checked("scGetLastStepTime", self.h)
 
     def save_state(self):
         _checked("scSaveState", self.h)

@@ -662,7 +662,7 @@ void scWarmStartSetLearningRate(scWarmStart ws, float lr)
 {
     guard([&] {
         if (!(lr > 0.f) || !std::isfinite(lr)) throw std::runtime_error("learning rate must be positive");
-        WS(ws)->lr = lr;
+        WS(ws)->set_lr(lr);
     });
 }
 
@@ -732,6 +732,8 @@ int scGetWarmStartData(scModel model, int which, float* out, int count)
 {
     return guard_r<int>(-1, [&] { return M(model)->warm_data(which, out, count); });
 }
+
+double scGetLastStepTime(scModel model) { return guard_r<double>(-1.0, [&] { return M(model)->mLastStepMs; }); }
 
 void scSaveState(scModel model) { guard([&] { M(model)->save_state(); }); }
 

@@ -264,6 +264,9 @@ void GpuSolver::release_graphs()
 void GpuSolver::release()
 {
     release_graphs();
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    ev0 = ev1 = nullptr;
     dev_free(warm.xhat); dev_free(warm.corr); dev_free(warm.T); dev_free(warm.triStrain); dev_free(warm.X);
     dev_free(warm.adjOff); dev_free(warm.adj); dev_free(warm.vtOff); dev_free(warm.vt);
     dev_free(warm.snapPos); dev_free(warm.snapPrev); dev_free(warm.snapVel); dev_free(warm.snapCorr);
@@ -735,8 +738,7 @@ std::string ClothModel::graph_key(float h, const CollSet& cs) const
     put(&tv, sizeof(tv)); put(&ev, sizeof(ev));
     put(&gpu.self.buf[0].ids, sizeof(void*)); put(&gpu.self.buf[1].ids, sizeof(void*));
     const int wmode = mWarmPlainNow ? 0 : mWarmMode;
-    const float lr = mWarm ? mWarm->lr : 0.f;
-    put(&mWarm, sizeof(void*)); put(&wmode, 4); put(&mWarmIters, 4); put(&lr, 4);
+    put(&mWarm, sizeof(void*)); put(&wmode, 4); put(&mWarmIters, 4);
     put(&gpu.warm.corr, sizeof(void*)); put(&gpu.warm.X, sizeof(void*)); put(&gpu.warm.triStrain, sizeof(void*));
     put(&gpu.warm.adj, sizeof(void*)); put(&gpu.warm.vt, sizeof(void*)); put(&gpu.warm.nFree, 4);
     return k;
@@ -938,7 +940,17 @@ void ClothModel::run_substeps(float h, int n, const CollSet& cs, bool timed)
             g.graphKey = key;
             exec = g.graphExec;
         }
+        if (!g.ev0) {
+            HIP_OK(hipEventCreate(&g.ev0));
+            HIP_OK(hipEventCreate(&g.ev1));
+        }
+        HIP_OK(hipEventRecord(g.ev0, g.stream));
         for (int i = 0; i < n; i++) HIP_OK(hipGraphLaunch(exec, g.stream));
+        HIP_OK(hipEventRecord(g.ev1, g.stream));
+        HIP_OK(hipStreamSynchronize(g.stream));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+        mLastStepMs += ms;
     } else {
         for (int i = 0; i < n; i++) record_substep(h, cs, timed);
     }
@@ -983,6 +995,7 @@ void ClothModel::step_substeps(float h, int n)
     // exceeds the last plain step's (or the set threshold) is rerun plain
     // from the saved state, and the first step calibrates
     const bool judge = mWarm && mWarmMode == 2;
+    mLastStepMs = 0.0;
     if (judge) save_state();
     mWarmPlainNow = false;
     run_substeps(h, n, cs, timed);

@@ -150,9 +150,12 @@ public:
     void forward_host(const float* X, int n, float* Y);
     float gradient_host(const float* X, const float* T, const float* w, int n, float* grad);
     float train_host(const float* X, const float* T, const float* w, int n, int count);
-    float lr = 1e-3f;
+    void set_lr(float lr);
+    float lr() const { return mLr; }
 
 private:
+    float mLr = 1e-3f;
+    float* d_lr = nullptr;
     hipStream_t stream = nullptr;
     float *d_master = nullptr, *d_m = nullptr, *d_v = nullptr, *d_grad = nullptr, *d_partial = nullptr;
     uint16_t* d_packed = nullptr;
@@ -254,6 +257,7 @@ public:
     hipGraphExec_t graphExec2 = nullptr;
     std::string graphKey2;
     void release_graphs();
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around the graph launches of a step
     KernelTimer timer;
 };
 
@@ -356,6 +360,7 @@ public:
     void set_warm(WarmStart* ws, int mode, int iterations);
     void ensure_warm();
     WarmBufs warm_bufs() const;
+    double mLastStepMs = 0.0;  // device time of the last step's graph launches (warm-started and fallback)
     double residual();
     void residual_terms(double out[3]);  // RMS of warp/weft, of shear, of bend errors
     double residualSum = 0.0;

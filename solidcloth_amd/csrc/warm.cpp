@@ -50,13 +50,22 @@ WarmStart::WarmStart(unsigned seed)
     dev_alloc(d_partial, (size_t)kMlpMaxBlocks * kMlpStride);
     dev_alloc(d_packed, (size_t)kMlpOperands * 256);
     dev_alloc(d_step, 1);
+    dev_alloc(d_lr, 1);
+    set_lr(mLr);
     set_params(initial_params(seed));
+}
+
+void WarmStart::set_lr(float lr)
+{
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(d_lr, &lr, sizeof(float), hipMemcpyHostToDevice));
+    mLr = lr;
 }
 
 WarmStart::~WarmStart()
 {
     dev_free(d_master); dev_free(d_m); dev_free(d_v); dev_free(d_grad); dev_free(d_partial);
-    dev_free(d_packed); dev_free(d_step);
+    dev_free(d_packed); dev_free(d_step); dev_free(d_lr);
     dev_free(d_X); dev_free(d_Xf); dev_free(d_T); dev_free(d_Y);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -111,7 +120,7 @@ void WarmStart::enqueue_gradient(hipStream_t s, const uint16_t* X, const float4*
 void WarmStart::enqueue_train(hipStream_t s, const uint16_t* X, const float4* T, int n, float invCount)
 {
     enqueue_gradient(s, X, T, n, invCount);
-    launch_mlp_adam(s, d_master, d_m, d_v, d_grad, d_step, lr);
+    launch_mlp_adam(s, d_master, d_m, d_v, d_grad, d_step, d_lr);
     launch_mlp_pack(s, d_master, d_packed, d_step);
 }
 

@@ -78,7 +78,9 @@ int mlp_backward_blocks(int n);
 void launch_mlp_backward(hipStream_t s, MlpWeights w, const uint16_t* X, const float4* T, int n, float invCount,
                          float* partial, int blocks);
 void launch_mlp_reduce(hipStream_t s, const float* partial, int blocks, float* grad);
-void launch_mlp_adam(hipStream_t s, float* master, float* m, float* v, const float* grad, const int* step, float lr);
+// the step size is read from device memory, so a captured graph follows changes
+void launch_mlp_adam(hipStream_t s, float* master, float* m, float* v, const float* grad, const int* step,
+                     const float* lr);
 // BF16 operands from the masters; increments *step when step is non-null
 void launch_mlp_pack(hipStream_t s, const float* master, uint16_t* packed, int* step);
 // FP32 features (n x 12) to the BF16 layout

@@ -463,10 +463,11 @@ __global__ void k_mlp_reduce(const float* __restrict__ partial, int blocks, floa
 
 // Adam (Kingma & Ba 2015), beta1 0.9, beta2 0.999, eps 1e-8; step t = *step + 1.
 __global__ void k_mlp_adam(float* __restrict__ W, float* __restrict__ m, float* __restrict__ v,
-                           const float* __restrict__ grad, const int* __restrict__ step, float lr)
+                           const float* __restrict__ grad, const int* __restrict__ step, const float* __restrict__ lrp)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= kMlpParams) return;
+    const float lr = *lrp;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const float t = (float)(*step + 1);
     const float g = grad[p];
@@ -628,7 +629,8 @@ void launch_mlp_reduce(hipStream_t s, const float* partial, int blocks, float* g
     hipLaunchKernelGGL(k_mlp_reduce, dim3(wblocks(kMlpStride)), dim3(kWsBlock), 0, s, partial, blocks, grad);
 }
 
-void launch_mlp_adam(hipStream_t s, float* master, float* m, float* v, const float* grad, const int* step, float lr)
+void launch_mlp_adam(hipStream_t s, float* master, float* m, float* v, const float* grad, const int* step,
+                     const float* lr)
 {
     hipLaunchKernelGGL(k_mlp_adam, dim3(wblocks(kMlpParams)), dim3(kWsBlock), 0, s, master, m, v, grad, step, lr);
 }
