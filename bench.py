@@ -63,6 +63,7 @@ def run_batch(args):
     t0 = time.perf_counter()
     built = [S.build_patch(s, sc) for s in scenes]
     batch = sc.ClothBatch([p for _, p in built])
+    batch.SetUseGraph(not args.no_graph)
     print(f"batch512: built {len(built)} patches in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     dt = scenes[0].dt
     warmup = max(args.warmup, 7)  # the autotuner times its 6 candidates on the first steps
@@ -103,6 +104,7 @@ def run_batch(args):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": WORKLOADS["batch512"], "step": "one frame = 4 substeps of every cloth",
+                   "graph_replay": not args.no_graph,
                    "cloths": len(scenes), "vertices": V, "triangles": T, "bends": B,
                    "launch": {"block_size": bs, "lds_staging": bool(stage), "autotuned": tuned},
                    "tuning_ms_per_step": [[b, s_, round(ms, 3)] for b, s_, ms in batch.tuning()]},

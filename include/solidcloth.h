@@ -298,6 +298,8 @@ SCEXP void scBatchSetLaunch(scClothBatch batch, int blockSize, int stage);
 SCEXP void scBatchGetLaunch(scClothBatch batch, int* blockSize, int* stage, int* tuned);
 SCEXP int scBatchGetTuning(scClothBatch batch, int* configs, double* ms, int cap);
 SCEXP double scBatchGetStepTime(scClothBatch batch, int* launches);
+/* Replay the captured substep launches (default) or launch them directly. */
+SCEXP void scBatchSetUseGraph(scClothBatch batch, bool enable);
 
 /* Learned warm start (no counterpart in the reference). A per-vertex MLP
  * (12 one-ring features -> 64 -> 64 -> 3, GELU, BF16 MFMA with FP32

@@ -125,6 +125,7 @@ def _sig(L):
         "scBatchSetLaunch": (None, [H, i, i]),
         "scBatchGetLaunch": (None, [H, C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
         "scBatchGetTuning": (i, [H, vp, vp, i]), "scBatchGetStepTime": (C.c_double, [H, C.POINTER(i)]),
+        "scBatchSetUseGraph": (None, [H, b]),
         "scGetLastError": (C.c_char_p, []),
         "scGetErrorCount": (i, []), "scGetBuildInfo": (C.c_char_p, []),
     }
@@ -559,6 +560,9 @@ class ClothBatch:
         b, s, t = C.c_int(), C.c_int(), C.c_int()
         _checked("scBatchGetLaunch", self.h, C.byref(b), C.byref(s), C.byref(t))
         return b.value, s.value, bool(t.value)
+
+    def SetUseGraph(self, enable):
+        _checked("scBatchSetUseGraph", self.h, bool(enable))
 
     def last_step_time(self):
         """(device ms of the last step's substep launches, launch count)."""

@@ -739,6 +739,8 @@ void scSaveState(scModel model) { guard([&] { M(model)->save_state(); }); }
 
 void scRestoreState(scModel model) { guard([&] { M(model)->restore_state(); }); }
 
+void scBatchSetUseGraph(scClothBatch batch, bool enable) { guard([&] { B(batch)->useGraph = enable; }); }
+
 double scBatchGetStepTime(scClothBatch batch, int* launches)
 {
     return guard_r<double>(-1.0, [&] {

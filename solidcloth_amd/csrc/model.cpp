@@ -1489,7 +1489,7 @@ void ClothBatch::step(float dt)
         }
     std::string key = std::to_string(bs) + "/" + std::to_string(stageVerts) + "/" + std::to_string(n) + "/" +
                       std::to_string(m) + "/" + std::to_string((uintptr_t)d_descs);
-    if (!exec || key != graphKey) {
+    if (useGraph && (!exec || key != graphKey)) {
         release_graph();
         HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < n; i++) launch_batch_substep(stream, d_descs, m, bs, stageVerts);
@@ -1502,7 +1502,10 @@ void ClothBatch::step(float dt)
         HIP_OK(hipEventCreate(&ev1));
     }
     HIP_OK(hipEventRecord(ev0, stream));
-    HIP_OK(hipGraphLaunch(exec, stream));
+    if (useGraph)
+        HIP_OK(hipGraphLaunch(exec, stream));
+    else
+        for (int i = 0; i < n; i++) launch_batch_substep(stream, d_descs, m, bs, stageVerts);
     HIP_OK(hipEventRecord(ev1, stream));
     HIP_OK(hipStreamSynchronize(stream));
     HIP_OK(hipGetLastError());

@@ -399,6 +399,7 @@ public:
     int blockSize = 0;  // 0: autotune over the first steps
     int stage = -1;     // -1: autotune; 0: positions in HBM; 1: in LDS when they fit
     bool tuned = false;
+    bool useGraph = true;  // replay the captured substep launches (else launch directly)
     double lastStepMs = 0.0;  // device time of the last step's substep launches
     int lastSubsteps = 0;
     void add(ClothPatch* p);

@@ -452,13 +452,24 @@ __global__ void __launch_bounds__(kWsBlock) k_mlp_backward(MlpWeights w, const u
     for (int i = threadIdx.x; i < kMlpStride; i += kWsBlock) out[i] = acc[i];
 }
 
-__global__ void k_mlp_reduce(const float* __restrict__ partial, int blocks, float* __restrict__ grad)
+// 64 parameters per workgroup, 16 slices of the blocks per parameter: each
+// slice sums its blocks in order, then the slices are summed in order.
+__global__ void __launch_bounds__(1024) k_mlp_reduce(const float* __restrict__ partial, int blocks,
+                                                     float* __restrict__ grad)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= kMlpStride) return;
+    __shared__ float red[16][64];
+    const int pl = threadIdx.x & 63, sl = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + pl;
     float s = 0.f;
-    for (int k = 0; k < blocks; k++) s += partial[(size_t)k * kMlpStride + p];
-    grad[p] = s;
+    if (p < kMlpStride)
+        for (int k = sl; k < blocks; k += 16) s += partial[(size_t)k * kMlpStride + p];
+    red[sl][pl] = s;
+    __syncthreads();
+    if (sl == 0 && p < kMlpStride) {
+        float t = red[0][pl];
+        for (int k = 1; k < 16; k++) t += red[k][pl];
+        grad[p] = t;
+    }
 }
 
 // Adam (Kingma & Ba 2015), beta1 0.9, beta2 0.999, eps 1e-8; step t = *step + 1.
@@ -626,7 +637,7 @@ void launch_mlp_backward(hipStream_t s, MlpWeights w, const uint16_t* X, const f
 
 void launch_mlp_reduce(hipStream_t s, const float* partial, int blocks, float* grad)
 {
-    hipLaunchKernelGGL(k_mlp_reduce, dim3(wblocks(kMlpStride)), dim3(kWsBlock), 0, s, partial, blocks, grad);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpStride + 63) / 64), dim3(1024), 0, s, partial, blocks, grad);
 }
 
 void launch_mlp_adam(hipStream_t s, float* master, float* m, float* v, const float* grad, const int* step,

@@ -44,7 +44,7 @@ def training_plan():
     return plan
 
 
-def train(plan=None, seed=7, lr=1e-3, lr_final=3e-4, log=None):
+def train(plan=None, seed=7, lr=1e-3, lr_final=3e-4, log=None, use_graph=True):
     """Trains a fresh network over the plan; returns (network, substeps, seconds)."""
     import solidcloth_amd as sc
     from . import scenes as S
@@ -57,6 +57,7 @@ def train(plan=None, seed=7, lr=1e-3, lr_final=3e-4, log=None):
     for k, (scene, frames) in enumerate(plan):
         world, patch = S.build_patch(scene, sc)
         m = patch.GetModel()
+        m.SetUseGraph(use_graph)
         m.SetWarmStart(ws, "train")
         for f in range(frames):
             # linear decay of the step size over the whole plan
@@ -91,8 +92,12 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--frames-scale", type=float, default=1.0,
+                    help="fraction of each scene's frames (shorter runs, e.g. under a profiler)")
+    ap.add_argument("--no-graph", action="store_true", help="launch kernels directly instead of graph replay")
     args = ap.parse_args()
-    ws, n, secs = train(seed=args.seed, log=sys.stdout)
+    plan = [(s, max(1, int(round(f * args.frames_scale)))) for s, f in training_plan()]
+    ws, n, secs = train(plan=plan, seed=args.seed, log=sys.stdout, use_graph=not args.no_graph)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     np.save(args.out, ws.params())
     print(f"trained on {n} substeps in {secs:.1f} s; {ws.steps()} Adam steps; saved {args.out}", flush=True)
