@@ -17,7 +17,8 @@
  *
  * Difference: the reference reports no errors. This library additionally
  * records the last error (scGetLastError) and refuses calls it cannot honour
- * (for example a link-based model, which this build does not implement)
+ * (for example the mouse spring or a capsule collidable, which this build
+ * does not implement on the GPU substep)
  * instead of computing something else. All device work started by a call has
  * completed before any getter returns data.
  */
@@ -187,13 +188,18 @@ SCEXP int scSetParticlePositions(scModel model, const scVector3* in, int count);
 SCEXP int scGetParticleVelocities(scModel model, scVector3* out, int count);
 SCEXP int scSetParticleVelocities(scModel model, const scVector3* in, int count);
 
+/* Links in index order (Physics/ClothTypes.h:94-99): ids = (i1, i2),
+ * data = (rest length, stiffness) per link; returns the link count. */
+SCEXP int scGetLinks(scModel model, int* ids, float* data, int count);
+
 /* Number of bend constraints and their vertex quadruples (i1,i2,i3,i4). */
 SCEXP int scGetNumBends(scModel model);
 SCEXP int scGetBends(scModel model, int* out, int count);
 
 /* Gauss-Seidel order the GPU applies: kind 0 = triangles, 1 = bends,
  * 2 = obstacle-edge contacts (given as cloth edge indices), 3 = obstacle-
- * vertex contacts (given as cloth triangle indices). Writes the constraint
+ * vertex contacts (given as cloth triangle indices), 4 = links (the
+ * non-FEM stretch model, ClothPBD.cpp:9-50). Writes the constraint
  * indices in processing order and returns their count.
  * scGetColouring returns the number of launch phases and writes phase start
  * offsets into that order (numPhases + 1 values, colourOffsets may be NULL).
@@ -251,7 +257,9 @@ SCEXP void scSetSelfCapacity(scModel model, int capacity);
 
 /* Solver statistics of the last substep (Physics/ClothModel.h:32-40):
  * out[0] iterations, out[1] bend error, out[2] vertex-contact error,
- * out[3] self vertex-triangle error, out[4] self edge-edge error. */
+ * out[3] self vertex-triangle error, out[4] self edge-edge error,
+ * out[5] link error (link model; ClothModel.h:32-40 linksError). out holds
+ * 6 floats. */
 SCEXP void scGetSolverStats(scModel model, float* out);
 
 /* Kernel timing with HIP events on the solver's stream. which: 0 FEM sweep,

@@ -46,6 +46,7 @@ def lib():
         L.oc_get_counts.argtypes = [vp, vp]
         L.oc_get_triangles.argtypes = [vp, vp, vp]
         L.oc_get_bends.argtypes = [vp, vp]
+        L.oc_get_links.argtypes = [vp, vp, vp]
         L.oc_get_edges.argtypes = [vp, vp]
         L.oc_get_state.argtypes = [vp, _C.c_int, vp]
         L.oc_set_state.argtypes = [vp, _C.c_int, vp]
@@ -58,6 +59,8 @@ def lib():
         L.oc_solve_triangles.argtypes = [vp, _C.c_float]
         L.oc_solve_bends.argtypes = [vp]
         L.oc_solve_bends.restype = _C.c_double
+        L.oc_solve_links.argtypes = [vp, _C.c_float]
+        L.oc_solve_links.restype = _C.c_double
         L.oc_solve_contacts.argtypes = [vp, _C.c_float]
         L.oc_solve_contacts.restype = _C.c_double
         L.oc_substep_detect.argtypes = [vp, _C.c_float]
@@ -200,9 +203,17 @@ class Cloth:
         return out
 
     def counts(self):
-        c = _np.zeros(5, dtype=_np.int32)
+        c = _np.zeros(6, dtype=_np.int32)
         self.L.oc_get_counts(self.h, _ptr(c))
-        return dict(V=int(c[0]), T=int(c[1]), E=int(c[2]), B=int(c[3]), C=int(c[4]))
+        return dict(V=int(c[0]), T=int(c[1]), E=int(c[2]), B=int(c[3]), C=int(c[4]), L=int(c[5]))
+
+    def links(self):
+        """(L, 2) vertex pairs and (L, 2) (rest length, stiffness)."""
+        n = self.counts()["L"]
+        idx = _np.zeros((n, 2), dtype=_np.int32)
+        rest = _np.zeros((n, 2))
+        self.L.oc_get_links(self.h, _ptr(idx), _ptr(rest))
+        return idx, rest
 
     def triangles(self):
         n = self.counts()["T"]
@@ -244,10 +255,10 @@ class Cloth:
         return ids, data
 
     def stats(self):
-        s = _np.zeros(6)
+        s = _np.zeros(7)
         self.L.oc_get_stats(self.h, _ptr(s))
         return dict(iterations=int(s[0]), bendError=s[1], contactError=s[2], selfVTError=s[3],
-                    selfEEError=s[4], selfEECount=int(s[5]))
+                    selfEEError=s[4], selfEECount=int(s[5]), linkError=s[6])
 
     def substep_detect(self, h):
         self.L.oc_substep_detect(self.h, h)
@@ -278,12 +289,11 @@ class Cloth:
 
     def set_order(self, kind, order):
         o = _np.ascontiguousarray(order, dtype=_np.int32)
-        self.L.oc_set_order(self.h, 0 if kind in (0, "tri") else 1, _ptr(o), len(o))
+        k = {0: 0, "tri": 0, 1: 1, "bend": 1, 4: 2, "link": 2}[kind]
+        self.L.oc_set_order(self.h, k, _ptr(o), len(o))
 
     def step(self, dt):
         rc = self.L.oc_step(self.h, dt)
-        if rc == -1:
-            raise RuntimeError("oracle step failed (only the FEM stretch model is restated)")
         if rc != 0:
             raise RuntimeError("oracle step refused: grid-SDF edge contacts need vertex contacts")
 
@@ -298,6 +308,10 @@ class Cloth:
 
     def solve_bends(self):
         return self.L.oc_solve_bends(self.h)
+
+    def solve_links(self, h):
+        """One SolveLinks pass (ClothPBD.cpp:9-50); returns its error."""
+        return self.L.oc_solve_links(self.h, h)
 
     def solve_contacts(self, h):
         return self.L.oc_solve_contacts(self.h, h)
@@ -353,10 +367,13 @@ def from_scene(scene, real64=False, tri_order=None, bend_order=None):
     from solidcloth_amd import scenes as S
 
     c = Cloth(real64)
-    c.set(unit=scene.unit, fem=1, shear=scene.young, poisson=scene.poisson, bend=scene.bend,
+    c.set(unit=scene.unit, fem=int(scene.fem), dihedral=int(scene.dihedral), shear=scene.young,
+          poisson=scene.poisson, bend=scene.bend,
           iterations=scene.iterations, fixedIterations=int(scene.fixed_iterations), collFlags=scene.flags,
           thickness=scene.thickness, tolerance=scene.tolerance, friction=scene.friction,
           numSteps=scene.substeps)
+    if scene.stretch is not None:
+        c.set(stretch=scene.stretch)  # before the grid: links take their stiffness at creation
     if scene.analytic_sphere:
         c.add_sphere(scene.sphere_center, scene.sphere_radius)
     elif scene.sdf_cells:

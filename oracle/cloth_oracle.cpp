@@ -51,7 +51,7 @@ struct Params {
 
 struct Stats {
     int iterations = 0;
-    float bendError = 0, contactError = 0, selfVTError = 0, selfEEError = 0;
+    float bendError = 0, contactError = 0, selfVTError = 0, selfEEError = 0, linkError = 0;
     int selfEECount = 0;
 };
 
@@ -63,6 +63,13 @@ struct Tri {
 
 struct Bend {
     uint32_t i1, i2, i3, i4;
+};
+
+// Physics/ClothTypes.h:94-99 (Link: reference length and stiffness)
+template <class R>
+struct LinkC {
+    uint32_t i1, i2;
+    R len, stiffness;
 };
 
 struct Edge {
@@ -283,6 +290,7 @@ public:
     std::vector<R> uvx, uvy;
     std::vector<Tri<R>> tris;
     std::vector<Bend> bends;
+    std::vector<LinkC<R>> links;
     std::vector<Edge> edges;
     std::vector<uint32_t> meshIndices;
     std::vector<Contact<R>> contacts;
@@ -291,7 +299,7 @@ public:
     std::vector<ObstacleMesh> meshes;
     std::vector<EdgeContact<R>> edgeContacts;
     std::vector<TriContact<R>> triContacts;
-    std::vector<int> triOrder, bendOrder;
+    std::vector<int> triOrder, bendOrder, linkOrder;
     // processing order of edge and triangle contacts by cloth edge / triangle
     // (empty: ascending index, the reference's list order)
     std::vector<int> edgeContactRank, triContactRank;
@@ -303,8 +311,16 @@ public:
     void clear()
     {
         pos.clear(); prev.clear(); vel.clear(); invMass.clear(); uvx.clear(); uvy.clear();
-        tris.clear(); bends.clear(); edges.clear(); meshIndices.clear(); contacts.clear();
-        triOrder.clear(); bendOrder.clear();
+        tris.clear(); bends.clear(); links.clear(); edges.clear(); meshIndices.clear(); contacts.clear();
+        triOrder.clear(); bendOrder.clear(); linkOrder.clear();
+    }
+
+    // Physics/ClothModel.inl:12-31 (AddLink)
+    void add_link(uint32_t i1, uint32_t i2, float stiffness)
+    {
+        if (invMass[i1] == R(0) && invMass[i2] == R(0)) return;
+        if (stiffness == 0.f) return;
+        links.push_back(LinkC<R>{i1, i2, length(pos[i1] - pos[i2]), R(stiffness)});
     }
 
     // Physics/ClothModel.cpp:54-92
@@ -388,12 +404,19 @@ public:
         const V3<R> offset{R(offF.x), R(offF.y), R(offF.z)};
         R x = R(-0.5f) * R(divX) * inc;
         const R startY = x;
+        const float stretch = prm.stretch, shear = prm.shear, bend = prm.bend;
+        const bool dihedral = prm.dihedral;
         for (int j = 0, base = 0; j < divX; j++, base += divY) {
             pos[base] = horizontal ? (V3<R>{x, 0, startY} + offset) * scale : (V3<R>{x, startY, 0} + offset) * scale;
             prev[base] = pos[base];
             vel[base] = V3<R>{0, 0, 0};
             invMass[base] = 1;
             uvx[base] = x; uvy[base] = 0;
+            if (base >= divY) {
+                add_link(base - divY, base, stretch);
+                add_link(base - divY + 1, base, shear);
+                if (base >= 2 * divY && !dihedral) add_link(base - 2 * divY, base, bend);
+            }
             for (int i = 1; i < divY; i++) {
                 int idx = base + i;
                 if (horizontal)
@@ -404,6 +427,14 @@ public:
                 vel[idx] = V3<R>{0, 0, 0};
                 invMass[idx] = 1;
                 uvx[idx] = x; uvy[idx] = inc * R(i);
+                add_link(idx - 1, idx, stretch);
+                if (idx > base + 1 && !dihedral) add_link(idx - 2, idx, bend);
+                if (idx >= divY) {
+                    add_link(idx - divY, idx, stretch);
+                    if (i < divY - 1) add_link(idx - divY + 1, idx, shear);
+                    if (j > 1 && !dihedral) add_link(idx - 2 * divY, idx, bend);
+                }
+                if (idx > divY) add_link(idx - divY - 1, idx, shear);
                 if (idx > divY) {
                     if ((i & 1) == (j & 1)) {
                         add_triangle(idx - divY - 1, idx, idx - divY);
@@ -1339,21 +1370,56 @@ public:
         return error;
     }
 
+    // Physics/ClothPBD.cpp:9-50 (omega = 1)
+    void project_link(LinkC<R>& link, R k, R& error)
+    {
+        const R m1 = invMass[link.i1], m2 = invMass[link.i2];
+        if (m1 + m2 == R(0)) return;
+        const R mu = R(1.f) / (m1 + m2);
+        V3<R> delta = pos[link.i1] - pos[link.i2];
+        const R len0 = link.len;
+        const R len = length(delta);
+        if (len == R(0) || std::isnan(len) || std::isinf(len)) return;
+        const R err = len - len0;
+        scale(delta, R(1) / len);
+        const R epsilon = len0 / (k * link.stiffness);
+        const R s = R(1.f) / (R(1.f) + mu * epsilon);
+        const R lambda = err * mu * s;
+        const R e = s * err;
+        if (std::fabs(e) > error) error = std::fabs(e);
+        const V3<R> fc = (R(1.f) * lambda) * delta;
+        pos[link.i1] -= fc * m1;
+        pos[link.i2] += fc * m2;
+    }
+
+    R solve_links(R h)
+    {
+        const R h2 = h * h;
+        const R k = h2 * R(prm.area) * R(prm.unit);
+        R error = 0;
+        if (linkOrder.empty()) {
+            for (auto& l : links) project_link(l, k, error);
+        } else {
+            for (int i : linkOrder) project_link(links[i], k, error);
+        }
+        return error;
+    }
+
     // Physics/ClothPBD.cpp:324-369
     void solve_gs(R h)
     {
-        R cb = 0, cv = 0, sct = 0, sce = 0;
+        R cb = 0, cv = 0, sct = 0, sce = 0, cl = 0;
         int k = 0, sceCount = 0;
         while (true) {
             if (prm.fem) solve_triangles(h);
+            else cl = solve_links(h);
             if (prm.dihedral && k % prm.bendFreq == 0) cb = solve_bends();
             cv = solve_contacts(h);
             const R ce = solve_edge_contacts();
             const R ct = solve_tri_contacts();
             sct = solve_self_tris(h);
             sce = solve_self_edges(h, sceCount);
-            // the FEM model reports no link error, so cl = 0 in the exit test
-            const R cl = 0;
+            // the FEM model reports no link error: cl stays 0 in the exit test
             k++;
             const float eps = 0.01f;
             if (!prm.fixedIterations && cl < R(0.05f) && cb < R(0.005f) && sce < R(prm.selfCollThreshold) &&
@@ -1367,6 +1433,7 @@ public:
         stats.selfVTError = (float)sct;
         stats.selfEEError = (float)sce;
         stats.selfEECount = sceCount;
+        stats.linkError = (float)cl;
     }
 
     // Physics/ClothPBD.cpp:255-322
@@ -1407,11 +1474,9 @@ public:
         }
     }
 
-    // Physics/ClothPatch.cpp:323-335. Only the FEM stretch model is restated
-    // (ClothPBD.cpp:341-344); the link model returns an error.
+    // Physics/ClothPatch.cpp:323-335
     int step(float dt)
     {
-        if (!prm.fem) return -1;
         // grid-SDF edge queries without vertex contacts yield undefined
         // contacts in the reference (a stub query accepted at the border)
         for (const auto& c : world)
@@ -1599,7 +1664,7 @@ void oc_get_sdf_grads(void* h, int sdf, double* out)
 void oc_set_order(void* h, int kind, const int* order, int n)
 {
     with(h, [&](auto& c) {
-        std::vector<int>& o = kind == 0 ? c.triOrder : c.bendOrder;
+        std::vector<int>& o = kind == 0 ? c.triOrder : kind == 1 ? c.bendOrder : c.linkOrder;
         o.assign(order, order + n);
     });
 }
@@ -1612,6 +1677,7 @@ void oc_get_counts(void* h, int* out)
         out[2] = (int)c.edges.size();
         out[3] = (int)c.bends.size();
         out[4] = (int)c.contacts.size();
+        out[5] = (int)c.links.size();
     });
 }
 
@@ -1625,6 +1691,17 @@ void oc_get_triangles(void* h, int* idx, double* rest)
             double r[11] = {(double)t.du1, (double)t.du2, (double)t.dv1, (double)t.dv2, (double)t.det, (double)t.invDet,
                             (double)t.lu0, (double)t.lv0, (double)t.area, (double)t.lu2, (double)t.lv2};
             memcpy(rest + 11 * k, r, sizeof(r));
+        }
+    });
+}
+
+// out: L*2 ints; rest: L*2 doubles (len, stiffness)
+void oc_get_links(void* h, int* out, double* rest)
+{
+    with(h, [&](auto& c) {
+        for (size_t k = 0; k < c.links.size(); k++) {
+            out[2 * k] = c.links[k].i1; out[2 * k + 1] = c.links[k].i2;
+            rest[2 * k] = (double)c.links[k].len; rest[2 * k + 1] = (double)c.links[k].stiffness;
         }
     });
 }
@@ -1702,6 +1779,7 @@ void oc_get_stats(void* h, double* out)
         out[3] = c.stats.selfVTError;
         out[4] = c.stats.selfEEError;
         out[5] = c.stats.selfEECount;
+        out[6] = c.stats.linkError;
     });
 }
 
@@ -1780,6 +1858,7 @@ void oc_solve_triangles(void* h, float hstep)
     });
 }
 double oc_solve_bends(void* h) { return with(h, [&](auto& c) { return (double)c.solve_bends(); }); }
+double oc_solve_links(void* h, float step) { return with(h, [&](auto& c) { return (double)c.solve_links(step); }); }
 double oc_solve_contacts(void* h, float hstep)
 {
     return with(h, [&](auto& c) {

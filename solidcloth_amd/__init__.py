@@ -99,7 +99,7 @@ def _sig(L):
         "scCreateMeshFromArrays": (H, [vp, i, vp, i]), "scFreeMesh": (None, [H]),
         "scGetParticlePositions": (i, [H, vp, i]), "scSetParticlePositions": (i, [H, vp, i]),
         "scGetParticleVelocities": (i, [H, vp, i]), "scSetParticleVelocities": (i, [H, vp, i]),
-        "scGetNumBends": (i, [H]), "scGetBends": (i, [H, vp, i]),
+        "scGetNumBends": (i, [H]), "scGetBends": (i, [H, vp, i]), "scGetLinks": (i, [H, vp, vp, i]),
         "scGetConstraintOrder": (i, [H, i, vp, i]), "scGetColouring": (i, [H, i, vp, i]),
         "scGetSchedule": (i, [H, i, vp, vp, vp, i]), "scSetSweep": (None, [H, i, i]),
         "scGetSolverStats": (None, [H, vp]), "scEnableKernelTiming": (None, [H, b]),
@@ -386,10 +386,10 @@ class ClothModel:
         _checked("scGetBends", self.h, _ptr(out), n)
         return out
 
-    _KINDS = {0: 0, "tri": 0, 1: 1, "bend": 1, 2: 2, "edge_contact": 2, 3: 3, "tri_contact": 3}
+    _KINDS = {0: 0, "tri": 0, 1: 1, "bend": 1, 2: 2, "edge_contact": 2, 3: 3, "tri_contact": 3, 4: 4, "link": 4}
 
     def constraint_order(self, kind):
-        """kind 'tri' / 'bend' / 'edge_contact' / 'tri_contact' (or 0-3)."""
+        """kind 'tri' / 'bend' / 'edge_contact' / 'tri_contact' / 'link' (or 0-4)."""
         k = self._KINDS[kind]
         n = _checked("scGetConstraintOrder", self.h, k, None, 0)
         out = np.zeros(n, dtype=np.int32)
@@ -455,10 +455,18 @@ class ClothModel:
         return self._feature_contacts("scGetTriContactData", 11)
 
     def stats(self):
-        out = np.zeros(5, dtype=np.float32)
+        out = np.zeros(6, dtype=np.float32)
         _checked("scGetSolverStats", self.h, _ptr(out))
         return dict(iterations=int(out[0]), bendError=float(out[1]), contactError=float(out[2]),
-                    selfVTError=float(out[3]), selfEEError=float(out[4]))
+                    selfVTError=float(out[3]), selfEEError=float(out[4]), linkError=float(out[5]))
+
+    def links(self):
+        """(L, 2) vertex pairs and (L, 2) float32 (rest length, stiffness), index order."""
+        n = _checked("scGetLinks", self.h, None, None, 0)
+        ids = np.zeros((n, 2), dtype=np.int32)
+        data = np.zeros((n, 2), dtype=np.float32)
+        _checked("scGetLinks", self.h, _ptr(ids), _ptr(data), n)
+        return ids, data
 
     def self_contacts(self, kind):
         """kind 'vt' / 'ee' -> keys (n,), ids (n, 4), data (n, 8): w1 w2 w3,

@@ -791,7 +791,7 @@ int scGetBends(scModel model, int* out, int count)
 }
 
 // kind 0: FEM triangles, 1: bends, 2: obstacle-edge contacts (by cloth
-// edge), 3: obstacle-vertex contacts (by cloth triangle)
+// edge), 3: obstacle-vertex contacts (by cloth triangle), 4: links
 static const Schedule& schedule_of(ClothModel* m, int kind)
 {
     switch (kind) {
@@ -799,6 +799,7 @@ static const Schedule& schedule_of(ClothModel* m, int kind)
     case 1: return m->bendCol;
     case 2: return m->edgeContactCol;
     case 3: return m->triContactCol;
+    case 4: return m->linkCol;
     default: throw std::runtime_error("unknown constraint kind " + std::to_string(kind));
     }
 }
@@ -959,6 +960,20 @@ void scGetSolverStats(scModel model, float* out)
         out[2] = m->statContactError;
         out[3] = m->statSelfVTError;
         out[4] = m->statSelfEEError;
+        out[5] = m->statLinkError;
+    });
+}
+
+int scGetLinks(scModel model, int* ids, float* data, int count)
+{
+    return guard_r<int>(-1, [&] {
+        ClothModel* m = M(model);
+        const int n = (int)m->links.size();
+        for (int k = 0; k < n && k < count; k++) {
+            if (ids) { ids[2 * k] = (int)m->links[k].i1; ids[2 * k + 1] = (int)m->links[k].i2; }
+            if (data) { data[2 * k] = m->links[k].len; data[2 * k + 1] = m->links[k].stiffness; }
+        }
+        return n;
     });
 }
 

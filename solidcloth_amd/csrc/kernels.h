@@ -43,7 +43,7 @@ struct SolverCtl {
     int selfCount[2];    // self contacts found this substep (vertex-triangle, edge-edge)
     int selfColours[2];  // colour groups of each kind
     int selfOverflow;    // contacts beyond capacity, or colouring rounds exhausted
-    int pad;
+    unsigned cl;         // link error of the current iteration (link model)
 };
 
 enum { MAX_COLLIDABLES = 8 };
@@ -208,6 +208,15 @@ void launch_contacts(hipStream_t s, float4* pos, const float4* prev, float4* vel
                      float mu, float invH, SolverCtl* ctl);
 void launch_iter_end(hipStream_t s, SolverCtl* ctl, int maxIter, int fixedIter, int bendFreq, int dihedral,
                      float selfThreshold);
+// Clears error accumulators before the last pass of a fixed iteration count
+// (mask 1: links, contacts, self; 2: bends).
+void launch_reset_errors(hipStream_t s, SolverCtl* ctl, int mask);
+// Link model (ClothPBD.cpp:9-50): links (i1, i2) with (rest length,
+// stiffness) in schedule order; k = h^2 area unit.
+void launch_links(hipStream_t s, float4* pos, const int2* lidx, const float2* lrest, int start, int count, float k,
+                  SolverCtl* ctl);
+void launch_link_slab(hipStream_t s, float4* pos, const ushort4* lidx, const float2* lrest, SlabSet ss, int numSlabs,
+                      int maxVerts, float k, SolverCtl* ctl);
 void launch_velocity(hipStream_t s, float4* pos, const float4* prev, float4* vel, int n, float invH, SolverCtl* ctl);
 void launch_copy(hipStream_t s, const float4* src, float4* dst, long n);
 

@@ -31,6 +31,7 @@ __global__ void k_predict(float4* __restrict__ pos, float4* __restrict__ prev, f
         ctl->ct = 0;
         ctl->sct = 0;
         ctl->sce = 0;
+        ctl->cl = 0;
         ctl->selfCount[0] = 0;
         ctl->selfCount[1] = 0;
         ctl->iterations = 0;
@@ -133,6 +134,7 @@ __global__ void k_predict_detect(float4* __restrict__ pos, float4* __restrict__ 
         ctl->ct = 0;
         ctl->sct = 0;
         ctl->sce = 0;
+        ctl->cl = 0;
         ctl->selfCount[0] = 0;
         ctl->selfCount[1] = 0;
         ctl->iterations = 0;
@@ -254,6 +256,48 @@ __device__ __forceinline__ float bend_project(float4& P1, float4& P2, float4& P3
     P3 = f4(add(x3, mul(lambda * m3, q3)), m3);
     P4 = f4(add(x4, mul(lambda * m4, q4)), m4);
     return e > 0.f ? e : 0.f;
+}
+
+// Physics/ClothPBD.cpp:9-50 — one link projection (omega = 1). Returns
+// |s err| when the link was projected (skipped links and NaN errors count 0,
+// as in the reference's `>` comparison).
+__device__ __forceinline__ float link_project(float4& P1, float4& P2, const float2 L, float k)
+{
+    const float m1 = P1.w, m2 = P2.w;
+    if (m1 + m2 == 0.f) return 0.f;
+    const float mu = 1.f / (m1 + m2);
+    f3 delta = sub(xyz(P1), xyz(P2));
+    const float len0 = L.x;
+    const float l = len(delta);
+    if (l == 0.f || isnan(l) || isinf(l)) return 0.f;
+    const float err = l - len0;
+    delta = mul(1.f / l, delta);
+    const float epsilon = len0 / (k * L.y);
+    const float s = 1.f / (1.f + mu * epsilon);
+    const float lambda = err * mu * s;
+    const float e = fabsf(s * err);
+    const f3 fc = mul(1.f * lambda, delta);
+    P1 = f4(sub(xyz(P1), mul(m1, fc)), m1);
+    P2 = f4(add(xyz(P2), mul(m2, fc)), m2);
+    return e > 0.f ? e : 0.f;
+}
+
+// Colour schedule: one thread per link of one colour.
+__global__ void __launch_bounds__(kBlock) k_links(float4* __restrict__ pos, const int2* __restrict__ lidx,
+                                                  const float2* __restrict__ lrest, int start, int count, float k,
+                                                  SolverCtl* ctl)
+{
+    if (ctl->done) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    float err = 0.f;
+    if (t < count) {
+        const int2 id = lidx[start + t];
+        float4 P1 = pos[id.x], P2 = pos[id.y];
+        err = link_project(P1, P2, lrest[start + t], k);
+        pos[id.x] = P1;
+        pos[id.y] = P2;
+    }
+    wave_max_atomic(&ctl->cl, err);
 }
 
 // Colour schedule: one thread per triangle of one colour, positions in HBM.
@@ -491,6 +535,35 @@ __global__ void __launch_bounds__(kSlabBlock) k_bend_slab(float4* __restrict__ p
     wave_max_atomic(&ctl->cb, err);
 }
 
+// Slab schedule for links: as k_bend_slab with two vertices per constraint.
+__global__ void __launch_bounds__(kSlabBlock) k_link_slab(float4* __restrict__ pos, const ushort4* __restrict__ lidx,
+                                                          const float2* __restrict__ lrest, SlabSet ss, float k,
+                                                          SolverCtl* ctl)
+{
+    if (ctl->done) return;
+    extern __shared__ float4 sv[];
+    __shared__ int scol[kMaxSlabColours + 1];
+    int4 info;
+    SlabContacts none;
+    none.enabled = 0;
+    const int nc = slab_begin(ss, info, scol, sv, pos, none);
+    float err = 0.f;
+    for (int c = 0; c < nc; c++) {
+        const int e = scol[c + 1];
+        for (int t = scol[c] + threadIdx.x; t < e; t += blockDim.x) {
+            const ushort4 id = lidx[t];
+            float4 P1 = sv[id.x], P2 = sv[id.y];
+            const float e1 = link_project(P1, P2, lrest[t], k);
+            err = err < e1 ? e1 : err;
+            sv[id.x] = P1;
+            sv[id.y] = P2;
+        }
+        __syncthreads();
+    }
+    slab_end(ss, info, sv, pos);
+    wave_max_atomic(&ctl->cl, err);
+}
+
 // Physics/ClothCollisionHandler.cpp:134-202 — vertex contacts with friction.
 // One thread per vertex applies that vertex's contacts in list order;
 // contacts of different vertices commute, so this equals the reference's
@@ -522,18 +595,21 @@ __global__ void __launch_bounds__(kBlock) k_contacts_velocity(float4* __restrict
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v == 0) ctl->maxTv = 0;  // accumulated afresh by the next prediction
-    if (v >= cb.nv) return;
-    float4 P = pos[v];
-    const f3 pp = xyz(prev[v]);
-    f3 p = xyz(P);
-    f3 ve = xyz(vel[v]);
-    const int n = cb.count[v];
-    if (n > 0) {
-        vertex_contacts(v, n, p, pp, ve, cb, thickness, mu, invH);
-        pos[v] = f4(p, P.w);
+    float err = 0.f;
+    if (v < cb.nv) {
+        float4 P = pos[v];
+        const f3 pp = xyz(prev[v]);
+        f3 p = xyz(P);
+        f3 ve = xyz(vel[v]);
+        const int n = cb.count[v];
+        if (n > 0) {
+            err = vertex_contacts(v, n, p, pp, ve, cb, thickness, mu, invH);
+            pos[v] = f4(p, P.w);
+        }
+        if (P.w != 0.f) ve = mul(invH, sub(p, pp));
+        if (n > 0 || P.w != 0.f) vel[v] = f4(ve, 0.f);
     }
-    if (P.w != 0.f) ve = mul(invH, sub(p, pp));
-    if (n > 0 || P.w != 0.f) vel[v] = f4(ve, 0.f);
+    wave_max_atomic(&ctl->cv, err);  // the last iteration's vertex-contact error
 }
 
 // Physics/ClothPBD.cpp:354-361 — iteration counter and the early-exit test.
@@ -546,7 +622,7 @@ __global__ void k_iter_end(SolverCtl* ctl, int maxIter, int fixedIter, int bendF
     float cb = __uint_as_float(ctl->cb), cv = __uint_as_float(ctl->cv);
     const float ce = __uint_as_float(ctl->ce), ct = __uint_as_float(ctl->ct);
     const float sct = __uint_as_float(ctl->sct), sce = __uint_as_float(ctl->sce);
-    const float cl = 0.f, eps = 0.01f;
+    const float cl = __uint_as_float(ctl->cl), eps = 0.01f;
     bool exitNow = !fixedIter && cl < 0.05f && cb < 0.005f && sce < selfThr && sct < selfThr && cv < eps &&
                    ce < eps && ct < eps;
     if (exitNow || k >= maxIter) {
@@ -561,7 +637,29 @@ __global__ void k_iter_end(SolverCtl* ctl, int maxIter, int fixedIter, int bendF
     ctl->ct = 0;
     ctl->sct = 0;
     ctl->sce = 0;
+    ctl->cl = 0;
     if (dihedral && k % bendFreq == 0) ctl->cb = 0;
+}
+
+// Fixed iteration count: the error statistics are those of the last pass of
+// each kind (ClothPBD.cpp:337-361), so the accumulators are cleared before it.
+// mask: 1 per-iteration errors (links, contacts, self), 2 the bend error.
+__global__ void k_reset_errors(SolverCtl* ctl, int mask)
+{
+    if (mask & 1) {
+        ctl->cl = 0;
+        ctl->cv = 0;
+        ctl->ce = 0;
+        ctl->ct = 0;
+        ctl->sct = 0;
+        ctl->sce = 0;
+    }
+    if (mask & 2) ctl->cb = 0;
+}
+
+void launch_reset_errors(hipStream_t s, SolverCtl* ctl, int mask)
+{
+    hipLaunchKernelGGL(k_reset_errors, dim3(1), dim3(1), 0, s, ctl, mask);
 }
 
 // Physics/ClothPBD.cpp:314-321
@@ -822,6 +920,21 @@ void launch_iter_end(hipStream_t s, SolverCtl* ctl, int maxIter, int fixedIter, 
 {
     hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(1), 0, s, ctl, maxIter, fixedIter, bendFreq, dihedral,
                        selfThreshold);
+}
+
+void launch_links(hipStream_t s, float4* pos, const int2* lidx, const float2* lrest, int start, int count, float k,
+                  SolverCtl* ctl)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_links, dim3(blocks(count)), dim3(kBlock), 0, s, pos, lidx, lrest, start, count, k, ctl);
+}
+
+void launch_link_slab(hipStream_t s, float4* pos, const ushort4* lidx, const float2* lrest, SlabSet ss, int numSlabs,
+                      int maxVerts, float k, SolverCtl* ctl)
+{
+    if (numSlabs <= 0) return;
+    hipLaunchKernelGGL(k_link_slab, dim3(numSlabs), dim3(kSlabBlock), maxVerts * sizeof(float4), s, pos, lidx, lrest,
+                       ss, k, ctl);
 }
 
 void launch_velocity(hipStream_t s, float4* pos, const float4* prev, float4* vel, int n, float invH, SolverCtl* ctl)

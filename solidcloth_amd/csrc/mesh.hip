@@ -29,6 +29,7 @@ __global__ void __launch_bounds__(kBlock) k_predict_mesh(float4* __restrict__ po
         ctl->ct = 0;
         ctl->sct = 0;
         ctl->sce = 0;
+        ctl->cl = 0;
         ctl->selfCount[0] = 0;
         ctl->selfCount[1] = 0;
         ctl->iterations = 0;

@@ -275,7 +275,7 @@ void GpuSolver::release()
     warm.built = warm.saved = false;
     dev_free(d_pos); dev_free(d_prev); dev_free(d_vel);
     dev_free(d_tidx); dev_free(d_tA); dev_free(d_tB); dev_free(d_tC); dev_free(d_bidx);
-    dev_free(d_triColOff); dev_free(d_bendColOff);
+    dev_free(d_triColOff); dev_free(d_bendColOff); dev_free(d_lk); dev_free(d_lr);
     dev_free(d_cCount); dev_free(d_cP); dev_free(d_cN); dev_free(d_cSrc); dev_free(d_cFeat); dev_free(d_ctl);
     dev_free(d_geo); dev_free(d_cEdges); dev_free(d_cTris);
     geoHost.clear();
@@ -292,7 +292,7 @@ void GpuSolver::release()
     }
     self.cap = 0;
     self.built = false;
-    for (DeviceSlabs* d : {&triSlabs, &bendSlabs}) {
+    for (DeviceSlabs* d : {&triSlabs, &bendSlabs, &linkSlabs}) {
         dev_free(d->lidx); dev_free(d->info); dev_free(d->colOff); dev_free(d->verts);
     }
     slotCap = 0;
@@ -383,7 +383,7 @@ static int upload_slabs(const Schedule& s, DeviceSlabs& d, bool owners, int numV
     std::vector<ushort4> li(n);
     for (int k = 0; k < n; k++) {
         const uint16_t* l = &s.localIdx[(size_t)k * s.arity];
-        li[k] = make_ushort4(l[0], l[1], l[2], s.arity > 3 ? l[3] : 0);
+        li[k] = make_ushort4(l[0], l[1], s.arity > 2 ? l[2] : 0, s.arity > 3 ? l[3] : 0);
     }
     const int ns = (int)s.slabVertBegin.size() - 1;
     std::vector<int4> info(ns);
@@ -462,6 +462,22 @@ void ClothModel::build_device_topology()
                      hipMemcpyHostToDevice));
     g.triOrphans = upload_slabs(triCol, g.triSlabs, true, g.nv);
     upload_slabs(bendCol, g.bendSlabs, false, g.nv);
+    // links in schedule order
+    g.nl = (int)links.size();
+    std::vector<int2> lk(g.nl);
+    std::vector<float2> lr(g.nl);
+    for (int k = 0; k < g.nl; k++) {
+        const Link& l = links[linkCol.order[k]];
+        lk[k] = make_int2((int)l.i1, (int)l.i2);
+        lr[k] = make_float2(l.len, l.stiffness);
+    }
+    dev_alloc(g.d_lk, g.nl);
+    dev_alloc(g.d_lr, g.nl);
+    if (g.nl) {
+        HIP_OK(hipMemcpy(g.d_lk, lk.data(), g.nl * sizeof(int2), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(g.d_lr, lr.data(), g.nl * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    upload_slabs(linkCol, g.linkSlabs, false, g.nv);
     dev_alloc(g.d_cCount, g.nv);
     if (g.nv) HIP_OK(hipMemset(g.d_cCount, 0, g.nv * sizeof(int)));
     g.slotCap = 0;
@@ -519,17 +535,23 @@ void ClothModel::ensure_colouring()
     for (int e = 0; e < ne; e++) { ev[2 * e] = edges[e].i1; ev[2 * e + 1] = edges[e].i2; }
     edgeContactCol = schedule_colours(ne, 2, ev.data(), nv);
     triContactCol = schedule_colours(nt, 3, tv.data(), nv);
+    const int nl = (int)links.size();
+    std::vector<uint32_t> lv(2 * (size_t)nl);
+    for (int l = 0; l < nl; l++) { lv[2 * l] = links[l].i1; lv[2 * l + 1] = links[l].i2; }
     // LDS budget of a slab workgroup: 4000 float4 vertices (64,000 bytes)
     const int maxVerts = 4000;
     if (mSweepMode == SWEEP_COLOURS) {
         triCol = schedule_colours(nt, 3, tv.data(), nv);
         bendCol = schedule_colours(nb, 4, bv.data(), nv);
+        linkCol = schedule_colours(nl, 2, lv.data(), nv);
     } else if (mSlabSize > 0) {
         triCol = schedule_slabs(nt, 3, tv.data(), nv, pos.data(), mSlabSize, maxVerts);
         bendCol = schedule_slabs(nb, 4, bv.data(), nv, pos.data(), mSlabSize, maxVerts);
+        linkCol = schedule_slabs(nl, 2, lv.data(), nv, pos.data(), mSlabSize, maxVerts);
     } else {
         triCol = schedule_slabs_auto(nt, 3, tv.data(), nv, pos.data(), maxVerts);
         bendCol = schedule_slabs_auto(nb, 4, bv.data(), nv, pos.data(), maxVerts);
+        linkCol = schedule_slabs_auto(nl, 2, lv.data(), nv, pos.data(), maxVerts);
     }
     mColourDirty = false;
 }
@@ -726,7 +748,7 @@ std::string ClothModel::graph_key(float h, const CollSet& cs) const
     put(&h, 4); put(&mUnit, 4); put(&mThickness, 4); put(&mTolerance, 4); put(&mShearStiff, 4);
     put(&mBendStiff, 4); put(&mPoisson, 4); put(&mFriction, 4); put(&mDragCoeff, 4);
     put(&mNumIterations, 4); put(&mCollFlags, 4); put(&mBendFreq, 4);
-    put(&mDihedral, 1); put(&mFEM, 1); put(&mFixedIterations, 1); put(&mSweepMode, 4); put(&mSlabSize, 4);
+    put(&mArea, 4); put(&mDihedral, 1); put(&mFEM, 1); put(&mFixedIterations, 1); put(&mSweepMode, 4); put(&mSlabSize, 4);
     put(&cs, sizeof(cs));
     put(&gpu.d_cP, sizeof(void*));
     put(&gpu.d_cFeat, sizeof(void*));
@@ -843,7 +865,31 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
         if (wmode == 2) launch_mlp_apply(s, mWarm->weights(), g.warm.X, g.nv, g.d_pos, g.warm.ell / kWsScaleOut);
     }
     const int iterations = wmode == 2 ? mWarmIters : mNumIterations;
+    // ClothPBD.cpp:9-50: k = h^2 area unit, evaluated in the reference's order
+    const float linkK = h2 * mArea * mUnit;
+    // with a fixed count, the reported errors are the last passes' (the exit
+    // test's kernel clears them per iteration otherwise)
+    const int lastBend = mBendFreq > 0 ? ((iterations - 1) / mBendFreq) * mBendFreq : 0;
     for (int k = 0; k < iterations; k++) {
+        if (mFixedIterations) {
+            const int mask = (k == iterations - 1 && k > 0 ? 1 : 0) | (mDihedral && k == lastBend && k > 0 ? 2 : 0);
+            if (mask) launch_reset_errors(s, g.d_ctl, mask);
+        }
+        if (!mFEM) {
+            for (int p = 0; p < linkCol.numPhases(); p++) {
+                timed_begin(g.timer, timed, 0, s, ev);
+                if (linkCol.slabbed) {
+                    const int b = linkCol.phaseSlabBegin[p], ns = linkCol.phaseSlabBegin[p + 1] - b;
+                    SlabSet ss{g.linkSlabs.info, g.linkSlabs.colOff, g.linkSlabs.verts, b};
+                    launch_link_slab(s, g.d_pos, g.linkSlabs.lidx, g.d_lr, ss, ns, linkCol.maxSlabVerts, linkK,
+                                     g.d_ctl);
+                } else {
+                    const int st = linkCol.phaseOffsets[p], cnt = linkCol.phaseOffsets[p + 1] - st;
+                    launch_links(s, g.d_pos, g.d_lk, g.d_lr, st, cnt, linkK, g.d_ctl);
+                }
+                timed_end(g.timer, timed, 0, s, ev);
+            }
+        }
         if (mFEM) {
             for (int p = 0; p < triCol.numPhases(); p++) {
                 timed_begin(g.timer, timed, 0, s, ev);
@@ -960,7 +1006,6 @@ void ClothModel::run_substeps(float h, int n, const CollSet& cs, bool timed)
 
 void ClothModel::step_substeps(float h, int n)
 {
-    if (!mFEM) throw std::runtime_error("the link (non-FEM) stretch model is not implemented; call scSetUseFEM(true)");
     if (mNumIterations < 1) throw std::runtime_error("NumIterations must be >= 1");
     if (mBendFreq < 1) throw std::runtime_error("bend frequency must be >= 1");
     if (msActive) throw std::runtime_error("the mouse spring is not implemented on the GPU substep");
@@ -1050,6 +1095,7 @@ void ClothModel::fetch_stats()
     }
     statSelfVTError = __builtin_bit_cast(float, c.sct);
     statSelfEEError = __builtin_bit_cast(float, c.sce);
+    statLinkError = mFEM ? 0.f : __builtin_bit_cast(float, c.cl);
     mStatsPending = false;
 }
 
@@ -1305,7 +1351,7 @@ namespace sc {
 void ClothModel::validate_batch() const
 {
     if (pos.empty()) throw std::runtime_error("a batched patch has no cloth (scInitCloth first)");
-    if (!mFEM) throw std::runtime_error("the link (non-FEM) stretch model is not implemented; call scSetUseFEM(true)");
+    if (!mFEM) throw std::runtime_error("batched cloths use the FEM stretch model (scSetUseFEM(true))");
     if (mNumIterations < 1) throw std::runtime_error("NumIterations must be >= 1");
     if (mBendFreq < 1) throw std::runtime_error("bend frequency must be >= 1");
     if (msActive) throw std::runtime_error("the mouse spring is not implemented on the GPU substep");

@@ -203,6 +203,10 @@ public:
     float4 *d_tA = nullptr, *d_tB = nullptr;
     float2* d_tC = nullptr;
     int4* d_bidx = nullptr;
+    // links (non-FEM model) in schedule order: vertex pair, (rest length, stiffness)
+    int2* d_lk = nullptr;
+    float2* d_lr = nullptr;
+    int nl = 0;
     int *d_triColOff = nullptr, *d_bendColOff = nullptr;  // schedule phase offsets (colours in colour mode)
     int* d_cCount = nullptr;
     float4 *d_cP = nullptr, *d_cN = nullptr;
@@ -232,7 +236,7 @@ public:
         int cap = 0;
         bool built = false;
     } self;
-    DeviceSlabs triSlabs, bendSlabs;
+    DeviceSlabs triSlabs, bendSlabs, linkSlabs;
     int triOrphans = 0;  // vertices in no triangle slab
     int slots = 0, slotCap = 0;
     int activeSlots = 0;  // vertex-contact slots the world can fill (0: no vertex contacts)
@@ -295,6 +299,7 @@ public:
 
     int statIterations = 0;
     float statBendError = 0.f, statContactError = 0.f, statSelfVTError = 0.f, statSelfEEError = 0.f;
+    float statLinkError = 0.f;
 
     std::vector<ContactOut> contactsHost;
     std::vector<FeatureContactOut> edgeContactsHost, triContactsHost;
@@ -323,7 +328,7 @@ public:
     void ensure_colouring();
     void set_sweep(int mode, int slabSize);
 
-    Schedule triCol, bendCol;
+    Schedule triCol, bendCol, linkCol;
     // colour groups for obstacle-edge (cloth edges) and obstacle-vertex
     // (cloth triangles) contacts: greedy colourings in index order
     Schedule edgeContactCol, triContactCol;

@@ -82,6 +82,12 @@ class Scene:
     ccd: tuple = None
     # vertices pinned after Init (inverse mass 0); None = Init's own pins
     pins: tuple = None
+    # stretch model: FEM triangles (True) or links (False, ClothPBD.cpp:9-50);
+    # dihedral bends (True) or bend links; link stretch stiffness (None = the
+    # library default)
+    fem: bool = True
+    dihedral: bool = True
+    stretch: float = None
     extra: dict = field(default_factory=dict)
 
     @property
@@ -305,7 +311,10 @@ def build_patch(scene, sc=None):
     patch = sc.ClothPatch(world)
     m = patch.GetModel()
     m.SetUnit(scene.unit)
-    m.SetFEM(True)
+    m.SetFEM(scene.fem)
+    m.SetDihedral(scene.dihedral)
+    if scene.stretch is not None:
+        m.SetStretchStiffness(scene.stretch)
     m.SetShearStiffness(scene.young)
     m.SetPoisson(scene.poisson)
     m.SetBendStiffness(scene.bend)

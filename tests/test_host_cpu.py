@@ -111,6 +111,10 @@ def test_schedule_is_race_free(sc, n, mode):
     m.SetSweep(mode)
     for kind, cons in (("tri", m.triangles()), ("bend", m.bends())):
         validate_schedule(m.constraint_order(kind), m.colour_offsets(kind), m.schedule(kind), cons)
+    # links (the non-FEM model's constraints) are scheduled the same way
+    lm = S.build_patch(scene.with_(fem=False, dihedral=False), sc)[1].GetModel()
+    lm.SetSweep(mode)
+    validate_schedule(lm.constraint_order("link"), lm.colour_offsets("link"), lm.schedule("link"), lm.links()[0])
     nt = len(m.colour_offsets("tri")) - 1
     nb = len(m.colour_offsets("bend")) - 1
     if mode == 0:

@@ -196,10 +196,11 @@ def test_bend_skips_flat_and_pinned(oracle_mod):
 
 def test_free_fall_closed_form(oracle_mod):
     """Prediction and velocity update (ClothPBD.cpp:258-266, 314-321) with no
-    constraint active integrate x_k = x_0 + h^2 g unit k(k+1)/2 (symplectic
+    constraint active (link model with zero stiffnesses: AddLink skips every
+    link, ClothModel.inl:12-31) integrate x_k = x_0 + h^2 g unit k(k+1)/2 (symplectic
     Euler), g = (0, -9.8, 0); pinned vertices stay put."""
     unit = 2.0
-    c = oracle_mod.Cloth(real64=True).set(unit=unit, fem=0, dihedral=0)
+    c = oracle_mod.Cloth(real64=True).set(unit=unit, fem=0, dihedral=0, stretch=0, shear=0, bend=0)
     c.init_grid(4, 4, 0.1, (0, 1, 0), True, True)
     x0 = c.get("pos")
     m = c.get("invMass")
@@ -216,7 +217,7 @@ def test_free_fall_closed_form(oracle_mod):
 
 
 def test_drag_scales_velocity(oracle_mod):
-    c = oracle_mod.Cloth(real64=True).set(unit=1, fem=0, dihedral=0, drag=0.5)
+    c = oracle_mod.Cloth(real64=True).set(unit=1, fem=0, dihedral=0, drag=0.5, stretch=0, shear=0, bend=0)
     c.init_grid(3, 3, 0.1, (0, 0, 0), True, False)
     h = 2.0 ** -7
     c.substep(h)
