@@ -130,7 +130,7 @@ struct SlabSet {
     int base;  // first slab of the launched phase
 };
 
-enum { kSlabBlock = 256, kMaxSlabColours = 63, kMaxSelfColours = 4096 };
+enum { kSlabBlock = 256, kMaxSlabColours = 63, kMaxSelfColours = 4096, kSlabBatch = 8 };
 
 // Vertex contacts applied by the FEM slab kernel while it stages vertices
 // (the previous iteration's contacts, by each vertex's owner slab).

@@ -408,26 +408,58 @@ __device__ __forceinline__ float vertex_contacts(int v, int n, f3& p, const f3 p
 // the previous iteration while staging it — after every constraint of that
 // iteration (earlier launches) and before any constraint of this one, which
 // is where the reference's sequential loop puts them.
+// Workgroup barrier that orders LDS only: the colours of a slab exchange
+// vertices through LDS alone, so the constraint data already requested from
+// global memory for the next colour may stay in flight across it (a plain
+// __syncthreads waits for every outstanding global load).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ int slab_begin(const SlabSet& ss, int4& info, int* scol, float4* sv, const float4* pos,
                                           const SlabContacts& kc)
 {
     info = ss.info[ss.base + blockIdx.x];
     const int* vg = ss.verts + info.x;
     for (int k = threadIdx.x; k <= info.w; k += blockDim.x) scol[k] = ss.colOff[info.z + k];
-    for (int v = threadIdx.x; v < info.y; v += blockDim.x) {
-        const int raw = vg[v];
-        const int g = raw & 0x7fffffff;
-        float4 P = pos[g];
-        if (kc.enabled && raw < 0) {
-            const int n = kc.cb.count[g];
-            if (n > 0) {
-                f3 p = xyz(P), ve = xyz(kc.vel[g]);
-                vertex_contacts(g, n, p, xyz(kc.prev[g]), ve, kc.cb, kc.thickness, kc.mu, kc.invH);
-                P = f4(p, P.w);
-                kc.vel[g] = f4(ve, 0.f);
-            }
+    // vertex ids, then positions, kSlabBatch per thread at a time: the loads
+    // of one batch are in flight together instead of one after another. Loads
+    // past the slab's end repeat its last vertex (a slab has at least one)
+    // and their stores go to the spare LDS slot sv[info.y] (the launch
+    // allocates one slot more than the largest slab), so nothing is
+    // conditional and the compiler keeps the loads together.
+    const int last = info.y - 1;
+    for (int v0 = threadIdx.x; v0 < info.y; v0 += kSlabBatch * blockDim.x) {
+        int g[kSlabBatch];
+        float4 P[kSlabBatch];
+#pragma unroll
+        for (int u = 0; u < kSlabBatch; u++) g[u] = vg[min(v0 + u * (int)blockDim.x, last)] & 0x7fffffff;
+#pragma unroll
+        for (int u = 0; u < kSlabBatch; u++) P[u] = pos[g[u]];
+#pragma unroll
+        for (int u = 0; u < kSlabBatch; u++) {
+            const int v = v0 + u * blockDim.x;
+            sv[v < info.y ? v : info.y] = P[u];
         }
-        sv[v] = P;
+    }
+    if (kc.enabled) {
+        // the owner applies the vertex's contacts (same thread-to-vertex map
+        // as the staging loop, so no barrier is needed in between)
+        for (int v = threadIdx.x; v < info.y; v += blockDim.x) {
+            const int raw = vg[v];
+            if (raw >= 0) continue;
+            const int g = raw & 0x7fffffff;
+            const int n = kc.cb.count[g];
+            if (n == 0) continue;
+            const float4 P = sv[v];
+            f3 p = xyz(P), ve = xyz(kc.vel[g]);
+            vertex_contacts(g, n, p, xyz(kc.prev[g]), ve, kc.cb, kc.thickness, kc.mu, kc.invH);
+            sv[v] = f4(p, P.w);
+            kc.vel[g] = f4(ve, 0.f);
+        }
     }
     __syncthreads();
     return info.w;
@@ -437,7 +469,64 @@ __device__ __forceinline__ void slab_end(const SlabSet& ss, const int4& info, co
 {
     __syncthreads();
     const int* vg = ss.verts + info.x;
-    for (int v = threadIdx.x; v < info.y; v += blockDim.x) pos[vg[v] & 0x7fffffff] = sv[v];
+    const int last = info.y - 1;
+    for (int v0 = threadIdx.x; v0 < info.y; v0 += kSlabBatch * blockDim.x) {
+        int g[kSlabBatch];
+#pragma unroll
+        for (int u = 0; u < kSlabBatch; u++) g[u] = vg[min(v0 + u * (int)blockDim.x, last)] & 0x7fffffff;
+#pragma unroll
+        for (int u = 0; u < kSlabBatch; u++) {
+            const int v = v0 + u * blockDim.x;
+            if (v < info.y) pos[g[u]] = sv[v];
+        }
+    }
+}
+
+// One colour's constraint for this thread, fetched a colour ahead. Indices
+// past the colour are clamped to the slab's last constraint (`last`), so the
+// loads are unconditional and stay in flight across the LDS barrier; `have`
+// says whether the thread has a constraint in the colour.
+struct FemItem {
+    ushort4 id;
+    float4 A, B;
+    float2 C;
+    int t;
+    bool have;
+};
+
+__device__ __forceinline__ FemItem fem_fetch(TriStreams tr, const ushort4* __restrict__ lidx, int t, int end, int last)
+{
+    FemItem f;
+    const int u = min(t, last);
+    f.id = lidx[u];
+    f.A = tr.a[u];
+    f.B = tr.b[u];
+    f.C = tr.c[u];
+    f.t = t;
+    f.have = t < end;
+    return f;
+}
+
+// The thread's constraints of one colour ending at `e`: the prefetched one,
+// then (a colour wider than the workgroup) the rest, in order.
+__device__ __forceinline__ void fem_colour(const FemItem& f, int e, float4* sv, TriStreams tr,
+                                           const ushort4* __restrict__ lidx, float h2, float epsUV)
+{
+    if (f.have) {
+        float4 P1 = sv[f.id.x], P2 = sv[f.id.y], P3 = sv[f.id.z];
+        fem_project(P1, P2, P3, f.A, f.B, f.C, h2, epsUV);
+        sv[f.id.x] = P1;
+        sv[f.id.y] = P2;
+        sv[f.id.z] = P3;
+    }
+    for (int t2 = f.t + blockDim.x; t2 < e; t2 += blockDim.x) {
+        const ushort4 i2 = lidx[t2];
+        float4 P1 = sv[i2.x], P2 = sv[i2.y], P3 = sv[i2.z];
+        fem_project(P1, P2, P3, tr.a[t2], tr.b[t2], tr.c[t2], h2, epsUV);
+        sv[i2.x] = P1;
+        sv[i2.y] = P2;
+        sv[i2.z] = P3;
+    }
 }
 
 __global__ void __launch_bounds__(kSlabBlock) k_fem_slab(float4* __restrict__ pos, TriStreams tr,
@@ -450,42 +539,61 @@ __global__ void __launch_bounds__(kSlabBlock) k_fem_slab(float4* __restrict__ po
     __shared__ int scol[kMaxSlabColours + 1];
     int4 info;
     const int nc = slab_begin(ss, info, scol, sv, pos, kc);
-    int t = scol[0] + threadIdx.x;
-    bool have = t < scol[1];
-    ushort4 id = make_ushort4(0, 0, 0, 0);
-    float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = A;
-    float2 C = make_float2(0.f, 0.f);
-    if (have) { id = lidx[t]; A = tr.a[t]; B = tr.b[t]; C = tr.c[t]; }
-    for (int c = 0; c < nc; c++) {
-        const int e = scol[c + 1];
-        const int tn = e + threadIdx.x;
-        const bool haveN = c + 1 < nc && tn < scol[c + 2];
-        ushort4 idN = id;
-        float4 AN = A, BN = B;
-        float2 CN = C;
-        if (haveN) { idN = lidx[tn]; AN = tr.a[tn]; BN = tr.b[tn]; CN = tr.c[tn]; }
-        if (have) {
-            float4 P1 = sv[id.x], P2 = sv[id.y], P3 = sv[id.z];
-            fem_project(P1, P2, P3, A, B, C, h2, epsUV);
-            sv[id.x] = P1;
-            sv[id.y] = P2;
-            sv[id.z] = P3;
-        }
-        // a colour wider than the workgroup: remaining constraints, in order
-        for (int t2 = t + blockDim.x; t2 < e; t2 += blockDim.x) {
-            const ushort4 i2 = lidx[t2];
-            float4 P1 = sv[i2.x], P2 = sv[i2.y], P3 = sv[i2.z];
-            fem_project(P1, P2, P3, tr.a[t2], tr.b[t2], tr.c[t2], h2, epsUV);
-            sv[i2.x] = P1;
-            sv[i2.y] = P2;
-            sv[i2.z] = P3;
-        }
-        __syncthreads();
-        t = tn;
-        have = haveN;
-        id = idN; A = AN; B = BN; C = CN;
+    const int last = scol[nc] - 1;
+    // two colours per trip with the two items in turn: no register copies
+    // of fetched data (a copy would wait for the load before the barrier)
+    FemItem a = fem_fetch(tr, lidx, scol[0] + threadIdx.x, scol[1], last), b;
+    for (int c = 0; c < nc; c += 2) {
+        const bool n1 = c + 1 < nc;
+        b = fem_fetch(tr, lidx, scol[c + 1] + threadIdx.x, n1 ? scol[c + 2] : 0, last);
+        fem_colour(a, scol[c + 1], sv, tr, lidx, h2, epsUV);
+        lds_barrier();
+        if (!n1) break;
+        const bool n2 = c + 2 < nc;
+        a = fem_fetch(tr, lidx, scol[c + 2] + threadIdx.x, n2 ? scol[c + 3] : 0, last);
+        fem_colour(b, scol[c + 2], sv, tr, lidx, h2, epsUV);
+        lds_barrier();
     }
     slab_end(ss, info, sv, pos);
+}
+
+struct QuadItem {
+    ushort4 id;
+    int t;
+    bool have;
+};
+
+__device__ __forceinline__ QuadItem quad_fetch(const ushort4* __restrict__ lidx, int t, int end, int last)
+{
+    QuadItem f;
+    f.id = lidx[min(t, last)];
+    f.t = t;
+    f.have = t < end;
+    return f;
+}
+
+__device__ __forceinline__ void bend_colour(const QuadItem& f, int e, float4* sv, const ushort4* __restrict__ lidx,
+                                            float sStiff, float& err)
+{
+    if (f.have) {
+        float4 P1 = sv[f.id.x], P2 = sv[f.id.y], P3 = sv[f.id.z], P4 = sv[f.id.w];
+        const float e1 = bend_project(P1, P2, P3, P4, sStiff);
+        err = err < e1 ? e1 : err;
+        sv[f.id.x] = P1;
+        sv[f.id.y] = P2;
+        sv[f.id.z] = P3;
+        sv[f.id.w] = P4;
+    }
+    for (int t2 = f.t + blockDim.x; t2 < e; t2 += blockDim.x) {
+        const ushort4 i2 = lidx[t2];
+        float4 P1 = sv[i2.x], P2 = sv[i2.y], P3 = sv[i2.z], P4 = sv[i2.w];
+        const float e1 = bend_project(P1, P2, P3, P4, sStiff);
+        err = err < e1 ? e1 : err;
+        sv[i2.x] = P1;
+        sv[i2.y] = P2;
+        sv[i2.z] = P3;
+        sv[i2.w] = P4;
+    }
 }
 
 __global__ void __launch_bounds__(kSlabBlock) k_bend_slab(float4* __restrict__ pos, const ushort4* __restrict__ lidx,
@@ -498,41 +606,61 @@ __global__ void __launch_bounds__(kSlabBlock) k_bend_slab(float4* __restrict__ p
     SlabContacts none;
     none.enabled = 0;
     const int nc = slab_begin(ss, info, scol, sv, pos, none);
+    const int last = scol[nc] - 1;
     float err = 0.f;
-    int t = scol[0] + threadIdx.x;
-    bool have = t < scol[1];
-    ushort4 id = have ? lidx[t] : make_ushort4(0, 0, 0, 0);
-    for (int c = 0; c < nc; c++) {
-        const int e = scol[c + 1];
-        const int tn = e + threadIdx.x;
-        const bool haveN = c + 1 < nc && tn < scol[c + 2];
-        const ushort4 idN = haveN ? lidx[tn] : id;
-        if (have) {
-            float4 P1 = sv[id.x], P2 = sv[id.y], P3 = sv[id.z], P4 = sv[id.w];
-            float e1 = bend_project(P1, P2, P3, P4, sStiff);
-            err = err < e1 ? e1 : err;
-            sv[id.x] = P1;
-            sv[id.y] = P2;
-            sv[id.z] = P3;
-            sv[id.w] = P4;
-        }
-        for (int t2 = t + blockDim.x; t2 < e; t2 += blockDim.x) {
-            const ushort4 i2 = lidx[t2];
-            float4 P1 = sv[i2.x], P2 = sv[i2.y], P3 = sv[i2.z], P4 = sv[i2.w];
-            float e1 = bend_project(P1, P2, P3, P4, sStiff);
-            err = err < e1 ? e1 : err;
-            sv[i2.x] = P1;
-            sv[i2.y] = P2;
-            sv[i2.z] = P3;
-            sv[i2.w] = P4;
-        }
-        __syncthreads();
-        t = tn;
-        have = haveN;
-        id = idN;
+    QuadItem a = quad_fetch(lidx, scol[0] + threadIdx.x, scol[1], last), b;
+    for (int c = 0; c < nc; c += 2) {
+        const bool n1 = c + 1 < nc;
+        b = quad_fetch(lidx, scol[c + 1] + threadIdx.x, n1 ? scol[c + 2] : 0, last);
+        bend_colour(a, scol[c + 1], sv, lidx, sStiff, err);
+        lds_barrier();
+        if (!n1) break;
+        const bool n2 = c + 2 < nc;
+        a = quad_fetch(lidx, scol[c + 2] + threadIdx.x, n2 ? scol[c + 3] : 0, last);
+        bend_colour(b, scol[c + 2], sv, lidx, sStiff, err);
+        lds_barrier();
     }
     slab_end(ss, info, sv, pos);
     wave_max_atomic(&ctl->cb, err);
+}
+
+struct LinkItem {
+    ushort4 id;
+    float2 L;
+    int t;
+    bool have;
+};
+
+__device__ __forceinline__ LinkItem link_fetch(const ushort4* __restrict__ lidx, const float2* __restrict__ lrest,
+                                               int t, int end, int last)
+{
+    LinkItem f;
+    const int u = min(t, last);
+    f.id = lidx[u];
+    f.L = lrest[u];
+    f.t = t;
+    f.have = t < end;
+    return f;
+}
+
+__device__ __forceinline__ void link_colour(const LinkItem& f, int e, float4* sv, const ushort4* __restrict__ lidx,
+                                            const float2* __restrict__ lrest, float k, float& err)
+{
+    if (f.have) {
+        float4 P1 = sv[f.id.x], P2 = sv[f.id.y];
+        const float e1 = link_project(P1, P2, f.L, k);
+        err = err < e1 ? e1 : err;
+        sv[f.id.x] = P1;
+        sv[f.id.y] = P2;
+    }
+    for (int t2 = f.t + blockDim.x; t2 < e; t2 += blockDim.x) {
+        const ushort4 i2 = lidx[t2];
+        float4 P1 = sv[i2.x], P2 = sv[i2.y];
+        const float e1 = link_project(P1, P2, lrest[t2], k);
+        err = err < e1 ? e1 : err;
+        sv[i2.x] = P1;
+        sv[i2.y] = P2;
+    }
 }
 
 // Slab schedule for links: as k_bend_slab with two vertices per constraint.
@@ -547,18 +675,19 @@ __global__ void __launch_bounds__(kSlabBlock) k_link_slab(float4* __restrict__ p
     SlabContacts none;
     none.enabled = 0;
     const int nc = slab_begin(ss, info, scol, sv, pos, none);
+    const int last = scol[nc] - 1;
     float err = 0.f;
-    for (int c = 0; c < nc; c++) {
-        const int e = scol[c + 1];
-        for (int t = scol[c] + threadIdx.x; t < e; t += blockDim.x) {
-            const ushort4 id = lidx[t];
-            float4 P1 = sv[id.x], P2 = sv[id.y];
-            const float e1 = link_project(P1, P2, lrest[t], k);
-            err = err < e1 ? e1 : err;
-            sv[id.x] = P1;
-            sv[id.y] = P2;
-        }
-        __syncthreads();
+    LinkItem a = link_fetch(lidx, lrest, scol[0] + threadIdx.x, scol[1], last), b;
+    for (int c = 0; c < nc; c += 2) {
+        const bool n1 = c + 1 < nc;
+        b = link_fetch(lidx, lrest, scol[c + 1] + threadIdx.x, n1 ? scol[c + 2] : 0, last);
+        link_colour(a, scol[c + 1], sv, lidx, lrest, k, err);
+        lds_barrier();
+        if (!n1) break;
+        const bool n2 = c + 2 < nc;
+        a = link_fetch(lidx, lrest, scol[c + 2] + threadIdx.x, n2 ? scol[c + 3] : 0, last);
+        link_colour(b, scol[c + 2], sv, lidx, lrest, k, err);
+        lds_barrier();
     }
     slab_end(ss, info, sv, pos);
     wave_max_atomic(&ctl->cl, err);
@@ -873,7 +1002,7 @@ void launch_fem_slab(hipStream_t s, float4* pos, TriStreams tr, const ushort4* l
                      int maxVerts, float h2, float epsUV, const SolverCtl* ctl, const SlabContacts& kc)
 {
     if (numSlabs <= 0) return;
-    hipLaunchKernelGGL(k_fem_slab, dim3(numSlabs), dim3(kSlabBlock), maxVerts * sizeof(float4), s, pos, tr, lidx, ss,
+    hipLaunchKernelGGL(k_fem_slab, dim3(numSlabs), dim3(kSlabBlock), (maxVerts + 1) * sizeof(float4), s, pos, tr, lidx, ss,
                        h2, epsUV, ctl, kc);
 }
 
@@ -897,7 +1026,7 @@ void launch_bend_slab(hipStream_t s, float4* pos, const ushort4* lidx, SlabSet s
                       float stiff, SolverCtl* ctl)
 {
     if (numSlabs <= 0) return;
-    hipLaunchKernelGGL(k_bend_slab, dim3(numSlabs), dim3(kSlabBlock), maxVerts * sizeof(float4), s, pos, lidx, ss,
+    hipLaunchKernelGGL(k_bend_slab, dim3(numSlabs), dim3(kSlabBlock), (maxVerts + 1) * sizeof(float4), s, pos, lidx, ss,
                        stiff, ctl);
 }
 
@@ -933,7 +1062,7 @@ void launch_link_slab(hipStream_t s, float4* pos, const ushort4* lidx, const flo
                       int maxVerts, float k, SolverCtl* ctl)
 {
     if (numSlabs <= 0) return;
-    hipLaunchKernelGGL(k_link_slab, dim3(numSlabs), dim3(kSlabBlock), maxVerts * sizeof(float4), s, pos, lidx, lrest,
+    hipLaunchKernelGGL(k_link_slab, dim3(numSlabs), dim3(kSlabBlock), (maxVerts + 1) * sizeof(float4), s, pos, lidx, lrest,
                        ss, k, ctl);
 }
 

@@ -201,40 +201,20 @@ Schedule schedule_slabs(int n, int arity, const uint32_t* verts, int nv, const f
 
 Schedule schedule_slabs_auto(int n, int arity, const uint32_t* verts, int nv, const f3* rest, int maxVerts)
 {
-    // largest first: at equal cost fewer, fuller workgroups gather and
-    // scatter fewer shared vertices
-    const int sizes[] = {8192, 4096, 2048, 1024, 512, 256};
-    Schedule best;
-    long bestCost = -1;
-    int bestPhases = 1 << 30;
+    // A colour costs about one projection's latency whatever its size, while
+    // a larger slab gets more local colours and keeps more wavefronts busy
+    // on LDS per colour, and a smaller one needs more phases (launches).
+    // Measured on the 256^2 drape (FEM and bends, sizes 256 to 4096), 1024
+    // constraints per slab was fastest for both; smaller sizes are the
+    // fallback when a slab's vertices would not fit the LDS budget.
+    const int sizes[] = {1024, 512, 256};
     for (int sz : sizes) {
-        Schedule s;
         try {
-            s = schedule_slabs(n, arity, verts, nv, rest, sz, maxVerts);
+            return schedule_slabs(n, arity, verts, nv, rest, sz, maxVerts);
         } catch (const std::runtime_error&) {
-            continue;
-        }
-        // critical path of one pass: per phase, the slowest slab's number of
-        // barrier-separated rounds of 256 constraints
-        long cost = 0;
-        for (int p = 0; p < s.numPhases(); p++) {
-            long worst = 0;
-            for (int sl = s.phaseSlabBegin[p]; sl < s.phaseSlabBegin[p + 1]; sl++) {
-                long rounds = 0;
-                for (int k = s.slabColBegin[sl]; k < s.slabColBegin[sl + 1]; k++)
-                    rounds += (s.colOffsets[k + 1] - s.colOffsets[k] + 255) / 256;
-                worst = std::max(worst, rounds);
-            }
-            cost += worst + 8;  // a phase launch costs about as much as a few rounds
-        }
-        if (bestCost < 0 || s.numPhases() < bestPhases || (s.numPhases() == bestPhases && cost < bestCost)) {
-            best = std::move(s);
-            bestCost = cost;
-            bestPhases = best.numPhases();
         }
     }
-    if (bestCost < 0) throw std::runtime_error("no slab size fits the LDS budget");
-    return best;
+    throw std::runtime_error("no slab size fits the LDS budget");
 }
 
 }  // namespace sc

@@ -52,8 +52,8 @@ Schedule schedule_colours(int numConstraints, int arity, const uint32_t* verts, 
 Schedule schedule_slabs(int numConstraints, int arity, const uint32_t* verts, int numVertices, const f3* rest,
                         int slabSize, int maxVerts);
 
-// Largest slab size from {4096, 2048, 1024, 512, 256} whose schedule needs the
-// fewest phases (two for a strip-like cloth) and fits maxVerts.
+// Slabs of 1024 constraints (512 or 256 when the vertices of a slab would
+// exceed maxVerts).
 Schedule schedule_slabs_auto(int numConstraints, int arity, const uint32_t* verts, int numVertices,
                              const f3* rest, int maxVerts);
 

@@ -122,8 +122,9 @@ def test_schedule_is_race_free(sc, n, mode):
         # greedy colouring reaches it. Bends need at least 16.
         assert nt == 8 and 16 <= nb <= 20
     else:
-        # a strip of slabs along the cloth needs two phases (one if it fits one slab)
-        assert nt <= 2 and nb <= 2
+        # a strip of slabs along the cloth needs two phases (one if it fits one
+        # slab); thin bend slabs (a bend spans three grid rows) may need three
+        assert nt <= 2 and nb <= 3
         ph, sl, co = m.schedule("bend")
         assert co.max() + 1 <= 20
 
