@@ -42,8 +42,10 @@ struct SolverCtl {
     unsigned sce;    // self edge-edge error of the current iteration
     int selfCount[2];    // self contacts found this substep (vertex-triangle, edge-edge)
     int selfColours[2];  // colour groups of each kind
-    int selfOverflow;    // contacts beyond capacity, or colouring rounds exhausted
+    int selfOverflow;    // contacts or candidates beyond capacity, or colouring rounds exhausted
     unsigned cl;         // link error of the current iteration (link model)
+    int candCount[2];    // largest number of self candidate pairs of one query (vertex, edge)
+    int candExtra[2];    // pairs beyond a query's slots, in the shared pools (vertex, edge)
 };
 
 enum { MAX_COLLIDABLES = 8 };
@@ -243,12 +245,27 @@ void launch_tri_contacts(hipStream_t s, float4* pos, const float4* prev, const i
 // Self collision (self.hip): swept boxes of triangles and edges, continuous
 // vertex-triangle and edge-edge detection over the LBVHs, colouring of the
 // contacts, and their Gauss-Seidel passes.
+// (with vtxLo non-null also the vertices' swept boxes)
 void launch_self_boxes(hipStream_t s, const float4* pos, const float4* prev, const int4* cTris, int nt,
                        const int2* cEdges, int ne, float tol, float4* triLo, float4* triHi, float4* edgeLo,
-                       float4* edgeHi);
+                       float4* edgeHi, int nv, float4* vtxLo, float4* vtxHi);
+// Candidate pairs (triangle, vertex) and (edge1, edge2): query j of a kind
+// owns slots [j k, j k + k) and writes their number to cnt[j]; pairs beyond
+// its slots go to a shared pool (capacity extraCap per kind) through one
+// counter; then the continuous tests over them.
+struct SelfCandidates {
+    int2* vt;
+    int2* ee;
+    int* cntVT;
+    int* cntEE;
+    int2* extraVT;
+    int2* extraEE;
+    int kVT, kEE, extraCap;
+};
+// vertexOrder: the order in which vertices query the triangle tree (may be null)
 void launch_self_detect(hipStream_t s, const float4* pos, const float4* prev, const int4* cTris, const int2* cEdges,
-                        int nv, int ne, BvhView triBvh, BvhView edgeBvh, SelfParams sp, SelfBuf vt, SelfBuf ee,
-                        SolverCtl* ctl);
+                        int nv, int ne, BvhView triBvh, BvhView edgeBvh, const int* vertexOrder, SelfParams sp,
+                        SelfCandidates cand, SelfBuf vt, SelfBuf ee, SolverCtl* ctl);
 void launch_self_colour(hipStream_t s, SelfBuf b, int kind, unsigned long long* vmax, SolverCtl* ctl);
 void launch_self_solve(hipStream_t s, float4* pos, SelfBuf b, int kind, SelfParams sp, float h, SolverCtl* ctl);
 

@@ -34,6 +34,10 @@ __global__ void k_predict(float4* __restrict__ pos, float4* __restrict__ prev, f
         ctl->cl = 0;
         ctl->selfCount[0] = 0;
         ctl->selfCount[1] = 0;
+        ctl->candCount[0] = 0;
+        ctl->candCount[1] = 0;
+        ctl->candExtra[0] = 0;
+        ctl->candExtra[1] = 0;
         ctl->iterations = 0;
     }
     if (i >= n) return;
@@ -137,6 +141,10 @@ __global__ void k_predict_detect(float4* __restrict__ pos, float4* __restrict__ 
         ctl->cl = 0;
         ctl->selfCount[0] = 0;
         ctl->selfCount[1] = 0;
+        ctl->candCount[0] = 0;
+        ctl->candCount[1] = 0;
+        ctl->candExtra[0] = 0;
+        ctl->candExtra[1] = 0;
         ctl->iterations = 0;
     }
     if (i >= n) return;

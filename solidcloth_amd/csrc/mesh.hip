@@ -32,6 +32,10 @@ __global__ void __launch_bounds__(kBlock) k_predict_mesh(float4* __restrict__ po
         ctl->cl = 0;
         ctl->selfCount[0] = 0;
         ctl->selfCount[1] = 0;
+        ctl->candCount[0] = 0;
+        ctl->candCount[1] = 0;
+        ctl->candExtra[0] = 0;
+        ctl->candExtra[1] = 0;
         ctl->iterations = 0;
     }
     if (i < ec.groups) ec.groupCount[i] = 0;

@@ -285,6 +285,9 @@ void GpuSolver::release()
         f->slotCap = 0;
     }
     dev_free(self.triLo); dev_free(self.triHi); dev_free(self.edgeLo); dev_free(self.edgeHi); dev_free(self.vmax);
+    dev_free(self.candVT); dev_free(self.candEE); dev_free(self.vtxLo); dev_free(self.vtxHi);
+    dev_free(self.candCntVT); dev_free(self.candCntEE); dev_free(self.candExtraVT); dev_free(self.candExtraEE);
+    self.candKVT = self.candKEE = self.candExtraCap = 0;
     for (SelfBuf& b : self.buf) {
         dev_free(b.ids); dev_free(b.W); dev_free(b.N); dev_free(b.key); dev_free(b.colour); dev_free(b.order);
         dev_free(b.active); dev_free(b.groupOff);
@@ -645,16 +648,37 @@ void ClothModel::ensure_self()
         S.cap = cap;
         g.release_graphs();
     }
+    // candidate slots per query: 96 for vertices and for edges (the 512^2
+    // perturbed cloth with 2 mm thickness has about 65 and 53 on average, a
+    // few hundred where it folds); pairs beyond them go to shared pools of
+    // 16 Mi pairs per kind, grown when a step came near their end
+    const int kVT = 96, kEE = 96;
+    const int extraCap = std::max(16 << 20, std::max(S.candWantVT, S.candWantEE));
+    if (kVT != S.candKVT || kEE != S.candKEE || extraCap != S.candExtraCap) {
+        dev_alloc(S.candVT, (size_t)kVT * std::max(g.nv, 1));
+        dev_alloc(S.candEE, (size_t)kEE * std::max(g.ne, 1));
+        dev_alloc(S.candCntVT, std::max(g.nv, 1));
+        dev_alloc(S.candCntEE, std::max(g.ne, 1));
+        dev_alloc(S.candExtraVT, extraCap);
+        dev_alloc(S.candExtraEE, extraCap);
+        S.candKVT = kVT;
+        S.candKEE = kEE;
+        S.candExtraCap = extraCap;
+        g.release_graphs();
+    }
     if (!S.triLo) {
         dev_alloc(S.triLo, std::max(g.nt, 1)); dev_alloc(S.triHi, std::max(g.nt, 1));
         dev_alloc(S.edgeLo, std::max(g.ne, 1)); dev_alloc(S.edgeHi, std::max(g.ne, 1));
+        dev_alloc(S.vtxLo, std::max(g.nv, 1)); dev_alloc(S.vtxHi, std::max(g.nv, 1));
         dev_alloc(S.vmax, std::max(g.nv, 1));
     }
-    // hierarchy from the current configuration; substeps refit it
+    // hierarchy from the current configuration; substeps refit it (the
+    // vertex tree only orders the vertex queries, so it is not refit)
     launch_self_boxes(g.stream, g.d_pos, g.d_prev, g.d_cTris, g.nt, g.d_cEdges, g.ne, mThickness + mTolerance * 0.5f,
-                      S.triLo, S.triHi, S.edgeLo, S.edgeHi);
+                      S.triLo, S.triHi, S.edgeLo, S.edgeHi, g.nv, S.vtxLo, S.vtxHi);
     S.triB.build(g.stream, S.triLo, S.triHi, g.nt);
     S.edgeB.build(g.stream, S.edgeLo, S.edgeHi, g.ne);
+    S.vtxB.build(g.stream, S.vtxLo, S.vtxHi, g.nv);
     S.built = true;
 }
 
@@ -759,6 +783,11 @@ std::string ClothModel::graph_key(float h, const CollSet& cs) const
     const BvhView tv = gpu.self.triB.view(), ev = gpu.self.edgeB.view();
     put(&tv, sizeof(tv)); put(&ev, sizeof(ev));
     put(&gpu.self.buf[0].ids, sizeof(void*)); put(&gpu.self.buf[1].ids, sizeof(void*));
+    put(&gpu.self.candVT, sizeof(void*)); put(&gpu.self.candEE, sizeof(void*)); put(&gpu.self.candKVT, 4);
+    put(&gpu.self.candKEE, 4); put(&gpu.self.candCntVT, sizeof(void*)); put(&gpu.self.candExtraVT, sizeof(void*));
+    put(&gpu.self.candExtraCap, 4);
+    const BvhView vv = gpu.self.vtxB.view();
+    put(&vv, sizeof(vv));
     const int wmode = mWarmPlainNow ? 0 : mWarmMode;
     put(&mWarm, sizeof(void*)); put(&wmode, 4); put(&mWarmIters, 4);
     put(&gpu.warm.corr, sizeof(void*)); put(&gpu.warm.X, sizeof(void*)); put(&gpu.warm.triStrain, sizeof(void*));
@@ -830,11 +859,14 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
         if (selfColl) {
             GpuSolver::Self& S = g.self;
             launch_self_boxes(s, g.d_pos, g.d_prev, g.d_cTris, g.nt, g.d_cEdges, g.ne, sp.tol, S.triLo, S.triHi,
-                              S.edgeLo, S.edgeHi);
+                              S.edgeLo, S.edgeHi, g.nv, nullptr, nullptr);
             S.triB.refit(s, S.triLo, S.triHi);
             S.edgeB.refit(s, S.edgeLo, S.edgeHi);
             launch_self_detect(s, g.d_pos, g.d_prev, g.d_cTris, g.d_cEdges, g.nv, g.ne, S.triB.view(),
-                               S.edgeB.view(), sp, S.buf[0], S.buf[1], g.d_ctl);
+                               S.edgeB.view(), S.vtxB.view().leafPrim, sp,
+                               SelfCandidates{S.candVT, S.candEE, S.candCntVT, S.candCntEE, S.candExtraVT,
+                                              S.candExtraEE, S.candKVT, S.candKEE, S.candExtraCap},
+                               S.buf[0], S.buf[1], g.d_ctl);
             launch_self_colour(s, S.buf[0], 0, S.vmax, g.d_ctl);
             launch_self_colour(s, S.buf[1], 1, S.vmax, g.d_ctl);
         }
@@ -1065,9 +1097,16 @@ void ClothModel::step_substeps(float h, int n)
     if (mCollFlags & CF_SELF) {
         SolverCtl c;
         HIP_OK(hipMemcpy(&c, g.d_ctl, sizeof(c), hipMemcpyDeviceToHost));
+        GpuSolver::Self& S = g.self;
+        if (c.candExtra[0] > S.candExtraCap / 5 * 4) S.candWantVT = (int)std::min<long>(1 << 30, 2L * c.candExtra[0]);
+        if (c.candExtra[1] > S.candExtraCap / 5 * 4) S.candWantEE = (int)std::min<long>(1 << 30, 2L * c.candExtra[1]);
         if (c.selfOverflow)
             throw std::runtime_error(
-                "self collision overflowed in this step: last substep found " + std::to_string(c.selfCount[0]) +
+                "self collision overflowed in this step: the last substep put " + std::to_string(c.candExtra[0]) +
+                " vertex-triangle and " + std::to_string(c.candExtra[1]) +
+                " edge-edge candidate pairs beyond the per-query slots (pool capacity " +
+                std::to_string(S.candExtraCap) + ", grown for the next step) and found " +
+                std::to_string(c.selfCount[0]) +
                 " vertex-triangle and " + std::to_string(c.selfCount[1]) + " edge-edge contacts (capacity " +
                 std::to_string(g.self.cap) + ", raise with scSetSelfCapacity) in " +
                 std::to_string(c.selfColours[0]) + " / " + std::to_string(c.selfColours[1]) + " colour groups (at most " +

@@ -230,9 +230,15 @@ public:
     // contact lists of both kinds, per-vertex colouring bids
     struct Self {
         float4 *triLo = nullptr, *triHi = nullptr, *edgeLo = nullptr, *edgeHi = nullptr;
-        Lbvh triB, edgeB;
+        float4 *vtxLo = nullptr, *vtxHi = nullptr;
+        Lbvh triB, edgeB, vtxB;  // vtxB: Morton order of the vertex queries (built per step)
         SelfBuf buf[2]{};
         unsigned long long* vmax = nullptr;
+        int2 *candVT = nullptr, *candEE = nullptr;  // candidate pairs of the traversal, per query
+        int *candCntVT = nullptr, *candCntEE = nullptr;
+        int2 *candExtraVT = nullptr, *candExtraEE = nullptr;
+        int candExtraCap = 0;
+        int candKVT = 0, candKEE = 0, candWantVT = 0, candWantEE = 0;  // slots per query
         int cap = 0;
         bool built = false;
     } self;

@@ -192,9 +192,19 @@ __device__ __forceinline__ void swept(f3 p, f3& lo, f3& hi)
 __global__ void __launch_bounds__(kBlock) k_self_boxes(const float4* __restrict__ pos, const float4* __restrict__ prev,
                                                        const int4* __restrict__ cTris, int nt,
                                                        const int2* __restrict__ cEdges, int ne, float tol,
-                                                       float4* triLo, float4* triHi, float4* edgeLo, float4* edgeHi)
+                                                       float4* triLo, float4* triHi, float4* edgeLo, float4* edgeHi,
+                                                       int nv, float4* vtxLo, float4* vtxHi)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (vtxLo && i >= nt + ne && i < nt + ne + nv) {
+        const int v = i - nt - ne;
+        f3 lo{1e7f, 1e7f, 1e7f}, hi{-1e7f, -1e7f, -1e7f};
+        swept(xyz(prev[v]), lo, hi);
+        swept(xyz(pos[v]), lo, hi);
+        vtxLo[v] = make_float4(lo.x - tol, lo.y - tol, lo.z - tol, 0.f);
+        vtxHi[v] = make_float4(hi.x + tol, hi.y + tol, hi.z + tol, 0.f);
+        return;
+    }
     f3 lo{1e7f, 1e7f, 1e7f}, hi{-1e7f, -1e7f, -1e7f};
     if (i < nt) {
         const int4 t = cTris[i];
@@ -282,39 +292,60 @@ __device__ bool edge_edge(const float4* pos, const float4* prev, int i1, int i2,
     return true;
 }
 
-// SelfCollisionDetector.cpp:60-135, 137-190, 23-58 over the exact candidate
-// set: blocks [0, vertexBlocks) take vertices (against the triangle tree),
-// the rest edges (against the edge tree, partner index greater).
-__global__ void __launch_bounds__(kBlock) k_self_detect(const float4* __restrict__ pos,
-                                                        const float4* __restrict__ prev,
-                                                        const int4* __restrict__ cTris,
-                                                        const int2* __restrict__ cEdges, int nv, int ne,
-                                                        int vertexBlocks, BvhView triBvh, BvhView edgeBvh,
-                                                        SelfParams sp, SelfBuf vt, SelfBuf ee, SolverCtl* ctl)
+// SelfCollisionDetector.cpp:60-135 (Midphase) over the exact candidate set:
+// blocks [0, vertexBlocks) take vertices (against the triangle tree, own
+// triangles and pinned vertices skipped), the rest edges (against the edge
+// tree, partner index greater, no shared vertex). Queries run in Morton
+// order (the trees' leaf orders), so that neighbouring threads walk
+// neighbouring parts of a tree. Query j writes its pairs to its own slots
+// [j K, j K + K) and their number to cnt[j]: no shared counter. The
+// continuous tests run in k_self_ccd, so that the traversal's divergence and
+// the tests' FP64 registers do not meet in one kernel.
+__global__ void __launch_bounds__(kBlock) k_self_candidates(const float4* __restrict__ pos,
+                                                            const float4* __restrict__ prev,
+                                                            const int4* __restrict__ cTris,
+                                                            const int2* __restrict__ cEdges, int nv, int ne,
+                                                            int vertexBlocks, BvhView triBvh, BvhView edgeBvh,
+                                                            const int* __restrict__ vertexOrder, float tol,
+                                                            SelfCandidates cand, SolverCtl* ctl)
 {
-    const float tol = sp.tol;
-    if ((int)blockIdx.x < vertexBlocks) {
-        const int v = blockIdx.x * blockDim.x + threadIdx.x;
-        if (v >= nv) return;
+    const bool vertexPass = (int)blockIdx.x < vertexBlocks;
+    const int j = (vertexPass ? blockIdx.x : blockIdx.x - vertexBlocks) * blockDim.x + threadIdx.x;
+    if (j >= (vertexPass ? nv : ne)) return;
+    const int K = vertexPass ? cand.kVT : cand.kEE;
+    int2* out = (vertexPass ? cand.vt : cand.ee) + (size_t)j * K;
+    int2* extra = vertexPass ? cand.extraVT : cand.extraEE;
+    int* extraCount = &ctl->candExtra[vertexPass ? 0 : 1];
+    int n = 0;
+    auto emit = [&](int2 p) {
+        if (n < K) {
+            out[n] = p;
+        } else {
+            const int x = atomicAdd(extraCount, 1);
+            if (x < cand.extraCap) extra[x] = p;
+            else atomicExch(&ctl->selfOverflow, 1);
+        }
+        n++;
+    };
+    if (vertexPass) {
+        const int v = vertexOrder ? vertexOrder[j] : j;
         const float4 P = pos[v];
-        if (P.w == 0.f) return;
-        f3 lo{1e7f, 1e7f, 1e7f}, hi{-1e7f, -1e7f, -1e7f};
-        swept(xyz(prev[v]), lo, hi);
-        swept(xyz(P), lo, hi);
-        lo = f3{lo.x - tol, lo.y - tol, lo.z - tol};
-        hi = f3{hi.x + tol, hi.y + tol, hi.z + tol};
-        bvh_overlap(triBvh, lo, hi, [&](int leaf) {
-            const int t = __float_as_int(triBvh.leafLo[leaf].w);
-            const int4 tv = cTris[t];
-            if (v == tv.x || v == tv.y || v == tv.z) return;
-            float4 W, N;
-            if (vertex_triangle(pos, prev, tv.x, tv.y, tv.z, v, sp, W, N))
-                self_append(vt, 0, ctl, make_int4(tv.x, tv.y, tv.z, v), W, N,
-                            ((unsigned long long)t << 32) | (unsigned)v);
-        });
+        if (P.w != 0.f) {
+            f3 lo{1e7f, 1e7f, 1e7f}, hi{-1e7f, -1e7f, -1e7f};
+            swept(xyz(prev[v]), lo, hi);
+            swept(xyz(P), lo, hi);
+            lo = f3{lo.x - tol, lo.y - tol, lo.z - tol};
+            hi = f3{hi.x + tol, hi.y + tol, hi.z + tol};
+            bvh_overlap(triBvh, lo, hi, [&](int leaf) {
+                const int t = __float_as_int(triBvh.leafLo[leaf].w);
+                const int4 tv = cTris[t];
+                if (v == tv.x || v == tv.y || v == tv.z) return;
+                emit(make_int2(t, v));
+            });
+        }
+        cand.cntVT[j] = n;
     } else {
-        const int e1 = (blockIdx.x - vertexBlocks) * blockDim.x + threadIdx.x;
-        if (e1 >= ne) return;
+        const int e1 = edgeBvh.leafPrim[j];
         const int2 ev = cEdges[e1];
         f3 lo{1e7f, 1e7f, 1e7f}, hi{-1e7f, -1e7f, -1e7f};
         swept(xyz(prev[ev.x]), lo, hi);
@@ -328,11 +359,62 @@ __global__ void __launch_bounds__(kBlock) k_self_detect(const float4* __restrict
             if (e2 <= e1) return;
             const int2 fv = cEdges[e2];
             if (ev.x == fv.x || ev.y == fv.y || ev.x == fv.y || ev.y == fv.x) return;
-            float4 W, N;
-            if (edge_edge(pos, prev, ev.x, ev.y, fv.x, fv.y, sp, W, N))
-                self_append(ee, 1, ctl, make_int4(ev.x, ev.y, fv.x, fv.y), W, N,
-                            ((unsigned long long)e1 << 32) | (unsigned)e2);
+            emit(make_int2(e1, e2));
         });
+        cand.cntEE[j] = n;
+    }
+    // the largest count per kind sizes the slots of later steps
+    atomicMax(&ctl->candCount[vertexPass ? 0 : 1], n);
+}
+
+__device__ __forceinline__ void ccd_vt(const float4* pos, const float4* prev, const int4* cTris, int2 c,
+                                       const SelfParams& sp, SelfBuf vt, SolverCtl* ctl)
+{
+    const int4 tv = cTris[c.x];
+    float4 W, N;
+    if (vertex_triangle(pos, prev, tv.x, tv.y, tv.z, c.y, sp, W, N))
+        self_append(vt, 0, ctl, make_int4(tv.x, tv.y, tv.z, c.y), W, N, ((unsigned long long)c.x << 32) | (unsigned)c.y);
+}
+
+__device__ __forceinline__ void ccd_ee(const float4* pos, const float4* prev, const int2* cEdges, int2 c,
+                                       const SelfParams& sp, SelfBuf ee, SolverCtl* ctl)
+{
+    const int2 ev = cEdges[c.x], fv = cEdges[c.y];
+    float4 W, N;
+    if (edge_edge(pos, prev, ev.x, ev.y, fv.x, fv.y, sp, W, N))
+        self_append(ee, 1, ctl, make_int4(ev.x, ev.y, fv.x, fv.y), W, N, ((unsigned long long)c.x << 32) | (unsigned)c.y);
+}
+
+// SelfCollisionDetector.cpp:137-190, 23-58: the continuous tests of the
+// candidate pairs; thread j tests query j's pairs, the workgroups after the
+// queries' the shared pools' pairs (grid-stride, counts read on the device).
+__global__ void __launch_bounds__(kBlock) k_self_ccd(const float4* __restrict__ pos, const float4* __restrict__ prev,
+                                                     const int4* __restrict__ cTris, const int2* __restrict__ cEdges,
+                                                     int nv, int ne, int vertexBlocks, SelfParams sp,
+                                                     SelfCandidates cand, SelfBuf vt, SelfBuf ee, SolverCtl* ctl)
+{
+    const int queryBlocks = vertexBlocks + (ne + kBlock - 1) / kBlock;
+    if ((int)blockIdx.x >= queryBlocks) {
+        const int nx = min(ctl->candExtra[0], cand.extraCap), ny = min(ctl->candExtra[1], cand.extraCap);
+        const int stride = (gridDim.x - queryBlocks) * blockDim.x;
+        for (int i = (blockIdx.x - queryBlocks) * blockDim.x + threadIdx.x; i < nx + ny; i += stride) {
+            if (i < nx) ccd_vt(pos, prev, cTris, cand.extraVT[i], sp, vt, ctl);
+            else ccd_ee(pos, prev, cEdges, cand.extraEE[i - nx], sp, ee, ctl);
+        }
+        return;
+    }
+    const bool vertexPass = (int)blockIdx.x < vertexBlocks;
+    const int j = (vertexPass ? blockIdx.x : blockIdx.x - vertexBlocks) * blockDim.x + threadIdx.x;
+    if (vertexPass) {
+        if (j >= nv) return;
+        const int2* in = cand.vt + (size_t)j * cand.kVT;
+        const int n = min(cand.cntVT[j], cand.kVT);
+        for (int k = 0; k < n; k++) ccd_vt(pos, prev, cTris, in[k], sp, vt, ctl);
+    } else {
+        if (j >= ne) return;
+        const int2* in = cand.ee + (size_t)j * cand.kEE;
+        const int n = min(cand.cntEE[j], cand.kEE);
+        for (int k = 0; k < n; k++) ccd_ee(pos, prev, cEdges, in[k], sp, ee, ctl);
     }
 }
 
@@ -569,21 +651,25 @@ static inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
 
 void launch_self_boxes(hipStream_t s, const float4* pos, const float4* prev, const int4* cTris, int nt,
                        const int2* cEdges, int ne, float tol, float4* triLo, float4* triHi, float4* edgeLo,
-                       float4* edgeHi)
+                       float4* edgeHi, int nv, float4* vtxLo, float4* vtxHi)
 {
-    if (nt + ne <= 0) return;
-    hipLaunchKernelGGL(k_self_boxes, dim3(blocks(nt + ne)), dim3(kBlock), 0, s, pos, prev, cTris, nt, cEdges, ne, tol,
-                       triLo, triHi, edgeLo, edgeHi);
+    const int n = nt + ne + (vtxLo ? nv : 0);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_self_boxes, dim3(blocks(n)), dim3(kBlock), 0, s, pos, prev, cTris, nt, cEdges, ne, tol,
+                       triLo, triHi, edgeLo, edgeHi, nv, vtxLo, vtxHi);
 }
 
 void launch_self_detect(hipStream_t s, const float4* pos, const float4* prev, const int4* cTris, const int2* cEdges,
-                        int nv, int ne, BvhView triBvh, BvhView edgeBvh, SelfParams sp, SelfBuf vt, SelfBuf ee,
-                        SolverCtl* ctl)
+                        int nv, int ne, BvhView triBvh, BvhView edgeBvh, const int* vertexOrder, SelfParams sp,
+                        SelfCandidates cand, SelfBuf vt, SelfBuf ee, SolverCtl* ctl)
 {
     const int vb = blocks(nv), eb = blocks(ne);
     if (vb + eb == 0) return;
-    hipLaunchKernelGGL(k_self_detect, dim3(vb + eb), dim3(kBlock), 0, s, pos, prev, cTris, cEdges, nv, ne, vb, triBvh,
-                       edgeBvh, sp, vt, ee, ctl);
+    hipLaunchKernelGGL(k_self_candidates, dim3(vb + eb), dim3(kBlock), 0, s, pos, prev, cTris, cEdges, nv, ne, vb,
+                       triBvh, edgeBvh, vertexOrder, sp.tol, cand, ctl);
+    // the query workgroups, then 1024 more for the shared pools
+    hipLaunchKernelGGL(k_self_ccd, dim3(vb + eb + 1024), dim3(kBlock), 0, s, pos, prev, cTris, cEdges, nv, ne, vb, sp,
+                       cand, vt, ee, ctl);
 }
 
 void launch_self_colour(hipStream_t s, SelfBuf b, int kind, unsigned long long* vmax, SolverCtl* ctl)
