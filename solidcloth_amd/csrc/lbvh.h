@@ -24,8 +24,15 @@ struct BvhView {
     const float4* leafLo; // leaf boxes in sorted order
     const float4* leafHi;
     const int* leafPrim;  // sorted position -> primitive id
+    // escape ("rope") of each internal node and leaf: the node a depth-first,
+    // left-first walk visits after the subtree (internal c >= 0, leaf ~c,
+    // kRopeEnd after the last); with them an overlap query needs no stack
+    const int* ropeNode;
+    const int* ropeLeaf;
     int n;                // number of primitives (leaves)
 };
+
+constexpr int kRopeEnd = -0x7fffffff - 1;
 
 class Lbvh {
 public:
@@ -54,6 +61,7 @@ private:
     unsigned* bounds_ = nullptr;  // 6 orderable-encoded floats
     int2* range_ = nullptr;       // sorted-leaf range of each internal node
     unsigned* enc_ = nullptr;     // per internal node: order-encoded min (3) / max (3) words
+    int *ropeNode_ = nullptr, *ropeLeaf_ = nullptr;
 };
 
 }  // namespace sc

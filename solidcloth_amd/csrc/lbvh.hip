@@ -253,6 +253,33 @@ __global__ void __launch_bounds__(kBlock) k_karras(const unsigned* __restrict__ 
     if (i == 0) parentNode[0] = -1;
 }
 
+// Escape of every node (threads [0, n-1) internal nodes, [n-1, 2n-1) leaves):
+// climb while the node is its parent's right child; the escape is the right
+// sibling of the first ancestor-or-self that is a left child.
+__global__ void __launch_bounds__(kBlock) k_ropes(const float4* __restrict__ lo, const float4* __restrict__ hi,
+                                                  const int* __restrict__ parentNode,
+                                                  const int* __restrict__ parentLeaf, int n, int* ropeNode,
+                                                  int* ropeLeaf)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= 2 * n - 1) return;
+    const bool leaf = x >= n - 1;
+    const int l = x - (n - 1);
+    int cur = leaf ? ~l : x;
+    int p = leaf ? parentLeaf[l] : parentNode[x];
+    int esc = kRopeEnd;
+    while (p >= 0) {
+        if (__float_as_int(lo[p].w) == cur) {
+            esc = __float_as_int(hi[p].w);
+            break;
+        }
+        cur = p;
+        p = parentNode[p];
+    }
+    if (leaf) ropeLeaf[l] = esc;
+    else ropeNode[x] = esc;
+}
+
 __device__ __forceinline__ void child_box(int c, const float4* lo, const float4* hi, const float4* leafLo,
                                           const float4* leafHi, float4& a, float4& b)
 {
@@ -369,6 +396,7 @@ Lbvh::~Lbvh()
     dev_free(lo_); dev_free(hi_); dev_free(leafLo_); dev_free(leafHi_); dev_free(leafPrim_);
     dev_free(keys_); dev_free(keys2_); dev_free(vals_); dev_free(vals2_); dev_free(hist_);
     dev_free(parentNode_); dev_free(parentLeaf_); dev_free(flags_); dev_free(bounds_); dev_free(range_); dev_free(enc_);
+    dev_free(ropeNode_); dev_free(ropeLeaf_);
 }
 
 void Lbvh::reserve(int n)
@@ -380,6 +408,7 @@ void Lbvh::reserve(int n)
     dev_alloc(hist_, (size_t)256 * tiles);
     dev_alloc(parentNode_, n); dev_alloc(parentLeaf_, n); dev_alloc(flags_, n); dev_alloc(range_, n);
     dev_alloc(enc_, (size_t)6 * n);
+    dev_alloc(ropeNode_, n); dev_alloc(ropeLeaf_, n);
     if (!bounds_) dev_alloc(bounds_, 6);
     cap_ = n;
 }
@@ -410,6 +439,9 @@ void Lbvh::build(hipStream_t s, const float4* boxLo, const float4* boxHi, int n)
                            parentLeaf_, flags_, range_, enc_);
     hipLaunchKernelGGL(k_refit, dim3(nblocks(n, kRefitBlock)), dim3(kRefitBlock), 0, s, boxLo, boxHi, vi, n, lo_, hi_,
                        leafLo_, leafHi_, leafPrim_, parentNode_, parentLeaf_, flags_, range_, enc_);
+    if (n > 1)
+        hipLaunchKernelGGL(k_ropes, dim3(nblocks(2 * n - 1, kBlock)), dim3(kBlock), 0, s, lo_, hi_, parentNode_,
+                           parentLeaf_, n, ropeNode_, ropeLeaf_);
     HIP_OK(hipGetLastError());
 }
 
@@ -468,9 +500,42 @@ int Lbvh::verify(hipStream_t s, const float4* boxLo, const float4* boxHi) const
     }
     for (int k = 0; k < n; k++) bad += visits[k] != 1;
     if (pn[0] != -1) bad++;
+    if (bad) return bad;
+    // escape links against a depth-first, left-first walk on the host
+    std::vector<int> rn(n - 1), rl(n);
+    get(rn, ropeNode_);
+    get(rl, ropeLeaf_);
+    std::vector<int> order;  // tagged ids in walk order
+    order.reserve(2 * n - 1);
+    std::vector<int> stack{0};
+    while (!stack.empty()) {
+        const int x = stack.back();
+        stack.pop_back();
+        order.push_back(x);
+        if (x >= 0) {
+            stack.push_back(__builtin_bit_cast(int, hi[x].w));
+            stack.push_back(__builtin_bit_cast(int, lo[x].w));
+        }
+    }
+    // escape(x) = the first node after x's subtree in that order
+    std::vector<int> sizeInt(n - 1, 0);
+    for (int k = (int)order.size() - 1; k >= 0; k--) {
+        const int x = order[k];
+        if (x < 0) continue;
+        int sz = 1;
+        for (const int c : {__builtin_bit_cast(int, lo[x].w), __builtin_bit_cast(int, hi[x].w)})
+            sz += c < 0 ? 1 : sizeInt[c];
+        sizeInt[x] = sz;
+    }
+    for (int k = 0; k < (int)order.size(); k++) {
+        const int x = order[k];
+        const int after = k + (x < 0 ? 1 : sizeInt[x]);
+        const int expect = after < (int)order.size() ? order[after] : kRopeEnd;
+        if ((x < 0 ? rl[~x] : rn[x]) != expect) bad++;
+    }
     return bad;
 }
 
-BvhView Lbvh::view() const { return BvhView{lo_, hi_, leafLo_, leafHi_, leafPrim_, n_}; }
+BvhView Lbvh::view() const { return BvhView{lo_, hi_, leafLo_, leafHi_, leafPrim_, ropeNode_, ropeLeaf_, n_}; }
 
 }  // namespace sc

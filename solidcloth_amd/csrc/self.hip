@@ -146,7 +146,10 @@ __device__ __forceinline__ bool overlap(f3 alo, f3 ahi, float4 blo, float4 bhi)
 }
 
 // Visits every leaf whose box overlaps [lo, hi] (closed intervals, as
-// Geometry/Aabb3.h:72-87). Karras trees have depth < 64.
+// Geometry/Aabb3.h:72-87), depth-first and left-first along the escape
+// links: a node whose box overlaps is entered through its left child, any
+// other node (and every leaf) passes to its escape. No stack, one box load
+// per step.
 template <class Visit>
 __device__ __forceinline__ void bvh_overlap(const BvhView& bv, f3 lo, f3 hi, Visit visit)
 {
@@ -155,29 +158,22 @@ __device__ __forceinline__ void bvh_overlap(const BvhView& bv, f3 lo, f3 hi, Vis
         if (overlap(lo, hi, bv.leafLo[0], bv.leafHi[0])) visit(0);
         return;
     }
-    if (!overlap(lo, hi, bv.lo[0], bv.hi[0])) return;
-    int stack[64];
-    int sp = 0;
     int node = 0;
-    while (true) {
-        const float4 L = bv.lo[node], H = bv.hi[node];
-        const int c[2] = {__float_as_int(L.w), __float_as_int(H.w)};
-        int next = -1;
-#pragma unroll
-        for (int k = 0; k < 2; k++) {
-            if (c[k] < 0) {
-                const int leaf = ~c[k];
-                if (overlap(lo, hi, bv.leafLo[leaf], bv.leafHi[leaf])) visit(leaf);
-            } else if (overlap(lo, hi, bv.lo[c[k]], bv.hi[c[k]])) {
-                if (next < 0) next = c[k];
-                else if (sp < 64) stack[sp++] = c[k];
+    // a walk visits each of the 2n - 1 nodes at most once; the bound only
+    // guards against malformed links
+    for (int steps = 2 * bv.n; node != kRopeEnd && steps > 0; steps--) {
+        if (node >= 0) {
+            const float4 L = bv.lo[node];
+            if (overlap(lo, hi, L, bv.hi[node])) {
+                node = __float_as_int(L.w);
+                continue;
             }
+            node = bv.ropeNode[node];
+        } else {
+            const int leaf = ~node;
+            if (overlap(lo, hi, bv.leafLo[leaf], bv.leafHi[leaf])) visit(leaf);
+            node = bv.ropeLeaf[leaf];
         }
-        if (next < 0) {
-            if (sp == 0) return;
-            next = stack[--sp];
-        }
-        node = next;
     }
 }
 
@@ -292,6 +288,19 @@ __device__ bool edge_edge(const float4* pos, const float4* prev, int i1, int i2,
     return true;
 }
 
+// The hardware hands workgroups to the eight XCDs round-robin (workgroup b
+// to XCD b mod 8), and each XCD has its own L2. This maps hardware workgroup
+// b to a logical one so that every XCD gets a contiguous run of logical
+// workgroups: Morton-ordered queries of one XCD then share tree nodes in its
+// L2. Bijective on [0, nb).
+__device__ __forceinline__ int xcd_logical_block(int b, int nb)
+{
+    const int x = b & 7, k = b >> 3;
+    int start = 0;
+    for (int y = 0; y < x; y++) start += (nb - y + 7) >> 3;
+    return start + k;
+}
+
 // SelfCollisionDetector.cpp:60-135 (Midphase) over the exact candidate set:
 // blocks [0, vertexBlocks) take vertices (against the triangle tree, own
 // triangles and pinned vertices skipped), the rest edges (against the edge
@@ -309,8 +318,9 @@ __global__ void __launch_bounds__(kBlock) k_self_candidates(const float4* __rest
                                                             const int* __restrict__ vertexOrder, float tol,
                                                             SelfCandidates cand, SolverCtl* ctl)
 {
-    const bool vertexPass = (int)blockIdx.x < vertexBlocks;
-    const int j = (vertexPass ? blockIdx.x : blockIdx.x - vertexBlocks) * blockDim.x + threadIdx.x;
+    const int blk = xcd_logical_block(blockIdx.x, gridDim.x);
+    const bool vertexPass = blk < vertexBlocks;
+    const int j = (vertexPass ? blk : blk - vertexBlocks) * blockDim.x + threadIdx.x;
     if (j >= (vertexPass ? nv : ne)) return;
     const int K = vertexPass ? cand.kVT : cand.kEE;
     int2* out = (vertexPass ? cand.vt : cand.ee) + (size_t)j * K;
@@ -403,8 +413,9 @@ __global__ void __launch_bounds__(kBlock) k_self_ccd(const float4* __restrict__ 
         }
         return;
     }
-    const bool vertexPass = (int)blockIdx.x < vertexBlocks;
-    const int j = (vertexPass ? blockIdx.x : blockIdx.x - vertexBlocks) * blockDim.x + threadIdx.x;
+    const int blk = xcd_logical_block(blockIdx.x, queryBlocks);
+    const bool vertexPass = blk < vertexBlocks;
+    const int j = (vertexPass ? blk : blk - vertexBlocks) * blockDim.x + threadIdx.x;
     if (vertexPass) {
         if (j >= nv) return;
         const int2* in = cand.vt + (size_t)j * cand.kVT;
