@@ -116,7 +116,7 @@ class Cloth:
 
     FLOAT_PARAMS = ("unit", "thickness", "tolerance", "shear", "bend", "stretch", "poisson", "friction",
                     "area", "drag", "selfShrink", "selfThreshold", "ccdVolume", "ccdBisect", "ccdVtDistance")
-    INT_PARAMS = ("iterations", "collFlags", "bendFreq", "numSteps", "dihedral", "fem", "fixedIterations")
+    INT_PARAMS = ("iterations", "collFlags", "bendFreq", "numSteps", "dihedral", "fem", "fixedIterations", "accel")
 
     def __init__(self, real64=False):
         self.L = lib()

@@ -46,6 +46,9 @@ struct Params {
     // bisection stop, vertex-triangle distance
     float ccdVolume = 0.001f, ccdBisect = 0.01f, ccdVtDistance = 0.1f;
     int iterations = 50, collFlags = 0, bendFreq = 2, numSteps = 1;
+    // 1: broad phases through uniform grids (same results as the exhaustive
+    // scans, for timing the oracle at large sizes)
+    int accel = 0;
     bool dihedral = true, fem = false, fixedIterations = false;
 };
 
@@ -659,13 +662,34 @@ public:
         }
         const R radTol = R(prm.thickness) + smax(R(prm.tolerance), maxTv * R(1.5f));
         const size_t nt = M.idx.size() / 3;
+        std::vector<int> cand;
+        // the features whose boxes meet the box of radius r around [a, b]
+        // (every feature nearer than r does), or all of them
+        auto candidates = [&](const BoxGrid& g, size_t count, V3<R> a, V3<R> b, R r) {
+            cand.clear();
+            if (!prm.accel) {
+                for (size_t k = 0; k < count; k++) cand.push_back((int)k);
+                return;
+            }
+            const double pad = double(r) * (1.0 + 1e-6) + 1e-6 * (std::fabs(double(a.x)) + std::fabs(double(a.y)) +
+                                                                  std::fabs(double(a.z)) + 1.0);
+            const V3<float> lo{float(std::min(double(a.x), double(b.x)) - pad),
+                               float(std::min(double(a.y), double(b.y)) - pad),
+                               float(std::min(double(a.z), double(b.z)) - pad)};
+            const V3<float> hi{float(std::max(double(a.x), double(b.x)) + pad),
+                               float(std::max(double(a.y), double(b.y)) + pad),
+                               float(std::max(double(a.z), double(b.z)) + pad)};
+            g.query(lo, hi, cand);
+        };
         if (prm.collFlags & CF_VERTICES) {
             for (size_t i = 0; i < n; i++) {
                 const V3<R> q = prev[i];
                 R best = R(FLT_MAX);
                 int tri = -1;
                 V3<R> bcp{0, 0, 0}, bn{0, 0, 0};
-                for (size_t t = 0; t < nt; t++) {
+                candidates(M.triGrid, nt, q, q, radTol);
+                for (const int ct : cand) {
+                    const size_t t = (size_t)ct;
                     V3<R> pt, bar;
                     closest_pt_point_triangle(q, vR(M.v[M.idx[3 * t]]), vR(M.v[M.idx[3 * t + 1]]),
                                               vR(M.v[M.idx[3 * t + 2]]), pt, bar);
@@ -693,7 +717,15 @@ public:
                 R best = R(FLT_MAX), dist = R(-FLT_MAX);
                 int vtx = -1;
                 V3<R> bpt{0, 0, 0}, bn{0, 0, 0}, bbar{0, 0, 0};
-                for (size_t v = 0; v < M.v.size(); v++) {
+                {
+                    const R tvm = smax(tvs[i1], smax(tvs[i2], tvs[i3]));
+                    const R rtq = R(prm.thickness) + smax(R(prm.tolerance), tvm * R(1.2f));
+                    const V3<R> lo3{std::min({a.x, b.x, c.x}), std::min({a.y, b.y, c.y}), std::min({a.z, b.z, c.z})};
+                    const V3<R> hi3{std::max({a.x, b.x, c.x}), std::max({a.y, b.y, c.y}), std::max({a.z, b.z, c.z})};
+                    candidates(M.vtxGrid, M.v.size(), lo3, hi3, rtq);
+                }
+                for (const int cv : cand) {
+                    const size_t v = (size_t)cv;
                     V3<R> cp, bar;
                     const V3<R> mv = vR(M.v[v]);
                     closest_pt_point_triangle(mv, a, b, c, cp, bar);
@@ -729,7 +761,9 @@ public:
                 int edge = -1;
                 V3<R> bcp{0, 0, 0}, bn{0, 0, 0};
                 R bws = 0;
-                for (size_t e = 0; e < M.edges.size(); e++) {
+                candidates(M.edgeGrid, M.edges.size(), p, q, radTol);
+                for (const int ce : cand) {
+                    const size_t e = (size_t)ce;
                     R ps, pm;
                     V3<R> cs, cm;
                     closest_pt_segm_segm(p, q, vR(M.v[M.edges[e].i1]), vR(M.v[M.edges[e].i2]), ps, pm, cs, cm);
@@ -1237,29 +1271,59 @@ public:
         for (size_t t = 0; t < nt; t++) tb[t] = swept_box({tris[t].i1, tris[t].i2, tris[t].i3}, tol);
         for (size_t i = 0; i < nv; i++) vb[i] = swept_box({(uint32_t)i}, tol);
         for (size_t e = 0; e < ne; e++) eb[e] = swept_box({edges[e].i1, edges[e].i2}, tol);
+        // candidate pairs from grids (accel) or all pairs, in ascending order
+        // either way, so the contact lists come out the same
+        BoxGrid vg, eg;
+        std::vector<int> cand;
+        if (prm.accel) {
+            std::vector<V3<float>> lo(nv), hi(nv);
+            std::vector<char> pinned(nv);
+            for (size_t i = 0; i < nv; i++) {
+                lo[i] = vb[i].lo;
+                hi[i] = vb[i].hi;
+                pinned[i] = invMass[i] == R(0);
+            }
+            vg.build(lo, hi, &pinned);
+            lo.resize(ne);
+            hi.resize(ne);
+            for (size_t e = 0; e < ne; e++) {
+                lo[e] = eb[e].lo;
+                hi[e] = eb[e].hi;
+            }
+            eg.build(lo, hi);
+        }
+        auto each = [&](const BoxGrid& g, const Box& q, size_t count, size_t first, auto f) {
+            if (prm.accel) {
+                g.query(q.lo, q.hi, cand);
+                for (const int i : cand)
+                    if ((size_t)i >= first) f((size_t)i);
+            } else {
+                for (size_t i = first; i < count; i++) f(i);
+            }
+        };
         for (size_t t = 0; t < nt; t++) {
             const uint32_t j1 = tris[t].i1, j2 = tris[t].i2, j3 = tris[t].i3;
-            for (size_t i = 0; i < nv; i++) {
-                if (i == j1 || i == j2 || i == j3 || invMass[i] == R(0)) continue;
-                if (!overlap(tb[t], vb[i])) continue;
+            each(vg, tb[t], nv, 0, [&](size_t i) {
+                if (i == j1 || i == j2 || i == j3 || invMass[i] == R(0)) return;
+                if (!overlap(tb[t], vb[i])) return;
                 SelfContact<R> c;
                 if (vertex_triangle(j1, j2, j3, (uint32_t)i, c)) {
                     c.key = (uint64_t(t) << 32) | uint64_t(i);
                     selfTris.push_back(c);
                 }
-            }
+            });
         }
         for (size_t e1 = 0; e1 + 1 < ne; e1++) {
-            for (size_t e2 = e1 + 1; e2 < ne; e2++) {
+            each(eg, eb[e1], ne, e1 + 1, [&](size_t e2) {
                 const uint32_t i1 = edges[e1].i1, i2 = edges[e1].i2, i3 = edges[e2].i1, i4 = edges[e2].i2;
-                if (i1 == i3 || i2 == i4 || i1 == i4 || i2 == i3) continue;
-                if (!overlap(eb[e1], eb[e2])) continue;
+                if (i1 == i3 || i2 == i4 || i1 == i4 || i2 == i3) return;
+                if (!overlap(eb[e1], eb[e2])) return;
                 SelfContact<R> c;
                 if (edge_edge(i1, i2, i3, i4, c)) {
                     c.key = (uint64_t(e1) << 32) | uint64_t(e2);
                     selfEdges.push_back(c);
                 }
-            }
+            });
         }
     }
 
@@ -1555,6 +1619,7 @@ int oc_set_param_i(void* h, const char* name, int v)
         else if (n == "numSteps") p.numSteps = v;
         else if (n == "dihedral") p.dihedral = v != 0;
         else if (n == "fem") p.fem = v != 0;
+        else if (n == "accel") p.accel = v;
         else if (n == "fixedIterations") p.fixedIterations = v != 0;
         else return -1;
         return 0;

@@ -181,6 +181,95 @@ inline std::vector<MeshEdge> mesh_edges(const std::vector<uint32_t>& idx)
     return edges;
 }
 
+// Uniform grid over boxes for exact closed-interval overlap queries (a speed-up
+// of the oracle's exhaustive searches, used to time it at large sizes; it
+// returns exactly the boxes an exhaustive scan would). Each box is stored in
+// the cell of its minimum corner and the cell size is at least the largest box
+// extent, so a box overlapping the query box has its minimum corner in the
+// query's cells widened by one cell on each side. Cells are a dense CSR array
+// over the boxes' bounds.
+struct BoxGrid {
+    double cell = 1.0, ox = 0.0, oy = 0.0, oz = 0.0;
+    long nx = 0, ny = 0, nz = 0;
+    std::vector<V3<float>> lo, hi;
+    std::vector<int> start, items;
+
+    long ix(double v, double o) const { return (long)std::floor((v - o) / cell); }
+
+    // boxes with skip[i] set are left out
+    void build(const std::vector<V3<float>>& blo, const std::vector<V3<float>>& bhi,
+               const std::vector<char>* skip = nullptr)
+    {
+        lo = blo;
+        hi = bhi;
+        double ext = 0.0;
+        double mx = 1e300, my = 1e300, mz = 1e300, Mx = -1e300, My = -1e300, Mz = -1e300;
+        size_t count = 0;
+        for (size_t i = 0; i < lo.size(); i++) {
+            if (skip && (*skip)[i]) continue;
+            ext = std::max({ext, double(hi[i].x) - lo[i].x, double(hi[i].y) - lo[i].y, double(hi[i].z) - lo[i].z});
+            mx = std::min(mx, double(lo[i].x));
+            my = std::min(my, double(lo[i].y));
+            mz = std::min(mz, double(lo[i].z));
+            Mx = std::max(Mx, double(lo[i].x));
+            My = std::max(My, double(lo[i].y));
+            Mz = std::max(Mz, double(lo[i].z));
+            count++;
+        }
+        nx = ny = nz = 0;
+        start.assign(1, 0);
+        items.clear();
+        if (!count) return;
+        // no finer than the spacing of points spread over a surface of the
+        // bounds' size (boxes of zero extent), and at most 16M cells
+        const double span = std::max({Mx - mx, My - my, Mz - mz});
+        cell = std::max(ext, span / std::sqrt(double(count)));
+        if (!(cell > 0.0)) cell = 1.0;
+        ox = mx;
+        oy = my;
+        oz = mz;
+        for (;;) {
+            nx = ix(Mx, ox) + 1;
+            ny = ix(My, oy) + 1;
+            nz = ix(Mz, oz) + 1;
+            if (double(nx) * double(ny) * double(nz) <= double(1 << 24)) break;
+            cell *= 1.25;
+        }
+        start.assign(size_t(nx * ny * nz) + 1, 0);
+        auto at = [&](size_t i) {
+            return size_t((ix(lo[i].x, ox) * ny + ix(lo[i].y, oy)) * nz + ix(lo[i].z, oz));
+        };
+        for (size_t i = 0; i < lo.size(); i++)
+            if (!(skip && (*skip)[i])) start[at(i) + 1]++;
+        for (size_t c = 1; c < start.size(); c++) start[c] += start[c - 1];
+        items.resize(count);
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        for (size_t i = 0; i < lo.size(); i++)
+            if (!(skip && (*skip)[i])) items[fill[at(i)]++] = (int)i;
+    }
+
+    // the indices of the boxes overlapping [qlo, qhi], ascending
+    void query(V3<float> qlo, V3<float> qhi, std::vector<int>& out) const
+    {
+        out.clear();
+        if (!nx) return;
+        const long x0 = std::max(0L, ix(qlo.x, ox) - 1), x1 = std::min(nx - 1, ix(qhi.x, ox));
+        const long y0 = std::max(0L, ix(qlo.y, oy) - 1), y1 = std::min(ny - 1, ix(qhi.y, oy));
+        const long z0 = std::max(0L, ix(qlo.z, oz) - 1), z1 = std::min(nz - 1, ix(qhi.z, oz));
+        for (long x = x0; x <= x1; x++)
+            for (long y = y0; y <= y1; y++) {
+                const size_t row = size_t((x * ny + y) * nz);
+                for (int k = start[row + z0], e = z1 >= z0 ? start[row + z1 + 1] : k; k < e; k++) {
+                    const int i = items[k];
+                    if (!(qhi.x < lo[i].x || qhi.y < lo[i].y || qhi.z < lo[i].z || hi[i].x < qlo.x ||
+                          hi[i].y < qlo.y || hi[i].z < qlo.z))
+                        out.push_back(i);
+                }
+            }
+        std::sort(out.begin(), out.end());
+    }
+};
+
 // A static triangle mesh obstacle in float (the reference's Mesh stores
 // Vector3 = FP32): vertices, indices, angle-weighted vertex normals, face
 // normals, unique edges and their unweighted-average normals.
@@ -189,6 +278,7 @@ struct ObstacleMesh {
     std::vector<uint32_t> idx;
     std::vector<V3<float>> vn, tn, en;
     std::vector<MeshEdge> edges;
+    BoxGrid triGrid, vtxGrid, edgeGrid;  // feature boxes, for the accelerated queries
 
     void build()
     {
@@ -226,6 +316,24 @@ struct ObstacleMesh {
             }
             en[e] = n;
         }
+        std::vector<V3<float>> lo, hi;
+        auto box = [&](std::initializer_list<uint32_t> ids) {
+            V3<float> a{1e30f, 1e30f, 1e30f}, b{-1e30f, -1e30f, -1e30f};
+            for (uint32_t i : ids) {
+                a = V3<float>{std::min(a.x, v[i].x), std::min(a.y, v[i].y), std::min(a.z, v[i].z)};
+                b = V3<float>{std::max(b.x, v[i].x), std::max(b.y, v[i].y), std::max(b.z, v[i].z)};
+            }
+            lo.push_back(a);
+            hi.push_back(b);
+        };
+        for (size_t t = 0; t < nt; t++) box({idx[3 * t], idx[3 * t + 1], idx[3 * t + 2]});
+        triGrid.build(lo, hi);
+        lo.clear(); hi.clear();
+        for (size_t i = 0; i < v.size(); i++) box({(uint32_t)i});
+        vtxGrid.build(lo, hi);
+        lo.clear(); hi.clear();
+        for (const MeshEdge& e : edges) box({e.i1, e.i2});
+        edgeGrid.build(lo, hi);
     }
 };
 
