@@ -19,6 +19,12 @@ kernel trace of `bench.py --no-graph --no-cpu --copy-probe-mb 0` with the same
 --steps/--warmup sees exactly these frames twice, so its per-kernel averages
 are directly comparable. cpu_baseline: the CPU oracle (a single-threaded
 restatement of the reference solver) on a bounded sample of the same workload.
+
+roofline.traffic: HBM bytes per launch of the dominant kernel from the
+committed rocprofv3 counter summary profiles/drape256_traffic.md (FETCH_SIZE
+and WRITE_SIZE runs, each scaled by what the counter reads for a copy of known
+size); used only when that summary was taken with the same launch phases per
+sweep as this run, else null.
 """
 import argparse
 import json
@@ -31,6 +37,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "PBD substeps/sec at 256² cloth; wall time to 1e-4 residual, learned vs plain"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "drape256_traffic.md")
 BYTES_PER_TRI = 136
 BYTES_PER_BEND = 128
 KERNEL_NAMES = {0: "k_fem", 1: "k_bend", 2: "k_contacts", 3: "k_detect", 4: "k_predict+k_velocity"}
@@ -294,6 +301,22 @@ def learned_vs_plain(scene, start_frames=40, window=10, judged_frames=20):
     }
 
 
+def measured_traffic(kernel, phases):
+    """HBM bytes per launch of `kernel` ("k_fem" / "k_bend") from the counter
+    summary's JSON block, or None when absent or taken with other phases."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            text = f.read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        d = json.loads(block)
+    except (OSError, IndexError, ValueError):
+        return None
+    if d.get("launch_phases_per_pass") != phases:
+        return None
+    k = d.get("kernels", {}).get(kernel)
+    return k.get("hbm_bytes_per_launch") if k else None
+
+
 def cpu_baseline(scene, budget_s):
     import oracle
 
@@ -429,6 +452,9 @@ def main():
             "algorithmic_bytes_per_launch": round(bytes_per_launch) if bytes_per_launch else None,
         },
     }
+    if args.workload == "drape256" and dom in (0, 1):
+        t = measured_traffic(KERNEL_NAMES[dom], out["config"]["launch_phases_per_pass"])
+        out["roofline"]["traffic"] = round(t) if t else None
     if args.copy_probe_mb > 0:
         import solidcloth_amd as sc
 

@@ -210,8 +210,12 @@ SCEXP int scGetColouring(scModel model, int kind, int* colourOffsets, int maxCol
 SCEXP int scGetSchedule(scModel model, int kind, int* phase, int* slab, int* colour, int count);
 
 /* Sweep schedule: mode 0 = one launch per colour (positions in HBM),
- * mode 1 = slab phases with the slab's vertices staged in LDS (default).
- * slabSize 0 lets the library choose. Changes the Gauss-Seidel order. */
+ * mode 1 = slab phases with the slab's vertices staged in LDS (default),
+ * slabs being strips of slabSize constraints; mode 2 = slab phases with
+ * strip interiors of about slabSize constraints in one phase and the seams
+ * between strips in the next. slabSize 0 (modes 1 and 2) lets the library
+ * choose among both kinds by estimated sweep cost. Changes the Gauss-Seidel
+ * order. */
 SCEXP void scSetSweep(scModel model, int mode, int slabSize);
 
 /* Contacts of the last substep against mesh obstacles, in the reference's

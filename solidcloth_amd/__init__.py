@@ -413,7 +413,10 @@ class ClothModel:
         return ph, sl, co
 
     def SetSweep(self, mode, slab_size=0):
-        """mode 0: one launch per colour; mode 1: slab phases staged in LDS."""
+        """mode 0: one launch per colour; mode 1: slab phases staged in LDS
+        (strips of slab_size constraints); mode 2: strip interiors of about
+        slab_size constraints, then the seams between them. slab_size 0:
+        the library's choice."""
         _checked("scSetSweep", self.h, int(mode), int(slab_size))
 
     def GetNumContacts(self):

@@ -129,7 +129,8 @@ struct SlabSet {
     const int4* info;
     const int* colOff;
     const int* verts;
-    int base;  // first slab of the launched phase
+    int base;   // first slab of the launched phase
+    int count;  // constraints in the schedule (bound of prefetch indices)
 };
 
 enum { kSlabBlock = 256, kMaxSlabColours = 63, kMaxSelfColours = 4096, kSlabBatch = 8 };

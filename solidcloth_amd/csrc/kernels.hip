@@ -427,12 +427,37 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-__device__ __forceinline__ int slab_begin(const SlabSet& ss, int4& info, int* scol, float4* sv, const float4* pos,
-                                          const SlabContacts& kc)
+// Keeps the loads above it above the early exit below it: without it the
+// compiler tests the exit flag first and only then issues the loads, one
+// memory latency after another.
+__device__ __forceinline__ void issue_before_exit() { asm volatile("" ::: "memory"); }
+
+// The slab's descriptor and its first order position: every later load of
+// the launch depends only on these, so the kernels issue the first colour's
+// constraint loads before staging the vertices and both are in flight
+// together.
+__device__ __forceinline__ int4 slab_info(const SlabSet& ss, int& first)
 {
-    info = ss.info[ss.base + blockIdx.x];
+    const int4 info = ss.info[ss.base + blockIdx.x];
+    first = ss.colOff[info.z];
+    return info;
+}
+
+// Stages the slab's vertices in LDS. The global ids of this thread's first
+// kSlabBatch vertices are kept in `gk` for slab_end (all of a slab's ids when
+// it has at most kSlabBatch * kSlabBlock vertices). The slab kernels are
+// launched with kSlabBlock threads, and their index arithmetic uses that
+// constant.
+__device__ __forceinline__ int slab_begin(const SlabSet& ss, const int4& info, int* scol, float4* sv,
+                                          const float4* pos, const SlabContacts& kc, int (&gk)[kSlabBatch])
+{
     const int* vg = ss.verts + info.x;
-    for (int k = threadIdx.x; k <= info.w; k += blockDim.x) scol[k] = ss.colOff[info.z + k];
+    // a slab has at most kMaxSlabColours colours, fewer than the workgroup's
+    // threads: one offset per thread, held in a register until the vertices
+    // are staged so that its load does not hold up theirs
+    static_assert(kSlabBlock > kMaxSlabColours, "one colour offset per thread");
+    const bool hasCol = (int)threadIdx.x <= info.w;
+    const int colOff = ss.colOff[info.z + min((int)threadIdx.x, info.w)];
     // vertex ids, then positions, kSlabBatch per thread at a time: the loads
     // of one batch are in flight together instead of one after another. Loads
     // past the slab's end repeat its last vertex (a slab has at least one)
@@ -440,23 +465,29 @@ __device__ __forceinline__ int slab_begin(const SlabSet& ss, int4& info, int* sc
     // allocates one slot more than the largest slab), so nothing is
     // conditional and the compiler keeps the loads together.
     const int last = info.y - 1;
-    for (int v0 = threadIdx.x; v0 < info.y; v0 += kSlabBatch * blockDim.x) {
+#pragma unroll
+    for (int u = 0; u < kSlabBatch; u++) gk[u] = 0;
+    for (int v0 = threadIdx.x; v0 < info.y; v0 += kSlabBatch * kSlabBlock) {
         int g[kSlabBatch];
         float4 P[kSlabBatch];
 #pragma unroll
-        for (int u = 0; u < kSlabBatch; u++) g[u] = vg[min(v0 + u * (int)blockDim.x, last)] & 0x7fffffff;
+        for (int u = 0; u < kSlabBatch; u++) g[u] = vg[min(v0 + u * kSlabBlock, last)] & 0x7fffffff;
 #pragma unroll
         for (int u = 0; u < kSlabBatch; u++) P[u] = pos[g[u]];
 #pragma unroll
         for (int u = 0; u < kSlabBatch; u++) {
-            const int v = v0 + u * blockDim.x;
+            const int v = v0 + u * kSlabBlock;
             sv[v < info.y ? v : info.y] = P[u];
+        }
+        if (v0 == (int)threadIdx.x) {
+#pragma unroll
+            for (int u = 0; u < kSlabBatch; u++) gk[u] = g[u];
         }
     }
     if (kc.enabled) {
         // the owner applies the vertex's contacts (same thread-to-vertex map
         // as the staging loop, so no barrier is needed in between)
-        for (int v = threadIdx.x; v < info.y; v += blockDim.x) {
+        for (int v = threadIdx.x; v < info.y; v += kSlabBlock) {
             const int raw = vg[v];
             if (raw >= 0) continue;
             const int g = raw & 0x7fffffff;
@@ -469,40 +500,52 @@ __device__ __forceinline__ int slab_begin(const SlabSet& ss, int4& info, int* sc
             kc.vel[g] = f4(ve, 0.f);
         }
     }
+    if (hasCol) scol[threadIdx.x] = colOff;
     __syncthreads();
     return info.w;
 }
 
-__device__ __forceinline__ void slab_end(const SlabSet& ss, const int4& info, const float4* sv, float4* pos)
+__device__ __forceinline__ void slab_end(const SlabSet& ss, const int4& info, const float4* sv, float4* pos,
+                                         const int (&gk)[kSlabBatch])
 {
     __syncthreads();
-    const int* vg = ss.verts + info.x;
-    const int last = info.y - 1;
-    for (int v0 = threadIdx.x; v0 < info.y; v0 += kSlabBatch * blockDim.x) {
-        int g[kSlabBatch];
-#pragma unroll
-        for (int u = 0; u < kSlabBatch; u++) g[u] = vg[min(v0 + u * (int)blockDim.x, last)] & 0x7fffffff;
+    const int batch = kSlabBatch * kSlabBlock;
+    if (info.y <= batch) {
+        // the ids staged by this thread, kept in registers
 #pragma unroll
         for (int u = 0; u < kSlabBatch; u++) {
-            const int v = v0 + u * blockDim.x;
+            const int v = threadIdx.x + u * kSlabBlock;
+            if (v < info.y) pos[gk[u]] = sv[v];
+        }
+        return;
+    }
+    const int* vg = ss.verts + info.x;
+    const int last = info.y - 1;
+    for (int v0 = threadIdx.x; v0 < info.y; v0 += batch) {
+        int g[kSlabBatch];
+#pragma unroll
+        for (int u = 0; u < kSlabBatch; u++) g[u] = vg[min(v0 + u * kSlabBlock, last)] & 0x7fffffff;
+#pragma unroll
+        for (int u = 0; u < kSlabBatch; u++) {
+            const int v = v0 + u * kSlabBlock;
             if (v < info.y) pos[g[u]] = sv[v];
         }
     }
 }
 
 // One colour's constraint for this thread, fetched a colour ahead. Indices
-// past the colour are clamped to the slab's last constraint (`last`), so the
-// loads are unconditional and stay in flight across the LDS barrier; `have`
-// says whether the thread has a constraint in the colour.
+// past the colour are clamped to `last` (the slab's last constraint, or the
+// schedule's for the first colour), so the loads are unconditional and stay
+// in flight across the LDS barrier; the thread has a constraint in the
+// colour when its position t is below the colour's end.
 struct FemItem {
     ushort4 id;
     float4 A, B;
     float2 C;
     int t;
-    bool have;
 };
 
-__device__ __forceinline__ FemItem fem_fetch(TriStreams tr, const ushort4* __restrict__ lidx, int t, int end, int last)
+__device__ __forceinline__ FemItem fem_fetch(TriStreams tr, const ushort4* __restrict__ lidx, int t, int last)
 {
     FemItem f;
     const int u = min(t, last);
@@ -511,7 +554,6 @@ __device__ __forceinline__ FemItem fem_fetch(TriStreams tr, const ushort4* __res
     f.B = tr.b[u];
     f.C = tr.c[u];
     f.t = t;
-    f.have = t < end;
     return f;
 }
 
@@ -520,14 +562,14 @@ __device__ __forceinline__ FemItem fem_fetch(TriStreams tr, const ushort4* __res
 __device__ __forceinline__ void fem_colour(const FemItem& f, int e, float4* sv, TriStreams tr,
                                            const ushort4* __restrict__ lidx, float h2, float epsUV)
 {
-    if (f.have) {
+    if (f.t < e) {
         float4 P1 = sv[f.id.x], P2 = sv[f.id.y], P3 = sv[f.id.z];
         fem_project(P1, P2, P3, f.A, f.B, f.C, h2, epsUV);
         sv[f.id.x] = P1;
         sv[f.id.y] = P2;
         sv[f.id.z] = P3;
     }
-    for (int t2 = f.t + blockDim.x; t2 < e; t2 += blockDim.x) {
+    for (int t2 = f.t + kSlabBlock; t2 < e; t2 += kSlabBlock) {
         const ushort4 i2 = lidx[t2];
         float4 P1 = sv[i2.x], P2 = sv[i2.y], P3 = sv[i2.z];
         fem_project(P1, P2, P3, tr.a[t2], tr.b[t2], tr.c[t2], h2, epsUV);
@@ -542,48 +584,49 @@ __global__ void __launch_bounds__(kSlabBlock) k_fem_slab(float4* __restrict__ po
                                                          float epsUV, const SolverCtl* __restrict__ ctl,
                                                          SlabContacts kc)
 {
-    if (ctl->done) return;
     extern __shared__ float4 sv[];
     __shared__ int scol[kMaxSlabColours + 1];
-    int4 info;
-    const int nc = slab_begin(ss, info, scol, sv, pos, kc);
-    const int last = scol[nc] - 1;
+    const int done = ctl->done;
+    int first;
+    const int4 info = slab_info(ss, first);
     // two colours per trip with the two items in turn: no register copies
     // of fetched data (a copy would wait for the load before the barrier)
-    FemItem a = fem_fetch(tr, lidx, scol[0] + threadIdx.x, scol[1], last), b;
+    FemItem a = fem_fetch(tr, lidx, first + threadIdx.x, ss.count - 1), b;
+    issue_before_exit();
+    if (done) return;
+    int gk[kSlabBatch];
+    const int nc = slab_begin(ss, info, scol, sv, pos, kc, gk);
+    const int last = scol[nc] - 1;
     for (int c = 0; c < nc; c += 2) {
         const bool n1 = c + 1 < nc;
-        b = fem_fetch(tr, lidx, scol[c + 1] + threadIdx.x, n1 ? scol[c + 2] : 0, last);
+        b = fem_fetch(tr, lidx, scol[c + 1] + threadIdx.x, last);
         fem_colour(a, scol[c + 1], sv, tr, lidx, h2, epsUV);
         lds_barrier();
         if (!n1) break;
-        const bool n2 = c + 2 < nc;
-        a = fem_fetch(tr, lidx, scol[c + 2] + threadIdx.x, n2 ? scol[c + 3] : 0, last);
+        a = fem_fetch(tr, lidx, scol[c + 2] + threadIdx.x, last);
         fem_colour(b, scol[c + 2], sv, tr, lidx, h2, epsUV);
         lds_barrier();
     }
-    slab_end(ss, info, sv, pos);
+    slab_end(ss, info, sv, pos, gk);
 }
 
 struct QuadItem {
     ushort4 id;
     int t;
-    bool have;
 };
 
-__device__ __forceinline__ QuadItem quad_fetch(const ushort4* __restrict__ lidx, int t, int end, int last)
+__device__ __forceinline__ QuadItem quad_fetch(const ushort4* __restrict__ lidx, int t, int last)
 {
     QuadItem f;
     f.id = lidx[min(t, last)];
     f.t = t;
-    f.have = t < end;
     return f;
 }
 
 __device__ __forceinline__ void bend_colour(const QuadItem& f, int e, float4* sv, const ushort4* __restrict__ lidx,
                                             float sStiff, float& err)
 {
-    if (f.have) {
+    if (f.t < e) {
         float4 P1 = sv[f.id.x], P2 = sv[f.id.y], P3 = sv[f.id.z], P4 = sv[f.id.w];
         const float e1 = bend_project(P1, P2, P3, P4, sStiff);
         err = err < e1 ? e1 : err;
@@ -592,7 +635,7 @@ __device__ __forceinline__ void bend_colour(const QuadItem& f, int e, float4* sv
         sv[f.id.z] = P3;
         sv[f.id.w] = P4;
     }
-    for (int t2 = f.t + blockDim.x; t2 < e; t2 += blockDim.x) {
+    for (int t2 = f.t + kSlabBlock; t2 < e; t2 += kSlabBlock) {
         const ushort4 i2 = lidx[t2];
         float4 P1 = sv[i2.x], P2 = sv[i2.y], P3 = sv[i2.z], P4 = sv[i2.w];
         const float e1 = bend_project(P1, P2, P3, P4, sStiff);
@@ -607,28 +650,31 @@ __device__ __forceinline__ void bend_colour(const QuadItem& f, int e, float4* sv
 __global__ void __launch_bounds__(kSlabBlock) k_bend_slab(float4* __restrict__ pos, const ushort4* __restrict__ lidx,
                                                           SlabSet ss, float sStiff, SolverCtl* ctl)
 {
-    if (ctl->done) return;
     extern __shared__ float4 sv[];
     __shared__ int scol[kMaxSlabColours + 1];
-    int4 info;
+    const int done = ctl->done;
+    int first;
+    const int4 info = slab_info(ss, first);
+    QuadItem a = quad_fetch(lidx, first + threadIdx.x, ss.count - 1), b;
+    issue_before_exit();
+    if (done) return;
     SlabContacts none;
     none.enabled = 0;
-    const int nc = slab_begin(ss, info, scol, sv, pos, none);
+    int gk[kSlabBatch];
+    const int nc = slab_begin(ss, info, scol, sv, pos, none, gk);
     const int last = scol[nc] - 1;
     float err = 0.f;
-    QuadItem a = quad_fetch(lidx, scol[0] + threadIdx.x, scol[1], last), b;
     for (int c = 0; c < nc; c += 2) {
         const bool n1 = c + 1 < nc;
-        b = quad_fetch(lidx, scol[c + 1] + threadIdx.x, n1 ? scol[c + 2] : 0, last);
+        b = quad_fetch(lidx, scol[c + 1] + threadIdx.x, last);
         bend_colour(a, scol[c + 1], sv, lidx, sStiff, err);
         lds_barrier();
         if (!n1) break;
-        const bool n2 = c + 2 < nc;
-        a = quad_fetch(lidx, scol[c + 2] + threadIdx.x, n2 ? scol[c + 3] : 0, last);
+        a = quad_fetch(lidx, scol[c + 2] + threadIdx.x, last);
         bend_colour(b, scol[c + 2], sv, lidx, sStiff, err);
         lds_barrier();
     }
-    slab_end(ss, info, sv, pos);
+    slab_end(ss, info, sv, pos, gk);
     wave_max_atomic(&ctl->cb, err);
 }
 
@@ -636,32 +682,30 @@ struct LinkItem {
     ushort4 id;
     float2 L;
     int t;
-    bool have;
 };
 
 __device__ __forceinline__ LinkItem link_fetch(const ushort4* __restrict__ lidx, const float2* __restrict__ lrest,
-                                               int t, int end, int last)
+                                               int t, int last)
 {
     LinkItem f;
     const int u = min(t, last);
     f.id = lidx[u];
     f.L = lrest[u];
     f.t = t;
-    f.have = t < end;
     return f;
 }
 
 __device__ __forceinline__ void link_colour(const LinkItem& f, int e, float4* sv, const ushort4* __restrict__ lidx,
                                             const float2* __restrict__ lrest, float k, float& err)
 {
-    if (f.have) {
+    if (f.t < e) {
         float4 P1 = sv[f.id.x], P2 = sv[f.id.y];
         const float e1 = link_project(P1, P2, f.L, k);
         err = err < e1 ? e1 : err;
         sv[f.id.x] = P1;
         sv[f.id.y] = P2;
     }
-    for (int t2 = f.t + blockDim.x; t2 < e; t2 += blockDim.x) {
+    for (int t2 = f.t + kSlabBlock; t2 < e; t2 += kSlabBlock) {
         const ushort4 i2 = lidx[t2];
         float4 P1 = sv[i2.x], P2 = sv[i2.y];
         const float e1 = link_project(P1, P2, lrest[t2], k);
@@ -676,28 +720,31 @@ __global__ void __launch_bounds__(kSlabBlock) k_link_slab(float4* __restrict__ p
                                                           const float2* __restrict__ lrest, SlabSet ss, float k,
                                                           SolverCtl* ctl)
 {
-    if (ctl->done) return;
     extern __shared__ float4 sv[];
     __shared__ int scol[kMaxSlabColours + 1];
-    int4 info;
+    const int done = ctl->done;
+    int first;
+    const int4 info = slab_info(ss, first);
+    LinkItem a = link_fetch(lidx, lrest, first + threadIdx.x, ss.count - 1), b;
+    issue_before_exit();
+    if (done) return;
     SlabContacts none;
     none.enabled = 0;
-    const int nc = slab_begin(ss, info, scol, sv, pos, none);
+    int gk[kSlabBatch];
+    const int nc = slab_begin(ss, info, scol, sv, pos, none, gk);
     const int last = scol[nc] - 1;
     float err = 0.f;
-    LinkItem a = link_fetch(lidx, lrest, scol[0] + threadIdx.x, scol[1], last), b;
     for (int c = 0; c < nc; c += 2) {
         const bool n1 = c + 1 < nc;
-        b = link_fetch(lidx, lrest, scol[c + 1] + threadIdx.x, n1 ? scol[c + 2] : 0, last);
+        b = link_fetch(lidx, lrest, scol[c + 1] + threadIdx.x, last);
         link_colour(a, scol[c + 1], sv, lidx, lrest, k, err);
         lds_barrier();
         if (!n1) break;
-        const bool n2 = c + 2 < nc;
-        a = link_fetch(lidx, lrest, scol[c + 2] + threadIdx.x, n2 ? scol[c + 3] : 0, last);
+        a = link_fetch(lidx, lrest, scol[c + 2] + threadIdx.x, last);
         link_colour(b, scol[c + 2], sv, lidx, lrest, k, err);
         lds_barrier();
     }
-    slab_end(ss, info, sv, pos);
+    slab_end(ss, info, sv, pos, gk);
     wave_max_atomic(&ctl->cl, err);
 }
 

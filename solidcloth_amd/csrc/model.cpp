@@ -216,8 +216,10 @@ hipEvent_t KernelTimer::get()
         pool.pop_back();
         return e;
     }
+    // timing-only events: no system-scope fence (cache write-back) at each
+    // record, which would otherwise add microseconds to every bracketed launch
     hipEvent_t e;
-    HIP_OK(hipEventCreate(&e));
+    HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     return e;
 }
 
@@ -548,20 +550,22 @@ void ClothModel::ensure_colouring()
         bendCol = schedule_colours(nb, 4, bv.data(), nv);
         linkCol = schedule_colours(nl, 2, lv.data(), nv);
     } else if (mSlabSize > 0) {
-        triCol = schedule_slabs(nt, 3, tv.data(), nv, pos.data(), mSlabSize, maxVerts);
-        bendCol = schedule_slabs(nb, 4, bv.data(), nv, pos.data(), mSlabSize, maxVerts);
-        linkCol = schedule_slabs(nl, 2, lv.data(), nv, pos.data(), mSlabSize, maxVerts);
+        const bool seamed = mSweepMode == SWEEP_SEAMS;
+        triCol = schedule_slabs(nt, 3, tv.data(), nv, pos.data(), mSlabSize, maxVerts, seamed);
+        bendCol = schedule_slabs(nb, 4, bv.data(), nv, pos.data(), mSlabSize, maxVerts, seamed);
+        linkCol = schedule_slabs(nl, 2, lv.data(), nv, pos.data(), mSlabSize, maxVerts, seamed);
     } else {
-        triCol = schedule_slabs_auto(nt, 3, tv.data(), nv, pos.data(), maxVerts);
-        bendCol = schedule_slabs_auto(nb, 4, bv.data(), nv, pos.data(), maxVerts);
-        linkCol = schedule_slabs_auto(nl, 2, lv.data(), nv, pos.data(), maxVerts);
+        triCol = schedule_slabs_auto(nt, 3, tv.data(), nv, pos.data(), maxVerts, kSlabBlock);
+        bendCol = schedule_slabs_auto(nb, 4, bv.data(), nv, pos.data(), maxVerts, kSlabBlock);
+        linkCol = schedule_slabs_auto(nl, 2, lv.data(), nv, pos.data(), maxVerts, kSlabBlock);
     }
     mColourDirty = false;
 }
 
 void ClothModel::set_sweep(int mode, int slabSize)
 {
-    if (mode != SWEEP_COLOURS && mode != SWEEP_SLABS) throw std::runtime_error("unknown sweep mode");
+    if (mode != SWEEP_COLOURS && mode != SWEEP_SLABS && mode != SWEEP_SEAMS)
+        throw std::runtime_error("unknown sweep mode");
     if (slabSize < 0 || slabSize > 65536) throw std::runtime_error("slab size out of range");
     if (mode == mSweepMode && slabSize == mSlabSize) return;
     sync_to_host();
@@ -912,7 +916,7 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                 timed_begin(g.timer, timed, 0, s, ev);
                 if (linkCol.slabbed) {
                     const int b = linkCol.phaseSlabBegin[p], ns = linkCol.phaseSlabBegin[p + 1] - b;
-                    SlabSet ss{g.linkSlabs.info, g.linkSlabs.colOff, g.linkSlabs.verts, b};
+                    SlabSet ss{g.linkSlabs.info, g.linkSlabs.colOff, g.linkSlabs.verts, b, (int)linkCol.order.size()};
                     launch_link_slab(s, g.d_pos, g.linkSlabs.lidx, g.d_lr, ss, ns, linkCol.maxSlabVerts, linkK,
                                      g.d_ctl);
                 } else {
@@ -927,7 +931,7 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                 timed_begin(g.timer, timed, 0, s, ev);
                 if (triCol.slabbed) {
                     const int b = triCol.phaseSlabBegin[p], ns = triCol.phaseSlabBegin[p + 1] - b;
-                    SlabSet ss{g.triSlabs.info, g.triSlabs.colOff, g.triSlabs.verts, b};
+                    SlabSet ss{g.triSlabs.info, g.triSlabs.colOff, g.triSlabs.verts, b, (int)triCol.order.size()};
                     SlabContacts kc{cb, g.d_prev, g.d_vel, mThickness, mFriction, invH, (fuse && k > 0) ? 1 : 0};
                     launch_fem_slab(s, g.d_pos, tr, g.triSlabs.lidx, ss, ns, triCol.maxSlabVerts, h2, epsUV, g.d_ctl,
                                     kc);
@@ -943,7 +947,7 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                 timed_begin(g.timer, timed, 1, s, ev);
                 if (bendCol.slabbed) {
                     const int b = bendCol.phaseSlabBegin[p], ns = bendCol.phaseSlabBegin[p + 1] - b;
-                    SlabSet ss{g.bendSlabs.info, g.bendSlabs.colOff, g.bendSlabs.verts, b};
+                    SlabSet ss{g.bendSlabs.info, g.bendSlabs.colOff, g.bendSlabs.verts, b, (int)bendCol.order.size()};
                     launch_bend_slab(s, g.d_pos, g.bendSlabs.lidx, ss, ns, bendCol.maxSlabVerts, sStiff, g.d_ctl);
                 } else {
                     int st = bendCol.phaseOffsets[p], cnt = bendCol.phaseOffsets[p + 1] - st;

@@ -170,7 +170,7 @@ private:
 };
 
 // Sweep schedules: 0 = one launch per colour, 1 = slab phases with LDS staging
-enum { SWEEP_COLOURS = 0, SWEEP_SLABS = 1 };
+enum { SWEEP_COLOURS = 0, SWEEP_SLABS = 1, SWEEP_SEAMS = 2 };
 
 // Device copy of one constraint schedule's slab structure.
 struct DeviceSlabs {

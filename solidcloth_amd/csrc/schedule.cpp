@@ -72,12 +72,105 @@ Schedule schedule_colours(int n, int arity, const uint32_t* verts, int nv)
     return s;
 }
 
-Schedule schedule_slabs(int n, int arity, const uint32_t* verts, int nv, const f3* rest, int slabSize, int maxVerts)
+namespace {
+
+int longest_axis(const f3* rest, int nv)
+{
+    f3 lo = rest[0], hi = rest[0];
+    for (int i = 1; i < nv; i++) {
+        lo = f3{std::min(lo.x, rest[i].x), std::min(lo.y, rest[i].y), std::min(lo.z, rest[i].z)};
+        hi = f3{std::max(hi.x, rest[i].x), std::max(hi.y, rest[i].y), std::max(hi.z, rest[i].z)};
+    }
+    const float ext[3] = {hi.x - lo.x, hi.y - lo.y, hi.z - lo.z};
+    return (ext[1] > ext[0] && ext[1] >= ext[2]) ? 1 : (ext[2] > ext[0] && ext[2] > ext[1]) ? 2 : 0;
+}
+
+float axis_coord(const f3& p, int axis) { return axis == 0 ? p.x : axis == 1 ? p.y : p.z; }
+
+// Strips: constraints sorted by centroid along the axis and cut every
+// `slabSize` constraints.
+std::vector<std::vector<int>> strip_slabs(int n, int arity, const uint32_t* verts, const f3* rest, int axis,
+                                          int slabSize)
+{
+    // sort by centroid along the axis (sum of coordinates; ties by index)
+    std::vector<float> key(n);
+    for (int c = 0; c < n; c++) {
+        float k = 0.f;
+        for (int a = 0; a < arity; a++) k += axis_coord(rest[verts[(size_t)c * arity + a]], axis);
+        key[c] = k;
+    }
+    std::vector<int> byKey(n);
+    std::iota(byKey.begin(), byKey.end(), 0);
+    std::stable_sort(byKey.begin(), byKey.end(), [&](int a, int b) { return key[a] < key[b]; });
+    const int numSlabs = (n + slabSize - 1) / slabSize;
+    std::vector<std::vector<int>> slabs(numSlabs);
+    for (int k = 0; k < n; k++) slabs[k / slabSize].push_back(byKey[k]);
+    return slabs;
+}
+
+// Interiors and seams: the vertices are sorted along the axis (ties by index)
+// and cut into strips of about slabSize * nv / n vertices, and at least the
+// widest constraint's span in that order, so that no constraint spans three
+// strips. The constraints whose vertices all lie in strip i form interior
+// slab 2i; those spanning
+// strips i and i+1 form seam slab 2i+1. Interiors of different strips share
+// no vertex, so they all run in one phase, and the seams (narrow bands
+// around the cuts) in the next: two phases, of which only the interior one
+// has the full local colour count.
+std::vector<std::vector<int>> seamed_slabs(int n, int arity, const uint32_t* verts, int nv, const f3* rest,
+                                           int axis, int slabSize)
+{
+    std::vector<int> byCoord(nv);
+    std::iota(byCoord.begin(), byCoord.end(), 0);
+    std::stable_sort(byCoord.begin(), byCoord.end(),
+                     [&](int a, int b) { return axis_coord(rest[a], axis) < axis_coord(rest[b], axis); });
+    std::vector<int> rank(nv);
+    for (int k = 0; k < nv; k++) rank[byCoord[k]] = k;
+    // a strip at least as wide (in sorted vertices) as any constraint, so that
+    // no constraint spans three strips
+    long widest = 1;
+    for (int c = 0; c < n; c++) {
+        int lo = nv, hi = -1;
+        for (int a = 0; a < arity; a++) {
+            const int r = rank[verts[(size_t)c * arity + a]];
+            lo = std::min(lo, r);
+            hi = std::max(hi, r);
+        }
+        widest = std::max(widest, (long)(hi - lo));
+    }
+    const long stripVerts = std::max(widest, (long)slabSize * nv / std::max(1, n));
+    std::vector<int> strip(nv);
+    for (int k = 0; k < nv; k++) strip[byCoord[k]] = (int)(k / stripVerts);
+    const int numStrips = (int)((nv + stripVerts - 1) / stripVerts);
+    std::vector<std::vector<int>> slabs(2 * (size_t)numStrips);
+    for (int c = 0; c < n; c++) {
+        int lo = 1 << 30, hi = -1;
+        for (int a = 0; a < arity; a++) {
+            const int st = strip[verts[(size_t)c * arity + a]];
+            lo = std::min(lo, st);
+            hi = std::max(hi, st);
+        }
+        if (hi - lo > 1) throw std::logic_error("a constraint spans more than two strips");
+        slabs[2 * (size_t)lo + (hi > lo)].push_back(c);
+    }
+    // interiors first, so that the phase colouring gives them phase 0
+    std::vector<std::vector<int>> out;
+    for (int pass = 0; pass < 2; pass++)
+        for (size_t k = pass; k < slabs.size(); k += 2)
+            if (!slabs[k].empty()) out.push_back(std::move(slabs[k]));
+    return out;
+}
+
+}  // namespace
+
+Schedule schedule_slabs(int n, int arity, const uint32_t* verts, int nv, const f3* rest, int slabSize, int maxVerts,
+                        bool seamed)
 {
     Schedule s;
     s.arity = arity;
     s.slabbed = true;
     s.slabSize = slabSize;
+    s.seamed = seamed;
     if (n == 0) {
         s.phaseOffsets = {0};
         s.phaseSlabBegin = {0};
@@ -86,29 +179,10 @@ Schedule schedule_slabs(int n, int arity, const uint32_t* verts, int nv, const f
         s.colOffsets = {0};
         return s;
     }
-    // longest axis of the rest shape
-    f3 lo = rest[0], hi = rest[0];
-    for (int i = 1; i < nv; i++) {
-        lo = f3{std::min(lo.x, rest[i].x), std::min(lo.y, rest[i].y), std::min(lo.z, rest[i].z)};
-        hi = f3{std::max(hi.x, rest[i].x), std::max(hi.y, rest[i].y), std::max(hi.z, rest[i].z)};
-    }
-    const float ext[3] = {hi.x - lo.x, hi.y - lo.y, hi.z - lo.z};
-    const int axis = (ext[1] > ext[0] && ext[1] >= ext[2]) ? 1 : (ext[2] > ext[0] && ext[2] > ext[1]) ? 2 : 0;
-    auto coord = [&](uint32_t v) { return axis == 0 ? rest[v].x : axis == 1 ? rest[v].y : rest[v].z; };
-    // sort by centroid along the axis (sum of coordinates; ties by index)
-    std::vector<float> key(n);
-    for (int c = 0; c < n; c++) {
-        float k = 0.f;
-        for (int a = 0; a < arity; a++) k += coord(verts[(size_t)c * arity + a]);
-        key[c] = k;
-    }
-    std::vector<int> byKey(n);
-    std::iota(byKey.begin(), byKey.end(), 0);
-    std::stable_sort(byKey.begin(), byKey.end(), [&](int a, int b) { return key[a] < key[b]; });
-    // cut into slabs
-    const int numSlabs = (n + slabSize - 1) / slabSize;
-    std::vector<std::vector<int>> slabs(numSlabs);
-    for (int k = 0; k < n; k++) slabs[k / slabSize].push_back(byKey[k]);
+    const int axis = longest_axis(rest, nv);
+    std::vector<std::vector<int>> slabs = seamed ? seamed_slabs(n, arity, verts, nv, rest, axis, slabSize)
+                                                 : strip_slabs(n, arity, verts, rest, axis, slabSize);
+    const int numSlabs = (int)slabs.size();
     for (auto& sl : slabs) std::sort(sl.begin(), sl.end());
     // slab conflict graph through shared vertices, greedy-coloured in slab order
     std::vector<std::vector<int>> vslabs(nv);
@@ -199,22 +273,57 @@ Schedule schedule_slabs(int n, int arity, const uint32_t* verts, int nv, const f
     return s;
 }
 
-Schedule schedule_slabs_auto(int n, int arity, const uint32_t* verts, int nv, const f3* rest, int maxVerts)
+int schedule_cost(const Schedule& s, int threads)
 {
-    // A colour costs about one projection's latency whatever its size, while
-    // a larger slab gets more local colours and keeps more wavefronts busy
-    // on LDS per colour, and a smaller one needs more phases (launches).
-    // Measured on the 256^2 drape (FEM and bends, sizes 256 to 4096), 1024
-    // constraints per slab was fastest for both; smaller sizes are the
-    // fallback when a slab's vertices would not fit the LDS budget.
-    const int sizes[] = {1024, 512, 256};
-    for (int sz : sizes) {
+    // A sweep step (one local colour of the phase's slowest slab) costs about
+    // one projection's latency whatever its width, plus one more per extra
+    // constraint a thread takes when the colour is wider than the workgroup;
+    // staging and writing back a phase costs about three steps.
+    int cost = 0;
+    for (int p = 0; p < s.numPhases(); p++) {
+        int worst = 0;
+        for (int sl = s.phaseSlabBegin[p]; sl < s.phaseSlabBegin[p + 1]; sl++) {
+            int steps = 0;
+            for (int c = s.slabColBegin[sl]; c < s.slabColBegin[sl + 1]; c++) {
+                const int width = s.colOffsets[c + 1] - s.colOffsets[c];
+                steps += (width + threads - 1) / threads;
+            }
+            worst = std::max(worst, steps);
+        }
+        cost += worst + 3;
+    }
+    return cost;
+}
+
+Schedule schedule_slabs_auto(int n, int arity, const uint32_t* verts, int nv, const f3* rest, int maxVerts,
+                             int threads)
+{
+    // Strips of 1024 or 512 constraints, or interiors and seams at several
+    // interior sizes: the candidate with the least sweep cost is kept (the
+    // first on ties). Thin strips have few local colours but need two or
+    // three phases; interiors and seams need two phases, only one of them
+    // with the full colour count. Smaller strips are the fallback when a
+    // slab's vertices would not fit the LDS budget.
+    struct Cand {
+        int size;
+        bool seamed;
+    };
+    const Cand cands[] = {{1024, true}, {2048, true}, {3072, true}, {1024, false}, {512, false}, {256, false}};
+    Schedule best;
+    int bestCost = -1;
+    for (const Cand& c : cands) {
         try {
-            return schedule_slabs(n, arity, verts, nv, rest, sz, maxVerts);
+            Schedule s = schedule_slabs(n, arity, verts, nv, rest, c.size, maxVerts, c.seamed);
+            const int cost = schedule_cost(s, threads);
+            if (bestCost < 0 || cost < bestCost) {
+                bestCost = cost;
+                best = std::move(s);
+            }
         } catch (const std::runtime_error&) {
         }
     }
-    throw std::runtime_error("no slab size fits the LDS budget");
+    if (bestCost < 0) throw std::runtime_error("no slab size fits the LDS budget");
+    return best;
 }
 
 }  // namespace sc

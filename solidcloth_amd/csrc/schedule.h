@@ -6,10 +6,12 @@
 //  * colour schedule: greedy first-fit colouring in index order; one launch
 //    per colour, one constraint per thread, positions in HBM.
 //
-//  * slab schedule: constraints sorted along the longest axis of the rest
-//    shape and cut into slabs of about `slabSize` constraints; slabs sharing a
-//    vertex are coloured apart (a chain of slabs needs two phases); inside a
-//    slab the constraints are greedily coloured again. One launch per phase,
+//  * slab schedule: the constraints are grouped into slabs, either strips
+//    (sorted along the longest axis of the rest shape and cut every
+//    `slabSize` constraints) or strip interiors and the seams between strips;
+//    slabs sharing a vertex are coloured apart (a chain of strips needs two
+//    phases, interiors one and seams one); inside a slab the constraints are
+//    greedily coloured again. One launch per phase,
 //    one workgroup per slab: the workgroup stages the slab's vertices in LDS,
 //    sweeps its local colours with workgroup barriers and writes the vertices
 //    back. Slabs of one phase touch disjoint vertices.
@@ -42,19 +44,27 @@ struct Schedule {
     int maxSlabVerts = 0;
     int maxSlabColours = 0;
     int slabSize = 0;
+    bool seamed = false;
     int numPhases() const { return (int)phaseOffsets.size() - 1; }
 };
 
 Schedule schedule_colours(int numConstraints, int arity, const uint32_t* verts, int numVertices);
 
 // Returns a slab schedule with at most `maxVerts` vertices per slab; throws if
-// the constraints cannot be split that way.
+// the constraints cannot be split that way. Without `seamed`, slabs are strips
+// of `slabSize` constraints along the rest shape's longest axis; with it, the
+// vertices are cut into strips and each strip's interior constraints form a
+// slab (all in one phase) and the constraints across each cut a seam slab.
 Schedule schedule_slabs(int numConstraints, int arity, const uint32_t* verts, int numVertices, const f3* rest,
-                        int slabSize, int maxVerts);
+                        int slabSize, int maxVerts, bool seamed = false);
 
-// Slabs of 1024 constraints (512 or 256 when the vertices of a slab would
-// exceed maxVerts).
+// Estimated sweep cost of a slab schedule in projection latencies, for a
+// workgroup of `threads` threads.
+int schedule_cost(const Schedule& s, int threads);
+
+// The least-cost slab schedule among strips and interior/seam splits of
+// several sizes that fit maxVerts.
 Schedule schedule_slabs_auto(int numConstraints, int arity, const uint32_t* verts, int numVertices,
-                             const f3* rest, int maxVerts);
+                             const f3* rest, int maxVerts, int threads);
 
 }  // namespace sc
