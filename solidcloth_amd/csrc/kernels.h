@@ -126,8 +126,9 @@ struct TriStreams {
 // count; colour k of a slab spans constraint positions [colOff[k], colOff[k+1]).
 // Constraint vertices are given as slab-local indices (ushort4).
 struct SlabSet {
-    const int4* info;
-    const int* colOff;
+    const int4* info;   // per slab: first vertex, vertex count, first colour, colour count
+    const int* colOff;  // colour offsets (schedule positions) of all slabs' colours
+    const int* first;   // per slab: schedule position of its first constraint
     const int* verts;
     int base;   // first slab of the launched phase
     int count;  // constraints in the schedule (bound of prefetch indices)

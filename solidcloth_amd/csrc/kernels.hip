@@ -231,10 +231,13 @@ __device__ __forceinline__ float bend_project(float4& P1, float4& P2, float4& P3
     f3 x14 = sub(x4, x1);
     f3 n2 = cross(x14, x12);
     float l2 = len(n2);
+    // the reference computes 1/l1, 1/l2 and the unit normals in both its
+    // angle and its gradient; they are the same values, computed once here
+    // (unused when l1 or l2 is 0: the bend is then skipped)
+    const float il1 = 1.f / l1, il2 = 1.f / l2;
+    const f3 n1n = mul(il1, n1), n2n = mul(il2, n2);
     float cosine = 1.f, theta = 0.f;
     if (!(l1 == 0.f || l2 == 0.f)) {
-        float il1 = 1.f / l1, il2 = 1.f / l2;
-        f3 n1n = mul(il1, n1), n2n = mul(il2, n2);
         cosine = dot(n1n, n2n);
         cosine = smin(1.f, smax(-1.f, cosine));
         theta = acos_as(cosine);
@@ -242,8 +245,6 @@ __device__ __forceinline__ float bend_project(float4& P1, float4& P2, float4& P3
     if (fabsf(cosine) >= 1 - 0.001f) return 0.f;
     float h1 = l1 / l, h2 = l2 / l;
     float he = 0.5f * (h1 + h2);
-    float il1 = 1.f / l1, il2 = 1.f / l2;
-    f3 n1n = mul(il1, n1), n2n = mul(il2, n2);
     f3 n1p = sub(n1n, mul(cosine, n2n));
     f3 n2p = sub(n2n, mul(cosine, n1n));
     float sine = sqrtf(1 - cosine * cosine);
@@ -432,15 +433,14 @@ __device__ __forceinline__ void lds_barrier()
 // memory latency after another.
 __device__ __forceinline__ void issue_before_exit() { asm volatile("" ::: "memory"); }
 
-// The slab's descriptor and its first order position: every later load of
-// the launch depends only on these, so the kernels issue the first colour's
-// constraint loads before staging the vertices and both are in flight
-// together.
+// The slab's descriptor and its first order position (two independent
+// loads): every later load of the launch depends only on these, so the
+// kernels issue the first colour's constraint loads before staging the
+// vertices and both are in flight together.
 __device__ __forceinline__ int4 slab_info(const SlabSet& ss, int& first)
 {
-    const int4 info = ss.info[ss.base + blockIdx.x];
-    first = ss.colOff[info.z];
-    return info;
+    first = ss.first[ss.base + blockIdx.x];
+    return ss.info[ss.base + blockIdx.x];
 }
 
 // Stages the slab's vertices in LDS. The global ids of this thread's first

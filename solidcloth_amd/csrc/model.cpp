@@ -298,7 +298,7 @@ void GpuSolver::release()
     self.cap = 0;
     self.built = false;
     for (DeviceSlabs* d : {&triSlabs, &bendSlabs, &linkSlabs}) {
-        dev_free(d->lidx); dev_free(d->info); dev_free(d->colOff); dev_free(d->verts);
+        dev_free(d->lidx); dev_free(d->info); dev_free(d->colOff); dev_free(d->first); dev_free(d->verts);
     }
     slotCap = 0;
     if (stream) (void)hipStreamDestroy(stream);
@@ -382,7 +382,7 @@ void ClothModel::sync_to_host()
 // slab contains.
 static int upload_slabs(const Schedule& s, DeviceSlabs& d, bool owners, int numVertices)
 {
-    dev_free(d.lidx); dev_free(d.info); dev_free(d.colOff); dev_free(d.verts);
+    dev_free(d.lidx); dev_free(d.info); dev_free(d.colOff); dev_free(d.first); dev_free(d.verts);
     if (!s.slabbed || s.order.empty()) return numVertices;
     const int n = (int)s.order.size();
     std::vector<ushort4> li(n);
@@ -392,9 +392,12 @@ static int upload_slabs(const Schedule& s, DeviceSlabs& d, bool owners, int numV
     }
     const int ns = (int)s.slabVertBegin.size() - 1;
     std::vector<int4> info(ns);
-    for (int i = 0; i < ns; i++)
+    std::vector<int> first(ns);
+    for (int i = 0; i < ns; i++) {
         info[i] = make_int4(s.slabVertBegin[i], s.slabVertBegin[i + 1] - s.slabVertBegin[i], s.slabColBegin[i],
                             s.slabColBegin[i + 1] - s.slabColBegin[i]);
+        first[i] = s.colOffsets[s.slabColBegin[i]];
+    }
     std::vector<int> verts(s.slabVerts.begin(), s.slabVerts.end());
     std::vector<char> seen(numVertices, 0);
     int orphans = numVertices;
@@ -409,10 +412,12 @@ static int upload_slabs(const Schedule& s, DeviceSlabs& d, bool owners, int numV
     dev_alloc(d.lidx, n);
     dev_alloc(d.info, ns);
     dev_alloc(d.colOff, s.colOffsets.size());
+    dev_alloc(d.first, ns);
     dev_alloc(d.verts, verts.size());
     HIP_OK(hipMemcpy(d.lidx, li.data(), n * sizeof(ushort4), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(d.info, info.data(), ns * sizeof(int4), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(d.colOff, s.colOffsets.data(), s.colOffsets.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(d.first, first.data(), ns * sizeof(int), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(d.verts, verts.data(), verts.size() * sizeof(int), hipMemcpyHostToDevice));
     return orphans;
 }
@@ -916,7 +921,8 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                 timed_begin(g.timer, timed, 0, s, ev);
                 if (linkCol.slabbed) {
                     const int b = linkCol.phaseSlabBegin[p], ns = linkCol.phaseSlabBegin[p + 1] - b;
-                    SlabSet ss{g.linkSlabs.info, g.linkSlabs.colOff, g.linkSlabs.verts, b, (int)linkCol.order.size()};
+                    SlabSet ss{g.linkSlabs.info, g.linkSlabs.colOff, g.linkSlabs.first, g.linkSlabs.verts, b,
+                               (int)linkCol.order.size()};
                     launch_link_slab(s, g.d_pos, g.linkSlabs.lidx, g.d_lr, ss, ns, linkCol.maxSlabVerts, linkK,
                                      g.d_ctl);
                 } else {
@@ -931,7 +937,8 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                 timed_begin(g.timer, timed, 0, s, ev);
                 if (triCol.slabbed) {
                     const int b = triCol.phaseSlabBegin[p], ns = triCol.phaseSlabBegin[p + 1] - b;
-                    SlabSet ss{g.triSlabs.info, g.triSlabs.colOff, g.triSlabs.verts, b, (int)triCol.order.size()};
+                    SlabSet ss{g.triSlabs.info, g.triSlabs.colOff, g.triSlabs.first, g.triSlabs.verts, b,
+                               (int)triCol.order.size()};
                     SlabContacts kc{cb, g.d_prev, g.d_vel, mThickness, mFriction, invH, (fuse && k > 0) ? 1 : 0};
                     launch_fem_slab(s, g.d_pos, tr, g.triSlabs.lidx, ss, ns, triCol.maxSlabVerts, h2, epsUV, g.d_ctl,
                                     kc);
@@ -947,7 +954,8 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                 timed_begin(g.timer, timed, 1, s, ev);
                 if (bendCol.slabbed) {
                     const int b = bendCol.phaseSlabBegin[p], ns = bendCol.phaseSlabBegin[p + 1] - b;
-                    SlabSet ss{g.bendSlabs.info, g.bendSlabs.colOff, g.bendSlabs.verts, b, (int)bendCol.order.size()};
+                    SlabSet ss{g.bendSlabs.info, g.bendSlabs.colOff, g.bendSlabs.first, g.bendSlabs.verts, b,
+                               (int)bendCol.order.size()};
                     launch_bend_slab(s, g.d_pos, g.bendSlabs.lidx, ss, ns, bendCol.maxSlabVerts, sStiff, g.d_ctl);
                 } else {
                     int st = bendCol.phaseOffsets[p], cnt = bendCol.phaseOffsets[p + 1] - st;

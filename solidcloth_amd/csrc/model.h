@@ -177,6 +177,7 @@ struct DeviceSlabs {
     ushort4* lidx = nullptr;
     int4* info = nullptr;
     int* colOff = nullptr;
+    int* first = nullptr;
     int* verts = nullptr;
 };
 

@@ -7,7 +7,8 @@ group of constraints that runs concurrently touches disjoint vertices, so
 equality is expected to be bit-exact; the stated tolerance (1e-5 relative to
 the cloth extent) is the acceptance bar and bit-exactness is asserted on top.
 Both sweep schedules are covered: one launch per colour (mode 0) and slab
-phases staged in LDS (mode 1, the default).
+phases staged in LDS (mode 1, the default), with strips and interior/seam
+slabs of several sizes.
 """
 import numpy as np
 import pytest
@@ -47,6 +48,23 @@ def test_drape32_100_substeps_bit_exact(sc, oracle_mod, mode):
     err = _compare(gp, op, "drape32 positions after 100 substeps")
     assert err == 0.0, f"expected bit-exact parity, max abs diff {err:.3e}"
     assert np.array_equal(m.velocities().astype(np.float64), o.get("vel"))
+
+
+@pytest.mark.parametrize("mode,size", [(1, 1024), (1, 256), (2, 512), (2, 2048)])
+def test_slab_shapes_bit_exact(sc, oracle_mod, mode, size):
+    """Strips and interior/seam slabs of several sizes on a 96^2 drape (many
+    slabs per phase, constraint records and vertices staged in LDS)."""
+    scene = S.DRAPE32.with_(n=96)
+    print(f"slab shapes mode={mode} size={size}", flush=True)
+    world, patch = S.build_patch(scene, sc)
+    m = patch.GetModel()
+    m.SetSweep(mode, size)
+    o = oracle_mod.from_scene(scene, tri_order=m.constraint_order("tri"), bend_order=m.constraint_order("bend"))
+    for f in range(6):
+        patch.Step(scene.dt)
+        o.step(scene.dt)
+    err = _compare(m.positions(), o.get("pos"), f"96^2 drape, sweep {mode} slab size {size}")
+    assert err == 0.0, f"expected bit-exact parity, max abs diff {err:.3e}"
 
 
 @pytest.mark.parametrize("mode", MODES)
