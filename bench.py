@@ -317,6 +317,32 @@ def measured_traffic(kernel, phases):
     return k.get("hbm_bytes_per_launch") if k else None
 
 
+def render_sync(patch, with_cpu, calls=20):
+    """The render-mesh sync (scUpdateClothMesh: vertices, angle-weighted
+    normals, face normals, copied to the host mesh buffers) per call on the
+    device, and the oracle's host restatement of the same normals
+    (Mesh::ComputeNormals on the same vertices) beside it."""
+    patch.UpdateMesh()  # builds the vertex-triangle lists once
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        patch.UpdateMesh()
+    gpu_ms = 1000.0 * (time.perf_counter() - t0) / calls
+    verts, _, idx = patch.mesh_arrays()
+    out = {"vertices": int(len(verts)), "triangles": int(len(idx)), "gpu_ms_per_call": round(gpu_ms, 3),
+           "includes": "kernels + device-to-host copies of vertices and normals (1.6 MB)"}
+    if with_cpu:
+        import oracle
+
+        oracle.mesh_normals(verts, idx)
+        t0 = time.perf_counter()
+        n = 3
+        for _ in range(n):
+            oracle.mesh_normals(verts, idx)
+        out["cpu_oracle_ms_per_call"] = round(1000.0 * (time.perf_counter() - t0) / n, 3)
+        out["cpu_cores"] = 1
+    return out
+
+
 def cpu_baseline(scene, budget_s):
     import oracle
 
@@ -463,6 +489,8 @@ def main():
         if ms > 0:
             out["copy_probe"] = {"bytes_copied": nbytes, "avg_ms": round(ms, 4),
                                  "GB/s (read+write)": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
+    if args.workload == "drape256":
+        out["render_mesh_sync"] = render_sync(keep[1], not args.no_cpu)
     if args.workload == "drape256" and not args.no_learned:
         out["learned_vs_plain"] = learned_vs_plain(scene)
     if args.workload != "drape256":

@@ -536,6 +536,23 @@ class ClothPatch:
     def UpdateMesh(self):
         _checked("scUpdateClothMesh", self.h)
 
+    def mesh_arrays(self):
+        """Copies of the render mesh (scGetClothMesh): vertices (V, 3),
+        normals (V, 3) and triangle indices (T, 3); the normals are those of
+        the last UpdateMesh."""
+        L = lib()
+        mesh = L.scGetClothMesh(self.h)
+        nv = L.scGetMeshVertexCount(mesh)
+        ni = L.scGetMeshIndexCount(mesh)
+        vb, nb, ib = L.scGetMeshVertexBuffer(mesh), L.scGetMeshNormalBuffer(mesh), L.scGetMeshIndexBuffer(mesh)
+        if nv <= 0 or not vb:
+            raise SolidClothError("the patch has no render mesh")
+        verts = np.ctypeslib.as_array(C.cast(vb, C.POINTER(C.c_float)), shape=(nv, 3)).copy()
+        normals = (np.ctypeslib.as_array(C.cast(nb, C.POINTER(C.c_float)), shape=(nv, 3)).copy() if nb
+                   else np.zeros((nv, 3), dtype=np.float32))
+        idx = np.ctypeslib.as_array(ib, shape=(ni,)).copy().reshape(-1, 3)
+        return verts, normals, idx
+
 
 class ClothBatch:
     """Many patches stepped together, one launch per substep (scClothBatch).

@@ -223,6 +223,14 @@ void launch_link_slab(hipStream_t s, float4* pos, const ushort4* lidx, const flo
                       int maxVerts, float k, SolverCtl* ctl);
 void launch_velocity(hipStream_t s, float4* pos, const float4* prev, float4* vel, int n, float invH, SolverCtl* ctl);
 void launch_copy(hipStream_t s, const float4* src, float4* dst, long n);
+// Render mesh of the cloth (Physics/ClothPatch.cpp:266-273, Geometry/Mesh.cpp:10-63):
+// outV = pos + offset and outN = angle-weighted unit vertex normals (nv x 3
+// floats each), triN = unit face normals (nt x 3); idx = 3 nt render indices
+// into the particles; vt lists each vertex's (triangle * 4 + corner) in
+// ascending order, vtOff its CSR offsets; triA / triB are nt-sized scratch.
+void launch_render_mesh(hipStream_t s, const float4* pos, int nv, const unsigned* idx, int nt, f3 offset,
+                        const int* vtOff, const int* vt, float4* triA, float2* triB, float* outV, float* outN,
+                        float* triN);
 
 // Batch: one workgroup per cloth (cloths in launch order), one launch per
 // substep. blockSize 256, 512 or 1024; cloths with at most stageVerts

@@ -274,6 +274,9 @@ void GpuSolver::release()
     dev_free(warm.snapPos); dev_free(warm.snapPrev); dev_free(warm.snapVel); dev_free(warm.snapCorr);
     dev_free(warm.resPartial); dev_free(warm.resOut);
     warm.nv = -1;
+    dev_free(render.idx); dev_free(render.vtOff); dev_free(render.vt); dev_free(render.triA); dev_free(render.triB);
+    dev_free(render.outV); dev_free(render.outN); dev_free(render.triN);
+    render.nv = render.nt = -1;
     warm.built = warm.saved = false;
     dev_free(d_pos); dev_free(d_prev); dev_free(d_vel);
     dev_free(d_tidx); dev_free(d_tA); dev_free(d_tB); dev_free(d_tC); dev_free(d_bidx);
@@ -1354,15 +1357,62 @@ void ClothPatch::step(float dt)
 }
 
 // Physics/ClothPatch.cpp:266-273 with ClothModel::UpdateMesh (ClothModel.inl:123-130)
+// Physics/ClothPatch.cpp:266-273: the render mesh from the current state,
+// computed on the device (launch_render_mesh) and copied into the mesh's
+// vertex, normal and face-normal buffers.
 void ClothPatch::update_mesh()
 {
-    ClothModel& m = *mModel;
-    m.sync_to_host();
-    for (size_t i = 0; i < mMesh.vertices.size() && i < m.pos.size(); i++) {
-        f3 p = add(m.pos[i], mPosition);
-        mMesh.vertices[i] = scVector3{p.x, p.y, p.z};
+    if (mMesh.vertices.empty()) return;
+    mModel->render_mesh(mMesh, mPosition);
+}
+
+void ClothModel::render_mesh(Mesh& mesh, f3 offset)
+{
+    const int nv = (int)mesh.vertices.size(), nt = (int)(mesh.indices.size() / 3);
+    if (nv != (int)pos.size()) throw std::runtime_error("the render mesh does not match the particles");
+    sync_to_device();
+    GpuSolver& g = gpu;
+    GpuSolver::Render& r = g.render;
+    if (r.nv != nv || r.nt != nt) {
+        dev_free(r.idx); dev_free(r.vtOff); dev_free(r.vt); dev_free(r.triA); dev_free(r.triB);
+        dev_free(r.outV); dev_free(r.outN); dev_free(r.triN);
+        // each vertex's (triangle, corner) entries in ascending triangle order
+        std::vector<int> off(nv + 1, 0), list(3 * (size_t)nt);
+        for (int k = 0; k < 3 * nt; k++) {
+            const unsigned v = mesh.indices[k];
+            if (v >= (unsigned)nv) throw std::runtime_error("render mesh index out of range");
+            off[v + 1]++;
+        }
+        for (int v = 0; v < nv; v++) off[v + 1] += off[v];
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        for (int k = 0; k < 3 * nt; k++) list[fill[mesh.indices[k]]++] = (k / 3) * 4 + k % 3;
+        dev_alloc(r.idx, std::max(1, 3 * nt));
+        dev_alloc(r.vtOff, nv + 1);
+        dev_alloc(r.vt, std::max(1, 3 * nt));
+        dev_alloc(r.triA, std::max(1, nt));
+        dev_alloc(r.triB, std::max(1, nt));
+        dev_alloc(r.outV, 3 * (size_t)nv);
+        dev_alloc(r.outN, 3 * (size_t)nv);
+        dev_alloc(r.triN, std::max<size_t>(1, 3 * (size_t)nt));
+        if (nt > 0) {
+            HIP_OK(hipMemcpy(r.idx, mesh.indices.data(), 3 * (size_t)nt * sizeof(unsigned), hipMemcpyHostToDevice));
+            HIP_OK(hipMemcpy(r.vt, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+        HIP_OK(hipMemcpy(r.vtOff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+        r.nv = nv;
+        r.nt = nt;
     }
-    mMesh.compute_normals();
+    launch_render_mesh(g.stream, g.d_pos, nv, r.idx, nt, offset, r.vtOff, r.vt, r.triA, r.triB, r.outV, r.outN, r.triN);
+    HIP_OK(hipGetLastError());
+    mesh.normals.resize(nv);
+    mesh.triNormals.resize(nt);
+    static_assert(sizeof(scVector3) == 3 * sizeof(float), "scVector3 is three packed floats");
+    HIP_OK(hipMemcpyAsync(mesh.vertices.data(), r.outV, 3 * (size_t)nv * sizeof(float), hipMemcpyDeviceToHost, g.stream));
+    HIP_OK(hipMemcpyAsync(mesh.normals.data(), r.outN, 3 * (size_t)nv * sizeof(float), hipMemcpyDeviceToHost, g.stream));
+    if (nt > 0)
+        HIP_OK(hipMemcpyAsync(mesh.triNormals.data(), r.triN, 3 * (size_t)nt * sizeof(float), hipMemcpyDeviceToHost,
+                              g.stream));
+    HIP_OK(hipStreamSynchronize(g.stream));
 }
 
 // Physics/ClothPatch.cpp:337-381

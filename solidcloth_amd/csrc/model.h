@@ -260,6 +260,16 @@ public:
         float ell = 1.f;
         bool built = false, saved = false;
     } warm;
+    // render-mesh sync: indices, vertex -> (triangle, corner) lists, per-
+    // triangle normals and angles, packed outputs (built for nv, nt)
+    struct Render {
+        unsigned* idx = nullptr;
+        int *vtOff = nullptr, *vt = nullptr;
+        float4* triA = nullptr;
+        float2* triB = nullptr;
+        float *outV = nullptr, *outN = nullptr, *triN = nullptr;
+        int nv = -1, nt = -1;
+    } render;
     // two cached substep graphs (the warm-started and the plain substep)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -328,6 +338,9 @@ public:
     void mark_topology_dirty() { mTopoDirty = true; mHostDirty = true; mColourDirty = true; }
     void sync_to_host();
     void sync_to_device();
+    // the render mesh (vertices = particles + offset, normals, face normals)
+    // from the device state
+    void render_mesh(Mesh& mesh, f3 offset);
 
     void step_substeps(float h, int n);
     void download_contacts();
