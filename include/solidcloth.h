@@ -17,9 +17,9 @@
  *
  * Difference: the reference reports no errors. This library additionally
  * records the last error (scGetLastError) and refuses calls it cannot honour
- * (for example the mouse spring or a capsule collidable, which this build
- * does not implement on the GPU substep)
- * instead of computing something else. All device work started by a call has
+ * (for example a capsule collidable, which this build does not implement on
+ * the GPU substep, or a mouse spring on a batched patch) instead of
+ * computing something else. All device work started by a call has
  * completed before any getter returns data.
  */
 #ifndef SOLIDCLOTH_H
@@ -132,7 +132,10 @@ SCEXP void scSetNumIterations(scModel model, int val);                          
 SCEXP bool scGetUseFEM(scModel model);                                                         /* :131 */
 SCEXP void scSetUseFEM(scModel model, bool val);                                               /* :132 */
 
-/* ---- tree and mouse spring (Demo/ClothAPI.h:134-138) ---- */
+/* ---- tree and mouse spring (Demo/ClothAPI.h:134-138) ----
+ * The spring (stiffness 100, rest length 0, ClothModel.inl:86-98) acts in
+ * every substep of scStepCloth right after the prediction
+ * (Physics/ClothPBD.cpp:267-282); scStepClothBatch refuses a patch with one. */
 SCEXP void* scGetClothTree(scModel model);                                                     /* :134 */
 SCEXP void scAddMouseSpring(scModel model, int i1, int i2, int i3, scVector3 coords, scVector3 p); /* :136 */
 SCEXP void scUpdateMouseSpring(scModel model, scVector3 p);                                    /* :137 */

@@ -94,6 +94,7 @@ def lib():
         L.oc_signed_volume.argtypes = [vp, vp, vp, vp]
         L.oc_signed_volume.restype = _C.c_float
         L.oc_mesh_normals.argtypes = [_C.c_int, vp, _C.c_int, vp, vp]
+        L.oc_set_mouse_spring.argtypes = [vp] + [_C.c_int] * 4 + [_C.c_float] * 6
         _lib = L
     return _lib
 
@@ -259,6 +260,16 @@ class Cloth:
         self.L.oc_get_stats(self.h, _ptr(s))
         return dict(iterations=int(s[0]), bendError=s[1], contactError=s[2], selfVTError=s[3],
                     selfEEError=s[4], selfEECount=int(s[5]), linkError=s[6])
+
+    def set_mouse_spring(self, tri, coords, point):
+        """scAddMouseSpring / scUpdateMouseSpring: vertices tri (3), their
+        barycentric weights and the mouse point."""
+        self.L.oc_set_mouse_spring(self.h, 1, int(tri[0]), int(tri[1]), int(tri[2]), float(coords[0]),
+                                   float(coords[1]), float(coords[2]), float(point[0]), float(point[1]),
+                                   float(point[2]))
+
+    def remove_mouse_spring(self):
+        self.L.oc_set_mouse_spring(self.h, 0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0)
 
     def substep_detect(self, h):
         self.L.oc_substep_detect(self.h, h)

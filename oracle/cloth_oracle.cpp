@@ -1507,6 +1507,32 @@ public:
         substep_solve(h);
     }
 
+    // The mouse spring (ClothTypes.h:85-91; ClothModel.inl:86-98 sets its
+    // rest length 0 and stiffness 100)
+    struct MouseSpring {
+        bool active = false;
+        int i[3] = {0, 0, 0};
+        float a[3] = {0, 0, 0};
+        V3<float> point{0, 0, 0};
+        float len = 0.f, stiffness = 100.f;
+    } mouse;
+
+    // Physics/ClothPBD.cpp:267-282: pulls the barycentric point of the
+    // spring's triangle towards the mouse point, pinned particles included
+    void apply_mouse_spring(R h)
+    {
+        const MouseSpring& m = mouse;
+        const V3<R> p = (R(m.a[0]) * pos[m.i[0]] + R(m.a[1]) * pos[m.i[1]]) + R(m.a[2]) * pos[m.i[2]];
+        V3<R> delta = p - V3<R>{R(m.point.x), R(m.point.y), R(m.point.z)};
+        const R len = length(delta);
+        scale(delta, R(1) / len);
+        const V3<R> disp = (R(m.stiffness) * (len - R(m.len))) * delta;
+        for (int k = 0; k < 3; k++) {
+            pos[m.i[k]] -= (h * R(m.a[k])) * disp;
+            vel[m.i[k]] -= R(m.a[k]) * disp;
+        }
+    }
+
     // prediction, collision detection, constraint reset
     void substep_detect(R h)
     {
@@ -1519,6 +1545,7 @@ public:
             scale(vel[i], drag);
             pos[i] += h * vel[i];
         }
+        if (mouse.active) apply_mouse_spring(h);
         detect_collisions();
         for (auto& c : contacts) { c.lambda = 0; c.lambdaF = 0; }
     }
@@ -1604,6 +1631,22 @@ int oc_set_param_f(void* h, const char* name, float v)
         else if (n == "ccdBisect") p.ccdBisect = v;
         else if (n == "ccdVtDistance") p.ccdVtDistance = v;
         else return -1;
+        return 0;
+    });
+}
+
+// scAddMouseSpring / scUpdateMouseSpring / scRemoveMouseSpring
+// (ClothModel.h:150-152): active 0 removes the spring
+void oc_set_mouse_spring(void* h, int active, int i1, int i2, int i3, float a1, float a2, float a3, float px, float py,
+                         float pz)
+{
+    with(h, [&](auto& c) {
+        c.mouse.active = active != 0;
+        c.mouse.i[0] = i1; c.mouse.i[1] = i2; c.mouse.i[2] = i3;
+        c.mouse.a[0] = a1; c.mouse.a[1] = a2; c.mouse.a[2] = a3;
+        c.mouse.point = V3<float>{px, py, pz};
+        c.mouse.len = 0.f;
+        c.mouse.stiffness = 100.f;
         return 0;
     });
 }

@@ -482,6 +482,17 @@ class ClothModel:
         _checked("scGetSelfContacts", self.h, k, _ptr(keys), _ptr(ids), _ptr(data), n)
         return keys, ids, data
 
+    def AddMouseSpring(self, i1, i2, i3, coords, point):
+        """Physics/ClothModel.h:150: pulls the barycentric point (coords) of
+        particles i1, i2, i3 towards `point` in every following substep."""
+        _checked("scAddMouseSpring", self.h, int(i1), int(i2), int(i3), v3(coords), v3(point))
+
+    def UpdateMouseSpring(self, point):
+        _checked("scUpdateMouseSpring", self.h, v3(point))
+
+    def RemoveMouseSpring(self):
+        _checked("scRemoveMouseSpring", self.h)
+
     def SetSelfParams(self, shrink, threshold):
         _checked("scSetSelfParams", self.h, float(shrink), float(threshold))
 

@@ -434,6 +434,7 @@ void scAddMouseSpring(scModel model, int i1, int i2, int i3, scVector3 coords, s
         m->msI[0] = i1; m->msI[1] = i2; m->msI[2] = i3;
         m->msA[0] = coords.x; m->msA[1] = coords.y; m->msA[2] = coords.z;
         m->msPoint = f3{p.x, p.y, p.z};
+        m->msDirty = true;
     });
 }
 
@@ -441,7 +442,10 @@ void scUpdateMouseSpring(scModel model, scVector3 p)
 {
     guard([&] {
         ClothModel* m = M(model);
-        if (m->msActive) m->msPoint = f3{p.x, p.y, p.z};
+        if (m->msActive) {
+            m->msPoint = f3{p.x, p.y, p.z};
+            m->msDirty = true;
+        }
     });
 }
 

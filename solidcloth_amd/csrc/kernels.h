@@ -223,6 +223,22 @@ void launch_link_slab(hipStream_t s, float4* pos, const ushort4* lidx, const flo
                       int maxVerts, float k, SolverCtl* ctl);
 void launch_velocity(hipStream_t s, float4* pos, const float4* prev, float4* vel, int n, float invH, SolverCtl* ctl);
 void launch_copy(hipStream_t s, const float4* src, float4* dst, long n);
+// The mouse spring (Physics/ClothTypes.h:85-91): particles i1..i3 with
+// barycentric weights a1..a3, pulled towards (px, py, pz) with force
+// stiffness (|p - point| - len). Kept in device memory so that a captured
+// substep follows its moves.
+struct MouseSpringDev {
+    int i1, i2, i3;
+    float a1, a2, a3;
+    float px, py, pz;
+    float len, stiffness;
+};
+// Physics/ClothPBD.cpp:267-282, one thread; with resetTravel it also clears
+// ctl->maxTv for launch_max_travel
+void launch_mouse_spring(hipStream_t s, float4* pos, float4* vel, const MouseSpringDev* ms, float h, SolverCtl* ctl,
+                         int resetTravel);
+// ctl->maxTv = max over all particles of |pos - prev| (ClothCollision.cpp:95-104)
+void launch_max_travel(hipStream_t s, const float4* pos, const float4* prev, int n, SolverCtl* ctl);
 // Render mesh of the cloth (Physics/ClothPatch.cpp:266-273, Geometry/Mesh.cpp:10-63):
 // outV = pos + offset and outN = angle-weighted unit vertex normals (nv x 3
 // floats each), triN = unit face normals (nt x 3); idx = 3 nt render indices

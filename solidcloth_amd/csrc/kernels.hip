@@ -867,6 +867,56 @@ __global__ void __launch_bounds__(kBlock) k_copy(const float4* __restrict__ src,
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+// Physics/ClothPBD.cpp:267-282: the barycentric point p of the spring's
+// particles is pulled towards the mouse point; each particle moves by
+// -h a_k disp and its velocity by -a_k disp, pinned particles included, one
+// after another (as the reference's statements, should two indices agree).
+__global__ void k_mouse_spring(float4* __restrict__ pos, float4* __restrict__ vel,
+                               const MouseSpringDev* __restrict__ msp, float h, SolverCtl* ctl, int resetTravel)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const MouseSpringDev m = *msp;
+    const f3 x1 = xyz(pos[m.i1]), x2 = xyz(pos[m.i2]), x3 = xyz(pos[m.i3]);
+    const f3 p = add(add(mul(m.a1, x1), mul(m.a2, x2)), mul(m.a3, x3));
+    f3 delta = sub(p, f3{m.px, m.py, m.pz});
+    const float l = len(delta);
+    delta = mul(1.f / l, delta);
+    const f3 disp = mul(m.stiffness * (l - m.len), delta);
+    const int ids[3] = {m.i1, m.i2, m.i3};
+    const float as[3] = {m.a1, m.a2, m.a3};
+    for (int k = 0; k < 3; k++) {
+        const float4 P = pos[ids[k]];
+        pos[ids[k]] = f4(sub(xyz(P), mul(h * as[k], disp)), P.w);
+        const float4 V = vel[ids[k]];
+        vel[ids[k]] = f4(sub(xyz(V), mul(as[k], disp)), V.w);
+    }
+    if (resetTravel) ctl->maxTv = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_max_travel(const float4* __restrict__ pos,
+                                                       const float4* __restrict__ prev, int n, SolverCtl* ctl)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float tv = 0.f;
+    if (i < n) {
+        tv = len(sub(xyz(pos[i]), xyz(prev[i])));
+        tv = tv > 0.f ? tv : 0.f;  // a NaN never raises the reference's maximum
+    }
+    wave_max_atomic(&ctl->maxTv, tv);
+}
+
+void launch_mouse_spring(hipStream_t s, float4* pos, float4* vel, const MouseSpringDev* ms, float h, SolverCtl* ctl,
+                         int resetTravel)
+{
+    hipLaunchKernelGGL(k_mouse_spring, dim3(1), dim3(64), 0, s, pos, vel, ms, h, ctl, resetTravel);
+}
+
+void launch_max_travel(hipStream_t s, const float4* pos, const float4* prev, int n, SolverCtl* ctl)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_max_travel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pos, prev, n, ctl);
+}
+
 void launch_copy(hipStream_t s, const float4* src, float4* dst, long n)
 {
     if (n <= 0) return;

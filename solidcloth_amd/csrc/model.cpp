@@ -277,6 +277,7 @@ void GpuSolver::release()
     dev_free(render.idx); dev_free(render.vtOff); dev_free(render.vt); dev_free(render.triA); dev_free(render.triB);
     dev_free(render.outV); dev_free(render.outN); dev_free(render.triN);
     render.nv = render.nt = -1;
+    dev_free(d_mouse);
     warm.built = warm.saved = false;
     dev_free(d_pos); dev_free(d_prev); dev_free(d_vel);
     dev_free(d_tidx); dev_free(d_tA); dev_free(d_tB); dev_free(d_tC); dev_free(d_bidx);
@@ -785,6 +786,7 @@ std::string ClothModel::graph_key(float h, const CollSet& cs) const
     put(&mBendStiff, 4); put(&mPoisson, 4); put(&mFriction, 4); put(&mDragCoeff, 4);
     put(&mNumIterations, 4); put(&mCollFlags, 4); put(&mBendFreq, 4);
     put(&mArea, 4); put(&mDihedral, 1); put(&mFEM, 1); put(&mFixedIterations, 1); put(&mSweepMode, 4); put(&mSlabSize, 4);
+    put(&msActive, 1); put(&gpu.d_mouse, sizeof(void*));
     put(&cs, sizeof(cs));
     put(&gpu.d_cP, sizeof(void*));
     put(&gpu.d_cFeat, sizeof(void*));
@@ -861,6 +863,12 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
         // prediction and detection are separate launches
         timed_begin(g.timer, timed, 4, s, ev);
         launch_predict_mesh(s, g.d_pos, g.d_prev, g.d_vel, g.nv, h, gdv, mDragCoeff, g.d_ctl, ecb, tcb);
+        if (msActive) {
+            // the spring moves its particles after the prediction and before
+            // the travel maximum is taken (ClothPBD.cpp:267-285)
+            launch_mouse_spring(s, g.d_pos, g.d_vel, g.d_mouse, h, g.d_ctl, 1);
+            launch_max_travel(s, g.d_pos, g.d_prev, g.nv, g.d_ctl);
+        }
         timed_end(g.timer, timed, 4, s, ev);
         timed_begin(g.timer, timed, 3, s, ev);
         if (anyContacts)
@@ -883,6 +891,13 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
             launch_self_colour(s, S.buf[1], 1, S.vmax, g.d_ctl);
         }
         timed_end(g.timer, timed, 3, s, ev);
+    } else if (anyContacts && msActive) {
+        // detection must see the spring's moves: three launches
+        timed_begin(g.timer, timed, 3, s, ev);
+        launch_predict(s, g.d_pos, g.d_prev, g.d_vel, g.nv, h, gdv, mDragCoeff, g.d_ctl);
+        launch_mouse_spring(s, g.d_pos, g.d_vel, g.d_mouse, h, g.d_ctl, 0);
+        launch_detect(s, g.d_pos, g.d_prev, g.nv, cs, cb, mThickness, mTolerance, mCollFlags, g.d_ctl);
+        timed_end(g.timer, timed, 3, s, ev);
     } else if (anyContacts) {
         timed_begin(g.timer, timed, 3, s, ev);
         launch_predict_detect(s, g.d_pos, g.d_prev, g.d_vel, g.nv, h, gdv, mDragCoeff, g.d_ctl, cs, cb, mThickness,
@@ -891,6 +906,7 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
     } else {
         timed_begin(g.timer, timed, 4, s, ev);
         launch_predict(s, g.d_pos, g.d_prev, g.d_vel, g.nv, h, gdv, mDragCoeff, g.d_ctl);
+        if (msActive) launch_mouse_spring(s, g.d_pos, g.d_vel, g.d_mouse, h, g.d_ctl, 0);
         timed_end(g.timer, timed, 4, s, ev);
     }
 
@@ -1055,7 +1071,6 @@ void ClothModel::step_substeps(float h, int n)
 {
     if (mNumIterations < 1) throw std::runtime_error("NumIterations must be >= 1");
     if (mBendFreq < 1) throw std::runtime_error("bend frequency must be >= 1");
-    if (msActive) throw std::runtime_error("the mouse spring is not implemented on the GPU substep");
     const CollisionWorld* w = mOwner->mWorld;
     if (w->colls.size() > MAX_COLLIDABLES) throw std::runtime_error("too many collidables (max 8)");
     for (const auto& c : w->colls) {
@@ -1083,6 +1098,21 @@ void ClothModel::step_substeps(float h, int n)
     GpuSolver& g = gpu;
     const bool timed = g.timer.enabled;
     if (mWarm && mWarmMode != 0) ensure_warm();
+    if (msActive) {
+        const int nv = (int)pos.size();
+        for (int k = 0; k < 3; k++)
+            if (msI[k] < 0 || msI[k] >= nv) throw std::runtime_error("mouse spring particle index out of range");
+        if (!g.d_mouse) {
+            dev_alloc(g.d_mouse, 1);
+            msDirty = true;
+        }
+        if (msDirty) {
+            const MouseSpringDev d{msI[0], msI[1], msI[2], msA[0], msA[1], msA[2], msPoint.x, msPoint.y, msPoint.z,
+                                   msLen, msStiffness};
+            HIP_OK(hipMemcpy(g.d_mouse, &d, sizeof(d), hipMemcpyHostToDevice));
+            msDirty = false;
+        }
+    }
     // the plain solver judges a warm-started step: a step whose residual
     // exceeds the last plain step's (or the set threshold) is rerun plain
     // from the saved state, and the first step calibrates
@@ -1455,7 +1485,7 @@ void ClothModel::validate_batch() const
     if (!mFEM) throw std::runtime_error("batched cloths use the FEM stretch model (scSetUseFEM(true))");
     if (mNumIterations < 1) throw std::runtime_error("NumIterations must be >= 1");
     if (mBendFreq < 1) throw std::runtime_error("bend frequency must be >= 1");
-    if (msActive) throw std::runtime_error("the mouse spring is not implemented on the GPU substep");
+    if (msActive) throw std::runtime_error("a batched patch cannot carry a mouse spring (step it with scStepCloth)");
     if (mCollFlags & (CF_EDGES | CF_TRIANGLES | CF_SELF))
         throw std::runtime_error("batched cloths support vertex and wall contacts only");
     for (const auto& c : mOwner->mWorld->colls)

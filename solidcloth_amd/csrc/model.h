@@ -270,6 +270,7 @@ public:
         float *outV = nullptr, *outN = nullptr, *triN = nullptr;
         int nv = -1, nt = -1;
     } render;
+    MouseSpringDev* d_mouse = nullptr;
     // two cached substep graphs (the warm-started and the plain substep)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -313,6 +314,7 @@ public:
     float msA[3] = {0, 0, 0};
     f3 msPoint{0, 0, 0};
     float msLen = 0.f, msStiffness = 100.f;
+    bool msDirty = true;  // the device copy of the spring is stale
 
     int statIterations = 0;
     float statBendError = 0.f, statContactError = 0.f, statSelfVTError = 0.f, statSelfEEError = 0.f;
