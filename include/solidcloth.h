@@ -17,9 +17,8 @@
  *
  * Difference: the reference reports no errors. This library additionally
  * records the last error (scGetLastError) and refuses calls it cannot honour
- * (for example a capsule collidable, which this build does not implement on
- * the GPU substep, or a mouse spring on a batched patch) instead of
- * computing something else. All device work started by a call has
+ * (for example a capsule or a mouse spring on a batched patch, which the
+ * batch kernel does not handle) instead of computing something else. All device work started by a call has
  * completed before any getter returns data.
  */
 #ifndef SOLIDCLOTH_H

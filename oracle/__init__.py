@@ -39,6 +39,7 @@ def lib():
                                    _C.c_int, _C.c_int, _C.c_float]
         L.oc_add_walls.argtypes = [vp, vp]
         L.oc_add_sphere.argtypes = [vp, _C.c_float, _C.c_float, _C.c_float, _C.c_float]
+        L.oc_add_capsule.argtypes = [vp] + [_C.c_float] * 9 + [vp]
         L.oc_add_grid_sdf.argtypes = [vp, _C.c_int, _C.c_int, _C.c_int, vp, vp]
         L.oc_sdf_query.argtypes = [vp, _C.c_int, _C.c_float, _C.c_float, _C.c_float, vp]
         L.oc_get_sdf_grads.argtypes = [vp, _C.c_int, vp]
@@ -90,6 +91,8 @@ def lib():
         L.oc_closest_pt_point_triangle.argtypes = [vp, vp, vp, vp, vp]
         L.oc_closest_pt_point_triangle.restype = _C.c_int
         L.oc_closest_pt_segm_segm.argtypes = [vp, vp, vp, vp, vp]
+        L.oc_closest_pt_segm_triangle.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.oc_closest_pt_segm.argtypes = [vp, vp, vp, vp]
         L.oc_closest_pt_segm_segm.restype = _C.c_int
         L.oc_signed_volume.argtypes = [vp, vp, vp, vp]
         L.oc_signed_volume.restype = _C.c_float
@@ -192,6 +195,14 @@ class Cloth:
 
     def add_sphere(self, c, r):
         return self.L.oc_add_sphere(self.h, c[0], c[1], c[2], r)
+
+    def add_capsule(self, center, radius, half_height, rot=(1.0, 0.0, 0.0, 0.0)):
+        """Capsule collidable (scCreateCapsule); rot = (s, x, y, z). Returns
+        the world index and the axis ends (2, 3) as the oracle computed them."""
+        axis = _np.zeros(6, dtype=_np.float32)
+        i = self.L.oc_add_capsule(self.h, center[0], center[1], center[2], radius, half_height, rot[0], rot[1],
+                                  rot[2], rot[3], _ptr(axis))
+        return i, axis.reshape(2, 3)
 
     def add_grid_sdf(self, values, n, bmin, bmax):
         box = _np.array(list(bmin) + list(bmax), dtype=_np.float32)
@@ -358,6 +369,23 @@ def closest_pt_segm_segm(p1, q1, p2, q2):
     return r, out[0], out[1], out[2:5], out[5:8]
 
 
+def closest_pt_segm(p, a, b):
+    """(t, closest point) of p on segment [a, b] in FP32."""
+    out = _np.zeros(4, dtype=_np.float32)
+    args = [_f3(x) for x in (p, a, b)]
+    lib().oc_closest_pt_segm(*[_ptr(x) for x in args], _ptr(out))
+    return out[0], out[1:4]
+
+
+def closest_pt_segm_triangle(p, q, a, b, c):
+    """(distance, param, barycentric, closest on the segment, closest on the
+    triangle) in FP32."""
+    out = _np.zeros(11, dtype=_np.float32)
+    args = [_f3(x) for x in (p, q, a, b, c)]
+    lib().oc_closest_pt_segm_triangle(*[_ptr(x) for x in args], _ptr(out))
+    return out[0], out[1], out[2:5], out[5:8], out[8:11]
+
+
 def signed_volume(a0, a1, a2, a3):
     args = [_f3(x) for x in (a0, a1, a2, a3)]
     return lib().oc_signed_volume(*[_ptr(x) for x in args])
@@ -393,6 +421,8 @@ def from_scene(scene, real64=False, tri_order=None, bend_order=None):
     if scene.mesh_subdiv >= 0:
         v, t = S.icosphere(scene.mesh_subdiv, scene.mesh_radius, scene.mesh_center)
         c.add_mesh(v, t)
+    if scene.capsule is not None:
+        c.add_capsule(*scene.capsule)
     if scene.walls_min is not None:
         c.add_walls(scene.walls_min, scene.walls_max)
     c.init_grid(scene.n, scene.n, float(scene.inc), scene.offset, scene.horizontal, scene.attached)

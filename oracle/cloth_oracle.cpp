@@ -35,7 +35,7 @@
 namespace oracle {
 
 enum { CF_VERTICES = 1, CF_TRIANGLES = 2, CF_SELF = 4, CF_WALLS = 8, CF_EDGES = 16 };
-enum { CT_WALLS = 1, CT_SPHERE = 3, CT_MESH = 4, CT_SDF = 7 };
+enum { CT_WALLS = 1, CT_SPHERE = 3, CT_MESH = 4, CT_CAPSULE = 6, CT_SDF = 7 };
 
 struct Params {
     // defaults: Physics/ClothModel.cpp:10-36
@@ -278,7 +278,8 @@ struct Collidable {
     int type;
     V3<R> bmin, bmax;  // walls box
     V3<R> center;      // sphere
-    R radius = 0;
+    R radius = 0;      // sphere, capsule
+    V3<R> axisP, axisQ;  // capsule axis ends
     int sdf = -1;      // index into sdfs
     int mesh = -1;     // index into meshes
 };
@@ -492,6 +493,48 @@ public:
         return len - s.radius;
     }
 
+    // Physics/ClothCollisionSDF.h:57-122 (CapsuleSDF): QueryPoint,
+    // QuerySegment and QueryTriangle against the axis [P, Q] grown by r
+    static R capsule_query(const Collidable<R>& k, V3<R> x, V3<R>& cp, V3<R>& n)
+    {
+        R t;
+        const V3<R> s = closest_pt_segm(x, k.axisP, k.axisQ, t);
+        V3<R> d = x - s;
+        R len = length(d);
+        scale(d, R(1) / len);
+        n = d;
+        cp = s + k.radius * d;
+        return len - k.radius;
+    }
+
+    static R capsule_query_segment(const Collidable<R>& k, V3<R> p, V3<R> q, V3<R>& cp, V3<R>& n, R& w1, R& w2)
+    {
+        R s, t;
+        V3<R> c1, c2;
+        closest_pt_segm_segm(p, q, k.axisP, k.axisQ, s, t, c1, c2);
+        V3<R> d = c1 - c2;
+        R len = length(d);
+        scale(d, R(1) / len);
+        n = d;
+        cp = c2 + k.radius * d;
+        w1 = R(1.0f) - s;
+        w2 = s;
+        return len - k.radius;
+    }
+
+    static R capsule_query_triangle(const Collidable<R>& k, V3<R> a, V3<R> b, V3<R> c, V3<R>& cp, V3<R>& n,
+                                    V3<R>& bar)
+    {
+        R t;
+        V3<R> ce, ct;
+        const R len = closest_pt_segm_triangle(k.axisP, k.axisQ, a, b, c, t, bar, ce, ct);
+        V3<R> d = ct - ce;
+        scale(d, R(1) / len);
+        n = d;
+        cp = ce + k.radius * d;
+        return len - k.radius;
+    }
+
     // Physics/ClothCollision.cpp:430-470 (vertex branch of SDFCollisions; the
     // defaults handleInside = handleCurrent = false)
     template <class Q>
@@ -608,6 +651,18 @@ public:
                     },
                     [&](V3<R> a, V3<R> b, V3<R> cc, V3<R>& cp, V3<R>& n, V3<R>& bar) {
                         return sphere_query_triangle(c, a, b, cc, cp, n, bar);
+                    },
+                    (int)ci);
+            } else if (c.type == CT_CAPSULE) {
+                sdf_vertex_collisions([&](V3<R> p, V3<R>& cp, V3<R>& n) { return capsule_query(c, p, cp, n); },
+                                      (int)ci);
+                sdf_feature_collisions(
+                    [&](V3<R> p, V3<R> q, V3<R>& cp, V3<R>& n, R& w1, R& w2, R& d) {
+                        d = capsule_query_segment(c, p, q, cp, n, w1, w2);
+                        return true;
+                    },
+                    [&](V3<R> a, V3<R> b, V3<R> cc, V3<R>& cp, V3<R>& n, V3<R>& bar) {
+                        return capsule_query_triangle(c, a, b, cc, cp, n, bar);
                     },
                     (int)ci);
             } else if (c.type == CT_SDF) {
@@ -1725,6 +1780,42 @@ int oc_add_sphere(void* h, float cx, float cy, float cz, float r)
     });
 }
 
+// Math/Quaternion.cpp:116-147: qRotate(q, v) = (q (v q*)).v, the product of
+// a vector and a quaternion taking the vector as (0, v); FP32 as the
+// reference's Capsule stores its data.
+static V3<float> quat_rotate(float qs, V3<float> qv, V3<float> v)
+{
+    const V3<float> iv{-qv.x, -qv.y, -qv.z};
+    const float ts = 0.f * qs - dot(v, iv);
+    const V3<float> tv = (0.f * iv + qs * v) + cross(v, iv);
+    return (qs * tv + ts * qv) + cross(qv, tv);
+}
+
+// Capsule (Physics/Common.h; ClothCollisionSDF.h:62-68): axis ends
+// centre +- qRotate(rot, (0, hh, 0)); rotation (qs; qx, qy, qz)
+int oc_add_capsule(void* h, float cx, float cy, float cz, float r, float hh, float qs, float qx, float qy, float qz,
+                   float* axisOut)
+{
+    return with(h, [&](auto& c) {
+        using R = typename std::remove_reference<decltype(c.pos[0].x)>::type;
+        const V3<float> ctr{cx, cy, cz};
+        const V3<float> half = quat_rotate(qs, V3<float>{qx, qy, qz}, V3<float>{0.f, hh, 0.f});
+        const V3<float> P = ctr + half, Q = ctr - half;
+        if (axisOut) {
+            axisOut[0] = P.x; axisOut[1] = P.y; axisOut[2] = P.z;
+            axisOut[3] = Q.x; axisOut[4] = Q.y; axisOut[5] = Q.z;
+        }
+        Collidable<R> k;
+        k.type = CT_CAPSULE;
+        k.center = V3<R>{R(cx), R(cy), R(cz)};
+        k.radius = r;
+        k.axisP = V3<R>{R(P.x), R(P.y), R(P.z)};
+        k.axisQ = V3<R>{R(Q.x), R(Q.y), R(Q.z)};
+        c.world.push_back(k);
+        return (int)c.world.size() - 1;
+    });
+}
+
 // values: (nx+1)(ny+1)(nz+1) node samples, x fastest (Geometry/SDF.h:49-52)
 int oc_add_grid_sdf(void* h, int nx, int ny, int nz, const float* box6, const float* values)
 {
@@ -2075,6 +2166,28 @@ int oc_closest_pt_segm_segm(const float* p1, const float* q1, const float* p2, c
     float o[8] = {s, t, c1.x, c1.y, c1.z, c2.x, c2.y, c2.z};
     memcpy(out, o, sizeof(o));
     return r;
+}
+
+// out = t, closest point (3)
+void oc_closest_pt_segm(const float* p, const float* a, const float* b, float* out)
+{
+    float t = 0.f;
+    const V3<float> cp =
+        closest_pt_segm(V3<float>{p[0], p[1], p[2]}, V3<float>{a[0], a[1], a[2]}, V3<float>{b[0], b[1], b[2]}, t);
+    out[0] = t; out[1] = cp.x; out[2] = cp.y; out[3] = cp.z;
+}
+
+// out = dist, param, bar (3), closest on the segment (3), closest on the triangle (3)
+void oc_closest_pt_segm_triangle(const float* p, const float* q, const float* a, const float* b, const float* c,
+                                 float* out)
+{
+    float param = 0.f;
+    V3<float> bar{0, 0, 0}, ce{0, 0, 0}, ct{0, 0, 0};
+    const float d = closest_pt_segm_triangle(V3<float>{p[0], p[1], p[2]}, V3<float>{q[0], q[1], q[2]},
+                                             V3<float>{a[0], a[1], a[2]}, V3<float>{b[0], b[1], b[2]},
+                                             V3<float>{c[0], c[1], c[2]}, param, bar, ce, ct);
+    float o[11] = {d, param, bar.x, bar.y, bar.z, ce.x, ce.y, ce.z, ct.x, ct.y, ct.z};
+    memcpy(out, o, sizeof(o));
 }
 
 float oc_signed_volume(const float* a0, const float* a1, const float* a2, const float* a3)

@@ -11,6 +11,7 @@
 // (tests/golden/ref_collision3d.npz, ref_meshes.npz).
 #pragma once
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -137,6 +138,53 @@ int closest_pt_segm_segm(V3<R> p1, V3<R> q1, V3<R> p2, V3<R> q2, R& s, R& t, V3<
     c1 = p1 + s * d1;
     c2 = p2 + t * d2;
     return region;
+}
+
+// Geometry/Collision3D.cpp:143-210 (ClosestPtSegmTriangle): nearest of the
+// segment against each edge and each end point against the triangle, first
+// strictly nearer kept, with the reference's coordinates — including that the
+// query of end point q overwrites them whatever its outcome and, when q is
+// nearest, hands them p's.
+template <class R>
+R closest_pt_segm_triangle(V3<R> p, V3<R> q, V3<R> a, V3<R> b, V3<R> c, R& param, V3<R>& bar, V3<R>& closestEdge,
+                           V3<R>& closestTri)
+{
+    R minDist = R(FLT_MAX);
+    R s, t;
+    V3<R> c1, c2;
+    closest_pt_segm_segm(p, q, a, b, s, t, c1, c2);
+    R dist1 = length(c1 - c2);
+    if (dist1 < minDist) {
+        minDist = dist1; closestEdge = c1; closestTri = c2; param = s;
+        bar = V3<R>{R(1) - t, t, R(0)};
+    }
+    closest_pt_segm_segm(p, q, b, c, s, t, c1, c2);
+    R dist2 = length(c1 - c2);
+    if (dist2 < minDist) {
+        minDist = dist2; closestEdge = c1; closestTri = c2; param = s;
+        bar = V3<R>{R(0), R(1) - t, t};
+    }
+    closest_pt_segm_segm(p, q, c, a, s, t, c1, c2);
+    R dist3 = length(c1 - c2);
+    if (dist3 < minDist) {
+        minDist = dist3; closestEdge = c1; closestTri = c2; param = s;
+        bar = V3<R>{t, R(0), R(1) - t};
+    }
+    V3<R> coords, cp4;
+    closest_pt_point_triangle(p, a, b, c, cp4, coords);
+    R dist4 = length(p - cp4);
+    if (dist4 < minDist) {
+        minDist = dist4; closestEdge = p; closestTri = cp4; param = 0;
+        bar = coords;
+    }
+    V3<R> cp5;
+    closest_pt_point_triangle(q, a, b, c, cp5, bar);
+    R dist5 = length(q - cp5);
+    if (dist5 < minDist) {
+        minDist = dist5; closestEdge = q; closestTri = cp5; param = 1;
+        bar = coords;
+    }
+    return minDist;
 }
 
 template <class R>

@@ -75,6 +75,31 @@ int ref_closest_pt_segm_segm(const float* p1, const float* q1, const float* p2, 
     return r;
 }
 
+// Geometry/Collision3D.cpp:143-210 ClosestPtSegmTriangle; out = dist, param,
+// bar (3), closest on the segment (3), closest on the triangle (3)
+void ref_closest_pt_segm_triangle(const float* p, const float* q, const float* a, const float* b, const float* c,
+                                  float* out)
+{
+    float param = 0.f;
+    Math::BarycentricCoords bar;
+    Vector3 ce, ct;
+    float d = Geometry::ClosestPtSegmTriangle(Vector3(p[0], p[1], p[2]), Vector3(q[0], q[1], q[2]),
+                                              Vector3(a[0], a[1], a[2]), Vector3(b[0], b[1], b[2]),
+                                              Vector3(c[0], c[1], c[2]), param, bar, ce, ct);
+    float o[11] = {d, param, bar.x, bar.y, bar.z, ce.x, ce.y, ce.z, ct.x, ct.y, ct.z};
+    memcpy(out, o, sizeof(o));
+}
+
+// Geometry/Collision3D.h:125-141 ClosestPtSegm; out = t, closest point (3)
+void ref_closest_pt_segm(const float* p, const float* a, const float* b, float* out)
+{
+    float t = 0.f;
+    Vector3 cp = Geometry::ClosestPtSegm(Vector3(p[0], p[1], p[2]), Vector3(a[0], a[1], a[2]),
+                                         Vector3(b[0], b[1], b[2]), t);
+    float o[4] = {t, cp.x, cp.y, cp.z};
+    memcpy(out, o, sizeof(o));
+}
+
 // Geometry/Collision3D.cpp:298-301
 float ref_signed_volume(const float* a0, const float* a1, const float* a2, const float* a3)
 {

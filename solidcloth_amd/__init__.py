@@ -227,6 +227,14 @@ class CollisionWorld:
         self.AddCollidable(h)
         return h
 
+    def AddCapsule(self, center, radius, half_height, rot=(1.0, 0.0, 0.0, 0.0)):
+        """Capsule collidable (scCreateCapsule): axis along the rotated y axis,
+        rot = (s, x, y, z)."""
+        q = scQuaternion(float(rot[0]), v3(rot[1:4]))
+        h = _checked("scCreateCapsule", v3(center), float(radius), float(half_height), q)
+        self.AddCollidable(h)
+        return h
+
     def AddSDF(self, sdf):
         self._keep.append(sdf)
         h = _checked("scCreateSDF", sdf.h)

@@ -77,6 +77,55 @@ __device__ __forceinline__ float sphere_triangle(const CollDev& c, f3 a, f3 b, f
     return l - c.r;
 }
 
+// Physics/ClothCollisionSDF.h:57-122 (CapsuleSDF): a capsule is its axis
+// [P, Q] = (ax, ay, az)-(bx, by, bz) grown by radius r.
+// QueryPoint: distance of x to the surface, surface point and normal.
+__device__ __forceinline__ float capsule_point(const CollDev& c, f3 x, f3& cp, f3& n)
+{
+    const f3 P{c.ax, c.ay, c.az}, Q{c.bx, c.by, c.bz};
+    float t;
+    const f3 s = closest_pt_segm(x, P, Q, t);
+    f3 d = sub(x, s);
+    const float l = len(d);
+    d = mul(1.f / l, d);
+    n = d;
+    cp = add(s, mul(c.r, d));
+    return l - c.r;
+}
+
+// QuerySegment: segment [p, q] against the axis; coordinates (1 - s, s) on
+// the segment.
+__device__ __forceinline__ float capsule_segment(const CollDev& c, f3 p, f3 q, f3& cp, f3& n, float& w1, float& w2)
+{
+    const f3 P{c.ax, c.ay, c.az}, Q{c.bx, c.by, c.bz};
+    float s, t;
+    f3 c1, c2;
+    closest_pt_segm_segm(p, q, P, Q, s, t, c1, c2);
+    f3 d = sub(c1, c2);
+    const float l = len(d);
+    d = mul(1.f / l, d);
+    n = d;
+    cp = add(c2, mul(c.r, d));
+    w1 = 1.0f - s;
+    w2 = s;
+    return l - c.r;
+}
+
+// QueryTriangle: the axis against triangle (a, b, cc), coordinates as
+// closest_pt_segm_triangle leaves them.
+__device__ __forceinline__ float capsule_triangle(const CollDev& c, f3 a, f3 b, f3 cc, f3& cp, f3& n, f3& bar)
+{
+    const f3 P{c.ax, c.ay, c.az}, Q{c.bx, c.by, c.bz};
+    float t;
+    f3 ce, ct;
+    const float l = closest_pt_segm_triangle(P, Q, a, b, cc, t, bar, ce, ct);
+    f3 d = sub(ct, ce);
+    d = mul(1.f / l, d);
+    n = d;
+    cp = add(ce, mul(c.r, d));
+    return l - c.r;
+}
+
 // Physics/ClothCollisionSDF.h:148-205 (GridSDF::QueryTriangle): Frank-Wolfe
 // minimisation of the SDF over the triangle from 0.33 (a + b + c), step
 // 2 / (i + 2), at most 99 steps, stopped at a relative change below 1e-4 or

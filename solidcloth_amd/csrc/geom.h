@@ -107,6 +107,69 @@ SC_HD f3 closest_pt_segm(f3 p, f3 a, f3 b, float& t)
     return add(a, mul(t, u));
 }
 
+// Geometry/Collision3D.cpp:143-210: the nearest pair of points of segment
+// [p, q] and triangle (a, b, c) among the segment against each edge and each
+// end point against the triangle, first strictly nearer kept. Coordinates as
+// the reference leaves them: an edge sets (1 - t, t) on its two corners, end
+// point p its barycentric coordinates; the query of end point q writes q's
+// coordinates whatever the outcome, and when q is nearest they are then
+// replaced by p's. Returns the distance.
+SC_HD float closest_pt_segm_triangle(f3 p, f3 q, f3 a, f3 b, f3 c, float& param, f3& bar, f3& closestEdge,
+                                     f3& closestTri)
+{
+    float minDist = 3.402823466e+38f;  // FLT_MAX
+    float s, t;
+    f3 c1, c2;
+    closest_pt_segm_segm(p, q, a, b, s, t, c1, c2);
+    const float dist1 = len(sub(c1, c2));
+    if (dist1 < minDist) {
+        minDist = dist1;
+        closestEdge = c1;
+        closestTri = c2;
+        param = s;
+        bar = f3{1.f - t, t, 0.f};
+    }
+    closest_pt_segm_segm(p, q, b, c, s, t, c1, c2);
+    const float dist2 = len(sub(c1, c2));
+    if (dist2 < minDist) {
+        minDist = dist2;
+        closestEdge = c1;
+        closestTri = c2;
+        param = s;
+        bar = f3{0.f, 1.f - t, t};
+    }
+    closest_pt_segm_segm(p, q, c, a, s, t, c1, c2);
+    const float dist3 = len(sub(c1, c2));
+    if (dist3 < minDist) {
+        minDist = dist3;
+        closestEdge = c1;
+        closestTri = c2;
+        param = s;
+        bar = f3{t, 0.f, 1.f - t};
+    }
+    f3 coords, cp4;
+    closest_pt_point_triangle(p, a, b, c, cp4, coords);
+    const float dist4 = len(sub(p, cp4));
+    if (dist4 < minDist) {
+        minDist = dist4;
+        closestEdge = p;
+        closestTri = cp4;
+        param = 0.f;
+        bar = coords;
+    }
+    f3 cp5;
+    closest_pt_point_triangle(q, a, b, c, cp5, bar);
+    const float dist5 = len(sub(q, cp5));
+    if (dist5 < minDist) {
+        minDist = dist5;
+        closestEdge = q;
+        closestTri = cp5;
+        param = 1.f;
+        bar = coords;
+    }
+    return minDist;
+}
+
 // Collision3D.h:42-58: barycentric coordinates of p in (a, b, c)
 SC_HD f3 barycentric(f3 a, f3 b, f3 c, f3 p)
 {

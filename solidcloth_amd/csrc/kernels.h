@@ -68,9 +68,9 @@ struct MeshGeo {
 struct CollDev {
     int type;
     int pad;
-    float ax, ay, az;  // walls: box min; sphere: centre; sdf: box min
-    float bx, by, bz;  // walls: box max; sdf: cell steps
-    float r;           // sphere radius
+    float ax, ay, az;  // walls: box min; sphere: centre; sdf: box min; capsule: axis end P
+    float bx, by, bz;  // walls: box max; sdf: cell steps; capsule: axis end Q
+    float r;           // sphere / capsule radius
     const float4* nodes;
     const MeshGeo* mesh;  // mesh obstacle (device copy)
     int nc0, nc1, nc2;

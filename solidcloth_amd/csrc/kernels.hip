@@ -93,6 +93,11 @@ __device__ __forceinline__ void detect_vertex(int v, const f3 p, const f3 pv, co
             f3 cp = add(ctr, mul(c.r, d));
             float dist = l - c.r;
             if (dist < radTol && dist >= 0.f) put_contact(cb, v, cnt, cp, d, ci);
+        } else if (c.type == CT_CAPSULE) {
+            if (!(collFlags & CF_VERTICES)) continue;
+            f3 cp, n;
+            const float dist = capsule_point(c, pv, cp, n);
+            if (dist < radTol && dist >= 0.f) put_contact(cb, v, cnt, cp, n, ci);
         } else if (c.type == CT_SDF) {
             if (!(collFlags & CF_VERTICES)) continue;
             // Physics/ClothCollisionSDF.h:130-139

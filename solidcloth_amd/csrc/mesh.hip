@@ -115,10 +115,11 @@ __global__ void __launch_bounds__(kBlock) k_detect_features(const float4* __rest
                 NearestResult r;
                 if (nearest_edge(*c.mesh, p, q, meshRad, r))
                     put_feature(ec, e, cnt, r.point, r.normal, 1.0f - r.param, r.param, 0.f, r.feature, ci);
-            } else if (c.type == CT_SPHERE) {
+            } else if (c.type == CT_SPHERE || c.type == CT_CAPSULE) {
                 f3 cp, n;
                 float w1, w2;
-                const float dist = sphere_segment(c, p, q, cp, n, w1, w2);
+                const float dist = c.type == CT_SPHERE ? sphere_segment(c, p, q, cp, n, w1, w2)
+                                                       : capsule_segment(c, p, q, cp, n, w1, w2);
                 if (dist < sdfRad && dist >= 0.f &&
                     ((w1 > eps && w1 < 1.f - eps && w2 > eps && w2 < 1.f - eps) || borderE))
                     put_feature(ec, e, cnt, cp, n, w1, w2, 0.f, -1, ci);
@@ -145,10 +146,11 @@ __global__ void __launch_bounds__(kBlock) k_detect_features(const float4* __rest
                 NearestResult r;
                 if (nearest_vertex(*c.mesh, a, b, cc, meshRad, r))
                     put_feature(tc, t, cnt, r.point, r.normal, r.bary.x, r.bary.y, r.bary.z, r.feature, ci);
-            } else if (c.type == CT_SPHERE || c.type == CT_SDF) {
+            } else if (c.type == CT_SPHERE || c.type == CT_SDF || c.type == CT_CAPSULE) {
                 f3 cp, n, bar;
-                const float dist = c.type == CT_SPHERE ? sphere_triangle(c, a, b, cc, cp, n, bar)
-                                                       : sdf_triangle(c, a, b, cc, cp, n, bar);
+                const float dist = c.type == CT_SPHERE    ? sphere_triangle(c, a, b, cc, cp, n, bar)
+                                   : c.type == CT_CAPSULE ? capsule_triangle(c, a, b, cc, cp, n, bar)
+                                                          : sdf_triangle(c, a, b, cc, cp, n, bar);
                 if (dist < sdfRad && dist >= 0.f && ((bar.x > eps && bar.y > eps && bar.z > eps) || borderT))
                     put_feature(tc, t, cnt, cp, n, bar.x, bar.y, bar.z, -1, ci);
             }

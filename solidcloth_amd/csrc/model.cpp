@@ -105,6 +105,29 @@ void Mesh::construct_edges()
     edges.push_back(cur);
 }
 
+// Math/Quaternion.cpp:116-122,128-131,142-147: qRotate(q, v) = (q (v q*)).v
+// with qMultiply(a, b) = (a.s b.s - a.v.b.v, a.s b.v + b.s a.v + a.v x b.v)
+// and the product of a vector v with a quaternion taken as (0, v).
+static f3 quat_rotate(const scQuaternion& q, f3 v)
+{
+    const f3 qv{q.v.x, q.v.y, q.v.z};
+    const f3 iv = neg(qv);  // conjugate (q.s, -q.v)
+    // t = (0, v) * (q.s, -q.v)
+    const float ts = 0.f * q.s - dot(v, iv);
+    const f3 tv = add(add(mul(0.f, iv), mul(q.s, v)), cross(v, iv));
+    // r = q * t; only its vector part is kept
+    return add(add(mul(q.s, tv), mul(ts, qv)), cross(qv, tv));
+}
+
+// Physics/ClothCollisionSDF.h:62-68 (CapsuleSDF): the axis ends
+// centre +- qRotate(rot, (0, hh, 0)).
+void capsule_axis(const Capsule& k, f3& P, f3& Q)
+{
+    const f3 half = quat_rotate(k.rot, f3{0.f, k.hh, 0.f});
+    P = add(k.center, half);
+    Q = sub(k.center, half);
+}
+
 // Geometry/Mesh.cpp:10-63 (angle-weighted vertex normals), for rendering.
 void Mesh::compute_normals()
 {
@@ -749,6 +772,16 @@ CollSet ClothModel::make_collset(int& slots, int& edgeSlots, int& triSlots)
                 throw std::runtime_error("edge contacts against a grid SDF need CF_VERTICES "
                                          "(the reference's grid-SDF segment query is a stub)");
             triSlots += ft;
+        } else if (c->type == CT_CAPSULE) {
+            const Capsule* k = (const Capsule*)c;
+            f3 P, Q;
+            capsule_axis(*k, P, Q);
+            d.ax = P.x; d.ay = P.y; d.az = P.z;
+            d.bx = Q.x; d.by = Q.y; d.bz = Q.z;
+            d.r = k->r;
+            if (mCollFlags & CF_VERTICES) slots += 1;
+            edgeSlots += fe;
+            triSlots += ft;
         } else if (c->type == CT_MESH) {
             Mesh* me = ((const CollisionMesh*)c)->mesh;
             if (!me) throw std::runtime_error("mesh collidable without a mesh");
@@ -1074,7 +1107,8 @@ void ClothModel::step_substeps(float h, int n)
     const CollisionWorld* w = mOwner->mWorld;
     if (w->colls.size() > MAX_COLLIDABLES) throw std::runtime_error("too many collidables (max 8)");
     for (const auto& c : w->colls) {
-        if (c->type != CT_WALLS && c->type != CT_SPHERE && c->type != CT_SDF && c->type != CT_MESH)
+        if (c->type != CT_WALLS && c->type != CT_SPHERE && c->type != CT_SDF && c->type != CT_MESH &&
+            c->type != CT_CAPSULE)
             throw std::runtime_error("collidable type " + std::to_string(c->type) +
                                      " is not supported by the GPU substep");
         if (c->type == CT_SDF && (mCollFlags & CF_EDGES) && !(mCollFlags & CF_VERTICES))

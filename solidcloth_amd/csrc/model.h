@@ -36,6 +36,8 @@ struct Sphere : Collidable {
 struct Capsule : Collidable {
     float r = 0.f, hh = 0.f;
 };
+// The ends of a capsule's axis, P = centre + rot (0, hh, 0), Q = centre - it.
+void capsule_axis(const Capsule& k, f3& P, f3& Q);
 
 // Node-sampled grid SDF (Geometry/SDF.h:13-251)
 struct SDF {

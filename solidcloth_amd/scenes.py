@@ -70,6 +70,8 @@ class Scene:
     walls_max: tuple = None
     # icosphere triangle-mesh obstacle (subdivision < 0 = none)
     mesh_subdiv: int = -1
+    # capsule collidable: (centre, radius, half height, rotation (s, x, y, z))
+    capsule: tuple = None
     mesh_center: tuple = (0.0, 0.0, 0.0)
     mesh_radius: float = 0.0
     # height perturbation: three sinusoids of this amplitude (0 = flat)
@@ -306,6 +308,8 @@ def build_patch(scene, sc=None):
     if scene.mesh_subdiv >= 0:
         v, t = icosphere(scene.mesh_subdiv, scene.mesh_radius, scene.mesh_center)
         world.AddMesh(sc.Mesh(v, t))
+    if scene.capsule is not None:
+        world.AddCapsule(*scene.capsule)
     if scene.walls_min is not None:
         world.AddWalls(scene.walls_min, scene.walls_max)
     patch = sc.ClothPatch(world)

@@ -116,5 +116,36 @@ def main():
     print("wrote", sorted(out.keys())[:3], "... and ref_collision3d.npz")
 
 
+def segm_triangle(L):
+    """Geometry/Collision3D.cpp:143-210 (ClosestPtSegmTriangle, the capsule's
+    triangle query) and Collision3D.h:125-141 (ClosestPtSegm) on random,
+    parallel, touching and end-point-nearest inputs."""
+    rng = np.random.default_rng(23)
+    K = 600
+    P = rng.normal(size=(K, 5, 3)).astype(np.float32)  # p, q, a, b, c
+    P[:40, 1] = P[:40, 0] + (P[:40, 3] - P[:40, 2])     # segment parallel to edge ab
+    P[40:80, 0] = 0.3 * P[40:80, 2] + 0.3 * P[40:80, 3] + 0.4 * P[40:80, 4]  # p on the triangle
+    P[80:120, 0] = P[80:120, 2] + np.float32(0.01) * rng.normal(size=(40, 3)).astype(np.float32)
+    P[120:140, 1] = P[120:140, 0]                       # degenerate segment
+    P[140:160, 4] = P[140:160, 3]                       # degenerate triangle
+    out = np.zeros((K, 11), dtype=np.float32)
+    seg = np.zeros((K, 4), dtype=np.float32)
+    for k in range(K):
+        x = [np.ascontiguousarray(P[k, i]) for i in range(5)]
+        o = np.zeros(11, dtype=np.float32)
+        L.ref_closest_pt_segm_triangle(p(x[0]), p(x[1]), p(x[2]), p(x[3]), p(x[4]), p(o))
+        out[k] = o
+        o2 = np.zeros(4, dtype=np.float32)
+        L.ref_closest_pt_segm(p(x[4]), p(x[0]), p(x[1]), p(o2))  # point c against segment [p, q]
+        seg[k] = o2
+    np.savez_compressed(os.path.join(HERE, "ref_segm_triangle.npz"), points=P, segm_tri_out=out, point_segm_out=seg)
+    print("wrote ref_segm_triangle.npz")
+
+
 if __name__ == "__main__":
-    main()
+    # no argument: every fixture; "segm_triangle": only ref_segm_triangle.npz
+    if sys.argv[1:] == ["segm_triangle"]:
+        segm_triangle(ref_lib())
+    else:
+        main()
+        segm_triangle(ref_lib())
