@@ -211,6 +211,14 @@ SCEXP int scGetConstraintOrder(scModel model, int kind, int* order, int count);
 SCEXP int scGetColouring(scModel model, int kind, int* colourOffsets, int maxColours);
 SCEXP int scGetSchedule(scModel model, int kind, int* phase, int* slab, int* colour, int count);
 
+/* Solver (ClothModel::SetSolver / GetSolver, Physics/ClothModel.h:137-138;
+ * values of SolverType, :23-30): 1 Gauss-Seidel (default), 0 Jacobi
+ * (SolveJacobi), 2-4 the conjugate-residual solver (SolveCR, which the
+ * reference runs for every other value). Jacobi and CR project the links only,
+ * as in the reference (no triangles, bends or contacts). */
+SCEXP void scSetSolver(scModel model, int solver);
+SCEXP int scGetSolver(scModel model);
+
 /* Sweep schedule: mode 0 = one launch per colour (positions in HBM),
  * mode 1 = slab phases with the slab's vertices staged in LDS (default),
  * slabs being strips of slabSize constraints; mode 2 = slab phases with

@@ -120,7 +120,7 @@ class Cloth:
 
     FLOAT_PARAMS = ("unit", "thickness", "tolerance", "shear", "bend", "stretch", "poisson", "friction",
                     "area", "drag", "selfShrink", "selfThreshold", "ccdVolume", "ccdBisect", "ccdVtDistance")
-    INT_PARAMS = ("iterations", "collFlags", "bendFreq", "numSteps", "dihedral", "fem", "fixedIterations", "accel")
+    INT_PARAMS = ("iterations", "collFlags", "bendFreq", "numSteps", "dihedral", "fem", "fixedIterations", "accel", "solver")
 
     def __init__(self, real64=False):
         self.L = lib()

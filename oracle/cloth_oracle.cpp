@@ -49,6 +49,9 @@ struct Params {
     // 1: broad phases through uniform grids (same results as the exhaustive
     // scans, for timing the oracle at large sizes)
     int accel = 0;
+    // Physics/ClothModel.h:23-30 SolverType: 0 Jacobi, 1 Gauss-Seidel
+    // (ClothModel.cpp:24 default), any other the conjugate-residual solver
+    int solver = 1;
     bool dihedral = true, fem = false, fixedIterations = false;
 };
 
@@ -1555,6 +1558,74 @@ public:
         stats.linkError = (float)cl;
     }
 
+    // Physics/ClothPBD.cpp:616-651 (SolveLinksJacobi): every link's
+    // displacement from the positions at the start of the pass, then all
+    // applied in link order
+    void solve_links_jacobi(R h, R omega)
+    {
+        const R h2 = h * h;
+        const R k = h2 * R(prm.area) * R(prm.unit);
+        std::vector<V3<R>> disp(links.size());
+        for (size_t i = 0; i < links.size(); i++) {
+            const LinkC<R>& l = links[i];
+            const R mu = R(1.f) / (invMass[l.i1] + invMass[l.i2]);
+            V3<R> delta = pos[l.i1] - pos[l.i2];
+            const R len0 = l.len;
+            const R len = length(delta);
+            scale(delta, R(1) / len);
+            const R err = len - len0;
+            const R epsilon = len0 / (k * l.stiffness);
+            const R s = R(1.f) / (R(1.f) + mu * epsilon);
+            const R lambda = err * mu * s;
+            disp[i] = (omega * lambda) * delta;
+        }
+        for (size_t i = 0; i < links.size(); i++) {
+            const LinkC<R>& l = links[i];
+            pos[l.i1] -= disp[i] * invMass[l.i1];
+            pos[l.i2] += disp[i] * invMass[l.i2];
+        }
+    }
+
+    // Physics/ClothPBD.cpp:653-662 (SolveJacobi)
+    void solve_jacobi(R h)
+    {
+        for (int k = 0; k < prm.iterations; ++k) solve_links_jacobi(h, R(0.5f));
+    }
+
+    // Physics/ClothPBD.cpp:371-478 (ComputeErrorAndNormal and SolveCR as
+    // compiled: SKIP_ALPHA and SKIP_BETA are defined, :381-382)
+    void solve_cr(R h, R alpha0)
+    {
+        const size_t nl = links.size();
+        if (nl == 0) return;
+        std::vector<R> r(nl), d(nl, R(0));
+        std::vector<V3<R>> normal(nl);
+        const int N = prm.iterations;
+        for (int iter = 0; iter < N; iter++) {
+            for (size_t i = 0; i < nl; i++) {
+                const LinkC<R>& l = links[i];
+                V3<R> delta = pos[l.i2] - pos[l.i1];
+                const R c = (length(delta) - l.len) * l.stiffness * R(prm.unit);
+                scale(delta, R(1.f) / length(delta));  // Vector3::Normalize
+                normal[i] = delta;
+                r[i] = -c;
+            }
+            // the beta schedule in FP32 with powf, as the reference's floats
+            float betaf = (float)iter / (float)(N - 1);
+            betaf = std::pow(betaf, 0.6f);
+            const R beta = R(betaf);
+            for (size_t i = 0; i < nl; i++) d[i] = r[i] + beta * d[i];
+            const R alpha = alpha0;
+            for (size_t i = 0; i < nl; i++) {
+                const LinkC<R>& l = links[i];
+                const R dLambda = alpha * d[i];
+                const V3<R> disp = dLambda * normal[i];
+                pos[l.i1] -= invMass[l.i1] * disp;
+                pos[l.i2] += invMass[l.i2] * disp;
+            }
+        }
+    }
+
     // Physics/ClothPBD.cpp:255-322
     void substep(R h)
     {
@@ -1612,7 +1683,9 @@ public:
         if (!selfEdgeOrder.empty()) order_self(selfEdges, selfEdgeOrder);
         selfTriOrder.clear();
         selfEdgeOrder.clear();
-        solve_gs(h);
+        if (prm.solver == 1) solve_gs(h);
+        else if (prm.solver == 0) solve_jacobi(h);
+        else solve_cr(h, R(0.1f));
         const R invH = R(1.f) / h;
         for (size_t i = 0; i < pos.size(); i++) {
             if (invMass[i] == R(0)) continue;
@@ -1718,6 +1791,7 @@ int oc_set_param_i(void* h, const char* name, int v)
         else if (n == "dihedral") p.dihedral = v != 0;
         else if (n == "fem") p.fem = v != 0;
         else if (n == "accel") p.accel = v;
+        else if (n == "solver") p.solver = v;
         else if (n == "fixedIterations") p.fixedIterations = v != 0;
         else return -1;
         return 0;

@@ -85,6 +85,7 @@ def _sig(L):
         "scGetFriction": (f, [H]), "scSetFriction": (None, [H, f]),
         "scGetNumIterations": (i, [H]), "scSetNumIterations": (None, [H, i]),
         "scGetUseFEM": (b, [H]), "scSetUseFEM": (None, [H, b]), "scGetClothTree": (vp, [H]),
+        "scSetSolver": (None, [H, i]), "scGetSolver": (i, [H]),
         "scAddMouseSpring": (None, [H, i, i, i, scVector3, scVector3]),
         "scUpdateMouseSpring": (None, [H, scVector3]), "scRemoveMouseSpring": (None, [H]),
         "scGetMeshVertexCount": (i, [H]), "scGetMeshIndexCount": (i, [H]),
@@ -489,6 +490,17 @@ class ClothModel:
         data = np.zeros((n, 8), dtype=np.float32)
         _checked("scGetSelfContacts", self.h, k, _ptr(keys), _ptr(ids), _ptr(data), n)
         return keys, ids, data
+
+    _SOLVERS = {"jacobi": 0, "gauss_seidel": 1, "jacobi_cr": 2, "implicit": 3, "explicit": 4}
+
+    def SetSolver(self, solver):
+        """Physics/ClothModel.h:138: "gauss_seidel" (default), "jacobi", or a
+        conjugate-residual type ("jacobi_cr", "implicit", "explicit"), or the
+        SolverType value."""
+        _checked("scSetSolver", self.h, self._SOLVERS.get(solver, solver) if isinstance(solver, str) else int(solver))
+
+    def GetSolver(self):
+        return _checked("scGetSolver", self.h)
 
     def AddMouseSpring(self, i1, i2, i3, coords, point):
         """Physics/ClothModel.h:150: pulls the barycentric point (coords) of

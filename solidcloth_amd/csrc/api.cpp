@@ -471,6 +471,14 @@ unsigned int* scGetMeshIndexBuffer(scMesh mesh)
 
 // ================================================================ extensions
 void scSetUnit(scModel model, float val) { guard([&] { M(model)->mUnit = val; }); }
+void scSetSolver(scModel model, int solver)
+{
+    guard([&] {
+        if (solver < SOLVER_JACOBI || solver > SOLVER_EXPLICIT) throw std::runtime_error("unknown solver type");
+        M(model)->mSolver = solver;
+    });
+}
+int scGetSolver(scModel model) { return guard_r<int>(-1, [&] { return M(model)->mSolver; }); }
 float scGetUnit(scModel model) { return guard_r<float>(0.f, [&] { return M(model)->mUnit; }); }
 void scSetShearStiffness(scModel model, float val) { guard([&] { M(model)->mShearStiff = val; }); }
 void scSetStretchStiffness(scModel model, float val) { guard([&] { M(model)->mStretchStiff = val; }); }

@@ -239,6 +239,25 @@ void launch_mouse_spring(hipStream_t s, float4* pos, float4* vel, const MouseSpr
                          int resetTravel);
 // ctl->maxTv = max over all particles of |pos - prev| (ClothCollision.cpp:95-104)
 void launch_max_travel(hipStream_t s, const float4* pos, const float4* prev, int n, SolverCtl* ctl);
+// Links in the reference's order for its Jacobi and conjugate-residual
+// solvers: ends lk, (rest length, stiffness) lr, per-link displacement and
+// search direction scratch, each particle's (link * 2 + end) entries in
+// ascending link order (CSR off / inc).
+struct JacobiLinks {
+    const int2* lk;
+    const float2* lr;
+    float4* disp;
+    float* d;
+    const int* off;
+    const int* inc;
+    int nl, nv;
+};
+// one SolveLinksJacobi pass (ClothPBD.cpp:616-651), k = h^2 area unit
+void launch_jacobi_pass(hipStream_t s, float4* pos, const JacobiLinks& J, float k, float omega);
+// SolveCR (ClothPBD.cpp:384-478 as compiled): d = 0 once per solve, then one
+// pass per iteration with beta = (iter / (N - 1))^0.6 and alpha
+void launch_cr_reset(hipStream_t s, const JacobiLinks& J);
+void launch_cr_pass(hipStream_t s, float4* pos, const JacobiLinks& J, float unit, float beta, float alpha);
 // Render mesh of the cloth (Physics/ClothPatch.cpp:266-273, Geometry/Mesh.cpp:10-63):
 // outV = pos + offset and outN = angle-weighted unit vertex normals (nv x 3
 // floats each), triN = unit face normals (nt x 3); idx = 3 nt render indices

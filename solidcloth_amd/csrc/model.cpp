@@ -301,6 +301,8 @@ void GpuSolver::release()
     dev_free(render.outV); dev_free(render.outN); dev_free(render.triN);
     render.nv = render.nt = -1;
     dev_free(d_mouse);
+    dev_free(jac.lk); dev_free(jac.lr); dev_free(jac.disp); dev_free(jac.d); dev_free(jac.off); dev_free(jac.inc);
+    jac.nl = jac.nv = -1;
     warm.built = warm.saved = false;
     dev_free(d_pos); dev_free(d_prev); dev_free(d_vel);
     dev_free(d_tidx); dev_free(d_tA); dev_free(d_tB); dev_free(d_tC); dev_free(d_bidx);
@@ -455,6 +457,7 @@ void ClothModel::build_device_topology()
     if (!g.stream) HIP_OK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     g.release_graphs();
     g.warm.built = false;
+    g.jac.nl = g.jac.nv = -1;  // rebuilt from the new links when a Jacobi-type solver runs
     g.nv = (int)pos.size();
     g.nt = (int)tris.size();
     g.nb = (int)bends.size();
@@ -819,7 +822,7 @@ std::string ClothModel::graph_key(float h, const CollSet& cs) const
     put(&mBendStiff, 4); put(&mPoisson, 4); put(&mFriction, 4); put(&mDragCoeff, 4);
     put(&mNumIterations, 4); put(&mCollFlags, 4); put(&mBendFreq, 4);
     put(&mArea, 4); put(&mDihedral, 1); put(&mFEM, 1); put(&mFixedIterations, 1); put(&mSweepMode, 4); put(&mSlabSize, 4);
-    put(&msActive, 1); put(&gpu.d_mouse, sizeof(void*));
+    put(&msActive, 1); put(&gpu.d_mouse, sizeof(void*)); put(&mSolver, 4); put(&gpu.jac.lk, sizeof(void*));
     put(&cs, sizeof(cs));
     put(&gpu.d_cP, sizeof(void*));
     put(&gpu.d_cFeat, sizeof(void*));
@@ -888,7 +891,8 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
     // error at the end of each iteration, so that mode keeps separate launches.
     // Edge and triangle contacts follow the vertex contacts inside an
     // iteration, so they also keep the launches separate.
-    const bool fuse = anyContacts && mFEM && mFixedIterations && triCol.slabbed && triCol.numPhases() > 0 &&
+    const bool fuse = mSolver == SOLVER_GAUSS_SEIDEL && anyContacts && mFEM && mFixedIterations && triCol.slabbed &&
+                      triCol.numPhases() > 0 &&
                       g.triOrphans == 0 && !edgeContacts && !triContacts && !selfColl;
     if (anyMesh || edgeContacts || triContacts || selfColl) {
         // the mesh query radius needs max |pos - prev| over all vertices
@@ -963,7 +967,25 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
     // with a fixed count, the reported errors are the last passes' (the exit
     // test's kernel clears them per iteration otherwise)
     const int lastBend = mBendFreq > 0 ? ((iterations - 1) / mBendFreq) * mBendFreq : 0;
-    for (int k = 0; k < iterations; k++) {
+    if (mSolver != SOLVER_GAUSS_SEIDEL) {
+        // ClothPBD.cpp:290-298: Jacobi or conjugate-residual passes over the
+        // links only (no triangles, bends or contacts), a fixed count
+        const JacobiLinks J{g.jac.lk, g.jac.lr, g.jac.disp, g.jac.d, g.jac.off, g.jac.inc, g.jac.nl, g.jac.nv};
+        timed_begin(g.timer, timed, 0, s, ev);
+        if (mSolver == SOLVER_JACOBI) {
+            for (int k = 0; k < mNumIterations; k++) launch_jacobi_pass(s, g.d_pos, J, linkK, 0.5f);
+        } else {
+            launch_cr_reset(s, J);
+            for (int k = 0; k < mNumIterations; k++) {
+                // ClothPBD.cpp:442-443 (SKIP_BETA): pow on the host, as the reference
+                float beta = (float)k / (float)(mNumIterations - 1);
+                beta = std::pow(beta, 0.6f);
+                launch_cr_pass(s, g.d_pos, J, mUnit, beta, 0.1f);
+            }
+        }
+        timed_end(g.timer, timed, 0, s, ev);
+    }
+    for (int k = 0; k < (mSolver == SOLVER_GAUSS_SEIDEL ? iterations : 0); k++) {
         if (mFixedIterations) {
             const int mask = (k == iterations - 1 && k > 0 ? 1 : 0) | (mDihedral && k == lastBend && k > 0 ? 2 : 0);
             if (mask) launch_reset_errors(s, g.d_ctl, mask);
@@ -1132,6 +1154,11 @@ void ClothModel::step_substeps(float h, int n)
     GpuSolver& g = gpu;
     const bool timed = g.timer.enabled;
     if (mWarm && mWarmMode != 0) ensure_warm();
+    if (mSolver != SOLVER_GAUSS_SEIDEL) {
+        if (mWarm && mWarmMode != 0)
+            throw std::runtime_error("the learned warm start works with the Gauss-Seidel solver only");
+        ensure_jacobi();
+    }
     if (msActive) {
         const int nv = (int)pos.size();
         for (int k = 0; k < 3; k++)
@@ -1430,6 +1457,43 @@ void ClothPatch::update_mesh()
     mModel->render_mesh(mMesh, mPosition);
 }
 
+// The links in the reference's order with each particle's incident links,
+// for SolveJacobi / SolveCR
+void ClothModel::ensure_jacobi()
+{
+    GpuSolver::Jacobi& J = gpu.jac;
+    const int nl = (int)links.size(), nv = (int)pos.size();
+    if (J.nl == nl && J.nv == nv) return;
+    dev_free(J.lk); dev_free(J.lr); dev_free(J.disp); dev_free(J.d); dev_free(J.off); dev_free(J.inc);
+    std::vector<int2> lk(std::max(1, nl));
+    std::vector<float2> lr(std::max(1, nl));
+    std::vector<int> off(nv + 1, 0), inc(std::max(1, 2 * nl));
+    for (int l = 0; l < nl; l++) {
+        lk[l] = make_int2((int)links[l].i1, (int)links[l].i2);
+        lr[l] = make_float2(links[l].len, links[l].stiffness);
+        off[links[l].i1 + 1]++;
+        off[links[l].i2 + 1]++;
+    }
+    for (int v = 0; v < nv; v++) off[v + 1] += off[v];
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int l = 0; l < nl; l++) {
+        inc[fill[links[l].i1]++] = 2 * l;
+        inc[fill[links[l].i2]++] = 2 * l + 1;
+    }
+    dev_alloc(J.lk, lk.size());
+    dev_alloc(J.lr, lr.size());
+    dev_alloc(J.disp, lk.size());
+    dev_alloc(J.d, lk.size());
+    dev_alloc(J.off, off.size());
+    dev_alloc(J.inc, inc.size());
+    HIP_OK(hipMemcpy(J.lk, lk.data(), lk.size() * sizeof(int2), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(J.lr, lr.data(), lr.size() * sizeof(float2), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(J.off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(J.inc, inc.data(), inc.size() * sizeof(int), hipMemcpyHostToDevice));
+    J.nl = nl;
+    J.nv = nv;
+}
+
 void ClothModel::render_mesh(Mesh& mesh, f3 offset)
 {
     const int nv = (int)mesh.vertices.size(), nt = (int)(mesh.indices.size() / 3);
@@ -1520,6 +1584,7 @@ void ClothModel::validate_batch() const
     if (mNumIterations < 1) throw std::runtime_error("NumIterations must be >= 1");
     if (mBendFreq < 1) throw std::runtime_error("bend frequency must be >= 1");
     if (msActive) throw std::runtime_error("a batched patch cannot carry a mouse spring (step it with scStepCloth)");
+    if (mSolver != SOLVER_GAUSS_SEIDEL) throw std::runtime_error("batched cloths use the Gauss-Seidel solver");
     if (mCollFlags & (CF_EDGES | CF_TRIANGLES | CF_SELF))
         throw std::runtime_error("batched cloths support vertex and wall contacts only");
     for (const auto& c : mOwner->mWorld->colls)

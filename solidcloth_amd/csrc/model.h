@@ -173,6 +173,9 @@ private:
 
 // Sweep schedules: 0 = one launch per colour, 1 = slab phases with LDS staging
 enum { SWEEP_COLOURS = 0, SWEEP_SLABS = 1, SWEEP_SEAMS = 2 };
+// Physics/ClothModel.h:23-30 (SolverType); Step runs SolveGS for
+// Gauss-Seidel, SolveJacobi for Jacobi and SolveCR for every other value
+enum { SOLVER_JACOBI = 0, SOLVER_GAUSS_SEIDEL = 1, SOLVER_JACOBI_CR = 2, SOLVER_IMPLICIT = 3, SOLVER_EXPLICIT = 4 };
 
 // Device copy of one constraint schedule's slab structure.
 struct DeviceSlabs {
@@ -273,6 +276,16 @@ public:
         int nv = -1, nt = -1;
     } render;
     MouseSpringDev* d_mouse = nullptr;
+    // links in the reference's order for the Jacobi / conjugate-residual
+    // solvers (built when one is selected)
+    struct Jacobi {
+        int2* lk = nullptr;
+        float2* lr = nullptr;
+        float4* disp = nullptr;
+        float* d = nullptr;
+        int *off = nullptr, *inc = nullptr;
+        int nl = -1, nv = -1;
+    } jac;
     // two cached substep graphs (the warm-started and the plain substep)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -317,6 +330,7 @@ public:
     f3 msPoint{0, 0, 0};
     float msLen = 0.f, msStiffness = 100.f;
     bool msDirty = true;  // the device copy of the spring is stale
+    int mSolver = SOLVER_GAUSS_SEIDEL;  // ClothModel.cpp:24
 
     int statIterations = 0;
     float statBendError = 0.f, statContactError = 0.f, statSelfVTError = 0.f, statSelfEEError = 0.f;
@@ -345,6 +359,7 @@ public:
     // the render mesh (vertices = particles + offset, normals, face normals)
     // from the device state
     void render_mesh(Mesh& mesh, f3 offset);
+    void ensure_jacobi();
 
     void step_substeps(float h, int n);
     void download_contacts();
