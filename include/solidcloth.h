@@ -211,6 +211,17 @@ SCEXP int scGetConstraintOrder(scModel model, int kind, int* order, int count);
 SCEXP int scGetColouring(scModel model, int kind, int* colourOffsets, int maxColours);
 SCEXP int scGetSchedule(scModel model, int kind, int* phase, int* slab, int* colour, int count);
 
+/* A patch from a triangle mesh (ClothPatch::Init(const Mesh&, offset,
+ * attached), Physics/ClothPatch.cpp:43-133, which the reference's demo calls
+ * from C++): vertices at equal positions are welded into one particle,
+ * triangles take the mesh's material coordinates (scSetMeshUVs) and, with
+ * attached, vertices whose colour (scSetMeshColors) has a non-zero red are
+ * pinned; links, edges and bends come from the mesh's edges. The render mesh
+ * (scGetClothMesh) keeps the unwelded vertices. */
+SCEXP void scInitClothFromMesh(scClothPatch patch, scMesh mesh, scVector3 offset, bool attached);
+SCEXP int scSetMeshUVs(scMesh mesh, const scVector2* uvs, int count);
+SCEXP int scSetMeshColors(scMesh mesh, const scVector3* colors, int count);
+
 /* Solver (ClothModel::SetSolver / GetSolver, Physics/ClothModel.h:137-138;
  * values of SolverType, :23-30): 1 Gauss-Seidel (default), 0 Jacobi
  * (SolveJacobi), 2-4 the conjugate-residual solver (SolveCR, which the

@@ -93,6 +93,8 @@ def lib():
         L.oc_closest_pt_segm_segm.argtypes = [vp, vp, vp, vp, vp]
         L.oc_closest_pt_segm_triangle.argtypes = [vp, vp, vp, vp, vp, vp]
         L.oc_closest_pt_segm.argtypes = [vp, vp, vp, vp]
+        L.oc_init_mesh.argtypes = [vp, _C.c_int, vp, _C.c_int, vp, vp, vp, _C.c_float, _C.c_float, _C.c_float,
+                                   _C.c_int, vp]
         L.oc_closest_pt_segm_segm.restype = _C.c_int
         L.oc_signed_volume.argtypes = [vp, vp, vp, vp]
         L.oc_signed_volume.restype = _C.c_float
@@ -149,6 +151,19 @@ class Cloth:
         self.L.oc_init_grid(self.h, divX, divY, inc, offset[0], offset[1], offset[2], int(horizontal),
                             int(attached), incline)
         return self
+
+    def init_mesh(self, vertices, indices, uvs=None, colors=None, offset=(0, 0, 0), attached=False):
+        """ClothPatch::Init from a mesh (Physics/ClothPatch.cpp:43-133);
+        returns the vertex -> particle map."""
+        v = _np.ascontiguousarray(vertices, dtype=_np.float32).reshape(-1, 3)
+        i = _np.ascontiguousarray(indices, dtype=_np.uint32).ravel()
+        u = None if uvs is None else _np.ascontiguousarray(uvs, dtype=_np.float32).reshape(-1, 2)
+        cx = None if colors is None else _np.ascontiguousarray(_np.asarray(colors, dtype=_np.float32)[:, 0])
+        out = _np.zeros(len(v), dtype=_np.int32)
+        self.L.oc_init_mesh(self.h, len(v), _ptr(v), i.size, _ptr(i), None if u is None else _ptr(u),
+                            None if cx is None else _ptr(cx), offset[0], offset[1], offset[2], int(attached),
+                            _ptr(out))
+        return out
 
     def init_arrays(self, pos, uv, inv_mass, tris):
         p = _np.ascontiguousarray(pos, dtype=_np.float64)

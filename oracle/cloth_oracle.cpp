@@ -333,10 +333,15 @@ public:
     // Physics/ClothModel.cpp:54-92
     void add_triangle(uint32_t i1, uint32_t i2, uint32_t i3)
     {
+        add_triangle_uv(i1, i2, i3, uvx[i1], uvy[i1], uvx[i2], uvy[i2], uvx[i3], uvy[i3]);
+    }
+
+    void add_triangle_uv(uint32_t i1, uint32_t i2, uint32_t i3, R u1x, R u1y, R u2x, R u2y, R u3x, R u3y)
+    {
         Tri<R> t;
         t.i1 = i1; t.i2 = i2; t.i3 = i3;
-        R d1x = uvx[i2] - uvx[i1], d1y = uvy[i2] - uvy[i1];
-        R d2x = uvx[i3] - uvx[i1], d2y = uvy[i3] - uvy[i1];
+        R d1x = u2x - u1x, d1y = u2y - u1y;
+        R d2x = u3x - u1x, d2y = u3y - u1y;
         t.du1 = d1x; t.du2 = d2x; t.dv1 = d1y; t.dv2 = d2y;
         t.det = t.du1 * t.dv2 - t.du2 * t.dv1;
         t.invDet = (t.det != R(0)) ? R(1.f) / t.det : R(0);
@@ -398,6 +403,74 @@ public:
         b.i3 = opposite(tris[e.t1], e.i1, e.i2);
         b.i4 = opposite(tris[e.t2], e.i1, e.i2);
         bends.push_back(b);
+    }
+
+    // Physics/ClothPatch.cpp:43-133 (ClothPatch::Init from a mesh): welding
+    // by the quadratic squared-distance-0 search, triangles from every third
+    // corner (the last one after the loop), links / edges / bends from the
+    // sorted corner edges, the last triangle's closing edge left out as the
+    // reference's corner loop stops one short. map receives vertex -> particle.
+    void init_mesh(const std::vector<V3<float>>& v, const std::vector<uint32_t>& idx, const float* uv,
+                   const float* colorsX, V3<float> off, bool attached, std::vector<int>& map)
+    {
+        clear();
+        const size_t nv = v.size(), n = idx.size();
+        map.assign(nv, -1);
+        for (size_t i = 0; i < nv; i++) {
+            if (map[i] >= 0) continue;
+            const int np = (int)pos.size();
+            for (size_t j = i + 1; j < nv; j++) {
+                const V3<float> d = v[i] - v[j];
+                if (dot(d, d) == 0.f) map[j] = np;
+            }
+            map[i] = np;
+            const V3<float> p = v[i] + off;
+            pos.push_back(V3<R>{R(p.x), R(p.y), R(p.z)});
+            prev.push_back(pos.back());
+            vel.push_back(V3<R>{0, 0, 0});
+            invMass.push_back((attached && colorsX && colorsX[i] != 0.f) ? R(0) : R(1));
+            uvx.push_back(uv ? R(uv[2 * i]) : R(0));
+            uvy.push_back(uv ? R(uv[2 * i + 1]) : R(0));
+        }
+        auto tri = [&](size_t a, size_t b, size_t c, uint32_t i1, uint32_t i2, uint32_t i3) {
+            if (uv)
+                add_triangle_uv(i1, i2, i3, R(uv[2 * a]), R(uv[2 * a + 1]), R(uv[2 * b]), R(uv[2 * b + 1]),
+                                R(uv[2 * c]), R(uv[2 * c + 1]));
+            else
+                add_triangle(i1, i2, i3);
+        };
+        std::vector<Edge> dup(n - 1);
+        for (size_t i = 0; i + 1 < n; i++) {
+            uint32_t i1 = (uint32_t)map[idx[i]], i2 = (uint32_t)map[idx[i + 1]];
+            if ((i + 1) % 3 == 0) {
+                i2 = (uint32_t)map[idx[i - 2]];
+                tri(idx[i - 2], idx[i - 1], idx[i], i2, (uint32_t)map[idx[i - 1]], i1);
+            }
+            if (i1 > i2) std::swap(i1, i2);
+            Edge e;
+            e.i1 = i1; e.i2 = i2; e.t1 = (int)(i / 3);
+            dup[i] = e;
+        }
+        tri(idx[n - 3], idx[n - 2], idx[n - 1], (uint32_t)map[idx[n - 3]], (uint32_t)map[idx[n - 2]],
+            (uint32_t)map[idx[n - 1]]);
+        std::sort(dup.begin(), dup.end(), [](const Edge& a, const Edge& b) {
+            return a.i1 < b.i1 || (a.i1 == b.i1 && a.i2 < b.i2);
+        });
+        const float stretch = prm.stretch;
+        add_link(dup[0].i1, dup[0].i2, stretch);
+        Edge cur = dup[0];
+        for (size_t i = 1; i < dup.size(); i++) {
+            if (dup[i].i1 == dup[i - 1].i1 && dup[i].i2 == dup[i - 1].i2) {
+                cur.t2 = dup[i].t1;
+                continue;
+            }
+            add_link(dup[i].i1, dup[i].i2, stretch);
+            edges.push_back(cur);
+            add_bend(cur);
+            cur = dup[i];
+        }
+        edges.push_back(cur);
+        add_bend(cur);
     }
 
     // Physics/ClothPatch.cpp:135-246 (grid branch)
@@ -2161,6 +2234,22 @@ int oc_add_mesh(void* h, int nv, const float* verts, int ni, const unsigned* idx
         col.mesh = (int)c.meshes.size() - 1;
         c.world.push_back(col);
         return (int)c.world.size() - 1;
+    });
+}
+
+// Physics/ClothPatch.cpp:43-133; uv (2 nv floats) and colorsX (nv red
+// components) may be null; mapOut (nv ints) receives vertex -> particle
+void oc_init_mesh(void* h, int nv, const float* verts, int ni, const unsigned* idx, const float* uv,
+                  const float* colorsX, float ox, float oy, float oz, int attached, int* mapOut)
+{
+    with(h, [&](auto& c) {
+        std::vector<V3<float>> v(nv);
+        for (int i = 0; i < nv; i++) v[i] = V3<float>{verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]};
+        std::vector<uint32_t> ix(idx, idx + ni);
+        std::vector<int> map;
+        c.init_mesh(v, ix, uv, colorsX, V3<float>{ox, oy, oz}, attached != 0, map);
+        if (mapOut) std::copy(map.begin(), map.end(), mapOut);
+        return 0;
     });
 }
 

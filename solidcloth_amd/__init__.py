@@ -85,7 +85,8 @@ def _sig(L):
         "scGetFriction": (f, [H]), "scSetFriction": (None, [H, f]),
         "scGetNumIterations": (i, [H]), "scSetNumIterations": (None, [H, i]),
         "scGetUseFEM": (b, [H]), "scSetUseFEM": (None, [H, b]), "scGetClothTree": (vp, [H]),
-        "scSetSolver": (None, [H, i]), "scGetSolver": (i, [H]),
+        "scInitClothFromMesh": (None, [H, H, scVector3, b]), "scSetMeshUVs": (i, [H, vp, i]),
+        "scSetMeshColors": (i, [H, vp, i]), "scSetSolver": (None, [H, i]), "scGetSolver": (i, [H]),
         "scAddMouseSpring": (None, [H, i, i, i, scVector3, scVector3]),
         "scUpdateMouseSpring": (None, [H, scVector3]), "scRemoveMouseSpring": (None, [H]),
         "scGetMeshVertexCount": (i, [H]), "scGetMeshIndexCount": (i, [H]),
@@ -193,6 +194,17 @@ class Mesh:
         i = np.ascontiguousarray(indices, dtype=np.uint32).ravel()
         self.h = _checked("scCreateMeshFromArrays", _ptr(v), len(v), _ptr(i), i.size)
         self.vertices, self.indices = v, i
+
+    def set_uvs(self, uvs):
+        """Material coordinates per vertex (Mesh::uvs), used by a mesh-based patch."""
+        u = np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1, 2)
+        _checked("scSetMeshUVs", self.h, _ptr(u), len(u))
+
+    def set_colors(self, colors):
+        """Colours per vertex (Mesh::colors); a mesh-based patch with attached
+        pins the vertices whose red is non-zero."""
+        c = np.ascontiguousarray(colors, dtype=np.float32).reshape(-1, 3)
+        _checked("scSetMeshColors", self.h, _ptr(c), len(c))
 
     def __del__(self):
         try:
@@ -544,6 +556,12 @@ class ClothPatch:
             lib().scFreeClothPatch(self.h)
         except Exception:
             pass
+
+    def InitFromMesh(self, mesh, offset=(0, 0, 0), attached=False):
+        """Physics/ClothPatch.cpp:43-133: a patch from a Mesh (welded vertices,
+        the mesh's uvs and colours)."""
+        self._mesh = mesh
+        _checked("scInitClothFromMesh", self.h, mesh.h, v3(offset), bool(attached))
 
     def Init(self, divX, divY, inc, offset=(0, 0, 0), horizontal=True, attached=False):
         _checked("scInitCloth", self.h, int(divX), int(divY), float(inc), v3(offset), bool(horizontal),

@@ -471,6 +471,34 @@ unsigned int* scGetMeshIndexBuffer(scMesh mesh)
 
 // ================================================================ extensions
 void scSetUnit(scModel model, float val) { guard([&] { M(model)->mUnit = val; }); }
+void scInitClothFromMesh(scClothPatch patch, scMesh mesh, scVector3 offset, bool attached)
+{
+    guard([&] {
+        if (!mesh.ptr) throw std::runtime_error("null mesh");
+        P(patch)->init_mesh(*ME(mesh), f3{offset.x, offset.y, offset.z}, attached);
+    });
+}
+
+int scSetMeshUVs(scMesh mesh, const scVector2* uvs, int count)
+{
+    return guard_r<int>(-1, [&] {
+        Mesh* me = ME(mesh);
+        if (count != (int)me->vertices.size()) throw std::runtime_error("one uv per mesh vertex expected");
+        me->uvs.assign(uvs, uvs + count);
+        return count;
+    });
+}
+
+int scSetMeshColors(scMesh mesh, const scVector3* colors, int count)
+{
+    return guard_r<int>(-1, [&] {
+        Mesh* me = ME(mesh);
+        if (count != (int)me->vertices.size()) throw std::runtime_error("one colour per mesh vertex expected");
+        me->colors.assign(colors, colors + count);
+        return count;
+    });
+}
+
 void scSetSolver(scModel model, int solver)
 {
     guard([&] {

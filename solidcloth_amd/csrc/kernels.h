@@ -263,9 +263,10 @@ void launch_cr_pass(hipStream_t s, float4* pos, const JacobiLinks& J, float unit
 // floats each), triN = unit face normals (nt x 3); idx = 3 nt render indices
 // into the particles; vt lists each vertex's (triangle * 4 + corner) in
 // ascending order, vtOff its CSR offsets; triA / triB are nt-sized scratch.
-void launch_render_mesh(hipStream_t s, const float4* pos, int nv, const unsigned* idx, int nt, f3 offset,
-                        const int* vtOff, const int* vt, float4* triA, float2* triB, float* outV, float* outN,
-                        float* triN);
+// map (null: identity) gives each render vertex's particle.
+void launch_render_mesh(hipStream_t s, const float4* pos, const int* map, int nv, const unsigned* idx, int nt,
+                        f3 offset, const int* vtOff, const int* vt, float4* triA, float2* triB, float* outV,
+                        float* outN, float* triN);
 
 // Batch: one workgroup per cloth (cloths in launch order), one launch per
 // substep. blockSize 256, 512 or 1024; cloths with at most stageVerts

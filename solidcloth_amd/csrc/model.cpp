@@ -69,7 +69,7 @@ void SDF::upload()
 
 void Mesh::clear()
 {
-    vertices.clear(); indices.clear(); normals.clear(); uvs.clear(); edges.clear();
+    vertices.clear(); indices.clear(); normals.clear(); uvs.clear(); colors.clear(); edges.clear();
     triNormals.clear(); edgeNormals.clear();
     dev.reset();
 }
@@ -297,7 +297,7 @@ void GpuSolver::release()
     dev_free(warm.snapPos); dev_free(warm.snapPrev); dev_free(warm.snapVel); dev_free(warm.snapCorr);
     dev_free(warm.resPartial); dev_free(warm.resOut);
     warm.nv = -1;
-    dev_free(render.idx); dev_free(render.vtOff); dev_free(render.vt); dev_free(render.triA); dev_free(render.triB);
+    dev_free(render.idx); dev_free(render.map); dev_free(render.vtOff); dev_free(render.vt); dev_free(render.triA); dev_free(render.triB);
     dev_free(render.outV); dev_free(render.outN); dev_free(render.triN);
     render.nv = render.nt = -1;
     dev_free(d_mouse);
@@ -347,10 +347,15 @@ void ClothModel::clear()
 // Physics/ClothModel.cpp:54-92 (rest state from material coordinates)
 void ClothModel::add_triangle(uint32_t i1, uint32_t i2, uint32_t i3)
 {
+    add_triangle_uv(i1, i2, i3, uv[i1], uv[i2], uv[i3]);
+}
+
+void ClothModel::add_triangle_uv(uint32_t i1, uint32_t i2, uint32_t i3, scVector2 uv1, scVector2 uv2, scVector2 uv3)
+{
     Triangle t{};
     t.i1 = i1; t.i2 = i2; t.i3 = i3;
-    float d1x = uv[i2].x - uv[i1].x, d1y = uv[i2].y - uv[i1].y;
-    float d2x = uv[i3].x - uv[i1].x, d2y = uv[i3].y - uv[i1].y;
+    float d1x = uv2.x - uv1.x, d1y = uv2.y - uv1.y;
+    float d2x = uv3.x - uv1.x, d2y = uv3.y - uv1.y;
     t.du1 = d1x; t.du2 = d2x; t.dv1 = d1y; t.dv2 = d2y;
     t.det = t.du1 * t.dv2 - t.du2 * t.dv1;
     t.invDet = (t.det != 0.f) ? 1.f / t.det : 0.f;
@@ -1359,8 +1364,98 @@ void ClothModel::download_contacts()
 // -------------------------------------------------------------- patch ----
 
 // Physics/ClothPatch.cpp:135-246 (grid branch)
+// Physics/ClothPatch.cpp:43-133. Vertices at the same position are welded
+// into one particle (the reference's quadratic search, kept for its exact
+// test: squared distance 0); each triangle corner takes the mesh's material
+// coordinates; links, edges and bends come from the sorted corner edges, in
+// which the closing edge of the last triangle is missing as in the reference
+// (its loop over corners stops one short).
+void ClothPatch::init_mesh(const Mesh& mesh, f3 offset, bool attached)
+{
+    if (mesh.vertices.empty() || mesh.indices.empty()) return;
+    const size_t n = mesh.indices.size();
+    if (n % 3 != 0) throw std::runtime_error("mesh index count is not a multiple of 3");
+    for (unsigned v : mesh.indices)
+        if (v >= mesh.vertices.size()) throw std::runtime_error("mesh index out of range");
+    if (!mesh.uvs.empty() && mesh.uvs.size() != mesh.vertices.size())
+        throw std::runtime_error("mesh uv count differs from its vertex count");
+    if (!mesh.colors.empty() && mesh.colors.size() != mesh.vertices.size())
+        throw std::runtime_error("mesh colour count differs from its vertex count");
+    ClothModel& m = *mModel;
+    mMesh.clear();
+    mMesh.vertices = mesh.vertices;
+    mMesh.indices = mesh.indices;
+    mMesh.uvs = mesh.uvs;
+    mMesh.colors = mesh.colors;
+    m.clear();
+    const size_t nv = mesh.vertices.size();
+    mMap.assign(nv, -1);
+    auto V = [&](size_t i) { return f3{mesh.vertices[i].x, mesh.vertices[i].y, mesh.vertices[i].z}; };
+    for (size_t i = 0; i < nv; i++) {
+        if (mMap[i] >= 0) continue;
+        const int np = (int)m.pos.size();
+        for (size_t j = i + 1; j < nv; j++) {
+            const f3 d = sub(V(i), V(j));
+            if (dot(d, d) == 0.f) mMap[j] = np;
+        }
+        mMap[i] = np;
+        const f3 p = add(V(i), offset);
+        m.pos.push_back(p);
+        m.prev.push_back(p);
+        m.vel.push_back(f3{0.f, 0.f, 0.f});
+        m.invMass.push_back((attached && !mesh.colors.empty() && mesh.colors[i].x != 0.f) ? 0.f : 1.f);
+        m.uv.push_back(mesh.uvs.empty() ? scVector2{0.f, 0.f} : mesh.uvs[i]);
+    }
+    const bool uvs = !mesh.uvs.empty();
+    auto tri = [&](size_t a, size_t b, size_t c, uint32_t i1, uint32_t i2, uint32_t i3) {
+        if (uvs) m.add_triangle_uv(i1, i2, i3, mesh.uvs[a], mesh.uvs[b], mesh.uvs[c]);
+        else m.add_triangle(i1, i2, i3);
+    };
+    const std::vector<unsigned>& idx = mesh.indices;
+    std::vector<MeshEdge> dup(n - 1);
+    for (size_t i = 0; i + 1 < n; i++) {
+        uint32_t i1 = (uint32_t)mMap[idx[i]], i2 = (uint32_t)mMap[idx[i + 1]];
+        if ((i + 1) % 3 == 0) {
+            i2 = (uint32_t)mMap[idx[i - 2]];
+            tri(idx[i - 2], idx[i - 1], idx[i], i2, (uint32_t)mMap[idx[i - 1]], i1);
+        }
+        if (i1 > i2) std::swap(i1, i2);
+        MeshEdge e;
+        e.i1 = i1;
+        e.i2 = i2;
+        e.t1 = (int)(i / 3);
+        dup[i] = e;
+    }
+    tri(idx[n - 3], idx[n - 2], idx[n - 1], (uint32_t)mMap[idx[n - 3]], (uint32_t)mMap[idx[n - 2]],
+        (uint32_t)mMap[idx[n - 1]]);
+    std::sort(dup.begin(), dup.end(), [](const MeshEdge& a, const MeshEdge& b) {
+        return a.i1 < b.i1 || (a.i1 == b.i1 && a.i2 < b.i2);
+    });
+    const float stretch = m.mStretchStiff;
+    if (!dup.empty()) {
+        m.add_link(dup[0].i1, dup[0].i2, stretch);
+        MeshEdge cur = dup[0];
+        for (size_t i = 1; i < dup.size(); i++) {
+            if (dup[i].i1 == dup[i - 1].i1 && dup[i].i2 == dup[i - 1].i2) {
+                cur.t2 = dup[i].t1;
+                continue;
+            }
+            m.add_link(dup[i].i1, dup[i].i2, stretch);
+            mMesh.edges.push_back(cur);
+            m.edges.push_back(cur);
+            m.add_bend(cur);
+            cur = dup[i];
+        }
+        mMesh.edges.push_back(cur);
+        m.edges.push_back(cur);
+        m.add_bend(cur);
+    }
+    m.mark_topology_dirty();
+}
+
 void ClothPatch::init_grid(int divX, int divY, float inc, f3 offset, bool horizontal, bool attached)
 {
+    mMap.clear();
     ClothModel& m = *mModel;
     mMesh.clear();
     m.clear();
@@ -1454,7 +1549,7 @@ void ClothPatch::step(float dt)
 void ClothPatch::update_mesh()
 {
     if (mMesh.vertices.empty()) return;
-    mModel->render_mesh(mMesh, mPosition);
+    mModel->render_mesh(mMesh, mPosition, mMap);
 }
 
 // The links in the reference's order with each particle's incident links,
@@ -1494,16 +1589,23 @@ void ClothModel::ensure_jacobi()
     J.nv = nv;
 }
 
-void ClothModel::render_mesh(Mesh& mesh, f3 offset)
+void ClothModel::render_mesh(Mesh& mesh, f3 offset, const std::vector<int>& map)
 {
     const int nv = (int)mesh.vertices.size(), nt = (int)(mesh.indices.size() / 3);
-    if (nv != (int)pos.size()) throw std::runtime_error("the render mesh does not match the particles");
+    if (map.empty() ? nv != (int)pos.size() : (int)map.size() != nv)
+        throw std::runtime_error("the render mesh does not match the particles");
     sync_to_device();
     GpuSolver& g = gpu;
     GpuSolver::Render& r = g.render;
     if (r.nv != nv || r.nt != nt) {
-        dev_free(r.idx); dev_free(r.vtOff); dev_free(r.vt); dev_free(r.triA); dev_free(r.triB);
+        dev_free(r.idx); dev_free(r.map); dev_free(r.vtOff); dev_free(r.vt); dev_free(r.triA); dev_free(r.triB);
         dev_free(r.outV); dev_free(r.outN); dev_free(r.triN);
+        if (!map.empty()) {
+            for (int v : map)
+                if (v < 0 || v >= (int)pos.size()) throw std::runtime_error("render mesh map out of range");
+            dev_alloc(r.map, nv);
+            HIP_OK(hipMemcpy(r.map, map.data(), nv * sizeof(int), hipMemcpyHostToDevice));
+        }
         // each vertex's (triangle, corner) entries in ascending triangle order
         std::vector<int> off(nv + 1, 0), list(3 * (size_t)nt);
         for (int k = 0; k < 3 * nt; k++) {
@@ -1530,7 +1632,8 @@ void ClothModel::render_mesh(Mesh& mesh, f3 offset)
         r.nv = nv;
         r.nt = nt;
     }
-    launch_render_mesh(g.stream, g.d_pos, nv, r.idx, nt, offset, r.vtOff, r.vt, r.triA, r.triB, r.outV, r.outN, r.triN);
+    launch_render_mesh(g.stream, g.d_pos, r.map, nv, r.idx, nt, offset, r.vtOff, r.vt, r.triA, r.triB, r.outV, r.outN,
+                       r.triN);
     HIP_OK(hipGetLastError());
     mesh.normals.resize(nv);
     mesh.triNormals.resize(nt);

@@ -82,6 +82,7 @@ struct Mesh {
     std::vector<scVector3> triNormals;   // unit face normals (Mesh.cpp:24-27)
     std::vector<scVector3> edgeNormals;  // edge pseudo-normals (Mesh.cpp:105, 125-126)
     std::vector<scVector2> uvs;
+    std::vector<scVector3> colors;  // per vertex; a mesh-based patch pins vertices whose red is non-zero
     std::vector<MeshEdge> edges;
     std::unique_ptr<MeshDevice> dev;
     void clear();
@@ -269,6 +270,7 @@ public:
     // triangle normals and angles, packed outputs (built for nv, nt)
     struct Render {
         unsigned* idx = nullptr;
+        int* map = nullptr;
         int *vtOff = nullptr, *vt = nullptr;
         float4* triA = nullptr;
         float2* triB = nullptr;
@@ -348,6 +350,8 @@ public:
 
     void clear();
     void add_triangle(uint32_t i1, uint32_t i2, uint32_t i3);
+    // ClothModel.cpp:54-92 with explicit material coordinates
+    void add_triangle_uv(uint32_t i1, uint32_t i2, uint32_t i3, scVector2 uv1, scVector2 uv2, scVector2 uv3);
     void add_link(uint32_t i1, uint32_t i2, float stiffness);
     void add_bend(const MeshEdge& e);
 
@@ -358,7 +362,7 @@ public:
     void sync_to_device();
     // the render mesh (vertices = particles + offset, normals, face normals)
     // from the device state
-    void render_mesh(Mesh& mesh, f3 offset);
+    void render_mesh(Mesh& mesh, f3 offset, const std::vector<int>& map);
     void ensure_jacobi();
 
     void step_substeps(float h, int n);
@@ -422,6 +426,8 @@ class ClothPatch {
 public:
     explicit ClothPatch(CollisionWorld* world) : mWorld(world), mModel(new ClothModel(this)) {}
     void init_grid(int divX, int divY, float inc, f3 offset, bool horizontal, bool attached);
+    // Physics/ClothPatch.cpp:43-133: a patch from a triangle mesh
+    void init_mesh(const Mesh& mesh, f3 offset, bool attached);
     void step(float dt);
     void update_mesh();
     void compute_strain_map(scVector3* colors, int count);
@@ -432,6 +438,9 @@ public:
     int mNumSteps = 1;
     float mIncline = 0.f;
     f3 mPosition{0.f, 0.f, 0.f};
+    // mesh-based patches: render-mesh vertex -> particle (duplicate vertices
+    // welded); empty for grid patches, whose render vertices are the particles
+    std::vector<int> mMap;
 };
 
 // Many independent cloths stepped together: one launch per substep, one

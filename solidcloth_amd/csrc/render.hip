@@ -23,14 +23,19 @@ inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
 
 // Geometry/Mesh.cpp:18-47 for triangle t: unit normal (xyz) and the corner
 // angle at its first vertex (w); the other two corner angles in B.
-__global__ void __launch_bounds__(kBlock) k_render_tris(const float4* __restrict__ pos,
+__global__ void __launch_bounds__(kBlock) k_render_tris(const float4* __restrict__ pos, const int* __restrict__ map,
                                                         const unsigned* __restrict__ idx, int nt, f3 offset,
                                                         float4* __restrict__ triA, float2* __restrict__ triB,
                                                         float* __restrict__ triN)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
-    const unsigned i1 = idx[3 * t], i2 = idx[3 * t + 1], i3 = idx[3 * t + 2];
+    unsigned i1 = idx[3 * t], i2 = idx[3 * t + 1], i3 = idx[3 * t + 2];
+    if (map) {
+        i1 = map[i1];
+        i2 = map[i2];
+        i3 = map[i3];
+    }
     const f3 v1 = add(xyz(pos[i1]), offset), v2 = add(xyz(pos[i2]), offset), v3 = add(xyz(pos[i3]), offset);
     const f3 n = normalized(cross(sub(v2, v1), sub(v3, v1)));
     const f3 e1 = normalized(sub(v2, v1)), e2 = normalized(sub(v3, v2)), e3 = normalized(sub(v1, v3));
@@ -48,7 +53,8 @@ __global__ void __launch_bounds__(kBlock) k_render_tris(const float4* __restrict
 // Geometry/Mesh.cpp:44-62 for vertex v: the sum over its (triangle, corner)
 // entries vt[vtOff[v] .. vtOff[v+1]) (ascending triangle, then corner), then
 // normalised unless zero; and the render position pos + offset.
-__global__ void __launch_bounds__(kBlock) k_render_verts(const float4* __restrict__ pos, int nv, f3 offset,
+__global__ void __launch_bounds__(kBlock) k_render_verts(const float4* __restrict__ pos, const int* __restrict__ map,
+                                                         int nv, f3 offset,
                                                          const int* __restrict__ vtOff, const int* __restrict__ vt,
                                                          const float4* __restrict__ triA,
                                                          const float2* __restrict__ triB, float* __restrict__ outV,
@@ -66,7 +72,7 @@ __global__ void __launch_bounds__(kBlock) k_render_verts(const float4* __restric
         acc = add(acc, mul(w, xyz(A)));
     }
     const f3 n = len(acc) == 0.f ? f3{0.f, 0.f, 0.f} : normalized(acc);
-    const f3 p = add(xyz(pos[v]), offset);
+    const f3 p = add(xyz(pos[map ? map[v] : v]), offset);
     outV[3 * v] = p.x;
     outV[3 * v + 1] = p.y;
     outV[3 * v + 2] = p.z;
@@ -77,15 +83,16 @@ __global__ void __launch_bounds__(kBlock) k_render_verts(const float4* __restric
 
 }  // namespace
 
-void launch_render_mesh(hipStream_t s, const float4* pos, int nv, const unsigned* idx, int nt, f3 offset,
-                        const int* vtOff, const int* vt, float4* triA, float2* triB, float* outV, float* outN,
-                        float* triN)
+void launch_render_mesh(hipStream_t s, const float4* pos, const int* map, int nv, const unsigned* idx, int nt,
+                        f3 offset, const int* vtOff, const int* vt, float4* triA, float2* triB, float* outV,
+                        float* outN, float* triN)
 {
     if (nt > 0)
-        hipLaunchKernelGGL(k_render_tris, dim3(blocks(nt)), dim3(kBlock), 0, s, pos, idx, nt, offset, triA, triB, triN);
+        hipLaunchKernelGGL(k_render_tris, dim3(blocks(nt)), dim3(kBlock), 0, s, pos, map, idx, nt, offset, triA, triB,
+                           triN);
     if (nv > 0)
-        hipLaunchKernelGGL(k_render_verts, dim3(blocks(nv)), dim3(kBlock), 0, s, pos, nv, offset, vtOff, vt, triA, triB,
-                           outV, outN);
+        hipLaunchKernelGGL(k_render_verts, dim3(blocks(nv)), dim3(kBlock), 0, s, pos, map, nv, offset, vtOff, vt, triA,
+                           triB, outV, outN);
 }
 
 }  // namespace sc
