@@ -179,10 +179,33 @@ SCEXP void scSetInitIncline(scClothPatch patch, float incline);
 SCEXP scSDF scCreateGridSDF(const float* values, int nx, int ny, int nz, scAabb3 box);
 SCEXP void scFreeSDF(scSDF sdf);
 
+/* SDF file (Geometry/SDF.cpp:75-151, WriteToFile / LoadFromFile): 3 int32 cell
+ * counts, the box as 6 doubles (min.x, max.x, min.y, max.y, min.z, max.z),
+ * then (nx+1)(ny+1)(nz+1) double node values, x fastest, native byte order.
+ * Loading narrows box and values to float and computes the gradients; a file
+ * that cannot be opened or is short records an error and returns a null
+ * handle (the reference leaves the SDF unchanged). Writing returns 1, or 0
+ * with an error recorded. scGetSDFGrid copies the cell counts (3 ints), the
+ * box and the node values (any pointer may be null) and returns the node
+ * count. */
+SCEXP scSDF scLoadSDFFile(const char* path);
+SCEXP int scWriteSDFFile(scSDF sdf, const char* path);
+SCEXP int scGetSDFGrid(scSDF sdf, int* cells, scAabb3* box, float* values);
+
 /* Indexed triangle mesh from arrays (replaces Geometry::Mesh* in scCreateMesh). */
 SCEXP scMesh scCreateMeshFromArrays(const scVector3* vertices, int numVertices, const unsigned int* indices,
                                     int numIndices);
 SCEXP void scFreeMesh(scMesh mesh);
+
+/* Wavefront OBJ (Geometry/Assets.cpp:309-451, LoadMeshFromObj): v, vt and f
+ * lines, the first three corners of each face, 1-based indices. naive: one
+ * vertex per v line. Otherwise one vertex per distinct (position, normal, uv)
+ * corner in order of first appearance, with its uv (scGetMeshUVBuffer). Normals
+ * are computed from the triangles, not read. An unreadable file or a bad
+ * index records an error and returns a null handle. */
+SCEXP scMesh scLoadMeshFromObj(const char* path, bool naive);
+/* Per-vertex material coordinates of a mesh, or null when it has none. */
+SCEXP scVector2* scGetMeshUVBuffer(scMesh mesh);
 
 /* Bulk state access (host arrays of count elements). */
 SCEXP int scGetParticlePositions(scModel model, scVector3* out, int count);

@@ -98,6 +98,9 @@ def _sig(L):
         "scSetNumSteps": (None, [H, i]), "scGetNumSteps": (i, [H]),
         "scSetFixedIterations": (None, [H, b]), "scSetInitIncline": (None, [H, f]),
         "scCreateGridSDF": (H, [vp, i, i, i, scAabb3]), "scFreeSDF": (None, [H]),
+        "scLoadSDFFile": (H, [C.c_char_p]), "scWriteSDFFile": (i, [H, C.c_char_p]),
+        "scGetSDFGrid": (i, [H, vp, vp, vp]),
+        "scLoadMeshFromObj": (H, [C.c_char_p, b]), "scGetMeshUVBuffer": (C.POINTER(scVector2), [H]),
         "scCreateMeshFromArrays": (H, [vp, i, vp, i]), "scFreeMesh": (None, [H]),
         "scGetParticlePositions": (i, [H, vp, i]), "scSetParticlePositions": (i, [H, vp, i]),
         "scGetParticleVelocities": (i, [H, vp, i]), "scSetParticleVelocities": (i, [H, vp, i]),
@@ -184,6 +187,29 @@ class SDF:
         self.h = _checked("scCreateGridSDF", _ptr(v), nx, ny, nz, box)
         self.n, self.bmin, self.bmax = tuple(n), tuple(bmin), tuple(bmax)
 
+    @classmethod
+    def from_file(cls, path):
+        """SDF read from the reference's binary file (SDF::LoadFromFile)."""
+        self = cls.__new__(cls)
+        self.h = _checked("scLoadSDFFile", os.fsencode(path))
+        cells, box = (C.c_int * 3)(), scAabb3()
+        _checked("scGetSDFGrid", self.h, C.cast(cells, C.c_void_p), C.cast(C.pointer(box), C.c_void_p), None)
+        self.n = tuple(cells)
+        self.bmin = (box.min.x, box.min.y, box.min.z)
+        self.bmax = (box.max.x, box.max.y, box.max.z)
+        return self
+
+    def write(self, path):
+        """Writes the SDF in the reference's binary format (SDF::WriteToFile)."""
+        _checked("scWriteSDFFile", self.h, os.fsencode(path))
+
+    def values(self):
+        """Node values, shape (nz+1, ny+1, nx+1)."""
+        nx, ny, nz = self.n
+        out = np.empty((nz + 1, ny + 1, nx + 1), dtype=np.float32)
+        _checked("scGetSDFGrid", self.h, None, None, _ptr(out))
+        return out
+
 
 class Mesh:
     """Indexed triangle mesh (Geometry/Mesh.h) built from vertex and index
@@ -194,6 +220,22 @@ class Mesh:
         i = np.ascontiguousarray(indices, dtype=np.uint32).ravel()
         self.h = _checked("scCreateMeshFromArrays", _ptr(v), len(v), _ptr(i), i.size)
         self.vertices, self.indices = v, i
+
+    @classmethod
+    def from_obj(cls, path, naive=False):
+        """Mesh read from a Wavefront OBJ file (Assets LoadMeshFromObj); its
+        uvs, when the file has them and naive is False, are in self.uvs."""
+        self = cls.__new__(cls)
+        self.h = _checked("scLoadMeshFromObj", os.fsencode(path), bool(naive))
+        L = lib()
+        nv, ni = L.scGetMeshVertexCount(self.h), L.scGetMeshIndexCount(self.h)
+        vb, ib, ub = L.scGetMeshVertexBuffer(self.h), L.scGetMeshIndexBuffer(self.h), L.scGetMeshUVBuffer(self.h)
+        self.vertices = (np.ctypeslib.as_array(C.cast(vb, C.POINTER(C.c_float)), shape=(nv, 3)).copy()
+                         if nv else np.zeros((0, 3), np.float32))
+        self.indices = np.ctypeslib.as_array(ib, shape=(ni,)).copy() if ni else np.zeros(0, np.uint32)
+        self.uvs = (np.ctypeslib.as_array(C.cast(ub, C.POINTER(C.c_float)), shape=(nv, 2)).copy()
+                    if ub else None)
+        return self
 
     def set_uvs(self, uvs):
         """Material coordinates per vertex (Mesh::uvs), used by a mesh-based patch."""

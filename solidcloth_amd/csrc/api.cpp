@@ -4,7 +4,13 @@
 // thread's last error and the call returns a neutral value.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
 #include <stdexcept>
 
 #include "lbvh.h"
@@ -566,6 +572,195 @@ scMesh scCreateMeshFromArrays(const scVector3* vertices, int numVertices, const 
 }
 
 void scFreeMesh(scMesh mesh) { guard([&] { delete (Mesh*)mesh.ptr; }); }
+
+// ------------------------------------------------------------ file formats
+
+namespace {
+
+using FileCloser = std::unique_ptr<FILE, int (*)(FILE*)>;
+
+FileCloser open_file(const char* path, const char* mode, const char* what)
+{
+    if (!path) throw std::runtime_error(std::string("no ") + what + " path");
+    FileCloser f(fopen(path, mode), fclose);
+    if (!f) throw std::runtime_error(std::string("cannot open ") + what + " file " + path);
+    return f;
+}
+
+// Whitespace-separated words of one line.
+std::vector<std::string> words(const std::string& line)
+{
+    std::vector<std::string> w;
+    size_t i = 0;
+    while (i < line.size()) {
+        while (i < line.size() && isspace((unsigned char)line[i])) i++;
+        size_t j = i;
+        while (j < line.size() && !isspace((unsigned char)line[j])) j++;
+        if (j > i) w.push_back(line.substr(i, j - i));
+        i = j;
+    }
+    return w;
+}
+
+unsigned obj_index(const std::string& tok, int lineNo)
+{
+    char* end = nullptr;
+    long v = strtol(tok.c_str(), &end, 10);
+    if (end == tok.c_str() || v < 1)
+        throw std::runtime_error("OBJ line " + std::to_string(lineNo) + ": unsupported index '" + tok + "'");
+    return (unsigned)(v - 1);
+}
+
+}  // namespace
+
+// Geometry/SDF.cpp:115-151: 3 int cell counts, the box as 6 doubles
+// (min.x, max.x, min.y, max.y, min.z, max.z), then the node values as doubles,
+// x fastest; values and box are narrowed to float as the reference stores them.
+scSDF scLoadSDFFile(const char* path)
+{
+    scSDF s;
+    s.ptr = guard_r<void*>(nullptr, [&] {
+        FileCloser f = open_file(path, "rb", "SDF");
+        int32_t n[3];
+        double b[6];
+        if (fread(n, sizeof(int32_t), 3, f.get()) != 3 || fread(b, sizeof(double), 6, f.get()) != 6)
+            throw std::runtime_error(std::string("truncated SDF header in ") + path);
+        for (int a = 0; a < 3; a++)
+            if (n[a] < 1 || n[a] > 4095) throw std::runtime_error("SDF cell count out of range (1..4095)");
+        size_t count = (size_t)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
+        std::vector<double> v(count);
+        if (fread(v.data(), sizeof(double), count, f.get()) != count)
+            throw std::runtime_error(std::string("truncated SDF values in ") + path);
+        scAabb3 box{{(float)b[0], (float)b[2], (float)b[4]}, {(float)b[1], (float)b[3], (float)b[5]}};
+        std::unique_ptr<SDF> g(new SDF());
+        g->create(n[0], n[1], n[2], box);
+        for (size_t i = 0; i < count; i++) g->vals[i] = (float)v[i];
+        g->compute_gradient();
+        return (void*)g.release();
+    });
+    return s;
+}
+
+// Geometry/SDF.cpp:75-113 (WriteToFile): the same layout as scLoadSDFFile reads.
+int scWriteSDFFile(scSDF sdf, const char* path)
+{
+    return guard_r<int>(0, [&] {
+        const SDF* g = (const SDF*)sdf.ptr;
+        if (!g) throw std::runtime_error("null SDF");
+        FileCloser f = open_file(path, "wb", "SDF");
+        int32_t n[3] = {g->nc[0], g->nc[1], g->nc[2]};
+        double b[6] = {g->box.min.x, g->box.max.x, g->box.min.y, g->box.max.y, g->box.min.z, g->box.max.z};
+        std::vector<double> v(g->vals.begin(), g->vals.end());
+        if (fwrite(n, sizeof(int32_t), 3, f.get()) != 3 || fwrite(b, sizeof(double), 6, f.get()) != 6 ||
+            fwrite(v.data(), sizeof(double), v.size(), f.get()) != v.size())
+            throw std::runtime_error(std::string("cannot write SDF file ") + path);
+        return 1;
+    });
+}
+
+int scGetSDFGrid(scSDF sdf, int* cells, scAabb3* box, float* values)
+{
+    return guard_r<int>(0, [&] {
+        const SDF* g = (const SDF*)sdf.ptr;
+        if (!g) throw std::runtime_error("null SDF");
+        if (cells) for (int a = 0; a < 3; a++) cells[a] = g->nc[a];
+        if (box) *box = g->box;
+        if (values) memcpy(values, g->vals.data(), g->vals.size() * sizeof(float));
+        return (int)g->vals.size();
+    });
+}
+
+// Geometry/Assets.cpp:309-451 (LoadMeshFromObj). Reads v, vt, f lines (vn are
+// skipped: the mesh's normals are computed from its triangles, as for every
+// mesh built here); each face contributes its first three corners. naive:
+// one mesh vertex per v line and the faces' position indices. Otherwise one
+// mesh vertex per distinct (position, normal, uv) corner in order of first
+// appearance, carrying its position and uv.
+scMesh scLoadMeshFromObj(const char* path, bool naive)
+{
+    scMesh m;
+    m.ptr = guard_r<void*>(nullptr, [&] {
+        if (!path) throw std::runtime_error("no OBJ path");
+        std::ifstream in(path);
+        if (!in) throw std::runtime_error(std::string("cannot open OBJ file ") + path);
+        std::vector<scVector3> pos;
+        std::vector<scVector2> tex;
+        std::vector<unsigned> idxV, idxU;
+        std::vector<std::tuple<unsigned, unsigned, unsigned>> corners;  // (position, normal, uv)
+        std::map<std::tuple<unsigned, unsigned, unsigned>, unsigned> first;
+        std::string line;
+        int lineNo = 0;
+        while (std::getline(in, line)) {
+            lineNo++;
+            if (line.size() < 2) continue;
+            std::vector<std::string> w = words(line);
+            if (w.empty()) continue;
+            if (w[0] == "v") {
+                if (w.size() < 4) throw std::runtime_error("OBJ line " + std::to_string(lineNo) + ": short vertex");
+                pos.push_back({strtof(w[1].c_str(), nullptr), strtof(w[2].c_str(), nullptr),
+                               strtof(w[3].c_str(), nullptr)});
+            } else if (w[0] == "vt") {
+                // u from the first number, v from the last (atof, then narrowed)
+                if (w.size() < 3) throw std::runtime_error("OBJ line " + std::to_string(lineNo) + ": short uv");
+                tex.push_back({(float)strtod(w[1].c_str(), nullptr), (float)strtod(w.back().c_str(), nullptr)});
+            } else if (w[0] == "f") {
+                if (w.size() < 4) throw std::runtime_error("OBJ line " + std::to_string(lineNo) + ": face with < 3 corners");
+                for (int c = 1; c <= 3; c++) {
+                    const std::string& t = w[c];
+                    size_t s1 = t.find('/');
+                    unsigned v = obj_index(t.substr(0, s1), lineNo), uv = 0, nn = 0;
+                    if (s1 != std::string::npos) {
+                        size_t s2 = t.find('/', s1 + 1);
+                        std::string ts = t.substr(s1 + 1, s2 == std::string::npos ? std::string::npos : s2 - s1 - 1);
+                        if (!ts.empty()) uv = obj_index(ts, lineNo);
+                        if (s2 != std::string::npos && s2 + 1 < t.size()) nn = obj_index(t.substr(s2 + 1), lineNo);
+                    }
+                    idxV.push_back(v);
+                    auto key = std::make_tuple(v, nn, uv);
+                    auto it = first.find(key);
+                    if (it == first.end()) {
+                        it = first.emplace(key, (unsigned)corners.size()).first;
+                        corners.push_back(key);
+                    }
+                    idxU.push_back(it->second);
+                }
+            }
+        }
+        std::unique_ptr<Mesh> me(new Mesh());
+        for (unsigned v : idxV)
+            if (v >= pos.size()) throw std::runtime_error("OBJ face refers to a missing vertex");
+        if (naive) {
+            me->vertices = pos;
+            me->indices = idxV;
+        } else {
+            me->vertices.resize(corners.size());
+            if (!tex.empty()) me->uvs.resize(corners.size());
+            for (size_t k = 0; k < corners.size(); k++) {
+                me->vertices[k] = pos[std::get<0>(corners[k])];
+                if (!tex.empty()) {
+                    unsigned uv = std::get<2>(corners[k]);
+                    if (uv >= tex.size()) throw std::runtime_error("OBJ face refers to a missing uv");
+                    me->uvs[k] = tex[uv];
+                }
+            }
+            me->indices = idxU;
+        }
+        me->compute_normals();
+        me->construct_edges();
+        me->compute_edge_normals();
+        return (void*)me.release();
+    });
+    return m;
+}
+
+scVector2* scGetMeshUVBuffer(scMesh mesh)
+{
+    return guard_r<scVector2*>(nullptr, [&] {
+        Mesh* me = (Mesh*)mesh.ptr;
+        if (!me) throw std::runtime_error("null mesh");
+        return me->uvs.empty() ? (scVector2*)nullptr : me->uvs.data();
+    });
+}
 
 int scGetParticlePositions(scModel model, scVector3* out, int count)
 {
