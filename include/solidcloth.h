@@ -185,12 +185,13 @@ SCEXP void scFreeSDF(scSDF sdf);
  * Loading narrows box and values to float and computes the gradients; a file
  * that cannot be opened or is short records an error and returns a null
  * handle (the reference leaves the SDF unchanged). Writing returns 1, or 0
- * with an error recorded. scGetSDFGrid copies the cell counts (3 ints), the
- * box and the node values (any pointer may be null) and returns the node
+ * with an error recorded. Grids of more than 2^31 - 1 nodes are refused.
+ * scGetSDFGrid copies the cell counts (3 ints), the box and at most
+ * `capacity` node values (any pointer may be null) and returns the node
  * count. */
 SCEXP scSDF scLoadSDFFile(const char* path);
 SCEXP int scWriteSDFFile(scSDF sdf, const char* path);
-SCEXP int scGetSDFGrid(scSDF sdf, int* cells, scAabb3* box, float* values);
+SCEXP int scGetSDFGrid(scSDF sdf, int* cells, scAabb3* box, float* values, int capacity);
 
 /* Indexed triangle mesh from arrays (replaces Geometry::Mesh* in scCreateMesh). */
 SCEXP scMesh scCreateMeshFromArrays(const scVector3* vertices, int numVertices, const unsigned int* indices,
@@ -296,12 +297,21 @@ SCEXP int scGetTriContactData(scModel model, int* ids, float* data, int cap);
  * scSetSelfCCD: the detector's absolute thresholds (coplanarity volume 0.001,
  * bisection 0.01, vertex-triangle distance 0.1 in the reference) for scenes
  * at other length scales.
- * scSetSelfCapacity: contacts per kind per substep (0 = automatic); a step
- * that exceeds it reports an error. */
+ * scSetSelfCapacity: initial contacts per kind per substep (0 = automatic).
+ * A step whose contacts or candidate pairs outgrow their buffers is rerun
+ * from its starting state with the buffers grown to the step's peaks; a
+ * step that still cannot complete (colour groups exhausted) restores its
+ * starting state and records an error.
+ * scGetSelfStats: out[0] contact capacity in use, out[1] steps rerun after
+ * an overflow so far, out[2] / out[3] most vertex-triangle / edge-edge
+ * contacts in one substep of the last step, out[4] candidate-pair pool
+ * capacity per kind. Returns the number of values (5), writing at most
+ * count. No counterpart in the reference (its lists are std::vectors). */
 SCEXP int scGetSelfContacts(scModel model, int kind, unsigned long long* keys, int* ids, float* data, int cap);
 SCEXP void scSetSelfParams(scModel model, float shrink, float threshold);
 SCEXP void scSetSelfCCD(scModel model, float volume, float bisect, float vtDistance);
 SCEXP void scSetSelfCapacity(scModel model, int capacity);
+SCEXP int scGetSelfStats(scModel model, int* out, int count);
 
 /* Solver statistics of the last substep (Physics/ClothModel.h:32-40):
  * out[0] iterations, out[1] bend error, out[2] vertex-contact error,

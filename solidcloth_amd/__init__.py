@@ -99,7 +99,7 @@ def _sig(L):
         "scSetFixedIterations": (None, [H, b]), "scSetInitIncline": (None, [H, f]),
         "scCreateGridSDF": (H, [vp, i, i, i, scAabb3]), "scFreeSDF": (None, [H]),
         "scLoadSDFFile": (H, [C.c_char_p]), "scWriteSDFFile": (i, [H, C.c_char_p]),
-        "scGetSDFGrid": (i, [H, vp, vp, vp]),
+        "scGetSDFGrid": (i, [H, vp, vp, vp, i]),
         "scLoadMeshFromObj": (H, [C.c_char_p, b]), "scGetMeshUVBuffer": (C.POINTER(scVector2), [H]),
         "scCreateMeshFromArrays": (H, [vp, i, vp, i]), "scFreeMesh": (None, [H]),
         "scGetParticlePositions": (i, [H, vp, i]), "scSetParticlePositions": (i, [H, vp, i]),
@@ -115,6 +115,7 @@ def _sig(L):
         "scLbvhSelfCheck": (i, [i, C.c_uint, i, vp]),
         "scGetSelfContacts": (i, [H, i, vp, vp, vp, i]), "scSetSelfParams": (None, [H, f, f]),
         "scSetSelfCCD": (None, [H, f, f, f]), "scSetSelfCapacity": (None, [H, i]),
+        "scGetSelfStats": (i, [H, vp, i]),
         "scSetParticleInvMasses": (i, [H, vp, i]),
         "scCreateWarmStart": (H, [C.c_uint]), "scFreeWarmStart": (None, [H]),
         "scWarmStartGetParams": (i, [H, vp, i]), "scWarmStartSetParams": (i, [H, vp, i]),
@@ -193,7 +194,7 @@ class SDF:
         self = cls.__new__(cls)
         self.h = _checked("scLoadSDFFile", os.fsencode(path))
         cells, box = (C.c_int * 3)(), scAabb3()
-        _checked("scGetSDFGrid", self.h, C.cast(cells, C.c_void_p), C.cast(C.pointer(box), C.c_void_p), None)
+        _checked("scGetSDFGrid", self.h, C.cast(cells, C.c_void_p), C.cast(C.pointer(box), C.c_void_p), None, 0)
         self.n = tuple(cells)
         self.bmin = (box.min.x, box.min.y, box.min.z)
         self.bmax = (box.max.x, box.max.y, box.max.z)
@@ -207,7 +208,7 @@ class SDF:
         """Node values, shape (nz+1, ny+1, nx+1)."""
         nx, ny, nz = self.n
         out = np.empty((nz + 1, ny + 1, nx + 1), dtype=np.float32)
-        _checked("scGetSDFGrid", self.h, None, None, _ptr(out))
+        _checked("scGetSDFGrid", self.h, None, None, _ptr(out), out.size)
         return out
 
 
@@ -575,6 +576,15 @@ class ClothModel:
 
     def SetSelfCapacity(self, capacity):
         _checked("scSetSelfCapacity", self.h, int(capacity))
+
+    def self_stats(self):
+        """Contact capacity in use, steps rerun after an overflow, most
+        vertex-triangle / edge-edge contacts in one substep of the last step,
+        candidate-pair pool capacity."""
+        out = np.zeros(5, dtype=np.int32)
+        _checked("scGetSelfStats", self.h, _ptr(out), 5)
+        return dict(capacity=int(out[0]), reruns=int(out[1]), peak_vt=int(out[2]), peak_ee=int(out[3]),
+                    pool=int(out[4]))
 
     def enable_kernel_timing(self, on=True):
         _checked("scEnableKernelTiming", self.h, bool(on))

@@ -2,6 +2,7 @@
 // reference's Demo/ClothAPI.cpp (cited per function) over the MI355X-native
 // objects of model.h. Exceptions never cross the boundary: they become the
 // thread's last error and the call returns a neutral value.
+#include <climits>
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -627,7 +628,17 @@ scSDF scLoadSDFFile(const char* path)
             throw std::runtime_error(std::string("truncated SDF header in ") + path);
         for (int a = 0; a < 3; a++)
             if (n[a] < 1 || n[a] > 4095) throw std::runtime_error("SDF cell count out of range (1..4095)");
+        if ((double)(n[0] + 1.0) * (n[1] + 1.0) * (n[2] + 1.0) > (double)INT_MAX)
+            throw std::runtime_error("SDF grid out of range: more than 2^31 - 1 nodes");
         size_t count = (size_t)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
+        // the values must all be there before they are allocated
+        const long here = ftell(f.get());
+        if (here < 0 || fseek(f.get(), 0, SEEK_END) != 0)
+            throw std::runtime_error(std::string("cannot seek in SDF file ") + path);
+        const long end = ftell(f.get());
+        if (end < 0 || (unsigned long)(end - here) < count * sizeof(double))
+            throw std::runtime_error(std::string("truncated SDF values in ") + path);
+        if (fseek(f.get(), here, SEEK_SET) != 0) throw std::runtime_error(std::string("cannot seek in SDF file ") + path);
         std::vector<double> v(count);
         if (fread(v.data(), sizeof(double), count, f.get()) != count)
             throw std::runtime_error(std::string("truncated SDF values in ") + path);
@@ -658,14 +669,15 @@ int scWriteSDFFile(scSDF sdf, const char* path)
     });
 }
 
-int scGetSDFGrid(scSDF sdf, int* cells, scAabb3* box, float* values)
+int scGetSDFGrid(scSDF sdf, int* cells, scAabb3* box, float* values, int capacity)
 {
     return guard_r<int>(0, [&] {
         const SDF* g = (const SDF*)sdf.ptr;
         if (!g) throw std::runtime_error("null SDF");
         if (cells) for (int a = 0; a < 3; a++) cells[a] = g->nc[a];
         if (box) *box = g->box;
-        if (values) memcpy(values, g->vals.data(), g->vals.size() * sizeof(float));
+        if (values && capacity > 0)
+            memcpy(values, g->vals.data(), std::min((size_t)capacity, g->vals.size()) * sizeof(float));
         return (int)g->vals.size();
     });
 }
@@ -1162,6 +1174,17 @@ void scSetSelfCapacity(scModel model, int capacity)
     guard([&] {
         if (capacity < 0) throw std::runtime_error("negative self-contact capacity");
         M(model)->mSelfCapacity = capacity;
+    });
+}
+
+int scGetSelfStats(scModel model, int* out, int count)
+{
+    return guard_r<int>(-1, [&] {
+        ClothModel* m = M(model);
+        const int v[5] = {m->gpu.self.cap, m->selfReruns, m->lastSelfPeak[0], m->lastSelfPeak[1],
+                          m->gpu.self.candExtraCap};
+        for (int k = 0; k < std::min(count, 5); k++) out[k] = v[k];
+        return 5;
     });
 }
 

@@ -46,6 +46,9 @@ struct SolverCtl {
     unsigned cl;         // link error of the current iteration (link model)
     int candCount[2];    // largest number of self candidate pairs of one query (vertex, edge)
     int candExtra[2];    // pairs beyond a query's slots, in the shared pools (vertex, edge)
+    // largest selfCount[0..1] and candExtra[0..1] of the step's earlier
+    // substeps (the host clears them before a step and folds in the last)
+    int selfPeak[4];
 };
 
 enum { MAX_COLLIDABLES = 8 };

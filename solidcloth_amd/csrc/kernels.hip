@@ -9,6 +9,7 @@
 
 #include "bvh_query.h"
 #include "collide_dev.h"
+#include "ctl_dev.h"
 #include "devutil.h"
 #include "hiputil.h"
 
@@ -23,22 +24,7 @@ __global__ void k_predict(float4* __restrict__ pos, float4* __restrict__ prev, f
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
-        ctl->done = 0;
-        ctl->k = 0;
-        ctl->cb = 0;
-        ctl->cv = 0;
-        ctl->ce = 0;
-        ctl->ct = 0;
-        ctl->sct = 0;
-        ctl->sce = 0;
-        ctl->cl = 0;
-        ctl->selfCount[0] = 0;
-        ctl->selfCount[1] = 0;
-        ctl->candCount[0] = 0;
-        ctl->candCount[1] = 0;
-        ctl->candExtra[0] = 0;
-        ctl->candExtra[1] = 0;
-        ctl->iterations = 0;
+        begin_substep_ctl(ctl);
     }
     if (i >= n) return;
     float4 P = pos[i];
@@ -135,22 +121,7 @@ __global__ void k_predict_detect(float4* __restrict__ pos, float4* __restrict__ 
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
-        ctl->done = 0;
-        ctl->k = 0;
-        ctl->cb = 0;
-        ctl->cv = 0;
-        ctl->ce = 0;
-        ctl->ct = 0;
-        ctl->sct = 0;
-        ctl->sce = 0;
-        ctl->cl = 0;
-        ctl->selfCount[0] = 0;
-        ctl->selfCount[1] = 0;
-        ctl->candCount[0] = 0;
-        ctl->candCount[1] = 0;
-        ctl->candExtra[0] = 0;
-        ctl->candExtra[1] = 0;
-        ctl->iterations = 0;
+        begin_substep_ctl(ctl);
     }
     if (i >= n) return;
     float4 P = pos[i];

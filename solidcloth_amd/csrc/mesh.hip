@@ -4,6 +4,7 @@
 // the reference's operation order (see kernels.hip).
 #include "bvh_query.h"
 #include "collide_dev.h"
+#include "ctl_dev.h"
 #include "devutil.h"
 #include "kernels.h"
 
@@ -21,22 +22,7 @@ __global__ void __launch_bounds__(kBlock) k_predict_mesh(float4* __restrict__ po
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
-        ctl->done = 0;
-        ctl->k = 0;
-        ctl->cb = 0;
-        ctl->cv = 0;
-        ctl->ce = 0;
-        ctl->ct = 0;
-        ctl->sct = 0;
-        ctl->sce = 0;
-        ctl->cl = 0;
-        ctl->selfCount[0] = 0;
-        ctl->selfCount[1] = 0;
-        ctl->candCount[0] = 0;
-        ctl->candCount[1] = 0;
-        ctl->candExtra[0] = 0;
-        ctl->candExtra[1] = 0;
-        ctl->iterations = 0;
+        begin_substep_ctl(ctl);
     }
     if (i < ec.groups) ec.groupCount[i] = 0;
     if (i < tc.groups) tc.groupCount[i] = 0;

@@ -1,5 +1,6 @@
 // Host implementation: cloth construction, constraint colouring, device
 // buffers, substep recording and hipGraph replay.
+#include <climits>
 #include "model.h"
 
 #include <algorithm>
@@ -19,6 +20,9 @@ SDF::~SDF() { dev_free(d_nodes); }
 // Geometry/SDF.cpp:10-40
 void SDF::create(int nx, int ny, int nz, const scAabb3& b)
 {
+    // the device lookup indexes nodes with 32-bit ints
+    if (nx < 1 || ny < 1 || nz < 1 || (double)(nx + 1.0) * (ny + 1.0) * (nz + 1.0) > (double)INT_MAX)
+        throw std::runtime_error("SDF grid out of range: cell counts must be >= 1 and the node count at most 2^31 - 1");
     nc[0] = nx; nc[1] = ny; nc[2] = nz;
     nv[0] = nx + 1; nv[1] = ny + 1; nv[2] = nz + 1;
     box = b;
@@ -297,6 +301,8 @@ void GpuSolver::release()
     dev_free(warm.snapPos); dev_free(warm.snapPrev); dev_free(warm.snapVel); dev_free(warm.snapCorr);
     dev_free(warm.resPartial); dev_free(warm.resOut);
     warm.nv = -1;
+    dev_free(snap.pos); dev_free(snap.prev); dev_free(snap.vel); dev_free(snap.corr);
+    snap.nv = -1;
     dev_free(render.idx); dev_free(render.map); dev_free(render.vtOff); dev_free(render.vt); dev_free(render.triA); dev_free(render.triB);
     dev_free(render.outV); dev_free(render.outN); dev_free(render.triN);
     render.nv = render.nt = -1;
@@ -463,6 +469,7 @@ void ClothModel::build_device_topology()
     g.release_graphs();
     g.warm.built = false;
     g.jac.nl = g.jac.nv = -1;  // rebuilt from the new links when a Jacobi-type solver runs
+    g.render.nv = g.render.nt = -1;  // the render topology and weld map are rebuilt on the next sync
     g.nv = (int)pos.size();
     g.nt = (int)tris.size();
     g.nb = (int)bends.size();
@@ -680,7 +687,8 @@ void ClothModel::ensure_self()
 {
     GpuSolver& g = gpu;
     GpuSolver::Self& S = g.self;
-    const int cap = mSelfCapacity > 0 ? mSelfCapacity : std::max(4096, 2 * (g.nv + g.ne));
+    const int cap = std::max(mSelfCapacity > 0 ? mSelfCapacity : std::max(4096, 2 * (g.nv + g.ne)),
+                             mSelfCapacityGrown);
     if (cap != S.cap) {
         for (SelfBuf& b : S.buf) {
             dev_alloc(b.ids, cap); dev_alloc(b.W, cap); dev_alloc(b.N, cap); dev_alloc(b.key, cap);
@@ -1153,8 +1161,7 @@ void ClothModel::step_substeps(float h, int n)
     ensure_feature_capacity(edgeSlots, triSlots);
     if (mCollFlags & CF_SELF) {
         ensure_self();
-        static const int zero = 0;
-        HIP_OK(hipMemcpyAsync(&gpu.d_ctl->selfOverflow, &zero, sizeof(int), hipMemcpyHostToDevice, gpu.stream));
+        clear_self_counters();
     }
     GpuSolver& g = gpu;
     const bool timed = g.timer.enabled;
@@ -1183,50 +1190,122 @@ void ClothModel::step_substeps(float h, int n)
     // exceeds the last plain step's (or the set threshold) is rerun plain
     // from the saved state, and the first step calibrates
     const bool judge = mWarm && mWarmMode == 2;
-    mLastStepMs = 0.0;
-    if (judge) save_state();
-    mWarmPlainNow = false;
-    run_substeps(h, n, cs, timed);
-    mWarmLastPlain = !judge;
-    if (judge) {
-        const double r = residual();
-        mWarmSteps++;
-        mWarmResidual = r;
-        const double tau = mWarmThreshold >= 0.f ? (double)mWarmThreshold : mWarmTau;
-        if (tau < 0.0 || !(r <= tau)) {
-            if (tau < 0.0) mWarmCalibrations++; else mWarmFallbacks++;
-            restore_state();
-            mWarmPlainNow = true;
-            run_substeps(h, n, cs, timed);
-            mWarmPlainNow = false;
-            mWarmLastPlain = true;
-            const double rp = residual();
-            if (mWarmThreshold < 0.f) mWarmTau = rp;
-            mWarmResidual = rp;
+    // continuous self collision snapshots the step's start: a step whose
+    // contacts or candidate pairs outgrew their buffers is rerun from the
+    // snapshot with the buffers grown to the step's peaks, so that no step
+    // advances with an incomplete contact set
+    const bool selfStep = (mCollFlags & CF_SELF) != 0;
+    if (selfStep) snapshot_step();
+    mLastStepMs = 0.0;  // a rerun's attempts are all charged
+    for (int attempt = 0;; attempt++) {
+        if (selfStep && attempt > 0) {
+            restore_step();
+            ensure_self();
+            clear_self_counters();
         }
+        if (judge) save_state();
+        mWarmPlainNow = false;
+        run_substeps(h, n, cs, timed);
+        mWarmLastPlain = !judge;
+        if (judge) {
+            const double r = residual();
+            mWarmSteps++;
+            mWarmResidual = r;
+            const double tau = mWarmThreshold >= 0.f ? (double)mWarmThreshold : mWarmTau;
+            if (tau < 0.0 || !(r <= tau)) {
+                if (tau < 0.0) mWarmCalibrations++; else mWarmFallbacks++;
+                restore_state();
+                mWarmPlainNow = true;
+                run_substeps(h, n, cs, timed);
+                mWarmPlainNow = false;
+                mWarmLastPlain = true;
+                const double rp = residual();
+                if (mWarmThreshold < 0.f) mWarmTau = rp;
+                mWarmResidual = rp;
+            }
+        }
+        if (timed) g.timer.harvest();
+        if (!selfStep || !self_step_overflowed(attempt)) break;
     }
-    if (mCollFlags & CF_SELF) {
-        SolverCtl c;
-        HIP_OK(hipMemcpy(&c, g.d_ctl, sizeof(c), hipMemcpyDeviceToHost));
-        GpuSolver::Self& S = g.self;
-        if (c.candExtra[0] > S.candExtraCap / 5 * 4) S.candWantVT = (int)std::min<long>(1 << 30, 2L * c.candExtra[0]);
-        if (c.candExtra[1] > S.candExtraCap / 5 * 4) S.candWantEE = (int)std::min<long>(1 << 30, 2L * c.candExtra[1]);
-        if (c.selfOverflow)
-            throw std::runtime_error(
-                "self collision overflowed in this step: the last substep put " + std::to_string(c.candExtra[0]) +
-                " vertex-triangle and " + std::to_string(c.candExtra[1]) +
-                " edge-edge candidate pairs beyond the per-query slots (pool capacity " +
-                std::to_string(S.candExtraCap) + ", grown for the next step) and found " +
-                std::to_string(c.selfCount[0]) +
-                " vertex-triangle and " + std::to_string(c.selfCount[1]) + " edge-edge contacts (capacity " +
-                std::to_string(g.self.cap) + ", raise with scSetSelfCapacity) in " +
-                std::to_string(c.selfColours[0]) + " / " + std::to_string(c.selfColours[1]) + " colour groups (at most " +
-                std::to_string(kMaxSelfColours) + "); the step's self contacts are incomplete");
-    }
-    if (timed) g.timer.harvest();
     mDeviceNewer = true;
     mStatsPending = true;
     contactsValid = false;
+}
+
+// Copies of the state a step starts from (positions, previous positions,
+// velocities and the warm start's last correction).
+void ClothModel::snapshot_step()
+{
+    GpuSolver& g = gpu;
+    GpuSolver::StepSnap& S = g.snap;
+    if (S.nv != g.nv) {
+        dev_alloc(S.pos, std::max(g.nv, 1));
+        dev_alloc(S.prev, std::max(g.nv, 1));
+        dev_alloc(S.vel, std::max(g.nv, 1));
+        dev_alloc(S.corr, std::max(g.nv, 1));
+        S.nv = g.nv;
+    }
+    launch_copy(g.stream, g.d_pos, S.pos, g.nv);
+    launch_copy(g.stream, g.d_prev, S.prev, g.nv);
+    launch_copy(g.stream, g.d_vel, S.vel, g.nv);
+    S.hasCorr = g.warm.built && g.warm.corr && g.warm.nv == g.nv;
+    if (S.hasCorr) launch_copy(g.stream, g.warm.corr, S.corr, g.nv);
+}
+
+void ClothModel::restore_step()
+{
+    GpuSolver& g = gpu;
+    GpuSolver::StepSnap& S = g.snap;
+    launch_copy(g.stream, S.pos, g.d_pos, g.nv);
+    launch_copy(g.stream, S.prev, g.d_prev, g.nv);
+    launch_copy(g.stream, S.vel, g.d_vel, g.nv);
+    if (S.hasCorr) launch_copy(g.stream, S.corr, g.warm.corr, g.nv);
+    HIP_OK(hipStreamSynchronize(g.stream));
+}
+
+void ClothModel::clear_self_counters()
+{
+    GpuSolver& g = gpu;
+    HIP_OK(hipMemsetAsync(&g.d_ctl->selfOverflow, 0, sizeof(int), g.stream));
+    HIP_OK(hipMemsetAsync(g.d_ctl->selfPeak, 0, sizeof(g.d_ctl->selfPeak), g.stream));
+}
+
+// After a self-colliding step: grow the candidate pools and contact buffers
+// from the step's peaks. Returns true when the step overflowed and must be
+// rerun; throws (with the state of the step's start restored) when the rerun
+// cannot help.
+bool ClothModel::self_step_overflowed(int attempt)
+{
+    GpuSolver& g = gpu;
+    GpuSolver::Self& S = g.self;
+    SolverCtl c;
+    HIP_OK(hipMemcpy(&c, g.d_ctl, sizeof(c), hipMemcpyDeviceToHost));
+    const long peak[4] = {std::max(c.selfPeak[0], c.selfCount[0]), std::max(c.selfPeak[1], c.selfCount[1]),
+                          std::max(c.selfPeak[2], c.candExtra[0]), std::max(c.selfPeak[3], c.candExtra[1])};
+    lastSelfPeak[0] = (int)peak[0];
+    lastSelfPeak[1] = (int)peak[1];
+    const long limit = 1L << 30;
+    if (peak[2] > S.candExtraCap / 5 * 4) S.candWantVT = (int)std::min(limit, 2 * peak[2]);
+    if (peak[3] > S.candExtraCap / 5 * 4) S.candWantEE = (int)std::min(limit, 2 * peak[3]);
+    if (!c.selfOverflow) return false;
+    selfReruns++;
+    const long need = std::max(peak[0], peak[1]);
+    const bool contactsFull = need > S.cap;
+    const bool poolsFull = peak[2] > S.candExtraCap || peak[3] > S.candExtraCap;
+    if (contactsFull) mSelfCapacityGrown = (int)std::min(limit, std::max(2L * S.cap, need + need / 2));
+    if ((!contactsFull && !poolsFull) || attempt >= 8 || need >= limit) {
+        restore_step();
+        after_device_step();
+        throw std::runtime_error(
+            "self collision overflowed and the step was not taken: " + std::to_string(peak[0]) +
+            " vertex-triangle and " + std::to_string(peak[1]) + " edge-edge contacts (capacity " +
+            std::to_string(S.cap) + "), " + std::to_string(peak[2]) + " / " + std::to_string(peak[3]) +
+            " candidate pairs beyond the per-query slots (pool " + std::to_string(S.candExtraCap) + "), " +
+            std::to_string(c.selfColours[0]) + " / " + std::to_string(c.selfColours[1]) +
+            " colour groups (at most " + std::to_string(kMaxSelfColours) + ") after " + std::to_string(attempt + 1) +
+            " attempts");
+    }
+    return true;
 }
 
 void ClothModel::fetch_stats()
@@ -1600,6 +1679,7 @@ void ClothModel::render_mesh(Mesh& mesh, f3 offset, const std::vector<int>& map)
     if (r.nv != nv || r.nt != nt) {
         dev_free(r.idx); dev_free(r.map); dev_free(r.vtOff); dev_free(r.vt); dev_free(r.triA); dev_free(r.triB);
         dev_free(r.outV); dev_free(r.outN); dev_free(r.triN);
+        r.nv = r.nt = -1;  // stays invalid until every buffer below is rebuilt
         if (!map.empty()) {
             for (int v : map)
                 if (v < 0 || v >= (int)pos.size()) throw std::runtime_error("render mesh map out of range");

@@ -278,6 +278,12 @@ public:
         int nv = -1, nt = -1;
     } render;
     MouseSpringDev* d_mouse = nullptr;
+    // the state a self-colliding step starts from, for its rerun after an overflow
+    struct StepSnap {
+        float4 *pos = nullptr, *prev = nullptr, *vel = nullptr, *corr = nullptr;
+        int nv = -1;
+        bool hasCorr = false;
+    } snap;
     // links in the reference's order for the Jacobi / conjugate-residual
     // solvers (built when one is selected)
     struct Jacobi {
@@ -312,6 +318,9 @@ public:
     // 296, 313-317, 446 constants by default); capacity 0 = automatic
     float mCcdVolume = 0.001f, mCcdBisect = 0.01f, mCcdVtDistance = 0.1f;
     int mSelfCapacity = 0;
+    int mSelfCapacityGrown = 0;  // contact capacity an overflowed step grew to
+    int selfReruns = 0;          // steps rerun after a self-collision overflow
+    int lastSelfPeak[2] = {0, 0};  // most vertex-triangle / edge-edge contacts in one substep of the last step
     int mNumIterations = 50, mCollFlags = 0, mBendFreq = 2;
     bool mDihedral = true, mFEM = false, mFixedIterations = false, mUseGraph = true;
 
@@ -388,6 +397,10 @@ private:
     void ensure_feature_capacity(int edgeSlots, int triSlots);
     void ensure_self();
     SelfParams self_params() const;
+    void snapshot_step();
+    void restore_step();
+    void clear_self_counters();
+    bool self_step_overflowed(int attempt);
 
 public:
     // batch support: validate and upload everything a substep needs, then

@@ -62,6 +62,38 @@ def test_sdf_file_errors(sc, tmp_path):
         sc.SDF.from_file(p)
 
 
+def test_sdf_grid_beyond_int_range_is_refused(sc, tmp_path):
+    """1400^3 cells are 1401^3 = 2.75e9 nodes, past the 32-bit node index of
+    the device lookup: the header alone must be refused as out of range
+    (not read as a truncated file of 22 GB of values)."""
+    p = tmp_path / "huge.sdf"
+    p.write_bytes(sdf_bytes((1400, 1400, 1400), (0, 0, 0), (1, 1, 1), np.zeros(0)))
+    with pytest.raises(sc.SolidClothError, match="out of range"):
+        sc.SDF.from_file(p)
+    # 1289^3 cells (1290^3 = 2,146,689,000 < 2^31 - 1 nodes) passes the range
+    # check and then fails on the missing values; 1290^3 cells would not
+    p.write_bytes(sdf_bytes((1289, 1289, 1289), (0, 0, 0), (1, 1, 1), np.zeros(0)))
+    with pytest.raises(sc.SolidClothError, match="truncated"):
+        sc.SDF.from_file(p)
+    p.write_bytes(sdf_bytes((1290, 1290, 1290), (0, 0, 0), (1, 1, 1), np.zeros(0)))
+    with pytest.raises(sc.SolidClothError, match="out of range"):
+        sc.SDF.from_file(p)
+
+
+def test_sdf_grid_copy_respects_capacity(sc):
+    """scGetSDFGrid writes at most `capacity` values and returns the node count."""
+    import ctypes as C
+
+    vals = np.arange(27, dtype=np.float32).reshape(3, 3, 3)
+    s = sc.SDF(vals, (2, 2, 2), (0, 0, 0), (1, 1, 1))
+    out = np.full(40, -7.0, dtype=np.float32)
+    n = sc.lib().scGetSDFGrid(s.h, None, None, out.ctypes.data_as(C.c_void_p), 10)
+    assert n == 27
+    assert np.array_equal(out[:10], vals.ravel()[:10]) and np.all(out[10:] == -7.0)
+    n = sc.lib().scGetSDFGrid(s.h, None, None, out.ctypes.data_as(C.c_void_p), 40)
+    assert n == 27 and np.array_equal(out[:27], vals.ravel()) and np.all(out[27:] == -7.0)
+
+
 QUAD_OBJ = """# two triangles and a quad face
 o quad
 v 0 0 0

@@ -47,3 +47,23 @@ def test_render_mesh_drape256_on_sphere(sc, oracle_mod):
     for f in range(30):  # draped over the sphere: curved, folded regions
         patch.Step(scene.dt)
     _check(patch, m, oracle_mod, "drape256 after 30 frames")
+
+
+def test_render_mesh_after_reinit_with_same_counts(sc, oracle_mod):
+    """A 4x8 grid and an 8x4 grid both have 45 vertices and 64 triangles:
+    re-initialising the patch must rebuild the render topology rather than
+    reuse the first grid's, so the normals follow the new triangles."""
+    print("render mesh re-init 4x8 -> 8x4", flush=True)
+    world = sc.CollisionWorld()
+    patch = sc.ClothPatch(world)
+    m = patch.GetModel()
+    patch.Init(4, 8, 0.1, (0, 0, 0), True, True)
+    for f in range(3):
+        patch.Step(1 / 60)
+    _check(patch, m, oracle_mod, "4x8 after 3 frames")
+    patch.Init(8, 4, 0.1, (0, 0, 0), True, True)
+    for f in range(3):
+        patch.Step(1 / 60)
+    verts, normals, idx = patch.mesh_arrays()
+    assert len(verts) == 45 and len(idx) == 3 * 64
+    _check(patch, m, oracle_mod, "8x4 after 3 frames")

@@ -74,3 +74,29 @@ def test_mesh_obstacle_with_self_collision_mid_size(sc, oracle_mod):
     print("mesh+self 40", flush=True)
     seen, m, o = _run(sc, oracle_mod, scene, 1, scene.frames * scene.substeps)
     assert sum(seen) > 0, seen
+
+
+def test_self_overflow_reruns_the_step(sc):
+    """A step whose self contacts outgrow the contact buffers is rerun from
+    its starting state with the buffers grown: a patch that starts with room
+    for 8 contacts per kind ends bit-identical to one with the automatic
+    capacity, reports the reruns, and never exposes a half-advanced state."""
+    scene = S.FOLD24
+    runs = []
+    for cap in (0, 8):
+        print(f"fold24 overflow rerun, capacity {cap}", flush=True)
+        world, patch = S.build_patch(scene, sc)
+        m = patch.GetModel()
+        m.SetSelfCapacity(cap)
+        peaks = []
+        for f in range(scene.frames):
+            patch.Step(scene.dt)
+            st = m.self_stats()
+            peaks.append((st["peak_vt"], st["peak_ee"]))
+        runs.append((m.positions(), m.velocities(), m.self_stats(), peaks))
+    (p0, v0, s0, k0), (p1, v1, s1, k1) = runs
+    assert max(max(k) for k in k0) > 8, "the scene must find more than 8 contacts of a kind in a substep"
+    assert s0["reruns"] == 0, s0
+    assert s1["reruns"] >= 1 and s1["capacity"] > 8, s1
+    assert k0 == k1, "both runs must see the same contact peaks step by step"
+    assert np.array_equal(p0, p1) and np.array_equal(v0, v1)
