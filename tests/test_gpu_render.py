@@ -50,9 +50,10 @@ def test_render_mesh_drape256_on_sphere(sc, oracle_mod):
 
 
 def test_render_mesh_after_reinit_with_same_counts(sc, oracle_mod):
-    """A 4x8 grid and an 8x4 grid both have 45 vertices and 64 triangles:
-    re-initialising the patch must rebuild the render topology rather than
-    reuse the first grid's, so the normals follow the new triangles."""
+    """A 4x8 grid and an 8x4 grid have the same vertex and triangle counts
+    (32 and 42): re-initialising the patch must rebuild the render topology
+    rather than reuse the first grid's, so the normals follow the new
+    triangles."""
     print("render mesh re-init 4x8 -> 8x4", flush=True)
     world = sc.CollisionWorld()
     patch = sc.ClothPatch(world)
@@ -61,9 +62,12 @@ def test_render_mesh_after_reinit_with_same_counts(sc, oracle_mod):
     for f in range(3):
         patch.Step(1 / 60)
     _check(patch, m, oracle_mod, "4x8 after 3 frames")
+    v0, _, i0 = patch.mesh_arrays()
     patch.Init(8, 4, 0.1, (0, 0, 0), True, True)
     for f in range(3):
         patch.Step(1 / 60)
+    patch.UpdateMesh()
     verts, normals, idx = patch.mesh_arrays()
-    assert len(verts) == 45 and len(idx) == 3 * 64
+    assert len(verts) == len(v0) and len(idx) == len(i0)
+    assert not np.array_equal(idx, i0), "the two grids must be triangulated differently"
     _check(patch, m, oracle_mod, "8x4 after 3 frames")
