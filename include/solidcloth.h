@@ -416,6 +416,17 @@ SCEXP int scWarmStartSetParams(scWarmStart ws, const float* in, int count);
 SCEXP void scWarmStartSetLearningRate(scWarmStart ws, float lr);
 SCEXP int scWarmStartGetSteps(scWarmStart ws);
 SCEXP int scWarmStartForward(scWarmStart ws, const float* features, int n, float* out);
+/* scWarmStartForwardNoAlloc: no counterpart in the reference. For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): scWarmStartForward with
+ * DEVICE pointers, features (n x 12 FP32) in and out (n x 3 FP32) out, the
+ * BF16 staging of the rows taken from a static per-GPU array of the library
+ * (8 MiB, held until the process ends) instead of allocated memory. At most
+ * 262144 samples (larger n records an error and returns -1); no hipMalloc,
+ * hipFree or other allocation. Enqueued on `stream` (a hipStream_t; NULL =
+ * the null stream) and returns n without waiting; the caller synchronizes.
+ * Calls that overlap on one GPU share the staging array and race. */
+SCEXP int scWarmStartForwardNoAlloc(scWarmStart ws, const float* features, int n, float* out, void* stream);
 SCEXP float scWarmStartGradient(scWarmStart ws, const float* features, const float* targets, const float* weights,
                                 int n, float* grad);
 SCEXP float scWarmStartTrain(scWarmStart ws, const float* features, const float* targets, const float* weights,

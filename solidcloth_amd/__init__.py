@@ -120,7 +120,7 @@ def _sig(L):
         "scCreateWarmStart": (H, [C.c_uint]), "scFreeWarmStart": (None, [H]),
         "scWarmStartGetParams": (i, [H, vp, i]), "scWarmStartSetParams": (i, [H, vp, i]),
         "scWarmStartSetLearningRate": (None, [H, f]), "scWarmStartGetSteps": (i, [H]),
-        "scWarmStartForward": (i, [H, vp, i, vp]), "scWarmStartGradient": (f, [H, vp, vp, vp, i, vp]),
+        "scWarmStartForward": (i, [H, vp, i, vp]), "scWarmStartForwardNoAlloc": (i, [H, vp, i, vp, vp]), "scWarmStartGradient": (f, [H, vp, vp, vp, i, vp]),
         "scWarmStartTrain": (f, [H, vp, vp, vp, i, i]), "scSetWarmStart": (None, [H, H, i, i]),
         "scSetWarmStartFallback": (None, [H, f]), "scGetResidual": (C.c_double, [H]),
         "scGetResidualTerms": (None, [H, vp]),
@@ -741,6 +741,13 @@ class WarmStart:
         out = np.zeros((X.shape[0], 3), dtype=np.float32)
         _checked("scWarmStartForward", self.h, _ptr(X), X.shape[0], _ptr(out))
         return out
+
+    def forward_noalloc(self, features_ptr, n, out_ptr, stream=None):
+        """scWarmStartForwardNoAlloc: device pointers in and out (n x 12 and
+        n x 3 FP32), staged in the library's static scratch; enqueued on
+        `stream` (a hipStream_t as an int, None = the null stream)."""
+        return _checked("scWarmStartForwardNoAlloc", self.h, C.c_void_p(int(features_ptr)), int(n),
+                        C.c_void_p(int(out_ptr)), C.c_void_p(int(stream) if stream else None))
 
     def _tw(self, X, T, w):
         X = np.ascontiguousarray(X, dtype=np.float32)

@@ -924,6 +924,15 @@ int scWarmStartForward(scWarmStart ws, const float* features, int n, float* out)
     });
 }
 
+int scWarmStartForwardNoAlloc(scWarmStart ws, const float* features, int n, float* out, void* stream)
+{
+    return guard_r<int>(-1, [&] {
+        if (!features || !out) throw std::runtime_error("null feature or output pointer");
+        WS(ws)->forward_device_noalloc(features, n, out, (hipStream_t)stream);
+        return n;
+    });
+}
+
 float scWarmStartGradient(scWarmStart ws, const float* features, const float* targets, const float* weights, int n,
                           float* grad)
 {

@@ -151,6 +151,11 @@ public:
     void enqueue_gradient(hipStream_t s, const uint16_t* X, const float4* T, int n, float invCount);
     void enqueue_train(hipStream_t s, const uint16_t* X, const float4* T, int n, float invCount);
     void forward_host(const float* X, int n, float* Y);
+    // device features in, device outputs out, staged in static scratch (no allocation)
+    void forward_device_noalloc(const float* X, int n, float* Y, hipStream_t s)
+    {
+        launch_mlp_forward_noalloc(s, weights(), X, n, Y);
+    }
     float gradient_host(const float* X, const float* T, const float* w, int n, float* grad);
     float train_host(const float* X, const float* T, const float* w, int n, int count);
     void set_lr(float lr);

@@ -86,6 +86,12 @@ void launch_mlp_pack(hipStream_t s, const float* master, uint16_t* packed, int* 
 // FP32 features (n x 12) to the BF16 layout
 void launch_pack_features(hipStream_t s, const float* Xf, int n, uint16_t* X);
 void launch_unpack_features(hipStream_t s, const uint16_t* X, int n, float* Xf);
+// Forward pass over device features Xf (n x 12 FP32) into device Y (n x 3)
+// with the BF16 rows staged in the library's static per-GPU scratch array
+// instead of allocated memory: n <= kNoAllocRows, enqueued on s. Calls that
+// overlap on one GPU share the array and race.
+enum { kNoAllocRows = 1 << 18 };
+void launch_mlp_forward_noalloc(hipStream_t s, MlpWeights w, const float* Xf, int n, float* Y);
 // sums of squared constraint errors in FP64: out[0] = sum of warp^2 + weft^2,
 // out[1] = sum of shear^2, out[2] = sum of (sStiff theta)^2 over the bends;
 // partial holds 3 doubles per block

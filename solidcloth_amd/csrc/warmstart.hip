@@ -12,6 +12,10 @@
 // transposed.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
 #include "devutil.h"
 #include "hiputil.h"
 #include "warmstart.h"
@@ -655,6 +659,46 @@ void launch_pack_features(hipStream_t s, const float* Xf, int n, uint16_t* X)
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_pack_features, dim3(wblocks((long)n * 16)), dim3(kWsBlock), 0, s, Xf, n, X);
+}
+
+// ------------------------------------------------- no-allocation forward ----
+// The BF16 feature rows (16 values, 32 bytes each) of up to kNoAllocRows
+// samples: 8 MiB per GPU, held until the process ends. Every row a call uses
+// is written by k_pack_features before the forward pass reads it, so nothing
+// needs clearing between calls. In host code `scratch` names a host-side
+// stand-in; only the address from hipGetSymbolAddress is used.
+static __device__ __align__(256) uint16_t scratch[(size_t)kNoAllocRows * 16];
+
+enum { kNoAllocMaxDevices = 64 };
+static std::mutex gScratchMutex;
+static uint16_t* gScratchAddr[kNoAllocMaxDevices] = {};
+
+// This GPU's copy of `scratch`, looked up once per device with it current.
+static uint16_t* scratch_address()
+{
+    int dev = -1;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) throw std::runtime_error("device index out of range for the scratch table");
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (!gScratchAddr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        gScratchAddr[dev] = (uint16_t*)p;
+    }
+    return gScratchAddr[dev];
+}
+
+void launch_mlp_forward_noalloc(hipStream_t s, MlpWeights w, const float* Xf, int n, float* Y)
+{
+    if (n <= 0 || n > kNoAllocRows)
+        throw std::runtime_error("the no-allocation forward pass takes 1 to " + std::to_string((int)kNoAllocRows) +
+                                 " samples, got " + std::to_string(n));
+    uint16_t* X = scratch_address();
+    launch_pack_features(s, Xf, n, X);
+    launch_mlp_forward(s, w, X, n, Y);
+    HIP_OK(hipGetLastError());
 }
 
 void launch_unpack_features(hipStream_t s, const uint16_t* X, int n, float* Xf)
