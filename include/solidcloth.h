@@ -300,17 +300,28 @@ SCEXP int scGetTriContactData(scModel model, int* ids, float* data, int cap);
  * scSetSelfCapacity: initial contacts per kind per substep (0 = automatic).
  * A step whose contacts or candidate pairs outgrow their buffers is rerun
  * from its starting state with the buffers grown to the step's peaks; a
- * step that still cannot complete (colour groups exhausted) restores its
- * starting state and records an error.
+ * step that still cannot complete (more than 2^30 contacts or candidate
+ * pairs of a kind) restores its starting state and records an error.
  * scGetSelfStats: out[0] contact capacity in use, out[1] steps rerun after
  * an overflow so far, out[2] / out[3] most vertex-triangle / edge-edge
  * contacts in one substep of the last step, out[4] candidate-pair pool
- * capacity per kind. Returns the number of values (5), writing at most
- * count. No counterpart in the reference (its lists are std::vectors). */
+ * capacity per kind, out[5] / out[6] vertex-triangle / edge-edge colour
+ * groups of the last substep. Returns the number of values (7), writing at
+ * most count. No counterpart in the reference (its lists are std::vectors). */
 SCEXP int scGetSelfContacts(scModel model, int kind, unsigned long long* keys, int* ids, float* data, int cap);
 SCEXP void scSetSelfParams(scModel model, float shrink, float threshold);
 SCEXP void scSetSelfCCD(scModel model, float volume, float bisect, float vtDistance);
 SCEXP void scSetSelfCapacity(scModel model, int capacity);
+/* scSetSelfEdgeWeights: how a self edge-edge projection
+ * (Physics/ClothCollisionHandler.cpp:452-540) distributes its displacement.
+ * 0 (default) as the reference writes it: with the closest points
+ * p = (1 - t1) x1 + t1 x2 and q = (1 - t2) x3 + t2 x4, it moves x1 by t1 and
+ * x2 by 1 - t1 (likewise x3, x4; :531-534), the other end's weight, while its
+ * effective mass uses each end's own; a correction near x1 then lands mostly
+ * on x2 and does not close the gap, and dense self contact collapses the
+ * cloth. 1: each end moves by its own weight (x1 by 1 - t1, x2 by t1, ...).
+ * No counterpart in the reference (an extension). */
+SCEXP void scSetSelfEdgeWeights(scModel model, int mode);
 SCEXP int scGetSelfStats(scModel model, int* out, int count);
 
 /* Solver statistics of the last substep (Physics/ClothModel.h:32-40):

@@ -122,7 +122,8 @@ class Cloth:
 
     FLOAT_PARAMS = ("unit", "thickness", "tolerance", "shear", "bend", "stretch", "poisson", "friction",
                     "area", "drag", "selfShrink", "selfThreshold", "ccdVolume", "ccdBisect", "ccdVtDistance")
-    INT_PARAMS = ("iterations", "collFlags", "bendFreq", "numSteps", "dihedral", "fem", "fixedIterations", "accel", "solver")
+    INT_PARAMS = ("iterations", "collFlags", "bendFreq", "numSteps", "dihedral", "fem", "fixedIterations", "accel", "solver",
+                  "selfEdgeWeights")
 
     def __init__(self, real64=False):
         self.L = lib()
@@ -425,7 +426,7 @@ def from_scene(scene, real64=False, tri_order=None, bend_order=None):
           poisson=scene.poisson, bend=scene.bend,
           iterations=scene.iterations, fixedIterations=int(scene.fixed_iterations), collFlags=scene.flags,
           thickness=scene.thickness, tolerance=scene.tolerance, friction=scene.friction,
-          numSteps=scene.substeps)
+          numSteps=scene.substeps, drag=scene.drag, selfEdgeWeights=scene.self_edge_weights)
     if scene.stretch is not None:
         c.set(stretch=scene.stretch)  # before the grid: links take their stiffness at creation
     if scene.analytic_sphere:

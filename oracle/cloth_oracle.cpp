@@ -46,6 +46,10 @@ struct Params {
     // bisection stop, vertex-triangle distance
     float ccdVolume = 0.001f, ccdBisect = 0.01f, ccdVtDistance = 0.1f;
     int iterations = 50, collFlags = 0, bendFreq = 2, numSteps = 1;
+    // self edge-edge displacement weights: 0 as ClothCollisionHandler.cpp:
+    // 531-534 writes them (the first edge's ends moved by each other's
+    // barycentric weight, likewise the second's), 1 each end moved by its own
+    int selfEdgeWeights = 0;
     // 1: broad phases through uniform grids (same results as the exhaustive
     // scans, for timing the oracle at large sizes)
     int accel = 0;
@@ -1556,10 +1560,17 @@ public:
             const R dLambda = -soften * s * err;
             c.lambda += dLambda / h;
             const V3<R> disp = dLambda * n;
-            x1 -= disp * (omw1 * m1);
-            x2 -= disp * (c.w1 * m2);
-            x3 += disp * (omw2 * m3);
-            x4 += disp * (c.w2 * m4);
+            if (prm.selfEdgeWeights == 0) {
+                x1 -= disp * (omw1 * m1);
+                x2 -= disp * (c.w1 * m2);
+                x3 += disp * (omw2 * m3);
+                x4 += disp * (c.w2 * m4);
+            } else {
+                x1 -= disp * (c.w1 * m1);
+                x2 -= disp * (omw1 * m2);
+                x3 += disp * (c.w2 * m3);
+                x4 += disp * (omw2 * m4);
+            }
             count++;
         }
         return error;
@@ -1866,6 +1877,7 @@ int oc_set_param_i(void* h, const char* name, int v)
         else if (n == "accel") p.accel = v;
         else if (n == "solver") p.solver = v;
         else if (n == "fixedIterations") p.fixedIterations = v != 0;
+        else if (n == "selfEdgeWeights") p.selfEdgeWeights = v;
         else return -1;
         return 0;
     });

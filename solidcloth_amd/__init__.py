@@ -115,6 +115,7 @@ def _sig(L):
         "scLbvhSelfCheck": (i, [i, C.c_uint, i, vp]),
         "scGetSelfContacts": (i, [H, i, vp, vp, vp, i]), "scSetSelfParams": (None, [H, f, f]),
         "scSetSelfCCD": (None, [H, f, f, f]), "scSetSelfCapacity": (None, [H, i]),
+        "scSetSelfEdgeWeights": (None, [H, i]),
         "scGetSelfStats": (i, [H, vp, i]),
         "scSetParticleInvMasses": (i, [H, vp, i]),
         "scCreateWarmStart": (H, [C.c_uint]), "scFreeWarmStart": (None, [H]),
@@ -574,17 +575,21 @@ class ClothModel:
     def SetSelfCCD(self, volume, bisect, vt_distance):
         _checked("scSetSelfCCD", self.h, float(volume), float(bisect), float(vt_distance))
 
+    def SetSelfEdgeWeights(self, mode):
+        """0: the reference's edge-edge displacement weights; 1: each end's own."""
+        _checked("scSetSelfEdgeWeights", self.h, int(mode))
+
     def SetSelfCapacity(self, capacity):
         _checked("scSetSelfCapacity", self.h, int(capacity))
 
     def self_stats(self):
         """Contact capacity in use, steps rerun after an overflow, most
         vertex-triangle / edge-edge contacts in one substep of the last step,
-        candidate-pair pool capacity."""
-        out = np.zeros(5, dtype=np.int32)
-        _checked("scGetSelfStats", self.h, _ptr(out), 5)
+        candidate-pair pool capacity, colour groups of the last substep."""
+        out = np.zeros(7, dtype=np.int32)
+        _checked("scGetSelfStats", self.h, _ptr(out), 7)
         return dict(capacity=int(out[0]), reruns=int(out[1]), peak_vt=int(out[2]), peak_ee=int(out[3]),
-                    pool=int(out[4]))
+                    pool=int(out[4]), colours_vt=int(out[5]), colours_ee=int(out[6]))
 
     def enable_kernel_timing(self, on=True):
         _checked("scEnableKernelTiming", self.h, bool(on))

@@ -1186,14 +1186,22 @@ void scSetSelfCapacity(scModel model, int capacity)
     });
 }
 
+void scSetSelfEdgeWeights(scModel model, int mode)
+{
+    guard([&] {
+        if (mode != 0 && mode != 1) throw std::runtime_error("self edge-edge weight mode must be 0 or 1");
+        M(model)->mSelfEdgeWeights = mode;
+    });
+}
+
 int scGetSelfStats(scModel model, int* out, int count)
 {
     return guard_r<int>(-1, [&] {
         ClothModel* m = M(model);
-        const int v[5] = {m->gpu.self.cap, m->selfReruns, m->lastSelfPeak[0], m->lastSelfPeak[1],
-                          m->gpu.self.candExtraCap};
-        for (int k = 0; k < std::min(count, 5); k++) out[k] = v[k];
-        return 5;
+        const int v[7] = {m->gpu.self.cap, m->selfReruns, m->lastSelfPeak[0], m->lastSelfPeak[1],
+                          m->gpu.self.candExtraCap, m->lastSelfColours[0], m->lastSelfColours[1]};
+        for (int k = 0; k < std::min(count, 7); k++) out[k] = v[k];
+        return 7;
     });
 }
 

@@ -42,7 +42,7 @@ struct SolverCtl {
     unsigned sce;    // self edge-edge error of the current iteration
     int selfCount[2];    // self contacts found this substep (vertex-triangle, edge-edge)
     int selfColours[2];  // colour groups of each kind
-    int selfOverflow;    // contacts or candidates beyond capacity, or colouring rounds exhausted
+    int selfOverflow;    // contacts or candidate pairs beyond capacity
     unsigned cl;         // link error of the current iteration (link model)
     int candCount[2];    // largest number of self candidate pairs of one query (vertex, edge)
     int candExtra[2];    // pairs beyond a query's slots, in the shared pools (vertex, edge)
@@ -137,7 +137,7 @@ struct SlabSet {
     int count;  // constraints in the schedule (bound of prefetch indices)
 };
 
-enum { kSlabBlock = 256, kMaxSlabColours = 63, kMaxSelfColours = 4096, kSlabBatch = 8 };
+enum { kSlabBlock = 256, kMaxSlabColours = 63, kSlabBatch = 8 };
 
 // Vertex contacts applied by the FEM slab kernel while it stages vertices
 // (the previous iteration's contacts, by each vertex's owner slab).
@@ -173,6 +173,7 @@ struct SelfParams {
     float shrink;
     double volume, bisect;  // CCD thresholds (FP64 cubic)
     float vtDistance;
+    int eeWeights;  // 0: edge-edge ends moved as ClothCollisionHandler.cpp:531-534 writes it; 1: by their own weights
 };
 
 // One cloth of a batch: its own device buffers (the patch's), its greedy

@@ -678,6 +678,7 @@ SelfParams ClothModel::self_params() const
     sp.volume = (double)mCcdVolume;
     sp.bisect = (double)mCcdBisect;
     sp.vtDistance = mCcdVtDistance;
+    sp.eeWeights = mSelfEdgeWeights;
     return sp;
 }
 
@@ -693,9 +694,9 @@ void ClothModel::ensure_self()
         for (SelfBuf& b : S.buf) {
             dev_alloc(b.ids, cap); dev_alloc(b.W, cap); dev_alloc(b.N, cap); dev_alloc(b.key, cap);
             dev_alloc(b.colour, cap); dev_alloc(b.order, cap); dev_alloc(b.active, cap);
-            dev_alloc(b.groupOff, kMaxSelfColours + 1);
+            dev_alloc(b.groupOff, (size_t)cap + 1);  // at most one colour per contact
             b.cap = cap;
-            b.maxColours = kMaxSelfColours;
+            b.maxColours = cap;
         }
         S.cap = cap;
         g.release_graphs();
@@ -842,7 +843,7 @@ std::string ClothModel::graph_key(float h, const CollSet& cs) const
     put(&gpu.ec.P, sizeof(void*)); put(&gpu.ec.slots, 4);
     put(&gpu.tc.P, sizeof(void*)); put(&gpu.tc.slots, 4);
     put(&mSelfShrink, 4); put(&mSelfCollThreshold, 4); put(&mCcdVolume, 4); put(&mCcdBisect, 4);
-    put(&mCcdVtDistance, 4); put(&gpu.self.cap, 4);
+    put(&mCcdVtDistance, 4); put(&gpu.self.cap, 4); put(&mSelfEdgeWeights, 4);
     const BvhView tv = gpu.self.triB.view(), ev = gpu.self.edgeB.view();
     put(&tv, sizeof(tv)); put(&ev, sizeof(ev));
     put(&gpu.self.buf[0].ids, sizeof(void*)); put(&gpu.self.buf[1].ids, sizeof(void*));
@@ -1284,6 +1285,8 @@ bool ClothModel::self_step_overflowed(int attempt)
                           std::max(c.selfPeak[2], c.candExtra[0]), std::max(c.selfPeak[3], c.candExtra[1])};
     lastSelfPeak[0] = (int)peak[0];
     lastSelfPeak[1] = (int)peak[1];
+    lastSelfColours[0] = c.selfColours[0];
+    lastSelfColours[1] = c.selfColours[1];
     const long limit = 1L << 30;
     if (peak[2] > S.candExtraCap / 5 * 4) S.candWantVT = (int)std::min(limit, 2 * peak[2]);
     if (peak[3] > S.candExtraCap / 5 * 4) S.candWantEE = (int)std::min(limit, 2 * peak[3]);
@@ -1301,9 +1304,8 @@ bool ClothModel::self_step_overflowed(int attempt)
             " vertex-triangle and " + std::to_string(peak[1]) + " edge-edge contacts (capacity " +
             std::to_string(S.cap) + "), " + std::to_string(peak[2]) + " / " + std::to_string(peak[3]) +
             " candidate pairs beyond the per-query slots (pool " + std::to_string(S.candExtraCap) + "), " +
-            std::to_string(c.selfColours[0]) + " / " + std::to_string(c.selfColours[1]) +
-            " colour groups (at most " + std::to_string(kMaxSelfColours) + ") after " + std::to_string(attempt + 1) +
-            " attempts");
+            std::to_string(c.selfColours[0]) + " / " + std::to_string(c.selfColours[1]) + " colour groups after " +
+            std::to_string(attempt + 1) + " attempts");
     }
     return true;
 }

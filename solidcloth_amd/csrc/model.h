@@ -323,9 +323,11 @@ public:
     // 296, 313-317, 446 constants by default); capacity 0 = automatic
     float mCcdVolume = 0.001f, mCcdBisect = 0.01f, mCcdVtDistance = 0.1f;
     int mSelfCapacity = 0;
+    int mSelfEdgeWeights = 0;  // 0: the reference's edge-edge displacement weights; 1: each end's own
     int mSelfCapacityGrown = 0;  // contact capacity an overflowed step grew to
     int selfReruns = 0;          // steps rerun after a self-collision overflow
     int lastSelfPeak[2] = {0, 0};  // most vertex-triangle / edge-edge contacts in one substep of the last step
+    int lastSelfColours[2] = {0, 0};  // colour groups of each kind in the last substep
     int mNumIterations = 50, mCollFlags = 0, mBendFreq = 2;
     bool mDihedral = true, mFEM = false, mFixedIterations = false, mUseGraph = true;
 

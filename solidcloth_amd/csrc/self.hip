@@ -447,13 +447,16 @@ __device__ __forceinline__ unsigned long long bid_of(unsigned long long key)
 // round r every uncoloured contact bids at its four vertices; a contact that
 // holds the highest bid at all four takes colour r. Winners of a round share
 // no vertex, and the highest uncoloured bid always wins, so every round makes
-// progress; the result depends only on the set of contacts. The contacts are
-// then listed colour by colour.
+// progress and there are at most as many colours as contacts; the result
+// depends only on the set of contacts. The contacts are then listed colour by
+// colour: counts per colour in groupOff (global memory, capacity + 1
+// entries), an exclusive prefix sum over them, and a scatter through cursors
+// kept in the (by then unused) round list.
 __global__ void __launch_bounds__(kSoloBlock) k_self_colour(SelfBuf b, int kind, unsigned long long* vmax,
                                                             SolverCtl* ctl)
 {
     __shared__ int nextCount;
-    __shared__ int cnt[kMaxSelfColours + 1];
+    __shared__ int part[kSoloBlock];
     const int n = min(ctl->selfCount[kind], b.cap);
     const int tid = threadIdx.x, nt = blockDim.x;
     auto vreset = [&](int c) {
@@ -508,36 +511,40 @@ __global__ void __launch_bounds__(kSoloBlock) k_self_colour(SelfBuf b, int kind,
         cur = nxt;
         nxt = t;
         __syncthreads();
-        if (m == 0) break;
-        if (++r >= b.maxColours) {
-            if (tid == 0) ctl->selfOverflow = 1;
-            break;
-        }
+        r++;
     }
-    const int nc = n > 0 ? min(r + 1, b.maxColours) : 0;
-    for (int g = tid; g <= nc; g += nt) cnt[g] = 0;
+    // r colours; counts, then offsets
+    const int nc = r;
+    for (int g = tid; g <= nc; g += nt) b.groupOff[g] = 0;
     __syncthreads();
-    for (int c = tid; c < n; c += nt) {
-        const int col = b.colour[c];
-        if (col >= 0 && col < nc) atomicAdd(&cnt[col], 1);
-    }
+    for (int c = tid; c < n; c += nt) atomicAdd(&b.groupOff[b.colour[c]], 1);
+    __syncthreads();
+    // exclusive scan of groupOff[0 .. nc]: thread t owns a contiguous chunk
+    const int chunk = (nc + 1 + nt - 1) / nt;
+    const int lo = min(tid * chunk, nc + 1), hi = min(lo + chunk, nc + 1);
+    int sum = 0;
+    for (int g = lo; g < hi; g++) sum += b.groupOff[g];
+    part[tid] = sum;
     __syncthreads();
     if (tid == 0) {
         int run = 0;
-        for (int g = 0; g < nc; g++) {
-            const int k = cnt[g];
-            cnt[g] = run;
-            b.groupOff[g] = run;
+        for (int t = 0; t < nt; t++) {
+            const int k = part[t];
+            part[t] = run;
             run += k;
         }
-        b.groupOff[nc] = run;
-        ctl->selfColours[kind] = nc;
     }
     __syncthreads();
-    for (int c = tid; c < n; c += nt) {
-        const int col = b.colour[c];
-        if (col >= 0 && col < nc) b.order[atomicAdd(&cnt[col], 1)] = c;
+    int run = part[tid];
+    for (int g = lo; g < hi; g++) {
+        const int k = b.groupOff[g];
+        b.groupOff[g] = run;
+        b.active[g] = run;  // the scatter's cursor of colour g
+        run += k;
     }
+    if (tid == 0) ctl->selfColours[kind] = nc;
+    __syncthreads();
+    for (int c = tid; c < n; c += nt) b.order[atomicAdd(&b.active[b.colour[c]], 1)] = c;
 }
 
 // --------------------------------------------------------- projections ----
@@ -624,10 +631,19 @@ __device__ float self_ee(float4* pos, SelfBuf b, int c, const SelfParams& sp, fl
         const float dLambda = -0.9f * s * e;
         lambda += dLambda / h;
         const f3 disp = mul(dLambda, n);
-        x1 = sub(x1, mul(omw1 * m1, disp));
-        x2 = sub(x2, mul(w1 * m2, disp));
-        x3 = add(x3, mul(omw2 * m3, disp));
-        x4 = add(x4, mul(w2 * m4, disp));
+        if (sp.eeWeights == 0) {
+            // as the reference writes it: p1 moves by p2's barycentric weight
+            // and p2 by p1's (likewise p3, p4)
+            x1 = sub(x1, mul(omw1 * m1, disp));
+            x2 = sub(x2, mul(w1 * m2, disp));
+            x3 = add(x3, mul(omw2 * m3, disp));
+            x4 = add(x4, mul(w2 * m4, disp));
+        } else {
+            x1 = sub(x1, mul(w1 * m1, disp));
+            x2 = sub(x2, mul(omw1 * m2, disp));
+            x3 = add(x3, mul(w2 * m3, disp));
+            x4 = add(x4, mul(omw2 * m4, disp));
+        }
         pos[id.x] = f4(x1, m1);
         pos[id.y] = f4(x2, m2);
         pos[id.z] = f4(x3, m3);

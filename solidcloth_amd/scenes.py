@@ -84,6 +84,10 @@ class Scene:
     ccd: tuple = None
     # vertices pinned after Init (inverse mass 0); None = Init's own pins
     pins: tuple = None
+    # velocity damping per substep (ClothModel::SetDragCoeff; 1 = none)
+    drag: float = 1.0
+    # self edge-edge displacement weights (scSetSelfEdgeWeights): 0 = the reference's
+    self_edge_weights: int = 0
     # stretch model: FEM triangles (True) or links (False, ClothPBD.cpp:9-50);
     # dihedral bends (True) or bend links; link stretch stiffness (None = the
     # library default)
@@ -127,12 +131,23 @@ def ccd_for_spacing(d):
     return (0.001 * r ** 3, 0.01 * r ** 3, 0.1 * r)
 
 
+# 512x512 perturbed cloth dropped onto the icosphere mesh (subdivision 6,
+# radius 0.4), resting on a ground 5 cm below the sphere. Two choices the
+# workload fixes where the reference gives no value: velocities are damped
+# by 0.98 per substep (ClothModel::SetDragCoeff; the reference's default is
+# no damping), so that the hanging cloth lands on the ground at about 2 m/s
+# instead of 5; and self edge-edge contacts move each edge end by its own
+# barycentric weight (scSetSelfEdgeWeights 1). As the reference writes the
+# projection (ClothCollisionHandler.cpp:531-534) the weights of the two ends
+# are exchanged, and the dense self contact of this drop then collapses
+# patches of the cloth into knots whose contacts grow without bound (the
+# oracle and the GPU agree on that collapse bit for bit).
 MESH512 = Scene(
     name="mesh512", n=512, side=2.0, offset=(0.0, 0.5, 0.0), iterations=10, frames=300,
     flags=CF_VERTICES | CF_EDGES | CF_TRIANGLES | CF_WALLS | CF_SELF, thickness=0.002, tolerance=0.005,
     friction=0.3, walls_min=(-10.0, -0.45, -10.0), walls_max=(10.0, 10.0, 10.0),
     mesh_subdiv=6, mesh_radius=0.4, perturb_amplitude=0.05, perturb_seed=3,
-    ccd=ccd_for_spacing(2.0 / 511),
+    ccd=ccd_for_spacing(2.0 / 511), drag=0.98, self_edge_weights=1,
 )
 
 MESHSMALL = Scene(
@@ -328,6 +343,8 @@ def build_patch(scene, sc=None):
     m.SetThickness(scene.thickness)
     m.SetTolerance(scene.tolerance)
     m.SetFriction(scene.friction)
+    m.SetDragCoeff(scene.drag)
+    m.SetSelfEdgeWeights(scene.self_edge_weights)
     patch.SetNumSteps(scene.substeps)
     patch.Init(scene.n, scene.n, scene.inc, scene.offset, scene.horizontal, scene.attached)
     if scene.perturb_amplitude != 0.0:

@@ -47,6 +47,14 @@ def test_fold_self_collision_bit_exact(sc, oracle_mod, mode):
     assert seen[0] > 0 and seen[1] > 0, seen
 
 
+def test_fold_self_collision_own_edge_weights_bit_exact(sc, oracle_mod):
+    """The same fold with each self edge-edge end moved by its own weight
+    (scSetSelfEdgeWeights 1)."""
+    print("fold24 own edge weights", flush=True)
+    seen, m, o = _run(sc, oracle_mod, S.FOLD24.with_(self_edge_weights=1), 1, 48)
+    assert seen[1] > 0, seen
+
+
 def test_fold_self_collision_reference_early_exit(sc, oracle_mod):
     """The exit test includes the self contact errors against the threshold."""
     scene = S.FOLD24.with_(fixed_iterations=False, iterations=15)
@@ -70,7 +78,8 @@ def test_mesh_obstacle_with_self_collision_mid_size(sc, oracle_mod):
     ground."""
     scene = S.MESH512.with_(n=40, side=1.0, mesh_subdiv=2, mesh_radius=0.25, offset=(0.0, 0.27, 0.0),
                             walls_min=(-10.0, -0.3, -10.0), thickness=0.004, tolerance=0.005, frames=24,
-                            perturb_amplitude=0.02, ccd=S.ccd_for_spacing(1.0 / 39))
+                            perturb_amplitude=0.02, ccd=S.ccd_for_spacing(1.0 / 39), drag=1.0,
+                            self_edge_weights=0)
     print("mesh+self 40", flush=True)
     seen, m, o = _run(sc, oracle_mod, scene, 1, scene.frames * scene.substeps)
     assert sum(seen) > 0, seen

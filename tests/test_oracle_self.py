@@ -172,3 +172,59 @@ def test_touching_edges_use_the_contact_normal(oracle_mod):
     # separated along the contact normal (from edge 1 to edge 2 at t = 0)
     sep = np.dot(m2 - m1, normal)
     assert sep == pytest.approx(0.9 * 0.9 * 0.01, abs=2e-7)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_edge_edge_projection_weights(oracle_mod, mode):
+    """An edge-edge contact whose closest points sit off the edges' middles
+    (t1 = 0.2 on the first edge, t2 = 0.7 on the second), the edges 2 mm
+    apart, same side as at detection, unit masses, target 0.9 x thickness.
+    With s = 1 / (w1^2 + t1^2 + w2^2 + t2^2) (w = 1 - t), one pass moves the
+    points at those parameters apart by 0.9 x gap x S_eff / S_own, where
+    S_own = w1^2 + t1^2 + w2^2 + t2^2 when each end moves by its own weight
+    (mode 1: the full 90 %), and S_eff = 2 w1 t1 + 2 w2 t2 when each end
+    moves by the other end's weight, as ClothCollisionHandler.cpp:531-534
+    writes it (mode 0: 0.9 x 0.74 / 1.26 = 52.9 %). A mistake that swapped the
+    weights in either mode, or applied them to the wrong edge, fails here;
+    the midpoint test above (t = 1/2) cannot tell the modes apart."""
+    h, u, a, b = 0.004, 0.01, 0.003, 0.002
+    prev = np.array([[-0.1, h, 0.05], [0.4, h, 0.05], [0.0, 0.0, -0.35], [0.0, 0.0, 0.15],
+                     [5.0, 1.0, 5.0], [5.2, 1.0, 5.0]])
+    vel = np.array([[0, -u, a], [0, -u, -a], [b, 0, 0], [-b, 0, 0], [0, 0, 0], [0, 0, 0]])
+    c = oracle_mod.Cloth()
+    c.set(thickness=0.01, tolerance=0.01, collFlags=CF_SELF, friction=0.0, fem=1,
+          ccdVolume=1e-14, ccdBisect=1e-16, ccdVtDistance=0.001, selfEdgeWeights=mode)
+    c.init_arrays(prev, prev[:, [0, 2]], np.ones(6), np.array([[0, 1, 4], [2, 3, 5]], dtype=np.int32))
+    c.put("prev", prev)
+    c.put("pos", prev + vel)
+    c.detect()
+    keys, ids, data = c.self_contacts("ee")
+    assert [tuple(i) for i in ids] == [(0, 1, 2, 3)]
+    normal = data[0, 3:6]
+    np.testing.assert_allclose(normal, [0, -1, 0], atol=1e-5)
+    # the solve's configuration: edge 1 2 mm above edge 2, crossing over it
+    # at t1 = 0.2, t2 = 0.7 (exact in binary: -0.1 + 0.2 * 0.5 = 0)
+    gap = 0.002
+    cfg = np.array([[-0.1, gap, 0.0], [0.4, gap, 0.0], [0.0, 0.0, -0.35], [0.0, 0.0, 0.15],
+                    [5.0, 1.0, 5.0], [5.2, 1.0, 5.0]])
+    r, t1, t2, c1, c2 = oracle_mod.closest_pt_segm_segm(*cfg[:4].astype(np.float32))
+    assert t1 == pytest.approx(0.2, abs=1e-6) and t2 == pytest.approx(0.7, abs=1e-6)
+    c.put("pos", cfg)
+    c.solve_self("ee", 1.0 / 240)
+    p1 = c.get("pos")
+    w1, w2 = 1 - t1, 1 - t2
+    p = w1 * p1[0] + t1 * p1[1]
+    q = w2 * p1[2] + t2 * p1[3]
+    new_gap = float(np.dot(p - q, [0, 1, 0]))  # edge 1 above edge 2
+    s_own = w1 ** 2 + t1 ** 2 + w2 ** 2 + t2 ** 2
+    s_eff = s_own if mode == 1 else 2 * w1 * t1 + 2 * w2 * t2
+    target = 0.9 * 0.01
+    want = gap + 0.9 * (target - gap) * s_eff / s_own
+    assert new_gap == pytest.approx(want, abs=2e-7)
+    # the other edge ends: in mode 0 the far end (x2, weight t1 = 0.2) takes
+    # the larger share of the first edge's displacement
+    d1, d2 = p1[0, 1] - gap, p1[1, 1] - gap
+    if mode == 0:
+        assert d2 / d1 == pytest.approx(w1 / t1, rel=1e-5)
+    else:
+        assert d1 / d2 == pytest.approx(w1 / t1, rel=1e-5)
