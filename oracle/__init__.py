@@ -43,6 +43,8 @@ def lib():
         L.oc_add_grid_sdf.argtypes = [vp, _C.c_int, _C.c_int, _C.c_int, vp, vp]
         L.oc_sdf_query.argtypes = [vp, _C.c_int, _C.c_float, _C.c_float, _C.c_float, vp]
         L.oc_get_sdf_grads.argtypes = [vp, _C.c_int, vp]
+        L.oc_sdf_node_count.argtypes = [vp, _C.c_int]
+        L.oc_mesh_normals_at.argtypes = [vp, _C.c_int, vp, vp, _C.c_int, vp, vp, _C.c_int, vp, vp, vp]
         L.oc_set_order.argtypes = [vp, _C.c_int, vp, _C.c_int]
         L.oc_get_counts.argtypes = [vp, vp]
         L.oc_get_triangles.argtypes = [vp, vp, vp]
@@ -224,6 +226,29 @@ class Cloth:
         box = _np.array(list(bmin) + list(bmax), dtype=_np.float32)
         v = _np.ascontiguousarray(values, dtype=_np.float32)
         return self.L.oc_add_grid_sdf(self.h, n[0], n[1], n[2], _ptr(box), _ptr(v))
+
+    def mesh_normals_at(self, mesh_index, tris, coords, edges, params):
+        """Obstacle mesh `mesh_index` (order of add_mesh): interpolated triangle
+        normals at (tris, coords) and edge normals at (edges, params), both
+        (n, 3) float32, and the mesh's edges (i1, i2)."""
+        t = _np.ascontiguousarray(tris, dtype=_np.int32)
+        co = _np.ascontiguousarray(coords, dtype=_np.float32)
+        e = _np.ascontiguousarray(edges, dtype=_np.int32)
+        pa = _np.ascontiguousarray(params, dtype=_np.float32)
+        tn = _np.zeros((len(t), 3), dtype=_np.float32)
+        en = _np.zeros((len(e), 3), dtype=_np.float32)
+        ne = self.L.oc_mesh_normals_at(self.h, mesh_index, _ptr(t), _ptr(co), len(t), _ptr(e), _ptr(pa), len(e),
+                                       _ptr(tn), _ptr(en), None)
+        el = _np.zeros((ne, 2), dtype=_np.int32)
+        self.L.oc_mesh_normals_at(self.h, mesh_index, None, None, 0, None, None, 0, None, None, _ptr(el))
+        return tn, en, el
+
+    def sdf_grads(self, sdf_index):
+        """The stored node gradients of a grid SDF, shape (nodes, 3), x fastest."""
+        n = self.L.oc_sdf_node_count(self.h, sdf_index)
+        out = _np.zeros((n, 3))
+        self.L.oc_get_sdf_grads(self.h, sdf_index, _ptr(out))
+        return out
 
     def sdf_query(self, sdf_index, p):
         out = _np.zeros(10)

@@ -19,5 +19,6 @@ fi
 mkdir -p "$OUT"
 FLAGS="-std=c++17 -O2 -fPIC -ffp-contract=off -DLINUX -DDISABLE_PROFILER -include cfloat -include cmath -include cstring -I$REF"
 g++ $FLAGS -shared -o "$OUT/libref_geometry.so" \
-    "$REF/Geometry/Collision3D.cpp" "$REF/Geometry/Mesh.cpp" "$HERE/ref_harness.cpp"
+    "$REF/Geometry/Collision3D.cpp" "$REF/Geometry/Mesh.cpp" "$REF/Geometry/AabbTree.cpp" \
+    "$REF/Geometry/ClosestPointOnMesh.cpp" "$REF/Geometry/VoronoiRegions.cpp" "$HERE/ref_harness.cpp"
 echo "build_ref: built $OUT/libref_geometry.so"

@@ -786,6 +786,28 @@ public:
     // tree walk that keeps the first strictly nearer candidate in its walk
     // order; here the search is exhaustive and ties at equal FP32 distance go
     // to the lowest feature index. Distance 0 is skipped as in the reference.
+    // ComputeTriangleNormal (ClosestPointOnMesh.cpp:37-62) as the contact
+    // solve calls it (ClothCollisionHandler.cpp:159: no pseudo-normal, mesh
+    // normals): the vertex normals interpolated with the barycentric
+    // coordinates, normalised
+    V3<R> mesh_triangle_normal(const ObstacleMesh& M, int tri, V3<R> co) const
+    {
+        const uint32_t i0 = M.idx[3 * tri], i1 = M.idx[3 * tri + 1], i2 = M.idx[3 * tri + 2];
+        V3<R> nn = co.x * vR(M.vn[i0]) + co.y * vR(M.vn[i1]) + co.z * vR(M.vn[i2]);
+        normalize(nn);
+        return nn;
+    }
+
+    // ComputeEdgeNormal (ClosestPointOnMesh.cpp:483-497): the edge's
+    // pseudo-normal, or an end's vertex normal at parameter exactly 0 or 1
+    V3<R> mesh_edge_normal(const ObstacleMesh& M, int e, R pm) const
+    {
+        V3<float> nf = M.en[e];
+        if (pm == R(0)) nf = M.vn[M.edges[e].i1];
+        if (pm == R(1)) nf = M.vn[M.edges[e].i2];
+        return vR(nf);
+    }
+
     void mesh_collisions(const ObstacleMesh& M, int ci, int mi)
     {
         const size_t n = pos.size();
@@ -911,11 +933,7 @@ public:
                         edge = (int)e;
                         bcp = cm;
                         bws = ps;
-                        // ComputeEdgeNormal (ClosestPointOnMesh.cpp:483-497)
-                        V3<float> nf = M.en[e];
-                        if (pm == R(0)) nf = M.vn[M.edges[e].i1];
-                        if (pm == R(1)) nf = M.vn[M.edges[e].i2];
-                        bn = vR(nf);
+                        bn = mesh_edge_normal(M, (int)e, pm);
                     }
                 }
                 if (dist < radTol && dist >= R(0)) {
@@ -1175,9 +1193,7 @@ public:
                 V3<R> cp, co;
                 closest_pt_point_triangle(p, vR(M.v[i0]), vR(M.v[i1]), vR(M.v[i2]), cp, co);
                 c.point = cp;
-                V3<R> nn = co.x * vR(M.vn[i0]) + co.y * vR(M.vn[i1]) + co.z * vR(M.vn[i2]);
-                normalize(nn);
-                c.normal = nn;
+                c.normal = mesh_triangle_normal(M, (int)c.feature, co);
             }
             V3<R> delta = p - c.point;
             R len = dot(delta, c.normal);
@@ -2009,6 +2025,13 @@ void oc_sdf_query(void* h, int sdf, float px, float py, float pz, double* out)
     });
 }
 
+int oc_sdf_node_count(void* h, int sdf)
+{
+    int n = 0;
+    with(h, [&](auto& c) { n = (int)c.sdfs[sdf].grads.size(); });
+    return n;
+}
+
 void oc_get_sdf_grads(void* h, int sdf, double* out)
 {
     with(h, [&](auto& c) {
@@ -2246,6 +2269,33 @@ int oc_add_mesh(void* h, int nv, const float* verts, int ni, const unsigned* idx
         col.mesh = (int)c.meshes.size() - 1;
         c.world.push_back(col);
         return (int)c.world.size() - 1;
+    });
+}
+
+// The obstacle mesh `mesh`'s interpolated triangle normals at (tris[k],
+// coords[3k..3k+2]) and edge normals at (edges[k], params[k]); edgesOut
+// receives the mesh's edges (i1, i2) when not null. Returns the edge count.
+int oc_mesh_normals_at(void* h, int mesh, const int* tris, const float* coords, int nt, const int* edges,
+                       const float* params, int ne, float* triOut, float* edgeOut, int* edgesOut)
+{
+    return with(h, [&](auto& c) {
+        using R = typename std::remove_reference<decltype(c.pos[0].x)>::type;
+        const ObstacleMesh& M = c.meshes[mesh];
+        for (int k = 0; k < nt; k++) {
+            const V3<R> n = c.mesh_triangle_normal(M, tris[k], V3<R>{R(coords[3 * k]), R(coords[3 * k + 1]),
+                                                                      R(coords[3 * k + 2])});
+            triOut[3 * k] = (float)n.x; triOut[3 * k + 1] = (float)n.y; triOut[3 * k + 2] = (float)n.z;
+        }
+        for (int k = 0; k < ne; k++) {
+            const V3<R> n = c.mesh_edge_normal(M, edges[k], R(params[k]));
+            edgeOut[3 * k] = (float)n.x; edgeOut[3 * k + 1] = (float)n.y; edgeOut[3 * k + 2] = (float)n.z;
+        }
+        if (edgesOut)
+            for (size_t e = 0; e < M.edges.size(); e++) {
+                edgesOut[2 * e] = (int)M.edges[e].i1;
+                edgesOut[2 * e + 1] = (int)M.edges[e].i2;
+            }
+        return (int)M.edges.size();
     });
 }
 

@@ -4,10 +4,12 @@
 // tests/golden/make_golden.py, in a container that has the reference, to
 // write golden vectors; nothing on a GPU host loads it.
 //
-// Only Geometry/{Mesh,Collision3D}.cpp are compiled: they build with the
-// image's own headers. The Physics translation units do not (they need Win32
+// Only Geometry/{Mesh,Collision3D,AabbTree,ClosestPointOnMesh,VoronoiRegions}.cpp are
+// compiled: they build with the image's own headers. The Physics translation units do not (they need Win32
 // types from Engine/Engine.h and std::floorf, which libstdc++ 11 lacks), so
 // the solver itself is pinned by the oracle's mathematical tests instead.
+#include "Geometry/AabbTree.h"
+#include "Geometry/ClosestPointOnMesh.h"
 #include "Geometry/Collision3D.h"
 #include "Geometry/Mesh.h"
 
@@ -113,6 +115,67 @@ void ref_barycentric(const float* a, const float* b, const float* c, const float
     Vector3 r = Geometry::Barycentric(Vector3(a[0], a[1], a[2]), Vector3(b[0], b[1], b[2]), Vector3(c[0], c[1], c[2]),
                                       Vector3(p[0], p[1], p[2]));
     out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+// An obstacle mesh as the reference's demo prepares one
+// (Demo/ClothDemo.cpp:100-103).
+static void make_mesh(Geometry::Mesh& m, const float* verts, int nv, const unsigned* idx, int ni)
+{
+    m.vertices.resize(nv);
+    for (int i = 0; i < nv; i++) m.vertices[i] = Vector3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
+    m.indices.assign(idx, idx + ni);
+    m.ComputeNormals();
+    m.ConstructEdges();
+    m.ConstructEdgeOneRings();
+}
+
+// ClosestPointOnMeshToPointAcc (Geometry/ClosestPointOnMesh.cpp:304-319)
+// over the tree CollisionMesh::Update builds (Physics/Common.h:124-137:
+// triangles, vertices and edges, 20 levels, tolerance tol). out per query:
+// triangle, closest point (3), normal (3), distance, region.
+void ref_closest_point_on_mesh_acc(const float* verts, int nv, const unsigned* idx, int ni, float tol,
+                                   const float* q, int nq, float* out)
+{
+    Geometry::Mesh m;
+    make_mesh(m, verts, nv, idx, ni);
+    Geometry::AabbTree* tree = Geometry::ComputeMeshTree(
+        m, Geometry::ATF_TRIANGLES | Geometry::ATF_VERTICES | Geometry::ATF_EDGES, 20, tol);
+    for (int k = 0; k < nq; k++) {
+        Geometry::ClosestTriangleToPoint r =
+            Geometry::ClosestPointOnMeshToPointAcc(Vector3(q[3 * k], q[3 * k + 1], q[3 * k + 2]), m, tree);
+        float o[9] = {(float)r.tri, r.closestPtMesh.x, r.closestPtMesh.y, r.closestPtMesh.z, r.normal.x,
+                      r.normal.y, r.normal.z, r.distance, (float)r.region};
+        memcpy(out + 9 * k, o, sizeof(o));
+    }
+    delete tree;
+}
+
+// ComputeTriangleNormal (ClosestPointOnMesh.cpp:37-97) at (triangle, coords)
+void ref_triangle_normals(const float* verts, int nv, const unsigned* idx, int ni, const int* tris,
+                          const float* coords, int n, int usePseudoNormal, int useMeshNormal, float* out)
+{
+    Geometry::Mesh m;
+    make_mesh(m, verts, nv, idx, ni);
+    for (int k = 0; k < n; k++) {
+        Vector3 nn = Geometry::ComputeTriangleNormal(m, tris[k], Vector3(coords[3 * k], coords[3 * k + 1],
+                                                                          coords[3 * k + 2]),
+                                                     usePseudoNormal != 0, useMeshNormal != 0);
+        out[3 * k] = nn.x; out[3 * k + 1] = nn.y; out[3 * k + 2] = nn.z;
+    }
+}
+
+// ComputeEdgeNormal (ClosestPointOnMesh.cpp:483-497) of edge e at parameter
+// params[k]; edges are the mesh's after ConstructEdges. Returns the edge count.
+int ref_edge_normals(const float* verts, int nv, const unsigned* idx, int ni, const int* edges, const float* params,
+                     int n, float* out)
+{
+    Geometry::Mesh m;
+    make_mesh(m, verts, nv, idx, ni);
+    for (int k = 0; k < n; k++) {
+        Vector3 nn = Geometry::ComputeEdgeNormal(m, edges[k], params[k]);
+        out[3 * k] = nn.x; out[3 * k + 1] = nn.y; out[3 * k + 2] = nn.z;
+    }
+    return (int)m.edges.size();
 }
 
 }  // extern "C"

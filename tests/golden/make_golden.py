@@ -1,7 +1,8 @@
 """Writes the golden vectors under tests/golden/ from the reference's own
-Geometry code (Geometry/Mesh.cpp, Geometry/Collision3D.cpp), compiled from
-/root/reference by oracle/build_ref.sh into oracle/_ref/. Run in a container
-that has the reference:  python tests/golden/make_golden.py
+Geometry code (Geometry/Mesh.cpp, Collision3D.cpp, AabbTree.cpp,
+ClosestPointOnMesh.cpp), compiled from /root/reference by
+oracle/build_ref.sh into oracle/_ref/. Run in a container that has the
+reference:  python tests/golden/make_golden.py
 
 The fixtures are data only (inputs and the reference's outputs).
 """
@@ -142,10 +143,64 @@ def segm_triangle(L):
     print("wrote ref_segm_triangle.npz")
 
 
+def mesh_queries(L):
+    """ClosestPointOnMeshToPointAcc over the obstacle tree, and the solver's
+    ComputeTriangleNormal / ComputeEdgeNormal, on an icosphere (subdivision
+    2, radius 0.25). Vertex queries lie outside the sphere at up to 20 % of
+    its radius; only those whose nearest triangle is unique by a margin are
+    kept (the reference breaks ties by its tree order), found by brute force
+    with the reference's own ClosestPtPointTriangle."""
+    from solidcloth_amd import scenes as S
+
+    v, t = S.icosphere(2, 0.25)
+    v = np.ascontiguousarray(v, dtype=np.float32)
+    t = np.ascontiguousarray(t, dtype=np.uint32)
+    rng = np.random.default_rng(17)
+    d = rng.normal(size=(400, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    Q = (d * 0.25 * (1 + rng.uniform(0.005, 0.2, (400, 1)))).astype(np.float32)
+    tol = np.float32(0.002)
+    o = np.zeros(6, dtype=np.float32)
+    keep = []
+    for k, q in enumerate(Q):
+        dist = np.empty(len(t))
+        for j, (a, b, c) in enumerate(t):
+            L.ref_closest_pt_point_triangle(p(q), p(v[a]), p(v[b]), p(v[c]), p(o))
+            dist[j] = np.linalg.norm(q.astype(np.float64) - o[:3].astype(np.float64))
+        two = np.sort(dist)[:2]
+        if two[1] - two[0] > 1e-4 * two[0]:
+            keep.append(k)
+    Q = np.ascontiguousarray(Q[keep])
+    out = np.zeros((len(Q), 9), dtype=np.float32)
+    L.ref_closest_point_on_mesh_acc(p(v), len(v), p(t), t.size, C.c_float(tol), p(Q), len(Q), p(out))
+    # triangle normals at barycentric points: interior, on an edge, at a corner
+    nt = 300
+    tris = rng.integers(0, len(t), nt).astype(np.int32)
+    co = rng.dirichlet([1, 1, 1], nt).astype(np.float32)
+    co[100:200, rng.integers(0, 3)] = 0.0
+    co[200:] = np.eye(3, dtype=np.float32)[rng.integers(0, 3, 100)]
+    tn = np.zeros((nt, 3), dtype=np.float32)
+    L.ref_triangle_normals(p(v), len(v), p(t), t.size, p(tris), p(co), nt, 0, 1, p(tn))
+    # edge normals at parameters 0, 1 (an end's vertex normal) and in between
+    edges = np.zeros(1, dtype=np.int32)
+    ne = L.ref_edge_normals(p(v), len(v), p(t), t.size, p(edges), p(np.zeros(1, np.float32)), 0, p(np.zeros(3, np.float32)))
+    edges = np.repeat(np.arange(ne, dtype=np.int32), 3)
+    params = np.tile(np.array([0.0, 1.0, 0.37], dtype=np.float32), ne)
+    en = np.zeros((len(edges), 3), dtype=np.float32)
+    L.ref_edge_normals(p(v), len(v), p(t), t.size, p(edges), p(params), len(edges), p(en))
+    np.savez_compressed(os.path.join(HERE, "ref_mesh_queries.npz"), vertices=v, indices=t, tol=tol, queries=Q,
+                        query_out=out, tri=tris, coords=co, tri_normals=tn, edges=edges, params=params,
+                        edge_normals=en)
+    print(f"wrote ref_mesh_queries.npz ({len(Q)} tie-free queries of 400)")
+
+
 if __name__ == "__main__":
-    # no argument: every fixture; "segm_triangle": only ref_segm_triangle.npz
+    # no argument: every fixture; a name: only that fixture
     if sys.argv[1:] == ["segm_triangle"]:
         segm_triangle(ref_lib())
+    elif sys.argv[1:] == ["mesh_queries"]:
+        mesh_queries(ref_lib())
     else:
         main()
         segm_triangle(ref_lib())
+        mesh_queries(ref_lib())

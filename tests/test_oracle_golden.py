@@ -1,13 +1,15 @@
 """The oracle against golden vectors written by the reference's own Geometry
-code (tests/golden/make_golden.py compiles Geometry/Mesh.cpp and
-Geometry/Collision3D.cpp from the reference sources)."""
+code (tests/golden/make_golden.py compiles Geometry/Mesh.cpp, Collision3D.cpp,
+AabbTree.cpp, ClosestPointOnMesh.cpp and VoronoiRegions.cpp from the
+reference sources)."""
 import os
 
 import numpy as np
 import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-MESHES = np.load(os.path.join(HERE, "golden", "ref_meshes.npz"))
+GOLDEN = os.path.join(HERE, "golden")
+MESHES = np.load(os.path.join(GOLDEN, "ref_meshes.npz"))
 
 
 @pytest.mark.parametrize("name", ["grid32", "grid16", "ico1"])
@@ -90,3 +92,39 @@ def test_bends_from_reference_edges(oracle_mod):
         exp.append((i1, i2, o1, o2))
     assert np.array_equal(c.bends(), np.array(exp, dtype=np.int32))
     assert len(exp) == 2821
+
+
+def test_mesh_queries_match_the_compiled_reference(oracle_mod):
+    """Vertex -> nearest obstacle triangle (ClosestPointOnMeshToPointAcc over
+    the reference's own AabbTree, Geometry/ClosestPointOnMesh.cpp:304-319 and
+    AabbTree.cpp), the solver's interpolated triangle normal
+    (ComputeTriangleNormal, :37-62, as ClothCollisionHandler.cpp:159 calls it)
+    and the edge normal (ComputeEdgeNormal, :483-497), against the compiled
+    reference (tests/golden/ref_mesh_queries.npz; tie-free queries only). The
+    oracle answers the vertex queries through its contact detection (query
+    from prev, normal = delta / distance): triangle, closest point and normal
+    must be the reference's bit for bit."""
+    g = np.load(os.path.join(GOLDEN, "ref_mesh_queries.npz"))
+    v, t, Q, ref = g["vertices"], g["indices"], g["queries"], g["query_out"]
+    n = len(Q)
+    pos = np.vstack([Q.astype(np.float64), [[3, 3, 3], [3.1, 3, 3], [3, 3, 3.1]]])
+    c = oracle_mod.Cloth()
+    c.set(thickness=1e-4, tolerance=1.0, collFlags=1, fem=1)
+    c.init_arrays(pos, pos[:, [0, 2]], np.ones(len(pos)), [[n, n + 1, n + 2]])
+    c.add_mesh(v, t)
+    c.put("prev", pos)
+    c.detect()
+    ids, data = c.contacts()
+    feats = c.contact_features()
+    got = {int(i): (int(f), d) for i, f, d in zip(ids[:, 0], feats[:, 1], data)}
+    assert sorted(got) == list(range(n))
+    for k in range(n):
+        tri, d = got[k]
+        assert tri == int(ref[k, 0]), k
+        assert np.array_equal(d[0:3].astype(np.float32), ref[k, 1:4]), k
+        assert np.array_equal(d[3:6].astype(np.float32), ref[k, 4:7]), k
+    tn, en, el = c.mesh_normals_at(0, g["tri"], g["coords"], g["edges"], g["params"])
+    assert np.array_equal(tn, g["tri_normals"])
+    assert np.array_equal(en, g["edge_normals"])
+    # parameters exactly 0 and 1 took the ends' vertex normals
+    assert not np.array_equal(en[0::3], en[2::3])
