@@ -322,6 +322,14 @@ SCEXP void scSetSelfCapacity(scModel model, int capacity);
  * cloth. 1: each end moves by its own weight (x1 by 1 - t1, x2 by t1, ...).
  * No counterpart in the reference (an extension). */
 SCEXP void scSetSelfEdgeWeights(scModel model, int mode);
+/* scGetSelfCandidates: the broad-phase candidate pairs of the last substep
+ * (SelfCollisionDetector.cpp:60-135, Midphase, as one set: every pair whose
+ * swept boxes overlap as closed intervals, Geometry/Aabb3.h:72-87, after the
+ * filters of :92 and :122), kind 0 = (triangle, vertex), 1 = (edge1, edge2)
+ * with edge1 < edge2, sorted, each pair once (the reference's tree may list
+ * a pair twice). Writes at most cap pairs (2 ints each) and returns the
+ * count. No counterpart in the reference's API. */
+SCEXP long long scGetSelfCandidates(scModel model, int kind, int* pairs, long long cap);
 SCEXP int scGetSelfStats(scModel model, int* out, int count);
 
 /* Solver statistics of the last substep (Physics/ClothModel.h:32-40):

@@ -45,6 +45,8 @@ def lib():
         L.oc_get_sdf_grads.argtypes = [vp, _C.c_int, vp]
         L.oc_sdf_node_count.argtypes = [vp, _C.c_int]
         L.oc_mesh_normals_at.argtypes = [vp, _C.c_int, vp, vp, _C.c_int, vp, vp, _C.c_int, vp, vp, vp]
+        L.oc_get_self_candidates.argtypes = [vp, _C.c_int, vp, _C.c_long]
+        L.oc_get_self_candidates.restype = _C.c_long
         L.oc_set_order.argtypes = [vp, _C.c_int, vp, _C.c_int]
         L.oc_get_counts.argtypes = [vp, vp]
         L.oc_get_triangles.argtypes = [vp, vp, vp]
@@ -125,7 +127,7 @@ class Cloth:
     FLOAT_PARAMS = ("unit", "thickness", "tolerance", "shear", "bend", "stretch", "poisson", "friction",
                     "area", "drag", "selfShrink", "selfThreshold", "ccdVolume", "ccdBisect", "ccdVtDistance")
     INT_PARAMS = ("iterations", "collFlags", "bendFreq", "numSteps", "dihedral", "fem", "fixedIterations", "accel", "solver",
-                  "selfEdgeWeights")
+                  "selfEdgeWeights", "recordCandidates")
 
     def __init__(self, real64=False):
         self.L = lib()
@@ -338,6 +340,16 @@ class Cloth:
         contacts as their keys."""
         k = _np.ascontiguousarray(keys, dtype=_np.uint64)
         self.L.oc_set_self_order(self.h, 0 if kind == "vt" else 1, _ptr(k), len(k))
+
+    def self_candidates(self, kind):
+        """The last detection's self candidate pairs ('vt': (triangle, vertex),
+        'ee': (edge1, edge2)) as an (n, 2) array in enumeration order; needs
+        set(recordCandidates=1) before the detection."""
+        k = 0 if kind == "vt" else 1
+        n = self.L.oc_get_self_candidates(self.h, k, None, 0)
+        out = _np.zeros((n, 2), dtype=_np.int32)
+        self.L.oc_get_self_candidates(self.h, k, _ptr(out), n)
+        return out
 
     def self_contacts(self, kind):
         """kind 'vt' / 'ee' -> keys (n,), ids (n, 4), data (n, 8): w1 w2 w3,

@@ -46,6 +46,8 @@ struct Params {
     // bisection stop, vertex-triangle distance
     float ccdVolume = 0.001f, ccdBisect = 0.01f, ccdVtDistance = 0.1f;
     int iterations = 50, collFlags = 0, bendFreq = 2, numSteps = 1;
+    // 1: keep the last detection's self candidate pairs (selfCandVT / EE)
+    int recordCandidates = 0;
     // self edge-edge displacement weights: 0 as ClothCollisionHandler.cpp:
     // 531-534 writes them (the first edge's ends moved by each other's
     // barycentric weight, likewise the second's), 1 each end moved by its own
@@ -308,6 +310,7 @@ public:
     std::vector<Collidable<R>> world;
     std::vector<GridSDF<R>> sdfs;
     std::vector<ObstacleMesh> meshes;
+    std::vector<std::pair<int, int>> selfCandVT, selfCandEE;  // (triangle, vertex), (edge1, edge2)
     std::vector<EdgeContact<R>> edgeContacts;
     std::vector<TriContact<R>> triContacts;
     std::vector<int> triOrder, bendOrder, linkOrder;
@@ -1452,11 +1455,14 @@ public:
                 for (size_t i = first; i < count; i++) f(i);
             }
         };
+        selfCandVT.clear();
+        selfCandEE.clear();
         for (size_t t = 0; t < nt; t++) {
             const uint32_t j1 = tris[t].i1, j2 = tris[t].i2, j3 = tris[t].i3;
             each(vg, tb[t], nv, 0, [&](size_t i) {
                 if (i == j1 || i == j2 || i == j3 || invMass[i] == R(0)) return;
                 if (!overlap(tb[t], vb[i])) return;
+                if (prm.recordCandidates) selfCandVT.push_back({(int)t, (int)i});
                 SelfContact<R> c;
                 if (vertex_triangle(j1, j2, j3, (uint32_t)i, c)) {
                     c.key = (uint64_t(t) << 32) | uint64_t(i);
@@ -1469,6 +1475,7 @@ public:
                 const uint32_t i1 = edges[e1].i1, i2 = edges[e1].i2, i3 = edges[e2].i1, i4 = edges[e2].i2;
                 if (i1 == i3 || i2 == i4 || i1 == i4 || i2 == i3) return;
                 if (!overlap(eb[e1], eb[e2])) return;
+                if (prm.recordCandidates) selfCandEE.push_back({(int)e1, (int)e2});
                 SelfContact<R> c;
                 if (edge_edge(i1, i2, i3, i4, c)) {
                     c.key = (uint64_t(e1) << 32) | uint64_t(e2);
@@ -1894,6 +1901,7 @@ int oc_set_param_i(void* h, const char* name, int v)
         else if (n == "solver") p.solver = v;
         else if (n == "fixedIterations") p.fixedIterations = v != 0;
         else if (n == "selfEdgeWeights") p.selfEdgeWeights = v;
+        else if (n == "recordCandidates") p.recordCandidates = v;
         else return -1;
         return 0;
     });
@@ -2297,6 +2305,23 @@ int oc_mesh_normals_at(void* h, int mesh, const int* tris, const float* coords, 
             }
         return (int)M.edges.size();
     });
+}
+
+// The last self detection's candidate pairs of one kind (0: (triangle,
+// vertex), 1: (edge1, edge2)) in enumeration order; needs recordCandidates.
+// Returns the count, writing at most cap pairs.
+long oc_get_self_candidates(void* h, int kind, int* out, long cap)
+{
+    long n = 0;
+    with(h, [&](auto& c) {
+        const auto& v = kind == 0 ? c.selfCandVT : c.selfCandEE;
+        n = (long)v.size();
+        for (long k = 0; k < n && k < cap; k++) {
+            out[2 * k] = v[k].first;
+            out[2 * k + 1] = v[k].second;
+        }
+    });
+    return n;
 }
 
 // Physics/ClothPatch.cpp:43-133; uv (2 nv floats) and colorsX (nv red

@@ -115,7 +115,7 @@ def _sig(L):
         "scLbvhSelfCheck": (i, [i, C.c_uint, i, vp]),
         "scGetSelfContacts": (i, [H, i, vp, vp, vp, i]), "scSetSelfParams": (None, [H, f, f]),
         "scSetSelfCCD": (None, [H, f, f, f]), "scSetSelfCapacity": (None, [H, i]),
-        "scSetSelfEdgeWeights": (None, [H, i]),
+        "scSetSelfEdgeWeights": (None, [H, i]), "scGetSelfCandidates": (C.c_longlong, [H, i, vp, C.c_longlong]),
         "scGetSelfStats": (i, [H, vp, i]),
         "scSetParticleInvMasses": (i, [H, vp, i]),
         "scCreateWarmStart": (H, [C.c_uint]), "scFreeWarmStart": (None, [H]),
@@ -574,6 +574,16 @@ class ClothModel:
 
     def SetSelfCCD(self, volume, bisect, vt_distance):
         _checked("scSetSelfCCD", self.h, float(volume), float(bisect), float(vt_distance))
+
+    def self_candidates(self, kind):
+        """The last substep's broad-phase candidate pairs, 'vt' (triangle,
+        vertex) or 'ee' (edge1, edge2), as a sorted (n, 2) int32 array."""
+        k = 0 if kind == "vt" else 1
+        n = _checked("scGetSelfCandidates", self.h, k, None, 0)
+        out = np.zeros((n, 2), dtype=np.int32)
+        if n:
+            _checked("scGetSelfCandidates", self.h, k, _ptr(out), n)
+        return out
 
     def SetSelfEdgeWeights(self, mode):
         """0: the reference's edge-edge displacement weights; 1: each end's own."""

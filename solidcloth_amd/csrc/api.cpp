@@ -2,8 +2,8 @@
 // reference's Demo/ClothAPI.cpp (cited per function) over the MI355X-native
 // objects of model.h. Exceptions never cross the boundary: they become the
 // thread's last error and the call returns a neutral value.
-#include <climits>
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1183,6 +1183,19 @@ void scSetSelfCapacity(scModel model, int capacity)
     guard([&] {
         if (capacity < 0) throw std::runtime_error("negative self-contact capacity");
         M(model)->mSelfCapacity = capacity;
+    });
+}
+
+long long scGetSelfCandidates(scModel model, int kind, int* pairs, long long cap)
+{
+    return guard_r<long long>(-1, [&] {
+        if (kind != 0 && kind != 1) throw std::runtime_error("candidate kind must be 0 (vertex-triangle) or 1 (edge-edge)");
+        const auto v = M(model)->self_candidates(kind);
+        for (long long k = 0; k < (long long)v.size() && k < cap; k++) {
+            pairs[2 * k] = v[k].first;
+            pairs[2 * k + 1] = v[k].second;
+        }
+        return (long long)v.size();
     });
 }
 

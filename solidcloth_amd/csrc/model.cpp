@@ -1,10 +1,10 @@
 // Host implementation: cloth construction, constraint colouring, device
 // buffers, substep recording and hipGraph replay.
-#include <climits>
 #include "model.h"
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -1308,6 +1308,37 @@ bool ClothModel::self_step_overflowed(int attempt)
             std::to_string(attempt + 1) + " attempts");
     }
     return true;
+}
+
+// The last substep's broad-phase candidate pairs of one kind (0: (triangle,
+// vertex), 1: (edge1, edge2)): every query's own slots, then the shared
+// pool, sorted. Empty before the first self-colliding step.
+std::vector<std::pair<int, int>> ClothModel::self_candidates(int kind)
+{
+    std::vector<std::pair<int, int>> out;
+    GpuSolver& g = gpu;
+    GpuSolver::Self& S = g.self;
+    if (!(mCollFlags & CF_SELF) || !S.candVT || !g.d_ctl) return out;
+    HIP_OK(hipStreamSynchronize(g.stream));
+    SolverCtl c;
+    HIP_OK(hipMemcpy(&c, g.d_ctl, sizeof(c), hipMemcpyDeviceToHost));
+    const int queries = kind == 0 ? g.nv : g.ne;
+    const int K = kind == 0 ? S.candKVT : S.candKEE;
+    std::vector<int> cnt(queries);
+    std::vector<int2> slots((size_t)queries * K);
+    HIP_OK(hipMemcpy(cnt.data(), kind == 0 ? S.candCntVT : S.candCntEE, queries * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(slots.data(), kind == 0 ? S.candVT : S.candEE, slots.size() * sizeof(int2),
+                     hipMemcpyDeviceToHost));
+    const int extra = std::min(c.candExtra[kind], S.candExtraCap);
+    std::vector<int2> pool(std::max(extra, 0));
+    if (extra > 0)
+        HIP_OK(hipMemcpy(pool.data(), kind == 0 ? S.candExtraVT : S.candExtraEE, extra * sizeof(int2),
+                         hipMemcpyDeviceToHost));
+    for (int j = 0; j < queries; j++)
+        for (int k = 0; k < std::min(cnt[j], K); k++) out.push_back({slots[(size_t)j * K + k].x, slots[(size_t)j * K + k].y});
+    for (const int2& p : pool) out.push_back({p.x, p.y});
+    std::sort(out.begin(), out.end());
+    return out;
 }
 
 void ClothModel::fetch_stats()

@@ -384,6 +384,7 @@ public:
     void step_substeps(float h, int n);
     void download_contacts();
     void fetch_stats();
+    std::vector<std::pair<int, int>> self_candidates(int kind);
     void ensure_colouring();
     void set_sweep(int mode, int slabSize);
 

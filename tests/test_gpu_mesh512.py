@@ -1,7 +1,8 @@
 """The 512x512 self-colliding cloth on the icosphere mesh at full size
 (262,144 vertices, 522,242 triangles; obstacle 40,962 vertices, 81,920
 triangles): one substep from the GPU's own state after 12 frames against
-the oracle bit for bit (contact sets of every kind and positions), and the
+the oracle bit for bit (broad-phase candidate pairs against the oracle's
+grid broad phase, contact sets of every kind, and positions), and the
 whole 300-frame drop completing with finite positions, every step taken,
 nothing below the ground."""
 import time
@@ -31,7 +32,7 @@ def test_mesh512_full_size_substep_parity(sc, oracle_mod):
     o = oracle_mod.from_scene(SCENE, tri_order=m.constraint_order("tri"), bend_order=m.constraint_order("bend"))
     o.set_contact_order("edge", m.constraint_order("edge_contact"))
     o.set_contact_order("tri", m.constraint_order("tri_contact"))
-    o.set(accel=1, numSteps=1)
+    o.set(accel=1, numSteps=1, recordCandidates=1)
     p = m.positions().astype(np.float64)
     o.put("pos", p)
     o.put("prev", p)
@@ -41,6 +42,10 @@ def test_mesh512_full_size_substep_parity(sc, oracle_mod):
     patch.Step(h)
     t0 = time.perf_counter()
     o.substep_detect(h)
+    from test_gpu_self import _same_candidates
+
+    n = _same_candidates(m, o, "mesh512 substep")
+    print(f"mesh512: {n} candidate pairs equal as sets", flush=True)
     for kind in ("vt", "ee"):
         ok = o.self_contacts(kind)[0]
         gk = m.self_contacts(kind)[0]

@@ -1,9 +1,11 @@
 """GPU parity of continuous self collision (LBVH refit + overlap walks, FP64
 CCD cubic, Jones-Plassmann colouring of the contacts, coloured projections)
 against the CPU oracle's exhaustive candidate enumeration, substep by
-substep: the GPU solves its contacts colour group by colour group and hands
-that order to the oracle, which must then produce the same contacts (keys,
-ids, weights, normals, sides, lambdas) and the same positions bit for bit."""
+substep: the broad-phase candidate pairs must be the oracle's closed-interval
+overlap set exactly; the GPU solves its contacts colour group by colour group
+and hands that order to the oracle, which must then produce the same contacts
+(keys, ids, weights, normals, sides, lambdas) and the same positions bit for
+bit."""
 import numpy as np
 import pytest
 
@@ -18,14 +20,31 @@ def _rows(keys, ids, data):
     return sorted((int(k), tuple(int(x) for x in i), tuple(np.float32(d).tolist())) for k, i, d in zip(keys, ids, data))
 
 
+def _same_candidates(m, o, what):
+    """The GPU's broad-phase candidate pairs (LBVH overlap walks) equal the
+    oracle's exact closed-interval overlap set, as sets, for both kinds."""
+    n = 0
+    for kind in ("vt", "ee"):
+        g = m.self_candidates(kind)
+        r = o.self_candidates(kind)
+        r = r[np.lexsort((r[:, 1], r[:, 0]))] if len(r) else r.reshape(0, 2)
+        assert len(np.unique(g, axis=0)) == len(g), f"{what}: {kind} pair listed twice"
+        assert np.array_equal(g, r), f"{what}: {kind} candidate sets differ ({len(g)} vs {len(r)})"
+        n += len(g)
+    return n
+
+
 def _run(sc, oracle_mod, scene, mode, substeps):
     world, patch, m, o = pair(sc, oracle_mod, scene, mode)
+    o.set(recordCandidates=1)
     patch.SetNumSteps(1)
     h = float(np.float32(np.float32(scene.dt) / np.float32(scene.substeps)))
     seen = [0, 0]
+    cands = 0
     for k in range(substeps):
         patch.Step(h)
         o.substep_detect(h)
+        cands += _same_candidates(m, o, f"substep {k}")
         for kind, j in (("vt", 0), ("ee", 1)):
             o.set_self_order(kind, m.self_contacts(kind)[0])
         o.substep_solve(h)
@@ -37,6 +56,7 @@ def _run(sc, oracle_mod, scene, mode, substeps):
             assert np.array_equal(gk, ok), f"{kind} order at substep {k}"
             seen[j] += len(gk)
         same_positions(m, o, f"substep {k}")
+    assert cands > 0
     return seen, m, o
 
 
