@@ -392,8 +392,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels directly instead of replaying the captured substep graph")
-    ap.add_argument("--sweep", type=int, default=1, choices=[0, 1],
-                    help="0: one launch per colour; 1: slab phases staged in LDS")
+    ap.add_argument("--sweep", type=int, default=3, choices=[0, 1, 3],
+                    help="0: one launch per colour; 1: slab phases staged in LDS; 3: triangles on tiles with "
+                         "halos (one launch per FEM pass), bends on slabs")
     ap.add_argument("--copy-probe-mb", type=int, default=1024,
                     help="size of the streaming-copy bandwidth probe (0 disables)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
@@ -458,7 +459,7 @@ def main():
             "iterations": its,
             "substeps_per_frame": scene.substeps,
             "graph_replay": not args.no_graph,
-            "sweep": ["colour launches", "slab phases (LDS)"][args.sweep],
+            "sweep": {0: "colour launches", 1: "slab phases (LDS)", 3: "FEM tiles with halos, bend slab phases (LDS)"}[args.sweep],
             "launch_phases_per_pass": {"fem": len(m.colour_offsets("tri")) - 1,
                                        "bend": len(m.colour_offsets("bend")) - 1},
         },

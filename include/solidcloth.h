@@ -234,6 +234,16 @@ SCEXP int scGetBends(scModel model, int* out, int count);
 SCEXP int scGetConstraintOrder(scModel model, int kind, int* order, int count);
 SCEXP int scGetColouring(scModel model, int kind, int* colourOffsets, int maxColours);
 SCEXP int scGetSchedule(scModel model, int kind, int* phase, int* slab, int* colour, int count);
+/* The triangles' tile schedule (sweep mode 3; no counterpart in the
+ * reference): counts = (tiles, tile vertex entries, colours, instances);
+ * tileVertBegin (tiles + 1) delimits each tile's vertices in tileVerts
+ * (ascending global ids; owned[i] = 1 for the tile's own vertices, 0 for its
+ * halo); tileColOffsets (tiles * colours + 1) delimits, per tile and colour,
+ * the tile's constraint instances, whose triangle indices are in instances.
+ * Any array may be NULL. Returns the tile count (0 when the triangles are
+ * not on tiles). */
+SCEXP int scGetTileSchedule(scModel model, int* counts, int* tileVertBegin, int* tileVerts, int* owned,
+                            int* tileColOffsets, int* instances);
 
 /* A patch from a triangle mesh (ClothPatch::Init(const Mesh&, offset,
  * attached), Physics/ClothPatch.cpp:43-133, which the reference's demo calls
@@ -255,12 +265,16 @@ SCEXP void scSetSolver(scModel model, int solver);
 SCEXP int scGetSolver(scModel model);
 
 /* Sweep schedule: mode 0 = one launch per colour (positions in HBM),
- * mode 1 = slab phases with the slab's vertices staged in LDS (default),
+ * mode 1 = slab phases with the slab's vertices staged in LDS,
  * slabs being strips of slabSize constraints; mode 2 = slab phases with
  * strip interiors of about slabSize constraints in one phase and the seams
  * between strips in the next. slabSize 0 (modes 1 and 2) lets the library
- * choose among both kinds by estimated sweep cost. Changes the Gauss-Seidel
- * order. */
+ * choose among both kinds by estimated sweep cost. Mode 3 = the FEM
+ * triangles on tiles (one global colouring; each workgroup sweeps its own
+ * vertices and a halo as many rings wide as there are colours, in one
+ * launch per pass, writing back its own vertices), bends and links as in
+ * mode 1 (the default; a mesh whose tiles would not fit the workgroup's LDS
+ * keeps mode 1 for its triangles). Changes the Gauss-Seidel order. */
 SCEXP void scSetSweep(scModel model, int mode, int slabSize);
 
 /* Contacts of the last substep against mesh obstacles, in the reference's

@@ -115,7 +115,7 @@ def _sig(L):
         "scLbvhSelfCheck": (i, [i, C.c_uint, i, vp]),
         "scGetSelfContacts": (i, [H, i, vp, vp, vp, i]), "scSetSelfParams": (None, [H, f, f]),
         "scSetSelfCCD": (None, [H, f, f, f]), "scSetSelfCapacity": (None, [H, i]),
-        "scSetSelfEdgeWeights": (None, [H, i]), "scGetSelfCandidates": (C.c_longlong, [H, i, vp, C.c_longlong]),
+        "scSetSelfEdgeWeights": (None, [H, i]), "scGetTileSchedule": (i, [H, vp, vp, vp, vp, vp, vp]), "scGetSelfCandidates": (C.c_longlong, [H, i, vp, C.c_longlong]),
         "scGetSelfStats": (i, [H, vp, i]),
         "scSetParticleInvMasses": (i, [H, vp, i]),
         "scCreateWarmStart": (H, [C.c_uint]), "scFreeWarmStart": (None, [H]),
@@ -584,6 +584,23 @@ class ClothModel:
         if n:
             _checked("scGetSelfCandidates", self.h, k, _ptr(out), n)
         return out
+
+    def tile_schedule(self):
+        """The triangles' tile schedule (sweep mode 3) as a dict: vert_begin,
+        verts, owned, col_offsets (tiles x colours + 1), instances, colours;
+        None when the triangles are not on tiles."""
+        counts = np.zeros(4, dtype=np.int32)
+        nt = _checked("scGetTileSchedule", self.h, _ptr(counts), None, None, None, None, None)
+        if nt == 0:
+            return None
+        ntiles, nverts, nc, ninst = (int(x) for x in counts)
+        vb = np.zeros(ntiles + 1, dtype=np.int32)
+        vs = np.zeros(nverts, dtype=np.int32)
+        ow = np.zeros(nverts, dtype=np.int32)
+        co = np.zeros(ntiles * nc + 1, dtype=np.int32)
+        ins = np.zeros(ninst, dtype=np.int32)
+        _checked("scGetTileSchedule", self.h, None, _ptr(vb), _ptr(vs), _ptr(ow), _ptr(co), _ptr(ins))
+        return dict(vert_begin=vb, verts=vs, owned=ow, col_offsets=co, instances=ins, colours=nc)
 
     def SetSelfEdgeWeights(self, mode):
         """0: the reference's edge-edge displacement weights; 1: each end's own."""

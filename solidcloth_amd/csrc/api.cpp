@@ -1100,6 +1100,35 @@ int scGetSchedule(scModel model, int kind, int* phase, int* slab, int* colour, i
     });
 }
 
+int scGetTileSchedule(scModel model, int* counts, int* tileVertBegin, int* tileVerts, int* owned,
+                      int* tileColOffsets, int* instances)
+{
+    return guard_r<int>(-1, [&] {
+        ClothModel* m = M(model);
+        m->ensure_colouring();
+        const Schedule& c = m->triCol;
+        if (!c.tiled) {
+            if (counts) counts[0] = counts[1] = counts[2] = counts[3] = 0;
+            return 0;
+        }
+        const int nt = (int)c.slabVertBegin.size() - 1;
+        if (counts) {
+            counts[0] = nt;
+            counts[1] = (int)c.slabVerts.size();
+            counts[2] = c.maxSlabColours;
+            counts[3] = (int)c.instCons.size();
+        }
+        if (tileVertBegin) memcpy(tileVertBegin, c.slabVertBegin.data(), c.slabVertBegin.size() * sizeof(int));
+        for (size_t i = 0; i < c.slabVerts.size(); i++) {
+            if (tileVerts) tileVerts[i] = (int)c.slabVerts[i];
+            if (owned) owned[i] = c.slabOwned[i];
+        }
+        if (tileColOffsets) memcpy(tileColOffsets, c.colOffsets.data(), c.colOffsets.size() * sizeof(int));
+        if (instances) memcpy(instances, c.instCons.data(), c.instCons.size() * sizeof(int));
+        return nt;
+    });
+}
+
 int scGetContactFeatures(scModel model, int* out, int count)
 {
     return guard_r<int>(-1, [&] {

@@ -138,9 +138,18 @@ struct SlabSet {
 };
 
 enum { kSlabBlock = 256, kMaxSlabColours = 63, kSlabBatch = 8 };
+// Flags in the high bits of a slab's vertex ids: the vertex's owner (the
+// FEM slab that applies staged contacts, or a tile's own vertex) and, in
+// bend slabs, the vertex's last slab (which may apply the iteration's
+// contacts); ids are below 2^30.
+constexpr unsigned kSlabOwner = 0x80000000u, kSlabContacts = 0x40000000u;
+constexpr int kSlabIdMask = 0x3fffffff;
 
-// Vertex contacts applied by the FEM slab kernel while it stages vertices
-// (the previous iteration's contacts, by each vertex's owner slab).
+// Vertex contacts applied by the slab kernels: enabled 1 (FEM slabs), while
+// staging vertices (the previous iteration's contacts, by each vertex's owner
+// slab); 2 (FEM tiles), after the last colour (this iteration's contacts of
+// the tile's own vertices); 3 (bend slabs), after the last colour (this
+// iteration's contacts of the vertices whose last bend slab it is).
 struct SlabContacts {
     ContactBuf cb;
     const float4* prev;
@@ -204,14 +213,14 @@ void launch_fem(hipStream_t s, float4* pos, TriStreams tr, int start, int count,
                 const SolverCtl* ctl);
 void launch_bend(hipStream_t s, float4* pos, const int4* bidx, int start, int count, float stiff, SolverCtl* ctl);
 void launch_fem_slab(hipStream_t s, float4* pos, TriStreams tr, const ushort4* lidx, SlabSet ss, int numSlabs,
-                     int maxVerts, float h2, float epsUV, const SolverCtl* ctl, const SlabContacts& kc);
+                     int maxVerts, float h2, float epsUV, SolverCtl* ctl, const SlabContacts& kc, bool tiles);
 void launch_predict_detect(hipStream_t s, float4* pos, float4* prev, float4* vel, int n, float h, f3 gdv, float drag,
                            SolverCtl* ctl, const CollSet& cs, ContactBuf cb, float thickness, float tolerance,
                            int collFlags);
 void launch_contacts_velocity(hipStream_t s, float4* pos, const float4* prev, float4* vel, ContactBuf cb,
                               float thickness, float mu, float invH, SolverCtl* ctl);
 void launch_bend_slab(hipStream_t s, float4* pos, const ushort4* lidx, SlabSet ss, int numSlabs, int maxVerts,
-                      float stiff, SolverCtl* ctl);
+                      float stiff, SolverCtl* ctl, const SlabContacts& kc);
 void launch_contacts(hipStream_t s, float4* pos, const float4* prev, float4* vel, ContactBuf cb, float thickness,
                      float mu, float invH, SolverCtl* ctl);
 void launch_iter_end(hipStream_t s, SolverCtl* ctl, int maxIter, int fixedIter, int bendFreq, int dihedral,

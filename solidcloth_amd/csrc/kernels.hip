@@ -444,12 +444,12 @@ __device__ __forceinline__ int slab_begin(const SlabSet& ss, const int4& info, i
 #pragma unroll
     for (int u = 0; u < kSlabBatch; u++) gk[u] = 0;
     for (int v0 = threadIdx.x; v0 < info.y; v0 += kSlabBatch * kSlabBlock) {
-        int g[kSlabBatch];
+        int raw[kSlabBatch];
         float4 P[kSlabBatch];
 #pragma unroll
-        for (int u = 0; u < kSlabBatch; u++) g[u] = vg[min(v0 + u * kSlabBlock, last)] & 0x7fffffff;
+        for (int u = 0; u < kSlabBatch; u++) raw[u] = vg[min(v0 + u * kSlabBlock, last)];
 #pragma unroll
-        for (int u = 0; u < kSlabBatch; u++) P[u] = pos[g[u]];
+        for (int u = 0; u < kSlabBatch; u++) P[u] = pos[raw[u] & kSlabIdMask];
 #pragma unroll
         for (int u = 0; u < kSlabBatch; u++) {
             const int v = v0 + u * kSlabBlock;
@@ -457,16 +457,16 @@ __device__ __forceinline__ int slab_begin(const SlabSet& ss, const int4& info, i
         }
         if (v0 == (int)threadIdx.x) {
 #pragma unroll
-            for (int u = 0; u < kSlabBatch; u++) gk[u] = g[u];
+            for (int u = 0; u < kSlabBatch; u++) gk[u] = raw[u];
         }
     }
-    if (kc.enabled) {
+    if (kc.enabled == 1) {
         // the owner applies the vertex's contacts (same thread-to-vertex map
         // as the staging loop, so no barrier is needed in between)
         for (int v = threadIdx.x; v < info.y; v += kSlabBlock) {
             const int raw = vg[v];
             if (raw >= 0) continue;
-            const int g = raw & 0x7fffffff;
+            const int g = raw & kSlabIdMask;
             const int n = kc.cb.count[g];
             if (n == 0) continue;
             const float4 P = sv[v];
@@ -481,6 +481,9 @@ __device__ __forceinline__ int slab_begin(const SlabSet& ss, const int4& info, i
     return info.w;
 }
 
+// Writes the slab's vertices back (kOwnedOnly: only those whose id carries
+// the owner flag, as a tile writes back its own vertices but not its halo).
+template <bool kOwnedOnly = false>
 __device__ __forceinline__ void slab_end(const SlabSet& ss, const int4& info, const float4* sv, float4* pos,
                                          const int (&gk)[kSlabBatch])
 {
@@ -491,22 +494,49 @@ __device__ __forceinline__ void slab_end(const SlabSet& ss, const int4& info, co
 #pragma unroll
         for (int u = 0; u < kSlabBatch; u++) {
             const int v = threadIdx.x + u * kSlabBlock;
-            if (v < info.y) pos[gk[u]] = sv[v];
+            if (v < info.y && (!kOwnedOnly || gk[u] < 0)) pos[gk[u] & kSlabIdMask] = sv[v];
         }
         return;
     }
     const int* vg = ss.verts + info.x;
     const int last = info.y - 1;
     for (int v0 = threadIdx.x; v0 < info.y; v0 += batch) {
-        int g[kSlabBatch];
+        int raw[kSlabBatch];
 #pragma unroll
-        for (int u = 0; u < kSlabBatch; u++) g[u] = vg[min(v0 + u * kSlabBlock, last)] & 0x7fffffff;
+        for (int u = 0; u < kSlabBatch; u++) raw[u] = vg[min(v0 + u * kSlabBlock, last)];
 #pragma unroll
         for (int u = 0; u < kSlabBatch; u++) {
             const int v = v0 + u * kSlabBlock;
-            if (v < info.y) pos[g[u]] = sv[v];
+            if (v < info.y && (!kOwnedOnly || raw[u] < 0)) pos[raw[u] & kSlabIdMask] = sv[v];
         }
     }
+}
+
+// The iteration's vertex contacts of the slab vertices whose id carries
+// `flag`, applied after the slab's last colour: a tile's own vertices
+// (kSlabOwner, FEM tiles, kc.enabled == 2) or each vertex in the last bend
+// slab that touches it (kSlabContacts, kc.enabled == 3). No other workgroup
+// of the launch, nor any later constraint of the iteration, touches these
+// vertices, and no other workgroup reads their contact state.
+__device__ __forceinline__ void slab_contacts_end(const SlabSet& ss, const int4& info, float4* sv,
+                                                  const SlabContacts& kc, SolverCtl* ctl, unsigned flag)
+{
+    const int* vg = ss.verts + info.x;
+    float err = 0.f;
+    for (int v = threadIdx.x; v < info.y; v += kSlabBlock) {
+        const int raw = vg[v];
+        if (!((unsigned)raw & flag)) continue;
+        const int g = raw & kSlabIdMask;
+        const int n = kc.cb.count[g];
+        if (n == 0) continue;
+        const float4 P = sv[v];
+        f3 p = xyz(P), ve = xyz(kc.vel[g]);
+        const float e = vertex_contacts(g, n, p, xyz(kc.prev[g]), ve, kc.cb, kc.thickness, kc.mu, kc.invH);
+        err = err < e ? e : err;
+        sv[v] = f4(p, P.w);
+        kc.vel[g] = f4(ve, 0.f);
+    }
+    wave_max_atomic(&ctl->cv, err);
 }
 
 // One colour's constraint for this thread, fetched a colour ahead. Indices
@@ -555,10 +585,13 @@ __device__ __forceinline__ void fem_colour(const FemItem& f, int e, float4* sv, 
     }
 }
 
+// kTiles: the slabs are tiles of a tile schedule (extended by their halos):
+// only the tile's own vertices are written back, and with kc.enabled == 2
+// their contacts of this iteration are applied after the last colour.
+template <bool kTiles>
 __global__ void __launch_bounds__(kSlabBlock) k_fem_slab(float4* __restrict__ pos, TriStreams tr,
                                                          const ushort4* __restrict__ lidx, SlabSet ss, float h2,
-                                                         float epsUV, const SolverCtl* __restrict__ ctl,
-                                                         SlabContacts kc)
+                                                         float epsUV, SolverCtl* __restrict__ ctl, SlabContacts kc)
 {
     extern __shared__ float4 sv[];
     __shared__ int scol[kMaxSlabColours + 1];
@@ -583,7 +616,8 @@ __global__ void __launch_bounds__(kSlabBlock) k_fem_slab(float4* __restrict__ po
         fem_colour(b, scol[c + 2], sv, tr, lidx, h2, epsUV);
         lds_barrier();
     }
-    slab_end(ss, info, sv, pos, gk);
+    if (kTiles && kc.enabled == 2) slab_contacts_end(ss, info, sv, kc, ctl, kSlabOwner);
+    slab_end<kTiles>(ss, info, sv, pos, gk);
 }
 
 struct QuadItem {
@@ -623,8 +657,10 @@ __device__ __forceinline__ void bend_colour(const QuadItem& f, int e, float4* sv
     }
 }
 
+// With kc.enabled == 3, the vertices whose last bend slab this is get the
+// iteration's vertex contacts after the last colour.
 __global__ void __launch_bounds__(kSlabBlock) k_bend_slab(float4* __restrict__ pos, const ushort4* __restrict__ lidx,
-                                                          SlabSet ss, float sStiff, SolverCtl* ctl)
+                                                          SlabSet ss, float sStiff, SolverCtl* ctl, SlabContacts kc)
 {
     extern __shared__ float4 sv[];
     __shared__ int scol[kMaxSlabColours + 1];
@@ -650,6 +686,7 @@ __global__ void __launch_bounds__(kSlabBlock) k_bend_slab(float4* __restrict__ p
         bend_colour(b, scol[c + 2], sv, lidx, sStiff, err);
         lds_barrier();
     }
+    if (kc.enabled == 3) slab_contacts_end(ss, info, sv, kc, ctl, kSlabContacts);
     slab_end(ss, info, sv, pos, gk);
     wave_max_atomic(&ctl->cb, err);
 }
@@ -1080,11 +1117,15 @@ void launch_fem(hipStream_t s, float4* pos, TriStreams tr, int start, int count,
 }
 
 void launch_fem_slab(hipStream_t s, float4* pos, TriStreams tr, const ushort4* lidx, SlabSet ss, int numSlabs,
-                     int maxVerts, float h2, float epsUV, const SolverCtl* ctl, const SlabContacts& kc)
+                     int maxVerts, float h2, float epsUV, SolverCtl* ctl, const SlabContacts& kc, bool tiles)
 {
     if (numSlabs <= 0) return;
-    hipLaunchKernelGGL(k_fem_slab, dim3(numSlabs), dim3(kSlabBlock), (maxVerts + 1) * sizeof(float4), s, pos, tr, lidx, ss,
-                       h2, epsUV, ctl, kc);
+    if (tiles)
+        hipLaunchKernelGGL(k_fem_slab<true>, dim3(numSlabs), dim3(kSlabBlock), (maxVerts + 1) * sizeof(float4), s, pos,
+                           tr, lidx, ss, h2, epsUV, ctl, kc);
+    else
+        hipLaunchKernelGGL(k_fem_slab<false>, dim3(numSlabs), dim3(kSlabBlock), (maxVerts + 1) * sizeof(float4), s,
+                           pos, tr, lidx, ss, h2, epsUV, ctl, kc);
 }
 
 void launch_predict_detect(hipStream_t s, float4* pos, float4* prev, float4* vel, int n, float h, f3 gdv, float drag,
@@ -1104,11 +1145,11 @@ void launch_contacts_velocity(hipStream_t s, float4* pos, const float4* prev, fl
 }
 
 void launch_bend_slab(hipStream_t s, float4* pos, const ushort4* lidx, SlabSet ss, int numSlabs, int maxVerts,
-                      float stiff, SolverCtl* ctl)
+                      float stiff, SolverCtl* ctl, const SlabContacts& kc)
 {
     if (numSlabs <= 0) return;
     hipLaunchKernelGGL(k_bend_slab, dim3(numSlabs), dim3(kSlabBlock), (maxVerts + 1) * sizeof(float4), s, pos, lidx, ss,
-                       stiff, ctl);
+                       stiff, ctl, kc);
 }
 
 void launch_bend(hipStream_t s, float4* pos, const int4* bidx, int start, int count, float stiff, SolverCtl* ctl)

@@ -312,6 +312,7 @@ void GpuSolver::release()
     warm.built = warm.saved = false;
     dev_free(d_pos); dev_free(d_prev); dev_free(d_vel);
     dev_free(d_tidx); dev_free(d_tA); dev_free(d_tB); dev_free(d_tC); dev_free(d_bidx);
+    dev_free(d_ttA); dev_free(d_ttB); dev_free(d_ttC);
     dev_free(d_triColOff); dev_free(d_bendColOff); dev_free(d_lk); dev_free(d_lr);
     dev_free(d_cCount); dev_free(d_cP); dev_free(d_cN); dev_free(d_cSrc); dev_free(d_cFeat); dev_free(d_ctl);
     dev_free(d_geo); dev_free(d_cEdges); dev_free(d_cTris);
@@ -420,11 +421,12 @@ void ClothModel::sync_to_host()
 // With `owners`, bit 31 of a slab vertex id marks the vertex's first slab in
 // phase order; returns the number of vertices below `numVertices` that no
 // slab contains.
-static int upload_slabs(const Schedule& s, DeviceSlabs& d, bool owners, int numVertices)
+static int upload_slabs(const Schedule& s, DeviceSlabs& d, bool owners, int numVertices, bool lastFlags = false)
 {
     dev_free(d.lidx); dev_free(d.info); dev_free(d.colOff); dev_free(d.first); dev_free(d.verts);
     if (!s.slabbed || s.order.empty()) return numVertices;
-    const int n = (int)s.order.size();
+    // a tile schedule's positions are its instances
+    const int n = s.tiled ? (int)s.instCons.size() : (int)s.order.size();
     std::vector<ushort4> li(n);
     for (int k = 0; k < n; k++) {
         const uint16_t* l = &s.localIdx[(size_t)k * s.arity];
@@ -443,11 +445,22 @@ static int upload_slabs(const Schedule& s, DeviceSlabs& d, bool owners, int numV
     int orphans = numVertices;
     for (size_t i = 0; i < verts.size(); i++) {
         const int v = verts[i];
+        const bool own = s.tiled ? s.slabOwned[i] != 0 : (v >= 0 && v < numVertices && !seen[v]);
         if (v >= 0 && v < numVertices && !seen[v]) {
             seen[v] = 1;
             orphans--;
-            if (owners) verts[i] = (int)((unsigned)v | 0x80000000u);
         }
+        if (owners && own) verts[i] = (int)((unsigned)v | kSlabOwner);
+    }
+    if (lastFlags) {
+        // each vertex's last slab in phase order (slabs are numbered phase-major)
+        std::vector<int> lastAt(numVertices, -1);
+        for (size_t i = 0; i < s.slabVerts.size(); i++) {
+            const int v = (int)s.slabVerts[i];
+            if (v >= 0 && v < numVertices) lastAt[v] = (int)i;
+        }
+        for (int v = 0; v < numVertices; v++)
+            if (lastAt[v] >= 0) verts[lastAt[v]] = (int)((unsigned)verts[lastAt[v]] | kSlabContacts);
     }
     dev_alloc(d.lidx, n);
     dev_alloc(d.info, ns);
@@ -491,6 +504,25 @@ void ClothModel::build_device_topology()
         tb[k] = make_float4(t.invDet, t.lu2, t.lv2, t.area);
         tc[k] = make_float2(t.lu0, t.lv0);
     }
+    dev_free(g.d_ttA); dev_free(g.d_ttB); dev_free(g.d_ttC);
+    if (triCol.tiled) {
+        const size_t ni = triCol.instCons.size();
+        std::vector<float4> ia(ni), ib(ni);
+        std::vector<float2> ic(ni);
+        for (size_t k = 0; k < ni; k++) {
+            const Triangle& t = tris[triCol.instCons[k]];
+            ia[k] = make_float4(t.du1, t.du2, t.dv1, t.dv2);
+            ib[k] = make_float4(t.invDet, t.lu2, t.lv2, t.area);
+            ic[k] = make_float2(t.lu0, t.lv0);
+        }
+        dev_alloc(g.d_ttA, std::max<size_t>(ni, 1)); dev_alloc(g.d_ttB, std::max<size_t>(ni, 1));
+        dev_alloc(g.d_ttC, std::max<size_t>(ni, 1));
+        if (ni) {
+            HIP_OK(hipMemcpy(g.d_ttA, ia.data(), ni * sizeof(float4), hipMemcpyHostToDevice));
+            HIP_OK(hipMemcpy(g.d_ttB, ib.data(), ni * sizeof(float4), hipMemcpyHostToDevice));
+            HIP_OK(hipMemcpy(g.d_ttC, ic.data(), ni * sizeof(float2), hipMemcpyHostToDevice));
+        }
+    }
     dev_alloc(g.d_tidx, g.nt); dev_alloc(g.d_tA, g.nt); dev_alloc(g.d_tB, g.nt); dev_alloc(g.d_tC, g.nt);
     if (g.nt) {
         HIP_OK(hipMemcpy(g.d_tidx, ti.data(), g.nt * sizeof(int4), hipMemcpyHostToDevice));
@@ -513,7 +545,7 @@ void ClothModel::build_device_topology()
     HIP_OK(hipMemcpy(g.d_bendColOff, bendCol.phaseOffsets.data(), bendCol.phaseOffsets.size() * sizeof(int),
                      hipMemcpyHostToDevice));
     g.triOrphans = upload_slabs(triCol, g.triSlabs, true, g.nv);
-    upload_slabs(bendCol, g.bendSlabs, false, g.nv);
+    g.bendOrphans = upload_slabs(bendCol, g.bendSlabs, false, g.nv, true);
     // links in schedule order
     g.nl = (int)links.size();
     std::vector<int2> lk(g.nl);
@@ -572,6 +604,13 @@ void ClothModel::build_device_topology()
     mTopoDirty = false;
 }
 
+// Own vertices per tile of a tile schedule: one per thread of a slab
+// workgroup, so that a tile's extended region (own vertices and a halo of as
+// many rings as colours, about four times as many on a grid) has about one
+// constraint per thread per colour.
+constexpr int kTileVerts = kSlabBlock;
+static_assert(kMaxTileColours <= kMaxSlabColours, "tile colours must fit the slab kernels");
+
 void ClothModel::ensure_colouring()
 {
     if (!mColourDirty) return;
@@ -602,7 +641,18 @@ void ClothModel::ensure_colouring()
         bendCol = schedule_slabs(nb, 4, bv.data(), nv, pos.data(), mSlabSize, maxVerts, seamed);
         linkCol = schedule_slabs(nl, 2, lv.data(), nv, pos.data(), mSlabSize, maxVerts, seamed);
     } else {
-        triCol = schedule_slabs_auto(nt, 3, tv.data(), nv, pos.data(), maxVerts, kSlabBlock);
+        bool tiled = false;
+        if (mSweepMode == SWEEP_TILES && nt > 0) {
+            // tiles of about one owned vertex per thread of the workgroup;
+            // a mesh whose halo makes a tile too large keeps the slabs
+            try {
+                triCol = schedule_tiles(nt, 3, tv.data(), nv, pos.data(), kTileVerts, maxVerts);
+                tiled = true;
+            } catch (const std::runtime_error&) {
+                tiled = false;
+            }
+        }
+        if (!tiled) triCol = schedule_slabs_auto(nt, 3, tv.data(), nv, pos.data(), maxVerts, kSlabBlock);
         bendCol = schedule_slabs_auto(nb, 4, bv.data(), nv, pos.data(), maxVerts, kSlabBlock);
         linkCol = schedule_slabs_auto(nl, 2, lv.data(), nv, pos.data(), maxVerts, kSlabBlock);
     }
@@ -611,7 +661,7 @@ void ClothModel::ensure_colouring()
 
 void ClothModel::set_sweep(int mode, int slabSize)
 {
-    if (mode != SWEEP_COLOURS && mode != SWEEP_SLABS && mode != SWEEP_SEAMS)
+    if (mode != SWEEP_COLOURS && mode != SWEEP_SLABS && mode != SWEEP_SEAMS && mode != SWEEP_TILES)
         throw std::runtime_error("unknown sweep mode");
     if (slabSize < 0 || slabSize > 65536) throw std::runtime_error("slab size out of range");
     if (mode == mSweepMode && slabSize == mSlabSize) return;
@@ -906,8 +956,16 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
     // Edge and triangle contacts follow the vertex contacts inside an
     // iteration, so they also keep the launches separate.
     const bool fuse = mSolver == SOLVER_GAUSS_SEIDEL && anyContacts && mFEM && mFixedIterations && triCol.slabbed &&
-                      triCol.numPhases() > 0 &&
+                      !triCol.tiled && triCol.numPhases() > 0 &&
                       g.triOrphans == 0 && !edgeContacts && !triContacts && !selfColl;
+    // A tile schedule stages every tile's halo in the same launch, so a
+    // vertex's contacts cannot be applied while staging (another tile may
+    // stage it before or after its owner updates the contact state).
+    // Instead, in an iteration without a bend pass the FEM launch applies the
+    // iteration's vertex contacts at its end, each tile to its own vertices;
+    // iterations with a bend pass keep the contact launch after the bends.
+    const bool fuseTiles = mSolver == SOLVER_GAUSS_SEIDEL && anyContacts && mFEM && mFixedIterations &&
+                           triCol.tiled && !edgeContacts && !triContacts && !selfColl;
     if (anyMesh || edgeContacts || triContacts || selfColl) {
         // the mesh query radius needs max |pos - prev| over all vertices
         // before any detection, and the contact groups are cleared there, so
@@ -1020,16 +1078,22 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                 timed_end(g.timer, timed, 0, s, ev);
             }
         }
+        const bool bendPass = mDihedral && k % mBendFreq == 0;
+        const bool contactsInFem = fuseTiles && !bendPass;
+        // with a bend pass, each vertex's last bend slab applies its contacts
+        const bool contactsInBends = fuseTiles && bendPass && bendCol.slabbed && g.bendOrphans == 0;
         if (mFEM) {
             for (int p = 0; p < triCol.numPhases(); p++) {
                 timed_begin(g.timer, timed, 0, s, ev);
                 if (triCol.slabbed) {
                     const int b = triCol.phaseSlabBegin[p], ns = triCol.phaseSlabBegin[p + 1] - b;
-                    SlabSet ss{g.triSlabs.info, g.triSlabs.colOff, g.triSlabs.first, g.triSlabs.verts, b,
-                               (int)triCol.order.size()};
-                    SlabContacts kc{cb, g.d_prev, g.d_vel, mThickness, mFriction, invH, (fuse && k > 0) ? 1 : 0};
-                    launch_fem_slab(s, g.d_pos, tr, g.triSlabs.lidx, ss, ns, triCol.maxSlabVerts, h2, epsUV, g.d_ctl,
-                                    kc);
+                    const int positions = triCol.tiled ? (int)triCol.instCons.size() : (int)triCol.order.size();
+                    SlabSet ss{g.triSlabs.info, g.triSlabs.colOff, g.triSlabs.first, g.triSlabs.verts, b, positions};
+                    SlabContacts kc{cb, g.d_prev, g.d_vel, mThickness, mFriction, invH,
+                                    (fuse && k > 0) ? 1 : contactsInFem ? 2 : 0};
+                    const TriStreams fs = triCol.tiled ? TriStreams{g.d_tidx, g.d_ttA, g.d_ttB, g.d_ttC} : tr;
+                    launch_fem_slab(s, g.d_pos, fs, g.triSlabs.lidx, ss, ns, triCol.maxSlabVerts, h2, epsUV, g.d_ctl,
+                                    kc, triCol.tiled);
                 } else {
                     int st = triCol.phaseOffsets[p], cnt = triCol.phaseOffsets[p + 1] - st;
                     launch_fem(s, g.d_pos, tr, st, cnt, h2, epsUV, g.d_ctl);
@@ -1037,14 +1101,15 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                 timed_end(g.timer, timed, 0, s, ev);
             }
         }
-        if (mDihedral && k % mBendFreq == 0) {
+        if (bendPass) {
             for (int p = 0; p < bendCol.numPhases(); p++) {
                 timed_begin(g.timer, timed, 1, s, ev);
                 if (bendCol.slabbed) {
                     const int b = bendCol.phaseSlabBegin[p], ns = bendCol.phaseSlabBegin[p + 1] - b;
                     SlabSet ss{g.bendSlabs.info, g.bendSlabs.colOff, g.bendSlabs.first, g.bendSlabs.verts, b,
                                (int)bendCol.order.size()};
-                    launch_bend_slab(s, g.d_pos, g.bendSlabs.lidx, ss, ns, bendCol.maxSlabVerts, sStiff, g.d_ctl);
+                    SlabContacts kc{cb, g.d_prev, g.d_vel, mThickness, mFriction, invH, contactsInBends ? 3 : 0};
+                    launch_bend_slab(s, g.d_pos, g.bendSlabs.lidx, ss, ns, bendCol.maxSlabVerts, sStiff, g.d_ctl, kc);
                 } else {
                     int st = bendCol.phaseOffsets[p], cnt = bendCol.phaseOffsets[p + 1] - st;
                     launch_bend(s, g.d_pos, g.d_bidx, st, cnt, sStiff, g.d_ctl);
@@ -1052,7 +1117,7 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                 timed_end(g.timer, timed, 1, s, ev);
             }
         }
-        if (anyContacts && !fuse) {
+        if (anyContacts && !fuse && !contactsInFem && !contactsInBends) {
             timed_begin(g.timer, timed, 2, s, ev);
             launch_contacts(s, g.d_pos, g.d_prev, g.d_vel, cb, mThickness, mFriction, invH, g.d_ctl);
             timed_end(g.timer, timed, 2, s, ev);

@@ -177,8 +177,10 @@ private:
     void stage(const float* X, const float* T, const float* w, int n, float& invCount);
 };
 
-// Sweep schedules: 0 = one launch per colour, 1 = slab phases with LDS staging
-enum { SWEEP_COLOURS = 0, SWEEP_SLABS = 1, SWEEP_SEAMS = 2 };
+// Sweep schedules: 0 = one launch per colour, 1 = slab phases with LDS
+// staging, 2 = slab phases of strip interiors and seams, 3 = FEM triangles on
+// tiles with halos (one launch per pass), bends and links as in 1
+enum { SWEEP_COLOURS = 0, SWEEP_SLABS = 1, SWEEP_SEAMS = 2, SWEEP_TILES = 3 };
 // Physics/ClothModel.h:23-30 (SolverType); Step runs SolveGS for
 // Gauss-Seidel, SolveJacobi for Jacobi and SolveCR for every other value
 enum { SOLVER_JACOBI = 0, SOLVER_GAUSS_SEIDEL = 1, SOLVER_JACOBI_CR = 2, SOLVER_IMPLICIT = 3, SOLVER_EXPLICIT = 4 };
@@ -214,6 +216,9 @@ public:
     int4* d_tidx = nullptr;
     float4 *d_tA = nullptr, *d_tB = nullptr;
     float2* d_tC = nullptr;
+    // tile schedule: the triangles' rest data once per tile instance
+    float4 *d_ttA = nullptr, *d_ttB = nullptr;
+    float2* d_ttC = nullptr;
     int4* d_bidx = nullptr;
     // links (non-FEM model) in schedule order: vertex pair, (rest length, stiffness)
     int2* d_lk = nullptr;
@@ -256,6 +261,7 @@ public:
     } self;
     DeviceSlabs triSlabs, bendSlabs, linkSlabs;
     int triOrphans = 0;  // vertices in no triangle slab
+    int bendOrphans = 0;  // vertices in no bend slab
     int slots = 0, slotCap = 0;
     int activeSlots = 0;  // vertex-contact slots the world can fill (0: no vertex contacts)
     SolverCtl* d_ctl = nullptr;
@@ -392,7 +398,7 @@ public:
     // colour groups for obstacle-edge (cloth edges) and obstacle-vertex
     // (cloth triangles) contacts: greedy colourings in index order
     Schedule edgeContactCol, triContactCol;
-    int mSweepMode = SWEEP_SLABS;
+    int mSweepMode = SWEEP_TILES;
     int mSlabSize = 0;  // 0: chosen per constraint type by schedule_slabs_auto
 
     GpuSolver gpu;

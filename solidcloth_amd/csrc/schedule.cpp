@@ -2,6 +2,7 @@
 #include "schedule.h"
 
 #include <algorithm>
+#include <cmath>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -270,6 +271,152 @@ Schedule schedule_slabs(int n, int arity, const uint32_t* verts, int nv, const f
     }
     s.colOffsets.push_back((int)s.order.size());
     s.phaseOffsets.push_back((int)s.order.size());
+    return s;
+}
+
+Schedule schedule_tiles(int n, int arity, const uint32_t* verts, int nv, const f3* rest, int tileVerts,
+                        int maxVerts)
+{
+    Schedule s;
+    s.arity = arity;
+    s.slabbed = true;
+    s.tiled = true;
+    // one global colouring; the sequential order is colour-major
+    Greedy gr(nv);
+    std::vector<int> ids(n), col(n);
+    std::iota(ids.begin(), ids.end(), 0);
+    const int nc = n ? gr.colour(ids, arity, verts, col) : 0;
+    std::vector<int> cstart(nc + 1, 0);
+    for (int c : col) cstart[c + 1]++;
+    for (int k = 0; k < nc; k++) cstart[k + 1] += cstart[k];
+    s.order.resize(n);
+    s.phaseOf.assign(n, 0);
+    s.slabOf.assign(n, -1);
+    s.colourOf.resize(n);
+    {
+        std::vector<int> fill(cstart.begin(), cstart.end() - (nc ? 1 : 0));
+        for (int k = 0; k < n; k++) {
+            const int p = fill[col[k]]++;
+            s.order[p] = k;
+            s.colourOf[p] = col[k];
+        }
+    }
+    s.phaseOffsets = {0, n};
+    s.halo = nc;
+    // vertex -> constraints and vertex -> neighbours (sharing a constraint)
+    std::vector<int> vcOff(nv + 1, 0), vc((size_t)n * arity);
+    for (int k = 0; k < n * arity; k++) vcOff[verts[k] + 1]++;
+    for (int v = 0; v < nv; v++) vcOff[v + 1] += vcOff[v];
+    {
+        std::vector<int> fill(vcOff.begin(), vcOff.end() - 1);
+        for (int c = 0; c < n; c++)
+            for (int a = 0; a < arity; a++) vc[fill[verts[(size_t)c * arity + a]]++] = c;
+    }
+    // tiles: a grid over the rest shape's two widest axes
+    int ax0 = 0, ax1 = 2;
+    f3 lo{0, 0, 0}, hi{0, 0, 0};
+    if (nv > 0) {
+        lo = hi = rest[0];
+        for (int i = 1; i < nv; i++) {
+            lo = f3{std::min(lo.x, rest[i].x), std::min(lo.y, rest[i].y), std::min(lo.z, rest[i].z)};
+            hi = f3{std::max(hi.x, rest[i].x), std::max(hi.y, rest[i].y), std::max(hi.z, rest[i].z)};
+        }
+        const float ext[3] = {hi.x - lo.x, hi.y - lo.y, hi.z - lo.z};
+        const int narrow = (ext[0] <= ext[1] && ext[0] <= ext[2]) ? 0 : (ext[1] <= ext[2] ? 1 : 2);
+        ax0 = narrow == 0 ? 1 : 0;
+        ax1 = narrow == 2 ? 1 : 2;
+    }
+    const float e0 = axis_coord(hi, ax0) - axis_coord(lo, ax0), e1 = axis_coord(hi, ax1) - axis_coord(lo, ax1);
+    const double bins = std::max(1.0, (double)nv / std::max(1, tileVerts));
+    int nb0 = 1, nb1 = 1;
+    if (e0 > 0.f && e1 > 0.f) {
+        nb0 = std::max(1, (int)std::lround(std::sqrt(bins * e0 / e1)));
+        nb1 = std::max(1, (int)std::lround(bins / nb0));
+    } else if (e0 > 0.f) {
+        nb0 = std::max(1, (int)std::lround(bins));
+    } else if (e1 > 0.f) {
+        nb1 = std::max(1, (int)std::lround(bins));
+    }
+    auto bin = [&](float x, float l, float e, int nb) {
+        if (!(e > 0.f)) return 0;
+        return std::min(nb - 1, std::max(0, (int)((x - l) / e * (float)nb)));
+    };
+    std::vector<std::vector<int>> owned((size_t)nb0 * nb1);
+    for (int v = 0; v < nv; v++) {
+        const int b0 = bin(axis_coord(rest[v], ax0), axis_coord(lo, ax0), e0, nb0);
+        const int b1 = bin(axis_coord(rest[v], ax1), axis_coord(lo, ax1), e1, nb1);
+        owned[(size_t)b1 * nb0 + b0].push_back(v);
+    }
+    // per tile: the halo rings, the constraints inside, local indices
+    std::vector<int> stamp(nv, -1), local(nv, 0), cstamp(n, -1);
+    s.phaseSlabBegin = {0};
+    s.slabVertBegin = {0};
+    s.slabColBegin = {0};
+    s.colOffsets.clear();
+    int tile = 0;
+    for (const std::vector<int>& own : owned) {
+        if (own.empty()) continue;
+        std::vector<int> ext(own.begin(), own.end());
+        for (int v : own) stamp[v] = tile;
+        size_t frontBegin = 0;
+        for (int ring = 0; ring < s.halo; ring++) {
+            const size_t frontEnd = ext.size();
+            for (size_t i = frontBegin; i < frontEnd; i++) {
+                const int v = ext[i];
+                for (int k = vcOff[v]; k < vcOff[v + 1]; k++) {
+                    const uint32_t* cv = verts + (size_t)vc[k] * arity;
+                    for (int a = 0; a < arity; a++) {
+                        const int u = (int)cv[a];
+                        if (stamp[u] != tile) {
+                            stamp[u] = tile;
+                            ext.push_back(u);
+                        }
+                    }
+                }
+            }
+            frontBegin = frontEnd;
+        }
+        if ((int)ext.size() > maxVerts)
+            throw std::runtime_error("an extended tile has " + std::to_string(ext.size()) + " vertices (at most " +
+                                     std::to_string(maxVerts) + ")");
+        std::sort(ext.begin(), ext.end());
+        std::vector<char> isOwn(ext.size(), 0);
+        for (size_t i = 0; i < ext.size(); i++) local[ext[i]] = (int)i;
+        for (int v : own) isOwn[local[v]] = 1;
+        // constraints with every vertex in the extended region, by colour
+        std::vector<std::vector<int>> byCol(nc);
+        for (int v : ext)
+            for (int k = vcOff[v]; k < vcOff[v + 1]; k++) {
+                const int c = vc[k];
+                if (cstamp[c] == tile) continue;
+                cstamp[c] = tile;
+                bool inside = true;
+                for (int a = 0; a < arity && inside; a++) inside = stamp[verts[(size_t)c * arity + a]] == tile;
+                if (inside) byCol[col[c]].push_back(c);
+            }
+        for (int c = 0; c < nc; c++) {
+            std::sort(byCol[c].begin(), byCol[c].end());
+            s.colOffsets.push_back((int)s.instCons.size());
+            for (int k : byCol[c]) {
+                s.instCons.push_back(k);
+                for (int a = 0; a < arity; a++) s.localIdx.push_back((uint16_t)local[verts[(size_t)k * arity + a]]);
+            }
+        }
+        for (size_t i = 0; i < ext.size(); i++) {
+            s.slabVerts.push_back((uint32_t)ext[i]);
+            s.slabOwned.push_back(isOwn[i]);
+        }
+        s.slabVertBegin.push_back((int)s.slabVerts.size());
+        s.slabColBegin.push_back((int)s.colOffsets.size());
+        s.maxSlabVerts = std::max(s.maxSlabVerts, (int)ext.size());
+        tile++;
+    }
+    s.colOffsets.push_back((int)s.instCons.size());
+    s.phaseSlabBegin.push_back(tile);
+    s.maxSlabColours = nc;
+    if (nc > kMaxTileColours)
+        throw std::runtime_error("tile schedule: " + std::to_string(nc) + " colours (at most " +
+                                 std::to_string(kMaxTileColours) + ")");
     return s;
 }
 

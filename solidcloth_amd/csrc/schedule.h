@@ -16,8 +16,18 @@
 //    sweeps its local colours with workgroup barriers and writes the vertices
 //    back. Slabs of one phase touch disjoint vertices.
 //
+//  * tile schedule (FEM triangles): one global greedy colouring; the
+//    vertices are cut into tiles, each extended by a halo of as many rings of
+//    neighbours as there are colours, and each tile sweeps every colour over
+//    the constraints inside its extended region. A stale value at the
+//    region's border moves inwards by at most one ring per colour step, so
+//    the tile's own vertices end exactly as the sequential colour-major
+//    sweep leaves them; halos are computed redundantly and only owned
+//    vertices are written back. One phase, one launch, and the fewest
+//    sequential colour steps (the global colour count).
+//
 // The exported order is phase-major, then slab, then local colour, then
-// ascending constraint index.
+// ascending constraint index; for tiles, colour-major then ascending index.
 #pragma once
 
 #include <cstdint>
@@ -26,6 +36,9 @@
 #include "scmath.h"
 
 namespace sc {
+
+// Largest number of colours of a tile schedule (the slab kernels' limit).
+constexpr int kMaxTileColours = 63;
 
 struct Schedule {
     int arity = 0;
@@ -45,6 +58,14 @@ struct Schedule {
     int maxSlabColours = 0;
     int slabSize = 0;
     bool seamed = false;
+    // tile mode: slabs are tiles whose constraint lists are instances that
+    // may repeat a constraint in several tiles (instCons: the constraint of
+    // each instance; colOffsets and localIdx index instances, not `order`),
+    // and slabOwned marks each tile's own vertices in slabVerts
+    bool tiled = false;
+    int halo = 0;
+    std::vector<int> instCons;
+    std::vector<uint8_t> slabOwned;
     int numPhases() const { return (int)phaseOffsets.size() - 1; }
 };
 
@@ -57,6 +78,11 @@ Schedule schedule_colours(int numConstraints, int arity, const uint32_t* verts, 
 // slab (all in one phase) and the constraints across each cut a seam slab.
 Schedule schedule_slabs(int numConstraints, int arity, const uint32_t* verts, int numVertices, const f3* rest,
                         int slabSize, int maxVerts, bool seamed = false);
+
+// Tile schedule with tiles of about `tileVerts` own vertices; throws if an
+// extended tile would exceed `maxVerts` vertices.
+Schedule schedule_tiles(int numConstraints, int arity, const uint32_t* verts, int numVertices, const f3* rest,
+                        int tileVerts, int maxVerts);
 
 // Estimated sweep cost of a slab schedule in projection latencies, for a
 // workgroup of `threads` threads.

@@ -6,9 +6,10 @@ evaluate the same FP32 operation sequence without contraction, and every
 group of constraints that runs concurrently touches disjoint vertices, so
 equality is expected to be bit-exact; the stated tolerance (1e-5 relative to
 the cloth extent) is the acceptance bar and bit-exactness is asserted on top.
-Both sweep schedules are covered: one launch per colour (mode 0) and slab
-phases staged in LDS (mode 1, the default), with strips and interior/seam
-slabs of several sizes.
+Every sweep schedule is covered: one launch per colour (mode 0), slab phases
+staged in LDS (mode 1), with strips and interior/seam slabs of several sizes,
+and the triangles on tiles with halos (mode 3), whose vertex contacts of an
+iteration without a bend pass are applied at the end of the FEM launch.
 """
 import numpy as np
 import pytest
@@ -17,7 +18,7 @@ from solidcloth_amd import scenes as S
 
 pytestmark = pytest.mark.gpu
 REL_TOL = 1e-5
-MODES = [0, 1]
+MODES = [0, 1, 3]
 
 
 def _pair(sc, oracle_mod, scene, mode=1):
@@ -125,11 +126,12 @@ def test_drape256_full_size_parity(sc, oracle_mod, mode):
     assert np.array_equal(gp.astype(np.float64), op)
 
 
-def test_drape256_slab_schedule_long_run(sc, oracle_mod):
+@pytest.mark.parametrize("mode", [1, 3])
+def test_drape256_slab_schedule_long_run(sc, oracle_mod, mode):
     """25 frames (100 substeps) of the full-size workload: the cloth reaches
     the sphere, so contacts and friction are active in the compared run."""
     scene = S.DRAPE256
-    world, patch, m, o = _pair(sc, oracle_mod, scene, 1)
+    world, patch, m, o = _pair(sc, oracle_mod, scene, mode)
     for f in range(25):
         patch.Step(scene.dt)
         o.step(scene.dt)

@@ -326,3 +326,85 @@ def test_batch_pins_and_validation(sc):
     b2 = sc.ClothBatch([empty])
     with pytest.raises(sc.SolidClothError, match="no cloth"):
         b2.Step(1.0 / 60)
+
+
+def _emulated_sweeps(tris, nv, ts, seed):
+    """A deterministic nonlinear update of each triangle's three vertices,
+    run (a) one triangle after another in the schedule's colour-major order
+    and (b) tile by tile as the GPU runs it: each tile on a private copy of
+    its vertices, all colours over its own instances, writing back only its
+    own vertices. Returns both results."""
+    rng = np.random.default_rng(seed)
+    x0 = rng.normal(size=(nv, 2))
+
+    def project(x, t):
+        a, b, c = tris[t]
+        m = (x[a] + x[b] + x[c]) / 3.0
+        s = np.sin(x[a, 0] * 1.3 + x[b, 1] * 0.7 - x[c, 0] * 0.5) * 0.2
+        x[a] = x[a] + s * (m - x[a]) + 0.01
+        x[b] = x[b] + s * (m - x[b]) - 0.02
+        x[c] = x[c] + 0.5 * s * (m - x[c])
+
+    nc = ts["colours"]
+    ntiles = len(ts["vert_begin"]) - 1
+    seq = x0.copy()
+    # colour-major order: every instance's triangle, each once, colour by colour
+    col_of = {}
+    for tile in range(ntiles):
+        for col in range(nc):
+            lo, hi = ts["col_offsets"][tile * nc + col], ts["col_offsets"][tile * nc + col + 1]
+            for t in ts["instances"][lo:hi]:
+                col_of[int(t)] = col
+    order = sorted(col_of, key=lambda t: (col_of[t], t))
+    for t in order:
+        project(seq, t)
+    tiled = x0.copy()
+    for tile in range(ntiles):
+        vb, ve = ts["vert_begin"][tile], ts["vert_begin"][tile + 1]
+        verts, own = ts["verts"][vb:ve], ts["owned"][vb:ve]
+        x = x0.copy()  # the tile stages its vertices from the state before the launch
+        for col in range(nc):
+            lo, hi = ts["col_offsets"][tile * nc + col], ts["col_offsets"][tile * nc + col + 1]
+            for t in ts["instances"][lo:hi]:
+                project(x, int(t))
+        mine = verts[own == 1]
+        tiled[mine] = x[mine]
+    return seq, tiled, order
+
+
+@pytest.mark.parametrize("n", [40, 97, 256])
+def test_tile_schedule_equals_the_colour_major_sweep(sc, n):
+    """Sweep mode 3 (FEM triangles on tiles with halos). Every vertex is one
+    tile's own; each tile's instances of one colour share no vertex; a tile
+    holds every triangle inside its extended region; and a nonlinear
+    per-triangle update run tile by tile (private copies, own vertices
+    written back) equals running it sequentially in the exported
+    colour-major order, exactly, which holds only when every tile's halo
+    covers its own vertices' dependencies over all colour steps."""
+    world, patch = S.build_patch(S.DRAPE32.with_(n=n), sc)
+    m = patch.GetModel()
+    m.SetSweep(3, 0)
+    ts = m.tile_schedule()
+    assert ts is not None
+    tris = m.triangles()
+    nv = n * n
+    owners = np.bincount(ts["verts"][ts["owned"] == 1], minlength=nv)
+    assert np.all(owners == 1)
+    nc = ts["colours"]
+    assert nc == 8  # the grid's clique bound
+    ntiles = len(ts["vert_begin"]) - 1
+    for tile in range(ntiles):
+        vb, ve = ts["vert_begin"][tile], ts["vert_begin"][tile + 1]
+        region = set(ts["verts"][vb:ve].tolist())
+        inst = []
+        for col in range(nc):
+            lo, hi = ts["col_offsets"][tile * nc + col], ts["col_offsets"][tile * nc + col + 1]
+            vs = tris[ts["instances"][lo:hi]].ravel()
+            assert len(vs) == len(set(vs.tolist())), (tile, col)
+            inst.extend(ts["instances"][lo:hi].tolist())
+        inside = [t for t in range(len(tris)) if set(tris[t].tolist()) <= region]
+        assert sorted(inst) == inside, tile
+    # the exported order is the colour-major order
+    seq, tiled, order = _emulated_sweeps(tris, nv, ts, seed=n)
+    assert list(m.constraint_order("tri")) == order
+    assert np.array_equal(seq, tiled)
