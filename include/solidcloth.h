@@ -422,11 +422,14 @@ SCEXP void scBatchSetUseGraph(scClothBatch batch, bool enable);
  * plain solve; only the corrections are recorded, so that a later switch to
  * mode 2 starts with current features). The network must
  * outlive the attachment; models sharing a network step one at a time.
- * In mode 2 each scStepCloth is judged by the plain solver: the step's
- * residual (scGetResidual) is compared with the threshold
- * (scSetWarmStartFallback; < 0, the default, means the last plain step's
- * residual, and the first step runs plain to set it); above it, the step is
- * rerun plain from its starting state.
+ * In mode 2 each scStepCloth is judged by the plain solver. With
+ * scSetWarmStartFallback(t < 0) (the default) the plain step is run from the
+ * same starting state first and judges the warm-started step: whenever the
+ * warm-started residual (scGetResidual) exceeds the plain one, the plain
+ * result is kept (a verification mode: every step costs a plain and a
+ * warm-started step). With t >= 0 the warm-started step is rerun plain
+ * whenever its residual exceeds t; t = +infinity turns the judge off (the
+ * production mode: no snapshot, no residual, no plain step).
  * scGetResidual: RMS constraint residual of the current positions, FP64:
  * sqrt((sum_tri warp^2 + weft^2 + sum_bend (s theta)^2) / (2T + B)) with
  * warp, weft as in ClothPBD.cpp:78,89 and s theta the bend error of
@@ -434,9 +437,11 @@ SCEXP void scBatchSetUseGraph(scClothBatch batch, bool enable);
  * whose error after a substep is set by its compliance, is left out;
  * scGetResidualTerms writes the RMS of warp/weft, of shear and of the bend
  * errors separately.
- * scGetWarmStartStats: out[0] last step's residual, [1] threshold in use,
- * [2] fallbacks, [3] calibrations, [4] judged steps, [5] 1 if the last step
- * ran plain.
+ * scGetWarmStartStats (out holds 7 doubles): out[0] last step's residual,
+ * [1] bound in use (t, or the last plain step's residual), [2] fallbacks,
+ * [3] the last step's plain residual (-1 if none ran), [4] judged steps,
+ * [5] 1 if the last step's result is the plain one, [6] the last step's
+ * warm-started residual.
  * scGetWarmStartData: which 0 features (n x 12) of the last substep, 1 its
  * inertial positions (n x 3), 2 the last correction (n x 3), 3 training
  * targets and weights (n x 4), 4 (ell, free vertices); returns n (2 for 4).

@@ -405,10 +405,10 @@ class ClothModel:
         return out
 
     def warm_stats(self):
-        out = np.zeros(6)
+        out = np.zeros(7)
         _checked("scGetWarmStartStats", self.h, _ptr(out))
-        return dict(residual=out[0], threshold=out[1], fallbacks=int(out[2]), calibrations=int(out[3]),
-                    steps=int(out[4]), last_plain=bool(out[5]))
+        return dict(residual=out[0], threshold=out[1], fallbacks=int(out[2]), plain_residual=out[3],
+                    steps=int(out[4]), last_plain=bool(out[5]), warm_residual=out[6])
 
     def warm_data(self, which):
         """'features' (n,12), 'xhat' (n,3), 'corr' (n,3), 'targets' (n,4), 'scale' (ell, free count)."""

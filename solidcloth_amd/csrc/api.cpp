@@ -963,7 +963,6 @@ void scSetWarmStartFallback(scModel model, float threshold)
         if (std::isnan(threshold)) throw std::runtime_error("threshold must be a number");
         ClothModel* m = M(model);
         m->mWarmThreshold = threshold;
-        m->mWarmTau = -1.0;
     });
 }
 
@@ -976,11 +975,12 @@ void scGetWarmStartStats(scModel model, double* out)
     guard([&] {
         ClothModel* m = M(model);
         out[0] = m->mWarmResidual;
-        out[1] = m->mWarmThreshold >= 0.f ? (double)m->mWarmThreshold : m->mWarmTau;
+        out[1] = m->mWarmThreshold >= 0.f ? (double)m->mWarmThreshold : m->mWarmPlainResidual;
         out[2] = m->mWarmFallbacks;
-        out[3] = m->mWarmCalibrations;
+        out[3] = m->mWarmPlainResidual;
         out[4] = m->mWarmSteps;
         out[5] = m->mWarmLastPlain ? 1.0 : 0.0;
+        out[6] = m->mWarmWarmResidual;
     });
 }
 

@@ -299,6 +299,8 @@ void GpuSolver::release()
     dev_free(warm.xhat); dev_free(warm.corr); dev_free(warm.T); dev_free(warm.triStrain); dev_free(warm.X);
     dev_free(warm.adjOff); dev_free(warm.adj); dev_free(warm.vtOff); dev_free(warm.vt);
     dev_free(warm.snapPos); dev_free(warm.snapPrev); dev_free(warm.snapVel); dev_free(warm.snapCorr);
+    dev_free(warm.altPos); dev_free(warm.altPrev); dev_free(warm.altVel); dev_free(warm.altCorr);
+    warm.altNv = -1;
     dev_free(warm.resPartial); dev_free(warm.resOut);
     warm.nv = -1;
     dev_free(snap.pos); dev_free(snap.prev); dev_free(snap.vel); dev_free(snap.corr);
@@ -1255,7 +1257,8 @@ void ClothModel::step_substeps(float h, int n)
     // the plain solver judges a warm-started step: a step whose residual
     // exceeds the last plain step's (or the set threshold) is rerun plain
     // from the saved state, and the first step calibrates
-    const bool judge = mWarm && mWarmMode == 2;
+    // a threshold of +infinity: no judge (the warm-started step is kept as is)
+    const bool judge = mWarm && mWarmMode == 2 && !std::isinf(mWarmThreshold);
     // continuous self collision snapshots the step's start: a step whose
     // contacts or candidate pairs outgrew their buffers is rerun from the
     // snapshot with the buffers grown to the step's peaks, so that no step
@@ -1271,23 +1274,49 @@ void ClothModel::step_substeps(float h, int n)
         }
         if (judge) save_state();
         mWarmPlainNow = false;
-        run_substeps(h, n, cs, timed);
-        mWarmLastPlain = !judge;
-        if (judge) {
-            const double r = residual();
+        if (judge && mWarmThreshold < 0.f) {
+            // the plain step from the same start judges the warm-started one:
+            // its residual is the bound and its result is kept when the
+            // warm-started step leaves a larger residual
+            mWarmPlainNow = true;
+            run_substeps(h, n, cs, timed);
+            mWarmPlainNow = false;
+            const double rp = residual();
+            keep_plain_result();
+            restore_state();
+            run_substeps(h, n, cs, timed);
+            const double rw = residual();
             mWarmSteps++;
-            mWarmResidual = r;
-            const double tau = mWarmThreshold >= 0.f ? (double)mWarmThreshold : mWarmTau;
-            if (tau < 0.0 || !(r <= tau)) {
-                if (tau < 0.0) mWarmCalibrations++; else mWarmFallbacks++;
-                restore_state();
-                mWarmPlainNow = true;
-                run_substeps(h, n, cs, timed);
-                mWarmPlainNow = false;
+            mWarmPlainResidual = rp;
+            mWarmWarmResidual = rw;
+            if (!(rw <= rp)) {
+                restore_plain_result();
+                mWarmFallbacks++;
                 mWarmLastPlain = true;
-                const double rp = residual();
-                if (mWarmThreshold < 0.f) mWarmTau = rp;
                 mWarmResidual = rp;
+            } else {
+                mWarmLastPlain = false;
+                mWarmResidual = rw;
+            }
+        } else {
+            run_substeps(h, n, cs, timed);
+            mWarmLastPlain = !judge;
+            if (judge) {
+                // a fixed bound: a step above it is rerun plain
+                const double r = residual();
+                mWarmSteps++;
+                mWarmResidual = r;
+                mWarmWarmResidual = r;
+                mWarmPlainResidual = -1.0;
+                if (!(r <= (double)mWarmThreshold)) {
+                    mWarmFallbacks++;
+                    restore_state();
+                    mWarmPlainNow = true;
+                    run_substeps(h, n, cs, timed);
+                    mWarmPlainNow = false;
+                    mWarmLastPlain = true;
+                    mWarmResidual = mWarmPlainResidual = residual();
+                }
             }
         }
         if (timed) g.timer.harvest();

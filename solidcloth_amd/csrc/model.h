@@ -272,6 +272,8 @@ public:
         uint16_t* X = nullptr;
         int *adjOff = nullptr, *adj = nullptr, *vtOff = nullptr, *vt = nullptr;
         float4 *snapPos = nullptr, *snapPrev = nullptr, *snapVel = nullptr, *snapCorr = nullptr;
+        float4 *altPos = nullptr, *altPrev = nullptr, *altVel = nullptr, *altCorr = nullptr;  // plain judge's result
+        int altNv = -1;
         double *resPartial = nullptr, *resOut = nullptr;
         int resBlocks = 0, nv = -1, nFree = 0;
         float ell = 1.f;
@@ -428,9 +430,9 @@ public:
     // solve, 3 plain solve that only tracks the corrections (features stay current)
     WarmStart* mWarm = nullptr;
     int mWarmMode = 0, mWarmIters = 4;
-    float mWarmThreshold = -1.f;  // fallback residual; < 0: the last plain step's residual
-    double mWarmTau = -1.0, mWarmResidual = -1.0;
-    int mWarmFallbacks = 0, mWarmCalibrations = 0, mWarmSteps = 0;
+    float mWarmThreshold = -1.f;  // fallback residual; < 0: the plain step from the same start judges each step
+    double mWarmResidual = -1.0, mWarmPlainResidual = -1.0, mWarmWarmResidual = -1.0;
+    int mWarmFallbacks = 0, mWarmSteps = 0;
     bool mWarmResetCorr = true, mWarmPlainNow = false, mWarmLastPlain = false;
     void set_warm(WarmStart* ws, int mode, int iterations);
     void ensure_warm();
@@ -441,6 +443,8 @@ public:
     double residualSum = 0.0;
     void save_state();
     void restore_state();
+    void keep_plain_result();
+    void restore_plain_result();
     int warm_data(int which, float* out, int count);
     void run_substeps(float h, int n, const CollSet& cs, bool timed);
     CollSet make_collset(int& slots, int& edgeSlots, int& triSlots);

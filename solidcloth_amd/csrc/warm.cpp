@@ -205,9 +205,8 @@ void ClothModel::set_warm(WarmStart* ws, int mode, int iterations)
     mWarm = mode ? ws : nullptr;
     mWarmMode = mode;
     mWarmIters = iterations;
-    mWarmTau = -1.0;
-    mWarmResidual = -1.0;
-    mWarmFallbacks = mWarmCalibrations = mWarmSteps = 0;
+    mWarmResidual = mWarmPlainResidual = mWarmWarmResidual = -1.0;
+    mWarmFallbacks = mWarmSteps = 0;
 }
 
 void ClothModel::ensure_warm()
@@ -329,6 +328,34 @@ void ClothModel::save_state()
     launch_copy(g.stream, g.warm.corr, g.warm.snapCorr, g.nv);
     HIP_OK(hipStreamSynchronize(g.stream));
     g.warm.saved = true;
+}
+
+// The plain judge's result (positions, previous positions, velocities, the
+// warm start's correction), kept while the warm-started step runs.
+void ClothModel::keep_plain_result()
+{
+    GpuSolver& g = gpu;
+    GpuSolver::Warm& w = g.warm;
+    if (w.altNv != g.nv) {
+        dev_alloc(w.altPos, std::max(g.nv, 1)); dev_alloc(w.altPrev, std::max(g.nv, 1));
+        dev_alloc(w.altVel, std::max(g.nv, 1)); dev_alloc(w.altCorr, std::max(g.nv, 1));
+        w.altNv = g.nv;
+    }
+    launch_copy(g.stream, g.d_pos, w.altPos, g.nv);
+    launch_copy(g.stream, g.d_prev, w.altPrev, g.nv);
+    launch_copy(g.stream, g.d_vel, w.altVel, g.nv);
+    launch_copy(g.stream, w.corr, w.altCorr, g.nv);
+}
+
+void ClothModel::restore_plain_result()
+{
+    GpuSolver& g = gpu;
+    GpuSolver::Warm& w = g.warm;
+    launch_copy(g.stream, w.altPos, g.d_pos, g.nv);
+    launch_copy(g.stream, w.altPrev, g.d_prev, g.nv);
+    launch_copy(g.stream, w.altVel, g.d_vel, g.nv);
+    launch_copy(g.stream, w.altCorr, w.corr, g.nv);
+    HIP_OK(hipStreamSynchronize(g.stream));
 }
 
 void ClothModel::restore_state()

@@ -44,17 +44,19 @@ def training_plan():
     return plan
 
 
-def train(plan=None, seed=7, lr=1e-3, lr_final=3e-4, log=None, use_graph=True):
-    """Trains a fresh network over the plan; returns (network, substeps, seconds)."""
+def train(plan=None, seed=7, lr=1e-3, lr_final=3e-4, log=None, use_graph=True, epochs=1):
+    """Trains a fresh network over the plan, `epochs` times (the plain
+    trajectories are deterministic, so every epoch sees the same substeps);
+    returns (network, substeps per epoch, seconds)."""
     import solidcloth_amd as sc
     from . import scenes as S
 
     plan = training_plan() if plan is None else plan
     ws = sc.WarmStart(seed)
-    total = sum(f * s.substeps for s, f in plan)
+    total = epochs * sum(f * s.substeps for s, f in plan)
     done = 0
     t0 = time.perf_counter()
-    for k, (scene, frames) in enumerate(plan):
+    for k, (scene, frames) in enumerate(list(plan) * epochs):
         world, patch = S.build_patch(scene, sc)
         m = patch.GetModel()
         m.SetUseGraph(use_graph)
@@ -70,10 +72,10 @@ def train(plan=None, seed=7, lr=1e-3, lr_final=3e-4, log=None, use_graph=True):
                 break
             done += scene.substeps
         if log:
-            print(f"warm-start training: scene {k + 1}/{len(plan)} ({scene.name}, n={scene.n}) done, "
+            print(f"warm-start training: scene {k + 1}/{len(plan) * epochs} ({scene.name}, n={scene.n}) done, "
                   f"{done} substeps, {time.perf_counter() - t0:.1f} s", file=log, flush=True)
         del patch, world
-    return ws, done, time.perf_counter() - t0
+    return ws, done // epochs, time.perf_counter() - t0
 
 
 def load(path=DEFAULT_WEIGHTS, seed=0):
@@ -92,15 +94,17 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--epochs", type=int, default=3, help="passes over the same trajectories")
     ap.add_argument("--frames-scale", type=float, default=1.0,
                     help="fraction of each scene's frames (shorter runs, e.g. under a profiler)")
     ap.add_argument("--no-graph", action="store_true", help="launch kernels directly instead of graph replay")
     args = ap.parse_args()
     plan = [(s, max(1, int(round(f * args.frames_scale)))) for s, f in training_plan()]
-    ws, n, secs = train(plan=plan, seed=args.seed, log=sys.stdout, use_graph=not args.no_graph)
+    ws, n, secs = train(plan=plan, seed=args.seed, log=sys.stdout, use_graph=not args.no_graph,
+                          epochs=args.epochs)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     np.save(args.out, ws.params())
-    print(f"trained on {n} substeps in {secs:.1f} s; {ws.steps()} Adam steps; saved {args.out}", flush=True)
+    print(f"trained on {n} substeps x {args.epochs} epochs in {secs:.1f} s; {ws.steps()} Adam steps; saved {args.out}", flush=True)
 
 
 if __name__ == "__main__":

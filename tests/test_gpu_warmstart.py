@@ -178,18 +178,44 @@ def test_fallback_reruns_the_plain_step_bit_for_bit(sc):
     assert np.isfinite(mf.positions()).all()
 
 
-def test_automatic_threshold_calibrates_on_the_first_step(sc):
+def test_plain_judge_keeps_the_better_step(sc):
+    """The default judge (scSetWarmStartFallback < 0) runs the plain step from
+    the same start and keeps the warm-started result only when its residual
+    is not larger: step by step, from one saved state, the judged step's
+    positions equal the plain step's or the unjudged warm-started step's,
+    as their residuals decide."""
     scene = _small()
+    ws = sc.WarmStart(0)  # untrained: the plain solver wins some steps
     _, p = S.build_patch(scene, sc)
     m = p.GetModel()
-    m.SetWarmStart(sc.WarmStart(0), "solve", 4)
-    p.Step(scene.dt)
-    st = m.warm_stats()
-    assert st["calibrations"] == 1 and st["last_plain"] and st["threshold"] == pytest.approx(st["residual"])
+    m.SetWarmStart(ws, "track")
     for _ in range(3):
         p.Step(scene.dt)
-    st = m.warm_stats()
-    assert st["steps"] == 4 and st["calibrations"] == 1
+    kept = fell = 0
+    for step in range(6):
+        m.save_state()
+        m.SetWarmStart(ws, "track")
+        p.Step(scene.dt)
+        pp, rp = m.positions(), m.residual()
+        m.restore_state()
+        m.SetWarmStart(ws, "solve", 4)
+        m.SetWarmStartFallback(float("inf"))  # no judge: the warm-started step as it comes
+        p.Step(scene.dt)
+        pw, rw = m.positions(), m.residual()
+        m.restore_state()
+        m.SetWarmStartFallback(-1.0)
+        p.Step(scene.dt)
+        st = m.warm_stats()
+        assert st["steps"] == 1
+        assert st["plain_residual"] == pytest.approx(rp, rel=1e-12)
+        assert st["warm_residual"] == pytest.approx(rw, rel=1e-12)
+        if rw <= rp:
+            kept += 1
+            assert not st["last_plain"] and st["fallbacks"] == 0 and np.array_equal(m.positions(), pw), step
+        else:
+            fell += 1
+            assert st["last_plain"] and st["fallbacks"] == 1 and np.array_equal(m.positions(), pp), step
+    assert kept + fell == 6
 
 
 def _residual_after(patch, dt, setup, out):
