@@ -17,8 +17,15 @@ from those events, and achieved bandwidth is its algorithmic bytes per launch
 (136 B per FEM triangle, 128 B per bend) over that duration. A rocprofv3
 kernel trace of `bench.py --no-graph --no-cpu --copy-probe-mb 0` with the same
 --steps/--warmup sees exactly these frames twice, so its per-kernel averages
-are directly comparable. cpu_baseline: the CPU oracle (a single-threaded
-restatement of the reference solver) on a bounded sample of the same workload.
+are directly comparable. cpu_baseline: the CPU oracle (the restatement of the
+reference solver: serial Gauss-Seidel, OpenMP detection) on a bounded sample
+of the same workload, at the host's thread count and at one thread, with the
+CPU model.
+
+mesh512: the self-colliding 512x512 cloth over all its 300 frames on the GPU,
+with two CPU baselines: the algorithm-matched path (the oracle's grid broad
+phase + the reference's CCD) and the reference's live brute-force self path
+(sampled queries of substep 1, scaled), and the GPU/CPU ratios against 50x.
 
 roofline.traffic: HBM bytes per launch of the dominant kernel from the
 committed rocprofv3 counter summary profiles/drape256_traffic.md (FETCH_SIZE
@@ -368,26 +375,68 @@ def render_sync(patch, with_cpu, calls=20):
     return out
 
 
-def cpu_baseline(scene, budget_s):
+def host_cpu():
+    """The host's CPU model and the thread count the CPU baselines use: the
+    OpenMP thread count the environment sets (the host's share of cores), else
+    every core this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else avail
+    return {"model": model, "logical_cpus": os.cpu_count(), "available_cpus": avail,
+            "threads": max(1, min(threads, avail))}
+
+
+def _oracle_substeps(o, h, budget_s, limit=50):
+    """Times substeps of an oracle cloth until the budget is spent (at least one)."""
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step(h)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= limit:
+            return n / el, n
+
+
+def cpu_baseline(scene, budget_s, cpu):
+    """drape256 on the oracle from the initial state, at the host's thread
+    count and at one thread (Gauss-Seidel is serial, as the reference's; the
+    detection loops use OpenMP)."""
     import oracle
 
-    o = oracle.from_scene(scene)
-    # one frame to estimate, then as many frames as fit the budget
-    t0 = time.perf_counter()
-    o.step(scene.dt)
-    one = time.perf_counter() - t0
-    frames = max(1, min(50, int(budget_s / max(one, 1e-6))))
-    t0 = time.perf_counter()
-    for _ in range(frames):
-        o.step(scene.dt)
-    el = time.perf_counter() - t0
-    return frames * scene.substeps / el, frames
+    h = scene.dt / scene.substeps
+    out = {}
+    for threads in (cpu["threads"], 1):
+        oracle.set_threads(threads)
+        o = oracle.from_scene(scene)
+        o.set(numSteps=1)
+        out[threads] = _oracle_substeps(o, h, budget_s / 2)
+    oracle.set_threads(1)
+    (vn, nn), (v1, n1) = out[cpu["threads"]], out[1]
+    return {"value": round(vn, 3), "unit": "substeps/s", "cores": cpu["threads"], "kind": "port",
+            "sample": f"{nn} substeps of drape256 from the initial state, oracle restatement (serial Gauss-Seidel, "
+                      f"OpenMP detection) at {cpu['threads']} threads",
+            "single_thread": {"value": round(v1, 3), "sample": f"{n1} substeps from the initial state"},
+            "cpu_model": cpu["model"], "logical_cpus": cpu["logical_cpus"], "available_cpus": cpu["available_cpus"]}
 
 
-def cpu_baseline_from_state(scene, pos, vel, budget_s):
-    """Substeps/s of the oracle with its grid broad phases (the same contacts
-    as its exhaustive scans, proven by the CPU tests), started from the state
-    the GPU run ended in, so that the sample has the timed region's contacts."""
+def ccd_path_baseline(scene, pos, vel, budget_s, cpu):
+    """mesh512 on the oracle with its grid broad phases (the same contacts as
+    its exhaustive scans) and the reference's CCD, from the state the GPU run
+    ended in, so that the sample has the timed region's contacts; at the
+    host's thread count, then at one thread continuing from there."""
     import oracle
 
     o = oracle.from_scene(scene)
@@ -396,16 +445,134 @@ def cpu_baseline_from_state(scene, pos, vel, budget_s):
     o.put("prev", pos)
     o.put("vel", vel)
     h = scene.dt / scene.substeps
+    res = {}
+    for threads in (cpu["threads"], 1):
+        oracle.set_threads(threads)
+        rate, n = _oracle_substeps(o, h, budget_s / 2, limit=20)
+        c = o.counts()["C"] + len(o.self_contacts("vt")[0]) + len(o.self_contacts("ee")[0])
+        res[threads] = (rate, n, c)
+    oracle.set_threads(1)
+    (vn, nn, cn), (v1, n1, c1) = res[cpu["threads"]], res[1]
+    return {"value": round(vn, 4), "unit": "substeps/s", "cores": cpu["threads"], "kind": "port",
+            "sample": f"{nn} substeps of mesh512 from the state the GPU run ended in ({cn} contacts in the last), "
+                      f"oracle restatement with grid broad phases and the reference's CCD at {cpu['threads']} "
+                      "threads",
+            "single_thread": {"value": round(v1, 4), "sample": f"{n1} further substeps ({c1} contacts in the last)"}}
+
+
+LIVE_SAMPLE = (64, 128)  # vertex and edge queries timed per thread count
+
+
+def live_path_baseline(pos0, tris, rad_tol, cpu):
+    """The reference's live self path at substep 1 (brute force over every
+    triangle per vertex and every later edge per edge; ClosestPointsBetweenMeshes
+    .cpp:538-592), timed on evenly spaced vertex and edge queries and scaled to
+    all of them: per-vertex cost is the same for every vertex, and an edge's
+    cost falls linearly with its index, so midpoint-spaced edges sum to the
+    whole. The compiled reference's own functions in the reference's OpenMP
+    loop structure (oracle/_ref) when present, else the restatement."""
+    import ctypes as C
+
+    import numpy as np
+
+    import oracle
+
+    v = np.ascontiguousarray(pos0, dtype=np.float32)
+    idx = np.ascontiguousarray(tris, dtype=np.uint32).ravel()
+    nv, nt = len(v), len(idx) // 3
+    ne = len(oracle.construct_edges(idx))
+    kv, ke = LIVE_SAMPLE
+    vq = ((np.arange(kv) + 0.5) * nv / kv).astype(np.int32)
+    eq = ((np.arange(ke) + 0.5) * ne / ke).astype(np.int32)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libref_geometry.so")
+    runs = {}
+    kind = "reference" if os.path.exists(ref_path) else "port"
+    if kind == "reference":
+        R = C.CDLL(ref_path)
+        for threads in (cpu["threads"], 1):
+            R.ref_set_threads(C.c_int(threads))
+            cnt = np.zeros(2, np.int32)
+            sec = np.zeros(2, np.float64)
+            R.ref_live_self_sample(p(v), C.c_int(nv), p(idx), C.c_int(idx.size), C.c_float(rad_tol), p(vq),
+                                   C.c_int(kv), p(eq), C.c_int(ke), p(cnt), p(sec))
+            runs[threads] = (sec[0], sec[1])
+    live = oracle.LiveSelf(v, idx)
+    port = {}
+    for threads in ((cpu["threads"], 1) if kind == "port" else (1,)):
+        oracle.set_threads(threads)
+        t0 = time.perf_counter()
+        live.vertex_queries(rad_tol, vq)
+        t1 = time.perf_counter()
+        live.edge_queries(eq)
+        port[threads] = (t1 - t0, time.perf_counter() - t1)
+    oracle.set_threads(1)
+    if kind == "port":
+        runs = port
+
+    def per_substep(t):
+        return t[0] * nv / kv + t[1] * ne / ke
+
+    sn, s1 = per_substep(runs[cpu["threads"]]), per_substep(runs[1])
+    out = {"value": 1.0 / sn, "unit": "substeps/s", "cores": cpu["threads"], "kind": kind,
+           "seconds_per_substep": round(sn, 1),
+           "sample": f"substep 1 of mesh512 (radius {rad_tol:.4f}): {kv} of {nv} vertex queries and {ke} of {ne} "
+                     f"edge queries, scaled to all; {'compiled reference functions' if kind == 'reference' else 'restatement'}"
+                     f" at {cpu['threads']} threads",
+           "single_thread": {"seconds_per_substep": round(s1, 1)}}
+    if kind == "reference":
+        out["restatement_single_thread_seconds_per_substep"] = round(per_substep(port[1]), 1)
+    return out
+
+
+def mesh512_record(args, cpu):
+    """The self-colliding 512^2 cloth: all configured frames on the GPU, then
+    the CPU baselines of the algorithm-matched CCD path and of the reference's
+    live self path, and the GPU / CPU ratios against the 50x target."""
+    import solidcloth_amd as sc
+    from solidcloth_amd import scenes as S
+
+    scene = S.MESH512
+    world, patch = S.build_patch(scene, sc)
+    m = patch.GetModel()
+    m.SetUseGraph(not args.no_graph)
+    m.SetSweep(args.sweep)
+    pos0 = m.positions().copy()
     t0 = time.perf_counter()
-    o.step(h)
-    one = time.perf_counter() - t0
-    n = max(1, min(20, int(budget_s / max(one, 1e-6)) - 1))
+    patch.Step(scene.dt)
+    first = time.perf_counter() - t0
     t0 = time.perf_counter()
-    for _ in range(n):
-        o.step(h)
+    for f in range(1, scene.frames):
+        patch.Step(scene.dt)
+        if f % 50 == 0:
+            print(f"mesh512: frame {f}/{scene.frames}, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     el = time.perf_counter() - t0
-    c = o.counts()["C"] + len(o.self_contacts("vt")[0]) + len(o.self_contacts("ee")[0])
-    return n / el, n, c
+    value = (scene.frames - 1) * scene.substeps / el
+    st = m.self_stats()
+    pos = m.positions()
+    out = {"workload": WORKLOADS["mesh512"], "frames": scene.frames, "value": round(value, 3), "unit": "substeps/s",
+           "timed": f"frames 2-{scene.frames} (frame 1, with graph capture, took {1000 * first:.0f} ms)",
+           "ms_per_frame": round(1000.0 * el / (scene.frames - 1), 3),
+           "all_positions_finite": bool(__import__("numpy").isfinite(pos).all()),
+           "self_collision": st}
+    if args.no_cpu:
+        return out
+    tris = m.triangles()
+    ccd = ccd_path_baseline(scene, pos.astype("float64"), m.velocities().astype("float64"), args.cpu_budget, cpu)
+    # the live path at substep 1: the mesh is the initial positions, radius
+    # thickness + max(tolerance, 1.5 x the largest first prediction step)
+    h = scene.dt / scene.substeps
+    rad = scene.thickness + max(scene.tolerance, 1.5 * 9.8 * h * h)
+    live = live_path_baseline(pos0, tris, rad, cpu)
+    out["cpu_baseline"] = {"ccd_path": ccd, "live_path": live, "cpu_model": cpu["model"],
+                           "logical_cpus": cpu["logical_cpus"], "available_cpus": cpu["available_cpus"]}
+    rc, rl = value / ccd["value"], value / live["value"]
+    out["gpu_over_cpu"] = {"ccd_path": round(rc, 1), "live_path": round(rl), "target": 50,
+                           "met": bool(rc >= 50 and rl >= 50),
+                           "ccd_path_single_thread": round(value / ccd["single_thread"]["value"], 1),
+                           "live_path_single_thread": round(value * live["single_thread"]["seconds_per_substep"])}
+    live["value"] = round(live["value"], 8)
+    return out
 
 
 def main():
@@ -424,8 +591,10 @@ def main():
                     help="size of the streaming-copy bandwidth probe (0 disables)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
     ap.add_argument("--workload", default="drape256", choices=["drape256", "mesh512", "batch512"],
-                    help="drape256: the metric's workload; mesh512: the triangle-mesh obstacle path "
-                         "(reported beside it, not the metric)")
+                    help="drape256: the metric's workload; mesh512: the self-colliding triangle-mesh obstacle "
+                         "path alone (its record is also part of the drape256 line)")
+    ap.add_argument("--no-mesh512", action="store_true",
+                    help="leave the self-colliding 512^2 record (300 frames + CPU baselines) out of the line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only "
@@ -437,7 +606,16 @@ def main():
     if args.workload == "batch512":
         run_batch(args)
         return
-    scene = S.DRAPE256 if args.workload == "drape256" else S.MESH512
+    cpu = host_cpu()
+    if args.workload == "mesh512":
+        rec = mesh512_record(args, cpu)
+        print(json.dumps({"metric": "PBD substeps/sec, mesh512 (self-colliding 512x512 cloth, all frames)",
+                          "value": rec["value"], "unit": "substeps/s", "n_gpus": 1, "steps": S.MESH512.frames,
+                          "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                          "dtype": "f32", "data": "synthetic", "config": {"workload": rec["workload"]},
+                          "mesh512": rec}))
+        return
+    scene = S.DRAPE256
     elapsed, stats, keep = run_gpu(scene, args.steps, args.warmup, use_graph=not args.no_graph, sweep=args.sweep)
     m = keep[2]
     event_frames = args.warmup + args.steps
@@ -519,31 +697,10 @@ def main():
         out["render_mesh_sync"] = render_sync(keep[1], not args.no_cpu)
     if args.workload == "drape256" and not args.no_learned:
         out["learned_vs_plain"] = learned_vs_plain(scene)
-    if args.workload != "drape256":
-        out["metric"] = "PBD substeps/sec, " + args.workload
-        out["cpu_baseline"] = None
-        if not args.no_cpu:
-            pos = m.positions().astype("float64")
-            vel = m.velocities().astype("float64")
-            cpu_val, n, contacts = cpu_baseline_from_state(scene, pos, vel, args.cpu_budget)
-            out["cpu_baseline"] = {
-                "value": round(cpu_val, 4),
-                "unit": "substeps/s",
-                "cores": 1,
-                "kind": "port",
-                "sample": f"{n} substeps of mesh512 from the state the GPU run ended in ({contacts} contacts "
-                          "in the last), single-threaded oracle restatement with grid broad phases",
-            }
-    elif not args.no_cpu:
-        cpu_val, frames = cpu_baseline(scene, args.cpu_budget)
-        out["cpu_baseline"] = {
-            "value": round(cpu_val, 3),
-            "unit": "substeps/s",
-            "cores": 1,
-            "kind": "port",
-            "sample": f"{frames} frames ({frames * scene.substeps} substeps) of drape256 from the initial state, "
-                      "single-threaded oracle restatement",
-        }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_budget, cpu)
+    if not args.no_mesh512:
+        out["mesh512"] = mesh512_record(args, cpu)
     print(json.dumps(out))
 
 

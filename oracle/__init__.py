@@ -104,8 +104,71 @@ def lib():
         L.oc_signed_volume.restype = _C.c_float
         L.oc_mesh_normals.argtypes = [_C.c_int, vp, _C.c_int, vp, vp]
         L.oc_set_mouse_spring.argtypes = [vp] + [_C.c_int] * 4 + [_C.c_float] * 6
+        L.oc_set_threads.argtypes = [_C.c_int]
+        L.oc_get_threads.restype = _C.c_int
+        L.ol_create.restype = vp
+        L.ol_create.argtypes = [_C.c_int, vp, _C.c_int, vp]
+        L.ol_free.argtypes = [vp]
+        L.ol_counts.argtypes = [vp, vp]
+        L.ol_vertex_queries.argtypes = [vp, _C.c_float, vp, _C.c_int, vp, vp, _C.c_long]
+        L.ol_vertex_queries.restype = _C.c_long
+        L.ol_edge_queries.argtypes = [vp, vp, _C.c_int, vp, vp, _C.c_long]
+        L.ol_edge_queries.restype = _C.c_long
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """OpenMP threads for the oracle's detection loops (results do not depend on it)."""
+    lib().oc_set_threads(int(n))
+
+
+def threads():
+    return int(lib().oc_get_threads())
+
+
+class LiveSelf:
+    """The reference's live self-collision detection on the cloth's own mesh
+    (MeshClosestPoints::ClosestPoints self, brute force; live_self.cpp), for
+    timing and parity. VT infos: ids (vtx, tri, region, regionType, feature)
+    and data (distance, point 3, barycentric 3, normal 3, side); EE infos: ids
+    (edge1, edge, region) and data (distance, coordsSegm 2, coordsMesh 2,
+    point on the segment 3, point on the mesh 3, normal 3, side)."""
+
+    def __init__(self, vertices, indices):
+        self.L = lib()
+        v = _np.ascontiguousarray(vertices, dtype=_np.float32).reshape(-1, 3)
+        idx = _np.ascontiguousarray(indices, dtype=_np.uint32).ravel()
+        self.h = _C.c_void_p(self.L.ol_create(len(v), _ptr(v), idx.size, _ptr(idx)))
+        c = _np.zeros(3, dtype=_np.int32)
+        self.L.ol_counts(self.h, _ptr(c))
+        self.nv, self.nt, self.ne = (int(x) for x in c)
+
+    def __del__(self):
+        try:
+            self.L.ol_free(self.h)
+        except Exception:
+            pass
+
+    @staticmethod
+    def _query(call, items, nid, ndata):
+        q = None if items is None else _np.ascontiguousarray(items, dtype=_np.int32)
+        qp, nq = (None, 0) if q is None else (_ptr(q), q.size)
+        cap = 0
+        while True:
+            ids = _np.zeros((max(cap, 1), nid), dtype=_np.int32)
+            data = _np.zeros((max(cap, 1), ndata), dtype=_np.float32)
+            n = call(qp, nq, _ptr(ids), _ptr(data), cap)
+            if n <= cap:
+                return ids[:n].copy(), data[:n].copy()
+            cap = n
+
+    def vertex_queries(self, rad_tol, vertices=None):
+        return self._query(lambda qp, nq, i, d, cap: self.L.ol_vertex_queries(self.h, rad_tol, qp, nq, i, d, cap),
+                           vertices, 5, 11)
+
+    def edge_queries(self, edges=None):
+        return self._query(lambda qp, nq, i, d, cap: self.L.ol_edge_queries(self.h, qp, nq, i, d, cap), edges, 3, 15)
 
 
 def construct_edges(indices):

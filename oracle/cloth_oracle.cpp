@@ -13,6 +13,9 @@
 //     compiled reference exactly (pinned by tests/golden).
 //   * Contacts are generated in collidable-major, vertex-minor order, i.e. the
 //     reference's order with OMP_NUM_THREADS=1 (ClothCollision.cpp:440-457).
+//     The detection loops run on OpenMP threads as the reference's do, in
+//     blocks whose outputs are appended in block order, so the lists are the
+//     same for every thread count; the solver is serial, as ClothLib's is.
 //   * Triangle::invDet is left uninitialised by the reference when det == 0
 //     (ClothModel.cpp:67-68); here it is 0.
 //   * The Gauss-Seidel loop may be given a constraint permutation (the GPU's
@@ -21,6 +24,8 @@
 //     mode because the link error stays 0.
 #include "oracle_geom.h"
 #include "oracle_vec.h"
+
+#include <omp.h>
 
 #include <algorithm>
 #include <cfloat>
@@ -36,6 +41,26 @@ namespace oracle {
 
 enum { CF_VERTICES = 1, CF_TRIANGLES = 2, CF_SELF = 4, CF_WALLS = 8, CF_EDGES = 16 };
 enum { CT_WALLS = 1, CT_SPHERE = 3, CT_MESH = 4, CT_CAPSULE = 6, CT_SDF = 7 };
+
+// Runs body(begin, end, part) over consecutive blocks of [0, n) on OpenMP
+// threads, each block writing its own part, and returns the parts in block
+// order: appending them in turn gives the serial loop's output for any
+// thread count.
+template <class Part, class F>
+std::vector<Part> ordered_blocks(size_t n, F body)
+{
+    const size_t blocks = std::min<size_t>(n, 1024);
+    std::vector<Part> parts(blocks);
+#pragma omp parallel for schedule(dynamic, 1) if (n >= 512)
+    for (long b = 0; b < (long)blocks; b++) body(n * b / blocks, n * (b + 1) / blocks, parts[b]);
+    return parts;
+}
+
+template <class T>
+void append_parts(std::vector<T>& out, std::vector<std::vector<T>>& parts)
+{
+    for (auto& p : parts) out.insert(out.end(), p.begin(), p.end());
+}
 
 struct Params {
     // defaults: Physics/ClothModel.cpp:10-36
@@ -554,6 +579,13 @@ public:
     }
 
     // Physics/ClothModel.inl:33-47
+    // a vertex contact found by a parallel detection loop, added in order
+    struct Hit {
+        size_t i;
+        V3<R> p, n;
+        int feature;
+    };
+
     void add_contact(size_t idx, V3<R> p, V3<R> n, int ci)
     {
         Contact<R> c;
@@ -624,13 +656,17 @@ public:
     void sdf_vertex_collisions(Q query, int ci)
     {
         if (!(prm.collFlags & CF_VERTICES)) return;
-        for (size_t i = 0; i < pos.size(); i++) {
-            V3<R> p, n;
-            R dist = query(prev[i], p, n);
-            R tv = length(prev[i] - pos[i]);
-            R radTol = R(prm.thickness) + smax(R(prm.tolerance), tv * R(1.1f));
-            if (dist < radTol && dist >= R(0)) add_contact(i, p, n, ci);
-        }
+        auto parts = ordered_blocks<std::vector<Hit>>(pos.size(), [&](size_t b, size_t e, std::vector<Hit>& out) {
+            for (size_t i = b; i < e; i++) {
+                V3<R> p, n;
+                R dist = query(prev[i], p, n);
+                R tv = length(prev[i] - pos[i]);
+                R radTol = R(prm.thickness) + smax(R(prm.tolerance), tv * R(1.1f));
+                if (dist < radTol && dist >= R(0)) out.push_back({i, p, n, -1});
+            }
+        });
+        for (auto& part : parts)
+            for (const Hit& h : part) add_contact(h.i, h.p, h.n, ci);
     }
 
     // Physics/ClothCollisionSDF.h:26-41 (SphereSDF::QuerySegment)
@@ -674,43 +710,52 @@ public:
         auto tv = [&](uint32_t i) { return length(prev[i] - pos[i]); };
         if (prm.collFlags & CF_EDGES) {
             const R eps = R(0.02f);
-            for (size_t e = 0; e < edges.size(); e++) {
-                const uint32_t i1 = edges[e].i1, i2 = edges[e].i2;
-                V3<R> cp, n;
-                R w1 = 0, w2 = 0;
-                R dist;
-                if (!seg(prev[i1], prev[i2], cp, n, w1, w2, dist)) continue;
-                const R t = smax(tv(i1), tv(i2));
-                const R radTol = R(prm.thickness) + smax(R(prm.tolerance), t * R(1.1f));
-                if (dist < radTol && dist >= R(0) &&
-                    ((w1 > eps && w1 < R(1.f) - eps && w2 > eps && w2 < R(1.f) - eps) || handleBorder)) {
-                    EdgeContact<R> ec;
-                    ec.i1 = i1; ec.i2 = i2;
-                    ec.clothEdge = (int)e; ec.feature = -1; ec.mesh = -1;
-                    ec.point = cp; ec.normal = n;
-                    ec.w1 = w1; ec.w2 = w2;
-                    edgeContacts.push_back(ec);
-                }
-            }
+            auto parts = ordered_blocks<std::vector<EdgeContact<R>>>(
+                edges.size(), [&](size_t b, size_t end, std::vector<EdgeContact<R>>& out) {
+                    for (size_t e = b; e < end; e++) {
+                        const uint32_t i1 = edges[e].i1, i2 = edges[e].i2;
+                        V3<R> cp, n;
+                        R w1 = 0, w2 = 0;
+                        R dist;
+                        if (!seg(prev[i1], prev[i2], cp, n, w1, w2, dist)) continue;
+                        const R t = smax(tv(i1), tv(i2));
+                        const R radTol = R(prm.thickness) + smax(R(prm.tolerance), t * R(1.1f));
+                        if (dist < radTol && dist >= R(0) &&
+                            ((w1 > eps && w1 < R(1.f) - eps && w2 > eps && w2 < R(1.f) - eps) || handleBorder)) {
+                            EdgeContact<R> ec;
+                            ec.i1 = i1; ec.i2 = i2;
+                            ec.clothEdge = (int)e; ec.feature = -1; ec.mesh = -1;
+                            ec.point = cp; ec.normal = n;
+                            ec.w1 = w1; ec.w2 = w2;
+                            out.push_back(ec);
+                        }
+                    }
+                });
+            append_parts(edgeContacts, parts);
         }
         handleBorder = handleBorder || (prm.collFlags & CF_EDGES) == 0;
         if (prm.collFlags & CF_TRIANGLES) {
             const R eps = R(0.001f);
-            for (size_t j = 0; j < tris.size(); j++) {
-                const uint32_t i1 = tris[j].i1, i2 = tris[j].i2, i3 = tris[j].i3;
-                V3<R> cp, n, bar;
-                const R dist = tri(prev[i1], prev[i2], prev[i3], cp, n, bar);
-                const R t = smax(tv(i1), smax(tv(i2), tv(i3)));
-                const R radTol = R(prm.thickness) + smax(R(prm.tolerance), t * R(1.1f));
-                if (dist < radTol && dist >= R(0) && ((bar.x > eps && bar.y > eps && bar.z > eps) || handleBorder)) {
-                    TriContact<R> tc;
-                    tc.i1 = i1; tc.i2 = i2; tc.i3 = i3;
-                    tc.clothTri = (int)j; tc.feature = -1; tc.mesh = -1;
-                    tc.point = cp; tc.normal = n;
-                    tc.w1 = bar.x; tc.w2 = bar.y; tc.w3 = bar.z;
-                    triContacts.push_back(tc);
-                }
-            }
+            auto parts = ordered_blocks<std::vector<TriContact<R>>>(
+                tris.size(), [&](size_t b, size_t end, std::vector<TriContact<R>>& out) {
+                    for (size_t j = b; j < end; j++) {
+                        const uint32_t i1 = tris[j].i1, i2 = tris[j].i2, i3 = tris[j].i3;
+                        V3<R> cp, n, bar;
+                        const R dist = tri(prev[i1], prev[i2], prev[i3], cp, n, bar);
+                        const R t = smax(tv(i1), smax(tv(i2), tv(i3)));
+                        const R radTol = R(prm.thickness) + smax(R(prm.tolerance), t * R(1.1f));
+                        if (dist < radTol && dist >= R(0) &&
+                            ((bar.x > eps && bar.y > eps && bar.z > eps) || handleBorder)) {
+                            TriContact<R> tc;
+                            tc.i1 = i1; tc.i2 = i2; tc.i3 = i3;
+                            tc.clothTri = (int)j; tc.feature = -1; tc.mesh = -1;
+                            tc.point = cp; tc.normal = n;
+                            tc.w1 = bar.x; tc.w2 = bar.y; tc.w3 = bar.z;
+                            out.push_back(tc);
+                        }
+                    }
+                });
+            append_parts(triContacts, parts);
         }
     }
 
@@ -822,10 +867,9 @@ public:
         }
         const R radTol = R(prm.thickness) + smax(R(prm.tolerance), maxTv * R(1.5f));
         const size_t nt = M.idx.size() / 3;
-        std::vector<int> cand;
         // the features whose boxes meet the box of radius r around [a, b]
         // (every feature nearer than r does), or all of them
-        auto candidates = [&](const BoxGrid& g, size_t count, V3<R> a, V3<R> b, R r) {
+        auto candidates = [&](std::vector<int>& cand, const BoxGrid& g, size_t count, V3<R> a, V3<R> b, R r) {
             cand.clear();
             if (!prm.accel) {
                 for (size_t k = 0; k < count; k++) cand.push_back((int)k);
@@ -842,36 +886,44 @@ public:
             g.query(lo, hi, cand);
         };
         if (prm.collFlags & CF_VERTICES) {
-            for (size_t i = 0; i < n; i++) {
-                const V3<R> q = prev[i];
-                R best = R(FLT_MAX);
-                int tri = -1;
-                V3<R> bcp{0, 0, 0}, bn{0, 0, 0};
-                candidates(M.triGrid, nt, q, q, radTol);
-                for (const int ct : cand) {
-                    const size_t t = (size_t)ct;
-                    V3<R> pt, bar;
-                    closest_pt_point_triangle(q, vR(M.v[M.idx[3 * t]]), vR(M.v[M.idx[3 * t + 1]]),
-                                              vR(M.v[M.idx[3 * t + 2]]), pt, bar);
-                    V3<R> delta = q - pt;
-                    R d = length(delta);
-                    if (d == R(0)) continue;
-                    if (d < best) {
-                        best = d;
-                        tri = (int)t;
-                        bcp = pt;
-                        bn = (R(1.f) / d) * delta;
+            auto parts = ordered_blocks<std::vector<Hit>>(n, [&](size_t b, size_t end, std::vector<Hit>& out) {
+                std::vector<int> cand;
+                for (size_t i = b; i < end; i++) {
+                    const V3<R> q = prev[i];
+                    R best = R(FLT_MAX);
+                    int tri = -1;
+                    V3<R> bcp{0, 0, 0}, bn{0, 0, 0};
+                    candidates(cand, M.triGrid, nt, q, q, radTol);
+                    for (const int ct : cand) {
+                        const size_t t = (size_t)ct;
+                        V3<R> pt, bar;
+                        closest_pt_point_triangle(q, vR(M.v[M.idx[3 * t]]), vR(M.v[M.idx[3 * t + 1]]),
+                                                  vR(M.v[M.idx[3 * t + 2]]), pt, bar);
+                        V3<R> delta = q - pt;
+                        R d = length(delta);
+                        if (d == R(0)) continue;
+                        if (d < best) {
+                            best = d;
+                            tri = (int)t;
+                            bcp = pt;
+                            bn = (R(1.f) / d) * delta;
+                        }
                     }
+                    if (tri >= 0 && best < radTol && best >= R(0)) out.push_back({i, bcp, bn, tri});
                 }
-                if (tri >= 0 && best < radTol && best >= R(0)) {
-                    add_contact(i, bcp, bn, ci);
-                    contacts.back().feature = tri;
+            });
+            for (auto& part : parts)
+                for (const Hit& h : part) {
+                    add_contact(h.i, h.p, h.n, ci);
+                    contacts.back().feature = h.feature;
                     contacts.back().mesh = mi;
                 }
-            }
         }
         if (prm.collFlags & CF_TRIANGLES) {
-            for (size_t j = 0; j < tris.size(); j++) {
+            auto parts = ordered_blocks<std::vector<TriContact<R>>>(
+                tris.size(), [&](size_t b, size_t end, std::vector<TriContact<R>>& out) {
+            std::vector<int> cand;
+            for (size_t j = b; j < end; j++) {
                 const uint32_t i1 = tris[j].i1, i2 = tris[j].i2, i3 = tris[j].i3;
                 const V3<R> a = prev[i1], b = prev[i2], c = prev[i3];
                 R best = R(FLT_MAX), dist = R(-FLT_MAX);
@@ -882,7 +934,7 @@ public:
                     const R rtq = R(prm.thickness) + smax(R(prm.tolerance), tvm * R(1.2f));
                     const V3<R> lo3{std::min({a.x, b.x, c.x}), std::min({a.y, b.y, c.y}), std::min({a.z, b.z, c.z})};
                     const V3<R> hi3{std::max({a.x, b.x, c.x}), std::max({a.y, b.y, c.y}), std::max({a.z, b.z, c.z})};
-                    candidates(M.vtxGrid, M.v.size(), lo3, hi3, rtq);
+                    candidates(cand, M.vtxGrid, M.v.size(), lo3, hi3, rtq);
                 }
                 for (const int cv : cand) {
                     const size_t v = (size_t)cv;
@@ -909,19 +961,24 @@ public:
                     tc.clothTri = (int)j; tc.feature = vtx; tc.mesh = mi;
                     tc.point = bpt; tc.normal = bn;
                     tc.w1 = bbar.x; tc.w2 = bbar.y; tc.w3 = bbar.z;
-                    triContacts.push_back(tc);
+                    out.push_back(tc);
                 }
             }
+                });
+            append_parts(triContacts, parts);
         }
         if (prm.collFlags & CF_EDGES) {
-            for (size_t e1 = 0; e1 < edges.size(); e1++) {
+            auto parts = ordered_blocks<std::vector<EdgeContact<R>>>(
+                edges.size(), [&](size_t b, size_t end, std::vector<EdgeContact<R>>& out) {
+            std::vector<int> cand;
+            for (size_t e1 = b; e1 < end; e1++) {
                 const uint32_t i1 = edges[e1].i1, i2 = edges[e1].i2;
                 const V3<R> p = prev[i1], q = prev[i2];
                 R best = R(FLT_MAX), dist = R(-FLT_MAX);
                 int edge = -1;
                 V3<R> bcp{0, 0, 0}, bn{0, 0, 0};
                 R bws = 0;
-                candidates(M.edgeGrid, M.edges.size(), p, q, radTol);
+                candidates(cand, M.edgeGrid, M.edges.size(), p, q, radTol);
                 for (const int ce : cand) {
                     const size_t e = (size_t)ce;
                     R ps, pm;
@@ -945,9 +1002,11 @@ public:
                     ec.clothEdge = (int)e1; ec.feature = edge; ec.mesh = mi;
                     ec.point = bcp; ec.normal = bn;
                     ec.w1 = R(1.0f) - bws; ec.w2 = bws;
-                    edgeContacts.push_back(ec);
+                    out.push_back(ec);
                 }
             }
+                });
+            append_parts(edgeContacts, parts);
         }
     }
 
@@ -1428,7 +1487,6 @@ public:
         // candidate pairs from grids (accel) or all pairs, in ascending order
         // either way, so the contact lists come out the same
         BoxGrid vg, eg;
-        std::vector<int> cand;
         if (prm.accel) {
             std::vector<V3<float>> lo(nv), hi(nv);
             std::vector<char> pinned(nv);
@@ -1446,7 +1504,7 @@ public:
             }
             eg.build(lo, hi);
         }
-        auto each = [&](const BoxGrid& g, const Box& q, size_t count, size_t first, auto f) {
+        auto each = [&](std::vector<int>& cand, const BoxGrid& g, const Box& q, size_t count, size_t first, auto f) {
             if (prm.accel) {
                 g.query(q.lo, q.hi, cand);
                 for (const int i : cand)
@@ -1457,32 +1515,51 @@ public:
         };
         selfCandVT.clear();
         selfCandEE.clear();
-        for (size_t t = 0; t < nt; t++) {
-            const uint32_t j1 = tris[t].i1, j2 = tris[t].i2, j3 = tris[t].i3;
-            each(vg, tb[t], nv, 0, [&](size_t i) {
-                if (i == j1 || i == j2 || i == j3 || invMass[i] == R(0)) return;
-                if (!overlap(tb[t], vb[i])) return;
-                if (prm.recordCandidates) selfCandVT.push_back({(int)t, (int)i});
-                SelfContact<R> c;
-                if (vertex_triangle(j1, j2, j3, (uint32_t)i, c)) {
-                    c.key = (uint64_t(t) << 32) | uint64_t(i);
-                    selfTris.push_back(c);
-                }
-            });
-        }
-        for (size_t e1 = 0; e1 + 1 < ne; e1++) {
-            each(eg, eb[e1], ne, e1 + 1, [&](size_t e2) {
-                const uint32_t i1 = edges[e1].i1, i2 = edges[e1].i2, i3 = edges[e2].i1, i4 = edges[e2].i2;
-                if (i1 == i3 || i2 == i4 || i1 == i4 || i2 == i3) return;
-                if (!overlap(eb[e1], eb[e2])) return;
-                if (prm.recordCandidates) selfCandEE.push_back({(int)e1, (int)e2});
-                SelfContact<R> c;
-                if (edge_edge(i1, i2, i3, i4, c)) {
-                    c.key = (uint64_t(e1) << 32) | uint64_t(e2);
-                    selfEdges.push_back(c);
-                }
-            });
-        }
+        struct Part {
+            std::vector<SelfContact<R>> contacts;
+            std::vector<std::pair<int, int>> cand;
+        };
+        auto merge = [](std::vector<Part>& parts, std::vector<SelfContact<R>>& list,
+                        std::vector<std::pair<int, int>>& cands) {
+            for (Part& p : parts) {
+                list.insert(list.end(), p.contacts.begin(), p.contacts.end());
+                cands.insert(cands.end(), p.cand.begin(), p.cand.end());
+            }
+        };
+        auto vt = ordered_blocks<Part>(nt, [&](size_t b, size_t end, Part& out) {
+            std::vector<int> cand;
+            for (size_t t = b; t < end; t++) {
+                const uint32_t j1 = tris[t].i1, j2 = tris[t].i2, j3 = tris[t].i3;
+                each(cand, vg, tb[t], nv, 0, [&](size_t i) {
+                    if (i == j1 || i == j2 || i == j3 || invMass[i] == R(0)) return;
+                    if (!overlap(tb[t], vb[i])) return;
+                    if (prm.recordCandidates) out.cand.push_back({(int)t, (int)i});
+                    SelfContact<R> c;
+                    if (vertex_triangle(j1, j2, j3, (uint32_t)i, c)) {
+                        c.key = (uint64_t(t) << 32) | uint64_t(i);
+                        out.contacts.push_back(c);
+                    }
+                });
+            }
+        });
+        merge(vt, selfTris, selfCandVT);
+        auto ee = ordered_blocks<Part>(ne ? ne - 1 : 0, [&](size_t b, size_t end, Part& out) {
+            std::vector<int> cand;
+            for (size_t e1 = b; e1 < end; e1++) {
+                each(cand, eg, eb[e1], ne, e1 + 1, [&](size_t e2) {
+                    const uint32_t i1 = edges[e1].i1, i2 = edges[e1].i2, i3 = edges[e2].i1, i4 = edges[e2].i2;
+                    if (i1 == i3 || i2 == i4 || i1 == i4 || i2 == i3) return;
+                    if (!overlap(eb[e1], eb[e2])) return;
+                    if (prm.recordCandidates) out.cand.push_back({(int)e1, (int)e2});
+                    SelfContact<R> c;
+                    if (edge_edge(i1, i2, i3, i4, c)) {
+                        c.key = (uint64_t(e1) << 32) | uint64_t(e2);
+                        out.contacts.push_back(c);
+                    }
+                });
+            }
+        });
+        merge(ee, selfEdges, selfCandEE);
     }
 
     // processing order of the self contacts, given as their keys
@@ -1833,6 +1910,10 @@ static auto with(void* hp, F&& fn)
 using namespace oracle;
 
 extern "C" {
+
+// OpenMP threads for the detection loops of later calls from this thread
+void oc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int oc_get_threads() { return omp_get_max_threads(); }
 
 void* oc_create(int real64)
 {

@@ -198,10 +198,15 @@ inline R signed_volume(V3<R> a0, V3<R> a1, V3<R> a2, V3<R> a3)
 struct MeshEdge {
     uint32_t i1, i2;
     int t1 = -1, t2 = -1;
+    bool swapped = false;  // the first triangle runs the edge i2 -> i1
+    int orig = -1;         // index of the first triangle side, 3 t + k
 };
 
-// Geometry/Mesh.cpp:65-147 on an index list (std::sort as the reference)
-inline std::vector<MeshEdge> mesh_edges(const std::vector<uint32_t>& idx)
+// Geometry/Mesh.cpp:65-147 on an index list (std::sort as the reference, so
+// the first of two duplicate sides, which sets t1 and swapped, is the
+// reference's). map, if given, receives per triangle side 3 t + k the unique
+// edge: k = 0 is (a, b), 1 is (b, c), 2 is (c, a) (Mesh::Triangle::e).
+inline std::vector<MeshEdge> mesh_edges(const std::vector<uint32_t>& idx, std::vector<int>* map = nullptr)
 {
     std::vector<MeshEdge> edges;
     const int ne = (int)idx.size();
@@ -210,23 +215,59 @@ inline std::vector<MeshEdge> mesh_edges(const std::vector<uint32_t>& idx)
     for (int i = 0; i < ne; i++) {
         uint32_t a = idx[i];
         uint32_t b = ((i + 1) % 3 == 0) ? idx[i - 2] : idx[i + 1];
+        dup[i].swapped = a > b;
         if (a > b) std::swap(a, b);
-        dup[i].i1 = a; dup[i].i2 = b; dup[i].t1 = i / 3;
+        dup[i].i1 = a; dup[i].i2 = b; dup[i].t1 = i / 3; dup[i].orig = i;
     }
     std::sort(dup.begin(), dup.end(), [](const MeshEdge& e1, const MeshEdge& e2) {
         return e1.i1 < e2.i1 || (e1.i1 == e2.i1 && e1.i2 < e2.i2);
     });
+    if (map) map->assign(ne, -1);
     MeshEdge cur = dup[0];
+    if (map) (*map)[cur.orig] = 0;
     for (size_t i = 1; i < dup.size(); i++) {
         if (dup[i].i1 == dup[i - 1].i1 && dup[i].i2 == dup[i - 1].i2) {
             cur.t2 = dup[i].t1;
+            if (map) (*map)[dup[i].orig] = (int)edges.size();
             continue;
         }
         edges.push_back(cur);
         cur = dup[i];
+        if (map) (*map)[cur.orig] = (int)edges.size();
     }
     edges.push_back(cur);
     return edges;
+}
+
+// Geometry/Mesh.cpp:10-63: unit face normals (tn) and angle-weighted unit
+// vertex normals (vn; zero for a vertex in no triangle)
+inline void mesh_face_vertex_normals(const std::vector<V3<float>>& v, const std::vector<uint32_t>& idx,
+                                     std::vector<V3<float>>& vn, std::vector<V3<float>>& tn)
+{
+    const size_t nt = idx.size() / 3;
+    vn.assign(v.size(), V3<float>{0, 0, 0});
+    tn.assign(nt, V3<float>{0, 0, 0});
+    for (size_t t = 0; t < nt; t++) {
+        const uint32_t i1 = idx[3 * t], i2 = idx[3 * t + 1], i3 = idx[3 * t + 2];
+        const V3<float> v1 = v[i1], v2 = v[i2], v3 = v[i3];
+        V3<float> n = cross(v2 - v1, v3 - v1);
+        normalize(n);
+        tn[t] = n;
+        V3<float> e1 = v2 - v1, e2 = v3 - v2, e3 = v1 - v3;
+        normalize(e1);
+        normalize(e2);
+        normalize(e3);
+        float c1 = -dot(e1, e3), c2 = -dot(e1, e2), c3 = -dot(e2, e3);
+        float s1 = length(cross(e1, -e3)), s2 = length(cross(e2, -e1)), s3 = length(cross(e3, -e1));
+        float t1 = std::fabs(c1) > 1 ? std::fabs((float)std::atan2(s1, c1)) : std::acos(c1);
+        float t2 = std::fabs(c2) > 1 ? std::fabs((float)std::atan2(s2, c2)) : std::acos(c2);
+        float t3 = std::fabs(c3) > 1 ? std::fabs((float)std::atan2(s3, c3)) : std::acos(c3);
+        vn[i1] += t1 * n;
+        vn[i2] += t2 * n;
+        vn[i3] += t3 * n;
+    }
+    for (auto& n : vn)
+        if (length(n) != 0.f) normalize(n);
 }
 
 // Uniform grid over boxes for exact closed-interval overlap queries (a speed-up
@@ -331,29 +372,7 @@ struct ObstacleMesh {
     void build()
     {
         const size_t nt = idx.size() / 3;
-        vn.assign(v.size(), V3<float>{0, 0, 0});
-        tn.assign(nt, V3<float>{0, 0, 0});
-        for (size_t t = 0; t < nt; t++) {
-            const uint32_t i1 = idx[3 * t], i2 = idx[3 * t + 1], i3 = idx[3 * t + 2];
-            const V3<float> v1 = v[i1], v2 = v[i2], v3 = v[i3];
-            V3<float> n = cross(v2 - v1, v3 - v1);
-            normalize(n);
-            tn[t] = n;
-            V3<float> e1 = v2 - v1, e2 = v3 - v2, e3 = v1 - v3;
-            normalize(e1);
-            normalize(e2);
-            normalize(e3);
-            float c1 = -dot(e1, e3), c2 = -dot(e1, e2), c3 = -dot(e2, e3);
-            float s1 = length(cross(e1, -e3)), s2 = length(cross(e2, -e1)), s3 = length(cross(e3, -e1));
-            float t1 = std::fabs(c1) > 1 ? std::fabs((float)std::atan2(s1, c1)) : std::acos(c1);
-            float t2 = std::fabs(c2) > 1 ? std::fabs((float)std::atan2(s2, c2)) : std::acos(c2);
-            float t3 = std::fabs(c3) > 1 ? std::fabs((float)std::atan2(s3, c3)) : std::acos(c3);
-            vn[i1] += t1 * n;
-            vn[i2] += t2 * n;
-            vn[i3] += t3 * n;
-        }
-        for (auto& n : vn)
-            if (length(n) != 0.f) normalize(n);
+        mesh_face_vertex_normals(v, idx, vn, tn);
         edges = mesh_edges(idx);
         en.resize(edges.size());
         for (size_t e = 0; e < edges.size(); e++) {

@@ -4,18 +4,30 @@
 // tests/golden/make_golden.py, in a container that has the reference, to
 // write golden vectors; nothing on a GPU host loads it.
 //
-// Only Geometry/{Mesh,Collision3D,AabbTree,ClosestPointOnMesh,VoronoiRegions}.cpp are
-// compiled: they build with the image's own headers. The Physics translation units do not (they need Win32
+// Only Geometry/{Mesh,Collision3D,AabbTree,ClosestPointOnMesh,VoronoiRegions,
+// ClosestPointsBetweenMeshes}.cpp are compiled: they build with the image's
+// own headers. The Physics translation units do not (they need Win32
 // types from Engine/Engine.h and std::floorf, which libstdc++ 11 lacks), so
 // the solver itself is pinned by the oracle's mathematical tests instead.
 #include "Geometry/AabbTree.h"
 #include "Geometry/ClosestPointOnMesh.h"
+#include "Geometry/ClosestPointsBetweenMeshes.h"
 #include "Geometry/Collision3D.h"
 #include "Geometry/Mesh.h"
 
+#include <omp.h>
+
+#include <chrono>
 #include <cstring>
 
 using Math::Vector3;
+
+// Defined in Geometry/ClosestPointsBetweenMeshes.cpp:441,473 (external
+// linkage, no header declares them)
+namespace Geometry {
+bool CheckAdjacency(const Mesh& mesh, int vtx, int tri, int region);
+bool CheckEdgeAdjacency(ClosestEdgeToSegment& info, const Mesh& mesh1, int i1, int i2, const Mesh::Edge& edge1);
+}  // namespace Geometry
 
 extern "C" {
 
@@ -176,6 +188,106 @@ int ref_edge_normals(const float* verts, int nv, const unsigned* idx, int ni, co
         out[3 * k] = nn.x; out[3 * k + 1] = nn.y; out[3 * k + 2] = nn.z;
     }
     return (int)m.edges.size();
+}
+
+// MeshClosestPoints::ClosestPoints on the cloth's own mesh
+// (Geometry/ClosestPointsBetweenMeshes.cpp:538-592), the live self path of
+// ClothModel::MeshCollisions (Physics/ClothCollision.cpp:89-105), over a mesh
+// prepared as ClothPatch::Init prepares it (Physics/ClothPatch.cpp:232-235).
+// Call with one OpenMP thread (ref_set_threads) for vertex and edge order.
+// Per VT info: ids (vtx, tri, region, regionType, feature) and data (distance,
+// closest point 3, barycentric 3, normal 3, side). Per EE info: ids (edge1,
+// edge, region) and data (distance, coordsSegm 2, coordsMesh 2, closest point
+// on the segment 3, on the mesh 3, normal 3, side). counts = (VT, EE).
+void ref_live_self(const float* verts, int nv, const unsigned* idx, int ni, float radTol, int flags, int* vtIds,
+                   float* vtData, int vtCap, int* eeIds, float* eeData, int eeCap, int* counts)
+{
+    Geometry::Mesh m;
+    m.vertices.resize(nv);
+    for (int i = 0; i < nv; i++) m.vertices[i] = Vector3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
+    m.indices.assign(idx, idx + ni);
+    m.ComputeNormals();
+    m.ConstructEdges();
+    m.ConstructVertexOneRings();
+    m.ConstructEdgeOneRings();
+    Geometry::MeshClosestPoints mcp;
+    mcp.SetParams(flags);
+    mcp.ClosestPoints(m, nullptr, radTol);
+    counts[0] = (int)mcp.mVertexInfos.size();
+    counts[1] = (int)mcp.mEdgeInfos.size();
+    for (int k = 0; k < counts[0] && k < vtCap; k++) {
+        const Geometry::ClosestTriangleToPoint& r = mcp.mVertexInfos[k];
+        const int ids[5] = {r.vtx, r.tri, r.region, r.regionType, r.feature};
+        const float d[11] = {r.distance, r.closestPtMesh.x, r.closestPtMesh.y, r.closestPtMesh.z, r.baryOnMesh.x,
+                             r.baryOnMesh.y, r.baryOnMesh.z, r.normal.x, r.normal.y, r.normal.z, r.side};
+        memcpy(vtIds + 5 * k, ids, sizeof(ids));
+        memcpy(vtData + 11 * k, d, sizeof(d));
+    }
+    for (int k = 0; k < counts[1] && k < eeCap; k++) {
+        const Geometry::ClosestEdgeToSegment& r = mcp.mEdgeInfos[k];
+        const int ids[3] = {r.edge1, r.edge, r.region};
+        const float d[15] = {r.distance, r.coordsSegm.x, r.coordsSegm.y, r.coordsMesh.x, r.coordsMesh.y,
+                             r.closestPtSegment.x, r.closestPtSegment.y, r.closestPtSegment.z, r.closestPtMesh.x,
+                             r.closestPtMesh.y, r.closestPtMesh.z, r.normal.x, r.normal.y, r.normal.z, r.side};
+        memcpy(eeIds + 3 * k, ids, sizeof(ids));
+        memcpy(eeData + 15 * k, d, sizeof(d));
+    }
+}
+
+// OpenMP threads for the reference's parallel loops
+void ref_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
+// The loop bodies of MeshClosestPoints::ClosestPoints (self)
+// (ClosestPointsBetweenMeshes.cpp:546-562 and 573-590) for the listed vertices
+// and edges only, in the same OpenMP structure (parallel for, appends under a
+// critical section), on a mesh prepared as ref_live_self prepares it. Times a
+// bounded sample of the live path: seconds = (VT loop, EE loop), the mesh
+// preparation excluded; counts = infos kept (VT, EE).
+void ref_live_self_sample(const float* verts, int nv, const unsigned* idx, int ni, float radTol, const int* vq,
+                          int nvq, const int* eq, int neq, int* counts, double* seconds)
+{
+    Geometry::Mesh m;
+    m.vertices.resize(nv);
+    for (int i = 0; i < nv; i++) m.vertices[i] = Vector3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
+    m.indices.assign(idx, idx + ni);
+    m.ComputeNormals();
+    m.ConstructEdges();
+    m.ConstructVertexOneRings();
+    m.ConstructEdgeOneRings();
+    std::vector<Geometry::ClosestTriangleToPoint> vertexInfos;
+    std::vector<Geometry::ClosestEdgeToSegment> edgeInfos;
+    auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel for
+    for (int k = 0; k < nvq; k++) {
+        const int i = vq[k];
+        std::vector<Geometry::ClosestTriangleToPoint> results;
+        Geometry::ClosestPointsOnMeshToPoint(m.vertices[i], m, results, radTol, i);
+        for (size_t j = 0; j < results.size(); j++) {
+            results[j].vtx = i;
+            if (results[j].tri >= 0 && !Geometry::CheckAdjacency(m, i, results[j].tri, results[j].region)) {
+#pragma omp critical
+                vertexInfos.push_back(results[j]);
+            }
+        }
+    }
+    auto t1 = std::chrono::steady_clock::now();
+#pragma omp parallel for
+    for (int k = 0; k < neq; k++) {
+        const int i = eq[k];
+        const Geometry::Mesh::Edge& edge1 = m.edges[i];
+        Geometry::ClosestEdgeToSegment info;
+        info.edge1 = i;
+        Geometry::ClosestPointOnMeshToSegment(m.vertices[edge1.i1], m.vertices[edge1.i2], m, info, i);
+        if (!Geometry::CheckEdgeAdjacency(info, m, edge1.i1, edge1.i2, edge1)) {
+#pragma omp critical
+            edgeInfos.push_back(info);
+        }
+    }
+    auto t2 = std::chrono::steady_clock::now();
+    counts[0] = (int)vertexInfos.size();
+    counts[1] = (int)edgeInfos.size();
+    seconds[0] = std::chrono::duration<double>(t1 - t0).count();
+    seconds[1] = std::chrono::duration<double>(t2 - t1).count();
 }
 
 }  // extern "C"

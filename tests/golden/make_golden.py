@@ -194,13 +194,87 @@ def mesh_queries(L):
     print(f"wrote ref_mesh_queries.npz ({len(Q)} tie-free queries of 400)")
 
 
+def folded_sheet(n, gap, seed, jitter=1e-3):
+    """An n x n grid of side 1 in the plane y = 0 folded in half over itself:
+    the columns past x = 0.5 come back at height `gap`, shifted by half a
+    spacing, so that vertices, edges and faces of the two layers face each
+    other. Both diagonal orientations, and a seeded jitter against ties."""
+    rng = np.random.default_rng(seed)
+    h = 1.0 / (n - 1)
+    v = []
+    for j in range(n):
+        for i in range(n):
+            x = i * h
+            v.append((x, 0.0, j * h) if x <= 0.5 else (1.0 - x + 0.5 * h, gap, j * h))
+    v = np.array(v) + rng.normal(0.0, jitter, (n * n, 3))
+    t = []
+    for j in range(n - 1):
+        for i in range(n - 1):
+            a, b, c, d = j * n + i, j * n + i + 1, (j + 1) * n + i, (j + 1) * n + i + 1
+            t += [a, b, d, a, d, c] if (i + j) % 2 else [a, b, c, b, d, c]
+    return v.astype(np.float32), np.array(t, dtype=np.uint32)
+
+
+def live_self(L):
+    """MeshClosestPoints::ClosestPoints on a cloth's own mesh
+    (Geometry/ClosestPointsBetweenMeshes.cpp:538-592, the brute-force live self
+    path), one OpenMP thread, on folded sheets and an irregular closed mesh
+    squashed flat."""
+    L.ref_set_threads(1)
+    cases = [folded_sheet(10, 0.04, 0) + (0.06,), folded_sheet(12, 0.02, 1) + (0.05,),
+             folded_sheet(14, 0.01, 2) + (0.03,)]
+    # the irregular closed mesh squashed into a two-layer disc (a convex mesh
+    # has no VT infos: no vertex lies in another feature's Voronoi region)
+    v, t = irregular_mesh(5)
+    v = v * np.float32([1.0, 0.08, 1.0])
+    cases.append((v.astype(np.float32), t.astype(np.uint32), 0.4))
+    # the same shapes eight and three times larger: with spacings well above
+    # the vertex-region eps of 0.1, closest points leave the query vertex's
+    # region and the edge- and vertex-region tests, their orientation flag and
+    # the border rules decide
+    v, t = folded_sheet(12, 0.02, 3)
+    cases.append(((v * 8).astype(np.float32), t, 0.4))
+    cases.append(((cases[3][0] * 3).astype(np.float32), cases[3][1], 1.2))
+    # two separate sheets of spacing 1, the second tilted and laid across the
+    # first: many border edges facing vertices from every side
+    a, ta = folded_sheet(7, 0.0, 4)
+    a = a * 6.0
+    rng = np.random.default_rng(6)
+    ang = 0.5
+    rot = np.array([[np.cos(ang), 0, np.sin(ang)], [0, 1, 0], [-np.sin(ang), 0, np.cos(ang)]]) @ \
+        np.array([[1, 0, 0], [0, np.cos(0.3), -np.sin(0.3)], [0, np.sin(0.3), np.cos(0.3)]])
+    b = (a - 3.0) @ rot.T + np.array([3.4, 0.45, 2.6]) + rng.normal(0, 0.01, a.shape)
+    cases.append((np.concatenate([a, b]).astype(np.float32),
+                  np.concatenate([ta, ta + len(a)]).astype(np.uint32), 0.9))
+    out = {}
+    for k, (v, t, rad) in enumerate(cases):
+        cap = 20000
+        vi = np.zeros((cap, 5), np.int32)
+        vd = np.zeros((cap, 11), np.float32)
+        ei = np.zeros((cap, 3), np.int32)
+        ed = np.zeros((cap, 15), np.float32)
+        cnt = np.zeros(2, np.int32)
+        L.ref_live_self(p(v), C.c_int(len(v)), p(t), C.c_int(t.size), C.c_float(rad), C.c_int(1 | 16), p(vi), p(vd),
+                        C.c_int(cap), p(ei), p(ed), C.c_int(cap), p(cnt))
+        assert cnt[0] <= cap and cnt[1] <= cap
+        out.update({f"c{k}_vertices": v, f"c{k}_indices": t, f"c{k}_rad": np.float32(rad),
+                    f"c{k}_vt_ids": vi[:cnt[0]], f"c{k}_vt_data": vd[:cnt[0]], f"c{k}_ee_ids": ei[:cnt[1]],
+                    f"c{k}_ee_data": ed[:cnt[1]]})
+        print(f"live self case {k}: {len(v)} vertices, {cnt[0]} VT and {cnt[1]} EE infos")
+    np.savez_compressed(os.path.join(HERE, "ref_live_self.npz"), cases=np.int32(len(cases)), **out)
+    print("wrote ref_live_self.npz")
+
+
 if __name__ == "__main__":
     # no argument: every fixture; a name: only that fixture
     if sys.argv[1:] == ["segm_triangle"]:
         segm_triangle(ref_lib())
     elif sys.argv[1:] == ["mesh_queries"]:
         mesh_queries(ref_lib())
+    elif sys.argv[1:] == ["live_self"]:
+        live_self(ref_lib())
     else:
         main()
         segm_triangle(ref_lib())
         mesh_queries(ref_lib())
+        live_self(ref_lib())

@@ -42,3 +42,36 @@ def test_grid_broad_phase_mesh_with_self(oracle_mod):
     b, nb = _run(oracle_mod, scene, 1, scene.frames)
     assert np.array_equal(a.get("pos"), b.get("pos"))
     assert na == nb and na > 0, (na, nb)
+
+
+@pytest.mark.parametrize("name", ["drape40", "meshsmall", "fold24", "mesh32self"])
+def test_detection_threads_do_not_change_results(oracle_mod, name):
+    """The oracle's detection loops run on OpenMP threads in blocks appended in
+    block order: one thread and four give the same contact lists in the same
+    order, hence the same positions bit for bit. Scenes cover the SDF vertex,
+    edge and triangle loops, the obstacle-mesh loops and the self-collision
+    loops (each above the size at which the loops go parallel)."""
+    scene = {"drape40": S.DRAPE256.with_(n=40, frames=10), "meshsmall": S.MESHSMALL, "fold24": S.FOLD24,
+             "mesh32self": S.MESH512.with_(n=32, mesh_subdiv=3, ccd=S.ccd_for_spacing(2.0 / 31), perturb_seed=3)}[name]
+    frames = 20 if name == "drape40" else 8  # the drape reaches the sphere after about 12 frames
+    runs = []
+    try:
+        for threads in (1, 4):
+            oracle_mod.set_threads(threads)
+            assert oracle_mod.threads() == threads
+            runs.append(_run(oracle_mod, scene.with_(flags=scene.flags | 2 | 16) if name == "drape40" else scene, 1,
+                             frames))
+    finally:
+        oracle_mod.set_threads(1)
+    (a, na), (b, nb) = runs
+    assert na == nb and na > 0, (na, nb)
+    assert np.array_equal(a.get("pos"), b.get("pos"))
+    for x, y in zip(a.contacts(), b.contacts()):
+        assert np.array_equal(np.asarray(x), np.asarray(y))
+    for x, y in zip(a.edge_contacts(), b.edge_contacts()):
+        assert np.array_equal(np.asarray(x), np.asarray(y))
+    for x, y in zip(a.tri_contacts(), b.tri_contacts()):
+        assert np.array_equal(np.asarray(x), np.asarray(y))
+    for kind in ("vt", "ee"):
+        for x, y in zip(a.self_contacts(kind), b.self_contacts(kind)):
+            assert np.array_equal(np.asarray(x), np.asarray(y))

@@ -128,3 +128,60 @@ def test_mesh_queries_match_the_compiled_reference(oracle_mod):
     assert np.array_equal(en, g["edge_normals"])
     # parameters exactly 0 and 1 took the ends' vertex normals
     assert not np.array_equal(en[0::3], en[2::3])
+
+
+LIVE = np.load(os.path.join(GOLDEN, "ref_live_self.npz"))
+
+
+@pytest.mark.parametrize("case", range(int(LIVE["cases"])))
+def test_live_self_path_matches_the_compiled_reference(oracle_mod, case):
+    """The brute-force live self path (MeshClosestPoints::ClosestPoints on the
+    cloth's own mesh, Geometry/ClosestPointsBetweenMeshes.cpp:538-592) against
+    the compiled reference, bit for bit and in order, on folded sheets, a
+    squashed irregular mesh, larger copies of both and two crossing sheets
+    (tests/golden/make_golden.py). Each of these mistakes, planted in the
+    restatement, fails at least one case: the edge's orientation flag
+    ignored, edge-region eps 0 instead of 0.002, vertex-region eps 0.01
+    instead of 0.1, the border-vertex rule dropped, the duplicate-feature
+    check dropped, an edge adjacency rule dropped, the vertex-region test run
+    at the vertex instead of at the closest point, the EE search over all
+    edges instead of the later ones, a same-triangle EE rejection dropped.
+    Two forms are equivalent and so not told apart: the face test's chained
+    (signbit == signbit) == signbit and "all three signs equal" (a point
+    whose closest point is a face interior has all three signs negative), and
+    the border-edge rule (a vertex nearest a border edge's interior lies on
+    its outer side, where the edge-region test already passes)."""
+    g = lambda k: LIVE[f"c{case}_{k}"]
+    live = oracle_mod.LiveSelf(g("vertices"), g("indices"))
+    vi, vd = live.vertex_queries(float(g("rad")))
+    ei, ed = live.edge_queries()
+    assert len(g("vt_ids")) > 0 and len(g("ee_ids")) > 0
+    assert np.array_equal(vi, g("vt_ids"))
+    assert np.array_equal(vd.view(np.uint32), g("vt_data").view(np.uint32))
+    assert np.array_equal(ei, g("ee_ids"))
+    assert np.array_equal(ed.view(np.uint32), g("ee_data").view(np.uint32))
+    # region types: faces, edges and vertices all occur across the cases
+    if case == 1:
+        assert set(np.unique(vi[:, 3])) == {0, 1, 2}
+
+
+def test_live_self_threads_and_subsets(oracle_mod):
+    """Any thread count gives the serial order, and a query list gives the
+    infos of those items only, in list order."""
+    g = lambda k: LIVE[f"c1_{k}"]
+    live = oracle_mod.LiveSelf(g("vertices"), g("indices"))
+    try:
+        oracle_mod.set_threads(4)
+        vi, vd = live.vertex_queries(float(g("rad")))
+        ei, ed = live.edge_queries()
+    finally:
+        oracle_mod.set_threads(1)
+    assert np.array_equal(vi, g("vt_ids")) and np.array_equal(ei, g("ee_ids"))
+    q = np.array([100, 3, 57], np.int32)
+    si, _ = live.vertex_queries(float(g("rad")), q)
+    want = np.concatenate([g("vt_ids")[g("vt_ids")[:, 0] == k] for k in q])
+    assert np.array_equal(si, want)
+    qe = np.array([5, 200, 11], np.int32)
+    se, _ = live.edge_queries(qe)
+    want = np.concatenate([g("ee_ids")[g("ee_ids")[:, 0] == k] for k in qe])
+    assert np.array_equal(se, want)
