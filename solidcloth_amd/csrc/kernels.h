@@ -296,10 +296,12 @@ void launch_predict_mesh(hipStream_t s, float4* pos, float4* prev, float4* vel, 
 void launch_detect_features(hipStream_t s, const float4* pos, const float4* prev, const int2* cEdges,
                             const int4* cTris, const CollSet& cs, FeatureContactBuf ec, FeatureContactBuf tc,
                             float thickness, float tolerance, int collFlags, const SolverCtl* ctl);
+// One launch per colour group, in group order; groupBase is the host copy of
+// the groups' offsets (each group's capacity sizes its grid).
 void launch_edge_contacts(hipStream_t s, float4* pos, const float4* prev, const int2* cEdges, FeatureContactBuf ec,
-                          const MeshGeo* geo, float thickness, float mu, SolverCtl* ctl);
+                          const int* groupBase, const MeshGeo* geo, float thickness, float mu, SolverCtl* ctl);
 void launch_tri_contacts(hipStream_t s, float4* pos, const float4* prev, const int4* cTris, FeatureContactBuf tc,
-                         const MeshGeo* geo, float thickness, float mu, SolverCtl* ctl);
+                         const int* groupBase, const MeshGeo* geo, float thickness, float mu, SolverCtl* ctl);
 
 // Self collision (self.hip): swept boxes of triangles and edges, continuous
 // vertex-triangle and edge-edge detection over the LBVHs, colouring of the

@@ -11,7 +11,7 @@
 namespace sc {
 
 static constexpr int kBlock = 256;
-static constexpr int kSolveBlock = 1024;
+static constexpr int kMaxSolveBlocks = 2048;
 
 // Physics/ClothPBD.cpp:258-266 plus max_i |pos_i - prev_i|
 // (ClothCollision.cpp:95-104). Thread 0 resets the control block, the first
@@ -184,134 +184,130 @@ __device__ __forceinline__ FeatureStep feature_step(f3 normal, float s, float de
     return st;
 }
 
-// Physics/ClothCollisionHandler.cpp:289-362 — obstacle-edge contacts. One
-// workgroup walks the colour groups in order; the cloth edges of a group share
-// no vertex, so their contacts are solved in parallel.
-__global__ void __launch_bounds__(kSolveBlock) k_edge_contacts(float4* __restrict__ pos,
-                                                               const float4* __restrict__ prev,
-                                                               const int2* __restrict__ cEdges, FeatureContactBuf ec,
-                                                               const MeshGeo* __restrict__ geo, float len0, float mu,
-                                                               SolverCtl* ctl)
+// Physics/ClothCollisionHandler.cpp:289-362 — obstacle-edge contacts of one
+// colour group (one launch per group, in group order). The cloth edges of a
+// group share no vertex, so their contacts are solved in parallel across the
+// grid; the group's contact count is read on the device and the grid, sized
+// for the group's capacity, strides over it.
+__global__ void __launch_bounds__(kBlock) k_edge_contacts(float4* __restrict__ pos, const float4* __restrict__ prev,
+                                                          const int2* __restrict__ cEdges, FeatureContactBuf ec,
+                                                          const MeshGeo* __restrict__ geo, float len0, float mu, int g,
+                                                          SolverCtl* ctl)
 {
     if (ctl->done) return;
+    const int cnt = ec.groupCount[g], base = ec.groupBase[g];
+    if ((int)(blockIdx.x * blockDim.x) >= cnt) return;
     float err = 0.f;
-    for (int g = 0; g < ec.groups; g++) {
-        const int cnt = ec.groupCount[g], base = ec.groupBase[g];
-        for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
-            const int e = ec.list[base + k];
-            const int2 ev = cEdges[e];
-            const float4 X1 = pos[ev.x], X2 = pos[ev.y];
-            f3 x1 = xyz(X1), x2 = xyz(X2);
-            const float m1 = X1.w, m2 = X2.w;
-            const int ns = ec.count[e];
-            for (int sl = 0; sl < ns; sl++) {
-                const int q = sl * ec.n + e;
-                const float4 CP = ec.P[q], CN = ec.N[q];
-                const int2 F = ec.F[q];
-                f3 point = xyz(CP), p;
-                float w1, w2;
-                if (F.x >= 0) {
-                    // mesh contact: closest points on the same obstacle edge again
-                    const MeshGeo& M = geo[F.y];
-                    const int2 me = M.edge[F.x];
-                    float s1, t;
-                    closest_pt_segm_segm(xyz(M.v[me.x]), xyz(M.v[me.y]), x1, x2, s1, t, point, p);
-                    w1 = 1.0f - t;
-                    w2 = t;
-                } else {
-                    const float4 W = ec.W[q];
-                    w1 = W.x;
-                    w2 = W.y;
-                    p = add(mul(w1, x1), mul(w2, x2));
-                }
-                const f3 normal = xyz(CN);
-                float lambda = CP.w, lambdaF = CN.w;
-                const float l = dot(normal, sub(p, point));
-                if (!(l > len0)) {
-                    const float s = 1.f / (w1 * w1 / m1 + w2 * w2 / m2);
-                    const float depth = len0 - l;
-                    if (depth > err) err = depth;
-                    const f3 p0 = add(mul(w1, xyz(prev[ev.x])), mul(w2, xyz(prev[ev.y])));
-                    const FeatureStep st = feature_step(normal, s, depth, p, p0, mu, lambda, lambdaF);
-                    if (st.fric) {
-                        x1 = add(x1, mul(w1 * m1, st.pf));
-                        x2 = add(x2, mul(w2 * m2, st.pf));
-                    }
-                    x1 = add(x1, mul(w1 * m1, st.disp));
-                    x2 = add(x2, mul(w2 * m2, st.disp));
-                }
-                ec.P[q] = f4(point, lambda);
-                ec.N[q] = make_float4(CN.x, CN.y, CN.z, lambdaF);
-                ec.W[q] = make_float4(w1, w2, 0.f, 0.f);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+        const int e = ec.list[base + k];
+        const int2 ev = cEdges[e];
+        const float4 X1 = pos[ev.x], X2 = pos[ev.y];
+        f3 x1 = xyz(X1), x2 = xyz(X2);
+        const float m1 = X1.w, m2 = X2.w;
+        const int ns = ec.count[e];
+        for (int sl = 0; sl < ns; sl++) {
+            const int q = sl * ec.n + e;
+            const float4 CP = ec.P[q], CN = ec.N[q];
+            const int2 F = ec.F[q];
+            f3 point = xyz(CP), p;
+            float w1, w2;
+            if (F.x >= 0) {
+                // mesh contact: closest points on the same obstacle edge again
+                const MeshGeo& M = geo[F.y];
+                const int2 me = M.edge[F.x];
+                float s1, t;
+                closest_pt_segm_segm(xyz(M.v[me.x]), xyz(M.v[me.y]), x1, x2, s1, t, point, p);
+                w1 = 1.0f - t;
+                w2 = t;
+            } else {
+                const float4 W = ec.W[q];
+                w1 = W.x;
+                w2 = W.y;
+                p = add(mul(w1, x1), mul(w2, x2));
             }
-            pos[ev.x] = f4(x1, m1);
-            pos[ev.y] = f4(x2, m2);
+            const f3 normal = xyz(CN);
+            float lambda = CP.w, lambdaF = CN.w;
+            const float l = dot(normal, sub(p, point));
+            if (!(l > len0)) {
+                const float s = 1.f / (w1 * w1 / m1 + w2 * w2 / m2);
+                const float depth = len0 - l;
+                if (depth > err) err = depth;
+                const f3 p0 = add(mul(w1, xyz(prev[ev.x])), mul(w2, xyz(prev[ev.y])));
+                const FeatureStep st = feature_step(normal, s, depth, p, p0, mu, lambda, lambdaF);
+                if (st.fric) {
+                    x1 = add(x1, mul(w1 * m1, st.pf));
+                    x2 = add(x2, mul(w2 * m2, st.pf));
+                }
+                x1 = add(x1, mul(w1 * m1, st.disp));
+                x2 = add(x2, mul(w2 * m2, st.disp));
+            }
+            ec.P[q] = f4(point, lambda);
+            ec.N[q] = make_float4(CN.x, CN.y, CN.z, lambdaF);
+            ec.W[q] = make_float4(w1, w2, 0.f, 0.f);
         }
-        __syncthreads();
+        pos[ev.x] = f4(x1, m1);
+        pos[ev.y] = f4(x2, m2);
     }
     wave_max_atomic(&ctl->ce, err);
 }
 
 // Physics/ClothCollisionHandler.cpp:204-287 — obstacle-vertex contacts on
-// cloth triangles, by colour group as above.
-__global__ void __launch_bounds__(kSolveBlock) k_tri_contacts(float4* __restrict__ pos,
-                                                              const float4* __restrict__ prev,
-                                                              const int4* __restrict__ cTris, FeatureContactBuf tc,
-                                                              const MeshGeo* __restrict__ geo, float len0, float mu,
-                                                              SolverCtl* ctl)
+// the cloth triangles of one colour group, launched per group as above.
+__global__ void __launch_bounds__(kBlock) k_tri_contacts(float4* __restrict__ pos, const float4* __restrict__ prev,
+                                                         const int4* __restrict__ cTris, FeatureContactBuf tc,
+                                                         const MeshGeo* __restrict__ geo, float len0, float mu, int g,
+                                                         SolverCtl* ctl)
 {
     if (ctl->done) return;
+    const int cnt = tc.groupCount[g], base = tc.groupBase[g];
+    if ((int)(blockIdx.x * blockDim.x) >= cnt) return;
     float err = 0.f;
-    for (int g = 0; g < tc.groups; g++) {
-        const int cnt = tc.groupCount[g], base = tc.groupBase[g];
-        for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
-            const int t = tc.list[base + k];
-            const int4 ix = cTris[t];
-            const float4 X1 = pos[ix.x], X2 = pos[ix.y], X3 = pos[ix.z];
-            f3 x1 = xyz(X1), x2 = xyz(X2), x3 = xyz(X3);
-            const float m1 = X1.w, m2 = X2.w, m3 = X3.w;
-            const int ns = tc.count[t];
-            for (int sl = 0; sl < ns; sl++) {
-                const int q = sl * tc.n + t;
-                const float4 CP = tc.P[q], CN = tc.N[q];
-                const int2 F = tc.F[q];
-                const f3 point = xyz(CP);
-                f3 p, co;
-                closest_pt_point_triangle(point, x1, x2, x3, p, co);
-                const float w1 = co.x, w2 = co.y, w3 = co.z;
-                f3 n = normalized(neg(cross(sub(x2, x1), sub(x3, x1))));
-                f3 normal = n;
-                if (F.x >= 0) {
-                    const f3 mn = xyz(geo[F.y].vn[F.x]);
-                    if (dot(n, mn) < 0.f) normal = neg(n);
-                }
-                float lambda = CP.w, lambdaF = CN.w;
-                const float l = dot(normal, sub(p, point));
-                if (!(l > len0)) {
-                    const float s = 1.f / (w1 * w1 / m1 + w2 * w2 / m2 + w3 * w3 / m3);
-                    const float depth = len0 - l;
-                    if (depth > err) err = depth;
-                    const f3 p0 =
-                        add(add(mul(w1, xyz(prev[ix.x])), mul(w2, xyz(prev[ix.y]))), mul(w3, xyz(prev[ix.z])));
-                    const FeatureStep st = feature_step(normal, s, depth, p, p0, mu, lambda, lambdaF);
-                    if (st.fric) {
-                        x1 = add(x1, mul(w1 * m1, st.pf));
-                        x2 = add(x2, mul(w2 * m2, st.pf));
-                        x3 = add(x3, mul(w3 * m3, st.pf));
-                    }
-                    x1 = add(x1, mul(w1 * m1, st.disp));
-                    x2 = add(x2, mul(w2 * m2, st.disp));
-                    x3 = add(x3, mul(w3 * m3, st.disp));
-                }
-                tc.P[q] = make_float4(CP.x, CP.y, CP.z, lambda);
-                tc.N[q] = f4(normal, lambdaF);
-                tc.W[q] = make_float4(w1, w2, w3, 0.f);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+        const int t = tc.list[base + k];
+        const int4 ix = cTris[t];
+        const float4 X1 = pos[ix.x], X2 = pos[ix.y], X3 = pos[ix.z];
+        f3 x1 = xyz(X1), x2 = xyz(X2), x3 = xyz(X3);
+        const float m1 = X1.w, m2 = X2.w, m3 = X3.w;
+        const int ns = tc.count[t];
+        for (int sl = 0; sl < ns; sl++) {
+            const int q = sl * tc.n + t;
+            const float4 CP = tc.P[q], CN = tc.N[q];
+            const int2 F = tc.F[q];
+            const f3 point = xyz(CP);
+            f3 p, co;
+            closest_pt_point_triangle(point, x1, x2, x3, p, co);
+            const float w1 = co.x, w2 = co.y, w3 = co.z;
+            f3 n = normalized(neg(cross(sub(x2, x1), sub(x3, x1))));
+            f3 normal = n;
+            if (F.x >= 0) {
+                const f3 mn = xyz(geo[F.y].vn[F.x]);
+                if (dot(n, mn) < 0.f) normal = neg(n);
             }
-            pos[ix.x] = f4(x1, m1);
-            pos[ix.y] = f4(x2, m2);
-            pos[ix.z] = f4(x3, m3);
+            float lambda = CP.w, lambdaF = CN.w;
+            const float l = dot(normal, sub(p, point));
+            if (!(l > len0)) {
+                const float s = 1.f / (w1 * w1 / m1 + w2 * w2 / m2 + w3 * w3 / m3);
+                const float depth = len0 - l;
+                if (depth > err) err = depth;
+                const f3 p0 =
+                    add(add(mul(w1, xyz(prev[ix.x])), mul(w2, xyz(prev[ix.y]))), mul(w3, xyz(prev[ix.z])));
+                const FeatureStep st = feature_step(normal, s, depth, p, p0, mu, lambda, lambdaF);
+                if (st.fric) {
+                    x1 = add(x1, mul(w1 * m1, st.pf));
+                    x2 = add(x2, mul(w2 * m2, st.pf));
+                    x3 = add(x3, mul(w3 * m3, st.pf));
+                }
+                x1 = add(x1, mul(w1 * m1, st.disp));
+                x2 = add(x2, mul(w2 * m2, st.disp));
+                x3 = add(x3, mul(w3 * m3, st.disp));
+            }
+            tc.P[q] = make_float4(CP.x, CP.y, CP.z, lambda);
+            tc.N[q] = f4(normal, lambdaF);
+            tc.W[q] = make_float4(w1, w2, w3, 0.f);
         }
-        __syncthreads();
+        pos[ix.x] = f4(x1, m1);
+        pos[ix.y] = f4(x2, m2);
+        pos[ix.z] = f4(x3, m3);
     }
     wave_max_atomic(&ctl->ct, err);
 }
@@ -338,18 +334,34 @@ void launch_detect_features(hipStream_t s, const float4* pos, const float4* prev
                        eb, thickness, tolerance, collFlags, ctl);
 }
 
-void launch_edge_contacts(hipStream_t s, float4* pos, const float4* prev, const int2* cEdges, FeatureContactBuf ec,
-                          const MeshGeo* geo, float thickness, float mu, SolverCtl* ctl)
+// grid for a group of `cap` cloth edges / triangles: one thread per possible
+// contact, up to kMaxSolveBlocks workgroups (the kernels stride beyond)
+static inline int group_blocks(int cap)
 {
-    hipLaunchKernelGGL(k_edge_contacts, dim3(1), dim3(kSolveBlock), 0, s, pos, prev, cEdges, ec, geo, thickness, mu,
-                       ctl);
+    const int b = blocks(cap);
+    return b < kMaxSolveBlocks ? b : kMaxSolveBlocks;
+}
+
+void launch_edge_contacts(hipStream_t s, float4* pos, const float4* prev, const int2* cEdges, FeatureContactBuf ec,
+                          const int* groupBase, const MeshGeo* geo, float thickness, float mu, SolverCtl* ctl)
+{
+    for (int g = 0; g < ec.groups; g++) {
+        const int cap = groupBase[g + 1] - groupBase[g];
+        if (cap > 0)
+            hipLaunchKernelGGL(k_edge_contacts, dim3(group_blocks(cap)), dim3(kBlock), 0, s, pos, prev, cEdges, ec,
+                               geo, thickness, mu, g, ctl);
+    }
 }
 
 void launch_tri_contacts(hipStream_t s, float4* pos, const float4* prev, const int4* cTris, FeatureContactBuf tc,
-                         const MeshGeo* geo, float thickness, float mu, SolverCtl* ctl)
+                         const int* groupBase, const MeshGeo* geo, float thickness, float mu, SolverCtl* ctl)
 {
-    hipLaunchKernelGGL(k_tri_contacts, dim3(1), dim3(kSolveBlock), 0, s, pos, prev, cTris, tc, geo, thickness, mu,
-                       ctl);
+    for (int g = 0; g < tc.groups; g++) {
+        const int cap = groupBase[g + 1] - groupBase[g];
+        if (cap > 0)
+            hipLaunchKernelGGL(k_tri_contacts, dim3(group_blocks(cap)), dim3(kBlock), 0, s, pos, prev, cTris, tc, geo,
+                               thickness, mu, g, ctl);
+    }
 }
 
 }  // namespace sc

@@ -596,6 +596,8 @@ void ClothModel::build_device_topology()
         if (n) HIP_OK(hipMemset(f.count, 0, n * sizeof(int)));
         HIP_OK(hipMemset(f.groupCount, 0, std::max(f.groups, 1) * sizeof(int)));
         HIP_OK(hipMemcpy(f.groupBase, c.phaseOffsets.data(), (f.groups + 1) * sizeof(int), hipMemcpyHostToDevice));
+        f.base = f.groups > 0 ? std::vector<int>(c.phaseOffsets.begin(), c.phaseOffsets.begin() + f.groups + 1)
+                              : std::vector<int>(1, 0);
         f.slotCap = 0;
         f.slots = 0;
         dev_free(f.P); dev_free(f.N); dev_free(f.W); dev_free(f.F);
@@ -1126,12 +1128,14 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
         }
         if (edgeContacts) {
             timed_begin(g.timer, timed, 2, s, ev);
-            launch_edge_contacts(s, g.d_pos, g.d_prev, g.d_cEdges, ecb, g.d_geo, mThickness, mFriction, g.d_ctl);
+            launch_edge_contacts(s, g.d_pos, g.d_prev, g.d_cEdges, ecb, g.ec.base.data(), g.d_geo, mThickness,
+                                 mFriction, g.d_ctl);
             timed_end(g.timer, timed, 2, s, ev);
         }
         if (triContacts) {
             timed_begin(g.timer, timed, 2, s, ev);
-            launch_tri_contacts(s, g.d_pos, g.d_prev, g.d_cTris, tcb, g.d_geo, mThickness, mFriction, g.d_ctl);
+            launch_tri_contacts(s, g.d_pos, g.d_prev, g.d_cTris, tcb, g.tc.base.data(), g.d_geo, mThickness,
+                                mFriction, g.d_ctl);
             timed_end(g.timer, timed, 2, s, ev);
         }
         if (selfColl) {

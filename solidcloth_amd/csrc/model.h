@@ -242,6 +242,7 @@ public:
         int2* F = nullptr;
         int *group = nullptr, *groupBase = nullptr, *groupCount = nullptr, *list = nullptr;
         int n = 0, groups = 0, slots = 0, slotCap = 0;
+        std::vector<int> base;  // host copy of groupBase
     } ec, tc;
     // continuous self collision: swept boxes, LBVHs over triangles and edges,
     // contact lists of both kinds, per-vertex colouring bids
