@@ -319,7 +319,7 @@ SCEXP int scGetTriContactData(scModel model, int* ids, float* data, int cap);
  * scGetSelfStats: out[0] contact capacity in use, out[1] steps rerun after
  * an overflow so far, out[2] / out[3] most vertex-triangle / edge-edge
  * contacts in one substep of the last step, out[4] candidate-pair pool
- * capacity per kind, out[5] / out[6] vertex-triangle / edge-edge colour
+ * capacity (both kinds, pairs), out[5] / out[6] vertex-triangle / edge-edge colour
  * groups of the last substep. Returns the number of values (7), writing at
  * most count. No counterpart in the reference (its lists are std::vectors). */
 SCEXP int scGetSelfContacts(scModel model, int kind, unsigned long long* keys, int* ids, float* data, int cap);

@@ -612,7 +612,7 @@ class ClothModel:
     def self_stats(self):
         """Contact capacity in use, steps rerun after an overflow, most
         vertex-triangle / edge-edge contacts in one substep of the last step,
-        candidate-pair pool capacity, colour groups of the last substep."""
+        candidate-pair pool capacity (both kinds), colour groups of the last substep."""
         out = np.zeros(7, dtype=np.int32)
         _checked("scGetSelfStats", self.h, _ptr(out), 7)
         return dict(capacity=int(out[0]), reruns=int(out[1]), peak_vt=int(out[2]), peak_ee=int(out[3]),

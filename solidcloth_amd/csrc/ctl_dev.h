@@ -13,8 +13,8 @@ __device__ __forceinline__ void begin_substep_ctl(SolverCtl* ctl)
 {
     ctl->selfPeak[0] = max(ctl->selfPeak[0], ctl->selfCount[0]);
     ctl->selfPeak[1] = max(ctl->selfPeak[1], ctl->selfCount[1]);
-    ctl->selfPeak[2] = max(ctl->selfPeak[2], ctl->candExtra[0]);
-    ctl->selfPeak[3] = max(ctl->selfPeak[3], ctl->candExtra[1]);
+    ctl->selfPeak[2] = max(ctl->selfPeak[2], ctl->candSlots[0]);
+    ctl->selfPeak[3] = max(ctl->selfPeak[3], ctl->candSlots[1]);
     ctl->done = 0;
     ctl->k = 0;
     ctl->cb = 0;
@@ -26,10 +26,8 @@ __device__ __forceinline__ void begin_substep_ctl(SolverCtl* ctl)
     ctl->cl = 0;
     ctl->selfCount[0] = 0;
     ctl->selfCount[1] = 0;
-    ctl->candCount[0] = 0;
-    ctl->candCount[1] = 0;
-    ctl->candExtra[0] = 0;
-    ctl->candExtra[1] = 0;
+    ctl->candSlots[0] = 0;
+    ctl->candSlots[1] = 0;
     ctl->iterations = 0;
 }
 

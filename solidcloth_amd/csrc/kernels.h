@@ -44,9 +44,8 @@ struct SolverCtl {
     int selfColours[2];  // colour groups of each kind
     int selfOverflow;    // contacts or candidate pairs beyond capacity
     unsigned cl;         // link error of the current iteration (link model)
-    int candCount[2];    // largest number of self candidate pairs of one query (vertex, edge)
-    int candExtra[2];    // pairs beyond a query's slots, in the shared pools (vertex, edge)
-    // largest selfCount[0..1] and candExtra[0..1] of the step's earlier
+    int candSlots[2];    // candidate pool slots reserved this substep (vertex, edge; may pass the capacity)
+    // largest selfCount[0..1] and candSlots[0..1] of the step's earlier
     // substeps (the host clears them before a step and folds in the last)
     int selfPeak[4];
 };
@@ -310,18 +309,17 @@ void launch_tri_contacts(hipStream_t s, float4* pos, const float4* prev, const i
 void launch_self_boxes(hipStream_t s, const float4* pos, const float4* prev, const int4* cTris, int nt,
                        const int2* cEdges, int ne, float tol, float4* triLo, float4* triHi, float4* edgeLo,
                        float4* edgeHi, int nv, float4* vtxLo, float4* vtxHi);
-// Candidate pairs (triangle, vertex) and (edge1, edge2): query j of a kind
-// owns slots [j k, j k + k) and writes their number to cnt[j]; pairs beyond
-// its slots go to a shared pool (capacity extraCap per kind) through one
-// counter; then the continuous tests over them.
+// Candidate pairs (triangle, vertex) and (edge1, edge2) in one flat pool per
+// kind, reserved in chunks of kCandChunk slots; fill[c] is the number of
+// pairs in chunk c (a chunk's tail may be unused). Capacities are multiples
+// of kCandChunk. Then the continuous tests over them.
+constexpr int kCandChunk = 256;
 struct SelfCandidates {
     int2* vt;
     int2* ee;
-    int* cntVT;
-    int* cntEE;
-    int2* extraVT;
-    int2* extraEE;
-    int kVT, kEE, extraCap;
+    int* fillVT;
+    int* fillEE;
+    int capVT, capEE;
 };
 // vertexOrder: the order in which vertices query the triangle tree (may be null)
 void launch_self_detect(hipStream_t s, const float4* pos, const float4* prev, const int4* cTris, const int2* cEdges,

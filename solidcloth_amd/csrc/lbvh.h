@@ -29,6 +29,7 @@ struct BvhView {
     // kRopeEnd after the last); with them an overlap query needs no stack
     const int* ropeNode;
     const int* ropeLeaf;
+    const int2* range;    // internal nodes: first and last sorted leaf below
     int n;                // number of primitives (leaves)
 };
 

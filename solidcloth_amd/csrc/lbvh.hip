@@ -536,6 +536,9 @@ int Lbvh::verify(hipStream_t s, const float4* boxLo, const float4* boxHi) const
     return bad;
 }
 
-BvhView Lbvh::view() const { return BvhView{lo_, hi_, leafLo_, leafHi_, leafPrim_, ropeNode_, ropeLeaf_, n_}; }
+BvhView Lbvh::view() const
+{
+    return BvhView{lo_, hi_, leafLo_, leafHi_, leafPrim_, ropeNode_, ropeLeaf_, range_, n_};
+}
 
 }  // namespace sc

@@ -326,8 +326,9 @@ void GpuSolver::release()
     }
     dev_free(self.triLo); dev_free(self.triHi); dev_free(self.edgeLo); dev_free(self.edgeHi); dev_free(self.vmax);
     dev_free(self.candVT); dev_free(self.candEE); dev_free(self.vtxLo); dev_free(self.vtxHi);
-    dev_free(self.candCntVT); dev_free(self.candCntEE); dev_free(self.candExtraVT); dev_free(self.candExtraEE);
-    self.candKVT = self.candKEE = self.candExtraCap = 0;
+    dev_free(self.candFillVT); dev_free(self.candFillEE);
+    self.candCapVT = self.candCapEE = 0;
+    self.candWantVT = self.candWantEE = 0;
     for (SelfBuf& b : self.buf) {
         dev_free(b.ids); dev_free(b.W); dev_free(b.N); dev_free(b.key); dev_free(b.colour); dev_free(b.order);
         dev_free(b.active); dev_free(b.groupOff);
@@ -755,22 +756,22 @@ void ClothModel::ensure_self()
         S.cap = cap;
         g.release_graphs();
     }
-    // candidate slots per query: 96 for vertices and for edges (the 512^2
-    // perturbed cloth with 2 mm thickness has about 65 and 53 on average, a
-    // few hundred where it folds); pairs beyond them go to shared pools of
-    // 16 Mi pairs per kind, grown when a step came near their end
-    const int kVT = 96, kEE = 96;
-    const int extraCap = std::max(16 << 20, std::max(S.candWantVT, S.candWantEE));
-    if (kVT != S.candKVT || kEE != S.candKEE || extraCap != S.candExtraCap) {
-        dev_alloc(S.candVT, (size_t)kVT * std::max(g.nv, 1));
-        dev_alloc(S.candEE, (size_t)kEE * std::max(g.ne, 1));
-        dev_alloc(S.candCntVT, std::max(g.nv, 1));
-        dev_alloc(S.candCntEE, std::max(g.ne, 1));
-        dev_alloc(S.candExtraVT, extraCap);
-        dev_alloc(S.candExtraEE, extraCap);
-        S.candKVT = kVT;
-        S.candKEE = kEE;
-        S.candExtraCap = extraCap;
+    // candidate pools: 80 pairs per vertex and 64 per edge to start with (the
+    // 512^2 perturbed cloth with 2 mm thickness has about 65 and 55), grown to
+    // 1.5 x a step's peak when it came near the end, in whole chunks
+    auto chunks = [](long pairs) {
+        const long c = (std::max(pairs, (long)kCandChunk) + kCandChunk - 1) / kCandChunk * kCandChunk;
+        return (int)std::min(c, (long)(INT_MAX / kCandChunk) * kCandChunk);
+    };
+    const int capVT = chunks(std::max(80L * g.nv, S.candWantVT));
+    const int capEE = chunks(std::max(64L * g.ne, S.candWantEE));
+    if (capVT != S.candCapVT || capEE != S.candCapEE) {
+        dev_alloc(S.candVT, (size_t)capVT);
+        dev_alloc(S.candEE, (size_t)capEE);
+        dev_alloc(S.candFillVT, capVT / kCandChunk);
+        dev_alloc(S.candFillEE, capEE / kCandChunk);
+        S.candCapVT = capVT;
+        S.candCapEE = capEE;
         g.release_graphs();
     }
     if (!S.triLo) {
@@ -901,9 +902,8 @@ std::string ClothModel::graph_key(float h, const CollSet& cs) const
     const BvhView tv = gpu.self.triB.view(), ev = gpu.self.edgeB.view();
     put(&tv, sizeof(tv)); put(&ev, sizeof(ev));
     put(&gpu.self.buf[0].ids, sizeof(void*)); put(&gpu.self.buf[1].ids, sizeof(void*));
-    put(&gpu.self.candVT, sizeof(void*)); put(&gpu.self.candEE, sizeof(void*)); put(&gpu.self.candKVT, 4);
-    put(&gpu.self.candKEE, 4); put(&gpu.self.candCntVT, sizeof(void*)); put(&gpu.self.candExtraVT, sizeof(void*));
-    put(&gpu.self.candExtraCap, 4);
+    put(&gpu.self.candVT, sizeof(void*)); put(&gpu.self.candEE, sizeof(void*)); put(&gpu.self.candCapVT, 4);
+    put(&gpu.self.candCapEE, 4); put(&gpu.self.candFillVT, sizeof(void*)); put(&gpu.self.candFillEE, sizeof(void*));
     const BvhView vv = gpu.self.vtxB.view();
     put(&vv, sizeof(vv));
     const int wmode = mWarmPlainNow ? 0 : mWarmMode;
@@ -997,8 +997,8 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
             S.edgeB.refit(s, S.edgeLo, S.edgeHi);
             launch_self_detect(s, g.d_pos, g.d_prev, g.d_cTris, g.d_cEdges, g.nv, g.ne, S.triB.view(),
                                S.edgeB.view(), S.vtxB.view().leafPrim, sp,
-                               SelfCandidates{S.candVT, S.candEE, S.candCntVT, S.candCntEE, S.candExtraVT,
-                                              S.candExtraEE, S.candKVT, S.candKEE, S.candExtraCap},
+                               SelfCandidates{S.candVT, S.candEE, S.candFillVT, S.candFillEE, S.candCapVT,
+                                              S.candCapEE},
                                S.buf[0], S.buf[1], g.d_ctl);
             launch_self_colour(s, S.buf[0], 0, S.vmax, g.d_ctl);
             launch_self_colour(s, S.buf[1], 1, S.vmax, g.d_ctl);
@@ -1380,19 +1380,19 @@ bool ClothModel::self_step_overflowed(int attempt)
     SolverCtl c;
     HIP_OK(hipMemcpy(&c, g.d_ctl, sizeof(c), hipMemcpyDeviceToHost));
     const long peak[4] = {std::max(c.selfPeak[0], c.selfCount[0]), std::max(c.selfPeak[1], c.selfCount[1]),
-                          std::max(c.selfPeak[2], c.candExtra[0]), std::max(c.selfPeak[3], c.candExtra[1])};
+                          std::max(c.selfPeak[2], c.candSlots[0]), std::max(c.selfPeak[3], c.candSlots[1])};
     lastSelfPeak[0] = (int)peak[0];
     lastSelfPeak[1] = (int)peak[1];
     lastSelfColours[0] = c.selfColours[0];
     lastSelfColours[1] = c.selfColours[1];
     const long limit = 1L << 30;
-    if (peak[2] > S.candExtraCap / 5 * 4) S.candWantVT = (int)std::min(limit, 2 * peak[2]);
-    if (peak[3] > S.candExtraCap / 5 * 4) S.candWantEE = (int)std::min(limit, 2 * peak[3]);
+    if (peak[2] > S.candCapVT / 5 * 4) S.candWantVT = std::min(limit, peak[2] + peak[2] / 2);
+    if (peak[3] > S.candCapEE / 5 * 4) S.candWantEE = std::min(limit, peak[3] + peak[3] / 2);
     if (!c.selfOverflow) return false;
     selfReruns++;
     const long need = std::max(peak[0], peak[1]);
     const bool contactsFull = need > S.cap;
-    const bool poolsFull = peak[2] > S.candExtraCap || peak[3] > S.candExtraCap;
+    const bool poolsFull = peak[2] > S.candCapVT || peak[3] > S.candCapEE;
     if (contactsFull) mSelfCapacityGrown = (int)std::min(limit, std::max(2L * S.cap, need + need / 2));
     if ((!contactsFull && !poolsFull) || attempt >= 8 || need >= limit) {
         restore_step();
@@ -1401,7 +1401,7 @@ bool ClothModel::self_step_overflowed(int attempt)
             "self collision overflowed and the step was not taken: " + std::to_string(peak[0]) +
             " vertex-triangle and " + std::to_string(peak[1]) + " edge-edge contacts (capacity " +
             std::to_string(S.cap) + "), " + std::to_string(peak[2]) + " / " + std::to_string(peak[3]) +
-            " candidate pairs beyond the per-query slots (pool " + std::to_string(S.candExtraCap) + "), " +
+            " candidate slots (pools " + std::to_string(S.candCapVT) + " / " + std::to_string(S.candCapEE) + "), " +
             std::to_string(c.selfColours[0]) + " / " + std::to_string(c.selfColours[1]) + " colour groups after " +
             std::to_string(attempt + 1) + " attempts");
     }
@@ -1409,8 +1409,8 @@ bool ClothModel::self_step_overflowed(int attempt)
 }
 
 // The last substep's broad-phase candidate pairs of one kind (0: (triangle,
-// vertex), 1: (edge1, edge2)): every query's own slots, then the shared
-// pool, sorted. Empty before the first self-colliding step.
+// vertex), 1: (edge1, edge2)): the filled slots of the pool's chunks,
+// sorted. Empty before the first self-colliding step.
 std::vector<std::pair<int, int>> ClothModel::self_candidates(int kind)
 {
     std::vector<std::pair<int, int>> out;
@@ -1420,21 +1420,16 @@ std::vector<std::pair<int, int>> ClothModel::self_candidates(int kind)
     HIP_OK(hipStreamSynchronize(g.stream));
     SolverCtl c;
     HIP_OK(hipMemcpy(&c, g.d_ctl, sizeof(c), hipMemcpyDeviceToHost));
-    const int queries = kind == 0 ? g.nv : g.ne;
-    const int K = kind == 0 ? S.candKVT : S.candKEE;
-    std::vector<int> cnt(queries);
-    std::vector<int2> slots((size_t)queries * K);
-    HIP_OK(hipMemcpy(cnt.data(), kind == 0 ? S.candCntVT : S.candCntEE, queries * sizeof(int), hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(slots.data(), kind == 0 ? S.candVT : S.candEE, slots.size() * sizeof(int2),
+    const int cap = kind == 0 ? S.candCapVT : S.candCapEE;
+    const int slots = std::min(c.candSlots[kind], cap);
+    if (slots <= 0) return out;
+    std::vector<int> fill(slots / kCandChunk);
+    std::vector<int2> pool(slots);
+    HIP_OK(hipMemcpy(fill.data(), kind == 0 ? S.candFillVT : S.candFillEE, fill.size() * sizeof(int),
                      hipMemcpyDeviceToHost));
-    const int extra = std::min(c.candExtra[kind], S.candExtraCap);
-    std::vector<int2> pool(std::max(extra, 0));
-    if (extra > 0)
-        HIP_OK(hipMemcpy(pool.data(), kind == 0 ? S.candExtraVT : S.candExtraEE, extra * sizeof(int2),
-                         hipMemcpyDeviceToHost));
-    for (int j = 0; j < queries; j++)
-        for (int k = 0; k < std::min(cnt[j], K); k++) out.push_back({slots[(size_t)j * K + k].x, slots[(size_t)j * K + k].y});
-    for (const int2& p : pool) out.push_back({p.x, p.y});
+    HIP_OK(hipMemcpy(pool.data(), kind == 0 ? S.candVT : S.candEE, pool.size() * sizeof(int2), hipMemcpyDeviceToHost));
+    for (size_t ch = 0; ch < fill.size(); ch++)
+        for (int k = 0; k < fill[ch]; k++) out.push_back({pool[ch * kCandChunk + k].x, pool[ch * kCandChunk + k].y});
     std::sort(out.begin(), out.end());
     return out;
 }

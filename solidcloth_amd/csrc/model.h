@@ -252,11 +252,10 @@ public:
         Lbvh triB, edgeB, vtxB;  // vtxB: Morton order of the vertex queries (built per step)
         SelfBuf buf[2]{};
         unsigned long long* vmax = nullptr;
-        int2 *candVT = nullptr, *candEE = nullptr;  // candidate pairs of the traversal, per query
-        int *candCntVT = nullptr, *candCntEE = nullptr;
-        int2 *candExtraVT = nullptr, *candExtraEE = nullptr;
-        int candExtraCap = 0;
-        int candKVT = 0, candKEE = 0, candWantVT = 0, candWantEE = 0;  // slots per query
+        int2 *candVT = nullptr, *candEE = nullptr;  // candidate pools (chunks of kCandChunk pairs)
+        int *candFillVT = nullptr, *candFillEE = nullptr;  // pairs per chunk
+        int candCapVT = 0, candCapEE = 0;    // pool capacities in pairs
+        long candWantVT = 0, candWantEE = 0;  // capacities asked for by earlier steps' peaks
         int cap = 0;
         bool built = false;
     } self;

@@ -13,6 +13,7 @@ namespace sc {
 
 static constexpr int kBlock = 256;
 static constexpr int kSoloBlock = 1024;
+static constexpr int kCcdBlocks = 4096;
 
 // --------------------------------------------------------------- FP64 ----
 
@@ -177,6 +178,35 @@ __device__ __forceinline__ void bvh_overlap(const BvhView& bv, f3 lo, f3 hi, Vis
     }
 }
 
+// The overlap walk over the leaves after sorted leaf `first` only: the walk
+// is left-first, so a subtree whose last leaf is at or before `first` is
+// skipped whole. Each unordered pair of one tree's leaves is then found by
+// exactly one of its two leaves' queries.
+template <class Visit>
+__device__ __forceinline__ void bvh_overlap_after(const BvhView& bv, f3 lo, f3 hi, int first, Visit visit)
+{
+    if (bv.n <= 1) return;
+    int node = 0;
+    for (int steps = 2 * bv.n; node != kRopeEnd && steps > 0; steps--) {
+        if (node >= 0) {
+            if (bv.range[node].y <= first) {
+                node = bv.ropeNode[node];
+                continue;
+            }
+            const float4 L = bv.lo[node];
+            if (overlap(lo, hi, L, bv.hi[node])) {
+                node = __float_as_int(L.w);
+                continue;
+            }
+            node = bv.ropeNode[node];
+        } else {
+            const int leaf = ~node;
+            if (leaf > first && overlap(lo, hi, bv.leafLo[leaf], bv.leafHi[leaf])) visit(leaf);
+            node = bv.ropeLeaf[leaf];
+        }
+    }
+}
+
 // ClothCollision.cpp:256-315: prev and pos of the primitive's vertices,
 // extruded by tol (Aabb3::Extrude).
 __device__ __forceinline__ void swept(f3 p, f3& lo, f3& hi)
@@ -301,15 +331,56 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb)
     return start + k;
 }
 
-// SelfCollisionDetector.cpp:60-135 (Midphase) over the exact candidate set:
-// blocks [0, vertexBlocks) take vertices (against the triangle tree, own
-// triangles and pinned vertices skipped), the rest edges (against the edge
-// tree, partner index greater, no shared vertex). Queries run in Morton
-// order (the trees' leaf orders), so that neighbouring threads walk
-// neighbouring parts of a tree. Query j writes its pairs to its own slots
-// [j K, j K + K) and their number to cnt[j]: no shared counter. The
-// continuous tests run in k_self_ccd, so that the traversal's divergence and
-// the tests' FP64 registers do not meet in one kernel.
+// Appends pair p for each lane of the wave that calls it at this point of its
+// walk: the calling lanes take consecutive slots of the wave's current chunk
+// of the pool (one ballot), and the leading lane reserves a fresh chunk of
+// kCandChunk slots with one atomic when the current one cannot take them
+// all (its unused tail is left, and its fill count records how many slots
+// hold pairs). The wave's chunk (base, used) lives in LDS, read and written
+// only by the leading lane of each call; the leader changes between calls,
+// so the words are volatile (every access goes to LDS). Chunks are reserved
+// whole, so a chunk lies either inside the pool's capacity or past it; a
+// pair past it sets the overflow.
+__device__ __forceinline__ void wave_emit(int2 p, int2* pool, int* fill, int cap, int* reserved,
+                                          volatile int* wbase, volatile int* wused, SolverCtl* ctl)
+{
+    const unsigned long long act = __ballot(1);
+    const int lane = __lane_id();
+    const int cnt = __popcll(act);
+    const int rank = __popcll(act & ((1ull << lane) - 1ull));
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    int base = 0, used = 0;
+    if (lane == leader) {
+        base = *wbase;
+        used = *wused;
+        if (used + cnt > kCandChunk) {
+            if (base >= 0 && base < cap) fill[base / kCandChunk] = used;
+            base = atomicAdd(reserved, kCandChunk);
+            used = 0;
+            *wbase = base;
+        }
+        *wused = used + cnt;
+    }
+    base = __shfl(base, leader);
+    used = __shfl(used, leader);
+    const int slot = base + used + rank;
+    if (slot < cap) pool[slot] = p;
+    else atomicExch(&ctl->selfOverflow, 1);
+}
+
+// Broad phase (SelfCollisionDetector.cpp:60-135 over the trees
+// ClothModel::ComputeTree builds, ClothCollision.cpp:240-315): every vertex
+// queries the triangle tree with its swept box, every edge the edge tree; a
+// pair whose boxes overlap (closed intervals, as Aabb3::Intersect) is a
+// candidate unless the vertex belongs to the triangle or the edges share a
+// vertex (and, like the reference's pass over leaf pairs, a pinned vertex is
+// never a candidate). Queries run in Morton order (the trees' leaf orders),
+// so that neighbouring threads walk neighbouring parts of a tree. An edge
+// query pairs only with the edge tree's later leaves, so each edge pair is
+// found once, and is written with the lower edge index first. Pairs go to
+// flat pools in chunks (wave_emit): waves write runs of consecutive slots.
+// The continuous tests run in k_self_ccd, so that the traversal's divergence
+// and the tests' FP64 registers do not meet in one kernel.
 __global__ void __launch_bounds__(kBlock) k_self_candidates(const float4* __restrict__ pos,
                                                             const float4* __restrict__ prev,
                                                             const int4* __restrict__ cTris,
@@ -318,26 +389,22 @@ __global__ void __launch_bounds__(kBlock) k_self_candidates(const float4* __rest
                                                             const int* __restrict__ vertexOrder, float tol,
                                                             SelfCandidates cand, SolverCtl* ctl)
 {
+    __shared__ volatile int wbase[kBlock / 64], wused[kBlock / 64];
     const int blk = xcd_logical_block(blockIdx.x, gridDim.x);
     const bool vertexPass = blk < vertexBlocks;
     const int j = (vertexPass ? blk : blk - vertexBlocks) * blockDim.x + threadIdx.x;
-    if (j >= (vertexPass ? nv : ne)) return;
-    const int K = vertexPass ? cand.kVT : cand.kEE;
-    int2* out = (vertexPass ? cand.vt : cand.ee) + (size_t)j * K;
-    int2* extra = vertexPass ? cand.extraVT : cand.extraEE;
-    int* extraCount = &ctl->candExtra[vertexPass ? 0 : 1];
-    int n = 0;
-    auto emit = [&](int2 p) {
-        if (n < K) {
-            out[n] = p;
-        } else {
-            const int x = atomicAdd(extraCount, 1);
-            if (x < cand.extraCap) extra[x] = p;
-            else atomicExch(&ctl->selfOverflow, 1);
-        }
-        n++;
-    };
-    if (vertexPass) {
+    const int w = threadIdx.x >> 6;
+    if (__lane_id() == 0) {
+        wbase[w] = -1;
+        wused[w] = kCandChunk;  // no chunk yet: the first pair reserves one
+    }
+    __builtin_amdgcn_wave_barrier();
+    int2* pool = vertexPass ? cand.vt : cand.ee;
+    int* fill = vertexPass ? cand.fillVT : cand.fillEE;
+    const int cap = vertexPass ? cand.capVT : cand.capEE;
+    int* reserved = &ctl->candSlots[vertexPass ? 0 : 1];
+    auto emit = [&](int2 p) { wave_emit(p, pool, fill, cap, reserved, &wbase[w], &wused[w], ctl); };
+    if (vertexPass && j < nv) {
         const int v = vertexOrder ? vertexOrder[j] : j;
         const float4 P = pos[v];
         if (P.w != 0.f) {
@@ -353,8 +420,7 @@ __global__ void __launch_bounds__(kBlock) k_self_candidates(const float4* __rest
                 emit(make_int2(t, v));
             });
         }
-        cand.cntVT[j] = n;
-    } else {
+    } else if (!vertexPass && j < ne) {
         const int e1 = edgeBvh.leafPrim[j];
         const int2 ev = cEdges[e1];
         f3 lo{1e7f, 1e7f, 1e7f}, hi{-1e7f, -1e7f, -1e7f};
@@ -364,17 +430,20 @@ __global__ void __launch_bounds__(kBlock) k_self_candidates(const float4* __rest
         swept(xyz(pos[ev.y]), lo, hi);
         lo = f3{lo.x - tol, lo.y - tol, lo.z - tol};
         hi = f3{hi.x + tol, hi.y + tol, hi.z + tol};
-        bvh_overlap(edgeBvh, lo, hi, [&](int leaf) {
+        // query j (sorted leaf j) pairs with the overlapping leaves after it
+        bvh_overlap_after(edgeBvh, lo, hi, j, [&](int leaf) {
             const int e2 = __float_as_int(edgeBvh.leafLo[leaf].w);
-            if (e2 <= e1) return;
             const int2 fv = cEdges[e2];
             if (ev.x == fv.x || ev.y == fv.y || ev.x == fv.y || ev.y == fv.x) return;
-            emit(make_int2(e1, e2));
+            emit(e1 < e2 ? make_int2(e1, e2) : make_int2(e2, e1));
         });
-        cand.cntEE[j] = n;
     }
-    // the largest count per kind sizes the slots of later steps
-    atomicMax(&ctl->candCount[vertexPass ? 0 : 1], n);
+    // every lane of the wave is back here: record the fill of its last chunk
+    __builtin_amdgcn_wave_barrier();
+    if (__lane_id() == 0) {
+        const int b = wbase[w];
+        if (b >= 0 && b < cap) fill[b / kCandChunk] = wused[w];
+    }
 }
 
 __device__ __forceinline__ void ccd_vt(const float4* pos, const float4* prev, const int4* cTris, int2 c,
@@ -396,36 +465,23 @@ __device__ __forceinline__ void ccd_ee(const float4* pos, const float4* prev, co
 }
 
 // SelfCollisionDetector.cpp:137-190, 23-58: the continuous tests of the
-// candidate pairs; thread j tests query j's pairs, the workgroups after the
-// queries' the shared pools' pairs (grid-stride, counts read on the device).
+// candidate pairs, one slot of the pools per thread (grid-stride; the
+// reserved slot counts are read on the device, and a slot past its chunk's
+// fill holds no pair).
 __global__ void __launch_bounds__(kBlock) k_self_ccd(const float4* __restrict__ pos, const float4* __restrict__ prev,
                                                      const int4* __restrict__ cTris, const int2* __restrict__ cEdges,
-                                                     int nv, int ne, int vertexBlocks, SelfParams sp,
-                                                     SelfCandidates cand, SelfBuf vt, SelfBuf ee, SolverCtl* ctl)
+                                                     SelfParams sp, SelfCandidates cand, SelfBuf vt, SelfBuf ee,
+                                                     SolverCtl* ctl)
 {
-    const int queryBlocks = vertexBlocks + (ne + kBlock - 1) / kBlock;
-    if ((int)blockIdx.x >= queryBlocks) {
-        const int nx = min(ctl->candExtra[0], cand.extraCap), ny = min(ctl->candExtra[1], cand.extraCap);
-        const int stride = (gridDim.x - queryBlocks) * blockDim.x;
-        for (int i = (blockIdx.x - queryBlocks) * blockDim.x + threadIdx.x; i < nx + ny; i += stride) {
-            if (i < nx) ccd_vt(pos, prev, cTris, cand.extraVT[i], sp, vt, ctl);
-            else ccd_ee(pos, prev, cEdges, cand.extraEE[i - nx], sp, ee, ctl);
+    const int nvt = min(ctl->candSlots[0], cand.capVT), nee = min(ctl->candSlots[1], cand.capEE);
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nvt + nee; i += stride) {
+        if (i < nvt) {
+            if (i % kCandChunk < cand.fillVT[i / kCandChunk]) ccd_vt(pos, prev, cTris, cand.vt[i], sp, vt, ctl);
+        } else {
+            const int k = i - nvt;
+            if (k % kCandChunk < cand.fillEE[k / kCandChunk]) ccd_ee(pos, prev, cEdges, cand.ee[k], sp, ee, ctl);
         }
-        return;
-    }
-    const int blk = xcd_logical_block(blockIdx.x, queryBlocks);
-    const bool vertexPass = blk < vertexBlocks;
-    const int j = (vertexPass ? blk : blk - vertexBlocks) * blockDim.x + threadIdx.x;
-    if (vertexPass) {
-        if (j >= nv) return;
-        const int2* in = cand.vt + (size_t)j * cand.kVT;
-        const int n = min(cand.cntVT[j], cand.kVT);
-        for (int k = 0; k < n; k++) ccd_vt(pos, prev, cTris, in[k], sp, vt, ctl);
-    } else {
-        if (j >= ne) return;
-        const int2* in = cand.ee + (size_t)j * cand.kEE;
-        const int n = min(cand.cntEE[j], cand.kEE);
-        for (int k = 0; k < n; k++) ccd_ee(pos, prev, cEdges, in[k], sp, ee, ctl);
     }
 }
 
@@ -694,9 +750,9 @@ void launch_self_detect(hipStream_t s, const float4* pos, const float4* prev, co
     if (vb + eb == 0) return;
     hipLaunchKernelGGL(k_self_candidates, dim3(vb + eb), dim3(kBlock), 0, s, pos, prev, cTris, cEdges, nv, ne, vb,
                        triBvh, edgeBvh, vertexOrder, sp.tol, cand, ctl);
-    // the query workgroups, then 1024 more for the shared pools
-    hipLaunchKernelGGL(k_self_ccd, dim3(vb + eb + 1024), dim3(kBlock), 0, s, pos, prev, cTris, cEdges, nv, ne, vb, sp,
-                       cand, vt, ee, ctl);
+    // enough workgroups to fill the chip several times over; they stride
+    hipLaunchKernelGGL(k_self_ccd, dim3(kCcdBlocks), dim3(kBlock), 0, s, pos, prev, cTris, cEdges, sp, cand, vt, ee,
+                       ctl);
 }
 
 void launch_self_colour(hipStream_t s, SelfBuf b, int kind, unsigned long long* vmax, SolverCtl* ctl)
