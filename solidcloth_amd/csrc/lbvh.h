@@ -30,6 +30,12 @@ struct BvhView {
     const int* ropeNode;
     const int* ropeLeaf;
     const int2* range;    // internal nodes: first and last sorted leaf below
+    // The same hierarchy packed for the rope walks, one cache line per step:
+    // internal node i at nodes[4 i .. 4 i + 2] = (min, left child), (max,
+    // right child), (rope, first leaf, last leaf, -) as int bits; leaf k at
+    // leaves[2 k], leaves[2 k + 1] = (min, primitive), (max, rope).
+    const float4* nodes;
+    const float4* leaves;
     int n;                // number of primitives (leaves)
 };
 
@@ -52,6 +58,7 @@ public:
 
 private:
     void reserve(int n);
+    void pack(hipStream_t s);
     int n_ = 0, cap_ = 0, tiles_ = 0;
     float4 *lo_ = nullptr, *hi_ = nullptr, *leafLo_ = nullptr, *leafHi_ = nullptr;
     int* leafPrim_ = nullptr;
@@ -63,6 +70,7 @@ private:
     int2* range_ = nullptr;       // sorted-leaf range of each internal node
     unsigned* enc_ = nullptr;     // per internal node: order-encoded min (3) / max (3) words
     int *ropeNode_ = nullptr, *ropeLeaf_ = nullptr;
+    float4 *nodes_ = nullptr, *leaves_ = nullptr;  // packed copies (BvhView::nodes, leaves)
 };
 
 }  // namespace sc

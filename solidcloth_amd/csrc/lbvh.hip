@@ -387,6 +387,29 @@ __global__ void __launch_bounds__(kRefitBlock) k_refit(const float4* __restrict_
     }
 }
 
+// The packed copy of the hierarchy (BvhView::nodes / leaves), written after
+// every build and refit: threads [0, n - 1) internal nodes, [n - 1, 2n - 1)
+// leaves.
+__global__ void __launch_bounds__(kBlock) k_pack(const float4* __restrict__ lo, const float4* __restrict__ hi,
+                                                 const float4* __restrict__ leafLo, const float4* __restrict__ leafHi,
+                                                 const int* __restrict__ ropeNode, const int* __restrict__ ropeLeaf,
+                                                 const int2* __restrict__ range, int n, float4* nodes, float4* leaves)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < n - 1) {
+        const int2 r = range[x];
+        nodes[4 * (size_t)x] = lo[x];
+        nodes[4 * (size_t)x + 1] = hi[x];
+        nodes[4 * (size_t)x + 2] =
+            make_float4(__int_as_float(ropeNode[x]), __int_as_float(r.x), __int_as_float(r.y), 0.f);
+    } else if (x < 2 * n - 1) {
+        const int k = x - (n - 1);
+        const float4 b = leafHi[k];
+        leaves[2 * (size_t)k] = leafLo[k];
+        leaves[2 * (size_t)k + 1] = make_float4(b.x, b.y, b.z, __int_as_float(ropeLeaf[k]));
+    }
+}
+
 inline int nblocks(int n, int b) { return (n + b - 1) / b; }
 
 }  // namespace
@@ -396,7 +419,7 @@ Lbvh::~Lbvh()
     dev_free(lo_); dev_free(hi_); dev_free(leafLo_); dev_free(leafHi_); dev_free(leafPrim_);
     dev_free(keys_); dev_free(keys2_); dev_free(vals_); dev_free(vals2_); dev_free(hist_);
     dev_free(parentNode_); dev_free(parentLeaf_); dev_free(flags_); dev_free(bounds_); dev_free(range_); dev_free(enc_);
-    dev_free(ropeNode_); dev_free(ropeLeaf_);
+    dev_free(ropeNode_); dev_free(ropeLeaf_); dev_free(nodes_); dev_free(leaves_);
 }
 
 void Lbvh::reserve(int n)
@@ -409,6 +432,7 @@ void Lbvh::reserve(int n)
     dev_alloc(parentNode_, n); dev_alloc(parentLeaf_, n); dev_alloc(flags_, n); dev_alloc(range_, n);
     dev_alloc(enc_, (size_t)6 * n);
     dev_alloc(ropeNode_, n); dev_alloc(ropeLeaf_, n);
+    dev_alloc(nodes_, (size_t)4 * n); dev_alloc(leaves_, (size_t)2 * n);
     if (!bounds_) dev_alloc(bounds_, 6);
     cap_ = n;
 }
@@ -442,7 +466,15 @@ void Lbvh::build(hipStream_t s, const float4* boxLo, const float4* boxHi, int n)
     if (n > 1)
         hipLaunchKernelGGL(k_ropes, dim3(nblocks(2 * n - 1, kBlock)), dim3(kBlock), 0, s, lo_, hi_, parentNode_,
                            parentLeaf_, n, ropeNode_, ropeLeaf_);
+    pack(s);
     HIP_OK(hipGetLastError());
+}
+
+void Lbvh::pack(hipStream_t s)
+{
+    if (n_ <= 1) return;  // a single leaf has no ropes: the walks test it directly
+    hipLaunchKernelGGL(k_pack, dim3(nblocks(2 * n_ - 1, kBlock)), dim3(kBlock), 0, s, lo_, hi_, leafLo_, leafHi_,
+                       ropeNode_, ropeLeaf_, range_, n_, nodes_, leaves_);
 }
 
 void Lbvh::refit(hipStream_t s, const float4* boxLo, const float4* boxHi)
@@ -451,6 +483,7 @@ void Lbvh::refit(hipStream_t s, const float4* boxLo, const float4* boxHi)
     // the sorted primitive order of the last build lives in vals_
     hipLaunchKernelGGL(k_refit, dim3(nblocks(n_, kRefitBlock)), dim3(kRefitBlock), 0, s, boxLo, boxHi, vals_, n_,
                        lo_, hi_, leafLo_, leafHi_, leafPrim_, parentNode_, parentLeaf_, flags_, range_, enc_);
+    pack(s);
 }
 
 int Lbvh::verify(hipStream_t s, const float4* boxLo, const float4* boxHi) const
@@ -538,7 +571,7 @@ int Lbvh::verify(hipStream_t s, const float4* boxLo, const float4* boxHi) const
 
 BvhView Lbvh::view() const
 {
-    return BvhView{lo_, hi_, leafLo_, leafHi_, leafPrim_, ropeNode_, ropeLeaf_, range_, n_};
+    return BvhView{lo_, hi_, leafLo_, leafHi_, leafPrim_, ropeNode_, ropeLeaf_, range_, nodes_, leaves_, n_};
 }
 
 }  // namespace sc

@@ -146,39 +146,45 @@ __device__ __forceinline__ bool overlap(f3 alo, f3 ahi, float4 blo, float4 bhi)
     return !(ahi.x < blo.x || ahi.y < blo.y || ahi.z < blo.z || bhi.x < alo.x || bhi.y < alo.y || bhi.z < alo.z);
 }
 
-// Visits every leaf whose box overlaps [lo, hi] (closed intervals, as
-// Geometry/Aabb3.h:72-87), depth-first and left-first along the escape
-// links: a node whose box overlaps is entered through its left child, any
-// other node (and every leaf) passes to its escape. No stack, one box load
-// per step.
+// Calls visit(primitive) for every leaf whose box overlaps [lo, hi] (closed
+// intervals, as Geometry/Aabb3.h:72-87), depth-first and left-first along the
+// escape links: a node whose box overlaps is entered through its left child,
+// any other node (and every leaf) passes to its escape. No stack, one packed
+// node (one cache line) per step.
 template <class Visit>
 __device__ __forceinline__ void bvh_overlap(const BvhView& bv, f3 lo, f3 hi, Visit visit)
 {
     if (bv.n <= 0) return;
     if (bv.n == 1) {
-        if (overlap(lo, hi, bv.leafLo[0], bv.leafHi[0])) visit(0);
+        const float4 L = bv.leafLo[0];
+        if (overlap(lo, hi, L, bv.leafHi[0])) visit(__float_as_int(L.w));
         return;
     }
     int node = 0;
     // a walk visits each of the 2n - 1 nodes at most once; the bound only
-    // guards against malformed links
+    // guards against malformed links. Steps read the packed copy: one node's
+    // box, child and rope share a cache line.
     for (int steps = 2 * bv.n; node != kRopeEnd && steps > 0; steps--) {
         if (node >= 0) {
-            const float4 L = bv.lo[node];
-            if (overlap(lo, hi, L, bv.hi[node])) {
+            const float4* N = bv.nodes + 4 * (size_t)node;
+            const float4 L = N[0];
+            if (overlap(lo, hi, L, N[1])) {
                 node = __float_as_int(L.w);
                 continue;
             }
-            node = bv.ropeNode[node];
+            node = __float_as_int(N[2].x);
         } else {
             const int leaf = ~node;
-            if (overlap(lo, hi, bv.leafLo[leaf], bv.leafHi[leaf])) visit(leaf);
-            node = bv.ropeLeaf[leaf];
+            const float4* F = bv.leaves + 2 * (size_t)leaf;
+            const float4 L = F[0], H = F[1];
+            if (overlap(lo, hi, L, H)) visit(__float_as_int(L.w));
+            node = __float_as_int(H.w);
         }
     }
 }
 
-// The overlap walk over the leaves after sorted leaf `first` only: the walk
+// The overlap walk (primitives passed to visit) over the leaves after sorted
+// leaf `first` only: the walk
 // is left-first, so a subtree whose last leaf is at or before `first` is
 // skipped whole. Each unordered pair of one tree's leaves is then found by
 // exactly one of its two leaves' queries.
@@ -189,20 +195,24 @@ __device__ __forceinline__ void bvh_overlap_after(const BvhView& bv, f3 lo, f3 h
     int node = 0;
     for (int steps = 2 * bv.n; node != kRopeEnd && steps > 0; steps--) {
         if (node >= 0) {
-            if (bv.range[node].y <= first) {
-                node = bv.ropeNode[node];
+            const float4* N = bv.nodes + 4 * (size_t)node;
+            const float4 A = N[2];  // rope, first leaf, last leaf
+            if (__float_as_int(A.z) <= first) {
+                node = __float_as_int(A.x);
                 continue;
             }
-            const float4 L = bv.lo[node];
-            if (overlap(lo, hi, L, bv.hi[node])) {
+            const float4 L = N[0];
+            if (overlap(lo, hi, L, N[1])) {
                 node = __float_as_int(L.w);
                 continue;
             }
-            node = bv.ropeNode[node];
+            node = __float_as_int(A.x);
         } else {
             const int leaf = ~node;
-            if (leaf > first && overlap(lo, hi, bv.leafLo[leaf], bv.leafHi[leaf])) visit(leaf);
-            node = bv.ropeLeaf[leaf];
+            const float4* F = bv.leaves + 2 * (size_t)leaf;
+            const float4 L = F[0], H = F[1];
+            if (leaf > first && overlap(lo, hi, L, H)) visit(__float_as_int(L.w));
+            node = __float_as_int(H.w);
         }
     }
 }
@@ -413,8 +423,7 @@ __global__ void __launch_bounds__(kBlock) k_self_candidates(const float4* __rest
             swept(xyz(P), lo, hi);
             lo = f3{lo.x - tol, lo.y - tol, lo.z - tol};
             hi = f3{hi.x + tol, hi.y + tol, hi.z + tol};
-            bvh_overlap(triBvh, lo, hi, [&](int leaf) {
-                const int t = __float_as_int(triBvh.leafLo[leaf].w);
+            bvh_overlap(triBvh, lo, hi, [&](int t) {
                 const int4 tv = cTris[t];
                 if (v == tv.x || v == tv.y || v == tv.z) return;
                 emit(make_int2(t, v));
@@ -431,8 +440,7 @@ __global__ void __launch_bounds__(kBlock) k_self_candidates(const float4* __rest
         lo = f3{lo.x - tol, lo.y - tol, lo.z - tol};
         hi = f3{hi.x + tol, hi.y + tol, hi.z + tol};
         // query j (sorted leaf j) pairs with the overlapping leaves after it
-        bvh_overlap_after(edgeBvh, lo, hi, j, [&](int leaf) {
-            const int e2 = __float_as_int(edgeBvh.leafLo[leaf].w);
+        bvh_overlap_after(edgeBvh, lo, hi, j, [&](int e2) {
             const int2 fv = cEdges[e2];
             if (ev.x == fv.x || ev.y == fv.y || ev.x == fv.y || ev.y == fv.x) return;
             emit(e1 < e2 ? make_int2(e1, e2) : make_int2(e2, e1));
