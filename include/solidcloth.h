@@ -162,6 +162,14 @@ SCEXP void scSetBendStiffness(scModel model, float val);      /* SetBendStiffnes
 SCEXP void scSetPoisson(scModel model, float val);            /* SetPoisson :129 */
 SCEXP void scSetDihedral(scModel model, bool val);            /* SetDihedral :123 */
 SCEXP void scSetDragCoeff(scModel model, float val);          /* SetDragCoeff :146 */
+SCEXP float scGetShearStiffness(scModel model);               /* GetShearStiffness :112 */
+SCEXP float scGetStretchStiffness(scModel model);             /* GetStretchStiffness :110 */
+SCEXP float scGetBendStiffness(scModel model);                /* GetBendStiffness :114 */
+SCEXP float scGetPoisson(scModel model);                      /* GetPoisson :128 */
+SCEXP bool scGetDihedral(scModel model);                      /* GetDihedral :122 */
+SCEXP float scGetDragCoeff(scModel model);                    /* GetDragCoeff :145 */
+/* GetSelfShrink :141 and GetSelfThreshold :134 (set with scSetSelfParams) */
+SCEXP void scGetSelfParams(scModel model, float* shrink, float* threshold);
 SCEXP void scSetNumSteps(scClothPatch patch, int val);        /* ClothPatch::SetNumSteps, ClothPatch.h:50 */
 SCEXP int scGetNumSteps(scClothPatch patch);
 

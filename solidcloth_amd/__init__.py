@@ -95,6 +95,9 @@ def _sig(L):
         "scSetUnit": (None, [H, f]), "scGetUnit": (f, [H]), "scSetShearStiffness": (None, [H, f]),
         "scSetStretchStiffness": (None, [H, f]), "scSetBendStiffness": (None, [H, f]),
         "scSetPoisson": (None, [H, f]), "scSetDihedral": (None, [H, b]), "scSetDragCoeff": (None, [H, f]),
+        "scGetShearStiffness": (f, [H]), "scGetStretchStiffness": (f, [H]), "scGetBendStiffness": (f, [H]),
+        "scGetPoisson": (f, [H]), "scGetDihedral": (b, [H]), "scGetDragCoeff": (f, [H]),
+        "scGetSelfParams": (None, [H, C.POINTER(f), C.POINTER(f)]),
         "scSetNumSteps": (None, [H, i]), "scGetNumSteps": (i, [H]),
         "scSetFixedIterations": (None, [H, b]), "scSetInitIncline": (None, [H, f]),
         "scCreateGridSDF": (H, [vp, i, i, i, scAabb3]), "scFreeSDF": (None, [H]),
@@ -334,6 +337,18 @@ class ClothModel:
 
     def GetThickness(self):
         return _checked("scGetThickness", self.h)
+
+    GetShearStiffness = lambda self: _checked("scGetShearStiffness", self.h)
+    GetStretchStiffness = lambda self: _checked("scGetStretchStiffness", self.h)
+    GetBendStiffness = lambda self: _checked("scGetBendStiffness", self.h)
+    GetPoisson = lambda self: _checked("scGetPoisson", self.h)
+    GetDihedral = lambda self: _checked("scGetDihedral", self.h)
+    GetDragCoeff = lambda self: _checked("scGetDragCoeff", self.h)
+
+    def GetSelfParams(self):
+        shrink, threshold = C.c_float(), C.c_float()
+        _checked("scGetSelfParams", self.h, C.byref(shrink), C.byref(threshold))
+        return shrink.value, threshold.value
 
     def GetNumIterations(self):
         return _checked("scGetNumIterations", self.h)

@@ -521,6 +521,20 @@ void scSetBendStiffness(scModel model, float val) { guard([&] { M(model)->mBendS
 void scSetPoisson(scModel model, float val) { guard([&] { M(model)->mPoisson = val; }); }
 void scSetDihedral(scModel model, bool val) { guard([&] { M(model)->mDihedral = val; }); }
 void scSetDragCoeff(scModel model, float val) { guard([&] { M(model)->mDragCoeff = val; }); }
+float scGetShearStiffness(scModel model) { return guard_r<float>(0.f, [&] { return M(model)->mShearStiff; }); }
+float scGetStretchStiffness(scModel model) { return guard_r<float>(0.f, [&] { return M(model)->mStretchStiff; }); }
+float scGetBendStiffness(scModel model) { return guard_r<float>(0.f, [&] { return M(model)->mBendStiff; }); }
+float scGetPoisson(scModel model) { return guard_r<float>(0.f, [&] { return M(model)->mPoisson; }); }
+bool scGetDihedral(scModel model) { return guard_r<bool>(false, [&] { return M(model)->mDihedral; }); }
+float scGetDragCoeff(scModel model) { return guard_r<float>(0.f, [&] { return M(model)->mDragCoeff; }); }
+void scGetSelfParams(scModel model, float* shrink, float* threshold)
+{
+    guard([&] {
+        ClothModel* m = M(model);
+        if (shrink) *shrink = m->mSelfShrink;
+        if (threshold) *threshold = m->mSelfCollThreshold;
+    });
+}
 void scSetNumSteps(scClothPatch patch, int val)
 {
     guard([&] {

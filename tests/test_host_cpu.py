@@ -408,3 +408,21 @@ def test_tile_schedule_equals_the_colour_major_sweep(sc, n):
     seq, tiled, order = _emulated_sweeps(tris, nv, ts, seed=n)
     assert list(m.constraint_order("tri")) == order
     assert np.array_equal(seq, tiled)
+
+
+def test_model_getters_read_back_the_setters(sc):
+    """The getters of Physics/ClothModel.h:110-146 that the library adds to
+    the reference's ABI return what the setters stored (FP32)."""
+    world, patch = S.build_patch(S.DRAPE32, sc)
+    m = patch.GetModel()
+    m.SetShearStiffness(123.5)
+    m.SetStretchStiffness(0.75)
+    m.SetBendStiffness(0.25)
+    m.SetPoisson(0.5)
+    m.SetDihedral(False)
+    m.SetDragCoeff(0.875)
+    m.SetSelfParams(0.625, 0.0078125)
+    assert (m.GetShearStiffness(), m.GetStretchStiffness(), m.GetBendStiffness(), m.GetPoisson()) == \
+        (123.5, 0.75, 0.25, 0.5)
+    assert m.GetDihedral() is False and m.GetDragCoeff() == 0.875
+    assert m.GetSelfParams() == (0.625, 0.0078125)
