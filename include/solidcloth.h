@@ -197,6 +197,15 @@ SCEXP void scFreeSDF(scSDF sdf);
  * scGetSDFGrid copies the cell counts (3 ints), the box and at most
  * `capacity` node values (any pointer may be null) and returns the node
  * count. */
+/* SDF::CreateFromMesh (Geometry/SDF.cpp:58-76) after SDF::Create(nx, ny, nz,
+ * box), computed on the GPU over the mesh's triangle LBVH: node values are
+ * the distance to the nearest triangle (distance 0 skipped; FLT_MAX when no
+ * triangle is left), unsigned, because the reference's query returns the
+ * unit delta as its normal and so its sign test never flips a value. The
+ * gradients follow as for scCreateGridSDF. The mesh is uploaded to the GPU
+ * on first use and stays there. Not in the reference's ABI (SDF creation
+ * there is C++ only). */
+SCEXP scSDF scCreateSDFFromMesh(scMesh mesh, int nx, int ny, int nz, scAabb3 box);
 SCEXP scSDF scLoadSDFFile(const char* path);
 SCEXP int scWriteSDFFile(scSDF sdf, const char* path);
 SCEXP int scGetSDFGrid(scSDF sdf, int* cells, scAabb3* box, float* values, int capacity);

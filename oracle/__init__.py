@@ -104,6 +104,7 @@ def lib():
         L.oc_signed_volume.restype = _C.c_float
         L.oc_mesh_normals.argtypes = [_C.c_int, vp, _C.c_int, vp, vp]
         L.oc_set_mouse_spring.argtypes = [vp] + [_C.c_int] * 4 + [_C.c_float] * 6
+        L.oc_sdf_from_mesh.argtypes = [_C.c_int, vp, _C.c_int, vp, _C.c_int, _C.c_int, _C.c_int, vp, vp]
         L.oc_set_threads.argtypes = [_C.c_int]
         L.oc_get_threads.restype = _C.c_int
         L.ol_create.restype = vp
@@ -116,6 +117,18 @@ def lib():
         L.ol_edge_queries.restype = _C.c_long
         _lib = L
     return _lib
+
+
+def sdf_from_mesh(vertices, indices, cells, bmin, bmax):
+    """SDF::CreateFromMesh restated (cloth_oracle.cpp): node values, x fastest,
+    as a (nz + 1, ny + 1, nx + 1) float32 array."""
+    v = _np.ascontiguousarray(vertices, dtype=_np.float32).reshape(-1, 3)
+    idx = _np.ascontiguousarray(indices, dtype=_np.uint32).ravel()
+    nx, ny, nz = (int(c) for c in cells)
+    box = _np.array(list(bmin) + list(bmax), dtype=_np.float32)
+    out = _np.zeros((nz + 1, ny + 1, nx + 1), dtype=_np.float32)
+    lib().oc_sdf_from_mesh(len(v), _ptr(v), idx.size, _ptr(idx), nx, ny, nz, _ptr(box), _ptr(out))
+    return out
 
 
 def set_threads(n):

@@ -2553,4 +2553,40 @@ int oc_construct_edges(const unsigned* idx, int ni, int* out, int cap)
     return n;
 }
 
+// SDF::Create + SDF::CreateFromMesh (Geometry/SDF.cpp:9-39, 58-76) over an
+// (nx, ny, nz)-cell grid of box6 = (min, max): node (x, y, z) sits at
+// (x, y, z) * steps + min (SDF.h:64-70) with steps = (max - min) * (1 / n)
+// (SDF.cpp:37-38). Its value is ClosestPointOnMeshToPointAcc's distance, the
+// smallest FP32 distance to a triangle (ClosestTriKernel,
+// ClosestPointOnMesh.cpp:99-125; distance 0 skipped), FLT_MAX when there is
+// none. The normal that query returns is the unit delta, so SDF.cpp:72-74's
+// sign test never flips the value: the field is unsigned. The tree only
+// prunes, so the minimum is taken over all triangles here. Nodes x-fastest,
+// (nx + 1) (ny + 1) (nz + 1) values.
+void oc_sdf_from_mesh(int nv, const float* verts, int ni, const unsigned* idx, int nx, int ny, int nz,
+                      const float* box6, float* out)
+{
+    std::vector<V3<float>> v(nv);
+    for (int i = 0; i < nv; i++) v[i] = V3<float>{verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]};
+    const V3<float> mn{box6[0], box6[1], box6[2]}, mx{box6[3], box6[4], box6[5]};
+    const V3<float> ext = mx - mn;
+    const V3<float> steps{ext.x * (1.f / (float)nx), ext.y * (1.f / (float)ny), ext.z * (1.f / (float)nz)};
+    const long vx = nx + 1, vy = ny + 1, count = vx * vy * (nz + 1);
+    const int nt = ni / 3;
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < count; k++) {
+        const long x = k % vx, w = k / vx, y = w % vy, z = w / vy;
+        const V3<float> org{(float)x * steps.x + mn.x, (float)y * steps.y + mn.y, (float)z * steps.z + mn.z};
+        float best = FLT_MAX;
+        for (int t = 0; t < nt; t++) {
+            V3<float> pt, bar;
+            closest_pt_point_triangle(org, v[idx[3 * t]], v[idx[3 * t + 1]], v[idx[3 * t + 2]], pt, bar);
+            const float d = length(org - pt);
+            if (d == 0.f) continue;
+            if (d < best) best = d;
+        }
+        out[k] = best;
+    }
+}
+
 }  // extern "C"

@@ -290,4 +290,35 @@ void ref_live_self_sample(const float* verts, int nv, const unsigned* idx, int n
     seconds[1] = std::chrono::duration<double>(t2 - t1).count();
 }
 
+// SDF::CreateFromMesh (Geometry/SDF.cpp:58-76; SDF.cpp itself needs Win32
+// functions and does not build here): the reference's own tree
+// (ComputeMeshTree with triangles, depth 10, tolerance 0.1) and query
+// (ClosestPointOnMeshToPointAcc) at every node, with the node position of
+// SDF::GetPosAt (SDF.h:64-70) and the steps of SDF::CacheData
+// (SDF.cpp:33-38), and the loop's sign rule. Nodes x-fastest.
+void ref_sdf_from_mesh(const float* verts, int nv, const unsigned* idx, int ni, int nx, int ny, int nz,
+                       const float* box6, float* out)
+{
+    Geometry::Mesh m;
+    make_mesh(m, verts, nv, idx, ni);
+    Geometry::AabbTree* tree = Geometry::ComputeMeshTree(m, Geometry::ATF_TRIANGLES, 10, 0.1f);
+    const Vector3 mn(box6[0], box6[1], box6[2]), mx(box6[3], box6[4], box6[5]);
+    Vector3 steps = mx - mn;
+    const Vector3 invRes(1.f / nx, 1.f / ny, 1.f / nz);
+    steps.Scale(invRes);
+    const int vx = nx + 1, vy = ny + 1, count = vx * vy * (nz + 1);
+    for (int k = 0; k < count; k++) {
+        const int x = k % vx, w = k / vx, y = w % vy, z = w / vy;
+        Vector3 org((float)x, (float)y, (float)z);
+        org.Scale(steps);
+        org += mn;
+        Geometry::ClosestTriangleToPoint info = Geometry::ClosestPointOnMeshToPointAcc(org, m, tree);
+        float val = info.distance;
+        Vector3 delta = org - info.closestPtMesh;
+        if (delta.Dot(info.normal) < 0) val = -val;
+        out[k] = val;
+    }
+    delete tree;
+}
+
 }  // extern "C"

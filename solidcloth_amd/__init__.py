@@ -100,7 +100,8 @@ def _sig(L):
         "scGetSelfParams": (None, [H, C.POINTER(f), C.POINTER(f)]),
         "scSetNumSteps": (None, [H, i]), "scGetNumSteps": (i, [H]),
         "scSetFixedIterations": (None, [H, b]), "scSetInitIncline": (None, [H, f]),
-        "scCreateGridSDF": (H, [vp, i, i, i, scAabb3]), "scFreeSDF": (None, [H]),
+        "scCreateGridSDF": (H, [vp, i, i, i, scAabb3]),
+        "scCreateSDFFromMesh": (H, [H, i, i, i, scAabb3]), "scFreeSDF": (None, [H]),
         "scLoadSDFFile": (H, [C.c_char_p]), "scWriteSDFFile": (i, [H, C.c_char_p]),
         "scGetSDFGrid": (i, [H, vp, vp, vp, i]),
         "scLoadMeshFromObj": (H, [C.c_char_p, b]), "scGetMeshUVBuffer": (C.POINTER(scVector2), [H]),
@@ -191,6 +192,16 @@ class SDF:
         box = scAabb3(v3(bmin), v3(bmax))
         self.h = _checked("scCreateGridSDF", _ptr(v), nx, ny, nz, box)
         self.n, self.bmin, self.bmax = tuple(n), tuple(bmin), tuple(bmax)
+
+    @classmethod
+    def from_mesh(cls, mesh, n, bmin, bmax):
+        """SDF::CreateFromMesh on the GPU: the unsigned distance of every node
+        of an n = (nx, ny, nz)-cell grid over [bmin, bmax] to the mesh."""
+        self = cls.__new__(cls)
+        nx, ny, nz = n
+        self.h = _checked("scCreateSDFFromMesh", mesh.h, nx, ny, nz, scAabb3(v3(bmin), v3(bmax)))
+        self.n, self.bmin, self.bmax = tuple(n), tuple(bmin), tuple(bmax)
+        return self
 
     @classmethod
     def from_file(cls, path):

@@ -14,6 +14,7 @@
 #include <tuple>
 #include <stdexcept>
 
+#include "hiputil.h"
 #include "lbvh.h"
 #include "model.h"
 
@@ -566,6 +567,38 @@ scSDF scCreateGridSDF(const float* values, int nx, int ny, int nz, scAabb3 box)
 }
 
 void scFreeSDF(scSDF sdf) { guard([&] { delete (SDF*)sdf.ptr; }); }
+
+scSDF scCreateSDFFromMesh(scMesh mesh, int nx, int ny, int nz, scAabb3 box)
+{
+    scSDF s;
+    s.ptr = guard_r<void*>(nullptr, [&] {
+        if (!mesh.ptr || nx < 1 || ny < 1 || nz < 1) throw std::runtime_error("bad SDF-from-mesh arguments");
+        Mesh* me = ME(mesh);
+        if (me->indices.empty()) throw std::runtime_error("the mesh has no triangles");
+        std::unique_ptr<SDF> g(new SDF());
+        g->create(nx, ny, nz, box);
+        hipStream_t st;
+        HIP_OK(hipStreamCreate(&st));
+        float* d = nullptr;
+        try {
+            const MeshGeo& geo = me->device_geo(st);
+            dev_alloc(d, g->vals.size());
+            launch_sdf_from_mesh(st, geo, nx, ny, nz, f3{box.min.x, box.min.y, box.min.z}, g->steps, d);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipMemcpyAsync(g->vals.data(), d, g->vals.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+        } catch (...) {
+            dev_free(d);
+            hipStreamDestroy(st);
+            throw;
+        }
+        dev_free(d);
+        HIP_OK(hipStreamDestroy(st));
+        g->compute_gradient();
+        return (void*)g.release();
+    });
+    return s;
+}
 
 scMesh scCreateMeshFromArrays(const scVector3* vertices, int numVertices, const unsigned int* indices, int numIndices)
 {

@@ -295,6 +295,11 @@ void launch_predict_mesh(hipStream_t s, float4* pos, float4* prev, float4* vel, 
 void launch_detect_features(hipStream_t s, const float4* pos, const float4* prev, const int2* cEdges,
                             const int4* cTris, const CollSet& cs, FeatureContactBuf ec, FeatureContactBuf tc,
                             float thickness, float tolerance, int collFlags, const SolverCtl* ctl);
+// SDF node values of an (nx, ny, nz)-cell grid from the nearest triangle of a
+// mesh obstacle (SDF::CreateFromMesh); (nx + 1) (ny + 1) (nz + 1) values,
+// x fastest, at most 2^31 - 1.
+void launch_sdf_from_mesh(hipStream_t s, const MeshGeo& g, int nx, int ny, int nz, f3 mn, f3 steps, float* out);
+
 // One launch per colour group, in group order; groupBase is the host copy of
 // the groups' offsets (each group's capacity sizes its grid).
 void launch_edge_contacts(hipStream_t s, float4* pos, const float4* prev, const int2* cEdges, FeatureContactBuf ec,

@@ -312,7 +312,34 @@ __global__ void __launch_bounds__(kBlock) k_tri_contacts(float4* __restrict__ po
     wave_max_atomic(&ctl->ct, err);
 }
 
+// SDF::CreateFromMesh (Geometry/SDF.cpp:58-76), one thread per node: node
+// (x, y, z) at (x, y, z) * steps + min (SDF.h:64-70); its value is the
+// distance ClosestPointOnMeshToPointAcc returns, the smallest FP32 distance
+// to a triangle with distance 0 skipped (FLT_MAX if none), here from the
+// obstacle's triangle LBVH. That query's normal is the unit delta, so the
+// reference's sign test never flips a value: the field is unsigned. Nodes
+// x-fastest.
+__global__ void __launch_bounds__(kBlock) k_sdf_from_mesh(MeshGeo g, int vx, int vy, int count, f3 mn, f3 steps,
+                                                          float* __restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const int x = k % vx, w = k / vx, y = w % vy, z = w / vy;
+    const f3 org{(float)x * steps.x + mn.x, (float)y * steps.y + mn.y, (float)z * steps.z + mn.z};
+    constexpr float kFltMax = 3.402823466e+38f;
+    NearestResult r;
+    out[k] = nearest_triangle(g, org, kFltMax, r) ? r.dist : kFltMax;
+}
+
 static inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
+
+void launch_sdf_from_mesh(hipStream_t s, const MeshGeo& g, int nx, int ny, int nz, f3 mn, f3 steps, float* out)
+{
+    const int count = (nx + 1) * (ny + 1) * (nz + 1);
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_sdf_from_mesh, dim3(blocks(count)), dim3(kBlock), 0, s, g, nx + 1, ny + 1, count, mn, steps,
+                       out);
+}
 
 void launch_predict_mesh(hipStream_t s, float4* pos, float4* prev, float4* vel, int n, float h, f3 gdv, float drag,
                          SolverCtl* ctl, FeatureContactBuf ec, FeatureContactBuf tc)

@@ -265,6 +265,44 @@ def live_self(L):
     print("wrote ref_live_self.npz")
 
 
+def sdf_from_mesh(L):
+    """SDF::CreateFromMesh (Geometry/SDF.cpp:58-76) through the reference's own
+    tree and closest-point query at every node (ref_sdf_from_mesh): an
+    irregular closed mesh on a grid with unequal steps, an icosphere, and a
+    flat grid mesh whose vertices and edges hold grid nodes (distance 0 to the
+    triangles through them, which the query skips)."""
+    import solidcloth_amd.scenes as S
+
+    cases = []
+    v, t = irregular_mesh(7)
+    lo, hi = v.min(0) - 0.3, v.max(0) + 0.25
+    cases.append((v, t, (12, 10, 9), lo, hi))
+    v, t = S.icosphere(2, 0.4, (0.05, -0.02, 0.0))
+    cases.append((np.asarray(v, np.float32), np.asarray(t, np.uint32), (16, 16, 16), (-0.6,) * 3, (0.6,) * 3))
+    # flat 5 x 5 vertex grid at integer x, z in the plane y = 0, both diagonals
+    fv = np.array([(x, 0.0, z) for z in range(5) for x in range(5)], np.float32)
+    ft = []
+    for z in range(4):
+        for x in range(4):
+            a, b, c, d = z * 5 + x, z * 5 + x + 1, (z + 1) * 5 + x, (z + 1) * 5 + x + 1
+            ft += [a, c, d, a, d, b] if (x + z) % 2 else [a, c, b, b, c, d]
+    cases.append((fv, np.array(ft, np.uint32), (8, 4, 8), (0.0, -2.0, 0.0), (4.0, 2.0, 4.0)))
+    out = {}
+    for k, (v, t, cells, lo, hi) in enumerate(cases):
+        v = np.ascontiguousarray(v, np.float32)
+        t = np.ascontiguousarray(t, np.uint32)
+        box = np.array(list(lo) + list(hi), np.float32)
+        nx, ny, nz = cells
+        vals = np.zeros((nz + 1, ny + 1, nx + 1), np.float32)
+        L.ref_sdf_from_mesh(p(v), C.c_int(len(v)), p(t), C.c_int(t.size), C.c_int(nx), C.c_int(ny), C.c_int(nz),
+                            p(box), p(vals))
+        out.update({f"c{k}_vertices": v, f"c{k}_indices": t, f"c{k}_cells": np.int32(cells), f"c{k}_box": box,
+                    f"c{k}_values": vals})
+        print(f"sdf from mesh case {k}: {len(v)} vertices, {vals.size} nodes, min {vals.min():.4f}")
+    np.savez_compressed(os.path.join(HERE, "ref_sdf_from_mesh.npz"), cases=np.int32(len(cases)), **out)
+    print("wrote ref_sdf_from_mesh.npz")
+
+
 if __name__ == "__main__":
     # no argument: every fixture; a name: only that fixture
     if sys.argv[1:] == ["segm_triangle"]:
@@ -273,8 +311,11 @@ if __name__ == "__main__":
         mesh_queries(ref_lib())
     elif sys.argv[1:] == ["live_self"]:
         live_self(ref_lib())
+    elif sys.argv[1:] == ["sdf_from_mesh"]:
+        sdf_from_mesh(ref_lib())
     else:
         main()
         segm_triangle(ref_lib())
         mesh_queries(ref_lib())
         live_self(ref_lib())
+        sdf_from_mesh(ref_lib())

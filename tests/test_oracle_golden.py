@@ -185,3 +185,28 @@ def test_live_self_threads_and_subsets(oracle_mod):
     se, _ = live.edge_queries(qe)
     want = np.concatenate([g("ee_ids")[g("ee_ids")[:, 0] == k] for k in qe])
     assert np.array_equal(se, want)
+
+
+SDFM = np.load(os.path.join(GOLDEN, "ref_sdf_from_mesh.npz"))
+
+
+@pytest.mark.parametrize("case", range(int(SDFM["cases"])))
+def test_sdf_from_mesh_matches_the_compiled_reference(oracle_mod, case):
+    """SDF::CreateFromMesh (Geometry/SDF.cpp:58-76) restated against the
+    reference's own tree and closest-point query at every node, bit for bit.
+    The reference's query returns the unit delta as its normal, so its sign
+    test never flips a value: every value is >= 0 even inside the closed
+    meshes (a restatement that signed the field with a triangle or vertex
+    normal fails cases 0 and 1). Case 0's unequal steps tell
+    (max - min) * (1 / n) from (max - min) / n; case 2's nodes on the flat mesh
+    skip the triangles at distance 0 (a restatement that kept them would give
+    0 there)."""
+    g = lambda k: SDFM[f"c{case}_{k}"]
+    box = g("box")
+    got = oracle_mod.sdf_from_mesh(g("vertices"), g("indices"), g("cells"), box[:3], box[3:])
+    want = g("values")
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert (want >= 0).all()
+    if case == 2:
+        # nodes on the mesh plane (y = 0): the nearest triangle not through the node
+        assert want[:, 2, :].min() > 0.3
