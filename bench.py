@@ -356,9 +356,10 @@ def learned_vs_plain(scene, start_frames=40, window=10, judged_frames=20, fixed_
     }
 
 
-def measured_traffic(kernel, phases):
+def measured_traffic(kernel, phases, sweep):
     """HBM bytes per launch of `kernel` ("k_fem" / "k_bend") from the counter
-    summary's JSON block, or None when absent or taken with other phases."""
+    summary's JSON block, or None when absent or taken with other phases or
+    another sweep mode."""
     try:
         with open(TRAFFIC_SUMMARY) as f:
             text = f.read()
@@ -366,7 +367,7 @@ def measured_traffic(kernel, phases):
         d = json.loads(block)
     except (OSError, IndexError, ValueError):
         return None
-    if d.get("launch_phases_per_pass") != phases:
+    if d.get("launch_phases_per_pass") != phases or d.get("sweep_mode") != sweep:
         return None
     k = d.get("kernels", {}).get(kernel)
     return k.get("hbm_bytes_per_launch") if k else None
@@ -607,9 +608,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels directly instead of replaying the captured substep graph")
-    ap.add_argument("--sweep", type=int, default=3, choices=[0, 1, 3],
+    ap.add_argument("--sweep", type=int, default=4, choices=[0, 1, 3, 4],
                     help="0: one launch per colour; 1: slab phases staged in LDS; 3: triangles on tiles with "
-                         "halos (one launch per FEM pass), bends on slabs")
+                         "halos (one launch per FEM pass), bends on slabs; 4: the schedules of 3 with each "
+                         "launch's data requested before its first colour")
     ap.add_argument("--copy-probe-mb", type=int, default=1024,
                     help="size of the streaming-copy bandwidth probe (0 disables)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
@@ -685,7 +687,10 @@ def main():
             "iterations": its,
             "substeps_per_frame": scene.substeps,
             "graph_replay": not args.no_graph,
-            "sweep": {0: "colour launches", 1: "slab phases (LDS)", 3: "FEM tiles with halos, bend slab phases (LDS)"}[args.sweep],
+            "sweep": {0: "colour launches", 1: "slab phases (LDS)", 3: "FEM tiles with halos, bend slab phases (LDS)",
+                      4: "FEM tiles with halos, bend slab phases (LDS), packed: all of a launch's data requested "
+                         "before its first colour"}[args.sweep],
+            "sweep_mode": args.sweep,
             "launch_phases_per_pass": {"fem": len(m.colour_offsets("tri")) - 1,
                                        "bend": len(m.colour_offsets("bend")) - 1},
         },
@@ -706,7 +711,7 @@ def main():
         },
     }
     if args.workload == "drape256" and dom in (0, 1):
-        t = measured_traffic(KERNEL_NAMES[dom], out["config"]["launch_phases_per_pass"])
+        t = measured_traffic(KERNEL_NAMES[dom], out["config"]["launch_phases_per_pass"], args.sweep)
         out["roofline"]["traffic"] = round(t) if t else None
     if args.copy_probe_mb > 0:
         import solidcloth_amd as sc

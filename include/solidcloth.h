@@ -290,8 +290,11 @@ SCEXP int scGetSolver(scModel model);
  * triangles on tiles (one global colouring; each workgroup sweeps its own
  * vertices and a halo as many rings wide as there are colours, in one
  * launch per pass, writing back its own vertices), bends and links as in
- * mode 1 (the default; a mesh whose tiles would not fit the workgroup's LDS
- * keeps mode 1 for its triangles). Changes the Gauss-Seidel order. */
+ * mode 1 (a mesh whose tiles would not fit the workgroup's LDS keeps mode 1
+ * for its triangles). Mode 4 (the default) = the schedules of mode 3, each
+ * laid out so that a launch requests all of its data before its first colour
+ * (where a schedule fits those kernels' limits, else as in mode 3); same
+ * order and results as mode 3. Changes the Gauss-Seidel order. */
 SCEXP void scSetSweep(scModel model, int mode, int slabSize);
 
 /* Contacts of the last substep against mesh obstacles, in the reference's

@@ -506,8 +506,10 @@ class ClothModel:
     def SetSweep(self, mode, slab_size=0):
         """mode 0: one launch per colour; mode 1: slab phases staged in LDS
         (strips of slab_size constraints); mode 2: strip interiors of about
-        slab_size constraints, then the seams between them. slab_size 0:
-        the library's choice."""
+        slab_size constraints, then the seams between them; mode 3: FEM
+        triangles on tiles with halos, bends on slab phases; mode 4 (the
+        default): mode 3's schedules with every launch's data requested
+        before its first colour. slab_size 0: the library's choice."""
         _checked("scSetSweep", self.h, int(mode), int(slab_size))
 
     def GetNumContacts(self):

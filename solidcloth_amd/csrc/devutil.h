@@ -10,14 +10,21 @@ namespace sc {
 __device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
 __device__ __forceinline__ float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
-// Wave-wide maximum of non-negative, NaN-free values, then one atomic per wave.
+// Wave-wide maximum of non-negative, NaN-free values (every lane of the wave
+// active), then one atomic per wave. Such floats order as their unsigned bit
+// patterns; the reduction moves lanes with DPP (shifts inside rows of 16,
+// then row broadcasts), so lane 63 ends with the maximum.
 __device__ __forceinline__ void wave_max_atomic(unsigned* dst, float v)
 {
-    for (int o = 32; o > 0; o >>= 1) {
-        float w = __shfl_xor(v, o, 64);
-        v = v < w ? w : v;
-    }
-    if ((threadIdx.x & 63) == 0 && v > 0.f) atomicMax(dst, __float_as_uint(v));
+    unsigned u = __float_as_uint(v);
+    u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x111, 0xf, 0xf, false));  // row_shr:1
+    u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x112, 0xf, 0xf, false));  // row_shr:2
+    u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x114, 0xf, 0xf, false));  // row_shr:4
+    u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x118, 0xf, 0xf, false));  // row_shr:8
+    u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    const unsigned m = (unsigned)__builtin_amdgcn_readlane((int)u, 63);
+    if ((threadIdx.x & 63) == 0 && m > 0u) atomicMax(dst, m);
 }
 
 }  // namespace sc

@@ -157,6 +157,38 @@ struct SlabContacts {
     int enabled;
 };
 
+// Packed sweep schedules (one phase of a slab or tile schedule), laid out so
+// that every load of a sweep launch is addressed by the workgroup and thread
+// alone: each slab has kv * kSlabBlock vertex slots and nc colours of `width`
+// constraint slots. Vertex slots hold a global id with flags (kSlabOwner,
+// kSlabContacts as in the slab lists, kPackPad for padding, which repeats a
+// vertex of the slab); constraint slots hold slab-local vertex ids, and
+// padding slots name the spare LDS entry kv * kSlabBlock.
+constexpr unsigned kPackPad = 0x20000000u;
+constexpr int kPackIdMask = 0x1fffffff;
+// FEM tiles: colour c of tile t has slots [(t nc + c) kSlabBlock, +kSlabBlock);
+// s0 = (local 1 | local 2 << 16, local 3, lu0 bits, lv0 bits), s1 = (du1,
+// du2, dv1, dv2), s2 = (invDet, lu2, lv2, area). Owned vertices come first
+// in a tile's list, at most one per thread.
+struct PackedFem {
+    const int* verts;
+    const uint4* s0;
+    const float4* s1;
+    const float4* s2;
+    int nc;  // colours in use (the rest of the kernel's colours are padding)
+};
+// Bend slabs: colour c of slab s has slots [(s nc + c) width, +width), each
+// the bend's four slab-local ids.
+struct PackedBend {
+    const int* verts;
+    const uint2* ids;
+    int nc, width;
+};
+// Largest colour count of a packed bend phase, and the vertex slots per
+// thread the packed kernels are built for.
+constexpr int kPackMaxBendColours = 16;
+constexpr int kPackMaxKv = 4;
+
 // Self contacts of one kind (vertex i4 against triangle i1 i2 i3, or edge i1 i2
 // against edge i3 i4), in detection order; `order` lists them colour group by
 // colour group (group g spans order[groupOff[g] .. groupOff[g+1])), members of
@@ -220,6 +252,13 @@ void launch_contacts_velocity(hipStream_t s, float4* pos, const float4* prev, fl
                               float thickness, float mu, float invH, SolverCtl* ctl);
 void launch_bend_slab(hipStream_t s, float4* pos, const ushort4* lidx, SlabSet ss, int numSlabs, int maxVerts,
                       float stiff, SolverCtl* ctl, const SlabContacts& kc);
+// Packed sweeps: numSlabs workgroups; nc (FEM: 8 or 12, colours padded to it)
+// and kv (1 .. kPackMaxKv) as packed. kc.enabled 2 (FEM) / 3 (bends) applies
+// vertex contacts as the slab kernels do.
+void launch_fem_packed(hipStream_t s, float4* pos, PackedFem pk, int numTiles, int nc, int kv, float h2, float epsUV,
+                       SolverCtl* ctl, const SlabContacts& kc);
+void launch_bend_packed(hipStream_t s, float4* pos, PackedBend pk, int numSlabs, int kv, float stiff, SolverCtl* ctl,
+                        const SlabContacts& kc);
 void launch_contacts(hipStream_t s, float4* pos, const float4* prev, float4* vel, ContactBuf cb, float thickness,
                      float mu, float invH, SolverCtl* ctl);
 void launch_iter_end(hipStream_t s, SolverCtl* ctl, int maxIter, int fixedIter, int bendFreq, int dihedral,

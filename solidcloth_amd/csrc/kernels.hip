@@ -243,6 +243,74 @@ __device__ __forceinline__ float bend_project(float4& P1, float4& P2, float4& P3
     return e > 0.f ? e : 0.f;
 }
 
+// The value of the partner lane (lanes 2b and 2b + 1 trade; DPP quad_perm
+// [1, 0, 3, 2]). Every lane of the wave must be active.
+__device__ __forceinline__ float pair_swap(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xf, 0xf, false));
+}
+__device__ __forceinline__ f3 pair_swap(f3 v) { return f3{pair_swap(v.x), pair_swap(v.y), pair_swap(v.z)}; }
+__device__ __forceinline__ f3 sel(bool c, f3 a, f3 b) { return f3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z}; }
+
+// bend_project computed by a pair of adjacent lanes: lane A (side b = false)
+// takes the triangle (x1, x2, x3), lane B the triangle (x1, x2, x4). Each lane
+// forms its side's normal, height and gradient terms with the same
+// instructions on its own operands, the pair trades them, and both evaluate
+// the shared terms, so every value is computed by the same operations in the
+// same order as in bend_project and the results agree to the bit. Returns
+// the bend's error (as bend_project) and the lane's two updated vertices in
+// Pa, Pb — (x1, x3) on lane A, (x2, x4) on lane B — with apply false when
+// bend_project would leave the bend alone. Both lanes of every pair in the
+// wave must be active.
+__device__ __forceinline__ float bend_project_pair(const float4 P1, const float4 P2, const float4 P3,
+                                                   const float4 P4, float sStiff, bool b, float4& Pa, float4& Pb,
+                                                   bool& apply)
+{
+    const float m1 = P1.w, m2 = P2.w, m3 = P3.w, m4 = P4.w;
+    const f3 x1 = xyz(P1), x2 = xyz(P2), x3 = xyz(P3), x4 = xyz(P4);
+    const f3 x12 = sub(x2, x1);
+    const float l = len(x12);
+    // this lane's side: n1 = x12 x x13 (A) or n2 = x14 x x12 (B)
+    const f3 xs = sub(b ? x4 : x3, x1);
+    const f3 ns = cross(sel(b, xs, x12), sel(b, x12, xs));
+    const float ls = len(ns);
+    const float ils = 1.f / ls;
+    const f3 nsn = mul(ils, ns);
+    const float hs = ls / l;
+    const f3 nso = pair_swap(nsn);
+    const float lso = pair_swap(ls), hso = pair_swap(hs);
+    // (IEEE products and sums are commutative, so dot(n1n, n2n) and h1 + h2
+    // come out the same on both lanes from their own and the partner's terms)
+    float cosine = 1.f, theta = 0.f;
+    if (!(ls == 0.f || lso == 0.f)) {
+        cosine = dot(nsn, nso);
+        cosine = smin(1.f, smax(-1.f, cosine));
+        theta = acos_as(cosine);
+    }
+    apply = m1 + m2 + m3 + m4 != 0.f && !(fabsf(cosine) >= 1 - 0.001f);
+    const float he = 0.5f * (hs + hso);
+    // the partner's n'p: n2p = n2n - cos n1n on lane A, n1p = n1n - cos n2n on lane B
+    const f3 npo = sub(nso, mul(cosine, nsn));
+    const float sine = sqrtf(1 - cosine * cosine);
+    const float fs = ils / sine / he;  // f1 on lane A, f2 on lane B
+    // q3 = f1 (x12 x n2p) and -f1 (x13 x n2p) on lane A; q4 = -f2 (x12 x n1p)
+    // and f2 (x14 x n1p) on lane B
+    const f3 qs = mul(b ? -fs : fs, cross(x12, npo));
+    const f3 ts = mul(b ? fs : -fs, cross(xs, npo));
+    const f3 qso = pair_swap(qs), tso = pair_swap(ts);
+    const f3 q3 = sel(b, qso, qs), q4 = sel(b, qs, qso);
+    const f3 q2 = add(ts, tso);
+    const f3 q1 = sub(sub(neg(q2), q3), q4);
+    const float diag = m1 * dot(q1, q1) + m2 * dot(q2, q2) + m3 * dot(q3, q3) + m4 * dot(q4, q4);
+    if (diag == 0.f) apply = false;
+    const float e = fabsf(sStiff * theta);
+    const float lambda = -(sStiff * theta / he / diag);
+    const float ma = b ? m2 : m1, mb = b ? m4 : m3;
+    Pa = f4(add(b ? x2 : x1, mul(lambda * ma, b ? q2 : q1)), ma);
+    Pb = f4(add(b ? x4 : x3, mul(lambda * mb, b ? q4 : q3)), mb);
+    return apply && e > 0.f ? e : 0.f;
+}
+
 // Physics/ClothPBD.cpp:9-50 — one link projection (omega = 1). Returns
 // |s err| when the link was projected (skipped links and NaN errors count 0,
 // as in the reference's `>` comparison).
@@ -320,15 +388,13 @@ __global__ void __launch_bounds__(kBlock) k_bend(float4* __restrict__ pos, const
     wave_max_atomic(&ctl->cb, errOut);
 }
 
-// The contacts of vertex v, in slot order, applied to its position p and
-// velocity ve (ClothCollisionHandler.cpp:141-199). Returns the largest depth.
-__device__ __forceinline__ float vertex_contacts(int v, int n, f3& p, const f3 pp, f3& ve, ContactBuf cb,
-                                                 float thickness, float mu, float invH)
+// Contact slot s of vertex v (point CP, normal CN as stored) applied to its
+// position p and velocity ve (ClothCollisionHandler.cpp:141-199); err keeps
+// the largest depth.
+__device__ __forceinline__ void vertex_contact_slot(int s, int v, float4 CP, float4 CN, f3& p, const f3 pp, f3& ve,
+                                                    ContactBuf cb, float thickness, float mu, float invH, float& err)
 {
-    float err = 0.f;
-    for (int s = 0; s < n; s++) {
-        float4 CP = cb.P[s * cb.nv + v];
-        float4 CN = cb.N[s * cb.nv + v];
+    {
         if (cb.geo) {
             const int f = cb.feat[s * cb.nv + v];
             if (f >= 0) {
@@ -354,7 +420,7 @@ __device__ __forceinline__ float vertex_contacts(int v, int n, f3& p, const f3 p
                 cb.P[s * cb.nv + v] = CP;
                 cb.N[s * cb.nv + v] = CN;
             }
-            continue;
+            return;
         }
         float depth = thickness - l;
         if (fabsf(depth) > err) err = fabsf(depth);
@@ -379,6 +445,16 @@ __device__ __forceinline__ float vertex_contacts(int v, int n, f3& p, const f3 p
         cb.P[s * cb.nv + v] = make_float4(CP.x, CP.y, CP.z, lam);
         cb.N[s * cb.nv + v] = make_float4(CN.x, CN.y, CN.z, lamF);
     }
+}
+
+// The contacts of vertex v, in slot order, applied to its position p and
+// velocity ve (ClothCollisionHandler.cpp:141-199). Returns the largest depth.
+__device__ __forceinline__ float vertex_contacts(int v, int n, f3& p, const f3 pp, f3& ve, ContactBuf cb,
+                                                 float thickness, float mu, float invH)
+{
+    float err = 0.f;
+    for (int s = 0; s < n; s++)
+        vertex_contact_slot(s, v, cb.P[s * cb.nv + v], cb.N[s * cb.nv + v], p, pp, ve, cb, thickness, mu, invH, err);
     return err;
 }
 
@@ -759,6 +835,238 @@ __global__ void __launch_bounds__(kSlabBlock) k_link_slab(float4* __restrict__ p
     }
     slab_end(ss, info, sv, pos, gk);
     wave_max_atomic(&ctl->cl, err);
+}
+
+// ---------------------------------------------------------------------------
+// Packed sweeps. A slab launch above waits on memory once per colour: the
+// colour's constraint data is requested one colour ahead, and a colour step
+// (one projection and an LDS barrier) is shorter than a load's latency. With
+// a packed schedule (PackedFem, PackedBend) every address of the launch
+// follows from the workgroup and thread, so a launch requests its vertex ids,
+// then the positions they name and the contact state of the vertices it
+// applies contacts to, then the constraint data of every colour at once, and
+// the colour steps run behind the data: two memory latencies per launch.
+// Padding slots compute on the spare LDS entry, so nothing is conditional.
+
+// Keeps the scheduler from moving loads across this point, so that loads are
+// issued in the order the waits below rely on (memory returns in order).
+__device__ __forceinline__ void issue_fence()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// A 16-byte load as four scalars (an aggregate copy would go through private
+// memory across issue_fence).
+__device__ __forceinline__ float4 load4(const float4* p)
+{
+    const float* f = reinterpret_cast<const float*>(p);
+    return make_float4(f[0], f[1], f[2], f[3]);
+}
+
+// The first two contact slots of a vertex, its previous position and
+// velocity, loaded ahead of the contact pass (slot 1 only when the buffer
+// has it; a vertex rarely holds more than two contacts, and later slots are
+// loaded when applied).
+struct ContactPre {
+    int cnt;
+    float4 C0P, C0N, C1P, C1N, PV, VE;
+};
+
+__device__ __forceinline__ ContactPre contact_pre(int g, const SlabContacts& kc)
+{
+    ContactPre c;
+    c.cnt = kc.cb.count[g];
+    c.C0P = load4(kc.cb.P + g);
+    c.C0N = load4(kc.cb.N + g);
+    const int g1 = kc.cb.slots > 1 ? kc.cb.nv + g : g;
+    c.C1P = load4(kc.cb.P + g1);
+    c.C1N = load4(kc.cb.N + g1);
+    c.PV = load4(kc.prev + g);
+    c.VE = load4(kc.vel + g);
+    return c;
+}
+
+// The vertex contacts of one staged vertex (position in LDS at sv[slot]) from
+// its preloaded state (ClothCollisionHandler.cpp:141-199, slots in order).
+__device__ __forceinline__ float packed_vertex_contacts(int g, const ContactPre& c, float4* sv, int slot,
+                                                        const SlabContacts& kc)
+{
+    const float4 P = sv[slot];
+    f3 p = xyz(P), ve = xyz(c.VE);
+    const f3 pp = xyz(c.PV);
+    float err = 0.f;
+    if (c.cnt > 0) vertex_contact_slot(0, g, c.C0P, c.C0N, p, pp, ve, kc.cb, kc.thickness, kc.mu, kc.invH, err);
+    if (c.cnt > 1) vertex_contact_slot(1, g, c.C1P, c.C1N, p, pp, ve, kc.cb, kc.thickness, kc.mu, kc.invH, err);
+    for (int s = 2; s < c.cnt; s++)
+        vertex_contact_slot(s, g, kc.cb.P[s * kc.cb.nv + g], kc.cb.N[s * kc.cb.nv + g], p, pp, ve, kc.cb,
+                            kc.thickness, kc.mu, kc.invH, err);
+    sv[slot] = f4(p, P.w);
+    kc.vel[g] = f4(ve, 0.f);
+    return err;
+}
+
+// One colour's FEM constraint slots of this thread.
+struct FemSlot {
+    uint4 a0;
+    float4 a1, a2;
+};
+
+__device__ __forceinline__ FemSlot fem_slot(const PackedFem& pk, size_t k)
+{
+    return FemSlot{pk.s0[k], load4(pk.s1 + k), load4(pk.s2 + k)};
+}
+
+// FEM triangles on packed tiles (NC colours, KV vertex slots per thread).
+// The constraint data streams kFemAhead colours ahead of the sweep: requested
+// all at once it would queue behind the position gathers and hold the
+// workgroup's first colour back. kContacts: the tile's own vertices get this
+// iteration's vertex contacts after the last colour (as k_fem_slab<true> with
+// kc.enabled == 2); their contact state is requested with the last colours.
+constexpr int kFemAhead = 3;
+
+template <int NC, int KV, bool kContacts>
+__global__ void __launch_bounds__(kSlabBlock) k_fem_packed(float4* pos, PackedFem pk, float h2,
+                                                           float epsUV, SolverCtl* __restrict__ ctl, SlabContacts kc)
+{
+    static_assert(NC > kFemAhead, "colours beyond the stream distance");
+    constexpr int kSpare = KV * kSlabBlock;
+    __shared__ float4 sv[kSpare + 1];
+    const int tid = threadIdx.x;
+    const int done = ctl->done;
+    const int* vg = pk.verts + (size_t)blockIdx.x * kSpare;
+    int raw[KV];
+#pragma unroll
+    for (int u = 0; u < KV; u++) raw[u] = vg[tid + u * kSlabBlock];
+    issue_fence();
+    if (done) return;
+    float4 P[KV];
+#pragma unroll
+    for (int u = 0; u < KV; u++) P[u] = load4(pos + (raw[u] & kPackIdMask));
+    issue_fence();
+    FemSlot f[NC];
+    const size_t sb = (size_t)blockIdx.x * NC * kSlabBlock + tid;
+#pragma unroll
+    for (int c = 0; c < kFemAhead; c++) f[c] = fem_slot(pk, sb + c * kSlabBlock);
+    issue_fence();
+#pragma unroll
+    for (int u = 0; u < KV; u++) sv[tid + u * kSlabBlock] = P[u];
+    lds_barrier();
+    // slot 0 holds the thread's own vertex, if any
+    const int g0 = raw[0] & kPackIdMask;
+    ContactPre cp{};
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        if (c + kFemAhead < NC) f[c + kFemAhead] = fem_slot(pk, sb + (c + kFemAhead) * kSlabBlock);
+        if (kContacts && c + kFemAhead == NC) cp = contact_pre(g0, kc);
+        issue_fence();
+        // (a branch per colour keeps each colour's arithmetic, and the wait
+        // for its data, in its own block)
+        if (c < pk.nc) {
+            const int i1 = f[c].a0.x & 0xffff, i2 = f[c].a0.x >> 16, i3 = f[c].a0.y;
+            float4 P1 = sv[i1], P2 = sv[i2], P3 = sv[i3];
+            fem_project(P1, P2, P3, f[c].a1, f[c].a2,
+                        make_float2(__uint_as_float(f[c].a0.z), __uint_as_float(f[c].a0.w)), h2, epsUV);
+            sv[i1] = P1;
+            sv[i2] = P2;
+            sv[i3] = P3;
+        }
+        lds_barrier();
+    }
+    const bool own = raw[0] < 0;  // kSlabOwner
+    if (kContacts) {
+        float err = 0.f;
+        if (own && cp.cnt > 0) err = packed_vertex_contacts(g0, cp, sv, tid, kc);
+        wave_max_atomic(&ctl->cv, err);
+    }
+    if (own) pos[g0] = sv[tid];
+}
+
+// Bends on packed slabs (KV vertex slots per thread, at most
+// kPackMaxBendColours colours). kContacts: the vertices whose last bend slab
+// this is get the iteration's vertex contacts after the last colour (as
+// k_bend_slab with kc.enabled == 3).
+// kPair: each bend on a pair of adjacent lanes (bend_project_pair; the
+// phase's width is at most half the workgroup, rounded so that whole waves
+// are active or idle).
+template <int KV, bool kContacts, bool kPair>
+__global__ void __launch_bounds__(kSlabBlock) k_bend_packed(float4* pos, PackedBend pk, float sStiff,
+                                                            SolverCtl* __restrict__ ctl, SlabContacts kc)
+{
+    constexpr int kSpare = KV * kSlabBlock;
+    __shared__ float4 sv[kSpare + 1];
+    const int tid = threadIdx.x;
+    const int done = ctl->done;
+    const int nc = pk.nc;
+    const int slot = kPair ? tid >> 1 : tid;
+    const bool lane = slot < pk.width;
+    const int* vg = pk.verts + (size_t)blockIdx.x * kSpare;
+    int raw[KV];
+#pragma unroll
+    for (int u = 0; u < KV; u++) raw[u] = vg[tid + u * kSlabBlock];
+    uint2 q[kPackMaxBendColours];
+    const size_t qb = (size_t)blockIdx.x * nc * pk.width + slot;
+#pragma unroll
+    for (int c = 0; c < kPackMaxBendColours; c++)
+        if (lane && c < nc) q[c] = pk.ids[qb + (size_t)c * pk.width];
+    issue_fence();
+    if (done) return;
+    float4 P[KV];
+#pragma unroll
+    for (int u = 0; u < KV; u++) P[u] = load4(pos + (raw[u] & kPackIdMask));
+    issue_fence();
+    ContactPre cp[KV];
+    if (kContacts) {
+#pragma unroll
+        for (int u = 0; u < KV; u++) cp[u] = contact_pre(raw[u] & kPackIdMask, kc);
+    }
+    issue_fence();
+#pragma unroll
+    for (int u = 0; u < KV; u++) sv[tid + u * kSlabBlock] = P[u];
+    lds_barrier();
+    float err = 0.f;
+#pragma unroll
+    for (int c = 0; c < kPackMaxBendColours; c++) {
+        if (c >= nc) continue;  // (not break: the loop must unroll, q lives in registers)
+        if (lane) {
+            const int i1 = q[c].x & 0xffff, i2 = q[c].x >> 16, i3 = q[c].y & 0xffff, i4 = q[c].y >> 16;
+            float4 P1 = sv[i1], P2 = sv[i2], P3 = sv[i3], P4 = sv[i4];
+            if (kPair) {
+                const bool b = tid & 1;
+                float4 Pa, Pb;
+                bool apply;
+                const float e1 = bend_project_pair(P1, P2, P3, P4, sStiff, b, Pa, Pb, apply);
+                err = err < e1 ? e1 : err;
+                if (apply) {
+                    sv[b ? i2 : i1] = Pa;
+                    sv[b ? i4 : i3] = Pb;
+                }
+            } else {
+                const float e1 = bend_project(P1, P2, P3, P4, sStiff);
+                err = err < e1 ? e1 : err;
+                sv[i1] = P1;
+                sv[i2] = P2;
+                sv[i3] = P3;
+                sv[i4] = P4;
+            }
+        }
+        lds_barrier();
+    }
+    if (kContacts) {
+        float cerr = 0.f;
+#pragma unroll
+        for (int u = 0; u < KV; u++) {
+            if (((unsigned)raw[u] & kSlabContacts) && cp[u].cnt > 0) {
+                const float e = packed_vertex_contacts(raw[u] & kPackIdMask, cp[u], sv, tid + u * kSlabBlock, kc);
+                cerr = cerr < e ? e : cerr;
+            }
+        }
+        wave_max_atomic(&ctl->cv, cerr);
+    }
+#pragma unroll
+    for (int u = 0; u < KV; u++)
+        if (!((unsigned)raw[u] & kPackPad)) pos[raw[u] & kPackIdMask] = sv[tid + u * kSlabBlock];
+    wave_max_atomic(&ctl->cb, err);
 }
 
 // Physics/ClothCollisionHandler.cpp:134-202 — vertex contacts with friction.
@@ -1150,6 +1458,58 @@ void launch_bend_slab(hipStream_t s, float4* pos, const ushort4* lidx, SlabSet s
     if (numSlabs <= 0) return;
     hipLaunchKernelGGL(k_bend_slab, dim3(numSlabs), dim3(kSlabBlock), (maxVerts + 1) * sizeof(float4), s, pos, lidx, ss,
                        stiff, ctl, kc);
+}
+
+template <int NC, int KV>
+static void fem_packed_kv(hipStream_t s, float4* pos, PackedFem pk, int numTiles, float h2, float epsUV,
+                          SolverCtl* ctl, const SlabContacts& kc)
+{
+    if (kc.enabled == 2)
+        hipLaunchKernelGGL((k_fem_packed<NC, KV, true>), dim3(numTiles), dim3(kSlabBlock), 0, s, pos, pk, h2, epsUV,
+                           ctl, kc);
+    else
+        hipLaunchKernelGGL((k_fem_packed<NC, KV, false>), dim3(numTiles), dim3(kSlabBlock), 0, s, pos, pk, h2, epsUV,
+                           ctl, kc);
+}
+
+void launch_fem_packed(hipStream_t s, float4* pos, PackedFem pk, int numTiles, int nc, int kv, float h2, float epsUV,
+                       SolverCtl* ctl, const SlabContacts& kc)
+{
+    if (numTiles <= 0) return;
+    if (nc == 8 && kv == 2) fem_packed_kv<8, 2>(s, pos, pk, numTiles, h2, epsUV, ctl, kc);
+    else if (nc == 8 && kv == 4) fem_packed_kv<8, 4>(s, pos, pk, numTiles, h2, epsUV, ctl, kc);
+    else if (nc == 12 && kv == 2) fem_packed_kv<12, 2>(s, pos, pk, numTiles, h2, epsUV, ctl, kc);
+    else if (nc == 12 && kv == 4) fem_packed_kv<12, 4>(s, pos, pk, numTiles, h2, epsUV, ctl, kc);
+    else throw std::runtime_error("packed FEM tiles: unsupported colour count or vertex slots");
+}
+
+template <int KV, bool kPair>
+static void bend_packed_kv(hipStream_t s, float4* pos, PackedBend pk, int numSlabs, float stiff, SolverCtl* ctl,
+                           const SlabContacts& kc)
+{
+    if (kc.enabled == 3)
+        hipLaunchKernelGGL((k_bend_packed<KV, true, kPair>), dim3(numSlabs), dim3(kSlabBlock), 0, s, pos, pk, stiff,
+                           ctl, kc);
+    else
+        hipLaunchKernelGGL((k_bend_packed<KV, false, kPair>), dim3(numSlabs), dim3(kSlabBlock), 0, s, pos, pk, stiff,
+                           ctl, kc);
+}
+
+void launch_bend_packed(hipStream_t s, float4* pos, PackedBend pk, int numSlabs, int kv, float stiff, SolverCtl* ctl,
+                        const SlabContacts& kc)
+{
+    if (numSlabs <= 0) return;
+    if (pk.nc < 1 || pk.nc > kPackMaxBendColours || pk.width < 1 || pk.width > kSlabBlock)
+        throw std::runtime_error("packed bend slabs: unsupported colour count or width");
+    // a pair of lanes per bend when a colour fits half the workgroup
+    const bool pair = pk.width <= kSlabBlock / 2 && pk.width % 32 == 0;
+    if (kv == 2) pair ? bend_packed_kv<2, true>(s, pos, pk, numSlabs, stiff, ctl, kc)
+                      : bend_packed_kv<2, false>(s, pos, pk, numSlabs, stiff, ctl, kc);
+    else if (kv == 3) pair ? bend_packed_kv<3, true>(s, pos, pk, numSlabs, stiff, ctl, kc)
+                           : bend_packed_kv<3, false>(s, pos, pk, numSlabs, stiff, ctl, kc);
+    else if (kv == 4) pair ? bend_packed_kv<4, true>(s, pos, pk, numSlabs, stiff, ctl, kc)
+                           : bend_packed_kv<4, false>(s, pos, pk, numSlabs, stiff, ctl, kc);
+    else throw std::runtime_error("packed bend slabs: unsupported vertex slots");
 }
 
 void launch_bend(hipStream_t s, float4* pos, const int4* bidx, int start, int count, float stiff, SolverCtl* ctl)

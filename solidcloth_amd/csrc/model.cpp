@@ -339,6 +339,10 @@ void GpuSolver::release()
     for (DeviceSlabs* d : {&triSlabs, &bendSlabs, &linkSlabs}) {
         dev_free(d->lidx); dev_free(d->info); dev_free(d->colOff); dev_free(d->first); dev_free(d->verts);
     }
+    for (DevicePacked* d : {&triPacked, &bendPacked}) {
+        dev_free(d->verts); dev_free(d->s0); dev_free(d->s1); dev_free(d->s2); dev_free(d->ids);
+        d->phases.clear();
+    }
     slotCap = 0;
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
@@ -424,7 +428,8 @@ void ClothModel::sync_to_host()
 // With `owners`, bit 31 of a slab vertex id marks the vertex's first slab in
 // phase order; returns the number of vertices below `numVertices` that no
 // slab contains.
-static int upload_slabs(const Schedule& s, DeviceSlabs& d, bool owners, int numVertices, bool lastFlags = false)
+static int upload_slabs(const Schedule& s, DeviceSlabs& d, bool owners, int numVertices, bool lastFlags = false,
+                        std::vector<int>* flaggedOut = nullptr)
 {
     dev_free(d.lidx); dev_free(d.info); dev_free(d.colOff); dev_free(d.first); dev_free(d.verts);
     if (!s.slabbed || s.order.empty()) return numVertices;
@@ -475,7 +480,137 @@ static int upload_slabs(const Schedule& s, DeviceSlabs& d, bool owners, int numV
     HIP_OK(hipMemcpy(d.colOff, s.colOffsets.data(), s.colOffsets.size() * sizeof(int), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(d.first, first.data(), ns * sizeof(int), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(d.verts, verts.data(), verts.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (flaggedOut) *flaggedOut = std::move(verts);
     return orphans;
+}
+
+// Packs a tile schedule (FEM triangles) or a slab schedule (bends) for the
+// preloading sweep kernels: per slab kv * kSlabBlock vertex slots, per colour
+// of a phase the same number of constraint slots (see PackedFem and
+// PackedBend). `flagged` is the slabs' vertex list with the flags
+// upload_slabs sets. Leaves `d` without phases when the schedule exceeds the
+// kernels' limits: a slab of more than kPackMaxKv * kSlabBlock vertices, a
+// colour of a slab wider than the workgroup, too many colours, a tile owning
+// more vertices than the workgroup has threads, or vertex ids of 2^29 or more.
+static void pack_schedule(bool enabled, const Schedule& s, const std::vector<int>& flagged, int numVertices,
+                          const std::vector<Triangle>* tris, DevicePacked& d)
+{
+    dev_free(d.verts); dev_free(d.s0); dev_free(d.s1); dev_free(d.s2); dev_free(d.ids);
+    d.phases.clear();
+    const bool fem = tris != nullptr;
+    if (!enabled || !s.slabbed || s.order.empty() || fem != s.tiled || numVertices >= (int)kPackIdMask) return;
+    const int arity = s.arity;
+    if (fem ? arity != 3 : arity != 4) return;
+    std::vector<DevicePacked::Phase> phases;
+    for (int p = 0; p < s.numPhases(); p++) {
+        DevicePacked::Phase ph;
+        int maxVerts = 0, maxCol = 0, maxWidth = 0;
+        for (int sl = s.phaseSlabBegin[p]; sl < s.phaseSlabBegin[p + 1]; sl++) {
+            maxVerts = std::max(maxVerts, s.slabVertBegin[sl + 1] - s.slabVertBegin[sl]);
+            maxCol = std::max(maxCol, s.slabColBegin[sl + 1] - s.slabColBegin[sl]);
+            for (int c = s.slabColBegin[sl]; c < s.slabColBegin[sl + 1]; c++)
+                maxWidth = std::max(maxWidth, s.colOffsets[c + 1] - s.colOffsets[c]);
+            if (fem) {
+                int owned = 0;
+                for (int i = s.slabVertBegin[sl]; i < s.slabVertBegin[sl + 1]; i++) owned += s.slabOwned[i] ? 1 : 0;
+                if (owned > kSlabBlock) return;
+            }
+        }
+        if (maxVerts > kPackMaxKv * kSlabBlock || maxWidth > kSlabBlock || maxCol < 1) return;
+        ph.slabs = s.phaseSlabBegin[p + 1] - s.phaseSlabBegin[p];
+        ph.kv = std::max(2, (maxVerts + kSlabBlock - 1) / kSlabBlock);
+        ph.nc = maxCol;
+        if (fem) {
+            // the kernel's colour count (instantiations for 8 and 12)
+            if (maxCol > 12) return;
+            ph.ncKernel = maxCol <= 8 ? 8 : 12;
+            ph.kv = ph.kv <= 2 ? 2 : 4;
+            ph.width = kSlabBlock;
+        } else {
+            if (maxCol > kPackMaxBendColours) return;
+            ph.ncKernel = maxCol;
+            ph.width = std::max(64, (maxWidth + 63) / 64 * 64);
+        }
+        phases.push_back(ph);
+    }
+    size_t nVert = 0, nSlot = 0;
+    for (DevicePacked::Phase& ph : phases) {
+        ph.vertOff = nVert;
+        ph.slotOff = nSlot;
+        nVert += (size_t)ph.slabs * ph.kv * kSlabBlock;
+        nSlot += (size_t)ph.slabs * ph.ncKernel * ph.width;
+    }
+    std::vector<int> verts(nVert);
+    std::vector<uint4> s0;
+    std::vector<float4> s1, s2;
+    std::vector<uint2> ids;
+    if (fem) {
+        s0.assign(nSlot, make_uint4(0, 0, 0, 0));
+        s1.assign(nSlot, make_float4(0, 0, 0, 0));
+        s2.assign(nSlot, make_float4(0, 0, 0, 0));
+    } else {
+        ids.assign(nSlot, make_uint2(0, 0));
+    }
+    std::vector<int> newLocal;
+    for (int p = 0; p < (int)phases.size(); p++) {
+        const DevicePacked::Phase& ph = phases[p];
+        const int spare = ph.kv * kSlabBlock;
+        for (int j = 0; j < ph.slabs; j++) {
+            const int sl = s.phaseSlabBegin[p] + j;
+            const int vb = s.slabVertBegin[sl], nvs = s.slabVertBegin[sl + 1] - vb;
+            // vertex slots: a tile's own vertices first (one per thread), then
+            // the rest in list order, then padding repeating the first
+            int* vs = verts.data() + ph.vertOff + (size_t)j * spare;
+            newLocal.assign(nvs, -1);
+            int k = 0;
+            if (fem)
+                for (int i = 0; i < nvs; i++)
+                    if (s.slabOwned[vb + i]) newLocal[i] = k++;
+            for (int i = 0; i < nvs; i++)
+                if (newLocal[i] < 0) newLocal[i] = k++;
+            for (int i = 0; i < nvs; i++) vs[newLocal[i]] = flagged[vb + i];
+            for (int i = nvs; i < spare; i++) vs[i] = (int)(((unsigned)flagged[vb] & (unsigned)kPackIdMask) | kPackPad);
+            // constraint slots, colour by colour
+            const int nc = s.slabColBegin[sl + 1] - s.slabColBegin[sl];
+            for (int c = 0; c < ph.ncKernel; c++) {
+                const size_t base = ph.slotOff + ((size_t)j * ph.ncKernel + c) * ph.width;
+                const int lo = c < nc ? s.colOffsets[s.slabColBegin[sl] + c] : 0;
+                const int cnt = c < nc ? s.colOffsets[s.slabColBegin[sl] + c + 1] - lo : 0;
+                for (int i = 0; i < ph.width; i++) {
+                    uint16_t l[4] = {(uint16_t)spare, (uint16_t)spare, (uint16_t)spare, (uint16_t)spare};
+                    if (i < cnt)
+                        for (int a = 0; a < arity; a++) l[a] = (uint16_t)newLocal[s.localIdx[(size_t)(lo + i) * arity + a]];
+                    if (fem) {
+                        uint4 w = make_uint4((unsigned)l[0] | ((unsigned)l[1] << 16), l[2], 0, 0);
+                        if (i < cnt) {
+                            const Triangle& t = (*tris)[s.instCons[lo + i]];
+                            float lu0 = t.lu0, lv0 = t.lv0;
+                            std::memcpy(&w.z, &lu0, 4);
+                            std::memcpy(&w.w, &lv0, 4);
+                            s1[base + i] = make_float4(t.du1, t.du2, t.dv1, t.dv2);
+                            s2[base + i] = make_float4(t.invDet, t.lu2, t.lv2, t.area);
+                        }
+                        s0[base + i] = w;
+                    } else {
+                        ids[base + i] = make_uint2((unsigned)l[0] | ((unsigned)l[1] << 16),
+                                                   (unsigned)l[2] | ((unsigned)l[3] << 16));
+                    }
+                }
+            }
+        }
+    }
+    dev_alloc(d.verts, nVert);
+    HIP_OK(hipMemcpy(d.verts, verts.data(), nVert * sizeof(int), hipMemcpyHostToDevice));
+    if (fem) {
+        dev_alloc(d.s0, nSlot); dev_alloc(d.s1, nSlot); dev_alloc(d.s2, nSlot);
+        HIP_OK(hipMemcpy(d.s0, s0.data(), nSlot * sizeof(uint4), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(d.s1, s1.data(), nSlot * sizeof(float4), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(d.s2, s2.data(), nSlot * sizeof(float4), hipMemcpyHostToDevice));
+    } else {
+        dev_alloc(d.ids, nSlot);
+        HIP_OK(hipMemcpy(d.ids, ids.data(), nSlot * sizeof(uint2), hipMemcpyHostToDevice));
+    }
+    d.phases = std::move(phases);
 }
 
 void ClothModel::build_device_topology()
@@ -547,8 +682,12 @@ void ClothModel::build_device_topology()
     dev_alloc(g.d_bendColOff, bendCol.phaseOffsets.size());
     HIP_OK(hipMemcpy(g.d_bendColOff, bendCol.phaseOffsets.data(), bendCol.phaseOffsets.size() * sizeof(int),
                      hipMemcpyHostToDevice));
-    g.triOrphans = upload_slabs(triCol, g.triSlabs, true, g.nv);
-    g.bendOrphans = upload_slabs(bendCol, g.bendSlabs, false, g.nv, true);
+    std::vector<int> triFlagged, bendFlagged;
+    g.triOrphans = upload_slabs(triCol, g.triSlabs, true, g.nv, false, &triFlagged);
+    g.bendOrphans = upload_slabs(bendCol, g.bendSlabs, false, g.nv, true, &bendFlagged);
+    const bool packed = mSweepMode == SWEEP_PACKED;
+    pack_schedule(packed, triCol, triFlagged, g.nv, &tris, g.triPacked);
+    pack_schedule(packed, bendCol, bendFlagged, g.nv, nullptr, g.bendPacked);
     // links in schedule order
     g.nl = (int)links.size();
     std::vector<int2> lk(g.nl);
@@ -647,7 +786,7 @@ void ClothModel::ensure_colouring()
         linkCol = schedule_slabs(nl, 2, lv.data(), nv, pos.data(), mSlabSize, maxVerts, seamed);
     } else {
         bool tiled = false;
-        if (mSweepMode == SWEEP_TILES && nt > 0) {
+        if ((mSweepMode == SWEEP_TILES || mSweepMode == SWEEP_PACKED) && nt > 0) {
             // tiles of about one owned vertex per thread of the workgroup;
             // a mesh whose halo makes a tile too large keeps the slabs
             try {
@@ -666,7 +805,8 @@ void ClothModel::ensure_colouring()
 
 void ClothModel::set_sweep(int mode, int slabSize)
 {
-    if (mode != SWEEP_COLOURS && mode != SWEEP_SLABS && mode != SWEEP_SEAMS && mode != SWEEP_TILES)
+    if (mode != SWEEP_COLOURS && mode != SWEEP_SLABS && mode != SWEEP_SEAMS && mode != SWEEP_TILES &&
+        mode != SWEEP_PACKED)
         throw std::runtime_error("unknown sweep mode");
     if (slabSize < 0 || slabSize > 65536) throw std::runtime_error("slab size out of range");
     if (mode == mSweepMode && slabSize == mSlabSize) return;
@@ -1096,8 +1236,15 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                     SlabContacts kc{cb, g.d_prev, g.d_vel, mThickness, mFriction, invH,
                                     (fuse && k > 0) ? 1 : contactsInFem ? 2 : 0};
                     const TriStreams fs = triCol.tiled ? TriStreams{g.d_tidx, g.d_ttA, g.d_ttB, g.d_ttC} : tr;
-                    launch_fem_slab(s, g.d_pos, fs, g.triSlabs.lidx, ss, ns, triCol.maxSlabVerts, h2, epsUV, g.d_ctl,
-                                    kc, triCol.tiled);
+                    if ((int)g.triPacked.phases.size() == triCol.numPhases() && kc.enabled != 1) {
+                        const DevicePacked::Phase& ph = g.triPacked.phases[p];
+                        const PackedFem pf{g.triPacked.verts + ph.vertOff, g.triPacked.s0 + ph.slotOff,
+                                           g.triPacked.s1 + ph.slotOff, g.triPacked.s2 + ph.slotOff, ph.nc};
+                        launch_fem_packed(s, g.d_pos, pf, ph.slabs, ph.ncKernel, ph.kv, h2, epsUV, g.d_ctl, kc);
+                    } else {
+                        launch_fem_slab(s, g.d_pos, fs, g.triSlabs.lidx, ss, ns, triCol.maxSlabVerts, h2, epsUV,
+                                        g.d_ctl, kc, triCol.tiled);
+                    }
                 } else {
                     int st = triCol.phaseOffsets[p], cnt = triCol.phaseOffsets[p + 1] - st;
                     launch_fem(s, g.d_pos, tr, st, cnt, h2, epsUV, g.d_ctl);
@@ -1113,7 +1260,15 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                     SlabSet ss{g.bendSlabs.info, g.bendSlabs.colOff, g.bendSlabs.first, g.bendSlabs.verts, b,
                                (int)bendCol.order.size()};
                     SlabContacts kc{cb, g.d_prev, g.d_vel, mThickness, mFriction, invH, contactsInBends ? 3 : 0};
-                    launch_bend_slab(s, g.d_pos, g.bendSlabs.lidx, ss, ns, bendCol.maxSlabVerts, sStiff, g.d_ctl, kc);
+                    if ((int)g.bendPacked.phases.size() == bendCol.numPhases()) {
+                        const DevicePacked::Phase& ph = g.bendPacked.phases[p];
+                        const PackedBend pb{g.bendPacked.verts + ph.vertOff, g.bendPacked.ids + ph.slotOff, ph.nc,
+                                            ph.width};
+                        launch_bend_packed(s, g.d_pos, pb, ph.slabs, ph.kv, sStiff, g.d_ctl, kc);
+                    } else {
+                        launch_bend_slab(s, g.d_pos, g.bendSlabs.lidx, ss, ns, bendCol.maxSlabVerts, sStiff, g.d_ctl,
+                                         kc);
+                    }
                 } else {
                     int st = bendCol.phaseOffsets[p], cnt = bendCol.phaseOffsets[p + 1] - st;
                     launch_bend(s, g.d_pos, g.d_bidx, st, cnt, sStiff, g.d_ctl);

@@ -180,7 +180,9 @@ private:
 // Sweep schedules: 0 = one launch per colour, 1 = slab phases with LDS
 // staging, 2 = slab phases of strip interiors and seams, 3 = FEM triangles on
 // tiles with halos (one launch per pass), bends and links as in 1
-enum { SWEEP_COLOURS = 0, SWEEP_SLABS = 1, SWEEP_SEAMS = 2, SWEEP_TILES = 3 };
+// SWEEP_PACKED: the schedules of SWEEP_TILES, laid out for the preloading
+// sweep kernels (k_fem_packed, k_bend_packed) where they fit their limits
+enum { SWEEP_COLOURS = 0, SWEEP_SLABS = 1, SWEEP_SEAMS = 2, SWEEP_TILES = 3, SWEEP_PACKED = 4 };
 // Physics/ClothModel.h:23-30 (SolverType); Step runs SolveGS for
 // Gauss-Seidel, SolveJacobi for Jacobi and SolveCR for every other value
 enum { SOLVER_JACOBI = 0, SOLVER_GAUSS_SEIDEL = 1, SOLVER_JACOBI_CR = 2, SOLVER_IMPLICIT = 3, SOLVER_EXPLICIT = 4 };
@@ -192,6 +194,21 @@ struct DeviceSlabs {
     int* colOff = nullptr;
     int* first = nullptr;
     int* verts = nullptr;
+};
+
+// Device copy of a schedule packed for the preloading sweep kernels
+// (PackedFem for FEM tiles, PackedBend for bend slabs); no phases when the
+// schedule is not packed.
+struct DevicePacked {
+    int* verts = nullptr;
+    uint4* s0 = nullptr;
+    float4 *s1 = nullptr, *s2 = nullptr;
+    uint2* ids = nullptr;
+    struct Phase {
+        int slabs = 0, kv = 0, nc = 0, ncKernel = 0, width = 0;
+        size_t vertOff = 0, slotOff = 0;
+    };
+    std::vector<Phase> phases;
 };
 
 struct KernelTimer {
@@ -260,6 +277,7 @@ public:
         bool built = false;
     } self;
     DeviceSlabs triSlabs, bendSlabs, linkSlabs;
+    DevicePacked triPacked, bendPacked;
     int triOrphans = 0;  // vertices in no triangle slab
     int bendOrphans = 0;  // vertices in no bend slab
     int slots = 0, slotCap = 0;
@@ -400,7 +418,7 @@ public:
     // colour groups for obstacle-edge (cloth edges) and obstacle-vertex
     // (cloth triangles) contacts: greedy colourings in index order
     Schedule edgeContactCol, triContactCol;
-    int mSweepMode = SWEEP_TILES;
+    int mSweepMode = SWEEP_PACKED;
     int mSlabSize = 0;  // 0: chosen per constraint type by schedule_slabs_auto
 
     GpuSolver gpu;

@@ -8,8 +8,10 @@ equality is expected to be bit-exact; the stated tolerance (1e-5 relative to
 the cloth extent) is the acceptance bar and bit-exactness is asserted on top.
 Every sweep schedule is covered: one launch per colour (mode 0), slab phases
 staged in LDS (mode 1), with strips and interior/seam slabs of several sizes,
-and the triangles on tiles with halos (mode 3), whose vertex contacts of an
-iteration without a bend pass are applied at the end of the FEM launch.
+the triangles on tiles with halos (mode 3), whose vertex contacts of an
+iteration without a bend pass are applied at the end of the FEM launch, and
+mode 3's schedules packed for the kernels that request a launch's data before
+its first colour (mode 4, the default).
 """
 import numpy as np
 import pytest
@@ -18,7 +20,7 @@ from solidcloth_amd import scenes as S
 
 pytestmark = pytest.mark.gpu
 REL_TOL = 1e-5
-MODES = [0, 1, 3]
+MODES = [0, 1, 3, 4]
 
 
 def _pair(sc, oracle_mod, scene, mode=1):
@@ -126,7 +128,7 @@ def test_drape256_full_size_parity(sc, oracle_mod, mode):
     assert np.array_equal(gp.astype(np.float64), op)
 
 
-@pytest.mark.parametrize("mode", [1, 3])
+@pytest.mark.parametrize("mode", [1, 3, 4])
 def test_drape256_slab_schedule_long_run(sc, oracle_mod, mode):
     """25 frames (100 substeps) of the full-size workload: the cloth reaches
     the sphere, so contacts and friction are active in the compared run."""
@@ -163,3 +165,27 @@ def test_run_twice_deterministic(sc):
             patch.Step(scene.dt)
         out.append(patch.GetModel().positions())
     assert np.array_equal(out[0], out[1])
+
+
+def test_packed_sweeps_equal_unpacked(sc):
+    """Mode 4 runs mode 3's schedules through the packed kernels: the same
+    constraint order and the same bits, on the full-size drape while it lies
+    on the sphere (contacts applied by FEM tiles and by bend slabs), with the
+    solver statistics alike."""
+    scene = S.DRAPE256
+    out = []
+    for mode in (3, 4):
+        print(f"drape256 sweep {mode}", flush=True)
+        world, patch = S.build_patch(scene, sc)
+        m = patch.GetModel()
+        m.SetSweep(mode)
+        for f in range(30):
+            patch.Step(scene.dt)
+        out.append((m.positions(), m.velocities(), m.constraint_order("tri"), m.constraint_order("bend"),
+                    m.stats(), len(m.contacts()[0])))
+    (p3, v3, t3, b3, s3, c3), (p4, v4, t4, b4, s4, c4) = out
+    assert np.array_equal(t3, t4) and np.array_equal(b3, b4)
+    assert c3 == c4 and c3 > 0
+    assert np.array_equal(p3, p4)
+    assert np.array_equal(v3, v4)
+    assert s3 == s4
