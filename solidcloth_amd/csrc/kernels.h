@@ -167,14 +167,15 @@ struct SlabContacts {
 constexpr unsigned kPackPad = 0x20000000u;
 constexpr int kPackIdMask = 0x1fffffff;
 // FEM tiles: colour c of tile t has slots [(t nc + c) kSlabBlock, +kSlabBlock);
-// s0 = (local 1 | local 2 << 16, local 3, lu0 bits, lv0 bits), s1 = (du1,
-// du2, dv1, dv2), s2 = (invDet, lu2, lv2, area). Owned vertices come first
+// s0 = (local 1 | local 2 << 10 | local 3 << 20, invDet bits, lu0 bits, lv0
+// bits), s1 = (du1, du2, dv1, dv2): 32 bytes a slot. A tile has fewer than
+// kv * kSlabBlock vertices (at most 1023: ten-bit local ids), and its last
+// vertex slot, always padding, is the spare entry. Owned vertices come first
 // in a tile's list, at most one per thread.
 struct PackedFem {
     const int* verts;
     const uint4* s0;
     const float4* s1;
-    const float4* s2;
     int nc;  // colours in use (the rest of the kernel's colours are padding)
 };
 // Bend slabs: colour c of slab s has slots [(s nc + c) width, +width), each

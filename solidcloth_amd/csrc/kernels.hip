@@ -909,12 +909,25 @@ __device__ __forceinline__ float packed_vertex_contacts(int g, const ContactPre&
 // One colour's FEM constraint slots of this thread.
 struct FemSlot {
     uint4 a0;
-    float4 a1, a2;
+    float4 a1;
 };
 
 __device__ __forceinline__ FemSlot fem_slot(const PackedFem& pk, size_t k)
 {
-    return FemSlot{pk.s0[k], load4(pk.s1 + k), load4(pk.s2 + k)};
+    return FemSlot{pk.s0[k], load4(pk.s1 + k)};
+}
+
+// The rest data of a packed FEM slot as fem_project takes it: the slot holds
+// du1, du2, dv1, dv2, invDet, lu0 and lv0; area = det / 2, lu2 = lu0^2 and
+// lv2 = lv0^2 are recomputed with the operations the host uses
+// (ClothModel.cpp:54-92), so they are the same values to the bit.
+__device__ __forceinline__ void fem_slot_rest(const FemSlot& f, float4& A, float4& B, float2& C)
+{
+    A = f.a1;
+    const float invDet = __uint_as_float(f.a0.y), lu0 = __uint_as_float(f.a0.z), lv0 = __uint_as_float(f.a0.w);
+    const float det = A.x * A.w - A.y * A.z;
+    B = make_float4(invDet, lu0 * lu0, lv0 * lv0, 0.5f * det);
+    C = make_float2(lu0, lv0);
 }
 
 // FEM triangles on packed tiles (NC colours, KV vertex slots per thread).
@@ -930,8 +943,8 @@ __global__ void __launch_bounds__(kSlabBlock) k_fem_packed(float4* pos, PackedFe
                                                            float epsUV, SolverCtl* __restrict__ ctl, SlabContacts kc)
 {
     static_assert(NC > kFemAhead, "colours beyond the stream distance");
-    constexpr int kSpare = KV * kSlabBlock;
-    __shared__ float4 sv[kSpare + 1];
+    constexpr int kSpare = KV * kSlabBlock;  // vertex slots; the last is padding and the spare entry
+    __shared__ float4 sv[kSpare];
     const int tid = threadIdx.x;
     const int done = ctl->done;
     const int* vg = pk.verts + (size_t)blockIdx.x * kSpare;
@@ -963,10 +976,12 @@ __global__ void __launch_bounds__(kSlabBlock) k_fem_packed(float4* pos, PackedFe
         // (a branch per colour keeps each colour's arithmetic, and the wait
         // for its data, in its own block)
         if (c < pk.nc) {
-            const int i1 = f[c].a0.x & 0xffff, i2 = f[c].a0.x >> 16, i3 = f[c].a0.y;
+            const int i1 = f[c].a0.x & 0x3ff, i2 = (f[c].a0.x >> 10) & 0x3ff, i3 = f[c].a0.x >> 20;
             float4 P1 = sv[i1], P2 = sv[i2], P3 = sv[i3];
-            fem_project(P1, P2, P3, f[c].a1, f[c].a2,
-                        make_float2(__uint_as_float(f[c].a0.z), __uint_as_float(f[c].a0.w)), h2, epsUV);
+            float4 A, B;
+            float2 C;
+            fem_slot_rest(f[c], A, B, C);
+            fem_project(P1, P2, P3, A, B, C, h2, epsUV);
             sv[i1] = P1;
             sv[i2] = P2;
             sv[i3] = P3;
@@ -1015,15 +1030,17 @@ __global__ void __launch_bounds__(kSlabBlock) k_bend_packed(float4* pos, PackedB
 #pragma unroll
     for (int u = 0; u < KV; u++) P[u] = load4(pos + (raw[u] & kPackIdMask));
     issue_fence();
+#pragma unroll
+    for (int u = 0; u < KV; u++) sv[tid + u * kSlabBlock] = P[u];
+    lds_barrier();
+    // the contact state is needed after the last colour: requested now, it
+    // arrives during the sweep without holding up the staging above
     ContactPre cp[KV];
     if (kContacts) {
 #pragma unroll
         for (int u = 0; u < KV; u++) cp[u] = contact_pre(raw[u] & kPackIdMask, kc);
     }
     issue_fence();
-#pragma unroll
-    for (int u = 0; u < KV; u++) sv[tid + u * kSlabBlock] = P[u];
-    lds_barrier();
     float err = 0.f;
 #pragma unroll
     for (int c = 0; c < kPackMaxBendColours; c++) {

@@ -340,7 +340,7 @@ void GpuSolver::release()
         dev_free(d->lidx); dev_free(d->info); dev_free(d->colOff); dev_free(d->first); dev_free(d->verts);
     }
     for (DevicePacked* d : {&triPacked, &bendPacked}) {
-        dev_free(d->verts); dev_free(d->s0); dev_free(d->s1); dev_free(d->s2); dev_free(d->ids);
+        dev_free(d->verts); dev_free(d->s0); dev_free(d->s1); dev_free(d->ids);
         d->phases.clear();
     }
     slotCap = 0;
@@ -495,7 +495,7 @@ static int upload_slabs(const Schedule& s, DeviceSlabs& d, bool owners, int numV
 static void pack_schedule(bool enabled, const Schedule& s, const std::vector<int>& flagged, int numVertices,
                           const std::vector<Triangle>* tris, DevicePacked& d)
 {
-    dev_free(d.verts); dev_free(d.s0); dev_free(d.s1); dev_free(d.s2); dev_free(d.ids);
+    dev_free(d.verts); dev_free(d.s0); dev_free(d.s1); dev_free(d.ids);
     d.phases.clear();
     const bool fem = tris != nullptr;
     if (!enabled || !s.slabbed || s.order.empty() || fem != s.tiled || numVertices >= (int)kPackIdMask) return;
@@ -517,6 +517,8 @@ static void pack_schedule(bool enabled, const Schedule& s, const std::vector<int
             }
         }
         if (maxVerts > kPackMaxKv * kSlabBlock || maxWidth > kSlabBlock || maxCol < 1) return;
+        // FEM tiles keep their last vertex slot free as the spare entry
+        if (fem && maxVerts >= kPackMaxKv * kSlabBlock) return;
         ph.slabs = s.phaseSlabBegin[p + 1] - s.phaseSlabBegin[p];
         ph.kv = std::max(2, (maxVerts + kSlabBlock - 1) / kSlabBlock);
         ph.nc = maxCol;
@@ -524,7 +526,7 @@ static void pack_schedule(bool enabled, const Schedule& s, const std::vector<int
             // the kernel's colour count (instantiations for 8 and 12)
             if (maxCol > 12) return;
             ph.ncKernel = maxCol <= 8 ? 8 : 12;
-            ph.kv = ph.kv <= 2 ? 2 : 4;
+            ph.kv = maxVerts < 2 * kSlabBlock ? 2 : 4;
             ph.width = kSlabBlock;
         } else {
             if (maxCol > kPackMaxBendColours) return;
@@ -547,20 +549,20 @@ static void pack_schedule(bool enabled, const Schedule& s, const std::vector<int
     if (fem) {
         s0.assign(nSlot, make_uint4(0, 0, 0, 0));
         s1.assign(nSlot, make_float4(0, 0, 0, 0));
-        s2.assign(nSlot, make_float4(0, 0, 0, 0));
     } else {
         ids.assign(nSlot, make_uint2(0, 0));
     }
     std::vector<int> newLocal;
     for (int p = 0; p < (int)phases.size(); p++) {
         const DevicePacked::Phase& ph = phases[p];
-        const int spare = ph.kv * kSlabBlock;
+        const int slots = ph.kv * kSlabBlock;
+        const int spare = fem ? slots - 1 : slots;
         for (int j = 0; j < ph.slabs; j++) {
             const int sl = s.phaseSlabBegin[p] + j;
             const int vb = s.slabVertBegin[sl], nvs = s.slabVertBegin[sl + 1] - vb;
             // vertex slots: a tile's own vertices first (one per thread), then
             // the rest in list order, then padding repeating the first
-            int* vs = verts.data() + ph.vertOff + (size_t)j * spare;
+            int* vs = verts.data() + ph.vertOff + (size_t)j * slots;
             newLocal.assign(nvs, -1);
             int k = 0;
             if (fem)
@@ -569,7 +571,7 @@ static void pack_schedule(bool enabled, const Schedule& s, const std::vector<int
             for (int i = 0; i < nvs; i++)
                 if (newLocal[i] < 0) newLocal[i] = k++;
             for (int i = 0; i < nvs; i++) vs[newLocal[i]] = flagged[vb + i];
-            for (int i = nvs; i < spare; i++) vs[i] = (int)(((unsigned)flagged[vb] & (unsigned)kPackIdMask) | kPackPad);
+            for (int i = nvs; i < slots; i++) vs[i] = (int)(((unsigned)flagged[vb] & (unsigned)kPackIdMask) | kPackPad);
             // constraint slots, colour by colour
             const int nc = s.slabColBegin[sl + 1] - s.slabColBegin[sl];
             for (int c = 0; c < ph.ncKernel; c++) {
@@ -581,14 +583,14 @@ static void pack_schedule(bool enabled, const Schedule& s, const std::vector<int
                     if (i < cnt)
                         for (int a = 0; a < arity; a++) l[a] = (uint16_t)newLocal[s.localIdx[(size_t)(lo + i) * arity + a]];
                     if (fem) {
-                        uint4 w = make_uint4((unsigned)l[0] | ((unsigned)l[1] << 16), l[2], 0, 0);
+                        uint4 w = make_uint4((unsigned)l[0] | ((unsigned)l[1] << 10) | ((unsigned)l[2] << 20), 0, 0, 0);
                         if (i < cnt) {
                             const Triangle& t = (*tris)[s.instCons[lo + i]];
-                            float lu0 = t.lu0, lv0 = t.lv0;
-                            std::memcpy(&w.z, &lu0, 4);
-                            std::memcpy(&w.w, &lv0, 4);
+                            const float rest[3] = {t.invDet, t.lu0, t.lv0};
+                            std::memcpy(&w.y, &rest[0], 4);
+                            std::memcpy(&w.z, &rest[1], 4);
+                            std::memcpy(&w.w, &rest[2], 4);
                             s1[base + i] = make_float4(t.du1, t.du2, t.dv1, t.dv2);
-                            s2[base + i] = make_float4(t.invDet, t.lu2, t.lv2, t.area);
                         }
                         s0[base + i] = w;
                     } else {
@@ -602,10 +604,9 @@ static void pack_schedule(bool enabled, const Schedule& s, const std::vector<int
     dev_alloc(d.verts, nVert);
     HIP_OK(hipMemcpy(d.verts, verts.data(), nVert * sizeof(int), hipMemcpyHostToDevice));
     if (fem) {
-        dev_alloc(d.s0, nSlot); dev_alloc(d.s1, nSlot); dev_alloc(d.s2, nSlot);
+        dev_alloc(d.s0, nSlot); dev_alloc(d.s1, nSlot);
         HIP_OK(hipMemcpy(d.s0, s0.data(), nSlot * sizeof(uint4), hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(d.s1, s1.data(), nSlot * sizeof(float4), hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(d.s2, s2.data(), nSlot * sizeof(float4), hipMemcpyHostToDevice));
     } else {
         dev_alloc(d.ids, nSlot);
         HIP_OK(hipMemcpy(d.ids, ids.data(), nSlot * sizeof(uint2), hipMemcpyHostToDevice));
@@ -1239,7 +1240,7 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                     if ((int)g.triPacked.phases.size() == triCol.numPhases() && kc.enabled != 1) {
                         const DevicePacked::Phase& ph = g.triPacked.phases[p];
                         const PackedFem pf{g.triPacked.verts + ph.vertOff, g.triPacked.s0 + ph.slotOff,
-                                           g.triPacked.s1 + ph.slotOff, g.triPacked.s2 + ph.slotOff, ph.nc};
+                                           g.triPacked.s1 + ph.slotOff, ph.nc};
                         launch_fem_packed(s, g.d_pos, pf, ph.slabs, ph.ncKernel, ph.kv, h2, epsUV, g.d_ctl, kc);
                     } else {
                         launch_fem_slab(s, g.d_pos, fs, g.triSlabs.lidx, ss, ns, triCol.maxSlabVerts, h2, epsUV,

@@ -202,7 +202,7 @@ struct DeviceSlabs {
 struct DevicePacked {
     int* verts = nullptr;
     uint4* s0 = nullptr;
-    float4 *s1 = nullptr, *s2 = nullptr;
+    float4* s1 = nullptr;
     uint2* ids = nullptr;
     struct Phase {
         int slabs = 0, kv = 0, nc = 0, ncKernel = 0, width = 0;
