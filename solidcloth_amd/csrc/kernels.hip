@@ -1033,17 +1033,18 @@ __global__ void __launch_bounds__(kSlabBlock) k_bend_packed(float4* pos, PackedB
 #pragma unroll
     for (int u = 0; u < KV; u++) sv[tid + u * kSlabBlock] = P[u];
     lds_barrier();
-    // the contact state is needed after the last colour: requested now, it
-    // arrives during the sweep without holding up the staging above
+    // the contact state is needed after the last colour: requested during the
+    // sweep, one vertex slot per colour (a slab has more colours than slots),
+    // it arrives without holding up the staging or any one colour step
+    static_assert(KV <= 4, "contact slots requested over the first colours");
     ContactPre cp[KV];
-    if (kContacts) {
-#pragma unroll
-        for (int u = 0; u < KV; u++) cp[u] = contact_pre(raw[u] & kPackIdMask, kc);
-    }
-    issue_fence();
     float err = 0.f;
 #pragma unroll
     for (int c = 0; c < kPackMaxBendColours; c++) {
+        if (kContacts && c < KV) {
+            cp[c] = contact_pre(raw[c] & kPackIdMask, kc);
+            issue_fence();
+        }
         if (c >= nc) continue;  // (not break: the loop must unroll, q lives in registers)
         if (lane) {
             const int i1 = q[c].x & 0xffff, i2 = q[c].x >> 16, i3 = q[c].y & 0xffff, i4 = q[c].y >> 16;
