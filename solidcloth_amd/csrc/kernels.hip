@@ -873,18 +873,17 @@ struct ContactPre {
     float4 C0P, C0N, C1P, C1N, PV, VE;
 };
 
-__device__ __forceinline__ ContactPre contact_pre(int g, const SlabContacts& kc)
+// The rest of the state of a vertex with cnt > 0 contacts (its count is
+// requested first, with the positions).
+__device__ __forceinline__ void contact_pre(int g, ContactPre& c, const SlabContacts& kc)
 {
-    ContactPre c;
-    c.cnt = kc.cb.count[g];
     c.C0P = load4(kc.cb.P + g);
     c.C0N = load4(kc.cb.N + g);
-    const int g1 = kc.cb.slots > 1 ? kc.cb.nv + g : g;
+    const int g1 = c.cnt > 1 ? kc.cb.nv + g : g;
     c.C1P = load4(kc.cb.P + g1);
     c.C1N = load4(kc.cb.N + g1);
     c.PV = load4(kc.prev + g);
     c.VE = load4(kc.vel + g);
-    return c;
 }
 
 // The vertex contacts of one staged vertex (position in LDS at sv[slot]) from
@@ -956,6 +955,12 @@ __global__ void __launch_bounds__(kSlabBlock) k_fem_packed(float4* pos, PackedFe
     float4 P[KV];
 #pragma unroll
     for (int u = 0; u < KV; u++) P[u] = load4(pos + (raw[u] & kPackIdMask));
+    // slot 0 holds the thread's own vertex, if any, whose contact count comes
+    // with the positions and the rest of whose contact state comes later
+    const int g0 = raw[0] & kPackIdMask;
+    ContactPre cp;
+    cp.cnt = 0;
+    if (kContacts && raw[0] < 0) cp.cnt = kc.cb.count[g0];
     issue_fence();
     FemSlot f[NC];
     const size_t sb = (size_t)blockIdx.x * NC * kSlabBlock + tid;
@@ -965,13 +970,10 @@ __global__ void __launch_bounds__(kSlabBlock) k_fem_packed(float4* pos, PackedFe
 #pragma unroll
     for (int u = 0; u < KV; u++) sv[tid + u * kSlabBlock] = P[u];
     lds_barrier();
-    // slot 0 holds the thread's own vertex, if any
-    const int g0 = raw[0] & kPackIdMask;
-    ContactPre cp{};
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         if (c + kFemAhead < NC) f[c + kFemAhead] = fem_slot(pk, sb + (c + kFemAhead) * kSlabBlock);
-        if (kContacts && c + kFemAhead == NC) cp = contact_pre(g0, kc);
+        if (kContacts && c + kFemAhead == NC && cp.cnt > 0) contact_pre(g0, cp, kc);
         issue_fence();
         // (a branch per colour keeps each colour's arithmetic, and the wait
         // for its data, in its own block)
@@ -1029,20 +1031,27 @@ __global__ void __launch_bounds__(kSlabBlock) k_bend_packed(float4* pos, PackedB
     float4 P[KV];
 #pragma unroll
     for (int u = 0; u < KV; u++) P[u] = load4(pos + (raw[u] & kPackIdMask));
+    // the contact counts of the vertices whose last bend slab this is come
+    // with the positions; the rest of their contact state, needed after the
+    // last colour, is requested during the sweep, one vertex slot per colour
+    // (counted from the first, whatever the slab's colour count), so that it
+    // holds up neither the staging nor any one colour step
+    static_assert(KV <= 4, "contact slots requested over the first colours");
+    ContactPre cp[KV];
+#pragma unroll
+    for (int u = 0; u < KV; u++) {
+        cp[u].cnt = 0;
+        if (kContacts && ((unsigned)raw[u] & kSlabContacts)) cp[u].cnt = kc.cb.count[raw[u] & kPackIdMask];
+    }
     issue_fence();
 #pragma unroll
     for (int u = 0; u < KV; u++) sv[tid + u * kSlabBlock] = P[u];
     lds_barrier();
-    // the contact state is needed after the last colour: requested during the
-    // sweep, one vertex slot per colour (a slab has more colours than slots),
-    // it arrives without holding up the staging or any one colour step
-    static_assert(KV <= 4, "contact slots requested over the first colours");
-    ContactPre cp[KV];
     float err = 0.f;
 #pragma unroll
     for (int c = 0; c < kPackMaxBendColours; c++) {
         if (kContacts && c < KV) {
-            cp[c] = contact_pre(raw[c] & kPackIdMask, kc);
+            if (cp[c].cnt > 0) contact_pre(raw[c] & kPackIdMask, cp[c], kc);
             issue_fence();
         }
         if (c >= nc) continue;  // (not break: the loop must unroll, q lives in registers)
