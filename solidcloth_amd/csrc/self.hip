@@ -516,6 +516,9 @@ __device__ __forceinline__ unsigned long long bid_of(unsigned long long key)
 // colour: counts per colour in groupOff (global memory, capacity + 1
 // entries), an exclusive prefix sum over them, and a scatter through cursors
 // kept in the (by then unused) round list.
+// The per-vertex bids (vmax) are touched by this one workgroup only (zero
+// before and after it), so their atomics, loads and stores are scoped to the
+// workgroup: they complete in the XCD's L2 instead of crossing to memory.
 __global__ void __launch_bounds__(kSoloBlock) k_self_colour(SelfBuf b, int kind, unsigned long long* vmax,
                                                             SolverCtl* ctl)
 {
@@ -525,11 +528,15 @@ __global__ void __launch_bounds__(kSoloBlock) k_self_colour(SelfBuf b, int kind,
     const int tid = threadIdx.x, nt = blockDim.x;
     auto vreset = [&](int c) {
         const int4 q = b.ids[c];
-        __hip_atomic_store(&vmax[q.x], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&vmax[q.y], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&vmax[q.z], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&vmax[q.w], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&vmax[q.x], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_store(&vmax[q.y], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_store(&vmax[q.z], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_store(&vmax[q.w], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     };
+    auto vbid = [&](int v, unsigned long long bid) {
+        __hip_atomic_fetch_max(&vmax[v], bid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    auto vget = [&](int v) { return __hip_atomic_load(&vmax[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); };
     // the uncoloured contacts, compacted after every round (two lists)
     int* cur = b.order;
     int* nxt = b.active;
@@ -546,21 +553,17 @@ __global__ void __launch_bounds__(kSoloBlock) k_self_colour(SelfBuf b, int kind,
             const int c = cur[i];
             const int4 q = b.ids[c];
             const unsigned long long bid = bid_of(b.key[c]);
-            atomicMax(&vmax[q.x], bid);
-            atomicMax(&vmax[q.y], bid);
-            atomicMax(&vmax[q.z], bid);
-            atomicMax(&vmax[q.w], bid);
+            vbid(q.x, bid);
+            vbid(q.y, bid);
+            vbid(q.z, bid);
+            vbid(q.w, bid);
         }
         __syncthreads();
         for (int i = tid; i < m; i += nt) {
             const int c = cur[i];
             const int4 q = b.ids[c];
             const unsigned long long bid = bid_of(b.key[c]);
-            const bool win =
-                __hip_atomic_load(&vmax[q.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == bid &&
-                __hip_atomic_load(&vmax[q.y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == bid &&
-                __hip_atomic_load(&vmax[q.z], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == bid &&
-                __hip_atomic_load(&vmax[q.w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == bid;
+            const bool win = vget(q.x) == bid && vget(q.y) == bid && vget(q.z) == bid && vget(q.w) == bid;
             if (win) b.colour[c] = r;
         }
         __syncthreads();
