@@ -48,6 +48,7 @@ struct SolverCtl {
     // largest selfCount[0..1] and candSlots[0..1] of the step's earlier
     // substeps (the host clears them before a step and folds in the last)
     int selfPeak[4];
+    int selfUniq[2];  // distinct vertices of each kind's contacts (this substep)
 };
 
 enum { MAX_COLLIDABLES = 8 };
@@ -204,6 +205,11 @@ struct SelfBuf {
     int* order;
     int* active;  // scratch list for the colouring rounds
     int* groupOff;
+    // the kind's distinct contact vertices (selfUniq of them, nv capacity)
+    // and each contact's vertices as indices into that list (valid when there
+    // are at most 65536 distinct vertices)
+    int* vlist;
+    ushort4* lid;
     int cap;
     int maxColours;
 };
@@ -370,7 +376,9 @@ struct SelfCandidates {
 void launch_self_detect(hipStream_t s, const float4* pos, const float4* prev, const int4* cTris, const int2* cEdges,
                         int nv, int ne, BvhView triBvh, BvhView edgeBvh, const int* vertexOrder, SelfParams sp,
                         SelfCandidates cand, SelfBuf vt, SelfBuf ee, SolverCtl* ctl);
-void launch_self_colour(hipStream_t s, SelfBuf b, int kind, unsigned long long* vmax, SolverCtl* ctl);
+// vloc: one int per cloth vertex, every byte 0x7f between calls (the
+// colouring claims and releases entries while it lists a kind's vertices)
+void launch_self_colour(hipStream_t s, SelfBuf b, int kind, unsigned long long* vmax, int* vloc, SolverCtl* ctl);
 void launch_self_solve(hipStream_t s, float4* pos, SelfBuf b, int kind, SelfParams sp, float h, SolverCtl* ctl);
 
 }  // namespace sc

@@ -329,9 +329,10 @@ void GpuSolver::release()
     dev_free(self.candFillVT); dev_free(self.candFillEE);
     self.candCapVT = self.candCapEE = 0;
     self.candWantVT = self.candWantEE = 0;
+    dev_free(self.vloc);
     for (SelfBuf& b : self.buf) {
         dev_free(b.ids); dev_free(b.W); dev_free(b.N); dev_free(b.key); dev_free(b.colour); dev_free(b.order);
-        dev_free(b.active); dev_free(b.groupOff);
+        dev_free(b.active); dev_free(b.groupOff); dev_free(b.lid); dev_free(b.vlist);
         b.cap = 0;
     }
     self.cap = 0;
@@ -891,6 +892,7 @@ void ClothModel::ensure_self()
             dev_alloc(b.ids, cap); dev_alloc(b.W, cap); dev_alloc(b.N, cap); dev_alloc(b.key, cap);
             dev_alloc(b.colour, cap); dev_alloc(b.order, cap); dev_alloc(b.active, cap);
             dev_alloc(b.groupOff, (size_t)cap + 1);  // at most one colour per contact
+            dev_alloc(b.lid, cap);
             b.cap = cap;
             b.maxColours = cap;
         }
@@ -920,6 +922,10 @@ void ClothModel::ensure_self()
         dev_alloc(S.edgeLo, std::max(g.ne, 1)); dev_alloc(S.edgeHi, std::max(g.ne, 1));
         dev_alloc(S.vtxLo, std::max(g.nv, 1)); dev_alloc(S.vtxHi, std::max(g.nv, 1));
         dev_alloc(S.vmax, std::max(g.nv, 1));
+        // the colouring's vertex claims: every byte 0x7f between colourings
+        dev_alloc(S.vloc, std::max(g.nv, 1));
+        HIP_OK(hipMemset(S.vloc, 0x7f, sizeof(int) * std::max(g.nv, 1)));
+        for (SelfBuf& b : S.buf) dev_alloc(b.vlist, std::max(g.nv, 1));
     }
     // hierarchy from the current configuration; substeps refit it (the
     // vertex tree only orders the vertex queries, so it is not refit)
@@ -1141,8 +1147,8 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                                SelfCandidates{S.candVT, S.candEE, S.candFillVT, S.candFillEE, S.candCapVT,
                                               S.candCapEE},
                                S.buf[0], S.buf[1], g.d_ctl);
-            launch_self_colour(s, S.buf[0], 0, S.vmax, g.d_ctl);
-            launch_self_colour(s, S.buf[1], 1, S.vmax, g.d_ctl);
+            launch_self_colour(s, S.buf[0], 0, S.vmax, S.vloc, g.d_ctl);
+            launch_self_colour(s, S.buf[1], 1, S.vmax, S.vloc, g.d_ctl);
         }
         timed_end(g.timer, timed, 3, s, ev);
     } else if (anyContacts && msActive) {

@@ -269,6 +269,7 @@ public:
         Lbvh triB, edgeB, vtxB;  // vtxB: Morton order of the vertex queries (built per step)
         SelfBuf buf[2]{};
         unsigned long long* vmax = nullptr;
+        int* vloc = nullptr;  // the colouring's per-vertex claims
         int2 *candVT = nullptr, *candEE = nullptr;  // candidate pools (chunks of kCandChunk pairs)
         int *candFillVT = nullptr, *candFillEE = nullptr;  // pairs per chunk
         int candCapVT = 0, candCapEE = 0;    // pool capacities in pairs

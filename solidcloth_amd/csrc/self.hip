@@ -516,69 +516,178 @@ __device__ __forceinline__ unsigned long long bid_of(unsigned long long key)
 // colour: counts per colour in groupOff (global memory, capacity + 1
 // entries), an exclusive prefix sum over them, and a scatter through cursors
 // kept in the (by then unused) round list.
-// The per-vertex bids (vmax) are touched by this one workgroup only (zero
-// before and after it), so their atomics, loads and stores are scoped to the
-// workgroup: they complete in the XCD's L2 instead of crossing to memory.
-__global__ void __launch_bounds__(kSoloBlock) k_self_colour(SelfBuf b, int kind, unsigned long long* vmax,
+//
+// First the kind's distinct vertices are listed (vlist) and each contact's
+// vertices renamed to their places in that list (lid): every vertex is
+// claimed by the least (contact, corner) that names it (atomic minimum in
+// vloc), its claimant appends it, and vloc holds its place until the
+// entries are released. With at most kColourPerThread contacts per thread
+// and kColourLdsVerts distinct vertices, the rounds then run on registers
+// and LDS alone (the bids of the listed vertices in LDS); otherwise on the
+// per-vertex bids in global memory (vmax), which this one workgroup owns
+// (zero before and after it), with atomics, loads and stores scoped to the
+// workgroup. Both give the same colours.
+constexpr int kColourPerThread = 8;
+constexpr int kColourLdsVerts = 12288;
+constexpr int kVlocFree = 0x7f7f7f7f, kVlocSlot = 0x40000000;
+
+__device__ __forceinline__ int corner(const int4 q, int j) { return j == 0 ? q.x : j == 1 ? q.y : j == 2 ? q.z : q.w; }
+
+__global__ void __launch_bounds__(kSoloBlock) k_self_colour(SelfBuf b, int kind, unsigned long long* vmax, int* vloc,
                                                             SolverCtl* ctl)
 {
-    __shared__ int nextCount;
+    __shared__ unsigned long long lbid[kColourLdsVerts];
+    __shared__ int nextCount, uniq, rem[2];
     __shared__ int part[kSoloBlock];
     const int n = min(ctl->selfCount[kind], b.cap);
     const int tid = threadIdx.x, nt = blockDim.x;
-    auto vreset = [&](int c) {
-        const int4 q = b.ids[c];
-        __hip_atomic_store(&vmax[q.x], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_store(&vmax[q.y], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_store(&vmax[q.z], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_store(&vmax[q.w], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    auto vatomic_min = [&](int v, int x) {
+        __hip_atomic_fetch_min(&vloc[v], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     };
-    auto vbid = [&](int v, unsigned long long bid) {
-        __hip_atomic_fetch_max(&vmax[v], bid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    auto vload = [&](int v) { return __hip_atomic_load(&vloc[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); };
+    auto vstore = [&](int v, int x) {
+        __hip_atomic_store(&vloc[v], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     };
-    auto vget = [&](int v) { return __hip_atomic_load(&vmax[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); };
-    // the uncoloured contacts, compacted after every round (two lists)
-    int* cur = b.order;
-    int* nxt = b.active;
-    for (int c = tid; c < n; c += nt) {
-        b.colour[c] = -1;
-        cur[c] = c;
-        vreset(c);
+    // the distinct vertices, in claim order
+    if (tid == 0) uniq = 0;
+    for (int i = tid; i < 4 * n; i += nt) vatomic_min(corner(b.ids[i >> 2], i & 3), i);
+    __syncthreads();
+    for (int i = tid; i < 4 * n; i += nt) {
+        const int v = corner(b.ids[i >> 2], i & 3);
+        if (vload(v) == i) {
+            const int slot = atomicAdd(&uniq, 1);
+            b.vlist[slot] = v;
+            vstore(v, kVlocSlot | slot);
+        }
     }
     __syncthreads();
-    int m = n, r = 0;
-    while (m > 0) {
-        if (tid == 0) nextCount = 0;
-        for (int i = tid; i < m; i += nt) {
-            const int c = cur[i];
+    const int m = uniq;
+    if (m <= 65536) {
+        for (int c = tid; c < n; c += nt) {
             const int4 q = b.ids[c];
-            const unsigned long long bid = bid_of(b.key[c]);
-            vbid(q.x, bid);
-            vbid(q.y, bid);
-            vbid(q.z, bid);
-            vbid(q.w, bid);
+            b.lid[c] = make_ushort4((unsigned short)(vload(q.x) & 0xffff), (unsigned short)(vload(q.y) & 0xffff),
+                                    (unsigned short)(vload(q.z) & 0xffff), (unsigned short)(vload(q.w) & 0xffff));
         }
+    }
+    __syncthreads();
+    for (int k = tid; k < m; k += nt) vstore(b.vlist[k], kVlocFree);
+    if (tid == 0) ctl->selfUniq[kind] = m;
+    int r = 0;
+    if (n <= kColourPerThread * kSoloBlock && m <= kColourLdsVerts) {
+        // this thread's contacts: local vertices, bid, colour (-1: none yet)
+        ushort4 ml[kColourPerThread];
+        unsigned long long mb[kColourPerThread];
+        int mc[kColourPerThread];
+#pragma unroll
+        for (int k = 0; k < kColourPerThread; k++) {
+            const int c = tid + k * nt;
+            mc[k] = -2;
+            if (c < n) {
+                ml[k] = b.lid[c];
+                mb[k] = bid_of(b.key[c]);
+                mc[k] = -1;
+            }
+        }
+        for (int k = tid; k < m; k += nt) lbid[k] = 0ull;
+        if (tid == 0) rem[0] = rem[1] = 0;
         __syncthreads();
-        for (int i = tid; i < m; i += nt) {
-            const int c = cur[i];
+        for (;;) {
+#pragma unroll
+            for (int k = 0; k < kColourPerThread; k++)
+                if (mc[k] == -1) {
+                    atomicMax(&lbid[ml[k].x], mb[k]);
+                    atomicMax(&lbid[ml[k].y], mb[k]);
+                    atomicMax(&lbid[ml[k].z], mb[k]);
+                    atomicMax(&lbid[ml[k].w], mb[k]);
+                }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kColourPerThread; k++)
+                if (mc[k] == -1 && lbid[ml[k].x] == mb[k] && lbid[ml[k].y] == mb[k] && lbid[ml[k].z] == mb[k] &&
+                    lbid[ml[k].w] == mb[k])
+                    mc[k] = r;
+            __syncthreads();
+            // clear this round's bids, count the contacts left (two counters
+            // in turn: the one read after this round is cleared in the next)
+            int left = 0;
+#pragma unroll
+            for (int k = 0; k < kColourPerThread; k++)
+                if (mc[k] == -1 || mc[k] == r) {
+                    lbid[ml[k].x] = 0ull;
+                    lbid[ml[k].y] = 0ull;
+                    lbid[ml[k].z] = 0ull;
+                    lbid[ml[k].w] = 0ull;
+                    left += mc[k] == -1;
+                }
+            if (left) atomicAdd(&rem[r & 1], left);
+            if (tid == 0) rem[(r + 1) & 1] = 0;
+            __syncthreads();
+            const int remaining = rem[r & 1];
+            r++;
+            if (remaining == 0) break;
+        }
+#pragma unroll
+        for (int k = 0; k < kColourPerThread; k++)
+            if (mc[k] >= 0) b.colour[tid + k * nt] = mc[k];
+        if (n == 0) r = 0;
+        __syncthreads();
+    } else {
+        auto vreset = [&](int c) {
             const int4 q = b.ids[c];
-            const unsigned long long bid = bid_of(b.key[c]);
-            const bool win = vget(q.x) == bid && vget(q.y) == bid && vget(q.z) == bid && vget(q.w) == bid;
-            if (win) b.colour[c] = r;
-        }
-        __syncthreads();
-        for (int i = tid; i < m; i += nt) {
-            const int c = cur[i];
+            __hip_atomic_store(&vmax[q.x], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(&vmax[q.y], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(&vmax[q.z], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(&vmax[q.w], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        };
+        auto vbid = [&](int v, unsigned long long bid) {
+            __hip_atomic_fetch_max(&vmax[v], bid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        };
+        auto vget = [&](int v) {
+            return __hip_atomic_load(&vmax[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        };
+        // the uncoloured contacts, compacted after every round (two lists)
+        int* cur = b.order;
+        int* nxt = b.active;
+        for (int c = tid; c < n; c += nt) {
+            b.colour[c] = -1;
+            cur[c] = c;
             vreset(c);
-            if (b.colour[c] < 0) nxt[atomicAdd(&nextCount, 1)] = c;
         }
         __syncthreads();
-        m = nextCount;
-        int* t = cur;
-        cur = nxt;
-        nxt = t;
-        __syncthreads();
-        r++;
+        int mm = n;
+        while (mm > 0) {
+            if (tid == 0) nextCount = 0;
+            for (int i = tid; i < mm; i += nt) {
+                const int c = cur[i];
+                const int4 q = b.ids[c];
+                const unsigned long long bid = bid_of(b.key[c]);
+                vbid(q.x, bid);
+                vbid(q.y, bid);
+                vbid(q.z, bid);
+                vbid(q.w, bid);
+            }
+            __syncthreads();
+            for (int i = tid; i < mm; i += nt) {
+                const int c = cur[i];
+                const int4 q = b.ids[c];
+                const unsigned long long bid = bid_of(b.key[c]);
+                const bool win = vget(q.x) == bid && vget(q.y) == bid && vget(q.z) == bid && vget(q.w) == bid;
+                if (win) b.colour[c] = r;
+            }
+            __syncthreads();
+            for (int i = tid; i < mm; i += nt) {
+                const int c = cur[i];
+                vreset(c);
+                if (b.colour[c] < 0) nxt[atomicAdd(&nextCount, 1)] = c;
+            }
+            __syncthreads();
+            mm = nextCount;
+            int* t = cur;
+            cur = nxt;
+            nxt = t;
+            __syncthreads();
+            r++;
+        }
     }
     // r colours; counts, then offsets
     const int nc = r;
@@ -616,14 +725,38 @@ __global__ void __launch_bounds__(kSoloBlock) k_self_colour(SelfBuf b, int kind,
 
 // --------------------------------------------------------- projections ----
 
-// ClothCollisionHandler.cpp:364-436 (SolveSelfTrisPosition)
-__device__ float self_vt(float4* pos, SelfBuf b, int c, const SelfParams& sp, float h)
+// A self contact's record as its projection reads it: vertices, weights and
+// side, normal and lambda. Only the contact's own projection writes W and N,
+// so the record can be read ahead of the contact's colour group.
+struct SelfRec {
+    int4 id;
+    ushort4 lid;  // places in the kind's vertex list (LDS solve)
+    float4 W, N;
+};
+
+__device__ __forceinline__ SelfRec self_rec(const SelfBuf& b, int c, bool local)
 {
-    const int4 id = b.ids[c];
-    const float4 P1 = pos[id.x], P2 = pos[id.y], P3 = pos[id.z], P4 = pos[id.w];
+    SelfRec r{};
+    if (c >= 0) {
+        if (local)
+            r.lid = b.lid[c];
+        else
+            r.id = b.ids[c];
+        r.W = b.W[c];
+        r.N = b.N[c];
+    }
+    return r;
+}
+
+// ClothCollisionHandler.cpp:364-436 (SolveSelfTrisPosition). at(j) is the
+// position of the contact's vertex j (global memory or an LDS copy).
+template <class At>
+__device__ __forceinline__ float self_vt(At at, SelfBuf b, int c, const SelfRec& r, const SelfParams& sp, float h)
+{
+    const float4 P1 = at(0), P2 = at(1), P3 = at(2), P4 = at(3);
     f3 x1 = xyz(P1), x2 = xyz(P2), x3 = xyz(P3), x4 = xyz(P4);
     const float m1 = P1.w, m2 = P2.w, m3 = P3.w, m4 = P4.w;
-    const float4 W = b.W[c], N = b.N[c];
+    const float4 W = r.W, N = r.N;
     const f3 cn = xyz(N);
     float len0 = sp.thickness * sp.shrink;
     f3 p, co;
@@ -654,10 +787,10 @@ __device__ float self_vt(float4* pos, SelfBuf b, int c, const SelfParams& sp, fl
         x2 = add(x2, mul(w2 * m2, disp));
         x3 = add(x3, mul(w3 * m3, disp));
         x4 = sub(x4, mul(m4, disp));
-        pos[id.x] = f4(x1, m1);
-        pos[id.y] = f4(x2, m2);
-        pos[id.z] = f4(x3, m3);
-        pos[id.w] = f4(x4, m4);
+        at(0) = f4(x1, m1);
+        at(1) = f4(x2, m2);
+        at(2) = f4(x3, m3);
+        at(3) = f4(x4, m4);
     }
     b.W[c] = make_float4(w1, w2, w3, W.w);
     b.N[c] = make_float4(N.x, N.y, N.z, lambda);
@@ -665,13 +798,13 @@ __device__ float self_vt(float4* pos, SelfBuf b, int c, const SelfParams& sp, fl
 }
 
 // ClothCollisionHandler.cpp:439-541 (SolveSelfEdgesPosition)
-__device__ float self_ee(float4* pos, SelfBuf b, int c, const SelfParams& sp, float h)
+template <class At>
+__device__ __forceinline__ float self_ee(At at, SelfBuf b, int c, const SelfRec& r, const SelfParams& sp, float h)
 {
-    const int4 id = b.ids[c];
-    const float4 P1 = pos[id.x], P2 = pos[id.y], P3 = pos[id.z], P4 = pos[id.w];
+    const float4 P1 = at(0), P2 = at(1), P3 = at(2), P4 = at(3);
     f3 x1 = xyz(P1), x2 = xyz(P2), x3 = xyz(P3), x4 = xyz(P4);
     const float m1 = P1.w, m2 = P2.w, m3 = P3.w, m4 = P4.w;
-    const float4 W = b.W[c], N = b.N[c];
+    const float4 W = r.W, N = r.N;
     float thick = sp.thickness * sp.shrink;
     const float side = signed_volume(x1, x2, x3, x4);
     f3 p, q;
@@ -711,10 +844,10 @@ __device__ float self_ee(float4* pos, SelfBuf b, int c, const SelfParams& sp, fl
             x3 = add(x3, mul(w2 * m3, disp));
             x4 = add(x4, mul(omw2 * m4, disp));
         }
-        pos[id.x] = f4(x1, m1);
-        pos[id.y] = f4(x2, m2);
-        pos[id.z] = f4(x3, m3);
-        pos[id.w] = f4(x4, m4);
+        at(0) = f4(x1, m1);
+        at(1) = f4(x2, m2);
+        at(2) = f4(x3, m3);
+        at(3) = f4(x4, m4);
     }
     b.W[c] = make_float4(w1, w2, 0.f, W.w);
     b.N[c] = make_float4(N.x, N.y, N.z, lambda);
@@ -722,22 +855,76 @@ __device__ float self_ee(float4* pos, SelfBuf b, int c, const SelfParams& sp, fl
 }
 
 // One workgroup walks the colour groups in order; a group's contacts share
-// no vertex and are projected in parallel.
+// no vertex and are projected in parallel. The group offsets sit in LDS and
+// each thread reads its contacts ahead: the contact index two groups ahead,
+// the record one group ahead. When the kind's distinct vertices fit
+// (kSolveLdsVerts), they are staged in LDS for the whole pass and written
+// back at its end, so a group waits on no memory at all; otherwise a group
+// waits only for the positions the previous group wrote.
+constexpr int kSelfGroupsLds = 4096;
+constexpr int kSolveLdsVerts = 8192;
+
 __global__ void __launch_bounds__(kSoloBlock) k_self_solve(float4* pos, SelfBuf b, int kind, SelfParams sp, float h,
                                                            SolverCtl* ctl)
 {
+    __shared__ int goff[kSelfGroupsLds + 1];
+    __shared__ float4 sv[kSolveLdsVerts];
     if (ctl->done) return;
     const int nc = ctl->selfColours[kind];
-    float err = 0.f;
-    for (int g = 0; g < nc; g++) {
-        const int e = b.groupOff[g + 1];
-        for (int k = b.groupOff[g] + threadIdx.x; k < e; k += blockDim.x) {
-            const int c = b.order[k];
-            const float x = kind == 0 ? self_vt(pos, b, c, sp, h) : self_ee(pos, b, c, sp, h);
-            err = err < x ? x : err;
+    const int m = ctl->selfUniq[kind];
+    const int tid = threadIdx.x;
+    const bool inLds = nc <= kSelfGroupsLds;
+    const bool local = m <= kSolveLdsVerts;
+    if (inLds)
+        for (int g = tid; g <= nc; g += kSoloBlock) goff[g] = b.groupOff[g];
+    if (local)
+        for (int k = tid; k < m; k += kSoloBlock) sv[k] = pos[b.vlist[k]];
+    __syncthreads();
+    auto project = [&](int c, const SelfRec& r) {
+        if (local) {
+            auto at = [&](int j) -> float4& {
+                return sv[j == 0 ? r.lid.x : j == 1 ? r.lid.y : j == 2 ? r.lid.z : r.lid.w];
+            };
+            return kind == 0 ? self_vt(at, b, c, r, sp, h) : self_ee(at, b, c, r, sp, h);
         }
-        __syncthreads();
-    }
+        auto at = [&](int j) -> float4& { return pos[j == 0 ? r.id.x : j == 1 ? r.id.y : j == 2 ? r.id.z : r.id.w]; };
+        return kind == 0 ? self_vt(at, b, c, r, sp, h) : self_ee(at, b, c, r, sp, h);
+    };
+    // the pass over the groups, with the offsets read through off(g) (one
+    // instance per source, so each reads its own memory directly)
+    auto sweep = [&](auto off) {
+        // this thread's first contact of group g, or -1
+        auto first = [&](int g) {
+            if (g >= nc) return -1;
+            const int k = off(g) + tid;
+            return k < off(g + 1) ? b.order[k] : -1;
+        };
+        float err = 0.f;
+        int c1 = first(0), c2 = first(1);
+        SelfRec r1 = self_rec(b, c1, local);
+        for (int g = 0; g < nc; g++) {
+            const int c3 = first(g + 2);
+            const SelfRec r2 = self_rec(b, c2, local);
+            if (c1 >= 0) {
+                const float x = project(c1, r1);
+                err = err < x ? x : err;
+            }
+            // a group wider than the workgroup: its further contacts in turn
+            for (int k = off(g) + tid + kSoloBlock; k < off(g + 1); k += kSoloBlock) {
+                const int c = b.order[k];
+                const float x = project(c, self_rec(b, c, local));
+                err = err < x ? x : err;
+            }
+            __syncthreads();
+            c1 = c2;
+            r1 = r2;
+            c2 = c3;
+        }
+        return err;
+    };
+    const float err = inLds ? sweep([&](int g) { return goff[g]; }) : sweep([&](int g) { return b.groupOff[g]; });
+    if (local)
+        for (int k = tid; k < m; k += kSoloBlock) pos[b.vlist[k]] = sv[k];
     wave_max_atomic(kind == 0 ? &ctl->sct : &ctl->sce, err);
 }
 
@@ -766,9 +953,9 @@ void launch_self_detect(hipStream_t s, const float4* pos, const float4* prev, co
                        ctl);
 }
 
-void launch_self_colour(hipStream_t s, SelfBuf b, int kind, unsigned long long* vmax, SolverCtl* ctl)
+void launch_self_colour(hipStream_t s, SelfBuf b, int kind, unsigned long long* vmax, int* vloc, SolverCtl* ctl)
 {
-    hipLaunchKernelGGL(k_self_colour, dim3(1), dim3(kSoloBlock), 0, s, b, kind, vmax, ctl);
+    hipLaunchKernelGGL(k_self_colour, dim3(1), dim3(kSoloBlock), 0, s, b, kind, vmax, vloc, ctl);
 }
 
 void launch_self_solve(hipStream_t s, float4* pos, SelfBuf b, int kind, SelfParams sp, float h, SolverCtl* ctl)
