@@ -566,10 +566,15 @@ def mesh512_record(args, cpu):
     patch.Step(scene.dt)
     first = time.perf_counter() - t0
     t0 = time.perf_counter()
+    most = {"vertices_vt": 0, "vertices_ee": 0}
     for f in range(1, scene.frames):
         patch.Step(scene.dt)
         if f % 50 == 0:
             print(f"mesh512: frame {f}/{scene.frames}, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        if f % 10 == 0:
+            sf = m.self_stats()
+            for k in most:
+                most[k] = max(most[k], sf[k])
     el = time.perf_counter() - t0
     value = (scene.frames - 1) * scene.substeps / el
     st = m.self_stats()
@@ -578,7 +583,7 @@ def mesh512_record(args, cpu):
            "timed": f"frames 2-{scene.frames} (frame 1, with graph capture, took {1000 * first:.0f} ms)",
            "ms_per_frame": round(1000.0 * el / (scene.frames - 1), 3),
            "all_positions_finite": bool(__import__("numpy").isfinite(pos).all()),
-           "self_collision": st}
+           "self_collision": dict(st, most_contact_vertices_sampled=most)}
     if args.no_cpu:
         return out
     tris = m.triangles()

@@ -340,8 +340,10 @@ SCEXP int scGetTriContactData(scModel model, int* ids, float* data, int cap);
  * an overflow so far, out[2] / out[3] most vertex-triangle / edge-edge
  * contacts in one substep of the last step, out[4] candidate-pair pool
  * capacity (both kinds, pairs), out[5] / out[6] vertex-triangle / edge-edge colour
- * groups of the last substep. Returns the number of values (7), writing at
- * most count. No counterpart in the reference (its lists are std::vectors). */
+ * groups of the last substep, out[7] / out[8] distinct vertices of the
+ * vertex-triangle / edge-edge contacts of the last substep. Returns the
+ * number of values (9), writing at most count. No counterpart in the
+ * reference (its lists are std::vectors). */
 SCEXP int scGetSelfContacts(scModel model, int kind, unsigned long long* keys, int* ids, float* data, int cap);
 SCEXP void scSetSelfParams(scModel model, float shrink, float threshold);
 SCEXP void scSetSelfCCD(scModel model, float volume, float bisect, float vtDistance);

@@ -640,11 +640,13 @@ class ClothModel:
     def self_stats(self):
         """Contact capacity in use, steps rerun after an overflow, most
         vertex-triangle / edge-edge contacts in one substep of the last step,
-        candidate-pair pool capacity (both kinds), colour groups of the last substep."""
-        out = np.zeros(7, dtype=np.int32)
-        _checked("scGetSelfStats", self.h, _ptr(out), 7)
+        candidate-pair pool capacity (both kinds), colour groups and distinct
+        contact vertices of the last substep."""
+        out = np.zeros(9, dtype=np.int32)
+        _checked("scGetSelfStats", self.h, _ptr(out), 9)
         return dict(capacity=int(out[0]), reruns=int(out[1]), peak_vt=int(out[2]), peak_ee=int(out[3]),
-                    pool=int(out[4]), colours_vt=int(out[5]), colours_ee=int(out[6]))
+                    pool=int(out[4]), colours_vt=int(out[5]), colours_ee=int(out[6]),
+                    vertices_vt=int(out[7]), vertices_ee=int(out[8]))
 
     def enable_kernel_timing(self, on=True):
         _checked("scEnableKernelTiming", self.h, bool(on))

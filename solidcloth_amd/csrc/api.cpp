@@ -1287,11 +1287,11 @@ int scGetSelfStats(scModel model, int* out, int count)
 {
     return guard_r<int>(-1, [&] {
         ClothModel* m = M(model);
-        const int v[7] = {m->gpu.self.cap, m->selfReruns, m->lastSelfPeak[0], m->lastSelfPeak[1],
+        const int v[9] = {m->gpu.self.cap, m->selfReruns, m->lastSelfPeak[0], m->lastSelfPeak[1],
                           m->gpu.self.candCapVT + m->gpu.self.candCapEE, m->lastSelfColours[0],
-                          m->lastSelfColours[1]};
-        for (int k = 0; k < std::min(count, 7); k++) out[k] = v[k];
-        return 7;
+                          m->lastSelfColours[1], m->lastSelfUniq[0], m->lastSelfUniq[1]};
+        for (int k = 0; k < std::min(count, 9); k++) out[k] = v[k];
+        return 9;
     });
 }
 

@@ -10,6 +10,17 @@ namespace sc {
 __device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
 __device__ __forceinline__ float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
+// Workgroup barrier that orders LDS only: workgroups that exchange data
+// through LDS alone (the colours of a slab, the groups of a staged contact
+// pass) may keep global loads and stores in flight across it (a plain
+// __syncthreads waits for every outstanding global access).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Wave-wide maximum of non-negative, NaN-free values (every lane of the wave
 // active), then one atomic per wave. Such floats order as their unsigned bit
 // patterns; the reduction moves lanes with DPP (shifts inside rows of 16,

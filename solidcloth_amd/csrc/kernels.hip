@@ -469,16 +469,6 @@ __device__ __forceinline__ float vertex_contacts(int v, int n, f3& p, const f3 p
 // the previous iteration while staging it — after every constraint of that
 // iteration (earlier launches) and before any constraint of this one, which
 // is where the reference's sequential loop puts them.
-// Workgroup barrier that orders LDS only: the colours of a slab exchange
-// vertices through LDS alone, so the constraint data already requested from
-// global memory for the next colour may stay in flight across it (a plain
-// __syncthreads waits for every outstanding global load).
-__device__ __forceinline__ void lds_barrier()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
 
 // Keeps the loads above it above the early exit below it: without it the
 // compiler tests the exit flag first and only then issues the loads, one

@@ -329,7 +329,7 @@ void GpuSolver::release()
     dev_free(self.candFillVT); dev_free(self.candFillEE);
     self.candCapVT = self.candCapEE = 0;
     self.candWantVT = self.candWantEE = 0;
-    dev_free(self.vloc);
+    dev_free(self.vloc); dev_free(self.triLeafVerts); dev_free(self.edgeLeafVerts);
     for (SelfBuf& b : self.buf) {
         dev_free(b.ids); dev_free(b.W); dev_free(b.N); dev_free(b.key); dev_free(b.colour); dev_free(b.order);
         dev_free(b.active); dev_free(b.groupOff); dev_free(b.lid); dev_free(b.vlist);
@@ -926,6 +926,8 @@ void ClothModel::ensure_self()
         dev_alloc(S.vloc, std::max(g.nv, 1));
         HIP_OK(hipMemset(S.vloc, 0x7f, sizeof(int) * std::max(g.nv, 1)));
         for (SelfBuf& b : S.buf) dev_alloc(b.vlist, std::max(g.nv, 1));
+        dev_alloc(S.triLeafVerts, std::max(g.nt, 1));
+        dev_alloc(S.edgeLeafVerts, std::max(g.ne, 1));
     }
     // hierarchy from the current configuration; substeps refit it (the
     // vertex tree only orders the vertex queries, so it is not refit)
@@ -933,6 +935,8 @@ void ClothModel::ensure_self()
                       S.triLo, S.triHi, S.edgeLo, S.edgeHi, g.nv, S.vtxLo, S.vtxHi);
     S.triB.build(g.stream, S.triLo, S.triHi, g.nt);
     S.edgeB.build(g.stream, S.edgeLo, S.edgeHi, g.ne);
+    launch_leaf_verts(g.stream, S.triB.view(), g.d_cTris, nullptr, S.triLeafVerts);
+    launch_leaf_verts(g.stream, S.edgeB.view(), nullptr, g.d_cEdges, S.edgeLeafVerts);
     S.vtxB.build(g.stream, S.vtxLo, S.vtxHi, g.nv);
     S.built = true;
 }
@@ -1145,7 +1149,7 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
             launch_self_detect(s, g.d_pos, g.d_prev, g.d_cTris, g.d_cEdges, g.nv, g.ne, S.triB.view(),
                                S.edgeB.view(), S.vtxB.view().leafPrim, sp,
                                SelfCandidates{S.candVT, S.candEE, S.candFillVT, S.candFillEE, S.candCapVT,
-                                              S.candCapEE},
+                                              S.candCapEE, S.triLeafVerts, S.edgeLeafVerts},
                                S.buf[0], S.buf[1], g.d_ctl);
             launch_self_colour(s, S.buf[0], 0, S.vmax, S.vloc, g.d_ctl);
             launch_self_colour(s, S.buf[1], 1, S.vmax, S.vloc, g.d_ctl);
@@ -1547,6 +1551,8 @@ bool ClothModel::self_step_overflowed(int attempt)
     lastSelfPeak[1] = (int)peak[1];
     lastSelfColours[0] = c.selfColours[0];
     lastSelfColours[1] = c.selfColours[1];
+    lastSelfUniq[0] = c.selfUniq[0];
+    lastSelfUniq[1] = c.selfUniq[1];
     const long limit = 1L << 30;
     if (peak[2] > S.candCapVT / 5 * 4) S.candWantVT = std::min(limit, peak[2] + peak[2] / 2);
     if (peak[3] > S.candCapEE / 5 * 4) S.candWantEE = std::min(limit, peak[3] + peak[3] / 2);

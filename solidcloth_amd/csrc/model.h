@@ -270,6 +270,7 @@ public:
         SelfBuf buf[2]{};
         unsigned long long* vmax = nullptr;
         int* vloc = nullptr;  // the colouring's per-vertex claims
+        int4 *triLeafVerts = nullptr, *edgeLeafVerts = nullptr;  // vertices per sorted leaf (rebuilt with the trees)
         int2 *candVT = nullptr, *candEE = nullptr;  // candidate pools (chunks of kCandChunk pairs)
         int *candFillVT = nullptr, *candFillEE = nullptr;  // pairs per chunk
         int candCapVT = 0, candCapEE = 0;    // pool capacities in pairs
@@ -355,6 +356,7 @@ public:
     int selfReruns = 0;          // steps rerun after a self-collision overflow
     int lastSelfPeak[2] = {0, 0};  // most vertex-triangle / edge-edge contacts in one substep of the last step
     int lastSelfColours[2] = {0, 0};  // colour groups of each kind in the last substep
+    int lastSelfUniq[2] = {0, 0};     // distinct vertices of each kind's contacts in the last substep
     int mNumIterations = 50, mCollFlags = 0, mBendFreq = 2;
     bool mDihedral = true, mFEM = false, mFixedIterations = false, mUseGraph = true;
 

@@ -151,13 +151,15 @@ __device__ __forceinline__ bool overlap(f3 alo, f3 ahi, float4 blo, float4 bhi)
 // escape links: a node whose box overlaps is entered through its left child,
 // any other node (and every leaf) passes to its escape. No stack, one packed
 // node (one cache line) per step.
+// visit(primitive, vertices) gets the leaf's vertex record (verts, one int4
+// per sorted leaf), read together with the leaf's box.
 template <class Visit>
-__device__ __forceinline__ void bvh_overlap(const BvhView& bv, f3 lo, f3 hi, Visit visit)
+__device__ __forceinline__ void bvh_overlap(const BvhView& bv, const int4* verts, f3 lo, f3 hi, Visit visit)
 {
     if (bv.n <= 0) return;
     if (bv.n == 1) {
         const float4 L = bv.leafLo[0];
-        if (overlap(lo, hi, L, bv.leafHi[0])) visit(__float_as_int(L.w));
+        if (overlap(lo, hi, L, bv.leafHi[0])) visit(__float_as_int(L.w), verts[0]);
         return;
     }
     int node = 0;
@@ -177,7 +179,8 @@ __device__ __forceinline__ void bvh_overlap(const BvhView& bv, f3 lo, f3 hi, Vis
             const int leaf = ~node;
             const float4* F = bv.leaves + 2 * (size_t)leaf;
             const float4 L = F[0], H = F[1];
-            if (overlap(lo, hi, L, H)) visit(__float_as_int(L.w));
+            const int4 V = verts[leaf];
+            if (overlap(lo, hi, L, H)) visit(__float_as_int(L.w), V);
             node = __float_as_int(H.w);
         }
     }
@@ -189,7 +192,8 @@ __device__ __forceinline__ void bvh_overlap(const BvhView& bv, f3 lo, f3 hi, Vis
 // skipped whole. Each unordered pair of one tree's leaves is then found by
 // exactly one of its two leaves' queries.
 template <class Visit>
-__device__ __forceinline__ void bvh_overlap_after(const BvhView& bv, f3 lo, f3 hi, int first, Visit visit)
+__device__ __forceinline__ void bvh_overlap_after(const BvhView& bv, const int4* verts, f3 lo, f3 hi, int first,
+                                                  Visit visit)
 {
     if (bv.n <= 1) return;
     int node = 0;
@@ -211,7 +215,8 @@ __device__ __forceinline__ void bvh_overlap_after(const BvhView& bv, f3 lo, f3 h
             const int leaf = ~node;
             const float4* F = bv.leaves + 2 * (size_t)leaf;
             const float4 L = F[0], H = F[1];
-            if (leaf > first && overlap(lo, hi, L, H)) visit(__float_as_int(L.w));
+            const int4 V = verts[leaf];
+            if (leaf > first && overlap(lo, hi, L, H)) visit(__float_as_int(L.w), V);
             node = __float_as_int(H.w);
         }
     }
@@ -423,8 +428,7 @@ __global__ void __launch_bounds__(kBlock) k_self_candidates(const float4* __rest
             swept(xyz(P), lo, hi);
             lo = f3{lo.x - tol, lo.y - tol, lo.z - tol};
             hi = f3{hi.x + tol, hi.y + tol, hi.z + tol};
-            bvh_overlap(triBvh, lo, hi, [&](int t) {
-                const int4 tv = cTris[t];
+            bvh_overlap(triBvh, cand.triLeafVerts, lo, hi, [&](int t, int4 tv) {
                 if (v == tv.x || v == tv.y || v == tv.z) return;
                 emit(make_int2(t, v));
             });
@@ -440,8 +444,7 @@ __global__ void __launch_bounds__(kBlock) k_self_candidates(const float4* __rest
         lo = f3{lo.x - tol, lo.y - tol, lo.z - tol};
         hi = f3{hi.x + tol, hi.y + tol, hi.z + tol};
         // query j (sorted leaf j) pairs with the overlapping leaves after it
-        bvh_overlap_after(edgeBvh, lo, hi, j, [&](int e2) {
-            const int2 fv = cEdges[e2];
+        bvh_overlap_after(edgeBvh, cand.edgeLeafVerts, lo, hi, j, [&](int e2, int4 fv) {
             if (ev.x == fv.x || ev.y == fv.y || ev.x == fv.y || ev.y == fv.x) return;
             emit(e1 < e2 ? make_int2(e1, e2) : make_int2(e2, e1));
         });
@@ -915,7 +918,12 @@ __global__ void __launch_bounds__(kSoloBlock) k_self_solve(float4* pos, SelfBuf 
                 const float x = project(c, self_rec(b, c, local));
                 err = err < x ? x : err;
             }
-            __syncthreads();
+            // staged vertices pass from group to group through LDS alone (a
+            // contact's own record is read by nothing else in the pass)
+            if (local)
+                lds_barrier();
+            else
+                __syncthreads();
             c1 = c2;
             r1 = r2;
             c2 = c3;
@@ -923,12 +931,35 @@ __global__ void __launch_bounds__(kSoloBlock) k_self_solve(float4* pos, SelfBuf 
         return err;
     };
     const float err = inLds ? sweep([&](int g) { return goff[g]; }) : sweep([&](int g) { return b.groupOff[g]; });
-    if (local)
+    if (local) {
+        __syncthreads();
         for (int k = tid; k < m; k += kSoloBlock) pos[b.vlist[k]] = sv[k];
+    }
     wave_max_atomic(kind == 0 ? &ctl->sct : &ctl->sce, err);
 }
 
+__global__ void __launch_bounds__(kBlock) k_leaf_verts(const int* __restrict__ leafPrim, int n,
+                                                       const int4* __restrict__ cTris,
+                                                       const int2* __restrict__ cEdges, int4* __restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int p = leafPrim[k];
+    if (cTris) {
+        out[k] = cTris[p];
+    } else {
+        const int2 e = cEdges[p];
+        out[k] = make_int4(e.x, e.y, -1, -1);
+    }
+}
+
 static inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
+
+void launch_leaf_verts(hipStream_t s, const BvhView& bv, const int4* cTris, const int2* cEdges, int4* out)
+{
+    if (bv.n <= 0) return;
+    hipLaunchKernelGGL(k_leaf_verts, dim3(blocks(bv.n)), dim3(kBlock), 0, s, bv.leafPrim, bv.n, cTris, cEdges, out);
+}
 
 void launch_self_boxes(hipStream_t s, const float4* pos, const float4* prev, const int4* cTris, int nt,
                        const int2* cEdges, int ne, float tol, float4* triLo, float4* triHi, float4* edgeLo,
