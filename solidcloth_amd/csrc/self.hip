@@ -146,82 +146,6 @@ __device__ __forceinline__ bool overlap(f3 alo, f3 ahi, float4 blo, float4 bhi)
     return !(ahi.x < blo.x || ahi.y < blo.y || ahi.z < blo.z || bhi.x < alo.x || bhi.y < alo.y || bhi.z < alo.z);
 }
 
-// Calls visit(primitive) for every leaf whose box overlaps [lo, hi] (closed
-// intervals, as Geometry/Aabb3.h:72-87), depth-first and left-first along the
-// escape links: a node whose box overlaps is entered through its left child,
-// any other node (and every leaf) passes to its escape. No stack, one packed
-// node (one cache line) per step.
-// visit(primitive, vertices) gets the leaf's vertex record (verts, one int4
-// per sorted leaf), read together with the leaf's box.
-template <class Visit>
-__device__ __forceinline__ void bvh_overlap(const BvhView& bv, const int4* verts, f3 lo, f3 hi, Visit visit)
-{
-    if (bv.n <= 0) return;
-    if (bv.n == 1) {
-        const float4 L = bv.leafLo[0];
-        if (overlap(lo, hi, L, bv.leafHi[0])) visit(__float_as_int(L.w), verts[0]);
-        return;
-    }
-    int node = 0;
-    // a walk visits each of the 2n - 1 nodes at most once; the bound only
-    // guards against malformed links. Steps read the packed copy: one node's
-    // box, child and rope share a cache line.
-    for (int steps = 2 * bv.n; node != kRopeEnd && steps > 0; steps--) {
-        if (node >= 0) {
-            const float4* N = bv.nodes + 4 * (size_t)node;
-            const float4 L = N[0];
-            if (overlap(lo, hi, L, N[1])) {
-                node = __float_as_int(L.w);
-                continue;
-            }
-            node = __float_as_int(N[2].x);
-        } else {
-            const int leaf = ~node;
-            const float4* F = bv.leaves + 2 * (size_t)leaf;
-            const float4 L = F[0], H = F[1];
-            const int4 V = verts[leaf];
-            if (overlap(lo, hi, L, H)) visit(__float_as_int(L.w), V);
-            node = __float_as_int(H.w);
-        }
-    }
-}
-
-// The overlap walk (primitives passed to visit) over the leaves after sorted
-// leaf `first` only: the walk
-// is left-first, so a subtree whose last leaf is at or before `first` is
-// skipped whole. Each unordered pair of one tree's leaves is then found by
-// exactly one of its two leaves' queries.
-template <class Visit>
-__device__ __forceinline__ void bvh_overlap_after(const BvhView& bv, const int4* verts, f3 lo, f3 hi, int first,
-                                                  Visit visit)
-{
-    if (bv.n <= 1) return;
-    int node = 0;
-    for (int steps = 2 * bv.n; node != kRopeEnd && steps > 0; steps--) {
-        if (node >= 0) {
-            const float4* N = bv.nodes + 4 * (size_t)node;
-            const float4 A = N[2];  // rope, first leaf, last leaf
-            if (__float_as_int(A.z) <= first) {
-                node = __float_as_int(A.x);
-                continue;
-            }
-            const float4 L = N[0];
-            if (overlap(lo, hi, L, N[1])) {
-                node = __float_as_int(L.w);
-                continue;
-            }
-            node = __float_as_int(A.x);
-        } else {
-            const int leaf = ~node;
-            const float4* F = bv.leaves + 2 * (size_t)leaf;
-            const float4 L = F[0], H = F[1];
-            const int4 V = verts[leaf];
-            if (leaf > first && overlap(lo, hi, L, H)) visit(__float_as_int(L.w), V);
-            node = __float_as_int(H.w);
-        }
-    }
-}
-
 // ClothCollision.cpp:256-315: prev and pos of the primitive's vertices,
 // extruded by tol (Aabb3::Extrude).
 __device__ __forceinline__ void swept(f3 p, f3& lo, f3& hi)
@@ -346,56 +270,105 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb)
     return start + k;
 }
 
-// Appends pair p for each lane of the wave that calls it at this point of its
-// walk: the calling lanes take consecutive slots of the wave's current chunk
-// of the pool (one ballot), and the leading lane reserves a fresh chunk of
-// kCandChunk slots with one atomic when the current one cannot take them
-// all (its unused tail is left, and its fill count records how many slots
-// hold pairs). The wave's chunk (base, used) lives in LDS, read and written
-// only by the leading lane of each call; the leader changes between calls,
-// so the words are volatile (every access goes to LDS). Chunks are reserved
-// whole, so a chunk lies either inside the pool's capacity or past it; a
-// pair past it sets the overflow.
-__device__ __forceinline__ void wave_emit(int2 p, int2* pool, int* fill, int cap, int* reserved,
-                                          volatile int* wbase, volatile int* wused, SolverCtl* ctl)
-{
-    const unsigned long long act = __ballot(1);
-    const int lane = __lane_id();
-    const int cnt = __popcll(act);
-    const int rank = __popcll(act & ((1ull << lane) - 1ull));
-    const int leader = __ffsll((unsigned long long)act) - 1;
-    int base = 0, used = 0;
-    if (lane == leader) {
-        base = *wbase;
-        used = *wused;
-        if (used + cnt > kCandChunk) {
-            if (base >= 0 && base < cap) fill[base / kCandChunk] = used;
-            base = atomicAdd(reserved, kCandChunk);
-            used = 0;
-            *wbase = base;
-        }
-        *wused = used + cnt;
-    }
-    base = __shfl(base, leader);
-    used = __shfl(used, leader);
-    const int slot = base + used + rank;
-    if (slot < cap) pool[slot] = p;
-    else atomicExch(&ctl->selfOverflow, 1);
-}
-
 // Broad phase (SelfCollisionDetector.cpp:60-135 over the trees
 // ClothModel::ComputeTree builds, ClothCollision.cpp:240-315): every vertex
 // queries the triangle tree with its swept box, every edge the edge tree; a
 // pair whose boxes overlap (closed intervals, as Aabb3::Intersect) is a
 // candidate unless the vertex belongs to the triangle or the edges share a
 // vertex (and, like the reference's pass over leaf pairs, a pinned vertex is
-// never a candidate). Queries run in Morton order (the trees' leaf orders),
-// so that neighbouring threads walk neighbouring parts of a tree. An edge
-// query pairs only with the edge tree's later leaves, so each edge pair is
-// found once, and is written with the lower edge index first. Pairs go to
-// flat pools in chunks (wave_emit): waves write runs of consecutive slots.
-// The continuous tests run in k_self_ccd, so that the traversal's divergence
-// and the tests' FP64 registers do not meet in one kernel.
+// never a candidate). Queries run in Morton order (the trees' leaf orders).
+// An edge query pairs only with the edge tree's later leaves, so each edge
+// pair is found once, and is written with the lower edge index first.
+//
+// The 64 queries of a wave walk the tree together: one node sequence for the
+// wave (depth-first, left-first along the escape links), a node entered when
+// any lane's box overlaps it (and, for edges, any such lane still wants one
+// of its leaves), each lane testing its own box at every leaf the walk
+// reaches. Each lane thus meets exactly the leaves its own walk would, in the
+// same order, while the node and leaf records, one address per wave, come
+// through the scalar cache and the control flow stays uniform. Pairs go to
+// flat pools in chunks of kCandChunk slots: the lanes that find a pair at a
+// leaf take consecutive slots of the wave's chunk (its base and fill are
+// wave-uniform, in scalar registers), and a fresh chunk costs one atomic.
+// The continuous tests run in k_self_ccd, so that the walk and the tests'
+// FP64 registers do not meet in one kernel.
+template <bool kEdges>
+__device__ __forceinline__ void wave_candidates(const BvhView& bv, const int4* __restrict__ verts, bool active, f3 lo,
+                                                f3 hi, int self, int first, int2 me, int2* pool, int* fill, int cap,
+                                                int* reserved, SolverCtl* ctl)
+{
+    const int lane = __lane_id();
+    int cbase = -1, cused = kCandChunk;  // the wave's chunk (no chunk yet)
+    auto take = [&](bool hit, int2 p) {
+        const unsigned long long mask = __ballot(hit);
+        if (!mask) return;
+        const int cnt = __popcll(mask);
+        if (cused + cnt > kCandChunk) {
+            const int lead = __ffsll(mask) - 1;
+            int got = 0;
+            if (lane == lead) {
+                if (cbase >= 0 && cbase < cap) fill[cbase / kCandChunk] = cused;
+                got = atomicAdd(reserved, kCandChunk);
+            }
+            cbase = __builtin_amdgcn_readlane(got, lead);
+            cused = 0;
+        }
+        if (hit) {
+            const int slot = cbase + cused + __popcll(mask & ((1ull << lane) - 1ull));
+            if (slot < cap)
+                pool[slot] = p;
+            else
+                atomicExch(&ctl->selfOverflow, 1);
+        }
+        cused += cnt;
+    };
+    // the tree records, read-only for the kernel: through the constant
+    // address space, a wave-uniform address becomes a scalar load
+    typedef const __attribute__((address_space(4))) float CF;
+    typedef const __attribute__((address_space(4))) int CI;
+    auto ldf4 = [](const float4* p) {
+        CF* f = (CF*)p;
+        return make_float4(f[0], f[1], f[2], f[3]);
+    };
+    auto ldi4 = [](const int4* p) {
+        CI* f = (CI*)p;
+        return make_int4(f[0], f[1], f[2], f[3]);
+    };
+    if (bv.n >= 1 && __ballot(active)) {
+        int node = bv.n == 1 ? ~0 : 0;
+        // a walk visits each of the 2n - 1 nodes at most once; the bound only
+        // guards against malformed links
+        for (int steps = 2 * bv.n; node != kRopeEnd && steps > 0; steps--) {
+            node = __builtin_amdgcn_readfirstlane(node);  // (the same on every lane)
+            if (node >= 0) {
+                const float4* N = bv.nodes + 4 * (size_t)node;
+                // (min, left), (max, right), (rope, first, last)
+                const float4 L = ldf4(N), H = ldf4(N + 1), A = ldf4(N + 2);
+                const bool want = active && (!kEdges || __float_as_int(A.z) > first) && overlap(lo, hi, L, H);
+                node = __ballot(want) ? __float_as_int(L.w) : __float_as_int(A.x);
+            } else {
+                const int leaf = ~node;
+                const float4* F = bv.leaves + 2 * (size_t)leaf;
+                const float4 L = ldf4(F), H = ldf4(F + 1);
+                const int4 V = ldi4(verts + leaf);
+                const int prim = __float_as_int(L.w);
+                bool hit = active && overlap(lo, hi, L, H);
+                int2 p;
+                if (kEdges) {
+                    hit = hit && leaf > first && !(me.x == V.x || me.y == V.y || me.x == V.y || me.y == V.x);
+                    p = self < prim ? make_int2(self, prim) : make_int2(prim, self);
+                } else {
+                    hit = hit && !(self == V.x || self == V.y || self == V.z);
+                    p = make_int2(prim, self);
+                }
+                take(hit, p);
+                node = bv.n == 1 ? kRopeEnd : __float_as_int(H.w);
+            }
+        }
+    }
+    if (lane == 0 && cbase >= 0 && cbase < cap) fill[cbase / kCandChunk] = cused;
+}
+
 __global__ void __launch_bounds__(kBlock) k_self_candidates(const float4* __restrict__ pos,
                                                             const float4* __restrict__ prev,
                                                             const int4* __restrict__ cTris,
@@ -404,56 +377,44 @@ __global__ void __launch_bounds__(kBlock) k_self_candidates(const float4* __rest
                                                             const int* __restrict__ vertexOrder, float tol,
                                                             SelfCandidates cand, SolverCtl* ctl)
 {
-    __shared__ volatile int wbase[kBlock / 64], wused[kBlock / 64];
     const int blk = xcd_logical_block(blockIdx.x, gridDim.x);
-    const bool vertexPass = blk < vertexBlocks;
+    const bool vertexPass = blk < vertexBlocks;  // uniform over the workgroup
     const int j = (vertexPass ? blk : blk - vertexBlocks) * blockDim.x + threadIdx.x;
-    const int w = threadIdx.x >> 6;
-    if (__lane_id() == 0) {
-        wbase[w] = -1;
-        wused[w] = kCandChunk;  // no chunk yet: the first pair reserves one
-    }
-    __builtin_amdgcn_wave_barrier();
-    int2* pool = vertexPass ? cand.vt : cand.ee;
-    int* fill = vertexPass ? cand.fillVT : cand.fillEE;
-    const int cap = vertexPass ? cand.capVT : cand.capEE;
-    int* reserved = &ctl->candSlots[vertexPass ? 0 : 1];
-    auto emit = [&](int2 p) { wave_emit(p, pool, fill, cap, reserved, &wbase[w], &wused[w], ctl); };
-    if (vertexPass && j < nv) {
-        const int v = vertexOrder ? vertexOrder[j] : j;
-        const float4 P = pos[v];
-        if (P.w != 0.f) {
-            f3 lo{1e7f, 1e7f, 1e7f}, hi{-1e7f, -1e7f, -1e7f};
-            swept(xyz(prev[v]), lo, hi);
-            swept(xyz(P), lo, hi);
+    f3 lo{1e7f, 1e7f, 1e7f}, hi{-1e7f, -1e7f, -1e7f};
+    if (vertexPass) {
+        bool active = false;
+        int v = 0;
+        if (j < nv) {
+            v = vertexOrder ? vertexOrder[j] : j;
+            const float4 P = pos[v];
+            if (P.w != 0.f) {
+                active = true;
+                swept(xyz(prev[v]), lo, hi);
+                swept(xyz(P), lo, hi);
+                lo = f3{lo.x - tol, lo.y - tol, lo.z - tol};
+                hi = f3{hi.x + tol, hi.y + tol, hi.z + tol};
+            }
+        }
+        wave_candidates<false>(triBvh, cand.triLeafVerts, active, lo, hi, v, 0, make_int2(0, 0), cand.vt,
+                               cand.fillVT, cand.capVT, &ctl->candSlots[0], ctl);
+    } else {
+        bool active = false;
+        int e1 = 0;
+        int2 ev = make_int2(-1, -1);
+        if (j < ne) {
+            active = true;
+            e1 = edgeBvh.leafPrim[j];
+            ev = cEdges[e1];
+            swept(xyz(prev[ev.x]), lo, hi);
+            swept(xyz(prev[ev.y]), lo, hi);
+            swept(xyz(pos[ev.x]), lo, hi);
+            swept(xyz(pos[ev.y]), lo, hi);
             lo = f3{lo.x - tol, lo.y - tol, lo.z - tol};
             hi = f3{hi.x + tol, hi.y + tol, hi.z + tol};
-            bvh_overlap(triBvh, cand.triLeafVerts, lo, hi, [&](int t, int4 tv) {
-                if (v == tv.x || v == tv.y || v == tv.z) return;
-                emit(make_int2(t, v));
-            });
         }
-    } else if (!vertexPass && j < ne) {
-        const int e1 = edgeBvh.leafPrim[j];
-        const int2 ev = cEdges[e1];
-        f3 lo{1e7f, 1e7f, 1e7f}, hi{-1e7f, -1e7f, -1e7f};
-        swept(xyz(prev[ev.x]), lo, hi);
-        swept(xyz(prev[ev.y]), lo, hi);
-        swept(xyz(pos[ev.x]), lo, hi);
-        swept(xyz(pos[ev.y]), lo, hi);
-        lo = f3{lo.x - tol, lo.y - tol, lo.z - tol};
-        hi = f3{hi.x + tol, hi.y + tol, hi.z + tol};
         // query j (sorted leaf j) pairs with the overlapping leaves after it
-        bvh_overlap_after(edgeBvh, cand.edgeLeafVerts, lo, hi, j, [&](int e2, int4 fv) {
-            if (ev.x == fv.x || ev.y == fv.y || ev.x == fv.y || ev.y == fv.x) return;
-            emit(e1 < e2 ? make_int2(e1, e2) : make_int2(e2, e1));
-        });
-    }
-    // every lane of the wave is back here: record the fill of its last chunk
-    __builtin_amdgcn_wave_barrier();
-    if (__lane_id() == 0) {
-        const int b = wbase[w];
-        if (b >= 0 && b < cap) fill[b / kCandChunk] = wused[w];
+        wave_candidates<true>(edgeBvh, cand.edgeLeafVerts, active, lo, hi, e1, j, ev, cand.ee, cand.fillEE,
+                              cand.capEE, &ctl->candSlots[1], ctl);
     }
 }
 
