@@ -146,6 +146,13 @@ __device__ __forceinline__ bool overlap(f3 alo, f3 ahi, float4 blo, float4 bhi)
     return !(ahi.x < blo.x || ahi.y < blo.y || ahi.z < blo.z || bhi.x < alo.x || bhi.y < alo.y || bhi.z < alo.z);
 }
 
+// overlap without short-circuit evaluation (no branches on the way)
+__device__ __forceinline__ bool overlap_nb(f3 alo, f3 ahi, float4 blo, float4 bhi)
+{
+    return !((ahi.x < blo.x) | (ahi.y < blo.y) | (ahi.z < blo.z) | (bhi.x < alo.x) | (bhi.y < alo.y) |
+             (bhi.z < alo.z));
+}
+
 // ClothCollision.cpp:256-315: prev and pos of the primitive's vertices,
 // extruded by tol (Aabb3::Extrude).
 __device__ __forceinline__ void swept(f3 p, f3& lo, f3& hi)
@@ -344,25 +351,32 @@ __device__ __forceinline__ void wave_candidates(const BvhView& bv, const int4* _
                 const float4* N = bv.nodes + 4 * (size_t)node;
                 // (min, left), (max, right), (rope, first, last)
                 const float4 L = ldf4(N), H = ldf4(N + 1), A = ldf4(N + 2);
-                const bool want = active && (!kEdges || __float_as_int(A.z) > first) && overlap(lo, hi, L, H);
-                node = __ballot(want) ? __float_as_int(L.w) : __float_as_int(A.x);
+                // (no short-circuit: the three records are read together)
+                const bool want = active & (!kEdges | (__float_as_int(A.z) > first)) & overlap_nb(lo, hi, L, H);
+                // both successors in hand with the box (else the compiler reads
+                // the chosen one after the vote: a second wait per step)
+                int left = __float_as_int(L.w), rope = __float_as_int(A.x);
+                asm volatile("" : "+s"(left), "+s"(rope));
+                node = __ballot(want) ? left : rope;
             } else {
                 const int leaf = ~node;
                 const float4* F = bv.leaves + 2 * (size_t)leaf;
                 const float4 L = ldf4(F), H = ldf4(F + 1);
                 const int4 V = ldi4(verts + leaf);
                 const int prim = __float_as_int(L.w);
-                bool hit = active && overlap(lo, hi, L, H);
+                bool hit = active & overlap_nb(lo, hi, L, H);
                 int2 p;
                 if (kEdges) {
-                    hit = hit && leaf > first && !(me.x == V.x || me.y == V.y || me.x == V.y || me.y == V.x);
+                    hit = hit & (leaf > first) & !((me.x == V.x) | (me.y == V.y) | (me.x == V.y) | (me.y == V.x));
                     p = self < prim ? make_int2(self, prim) : make_int2(prim, self);
                 } else {
-                    hit = hit && !(self == V.x || self == V.y || self == V.z);
+                    hit = hit & !((self == V.x) | (self == V.y) | (self == V.z));
                     p = make_int2(prim, self);
                 }
+                int next = __float_as_int(H.w);
+                asm volatile("" : "+s"(next));
                 take(hit, p);
-                node = bv.n == 1 ? kRopeEnd : __float_as_int(H.w);
+                node = bv.n == 1 ? kRopeEnd : next;
             }
         }
     }
