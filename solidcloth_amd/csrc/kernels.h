@@ -156,6 +156,10 @@ struct SlabContacts {
     float4* vel;
     float thickness, mu, invH;
     int enabled;
+    // error maxima this launch adds to the solver's accumulators: bit 0 the
+    // vertex-contact error, bit 1 the bend error (a fixed iteration count
+    // reports only the last passes', so earlier passes skip the atomics)
+    int report = 3;
 };
 
 // Packed sweep schedules (one phase of a slab or tile schedule), laid out so

@@ -984,7 +984,7 @@ __global__ void __launch_bounds__(kSlabBlock) k_fem_packed(float4* pos, PackedFe
     if (kContacts) {
         float err = 0.f;
         if (own && cp.cnt > 0) err = packed_vertex_contacts(g0, cp, sv, tid, kc);
-        wave_max_atomic(&ctl->cv, err);
+        if (kc.report & 1) wave_max_atomic(&ctl->cv, err);
     }
     if (own) pos[g0] = sv[tid];
 }
@@ -1078,12 +1078,12 @@ __global__ void __launch_bounds__(kSlabBlock) k_bend_packed(float4* pos, PackedB
                 cerr = cerr < e ? e : cerr;
             }
         }
-        wave_max_atomic(&ctl->cv, cerr);
+        if (kc.report & 1) wave_max_atomic(&ctl->cv, cerr);
     }
 #pragma unroll
     for (int u = 0; u < KV; u++)
         if (!((unsigned)raw[u] & kPackPad)) pos[raw[u] & kPackIdMask] = sv[tid + u * kSlabBlock];
-    wave_max_atomic(&ctl->cb, err);
+    if (kc.report & 2) wave_max_atomic(&ctl->cb, err);
 }
 
 // Physics/ClothCollisionHandler.cpp:134-202 — vertex contacts with friction.

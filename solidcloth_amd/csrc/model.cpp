@@ -1194,6 +1194,10 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
     // with a fixed count, the reported errors are the last passes' (the exit
     // test's kernel clears them per iteration otherwise)
     const int lastBend = mBendFreq > 0 ? ((iterations - 1) / mBendFreq) * mBendFreq : 0;
+    // accumulators that launches of this substep may have written since the
+    // substep cleared them (bit 0: the per-iteration errors, bit 1: the bend
+    // error); a fixed count clears only what earlier passes wrote
+    int dirty = 0;
     if (mSolver != SOLVER_GAUSS_SEIDEL) {
         // ClothPBD.cpp:290-298: Jacobi or conjugate-residual passes over the
         // links only (no triangles, bends or contacts), a fixed count
@@ -1214,8 +1218,10 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
     }
     for (int k = 0; k < (mSolver == SOLVER_GAUSS_SEIDEL ? iterations : 0); k++) {
         if (mFixedIterations) {
-            const int mask = (k == iterations - 1 && k > 0 ? 1 : 0) | (mDihedral && k == lastBend && k > 0 ? 2 : 0);
+            const int mask =
+                ((k == iterations - 1 && k > 0 ? 1 : 0) | (mDihedral && k == lastBend && k > 0 ? 2 : 0)) & dirty;
             if (mask) launch_reset_errors(s, g.d_ctl, mask);
+            dirty &= ~mask;
         }
         if (!mFEM) {
             for (int p = 0; p < linkCol.numPhases(); p++) {
@@ -1226,14 +1232,19 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                                (int)linkCol.order.size()};
                     launch_link_slab(s, g.d_pos, g.linkSlabs.lidx, g.d_lr, ss, ns, linkCol.maxSlabVerts, linkK,
                                      g.d_ctl);
+                    dirty |= 1;
                 } else {
                     const int st = linkCol.phaseOffsets[p], cnt = linkCol.phaseOffsets[p + 1] - st;
                     launch_links(s, g.d_pos, g.d_lk, g.d_lr, st, cnt, linkK, g.d_ctl);
+                    dirty |= 1;
                 }
                 timed_end(g.timer, timed, 0, s, ev);
             }
         }
         const bool bendPass = mDihedral && k % mBendFreq == 0;
+        // the error maxima that survive to the substep's report (a fixed
+        // count clears them before the last pass of each kind)
+        const int report = !mFixedIterations ? 3 : (k == iterations - 1 ? 1 : 0) | (k >= lastBend ? 2 : 0);
         const bool contactsInFem = fuseTiles && !bendPass;
         // with a bend pass, each vertex's last bend slab applies its contacts
         const bool contactsInBends = fuseTiles && bendPass && bendCol.slabbed && g.bendOrphans == 0;
@@ -1245,16 +1256,18 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                     const int positions = triCol.tiled ? (int)triCol.instCons.size() : (int)triCol.order.size();
                     SlabSet ss{g.triSlabs.info, g.triSlabs.colOff, g.triSlabs.first, g.triSlabs.verts, b, positions};
                     SlabContacts kc{cb, g.d_prev, g.d_vel, mThickness, mFriction, invH,
-                                    (fuse && k > 0) ? 1 : contactsInFem ? 2 : 0};
+                                    (fuse && k > 0) ? 1 : contactsInFem ? 2 : 0, report};
                     const TriStreams fs = triCol.tiled ? TriStreams{g.d_tidx, g.d_ttA, g.d_ttB, g.d_ttC} : tr;
                     if ((int)g.triPacked.phases.size() == triCol.numPhases() && kc.enabled != 1) {
                         const DevicePacked::Phase& ph = g.triPacked.phases[p];
                         const PackedFem pf{g.triPacked.verts + ph.vertOff, g.triPacked.s0 + ph.slotOff,
                                            g.triPacked.s1 + ph.slotOff, ph.nc};
                         launch_fem_packed(s, g.d_pos, pf, ph.slabs, ph.ncKernel, ph.kv, h2, epsUV, g.d_ctl, kc);
+                        if (kc.enabled == 2) dirty |= report & 1;
                     } else {
                         launch_fem_slab(s, g.d_pos, fs, g.triSlabs.lidx, ss, ns, triCol.maxSlabVerts, h2, epsUV,
                                         g.d_ctl, kc, triCol.tiled);
+                        if (kc.enabled) dirty |= 1;
                     }
                 } else {
                     int st = triCol.phaseOffsets[p], cnt = triCol.phaseOffsets[p + 1] - st;
@@ -1270,19 +1283,23 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                     const int b = bendCol.phaseSlabBegin[p], ns = bendCol.phaseSlabBegin[p + 1] - b;
                     SlabSet ss{g.bendSlabs.info, g.bendSlabs.colOff, g.bendSlabs.first, g.bendSlabs.verts, b,
                                (int)bendCol.order.size()};
-                    SlabContacts kc{cb, g.d_prev, g.d_vel, mThickness, mFriction, invH, contactsInBends ? 3 : 0};
+                    SlabContacts kc{cb, g.d_prev, g.d_vel, mThickness, mFriction, invH, contactsInBends ? 3 : 0,
+                                    report};
                     if ((int)g.bendPacked.phases.size() == bendCol.numPhases()) {
                         const DevicePacked::Phase& ph = g.bendPacked.phases[p];
                         const PackedBend pb{g.bendPacked.verts + ph.vertOff, g.bendPacked.ids + ph.slotOff, ph.nc,
                                             ph.width};
                         launch_bend_packed(s, g.d_pos, pb, ph.slabs, ph.kv, sStiff, g.d_ctl, kc);
+                        dirty |= (report & 2) | (kc.enabled == 3 ? report & 1 : 0);
                     } else {
                         launch_bend_slab(s, g.d_pos, g.bendSlabs.lidx, ss, ns, bendCol.maxSlabVerts, sStiff, g.d_ctl,
                                          kc);
+                        dirty |= kc.enabled == 3 ? 3 : 2;
                     }
                 } else {
                     int st = bendCol.phaseOffsets[p], cnt = bendCol.phaseOffsets[p + 1] - st;
                     launch_bend(s, g.d_pos, g.d_bidx, st, cnt, sStiff, g.d_ctl);
+                    dirty |= 2;
                 }
                 timed_end(g.timer, timed, 1, s, ev);
             }
@@ -1290,24 +1307,28 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
         if (anyContacts && !fuse && !contactsInFem && !contactsInBends) {
             timed_begin(g.timer, timed, 2, s, ev);
             launch_contacts(s, g.d_pos, g.d_prev, g.d_vel, cb, mThickness, mFriction, invH, g.d_ctl);
+            dirty |= 1;
             timed_end(g.timer, timed, 2, s, ev);
         }
         if (edgeContacts) {
             timed_begin(g.timer, timed, 2, s, ev);
             launch_edge_contacts(s, g.d_pos, g.d_prev, g.d_cEdges, ecb, g.ec.base.data(), g.d_geo, mThickness,
                                  mFriction, g.d_ctl);
+            dirty |= 1;
             timed_end(g.timer, timed, 2, s, ev);
         }
         if (triContacts) {
             timed_begin(g.timer, timed, 2, s, ev);
             launch_tri_contacts(s, g.d_pos, g.d_prev, g.d_cTris, tcb, g.tc.base.data(), g.d_geo, mThickness,
                                 mFriction, g.d_ctl);
+            dirty |= 1;
             timed_end(g.timer, timed, 2, s, ev);
         }
         if (selfColl) {
             timed_begin(g.timer, timed, 2, s, ev);
             launch_self_solve(s, g.d_pos, g.self.buf[0], 0, sp, h, g.d_ctl);
             launch_self_solve(s, g.d_pos, g.self.buf[1], 1, sp, h, g.d_ctl);
+            dirty |= 1;
             timed_end(g.timer, timed, 2, s, ev);
         }
         if (!mFixedIterations)
