@@ -963,7 +963,9 @@ __global__ void __launch_bounds__(kSlabBlock) k_fem_packed(float4* pos, PackedFe
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         if (c + kFemAhead < NC) f[c + kFemAhead] = fem_slot(pk, sb + (c + kFemAhead) * kSlabBlock);
-        if (kContacts && c + kFemAhead == NC && cp.cnt > 0) contact_pre(g0, cp, kc);
+        // every lane requests (vertex 0's state where there is no contact): a
+        // load under a branch makes the colour steps after it wait for it
+        if (kContacts && c + kFemAhead == NC) contact_pre(cp.cnt > 0 ? g0 : 0, cp, kc);
         issue_fence();
         // (a branch per colour keeps each colour's arithmetic, and the wait
         // for its data, in its own block)

@@ -352,7 +352,7 @@ __device__ __forceinline__ void wave_candidates(const BvhView& bv, const int4* _
                 // (min, left), (max, right), (rope, first, last)
                 const float4 L = ldf4(N), H = ldf4(N + 1), A = ldf4(N + 2);
                 // (no short-circuit: the three records are read together)
-                const bool want = active & (!kEdges | (__float_as_int(A.z) > first)) & overlap_nb(lo, hi, L, H);
+                const bool want = (int)active & (int)(!kEdges | (__float_as_int(A.z) > first)) & (int)overlap_nb(lo, hi, L, H);
                 // both successors in hand with the box (else the compiler reads
                 // the chosen one after the vote: a second wait per step)
                 int left = __float_as_int(L.w), rope = __float_as_int(A.x);
