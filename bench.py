@@ -192,7 +192,7 @@ def cpu_baseline_batch(scenes, budget_s, cpu):
             "cpu_model": cpu["model"], "logical_cpus": cpu["logical_cpus"], "available_cpus": cpu["available_cpus"]}
 
 
-def run_gpu(scene, steps, warmup, use_graph=True, sweep=1):
+def run_gpu(scene, steps, warmup, use_graph=True, sweep=1, slab_size=0):
     """The timed run: `warmup` frames, then `steps` frames under the clock."""
     import solidcloth_amd as sc
     from solidcloth_amd import scenes as S
@@ -200,7 +200,7 @@ def run_gpu(scene, steps, warmup, use_graph=True, sweep=1):
     world, patch = S.build_patch(scene, sc)
     m = patch.GetModel()
     m.SetUseGraph(use_graph)
-    m.SetSweep(sweep)
+    m.SetSweep(sweep, slab_size)
     for _ in range(warmup):
         patch.Step(scene.dt)
     t0 = time.perf_counter()
@@ -210,7 +210,7 @@ def run_gpu(scene, steps, warmup, use_graph=True, sweep=1):
     return t1 - t0, m.stats(), (world, patch, m)
 
 
-def run_events(scene, frames, sweep=1):
+def run_events(scene, frames, sweep=1, slab_size=0):
     """The same frames again from a fresh scene (the trajectory is bit-identical),
     every kernel launch bracketed by HIP events on the solver's stream."""
     import solidcloth_amd as sc
@@ -218,7 +218,7 @@ def run_events(scene, frames, sweep=1):
 
     world, patch = S.build_patch(scene, sc)
     m = patch.GetModel()
-    m.SetSweep(sweep)
+    m.SetSweep(sweep, slab_size)
     m.enable_kernel_timing(True)
     for _ in range(frames):
         patch.Step(scene.dt)
@@ -560,7 +560,7 @@ def mesh512_record(args, cpu):
     world, patch = S.build_patch(scene, sc)
     m = patch.GetModel()
     m.SetUseGraph(not args.no_graph)
-    m.SetSweep(args.sweep)
+    m.SetSweep(args.sweep, args.slab_size)
     pos0 = m.positions().copy()
     t0 = time.perf_counter()
     patch.Step(scene.dt)
@@ -617,6 +617,9 @@ def main():
                     help="0: one launch per colour; 1: slab phases staged in LDS; 3: triangles on tiles with "
                          "halos (one launch per FEM pass), bends on slabs; 4: the schedules of 3 with each "
                          "launch's data requested before its first colour")
+    ap.add_argument("--slab-size", type=int, default=0,
+                    help="slab size of the sweep (modes 3 and 4: the interiors of the bend slabs); 0: the "
+                         "library's choice")
     ap.add_argument("--copy-probe-mb", type=int, default=1024,
                     help="size of the streaming-copy bandwidth probe (0 disables)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
@@ -646,10 +649,11 @@ def main():
                           "mesh512": rec}))
         return
     scene = S.DRAPE256
-    elapsed, stats, keep = run_gpu(scene, args.steps, args.warmup, use_graph=not args.no_graph, sweep=args.sweep)
+    elapsed, stats, keep = run_gpu(scene, args.steps, args.warmup, use_graph=not args.no_graph, sweep=args.sweep,
+                                     slab_size=args.slab_size)
     m = keep[2]
     event_frames = args.warmup + args.steps
-    timing = run_events(scene, event_frames, sweep=args.sweep)
+    timing = run_events(scene, event_frames, sweep=args.sweep, slab_size=args.slab_size)
     substeps = args.steps * scene.substeps
     value = substeps / elapsed
     T = scene.num_triangles
@@ -696,6 +700,7 @@ def main():
                       4: "FEM tiles with halos, bend slab phases (LDS), packed: all of a launch's data requested "
                          "before its first colour"}[args.sweep],
             "sweep_mode": args.sweep,
+            "slab_size": args.slab_size,
             "launch_phases_per_pass": {"fem": len(m.colour_offsets("tri")) - 1,
                                        "bend": len(m.colour_offsets("bend")) - 1},
         },

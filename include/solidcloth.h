@@ -294,7 +294,9 @@ SCEXP int scGetSolver(scModel model);
  * for its triangles). Mode 4 (the default) = the schedules of mode 3, each
  * laid out so that a launch requests all of its data before its first colour
  * (where a schedule fits those kernels' limits, else as in mode 3); same
- * order and results as mode 3. Changes the Gauss-Seidel order. */
+ * order and results as mode 3. In modes 3 and 4 a slabSize above 0 sets
+ * the interiors of the bend and link slabs (interiors and seams, as in mode
+ * 2), 0 lets the library choose. Changes the Gauss-Seidel order. */
 SCEXP void scSetSweep(scModel model, int mode, int slabSize);
 
 /* Contacts of the last substep against mesh obstacles, in the reference's

@@ -509,7 +509,9 @@ class ClothModel:
         slab_size constraints, then the seams between them; mode 3: FEM
         triangles on tiles with halos, bends on slab phases; mode 4 (the
         default): mode 3's schedules with every launch's data requested
-        before its first colour. slab_size 0: the library's choice."""
+        before its first colour; in modes 3 and 4 slab_size sets the
+        interiors of the bend and link slabs. slab_size 0: the library's
+        choice."""
         _checked("scSetSweep", self.h, int(mode), int(slab_size))
 
     def GetNumContacts(self):

@@ -781,7 +781,7 @@ void ClothModel::ensure_colouring()
         triCol = schedule_colours(nt, 3, tv.data(), nv);
         bendCol = schedule_colours(nb, 4, bv.data(), nv);
         linkCol = schedule_colours(nl, 2, lv.data(), nv);
-    } else if (mSlabSize > 0) {
+    } else if (mSlabSize > 0 && mSweepMode != SWEEP_TILES && mSweepMode != SWEEP_PACKED) {
         const bool seamed = mSweepMode == SWEEP_SEAMS;
         triCol = schedule_slabs(nt, 3, tv.data(), nv, pos.data(), mSlabSize, maxVerts, seamed);
         bendCol = schedule_slabs(nb, 4, bv.data(), nv, pos.data(), mSlabSize, maxVerts, seamed);
@@ -798,9 +798,15 @@ void ClothModel::ensure_colouring()
                 tiled = false;
             }
         }
-        if (!tiled) triCol = schedule_slabs_auto(nt, 3, tv.data(), nv, pos.data(), maxVerts, kSlabBlock);
-        bendCol = schedule_slabs_auto(nb, 4, bv.data(), nv, pos.data(), maxVerts, kSlabBlock);
-        linkCol = schedule_slabs_auto(nl, 2, lv.data(), nv, pos.data(), maxVerts, kSlabBlock);
+        // a slab size given with the tiled modes sets the interiors of the
+        // other kinds' interior and seam slabs
+        auto slabs = [&](int n, int arity, const uint32_t* v) {
+            return mSlabSize > 0 ? schedule_slabs(n, arity, v, nv, pos.data(), mSlabSize, maxVerts, true)
+                                 : schedule_slabs_auto(n, arity, v, nv, pos.data(), maxVerts, kSlabBlock);
+        };
+        if (!tiled) triCol = slabs(nt, 3, tv.data());
+        bendCol = slabs(nb, 4, bv.data());
+        linkCol = slabs(nl, 2, lv.data());
     }
     mColourDirty = false;
 }

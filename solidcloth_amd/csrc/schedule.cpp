@@ -140,9 +140,42 @@ std::vector<std::vector<int>> seamed_slabs(int n, int arity, const uint32_t* ver
         widest = std::max(widest, (long)(hi - lo));
     }
     const long stripVerts = std::max(widest, (long)slabSize * nv / std::max(1, n));
+    // strips of stripVerts vertices in axis order; with `snap`, each cut is
+    // moved back to the first vertex of its coordinate, so that vertices of
+    // one coordinate (a grid's row) share a strip
     std::vector<int> strip(nv);
-    for (int k = 0; k < nv; k++) strip[byCoord[k]] = (int)(k / stripVerts);
-    const int numStrips = (int)((nv + stripVerts - 1) / stripVerts);
+    int numStrips = 0;
+    auto cut_strips = [&](bool snap) {
+        numStrips = 0;
+        for (long begin = 0; begin < nv;) {
+            long end = std::min<long>(begin + stripVerts, nv);
+            if (snap && end < nv) {
+                long cut = end;
+                while (cut > begin && axis_coord(rest[byCoord[cut - 1]], axis) == axis_coord(rest[byCoord[cut]], axis))
+                    cut--;
+                if (cut > begin) end = cut;
+            }
+            for (long k = begin; k < end; k++) strip[byCoord[k]] = numStrips;
+            numStrips++;
+            begin = end;
+        }
+    };
+    auto spans_three = [&]() {
+        for (int c = 0; c < n; c++) {
+            int lo = 1 << 30, hi = -1;
+            for (int a = 0; a < arity; a++) {
+                const int st = strip[verts[(size_t)c * arity + a]];
+                lo = std::min(lo, st);
+                hi = std::max(hi, st);
+            }
+            if (hi - lo > 1) return true;
+        }
+        return false;
+    };
+    // (snapped strips can be narrower than the widest constraint; then the
+    // plain cuts are kept)
+    cut_strips(true);
+    if (spans_three()) cut_strips(false);
     std::vector<std::vector<int>> slabs(2 * (size_t)numStrips);
     for (int c = 0; c < n; c++) {
         int lo = 1 << 30, hi = -1;
@@ -422,10 +455,14 @@ Schedule schedule_tiles(int n, int arity, const uint32_t* verts, int nv, const f
 
 int schedule_cost(const Schedule& s, int threads)
 {
-    // A sweep step (one local colour of the phase's slowest slab) costs about
-    // one projection's latency whatever its width, plus one more per extra
-    // constraint a thread takes when the colour is wider than the workgroup;
-    // staging and writing back a phase costs about three steps.
+    // In tenths of a sweep step. A step (one local colour of the phase's
+    // slowest slab) costs about one projection's latency whatever its width,
+    // plus one more per extra constraint a thread takes when the colour is
+    // wider than the workgroup; a bend colour that needs more than half the
+    // workgroup cannot split each bend over a pair of lanes and costs about
+    // 1.6 steps (measured on the 256^2 drape: 15 such colours in one launch
+    // took longer than 5 + 9 + 9 paired ones in three). Staging and writing
+    // back a phase costs about three steps.
     int cost = 0;
     for (int p = 0; p < s.numPhases(); p++) {
         int worst = 0;
@@ -433,11 +470,12 @@ int schedule_cost(const Schedule& s, int threads)
             int steps = 0;
             for (int c = s.slabColBegin[sl]; c < s.slabColBegin[sl + 1]; c++) {
                 const int width = s.colOffsets[c + 1] - s.colOffsets[c];
-                steps += (width + threads - 1) / threads;
+                const int step = s.arity == 4 && width > threads / 2 ? 16 : 10;
+                steps += step * ((width + threads - 1) / threads);
             }
             worst = std::max(worst, steps);
         }
-        cost += worst + 3;
+        cost += worst + 30;
     }
     return cost;
 }
