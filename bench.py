@@ -104,6 +104,7 @@ def run_batch(args):
     avg_us = 1000.0 * dev_ms / max(1, launches)
     achieved = bytes_launch / (avg_us * 1e-6) / 1e9
     bs, stage, tuned = batch.GetLaunch()
+    balance = batch_load_balance(batch, dt)
     out = {
         "metric": "PBD substeps/sec, batch512 (every substep advances all 512 cloths)",
         "value": round(value, 2),
@@ -122,6 +123,7 @@ def run_batch(args):
                    "cloths": len(scenes), "vertices": V, "triangles": T, "bends": B,
                    "launch": {"block_size": bs, "lds_staging": bool(stage), "autotuned": tuned},
                    "tuning_ms_per_step": [[b, s_, round(ms, 3)] for b, s_, ms in batch.tuning()]},
+        "load_balance": balance,
         "cloth_substeps_per_s": round(value * len(scenes), 1),
         "constraint_projections_per_s": round(proj * value, 1),
         "vertex_contacts_last_substep": contacts,
@@ -141,6 +143,39 @@ def run_batch(args):
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_batch(scenes, args.cpu_budget, host_cpu())
     print(json.dumps(out))
+
+
+def batch_load_balance(batch, dt):
+    """One more (untimed) step with per-workgroup clocks: how the last
+    substep launch's cloths filled the GPU. busy_fraction = the workgroups'
+    summed run time over (launch span x the most workgroups seen running at
+    once); the launch order is largest cloth first."""
+    import numpy as np
+
+    batch.RecordWorkgroupTimes(True)
+    batch.Step(dt)
+    wt = batch.workgroup_times()
+    batch.RecordWorkgroupTimes(False)
+    st, en = wt["start_us"], wt["end_us"]
+    dur = en - st
+    span = float(en.max() - st.min())
+    ev = sorted([(t, 1) for t in st] + [(t, -1) for t in en], key=lambda e: (e[0], e[1]))
+    running = concurrent = 0
+    for _, d in ev:
+        running += d
+        concurrent = max(concurrent, running)
+    units = len({tuple(u) for u in wt["unit"].tolist()})
+    return {
+        "cloths": int(len(st)),
+        "units_used": units,
+        "max_concurrent_workgroups": concurrent,
+        "span_us": round(span, 2),
+        "busy_fraction": round(float(dur.sum()) / (span * concurrent), 4),
+        "cloth_us": {"min": round(float(dur.min()), 2), "mean": round(float(dur.mean()), 2),
+                     "max": round(float(dur.max()), 2)},
+        "last_start_us": round(float(st.max() - st.min()), 2),
+        "note": "last substep launch of one extra step, per-workgroup s_memrealtime clocks",
+    }
 
 
 def cpu_baseline_batch(scenes, budget_s, cpu):

@@ -423,6 +423,17 @@ SCEXP int scBatchGetTuning(scClothBatch batch, int* configs, double* ms, int cap
 SCEXP double scBatchGetStepTime(scClothBatch batch, int* launches);
 /* Replay the captured substep launches (default) or launch them directly. */
 SCEXP void scBatchSetUseGraph(scClothBatch batch, bool enable);
+/* Per-cloth load balance of the batch kernel (no counterpart in the
+ * reference): with enable, every step records, for its last substep launch,
+ * when each cloth's workgroup started and ended on the device's
+ * constant-rate clock and the unit it ran on. scBatchGetWorkgroupTimes
+ * writes, per patch in the order added (cap entries at most; any pointer may
+ * be null), start and end in microseconds from the launch's earliest start
+ * and the unit as XCC_ID << 32 | HW_ID (the shader engine, array and compute
+ * unit are HW_ID bits 13-15, 12 and 8-11); returns the number of patches
+ * recorded (0 before a recorded step), or -1 on error. */
+SCEXP void scBatchRecordWorkgroupTimes(scClothBatch batch, bool enable);
+SCEXP int scBatchGetWorkgroupTimes(scClothBatch batch, double* startUs, double* endUs, long long* unit, int cap);
 
 /* Learned warm start (no counterpart in the reference). A per-vertex MLP
  * (12 one-ring features -> 64 -> 64 -> 3, GELU, BF16 MFMA with FP32

@@ -136,7 +136,8 @@ def _sig(L):
         "scBatchSetLaunch": (None, [H, i, i]),
         "scBatchGetLaunch": (None, [H, C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
         "scBatchGetTuning": (i, [H, vp, vp, i]), "scBatchGetStepTime": (C.c_double, [H, C.POINTER(i)]),
-        "scBatchSetUseGraph": (None, [H, b]),
+        "scBatchSetUseGraph": (None, [H, b]), "scBatchRecordWorkgroupTimes": (None, [H, b]),
+        "scBatchGetWorkgroupTimes": (i, [H, vp, vp, vp, i]),
         "scGetLastError": (C.c_char_p, []),
         "scGetErrorCount": (i, []), "scGetBuildInfo": (C.c_char_p, []),
     }
@@ -756,6 +757,24 @@ class ClothBatch:
 
     def SetUseGraph(self, enable):
         _checked("scBatchSetUseGraph", self.h, bool(enable))
+
+    def RecordWorkgroupTimes(self, enable):
+        _checked("scBatchRecordWorkgroupTimes", self.h, bool(enable))
+
+    def workgroup_times(self):
+        """Per patch, for the last recorded substep launch: start and end in
+        microseconds from its earliest start, and the unit the cloth's
+        workgroup ran on as (XCC, shader engine, array, compute unit); None
+        before a recorded step."""
+        n = _checked("scBatchGetWorkgroupTimes", self.h, None, None, None, 0)
+        if n == 0:
+            return None
+        st, en = np.zeros(n), np.zeros(n)
+        raw = np.zeros(n, dtype=np.int64)
+        _checked("scBatchGetWorkgroupTimes", self.h, _ptr(st), _ptr(en), _ptr(raw), n)
+        hw = raw & 0xffffffff
+        unit = np.stack([raw >> 32, (hw >> 13) & 7, (hw >> 12) & 1, (hw >> 8) & 15], axis=1)
+        return dict(start_us=st, end_us=en, unit=unit)
 
     def last_step_time(self):
         """(device ms of the last step's substep launches, launch count)."""

@@ -1044,6 +1044,32 @@ void scRestoreState(scModel model) { guard([&] { M(model)->restore_state(); }); 
 
 void scBatchSetUseGraph(scClothBatch batch, bool enable) { guard([&] { B(batch)->useGraph = enable; }); }
 
+void scBatchRecordWorkgroupTimes(scClothBatch batch, bool enable)
+{
+    guard([&] { B(batch)->recordClock = enable; });
+}
+
+int scBatchGetWorkgroupTimes(scClothBatch batch, double* startUs, double* endUs, long long* unit, int cap)
+{
+    return guard_r<int>(-1, [&] {
+        ClothBatch* b = B(batch);
+        const int n = (int)b->clockStart.size();
+        if (n == 0) return 0;
+        int dev = 0, kHz = 0;
+        HIP_OK(hipGetDevice(&dev));
+        HIP_OK(hipDeviceGetAttribute(&kHz, hipDeviceAttributeWallClockRate, dev));
+        if (kHz <= 0) throw std::runtime_error("device reports no wall clock rate");
+        const unsigned long long t0 = *std::min_element(b->clockStart.begin(), b->clockStart.end());
+        const double usPerTick = 1000.0 / kHz;
+        for (int k = 0; k < std::min(n, cap); k++) {
+            if (startUs) startUs[k] = (double)(b->clockStart[k] - t0) * usPerTick;
+            if (endUs) endUs[k] = (double)(b->clockEnd[k] - t0) * usPerTick;
+            if (unit) unit[k] = (long long)b->clockUnit[k];
+        }
+        return n;
+    });
+}
+
 double scBatchGetStepTime(scClothBatch batch, int* launches)
 {
     return guard_r<double>(-1.0, [&] {

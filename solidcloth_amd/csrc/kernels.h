@@ -333,7 +333,10 @@ void launch_render_mesh(hipStream_t s, const float4* pos, const int* map, int nv
 // Batch: one workgroup per cloth (cloths in launch order), one launch per
 // substep. blockSize 256, 512 or 1024; cloths with at most stageVerts
 // vertices keep their positions in LDS during the iterations.
-void launch_batch_substep(hipStream_t s, const ClothDesc* descs, int numCloths, int blockSize, int stageVerts);
+// wgClock (optional, 3 words per cloth): each workgroup's start and end on the
+// device's constant-rate clock and the unit it ran on (XCC id << 32 | HW_ID).
+void launch_batch_substep(hipStream_t s, const ClothDesc* descs, int numCloths, int blockSize, int stageVerts,
+                          unsigned long long* wgClock = nullptr);
 // Allows the batch kernel `bytes` of dynamic LDS (called before capture).
 void batch_allow_lds(int bytes);
 

@@ -1350,10 +1350,14 @@ __device__ __forceinline__ void batch_iterations(const ClothDesc& d, Pos P, floa
 }
 
 template <int BS>
-__global__ void __launch_bounds__(BS) k_batch_substep(const ClothDesc* __restrict__ descs, int stageVerts)
+__global__ void __launch_bounds__(BS) k_batch_substep(const ClothDesc* __restrict__ descs, int stageVerts,
+                                                      unsigned long long* __restrict__ wgClock)
 {
     extern __shared__ float4 sv[];
     __shared__ float red[BS / 64];
+    // with wgClock: the workgroup's start and end on the constant-rate clock
+    // and the unit it ran on (XCC id << 32 | HW_ID), three words per cloth
+    const unsigned long long t0 = wgClock ? wall_clock64() : 0ull;
     const ClothDesc d = descs[blockIdx.x];
     // prediction and detection (ClothPBD.cpp:258-266, ClothCollision.cpp)
     for (int i = threadIdx.x; i < d.nv; i += BS) {
@@ -1398,6 +1402,16 @@ __global__ void __launch_bounds__(BS) k_batch_substep(const ClothDesc* __restric
         d.ctl->bendError = cb;
         d.ctl->contactError = cv;
     }
+    if (wgClock) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned hw = __builtin_amdgcn_s_getreg((4) | (0 << 6) | (31 << 11));     // HW_ID
+            const unsigned xcc = __builtin_amdgcn_s_getreg((20) | (0 << 6) | (15 << 11));   // XCC_ID
+            wgClock[3 * (size_t)blockIdx.x] = t0;
+            wgClock[3 * (size_t)blockIdx.x + 1] = wall_clock64();
+            wgClock[3 * (size_t)blockIdx.x + 2] = ((unsigned long long)xcc << 32) | hw;
+        }
+    }
 }
 
 void batch_allow_lds(int bytes)
@@ -1407,16 +1421,17 @@ void batch_allow_lds(int bytes)
     HIP_OK(hipFuncSetAttribute((const void*)k_batch_substep<256>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
 }
 
-void launch_batch_substep(hipStream_t s, const ClothDesc* descs, int numCloths, int blockSize, int stageVerts)
+void launch_batch_substep(hipStream_t s, const ClothDesc* descs, int numCloths, int blockSize, int stageVerts,
+                          unsigned long long* wgClock)
 {
     if (numCloths <= 0) return;
     const size_t lds = (size_t)(stageVerts > 0 ? stageVerts : 0) * sizeof(float4);
     if (blockSize == 1024)
-        hipLaunchKernelGGL(k_batch_substep<1024>, dim3(numCloths), dim3(1024), lds, s, descs, stageVerts);
+        hipLaunchKernelGGL(k_batch_substep<1024>, dim3(numCloths), dim3(1024), lds, s, descs, stageVerts, wgClock);
     else if (blockSize == 512)
-        hipLaunchKernelGGL(k_batch_substep<512>, dim3(numCloths), dim3(512), lds, s, descs, stageVerts);
+        hipLaunchKernelGGL(k_batch_substep<512>, dim3(numCloths), dim3(512), lds, s, descs, stageVerts, wgClock);
     else
-        hipLaunchKernelGGL(k_batch_substep<256>, dim3(numCloths), dim3(256), lds, s, descs, stageVerts);
+        hipLaunchKernelGGL(k_batch_substep<256>, dim3(numCloths), dim3(256), lds, s, descs, stageVerts, wgClock);
 }
 
 static inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }

@@ -2154,6 +2154,7 @@ ClothBatch::~ClothBatch()
     release_graph();
     dev_free(d_descs);
     dev_free(d_cs);
+    dev_free(d_clock);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -2267,12 +2268,19 @@ void ClothBatch::step(float dt)
             const int nv = (int)p->mModel->pos.size();
             if (nv <= stageLimit) stageVerts = std::max(stageVerts, nv);
         }
+    if (recordClock && m > clockCap) {
+        release_graph();
+        dev_alloc(d_clock, 3 * (size_t)m);
+        clockCap = m;
+    }
+    unsigned long long* clock = recordClock ? d_clock : nullptr;
     std::string key = std::to_string(bs) + "/" + std::to_string(stageVerts) + "/" + std::to_string(n) + "/" +
-                      std::to_string(m) + "/" + std::to_string((uintptr_t)d_descs);
+                      std::to_string(m) + "/" + std::to_string((uintptr_t)d_descs) + "/" +
+                      std::to_string((uintptr_t)clock);
     if (useGraph && (!exec || key != graphKey)) {
         release_graph();
         HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < n; i++) launch_batch_substep(stream, d_descs, m, bs, stageVerts);
+        for (int i = 0; i < n; i++) launch_batch_substep(stream, d_descs, m, bs, stageVerts, clock);
         HIP_OK(hipStreamEndCapture(stream, &graph));
         HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         graphKey = key;
@@ -2285,7 +2293,7 @@ void ClothBatch::step(float dt)
     if (useGraph)
         HIP_OK(hipGraphLaunch(exec, stream));
     else
-        for (int i = 0; i < n; i++) launch_batch_substep(stream, d_descs, m, bs, stageVerts);
+        for (int i = 0; i < n; i++) launch_batch_substep(stream, d_descs, m, bs, stageVerts, clock);
     HIP_OK(hipEventRecord(ev1, stream));
     HIP_OK(hipStreamSynchronize(stream));
     HIP_OK(hipGetLastError());
@@ -2293,6 +2301,18 @@ void ClothBatch::step(float dt)
     HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));
     lastStepMs = ms;
     lastSubsteps = n;
+    if (clock) {
+        std::vector<unsigned long long> c(3 * (size_t)m);
+        HIP_OK(hipMemcpy(c.data(), clock, c.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        clockStart.assign(m, 0);
+        clockEnd.assign(m, 0);
+        clockUnit.assign(m, 0);
+        for (int k = 0; k < m; k++) {
+            clockStart[order[k]] = c[3 * (size_t)k];
+            clockEnd[order[k]] = c[3 * (size_t)k + 1];
+            clockUnit[order[k]] = c[3 * (size_t)k + 2];
+        }
+    }
     if (timing) {
         trialConfigs.push_back({bs, st});
         trialMs.push_back(ms);

@@ -507,6 +507,11 @@ public:
     bool useGraph = true;  // replay the captured substep launches (else launch directly)
     double lastStepMs = 0.0;  // device time of the last step's substep launches
     int lastSubsteps = 0;
+    // per-cloth workgroup clocks of the last substep launch (recordClock):
+    // start, end (device clock ticks) and the unit each cloth's workgroup
+    // ran on, in patch order
+    bool recordClock = false;
+    std::vector<unsigned long long> clockStart, clockEnd, clockUnit;
     void add(ClothPatch* p);
     void step(float dt);
     void set_launch(int bs, int st);
@@ -523,6 +528,8 @@ private:
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string graphKey;
     int stageLimit = 0;
+    unsigned long long* d_clock = nullptr;  // 3 words per cloth, dispatch order
+    int clockCap = 0;
     std::vector<std::pair<int, int>> trials;  // (block size, stage) candidates
 public:
     std::vector<std::pair<int, int>> trialConfigs;  // timed so far, with trialMs

@@ -103,3 +103,25 @@ def test_autotuner_times_every_candidate_and_keeps_results(sc):
     assert min(ms for _, _, ms in t) == [ms for b, s, ms in t if (b, s) == (bs, stage)][0]
     for (_, p), (_, q) in zip(built, ref_built):
         assert np.array_equal(p.GetModel().positions(), q.GetModel().positions())
+
+
+def test_batch_workgroup_times(sc):
+    """Per-cloth workgroup clocks: one entry per patch, every workgroup ends
+    after it starts, and recording leaves the results unchanged (the same
+    cloths stepped without it give the same bits)."""
+    scenes = S.batch_scenes(12, seed=3, n_range=(16, 48))
+    built, batch = _build(sc, scenes)
+    built2, batch2 = _build(sc, scenes)
+    for b in (batch, batch2):
+        b.SetLaunch(256, 1)
+    assert batch.workgroup_times() is None
+    batch.RecordWorkgroupTimes(True)
+    for _ in range(3):
+        batch.Step(scenes[0].dt)
+        batch2.Step(scenes[0].dt)
+    wt = batch.workgroup_times()
+    assert wt["start_us"].shape == (12,) and wt["unit"].shape == (12, 4)
+    assert (wt["start_us"] >= 0).all() and (wt["end_us"] > wt["start_us"]).all()
+    assert wt["start_us"].min() == 0.0
+    for (_, p), (_, q) in zip(built, built2):
+        assert np.array_equal(p.GetModel().positions(), q.GetModel().positions())
