@@ -378,14 +378,10 @@ struct SelfCandidates {
     int* fillVT;
     int* fillEE;
     int capVT, capEE;
-    // per sorted leaf of the triangle / edge tree: the primitive's vertices
-    // (x, y, z / x, y), so a walk tests shared vertices without a dependent load
-    const int4* triLeafVerts;
-    const int4* edgeLeafVerts;
 };
-// Writes out[k] = the vertices of the primitive at sorted leaf k of bv
-// (cTris: triangles; else cEdges: edges, z = w = -1). After every build.
-void launch_leaf_verts(hipStream_t s, const BvhView& bv, const int4* cTris, const int2* cEdges, int4* out);
+// The triangle and edge trees carry each leaf's vertices as its record's
+// payload (Lbvh::set_payload: the triangles' int4, the edges' (x, y, -1, -1)),
+// so a walk tests shared vertices without a dependent load.
 // vertexOrder: the order in which vertices query the triangle tree (may be null)
 void launch_self_detect(hipStream_t s, const float4* pos, const float4* prev, const int4* cTris, const int2* cEdges,
                         int nv, int ne, BvhView triBvh, BvhView edgeBvh, const int* vertexOrder, SelfParams sp,

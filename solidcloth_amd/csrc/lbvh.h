@@ -30,12 +30,16 @@ struct BvhView {
     const int* ropeNode;
     const int* ropeLeaf;
     const int2* range;    // internal nodes: first and last sorted leaf below
-    // The same hierarchy packed for the rope walks, one cache line per step:
-    // internal node i at nodes[4 i .. 4 i + 2] = (min, left child), (max,
-    // right child), (rope, first leaf, last leaf, -) as int bits; leaf k at
-    // leaves[2 k], leaves[2 k + 1] = (min, primitive), (max, rope).
-    const float4* nodes;
-    const float4* leaves;
+    // The same hierarchy packed for the walks: all 2n - 1 nodes and leaves in
+    // depth-first, left-first order, 64 bytes each (4 float4). Record p:
+    // [0] = (min, primitive; -1 for an internal node), [1] = (max, escape
+    // position: p + 1 for a leaf, p + its subtree's size for an internal
+    // node, 2n - 1 past the last), [2] = the leaf's payload (int4, set with
+    // Lbvh::set_payload; 0 for internal nodes), [3] = (sorted leaf index of a
+    // leaf / last sorted leaf below an internal node, 0, 0, 0), int bits.
+    // A walk that enters a node goes to p + 1, so it moves forward through
+    // the array, mostly one record at a time.
+    const float4* dfs;
     int n;                // number of primitives (leaves)
 };
 
@@ -51,6 +55,10 @@ public:
     void refit(hipStream_t s, const float4* boxLo, const float4* boxHi);
     BvhView view() const;
     int size() const { return n_; }
+    // int4 per primitive (by primitive id) copied into each leaf's record by
+    // every build and refit from then on; the array must outlive the tree's
+    // use (null: zeros).
+    void set_payload(const int4* perPrimitive) { payload_ = perPrimitive; }
     // Downloads the hierarchy and checks it on the host: every leaf appears
     // once, parent links agree with child links, node boxes equal the union
     // of their children. Returns the number of violations (synchronous).
@@ -70,7 +78,9 @@ private:
     int2* range_ = nullptr;       // sorted-leaf range of each internal node
     unsigned* enc_ = nullptr;     // per internal node: order-encoded min (3) / max (3) words
     int *ropeNode_ = nullptr, *ropeLeaf_ = nullptr;
-    float4 *nodes_ = nullptr, *leaves_ = nullptr;  // packed copies (BvhView::nodes, leaves)
+    int *dfsNode_ = nullptr, *dfsLeaf_ = nullptr;  // depth-first positions of nodes and leaves
+    float4* dfs_ = nullptr;                          // packed records (BvhView::dfs)
+    const int4* payload_ = nullptr;
 };
 
 }  // namespace sc

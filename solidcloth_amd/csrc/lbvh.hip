@@ -387,26 +387,68 @@ __global__ void __launch_bounds__(kRefitBlock) k_refit(const float4* __restrict_
     }
 }
 
-// The packed copy of the hierarchy (BvhView::nodes / leaves), written after
-// every build and refit: threads [0, n - 1) internal nodes, [n - 1, 2n - 1)
-// leaves.
+// Depth-first (pre-order, left first) position of every node and leaf
+// (threads [0, n - 1) internal nodes, [n - 1, 2n - 1) leaves): the nodes
+// before v are its ancestors and, for each ancestor whose right subtree holds
+// v, that ancestor's whole left subtree; those subtrees hold the `first`
+// leaves left of v's range and 2 first - R nodes, where R counts such
+// ancestors, so the position is 2 first + (the ancestors whose left subtree
+// holds v).
+__global__ void __launch_bounds__(kBlock) k_dfs_pos(const float4* __restrict__ lo, const int* __restrict__ parentNode,
+                                                    const int* __restrict__ parentLeaf,
+                                                    const int2* __restrict__ range, int n, int* dfsNode,
+                                                    int* dfsLeaf)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= 2 * n - 1) return;
+    if (n == 1) {
+        dfsLeaf[0] = 0;
+        return;
+    }
+    const bool leaf = x >= n - 1;
+    const int l = x - (n - 1);
+    int cur = leaf ? ~l : x;
+    int p = leaf ? parentLeaf[l] : parentNode[x];
+    int left = 0;
+    while (p >= 0) {
+        left += __float_as_int(lo[p].w) == cur;
+        cur = p;
+        p = parentNode[p];
+    }
+    const int first = leaf ? l : range[x].x;
+    if (leaf) dfsLeaf[l] = 2 * first + left;
+    else dfsNode[x] = 2 * first + left;
+}
+
+// The packed depth-first records (BvhView::dfs), written after every build
+// and refit: threads [0, n - 1) internal nodes, [n - 1, 2n - 1) leaves.
 __global__ void __launch_bounds__(kBlock) k_pack(const float4* __restrict__ lo, const float4* __restrict__ hi,
                                                  const float4* __restrict__ leafLo, const float4* __restrict__ leafHi,
-                                                 const int* __restrict__ ropeNode, const int* __restrict__ ropeLeaf,
-                                                 const int2* __restrict__ range, int n, float4* nodes, float4* leaves)
+                                                 const int* __restrict__ leafPrim, const int2* __restrict__ range,
+                                                 const int* __restrict__ dfsNode, const int* __restrict__ dfsLeaf,
+                                                 const int4* __restrict__ payload, int n, float4* dfs)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x < n - 1) {
         const int2 r = range[x];
-        nodes[4 * (size_t)x] = lo[x];
-        nodes[4 * (size_t)x + 1] = hi[x];
-        nodes[4 * (size_t)x + 2] =
-            make_float4(__int_as_float(ropeNode[x]), __int_as_float(r.x), __int_as_float(r.y), 0.f);
+        const int p = dfsNode[x];
+        const float4 a = lo[x], b = hi[x];
+        float4* R = dfs + 4 * (size_t)p;
+        R[0] = make_float4(a.x, a.y, a.z, __int_as_float(-1));
+        R[1] = make_float4(b.x, b.y, b.z, __int_as_float(p + 2 * (r.y - r.x) + 1));
+        R[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+        R[3] = make_float4(__int_as_float(r.y), 0.f, 0.f, 0.f);
     } else if (x < 2 * n - 1) {
         const int k = x - (n - 1);
-        const float4 b = leafHi[k];
-        leaves[2 * (size_t)k] = leafLo[k];
-        leaves[2 * (size_t)k + 1] = make_float4(b.x, b.y, b.z, __int_as_float(ropeLeaf[k]));
+        const int p = dfsLeaf[k];
+        const int prim = leafPrim[k];
+        const float4 a = leafLo[k], b = leafHi[k];
+        const int4 q = payload ? payload[prim] : make_int4(0, 0, 0, 0);
+        float4* R = dfs + 4 * (size_t)p;
+        R[0] = make_float4(a.x, a.y, a.z, __int_as_float(prim));
+        R[1] = make_float4(b.x, b.y, b.z, __int_as_float(p + 1));
+        R[2] = make_float4(__int_as_float(q.x), __int_as_float(q.y), __int_as_float(q.z), __int_as_float(q.w));
+        R[3] = make_float4(__int_as_float(k), 0.f, 0.f, 0.f);
     }
 }
 
@@ -419,7 +461,7 @@ Lbvh::~Lbvh()
     dev_free(lo_); dev_free(hi_); dev_free(leafLo_); dev_free(leafHi_); dev_free(leafPrim_);
     dev_free(keys_); dev_free(keys2_); dev_free(vals_); dev_free(vals2_); dev_free(hist_);
     dev_free(parentNode_); dev_free(parentLeaf_); dev_free(flags_); dev_free(bounds_); dev_free(range_); dev_free(enc_);
-    dev_free(ropeNode_); dev_free(ropeLeaf_); dev_free(nodes_); dev_free(leaves_);
+    dev_free(ropeNode_); dev_free(ropeLeaf_); dev_free(dfsNode_); dev_free(dfsLeaf_); dev_free(dfs_);
 }
 
 void Lbvh::reserve(int n)
@@ -432,7 +474,7 @@ void Lbvh::reserve(int n)
     dev_alloc(parentNode_, n); dev_alloc(parentLeaf_, n); dev_alloc(flags_, n); dev_alloc(range_, n);
     dev_alloc(enc_, (size_t)6 * n);
     dev_alloc(ropeNode_, n); dev_alloc(ropeLeaf_, n);
-    dev_alloc(nodes_, (size_t)4 * n); dev_alloc(leaves_, (size_t)2 * n);
+    dev_alloc(dfsNode_, n); dev_alloc(dfsLeaf_, n); dev_alloc(dfs_, (size_t)8 * n);
     if (!bounds_) dev_alloc(bounds_, 6);
     cap_ = n;
 }
@@ -466,15 +508,17 @@ void Lbvh::build(hipStream_t s, const float4* boxLo, const float4* boxHi, int n)
     if (n > 1)
         hipLaunchKernelGGL(k_ropes, dim3(nblocks(2 * n - 1, kBlock)), dim3(kBlock), 0, s, lo_, hi_, parentNode_,
                            parentLeaf_, n, ropeNode_, ropeLeaf_);
+    hipLaunchKernelGGL(k_dfs_pos, dim3(nblocks(2 * n - 1, kBlock)), dim3(kBlock), 0, s, lo_, parentNode_, parentLeaf_,
+                       range_, n, dfsNode_, dfsLeaf_);
     pack(s);
     HIP_OK(hipGetLastError());
 }
 
 void Lbvh::pack(hipStream_t s)
 {
-    if (n_ <= 1) return;  // a single leaf has no ropes: the walks test it directly
+    if (n_ < 1) return;
     hipLaunchKernelGGL(k_pack, dim3(nblocks(2 * n_ - 1, kBlock)), dim3(kBlock), 0, s, lo_, hi_, leafLo_, leafHi_,
-                       ropeNode_, ropeLeaf_, range_, n_, nodes_, leaves_);
+                       leafPrim_, range_, dfsNode_, dfsLeaf_, payload_, n_, dfs_);
 }
 
 void Lbvh::refit(hipStream_t s, const float4* boxLo, const float4* boxHi)
@@ -566,12 +610,30 @@ int Lbvh::verify(hipStream_t s, const float4* boxLo, const float4* boxHi) const
         const int expect = after < (int)order.size() ? order[after] : kRopeEnd;
         if ((x < 0 ? rl[~x] : rn[x]) != expect) bad++;
     }
+    // the packed records: the walk order's k-th node at position k, its
+    // escape k + subtree size, its box, primitive and leaf fields
+    std::vector<int> dn(n - 1), dl(n);
+    std::vector<float4> rec(4 * (size_t)(2 * n - 1));
+    get(dn, dfsNode_);
+    get(dl, dfsLeaf_);
+    get(rec, dfs_);
+    for (int k = 0; k < (int)order.size(); k++) {
+        const int x = order[k];
+        if ((x < 0 ? dl[~x] : dn[x]) != k) { bad++; continue; }
+        const float4* R = rec.data() + 4 * (size_t)k;
+        const int esc = k + (x < 0 ? 1 : sizeInt[x]);
+        const float4 a = x < 0 ? llo[~x] : lo[x], b = x < 0 ? lhi[~x] : hi[x];
+        if (R[0].x != a.x || R[0].y != a.y || R[0].z != a.z || R[1].x != b.x || R[1].y != b.y || R[1].z != b.z ||
+            __builtin_bit_cast(int, R[1].w) != esc ||
+            __builtin_bit_cast(int, R[0].w) != (x < 0 ? prim[~x] : -1))
+            bad++;
+    }
     return bad;
 }
 
 BvhView Lbvh::view() const
 {
-    return BvhView{lo_, hi_, leafLo_, leafHi_, leafPrim_, ropeNode_, ropeLeaf_, range_, nodes_, leaves_, n_};
+    return BvhView{lo_, hi_, leafLo_, leafHi_, leafPrim_, ropeNode_, ropeLeaf_, range_, dfs_, n_};
 }
 
 }  // namespace sc

@@ -329,7 +329,7 @@ void GpuSolver::release()
     dev_free(self.candFillVT); dev_free(self.candFillEE);
     self.candCapVT = self.candCapEE = 0;
     self.candWantVT = self.candWantEE = 0;
-    dev_free(self.vloc); dev_free(self.triLeafVerts); dev_free(self.edgeLeafVerts);
+    dev_free(self.vloc); dev_free(self.edgeVerts4);
     for (SelfBuf& b : self.buf) {
         dev_free(b.ids); dev_free(b.W); dev_free(b.N); dev_free(b.key); dev_free(b.colour); dev_free(b.order);
         dev_free(b.active); dev_free(b.groupOff); dev_free(b.lid); dev_free(b.vlist);
@@ -932,17 +932,19 @@ void ClothModel::ensure_self()
         dev_alloc(S.vloc, std::max(g.nv, 1));
         HIP_OK(hipMemset(S.vloc, 0x7f, sizeof(int) * std::max(g.nv, 1)));
         for (SelfBuf& b : S.buf) dev_alloc(b.vlist, std::max(g.nv, 1));
-        dev_alloc(S.triLeafVerts, std::max(g.nt, 1));
-        dev_alloc(S.edgeLeafVerts, std::max(g.ne, 1));
+        std::vector<int4> ev(std::max(g.ne, 1), make_int4(-1, -1, -1, -1));
+        for (int e = 0; e < g.ne; e++) ev[e] = make_int4((int)edges[e].i1, (int)edges[e].i2, -1, -1);
+        dev_alloc(S.edgeVerts4, ev.size());
+        HIP_OK(hipMemcpy(S.edgeVerts4, ev.data(), ev.size() * sizeof(int4), hipMemcpyHostToDevice));
     }
     // hierarchy from the current configuration; substeps refit it (the
     // vertex tree only orders the vertex queries, so it is not refit)
     launch_self_boxes(g.stream, g.d_pos, g.d_prev, g.d_cTris, g.nt, g.d_cEdges, g.ne, mThickness + mTolerance * 0.5f,
                       S.triLo, S.triHi, S.edgeLo, S.edgeHi, g.nv, S.vtxLo, S.vtxHi);
+    S.triB.set_payload(g.d_cTris);
+    S.edgeB.set_payload(S.edgeVerts4);
     S.triB.build(g.stream, S.triLo, S.triHi, g.nt);
     S.edgeB.build(g.stream, S.edgeLo, S.edgeHi, g.ne);
-    launch_leaf_verts(g.stream, S.triB.view(), g.d_cTris, nullptr, S.triLeafVerts);
-    launch_leaf_verts(g.stream, S.edgeB.view(), nullptr, g.d_cEdges, S.edgeLeafVerts);
     S.vtxB.build(g.stream, S.vtxLo, S.vtxHi, g.nv);
     S.built = true;
 }
@@ -1155,7 +1157,7 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
             launch_self_detect(s, g.d_pos, g.d_prev, g.d_cTris, g.d_cEdges, g.nv, g.ne, S.triB.view(),
                                S.edgeB.view(), S.vtxB.view().leafPrim, sp,
                                SelfCandidates{S.candVT, S.candEE, S.candFillVT, S.candFillEE, S.candCapVT,
-                                              S.candCapEE, S.triLeafVerts, S.edgeLeafVerts},
+                                              S.candCapEE},
                                S.buf[0], S.buf[1], g.d_ctl);
             launch_self_colour(s, S.buf[0], 0, S.vmax, S.vloc, g.d_ctl);
             launch_self_colour(s, S.buf[1], 1, S.vmax, S.vloc, g.d_ctl);

@@ -270,7 +270,7 @@ public:
         SelfBuf buf[2]{};
         unsigned long long* vmax = nullptr;
         int* vloc = nullptr;  // the colouring's per-vertex claims
-        int4 *triLeafVerts = nullptr, *edgeLeafVerts = nullptr;  // vertices per sorted leaf (rebuilt with the trees)
+        int4* edgeVerts4 = nullptr;  // each cloth edge's vertices as (i1, i2, -1, -1): the edge tree's leaf payload
         int2 *candVT = nullptr, *candEE = nullptr;  // candidate pools (chunks of kCandChunk pairs)
         int *candFillVT = nullptr, *candFillEE = nullptr;  // pairs per chunk
         int candCapVT = 0, candCapEE = 0;    // pool capacities in pairs

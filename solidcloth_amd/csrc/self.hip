@@ -288,21 +288,28 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb)
 // pair is found once, and is written with the lower edge index first.
 //
 // The 64 queries of a wave walk the tree together: one node sequence for the
-// wave (depth-first, left-first along the escape links), a node entered when
-// any lane's box overlaps it (and, for edges, any such lane still wants one
-// of its leaves), each lane testing its own box at every leaf the walk
-// reaches. Each lane thus meets exactly the leaves its own walk would, in the
-// same order, while the node and leaf records, one address per wave, come
-// through the scalar cache and the control flow stays uniform. Pairs go to
-// flat pools in chunks of kCandChunk slots: the lanes that find a pair at a
-// leaf take consecutive slots of the wave's chunk (its base and fill are
-// wave-uniform, in scalar registers), and a fresh chunk costs one atomic.
-// The continuous tests run in k_self_ccd, so that the walk and the tests'
-// FP64 registers do not meet in one kernel.
+// wave (depth-first, left-first), a node entered when any lane's box overlaps
+// it (and, for edges, any such lane still wants one of its leaves), each lane
+// testing its own box at every leaf the walk reaches. Each lane thus meets
+// exactly the leaves its own walk would, in the same order, and the control
+// flow stays uniform. The records lie in that depth-first order, so the walk
+// only moves forward: to the next record when it enters a node or leaves a
+// leaf, past the subtree when it skips one. The wave holds two windows of 16
+// consecutive records in registers (lane 4 r + q holds float4 q of record r)
+// and reads a step's fields from them with lane reads; the window after the
+// current one is requested when the walk enters the current one, so a walk
+// waits on memory only where a skip lands beyond both. Pairs go to flat pools
+// in chunks of kCandChunk slots: the lanes that find a pair at a leaf take
+// consecutive slots of the wave's chunk (its base and fill are wave-uniform,
+// in scalar registers), and a fresh chunk costs one atomic. The continuous
+// tests run in k_self_ccd, so that the walk and the tests' FP64 registers do
+// not meet in one kernel.
+constexpr int kWalkWindow = 16;  // records per window (one float4 per lane)
+
 template <bool kEdges>
-__device__ __forceinline__ void wave_candidates(const BvhView& bv, const int4* __restrict__ verts, bool active, f3 lo,
-                                                f3 hi, int self, int first, int2 me, int2* pool, int* fill, int cap,
-                                                int* reserved, SolverCtl* ctl)
+__device__ __forceinline__ void wave_candidates(const BvhView& bv, bool active, f3 lo, f3 hi, int self, int first,
+                                                int2 me, int2* pool, int* fill, int cap, int* reserved,
+                                                SolverCtl* ctl)
 {
     const int lane = __lane_id();
     int cbase = -1, cused = kCandChunk;  // the wave's chunk (no chunk yet)
@@ -329,55 +336,65 @@ __device__ __forceinline__ void wave_candidates(const BvhView& bv, const int4* _
         }
         cused += cnt;
     };
-    // the tree records, read-only for the kernel: through the constant
-    // address space, a wave-uniform address becomes a scalar load
-    typedef const __attribute__((address_space(4))) float CF;
-    typedef const __attribute__((address_space(4))) int CI;
-    auto ldf4 = [](const float4* p) {
-        CF* f = (CF*)p;
-        return make_float4(f[0], f[1], f[2], f[3]);
-    };
-    auto ldi4 = [](const int4* p) {
-        CI* f = (CI*)p;
-        return make_int4(f[0], f[1], f[2], f[3]);
-    };
+    const int end = 2 * bv.n - 1;  // records
     if (bv.n >= 1 && __ballot(active)) {
-        int node = bv.n == 1 ? ~0 : 0;
-        // a walk visits each of the 2n - 1 nodes at most once; the bound only
-        // guards against malformed links
-        for (int steps = 2 * bv.n; node != kRopeEnd && steps > 0; steps--) {
-            node = __builtin_amdgcn_readfirstlane(node);  // (the same on every lane)
-            if (node >= 0) {
-                const float4* N = bv.nodes + 4 * (size_t)node;
-                // (min, left), (max, right), (rope, first, last)
-                const float4 L = ldf4(N), H = ldf4(N + 1), A = ldf4(N + 2);
-                // (no short-circuit: the three records are read together)
-                const bool want = (int)active & (int)(!kEdges | (__float_as_int(A.z) > first)) & (int)overlap_nb(lo, hi, L, H);
-                // both successors in hand with the box (else the compiler reads
-                // the chosen one after the vote: a second wait per step)
-                int left = __float_as_int(L.w), rope = __float_as_int(A.x);
-                asm volatile("" : "+s"(left), "+s"(rope));
-                node = __ballot(want) ? left : rope;
-            } else {
-                const int leaf = ~node;
-                const float4* F = bv.leaves + 2 * (size_t)leaf;
-                const float4 L = ldf4(F), H = ldf4(F + 1);
-                const int4 V = ldi4(verts + leaf);
-                const int prim = __float_as_int(L.w);
-                bool hit = active & overlap_nb(lo, hi, L, H);
-                int2 p;
-                if (kEdges) {
-                    hit = hit & (leaf > first) & !((me.x == V.x) | (me.y == V.y) | (me.x == V.y) | (me.y == V.x));
-                    p = self < prim ? make_int2(self, prim) : make_int2(prim, self);
+        // this lane's share of a window starting at record `base` (records
+        // past the end repeat the last one; the walk never reads them)
+        const int sub = lane >> 2, part = lane & 3;
+        auto window = [&](int base) { return bv.dfs[4 * (size_t)min(base + sub, end - 1) + part]; };
+        auto fl = [](float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); };
+        auto in = [](float v, int l) { return __builtin_amdgcn_readlane(__float_as_int(v), l); };
+        // A holds records [wa, wa + 16), B the next 16 (perhaps in flight).
+        // Waiting for A alone (vmcnt 1: B, the youngest, may stay in flight)
+        // before the steps tells the compiler that A is in: otherwise the
+        // pair stores of the steps, counted by the same counter, would make
+        // it wait for B as well.
+        constexpr int kWaitAllButOne = 0x0F71;  // vmcnt(1), expcnt and lgkmcnt not waited on
+        int wa = 0, p = 0;
+        float4 A = window(0), B = window(kWalkWindow);
+        __builtin_amdgcn_s_waitcnt(kWaitAllButOne);
+        // each step moves forward, so there are at most `end` steps
+        for (int steps = end; steps > 0;) {
+            // the steps inside window A: no memory access
+            for (; p < wa + kWalkWindow && p < end && steps > 0; steps--) {
+                const int r = 4 * (p - wa);
+                // (min, primitive or -1), (max, escape), payload, (index, ...)
+                const float4 L = make_float4(fl(A.x, r), fl(A.y, r), fl(A.z, r), 0.f);
+                const float4 H = make_float4(fl(A.x, r + 1), fl(A.y, r + 1), fl(A.z, r + 1), 0.f);
+                const int prim = in(A.w, r);
+                const int idx = in(A.x, r + 3);  // leaf: its sorted index; node: the last one below it
+                const bool ov = (int)active & (int)overlap_nb(lo, hi, L, H);
+                if (prim < 0) {
+                    const bool want = (int)ov & (int)(!kEdges | (idx > first));
+                    p = __ballot(want) ? p + 1 : in(A.w, r + 1);
                 } else {
-                    hit = hit & !((self == V.x) | (self == V.y) | (self == V.z));
-                    p = make_int2(prim, self);
+                    const int vx = in(A.x, r + 2), vy = in(A.y, r + 2), vz = in(A.z, r + 2);
+                    bool hit = ov;
+                    int2 pr;
+                    if (kEdges) {
+                        hit = (int)hit & (int)(idx > first) &
+                              (int)!((me.x == vx) | (me.y == vy) | (me.x == vy) | (me.y == vx));
+                        pr = self < prim ? make_int2(self, prim) : make_int2(prim, self);
+                    } else {
+                        hit = (int)hit & (int)!((self == vx) | (self == vy) | (self == vz));
+                        pr = make_int2(prim, self);
+                    }
+                    take(hit, pr);
+                    p = p + 1;
                 }
-                int next = __float_as_int(H.w);
-                asm volatile("" : "+s"(next));
-                take(hit, p);
-                node = bv.n == 1 ? kRopeEnd : next;
+                p = __builtin_amdgcn_readfirstlane(p);  // (the same on every lane)
             }
+            if (p >= end) break;
+            // the next window: B if the walk landed in it, else a fresh one
+            if (p < wa + 2 * kWalkWindow) {
+                wa += kWalkWindow;
+                A = B;
+            } else {
+                wa = p;
+                A = window(wa);
+            }
+            B = window(wa + kWalkWindow);
+            __builtin_amdgcn_s_waitcnt(kWaitAllButOne);
         }
     }
     if (lane == 0 && cbase >= 0 && cbase < cap) fill[cbase / kCandChunk] = cused;
@@ -409,7 +426,7 @@ __global__ void __launch_bounds__(kBlock) k_self_candidates(const float4* __rest
                 hi = f3{hi.x + tol, hi.y + tol, hi.z + tol};
             }
         }
-        wave_candidates<false>(triBvh, cand.triLeafVerts, active, lo, hi, v, 0, make_int2(0, 0), cand.vt,
+        wave_candidates<false>(triBvh, active, lo, hi, v, 0, make_int2(0, 0), cand.vt,
                                cand.fillVT, cand.capVT, &ctl->candSlots[0], ctl);
     } else {
         bool active = false;
@@ -427,7 +444,7 @@ __global__ void __launch_bounds__(kBlock) k_self_candidates(const float4* __rest
             hi = f3{hi.x + tol, hi.y + tol, hi.z + tol};
         }
         // query j (sorted leaf j) pairs with the overlapping leaves after it
-        wave_candidates<true>(edgeBvh, cand.edgeLeafVerts, active, lo, hi, e1, j, ev, cand.ee, cand.fillEE,
+        wave_candidates<true>(edgeBvh, active, lo, hi, e1, j, ev, cand.ee, cand.fillEE,
                               cand.capEE, &ctl->candSlots[1], ctl);
     }
 }
@@ -913,28 +930,7 @@ __global__ void __launch_bounds__(kSoloBlock) k_self_solve(float4* pos, SelfBuf 
     wave_max_atomic(kind == 0 ? &ctl->sct : &ctl->sce, err);
 }
 
-__global__ void __launch_bounds__(kBlock) k_leaf_verts(const int* __restrict__ leafPrim, int n,
-                                                       const int4* __restrict__ cTris,
-                                                       const int2* __restrict__ cEdges, int4* __restrict__ out)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int p = leafPrim[k];
-    if (cTris) {
-        out[k] = cTris[p];
-    } else {
-        const int2 e = cEdges[p];
-        out[k] = make_int4(e.x, e.y, -1, -1);
-    }
-}
-
 static inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
-
-void launch_leaf_verts(hipStream_t s, const BvhView& bv, const int4* cTris, const int2* cEdges, int4* out)
-{
-    if (bv.n <= 0) return;
-    hipLaunchKernelGGL(k_leaf_verts, dim3(blocks(bv.n)), dim3(kBlock), 0, s, bv.leafPrim, bv.n, cTris, cEdges, out);
-}
 
 void launch_self_boxes(hipStream_t s, const float4* pos, const float4* prev, const int4* cTris, int nt,
                        const int2* cEdges, int ne, float tol, float4* triLo, float4* triHi, float4* edgeLo,
