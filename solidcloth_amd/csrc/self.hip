@@ -313,8 +313,7 @@ __device__ __forceinline__ void wave_candidates(const BvhView& bv, bool active, 
 {
     const int lane = __lane_id();
     int cbase = -1, cused = kCandChunk;  // the wave's chunk (no chunk yet)
-    auto take = [&](bool hit, int2 p) {
-        const unsigned long long mask = __ballot(hit);
+    auto take = [&](bool hit, unsigned long long mask, int2 p) {
         if (!mask) return;
         const int cnt = __popcll(mask);
         if (cused + cnt > kCandChunk) {
@@ -353,10 +352,11 @@ __device__ __forceinline__ void wave_candidates(const BvhView& bv, bool active, 
         int wa = 0, p = 0;
         float4 A = window(0), B = window(kWalkWindow);
         __builtin_amdgcn_s_waitcnt(kWaitAllButOne);
-        // each step moves forward, so there are at most `end` steps
-        for (int steps = end; steps > 0;) {
+        // every step moves forward (an escape below p + 1 is taken as p + 1),
+        // so there are at most `end` steps
+        while (p < end) {
             // the steps inside window A: no memory access
-            for (; p < wa + kWalkWindow && p < end && steps > 0; steps--) {
+            for (const int lim = min(wa + kWalkWindow, end); p < lim;) {
                 const int r = 4 * (p - wa);
                 // (min, primitive or -1), (max, escape), payload, (index, ...)
                 const float4 L = make_float4(fl(A.x, r), fl(A.y, r), fl(A.z, r), 0.f);
@@ -366,7 +366,7 @@ __device__ __forceinline__ void wave_candidates(const BvhView& bv, bool active, 
                 const bool ov = (int)active & (int)overlap_nb(lo, hi, L, H);
                 if (prim < 0) {
                     const bool want = (int)ov & (int)(!kEdges | (idx > first));
-                    p = __ballot(want) ? p + 1 : in(A.w, r + 1);
+                    p = __ballot(want) ? p + 1 : max(p + 1, in(A.w, r + 1));
                 } else {
                     const int vx = in(A.x, r + 2), vy = in(A.y, r + 2), vz = in(A.z, r + 2);
                     bool hit = ov;
@@ -379,7 +379,7 @@ __device__ __forceinline__ void wave_candidates(const BvhView& bv, bool active, 
                         hit = (int)hit & (int)!((self == vx) | (self == vy) | (self == vz));
                         pr = make_int2(prim, self);
                     }
-                    take(hit, pr);
+                    take(hit, __ballot(hit), pr);
                     p = p + 1;
                 }
                 p = __builtin_amdgcn_readfirstlane(p);  // (the same on every lane)
