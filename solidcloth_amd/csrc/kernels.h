@@ -371,7 +371,7 @@ void launch_self_boxes(hipStream_t s, const float4* pos, const float4* prev, con
 // kind, reserved in chunks of kCandChunk slots; fill[c] is the number of
 // pairs in chunk c (a chunk's tail may be unused). Capacities are multiples
 // of kCandChunk. Then the continuous tests over them.
-constexpr int kCandChunk = 256;
+constexpr int kCandChunk = 1024;
 struct SelfCandidates {
     int2* vt;
     int2* ee;
