@@ -31,4 +31,17 @@ __device__ __forceinline__ void begin_substep_ctl(SolverCtl* ctl)
     ctl->iterations = 0;
 }
 
+// Called by threads 0 .. kErrParts - 1 of the first kernel of a substep.
+__device__ __forceinline__ void begin_substep_parts(SolverCtl* ctl, int t)
+{
+    ctl->errPart[0][t * kErrStride] = 0;
+    ctl->errPart[1][t * kErrStride] = 0;
+}
+
+// This workgroup's part of error `which` (0 bend, 1 vertex contacts).
+__device__ __forceinline__ unsigned* err_part(SolverCtl* ctl, int which)
+{
+    return &ctl->errPart[which][(blockIdx.x % kErrParts) * kErrStride];
+}
+
 }  // namespace sc

@@ -27,6 +27,12 @@ enum { CT_INVALID = 0, CT_WALLS = 1, CT_SLAB = 2, CT_SPHERE = 3, CT_MESH = 4, CT
 
 // Per-substep solver control block, written by the kernels. Errors are
 // non-negative floats stored by their bit pattern so atomicMax orders them.
+// The packed sweeps' error maxima go to kErrParts words per error, one per
+// 128-byte line, chosen by workgroup, so that a launch's waves do not queue
+// on one address (a same-address atomic costs the launch about 2.6 ns per
+// wave); a reader takes the maximum of the parts and the plain field.
+constexpr int kErrParts = 64, kErrStride = 32;
+
 struct SolverCtl {
     unsigned done;
     int k;
@@ -49,6 +55,7 @@ struct SolverCtl {
     // substeps (the host clears them before a step and folds in the last)
     int selfPeak[4];
     int selfUniq[2];  // distinct vertices of each kind's contacts (this substep)
+    unsigned errPart[2][kErrParts * kErrStride];  // [0] bend, [1] vertex-contact error parts
 };
 
 enum { MAX_COLLIDABLES = 8 };

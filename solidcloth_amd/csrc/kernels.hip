@@ -26,6 +26,7 @@ __global__ void k_predict(float4* __restrict__ pos, float4* __restrict__ prev, f
     if (i == 0) {
         begin_substep_ctl(ctl);
     }
+    if (i < kErrParts) begin_substep_parts(ctl, i);
     if (i >= n) return;
     float4 P = pos[i];
     prev[i] = make_float4(P.x, P.y, P.z, 0.f);
@@ -123,6 +124,7 @@ __global__ void k_predict_detect(float4* __restrict__ pos, float4* __restrict__ 
     if (i == 0) {
         begin_substep_ctl(ctl);
     }
+    if (i < kErrParts) begin_substep_parts(ctl, i);
     if (i >= n) return;
     float4 P = pos[i];
     const f3 pv = xyz(P);
@@ -986,7 +988,7 @@ __global__ void __launch_bounds__(kSlabBlock) k_fem_packed(float4* pos, PackedFe
     if (kContacts) {
         float err = 0.f;
         if (own && cp.cnt > 0) err = packed_vertex_contacts(g0, cp, sv, tid, kc);
-        if (kc.report & 1) wave_max_atomic(&ctl->cv, err);
+        if (kc.report & 1) wave_max_atomic(err_part(ctl, 1), err);
     }
     if (own) pos[g0] = sv[tid];
 }
@@ -1080,12 +1082,12 @@ __global__ void __launch_bounds__(kSlabBlock) k_bend_packed(float4* pos, PackedB
                 cerr = cerr < e ? e : cerr;
             }
         }
-        if (kc.report & 1) wave_max_atomic(&ctl->cv, cerr);
+        if (kc.report & 1) wave_max_atomic(err_part(ctl, 1), cerr);
     }
 #pragma unroll
     for (int u = 0; u < KV; u++)
         if (!((unsigned)raw[u] & kPackPad)) pos[raw[u] & kPackIdMask] = sv[tid + u * kSlabBlock];
-    if (kc.report & 2) wave_max_atomic(&ctl->cb, err);
+    if (kc.report & 2) wave_max_atomic(err_part(ctl, 0), err);
 }
 
 // Physics/ClothCollisionHandler.cpp:134-202 — vertex contacts with friction.
@@ -1140,15 +1142,24 @@ __global__ void __launch_bounds__(kBlock) k_contacts_velocity(float4* __restrict
 // The FEM model reports no link error, so that term of the test is 0.
 __global__ void k_iter_end(SolverCtl* ctl, int maxIter, int fixedIter, int bendFreq, int dihedral, float selfThr)
 {
+    // one wave: lane t folds in part t of the bend and contact errors
+    const int t = threadIdx.x;
     if (ctl->done) return;
-    int k = ctl->k + 1;
-    ctl->k = k;
-    float cb = __uint_as_float(ctl->cb), cv = __uint_as_float(ctl->cv);
+    const unsigned pb = wave_max(ctl->errPart[0][t * kErrStride]), pv = wave_max(ctl->errPart[1][t * kErrStride]);
+    const int k = ctl->k + 1;
+    float cb = __uint_as_float(max(ctl->cb, pb)), cv = __uint_as_float(max(ctl->cv, pv));
     const float ce = __uint_as_float(ctl->ce), ct = __uint_as_float(ctl->ct);
     const float sct = __uint_as_float(ctl->sct), sce = __uint_as_float(ctl->sce);
     const float cl = __uint_as_float(ctl->cl), eps = 0.01f;
     bool exitNow = !fixedIter && cl < 0.05f && cb < 0.005f && sce < selfThr && sct < selfThr && cv < eps &&
                    ce < eps && ct < eps;
+    const bool resetBend = dihedral && k % bendFreq == 0;
+    if (!(exitNow || k >= maxIter)) {
+        ctl->errPart[1][t * kErrStride] = 0;
+        if (resetBend) ctl->errPart[0][t * kErrStride] = 0;
+    }
+    if (t != 0) return;
+    ctl->k = k;
     if (exitNow || k >= maxIter) {
         ctl->done = 1;
         ctl->iterations = k;
@@ -1162,7 +1173,7 @@ __global__ void k_iter_end(SolverCtl* ctl, int maxIter, int fixedIter, int bendF
     ctl->sct = 0;
     ctl->sce = 0;
     ctl->cl = 0;
-    if (dihedral && k % bendFreq == 0) ctl->cb = 0;
+    if (resetBend) ctl->cb = 0;
 }
 
 // Fixed iteration count: the error statistics are those of the last pass of
@@ -1170,6 +1181,10 @@ __global__ void k_iter_end(SolverCtl* ctl, int maxIter, int fixedIter, int bendF
 // mask: 1 per-iteration errors (links, contacts, self), 2 the bend error.
 __global__ void k_reset_errors(SolverCtl* ctl, int mask)
 {
+    const int t = threadIdx.x;  // one wave: lane t clears part t
+    if (mask & 1) ctl->errPart[1][t * kErrStride] = 0;
+    if (mask & 2) ctl->errPart[0][t * kErrStride] = 0;
+    if (t != 0) return;
     if (mask & 1) {
         ctl->cl = 0;
         ctl->cv = 0;
@@ -1183,7 +1198,7 @@ __global__ void k_reset_errors(SolverCtl* ctl, int mask)
 
 void launch_reset_errors(hipStream_t s, SolverCtl* ctl, int mask)
 {
-    hipLaunchKernelGGL(k_reset_errors, dim3(1), dim3(1), 0, s, ctl, mask);
+    hipLaunchKernelGGL(k_reset_errors, dim3(1), dim3(kErrParts), 0, s, ctl, mask);
 }
 
 // Physics/ClothPBD.cpp:314-321
@@ -1402,6 +1417,10 @@ __global__ void __launch_bounds__(BS) k_batch_substep(const ClothDesc* __restric
         d.ctl->bendError = cb;
         d.ctl->contactError = cv;
     }
+    if (threadIdx.x < kErrParts) {
+        d.ctl->errPart[0][threadIdx.x * kErrStride] = 0;
+        d.ctl->errPart[1][threadIdx.x * kErrStride] = 0;
+    }
     if (wgClock) {
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -1563,7 +1582,7 @@ void launch_contacts(hipStream_t s, float4* pos, const float4* prev, float4* vel
 void launch_iter_end(hipStream_t s, SolverCtl* ctl, int maxIter, int fixedIter, int bendFreq, int dihedral,
                      float selfThreshold)
 {
-    hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(1), 0, s, ctl, maxIter, fixedIter, bendFreq, dihedral,
+    hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(kErrParts), 0, s, ctl, maxIter, fixedIter, bendFreq, dihedral,
                        selfThreshold);
 }
 

@@ -24,6 +24,7 @@ __global__ void __launch_bounds__(kBlock) k_predict_mesh(float4* __restrict__ po
     if (i == 0) {
         begin_substep_ctl(ctl);
     }
+    if (i < kErrParts) begin_substep_parts(ctl, i);
     if (i < ec.groups) ec.groupCount[i] = 0;
     if (i < tc.groups) tc.groupCount[i] = 0;
     float tv = 0.f;

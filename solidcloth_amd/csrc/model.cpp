@@ -1638,8 +1638,13 @@ void ClothModel::fetch_stats()
     HIP_OK(hipMemcpy(&c, gpu.d_ctl, sizeof(c), hipMemcpyDeviceToHost));
     if (mFixedIterations) {
         statIterations = (mWarm && mWarmMode == 2 && !mWarmLastPlain) ? mWarmIters : mNumIterations;
-        statBendError = __builtin_bit_cast(float, c.cb);
-        statContactError = __builtin_bit_cast(float, c.cv);
+        unsigned cb = c.cb, cv = c.cv;
+        for (int t = 0; t < kErrParts; t++) {
+            cb = std::max(cb, c.errPart[0][t * kErrStride]);
+            cv = std::max(cv, c.errPart[1][t * kErrStride]);
+        }
+        statBendError = __builtin_bit_cast(float, cb);
+        statContactError = __builtin_bit_cast(float, cv);
     } else {
         statIterations = c.iterations;
         statBendError = c.bendError;
