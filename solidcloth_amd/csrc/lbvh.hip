@@ -127,27 +127,48 @@ __global__ void __launch_bounds__(kBlock) k_hist(const unsigned* __restrict__ ke
     hist[threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Exclusive prefix sum of m counts in one workgroup.
+// Exclusive prefix sum of m counts (m a multiple of 4) in one workgroup: in
+// chunks of four consecutive counts per thread, so that a chunk is one
+// coalesced pass; each chunk is scanned inside the waves with DPP (row
+// shifts, then row broadcasts), across them by an LDS prefix of the wave
+// totals, and offset by the running total of the chunks before it. The next
+// chunk's counts are requested before the current one is scanned.
 __global__ void __launch_bounds__(kScanBlock) k_scan(unsigned* a, int m)
 {
-    __shared__ unsigned part[kScanBlock];
-    const int per = (m + kScanBlock - 1) / kScanBlock;
-    const int b = threadIdx.x * per, e = min(b + per, m);
-    unsigned s = 0;
-    for (int i = b; i < e; i++) s += a[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 1; o < kScanBlock; o <<= 1) {
-        unsigned v = threadIdx.x >= o ? part[threadIdx.x - o] : 0u;
+    constexpr int kWaves = kScanBlock / 64, kChunk = 4 * kScanBlock;
+    __shared__ unsigned wsum[kWaves];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    auto load = [&](int base) {
+        const int i = base + 4 * t;
+        return i < m ? *(const uint4*)(a + i) : make_uint4(0u, 0u, 0u, 0u);
+    };
+    unsigned carry = 0;
+    uint4 cur = load(0);
+    for (int base = 0; base < m; base += kChunk) {
+        const uint4 nxt = load(base + kChunk);
+        const unsigned s0 = cur.x, s1 = s0 + cur.y, s2 = s1 + cur.z, s3 = s2 + cur.w;
+        unsigned incl = s3;  // inclusive scan of the thread totals over the wave
+        incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x111, 0xf, 0xf, false);  // row_shr:1
+        incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x112, 0xf, 0xf, false);  // row_shr:2
+        incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x114, 0xf, 0xf, false);  // row_shr:4
+        incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x118, 0xf, 0xf, false);  // row_shr:8
+        incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x142, 0xa, 0xf, false);  // row_bcast:15, rows 1, 3
+        incl += (unsigned)__builtin_amdgcn_update_dpp(0, (int)incl, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2, 3
+        if (lane == 63) wsum[w] = incl;
         __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    unsigned run = part[threadIdx.x] - s;
-    for (int i = b; i < e; i++) {
-        const unsigned v = a[i];
-        a[i] = run;
-        run += v;
+        unsigned before = 0, total = 0;
+#pragma unroll
+        for (int k = 0; k < kWaves; k++) {
+            const unsigned x = wsum[k];
+            before += k < w ? x : 0u;
+            total += x;
+        }
+        const unsigned off = carry + before + incl - s3;
+        const int i = base + 4 * t;
+        if (i < m) *(uint4*)(a + i) = make_uint4(off, off + s0, off + s1, off + s2);
+        carry += total;
+        __syncthreads();  // (wsum is rewritten by the next chunk)
+        cur = nxt;
     }
 }
 
@@ -198,8 +219,12 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const unsigned* __restrict__
         __syncthreads();
         if (valid) {
             const unsigned dst = base[d] + wcnt[wave][d] + below;
-            kout[dst] = key;
-            vout[dst] = val;
+            // (always below n for a correct histogram scan; the check keeps a
+            // wrong one from writing outside the arrays)
+            if (dst < (unsigned)n) {
+                kout[dst] = key;
+                vout[dst] = val;
+            }
         }
         __syncthreads();
     }
