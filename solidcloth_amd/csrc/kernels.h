@@ -107,8 +107,11 @@ struct ContactBuf {
 // Obstacle-edge contacts per cloth edge and obstacle-vertex contacts per cloth
 // triangle (Physics/ClothTypes.h:110-120), slot-major like ContactBuf. Cloth
 // edges (triangles) carrying a contact are listed per colour group: group g
-// holds up to groupBase[g+1] - groupBase[g] entries, groupCount[g] used; the
-// groups are processed in order, their members in parallel.
+// holds up to groupBase[g+1] - groupBase[g] entries, groupCount[g *
+// kGroupStride] used (one 128-byte line per counter, so that appends to
+// different groups do not queue on one line); the groups are processed in
+// order, their members in parallel.
+constexpr int kGroupStride = 32;
 struct FeatureContactBuf {
     int* count;        // per cloth edge / triangle
     float4* P;         // point, lambda

@@ -25,8 +25,8 @@ __global__ void __launch_bounds__(kBlock) k_predict_mesh(float4* __restrict__ po
         begin_substep_ctl(ctl);
     }
     if (i < kErrParts) begin_substep_parts(ctl, i);
-    if (i < ec.groups) ec.groupCount[i] = 0;
-    if (i < tc.groups) tc.groupCount[i] = 0;
+    if (i < ec.groups) ec.groupCount[i * kGroupStride] = 0;
+    if (i < tc.groups) tc.groupCount[i * kGroupStride] = 0;
     float tv = 0.f;
     if (i < n) {
         const float4 P = pos[i];
@@ -44,11 +44,24 @@ __global__ void __launch_bounds__(kBlock) k_predict_mesh(float4* __restrict__ po
     wave_max_atomic(&ctl->maxTv, tv);
 }
 
+// Lists cloth edge / triangle id in its colour group. The calling lanes of a
+// wave share one atomic per group among them: a group's lanes take
+// consecutive places after the leader's reservation.
 __device__ __forceinline__ void group_append(const FeatureContactBuf& b, int id)
 {
     const int g = b.group[id];
-    const int k = atomicAdd(&b.groupCount[g], 1);
-    b.list[b.groupBase[g] + k] = id;
+    const int lane = __lane_id();
+    unsigned long long todo = __ballot(true);
+    while (todo) {
+        const int leader = __ffsll(todo) - 1;
+        const int gl = __builtin_amdgcn_readlane(g, leader);
+        const unsigned long long same = __ballot(g == gl) & todo;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&b.groupCount[gl * kGroupStride], __popcll(same));
+        base = __builtin_amdgcn_readlane(base, leader);
+        if (g == gl) b.list[b.groupBase[g] + base + __popcll(same & ((1ull << lane) - 1ull))] = id;
+        todo &= ~same;
+    }
 }
 
 __device__ __forceinline__ void put_feature(const FeatureContactBuf& b, int id, int& cnt, f3 point, f3 normal,
@@ -196,7 +209,7 @@ __global__ void __launch_bounds__(kBlock) k_edge_contacts(float4* __restrict__ p
                                                           SolverCtl* ctl)
 {
     if (ctl->done) return;
-    const int cnt = ec.groupCount[g], base = ec.groupBase[g];
+    const int cnt = ec.groupCount[g * kGroupStride], base = ec.groupBase[g];
     if ((int)(blockIdx.x * blockDim.x) >= cnt) return;
     float err = 0.f;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
@@ -260,7 +273,7 @@ __global__ void __launch_bounds__(kBlock) k_tri_contacts(float4* __restrict__ po
                                                          SolverCtl* ctl)
 {
     if (ctl->done) return;
-    const int cnt = tc.groupCount[g], base = tc.groupBase[g];
+    const int cnt = tc.groupCount[g * kGroupStride], base = tc.groupBase[g];
     if ((int)(blockIdx.x * blockDim.x) >= cnt) return;
     float err = 0.f;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {

@@ -731,12 +731,12 @@ void ClothModel::build_device_topology()
             for (int k = c.phaseOffsets[p]; k < c.phaseOffsets[p + 1]; k++) grp[c.order[k]] = p;
         dev_alloc(f.group, n);
         dev_alloc(f.groupBase, f.groups + 1);
-        dev_alloc(f.groupCount, std::max(f.groups, 1));
+        dev_alloc(f.groupCount, std::max(f.groups, 1) * (size_t)kGroupStride);
         dev_alloc(f.list, n);
         dev_alloc(f.count, n);
         if (n) HIP_OK(hipMemcpy(f.group, grp.data(), n * sizeof(int), hipMemcpyHostToDevice));
         if (n) HIP_OK(hipMemset(f.count, 0, n * sizeof(int)));
-        HIP_OK(hipMemset(f.groupCount, 0, std::max(f.groups, 1) * sizeof(int)));
+        HIP_OK(hipMemset(f.groupCount, 0, std::max(f.groups, 1) * kGroupStride * sizeof(int)));
         HIP_OK(hipMemcpy(f.groupBase, c.phaseOffsets.data(), (f.groups + 1) * sizeof(int), hipMemcpyHostToDevice));
         f.base = f.groups > 0 ? std::vector<int>(c.phaseOffsets.begin(), c.phaseOffsets.begin() + f.groups + 1)
                               : std::vector<int>(1, 0);
