@@ -199,10 +199,16 @@ __global__ void __launch_bounds__(kBlock) k_self_boxes(const float4* __restrict_
     }
 }
 
+// Appends one contact; the calling lanes of a wave share one atomic on the
+// kind's counter (the leader reserves their places).
 __device__ __forceinline__ void self_append(SelfBuf b, int kind, SolverCtl* ctl, int4 ids, float4 W, float4 N,
                                             unsigned long long key)
 {
-    const int k = atomicAdd(&ctl->selfCount[kind], 1);
+    const unsigned long long act = __ballot(true);
+    const int lane = __lane_id(), leader = __ffsll(act) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&ctl->selfCount[kind], __popcll(act));
+    const int k = __builtin_amdgcn_readlane(base, leader) + __popcll(act & ((1ull << lane) - 1ull));
     if (k >= b.cap) {
         atomicExch(&ctl->selfOverflow, 1);
         return;
