@@ -117,6 +117,28 @@ def test_drape_with_contacts_reference_early_exit(sc, oracle_mod, mode):
 
 
 @pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("iterations", [20, 7, 1])
+def test_fixed_count_reports_last_pass_errors(sc, oracle_mod, mode, iterations):
+    """With a fixed iteration count the reported bend and contact errors are
+    those of the last bend pass and the last iteration's contacts, as the
+    oracle's loop leaves them (ClothPBD.cpp:337-361), whichever launches
+    carry them: equal to the bit while the cloth lies on the sphere. Catches
+    a launch that skips its report in the last pass, or an earlier pass's
+    error that survives into the report."""
+    scene = S.DRAPE256.with_(n=40, iterations=iterations)
+    world, patch, m, o = _pair(sc, oracle_mod, scene, mode)
+    seen = 0
+    for f in range(40):
+        patch.Step(scene.dt)
+        o.step(scene.dt)
+        gs, os_ = m.stats(), o.stats()
+        assert np.float32(gs["bendError"]) == np.float32(os_["bendError"]), (f, gs, os_)
+        assert np.float32(gs["contactError"]) == np.float32(os_["contactError"]), (f, gs, os_)
+        seen += os_["contactError"] > 0
+    assert seen > 0
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_drape256_full_size_parity(sc, oracle_mod, mode):
     scene = S.DRAPE256
     world, patch, m, o = _pair(sc, oracle_mod, scene, mode)
