@@ -170,6 +170,10 @@ struct SlabContacts {
     // vertex-contact error, bit 1 the bend error (a fixed iteration count
     // reports only the last passes', so earlier passes skip the atomics)
     int report = 3;
+    // FEM tiles applying the last iteration's contacts: each tile also sets
+    // its own vertices' velocities, v = (pos - prev) / h (ClothPBD.cpp:314-321),
+    // in place of the velocity launch
+    int velocity = 0;
 };
 
 // Packed sweep schedules (one phase of a slab or tile schedule), laid out so

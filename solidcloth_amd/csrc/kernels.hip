@@ -967,7 +967,7 @@ __global__ void __launch_bounds__(kSlabBlock) k_fem_packed(float4* pos, PackedFe
         if (c + kFemAhead < NC) f[c + kFemAhead] = fem_slot(pk, sb + (c + kFemAhead) * kSlabBlock);
         // every lane requests (vertex 0's state where there is no contact): a
         // load under a branch makes the colour steps after it wait for it
-        if (kContacts && c + kFemAhead == NC) contact_pre(cp.cnt > 0 ? g0 : 0, cp, kc);
+        if (kContacts && c + kFemAhead == NC) contact_pre(cp.cnt > 0 || kc.velocity ? g0 : 0, cp, kc);
         issue_fence();
         // (a branch per colour keeps each colour's arithmetic, and the wait
         // for its data, in its own block)
@@ -990,7 +990,13 @@ __global__ void __launch_bounds__(kSlabBlock) k_fem_packed(float4* pos, PackedFe
         if (own && cp.cnt > 0) err = packed_vertex_contacts(g0, cp, sv, tid, kc);
         if (kc.report & 1) wave_max_atomic(err_part(ctl, 1), err);
     }
-    if (own) pos[g0] = sv[tid];
+    if (own) {
+        const float4 P = sv[tid];
+        pos[g0] = P;
+        // (after the contacts' own velocity update, as the velocity launch)
+        if (kContacts && kc.velocity && P.w != 0.f) kc.vel[g0] = f4(mul(kc.invH, sub(xyz(P), xyz(cp.PV))), 0.f);
+    }
+    if (kContacts && kc.velocity && blockIdx.x == 0 && tid == 0) ctl->maxTv = 0;  // as the velocity launch
 }
 
 // Bends on packed slabs (KV vertex slots per thread, at most

@@ -1206,6 +1206,7 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
     // substep cleared them (bit 0: the per-iteration errors, bit 1: the bend
     // error); a fixed count clears only what earlier passes wrote
     int dirty = 0;
+    bool velocityDone = false;  // set by FEM tiles of the last iteration
     if (mSolver != SOLVER_GAUSS_SEIDEL) {
         // ClothPBD.cpp:290-298: Jacobi or conjugate-residual passes over the
         // links only (no triangles, bends or contacts), a fixed count
@@ -1270,6 +1271,12 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                         const DevicePacked::Phase& ph = g.triPacked.phases[p];
                         const PackedFem pf{g.triPacked.verts + ph.vertOff, g.triPacked.s0 + ph.slotOff,
                                            g.triPacked.s1 + ph.slotOff, ph.nc};
+                        // the last iteration's tiles also set the velocities when
+                        // every vertex is some tile's own
+                        if (kc.enabled == 2 && k == iterations - 1 && triCol.numPhases() == 1 && g.triOrphans == 0) {
+                            kc.velocity = 1;
+                            velocityDone = true;
+                        }
                         launch_fem_packed(s, g.d_pos, pf, ph.slabs, ph.ncKernel, ph.kv, h2, epsUV, g.d_ctl, kc);
                         if (kc.enabled == 2) dirty |= report & 1;
                     } else {
@@ -1345,7 +1352,7 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
     timed_begin(g.timer, timed, 4, s, ev);
     if (fuse)
         launch_contacts_velocity(s, g.d_pos, g.d_prev, g.d_vel, cb, mThickness, mFriction, invH, g.d_ctl);
-    else
+    else if (!velocityDone)
         launch_velocity(s, g.d_pos, g.d_prev, g.d_vel, g.nv, invH, g.d_ctl);
     timed_end(g.timer, timed, 4, s, ev);
     if (track) {
