@@ -539,6 +539,9 @@ __global__ void __launch_bounds__(kSoloBlock) k_self_colour(SelfBuf b, int kind,
 {
     __shared__ unsigned long long lbid[kColourLdsVerts];
     __shared__ int nextCount, uniq, rem[2];
+    __shared__ ushort4 tailLid[64];  // the last contacts, handed to one wave
+    __shared__ unsigned long long tailBid[64];
+    __shared__ int tailIdx[64];
     __shared__ int part[kSoloBlock];
     const int n = min(ctl->selfCount[kind], b.cap);
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -626,6 +629,54 @@ __global__ void __launch_bounds__(kSoloBlock) k_self_colour(SelfBuf b, int kind,
             const int remaining = rem[r & 1];
             r++;
             if (remaining == 0) break;
+            if (remaining <= 64) {
+                // The last few contacts go to wave 0, which runs the same
+                // rounds (same bids, so the same colours) with no workgroup
+                // barrier: a wave's LDS operations take effect in order. The
+                // counter rem[r & 1] was cleared before the last barrier.
+#pragma unroll
+                for (int k = 0; k < kColourPerThread; k++)
+                    if (mc[k] == -1) {
+                        const int slot = atomicAdd(&rem[r & 1], 1);
+                        tailLid[slot] = ml[k];
+                        tailBid[slot] = mb[k];
+                        tailIdx[slot] = tid + k * nt;
+                    }
+                __syncthreads();
+                if (tid < 64) {
+                    const bool have = tid < remaining;
+                    const ushort4 l = have ? tailLid[tid] : make_ushort4(0, 0, 0, 0);
+                    const unsigned long long bid = have ? tailBid[tid] : 0ull;
+                    int col = -1, rr = r;
+                    constexpr int kWaitLds = 0xC07F;  // lgkmcnt(0)
+                    while (__ballot(have && col < 0)) {
+                        if (have && col < 0) {
+                            atomicMax(&lbid[l.x], bid);
+                            atomicMax(&lbid[l.y], bid);
+                            atomicMax(&lbid[l.z], bid);
+                            atomicMax(&lbid[l.w], bid);
+                        }
+                        __builtin_amdgcn_s_waitcnt(kWaitLds);
+                        const bool open = have && col < 0;
+                        if (open && lbid[l.x] == bid && lbid[l.y] == bid && lbid[l.z] == bid && lbid[l.w] == bid)
+                            col = rr;
+                        __builtin_amdgcn_s_waitcnt(kWaitLds);
+                        if (open) {
+                            lbid[l.x] = 0ull;
+                            lbid[l.y] = 0ull;
+                            lbid[l.z] = 0ull;
+                            lbid[l.w] = 0ull;
+                        }
+                        __builtin_amdgcn_s_waitcnt(kWaitLds);
+                        rr++;
+                    }
+                    if (have) b.colour[tailIdx[tid]] = col;
+                    if (tid == 0) rem[0] = rr;
+                }
+                __syncthreads();
+                r = rem[0];
+                break;
+            }
         }
 #pragma unroll
         for (int k = 0; k < kColourPerThread; k++)
