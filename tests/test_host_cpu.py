@@ -426,3 +426,25 @@ def test_model_getters_read_back_the_setters(sc):
         (123.5, 0.75, 0.25, 0.5)
     assert m.GetDihedral() is False and m.GetDragCoeff() == 0.875
     assert m.GetSelfParams() == (0.625, 0.0078125)
+
+
+def test_bend_strips_keep_grid_rows_whole(sc):
+    """The default bend schedule of the 256^2 grid: interiors of two-row
+    strips (5 colours) and two seam phases (9 colours each, the most seam
+    bends at one vertex), each strip made of whole rows, because strip cuts
+    move back to the first vertex of a coordinate. Catches a cut inside a
+    row (a strip holding part of a row, or more colours) and a cost model
+    that picks the wider strips (15 unpaired interior colours)."""
+    world, patch = S.build_patch(S.DRAPE256, sc)
+    m = patch.GetModel()
+    bends = m.bends()
+    order = m.constraint_order("bend")
+    ph, sl, co = m.schedule("bend")
+    assert ph.max() == 2
+    for p, want in ((0, 5), (1, 9), (2, 9)):
+        assert co[ph == p].max() + 1 == want, p
+    rows = bends[order] // 256
+    for s in np.unique(sl[ph == 0]):
+        r = rows[sl == s]
+        # an interior slab: two whole rows 2k, 2k + 1
+        assert r.min() % 2 == 0 and r.max() == r.min() + 1, s
