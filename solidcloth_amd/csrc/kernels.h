@@ -205,9 +205,12 @@ struct PackedBend {
     int nc, width;
 };
 // Largest colour count of a packed bend phase, and the vertex slots per
-// thread the packed kernels are built for.
+// thread the packed kernels are built for: FEM tiles and bend slabs that
+// apply vertex contacts up to kPackMaxKv, other bend slabs up to
+// kPackMaxBendKv.
 constexpr int kPackMaxBendColours = 16;
 constexpr int kPackMaxKv = 4;
+constexpr int kPackMaxBendKv = 6;
 
 // Self contacts of one kind (vertex i4 against triangle i1 i2 i3, or edge i1 i2
 // against edge i3 i4), in detection order; `order` lists them colour group by

@@ -1036,7 +1036,7 @@ __global__ void __launch_bounds__(kSlabBlock) k_bend_packed(float4* pos, PackedB
     // last colour, is requested during the sweep, one vertex slot per colour
     // (counted from the first, whatever the slab's colour count), so that it
     // holds up neither the staging nor any one colour step
-    static_assert(KV <= 4, "contact slots requested over the first colours");
+    static_assert(!kContacts || KV <= 4, "contact slots requested over the first colours");
     ContactPre cp[KV];
 #pragma unroll
     for (int u = 0; u < KV; u++) {
@@ -1554,6 +1554,15 @@ static void bend_packed_kv(hipStream_t s, float4* pos, PackedBend pk, int numSla
                            ctl, kc);
 }
 
+// Wider slabs (kv 5, 6) without vertex contacts.
+template <int KV, bool kPair>
+static void bend_packed_nc(hipStream_t s, float4* pos, PackedBend pk, int numSlabs, float stiff, SolverCtl* ctl,
+                           const SlabContacts& kc)
+{
+    hipLaunchKernelGGL((k_bend_packed<KV, false, kPair>), dim3(numSlabs), dim3(kSlabBlock), 0, s, pos, pk, stiff, ctl,
+                       kc);
+}
+
 void launch_bend_packed(hipStream_t s, float4* pos, PackedBend pk, int numSlabs, int kv, float stiff, SolverCtl* ctl,
                         const SlabContacts& kc)
 {
@@ -1568,6 +1577,11 @@ void launch_bend_packed(hipStream_t s, float4* pos, PackedBend pk, int numSlabs,
                            : bend_packed_kv<3, false>(s, pos, pk, numSlabs, stiff, ctl, kc);
     else if (kv == 4) pair ? bend_packed_kv<4, true>(s, pos, pk, numSlabs, stiff, ctl, kc)
                            : bend_packed_kv<4, false>(s, pos, pk, numSlabs, stiff, ctl, kc);
+    else if (kc.enabled == 3) throw std::runtime_error("packed bend slabs: contacts need at most 4 vertex slots");
+    else if (kv == 5) pair ? bend_packed_nc<5, true>(s, pos, pk, numSlabs, stiff, ctl, kc)
+                           : bend_packed_nc<5, false>(s, pos, pk, numSlabs, stiff, ctl, kc);
+    else if (kv == 6) pair ? bend_packed_nc<6, true>(s, pos, pk, numSlabs, stiff, ctl, kc)
+                           : bend_packed_nc<6, false>(s, pos, pk, numSlabs, stiff, ctl, kc);
     else throw std::runtime_error("packed bend slabs: unsupported vertex slots");
 }
 

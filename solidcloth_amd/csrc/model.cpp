@@ -490,7 +490,8 @@ static int upload_slabs(const Schedule& s, DeviceSlabs& d, bool owners, int numV
 // of a phase the same number of constraint slots (see PackedFem and
 // PackedBend). `flagged` is the slabs' vertex list with the flags
 // upload_slabs sets. Leaves `d` without phases when the schedule exceeds the
-// kernels' limits: a slab of more than kPackMaxKv * kSlabBlock vertices, a
+// kernels' limits: a tile of kPackMaxKv * kSlabBlock vertices or more, a slab
+// of more than kPackMaxBendKv * kSlabBlock, a
 // colour of a slab wider than the workgroup, too many colours, a tile owning
 // more vertices than the workgroup has threads, or vertex ids of 2^29 or more.
 static void pack_schedule(bool enabled, const Schedule& s, const std::vector<int>& flagged, int numVertices,
@@ -517,7 +518,7 @@ static void pack_schedule(bool enabled, const Schedule& s, const std::vector<int
                 if (owned > kSlabBlock) return;
             }
         }
-        if (maxVerts > kPackMaxKv * kSlabBlock || maxWidth > kSlabBlock || maxCol < 1) return;
+        if (maxVerts > (fem ? kPackMaxKv : kPackMaxBendKv) * kSlabBlock || maxWidth > kSlabBlock || maxCol < 1) return;
         // FEM tiles keep their last vertex slot free as the spare entry
         if (fem && maxVerts >= kPackMaxKv * kSlabBlock) return;
         ph.slabs = s.phaseSlabBegin[p + 1] - s.phaseSlabBegin[p];
@@ -1300,7 +1301,11 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                                (int)bendCol.order.size()};
                     SlabContacts kc{cb, g.d_prev, g.d_vel, mThickness, mFriction, invH, contactsInBends ? 3 : 0,
                                     report};
-                    if ((int)g.bendPacked.phases.size() == bendCol.numPhases()) {
+                    // (slabs wider than kPackMaxKv slots apply no contacts)
+                    bool packed = (int)g.bendPacked.phases.size() == bendCol.numPhases();
+                    for (const DevicePacked::Phase& ph : g.bendPacked.phases)
+                        packed = packed && !(contactsInBends && ph.kv > kPackMaxKv);
+                    if (packed) {
                         const DevicePacked::Phase& ph = g.bendPacked.phases[p];
                         const PackedBend pb{g.bendPacked.verts + ph.vertOff, g.bendPacked.ids + ph.slotOff, ph.nc,
                                             ph.width};

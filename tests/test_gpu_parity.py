@@ -211,3 +211,28 @@ def test_packed_sweeps_equal_unpacked(sc):
     assert np.array_equal(p3, p4)
     assert np.array_equal(v3, v4)
     assert s3 == s4
+
+
+@pytest.mark.parametrize("flags", [0, S.CF_VERTICES | S.CF_EDGES])
+def test_wide_packed_bend_slabs(sc, oracle_mod, flags):
+    """A 400-vertex-wide cloth: its bend seams hold more than 1024 vertices,
+    so mode 4 runs them on packed slabs of five or six vertex slots per
+    thread, which apply no vertex contacts (without collision flags nothing
+    to apply; with edge contacts the contact launches stay separate). Same
+    bits as mode 3 and as the oracle."""
+    scene = S.DRAPE32.with_(n=400, flags=flags)
+    out = []
+    for mode in (3, 4):
+        world, patch = S.build_patch(scene, sc)
+        m = patch.GetModel()
+        m.SetSweep(mode)
+        for f in range(4):
+            patch.Step(scene.dt)
+        out.append((m.positions(), m.velocities(), m.constraint_order("bend")))
+    (p3, v3, b3), (p4, v4, b4) = out
+    assert np.array_equal(b3, b4)
+    assert np.array_equal(p3, p4) and np.array_equal(v3, v4)
+    o = oracle_mod.from_scene(scene, tri_order=m.constraint_order("tri"), bend_order=b4)
+    for f in range(4):
+        o.step(scene.dt)
+    assert np.array_equal(p4.astype(np.float64), o.get("pos"))
