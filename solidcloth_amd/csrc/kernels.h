@@ -405,7 +405,10 @@ void launch_self_detect(hipStream_t s, const float4* pos, const float4* prev, co
                         SelfCandidates cand, SelfBuf vt, SelfBuf ee, SolverCtl* ctl);
 // vloc: one int per cloth vertex, every byte 0x7f between calls (the
 // colouring claims and releases entries while it lists a kind's vertices)
-void launch_self_colour(hipStream_t s, SelfBuf b, int kind, unsigned long long* vmax, int* vloc, SolverCtl* ctl);
+// Colours both kinds' contacts at once (one workgroup each); vmax and vloc
+// hold 2 nv entries, the first nv for the vertex-triangle kind.
+void launch_self_colour(hipStream_t s, SelfBuf vt, SelfBuf ee, unsigned long long* vmax, int* vloc, int nv,
+                        SolverCtl* ctl);
 void launch_self_solve(hipStream_t s, float4* pos, SelfBuf b, int kind, SelfParams sp, float h, SolverCtl* ctl);
 
 }  // namespace sc

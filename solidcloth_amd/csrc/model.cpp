@@ -928,10 +928,10 @@ void ClothModel::ensure_self()
         dev_alloc(S.triLo, std::max(g.nt, 1)); dev_alloc(S.triHi, std::max(g.nt, 1));
         dev_alloc(S.edgeLo, std::max(g.ne, 1)); dev_alloc(S.edgeHi, std::max(g.ne, 1));
         dev_alloc(S.vtxLo, std::max(g.nv, 1)); dev_alloc(S.vtxHi, std::max(g.nv, 1));
-        dev_alloc(S.vmax, std::max(g.nv, 1));
+        dev_alloc(S.vmax, 2 * (size_t)std::max(g.nv, 1));
         // the colouring's vertex claims: every byte 0x7f between colourings
-        dev_alloc(S.vloc, std::max(g.nv, 1));
-        HIP_OK(hipMemset(S.vloc, 0x7f, sizeof(int) * std::max(g.nv, 1)));
+        dev_alloc(S.vloc, 2 * (size_t)std::max(g.nv, 1));
+        HIP_OK(hipMemset(S.vloc, 0x7f, 2 * sizeof(int) * std::max(g.nv, 1)));
         for (SelfBuf& b : S.buf) dev_alloc(b.vlist, std::max(g.nv, 1));
         std::vector<int4> ev(std::max(g.ne, 1), make_int4(-1, -1, -1, -1));
         for (int e = 0; e < g.ne; e++) ev[e] = make_int4((int)edges[e].i1, (int)edges[e].i2, -1, -1);
@@ -1160,8 +1160,7 @@ void ClothModel::record_substep(float h, const CollSet& cs, bool timed)
                                SelfCandidates{S.candVT, S.candEE, S.candFillVT, S.candFillEE, S.candCapVT,
                                               S.candCapEE},
                                S.buf[0], S.buf[1], g.d_ctl);
-            launch_self_colour(s, S.buf[0], 0, S.vmax, S.vloc, g.d_ctl);
-            launch_self_colour(s, S.buf[1], 1, S.vmax, S.vloc, g.d_ctl);
+            launch_self_colour(s, S.buf[0], S.buf[1], S.vmax, S.vloc, std::max(g.nv, 1), g.d_ctl);
         }
         timed_end(g.timer, timed, 3, s, ev);
     } else if (anyContacts && msActive) {

@@ -269,7 +269,7 @@ public:
         Lbvh triB, edgeB, vtxB;  // vtxB: Morton order of the vertex queries (built per step)
         SelfBuf buf[2]{};
         unsigned long long* vmax = nullptr;
-        int* vloc = nullptr;  // the colouring's per-vertex claims
+        int* vloc = nullptr;  // the colouring's per-vertex claims (vmax, vloc: one half per contact kind)
         int4* edgeVerts4 = nullptr;  // each cloth edge's vertices as (i1, i2, -1, -1): the edge tree's leaf payload
         int2 *candVT = nullptr, *candEE = nullptr;  // candidate pools (chunks of kCandChunk pairs)
         int *candFillVT = nullptr, *candFillEE = nullptr;  // pairs per chunk

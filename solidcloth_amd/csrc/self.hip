@@ -534,9 +534,15 @@ constexpr int kVlocFree = 0x7f7f7f7f, kVlocSlot = 0x40000000;
 
 __device__ __forceinline__ int corner(const int4 q, int j) { return j == 0 ? q.x : j == 1 ? q.y : j == 2 ? q.z : q.w; }
 
-__global__ void __launch_bounds__(kSoloBlock) k_self_colour(SelfBuf b, int kind, unsigned long long* vmax, int* vloc,
-                                                            SolverCtl* ctl)
+__global__ void __launch_bounds__(kSoloBlock) k_self_colour(SelfBuf bvt, SelfBuf bee, unsigned long long* vmax,
+                                                            int* vloc, int nv, SolverCtl* ctl)
 {
+    // workgroup 0 colours the vertex-triangle contacts, workgroup 1 the
+    // edge-edge ones, each with its own half of the per-vertex tables
+    const int kind = blockIdx.x;
+    const SelfBuf b = kind == 0 ? bvt : bee;
+    vmax += (size_t)kind * nv;
+    vloc += (size_t)kind * nv;
     __shared__ unsigned long long lbid[kColourLdsVerts];
     __shared__ int nextCount, uniq, rem[2];
     __shared__ ushort4 tailLid[64];  // the last contacts, handed to one wave
@@ -1012,9 +1018,10 @@ void launch_self_detect(hipStream_t s, const float4* pos, const float4* prev, co
                        ctl);
 }
 
-void launch_self_colour(hipStream_t s, SelfBuf b, int kind, unsigned long long* vmax, int* vloc, SolverCtl* ctl)
+void launch_self_colour(hipStream_t s, SelfBuf vt, SelfBuf ee, unsigned long long* vmax, int* vloc, int nv,
+                        SolverCtl* ctl)
 {
-    hipLaunchKernelGGL(k_self_colour, dim3(1), dim3(kSoloBlock), 0, s, b, kind, vmax, vloc, ctl);
+    hipLaunchKernelGGL(k_self_colour, dim3(2), dim3(kSoloBlock), 0, s, vt, ee, vmax, vloc, nv, ctl);
 }
 
 void launch_self_solve(hipStream_t s, float4* pos, SelfBuf b, int kind, SelfParams sp, float h, SolverCtl* ctl)
